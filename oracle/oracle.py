@@ -1,0 +1,365 @@
+"""ctypes front-end of the parity checkers (test infrastructure only).
+
+`Oracle` wraps liboracle.so, the C restatement of the reference algorithms;
+`Reference` wraps oracle/_ref/libsrfref_{serial,omp}.so, the reference's own
+render/ sources compiled by oracle/Makefile. Only tests/, the smoke check and
+the bench's cpu_baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+TRI_NODE = np.dtype([("bmin", "<f4", (2, 3)), ("bmax", "<f4", (2, 3)),
+                     ("flag", "<i4"), ("axis", "<i4"), ("data", "<u4", (2,))])
+TET_NODE = np.dtype([("bmin", "<f4", (3,)), ("bmax", "<f4", (3,)),
+                     ("flag", "<i4"), ("axis", "<i4"), ("data", "<u4", (2,))])
+SPH_NODE = np.dtype([("bmin", "<f4", (2, 3)), ("bmax", "<f4", (2, 3)),
+                     ("data", "<u4", (2,)), ("flag", "<i2"), ("axis", "<i2")])
+assert TRI_NODE.itemsize == 64 and TET_NODE.itemsize == 40 and SPH_NODE.itemsize == 60
+
+_p = C.c_void_p
+_i64 = C.c_int64
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Oracle:
+    """The C restatement (oracle/srf_oracle.c)."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        L = self.lib = C.CDLL(path)
+        for name in ("or_scene_bounds_tri", "or_scene_bounds_tet", "or_morton_tri",
+                     "or_morton_tet", "or_morton_pts", "or_sort_keys",
+                     "or_radix_tree30", "or_trace_tri", "or_trace_tet",
+                     "or_trace_sph", "or_tex3d_sample", "or_volume_fwd_f32",
+                     "or_volume_fwd_f64", "or_volume_bwd_f64",
+                     "or_scene_bounds_sph"):
+            getattr(L, name).restype = None
+        for name in ("or_build_tri", "or_build_tet", "or_build_sph"):
+            getattr(L, name).restype = _i64
+        L.or_morton30_point.restype = C.c_uint32
+
+    # ---- stages ----
+    def scene_bounds(self, kind, V, F=None, R=None, rconst=0.0):
+        bmin = np.zeros(3, np.float32)
+        bmax = np.zeros(3, np.float32)
+        V = _c(V, np.float32)
+        if kind == "tri":
+            F = _c(F, np.uint32)
+            self.lib.or_scene_bounds_tri(_ptr(V), _ptr(F), _i64(len(F)), _ptr(bmin), _ptr(bmax))
+        elif kind == "tet":
+            F = _c(F, np.uint32)
+            self.lib.or_scene_bounds_tet(_ptr(V), _ptr(F), _i64(len(F)), _ptr(bmin), _ptr(bmax))
+        else:
+            R = None if R is None else _c(R, np.float32)
+            self.lib.or_scene_bounds_sph(_ptr(V), _ptr(R), C.c_float(rconst),
+                                         _i64(len(V)), _ptr(bmin), _ptr(bmax))
+        return bmin, bmax
+
+    def morton(self, kind, V, F, bmin, bmax):
+        V = _c(V, np.float32)
+        bmin = _c(bmin, np.float32)
+        bmax = _c(bmax, np.float32)
+        if kind == "pts":
+            codes = np.zeros(len(V), np.uint32)
+            self.lib.or_morton_pts(_ptr(V), _i64(len(V)), _ptr(bmin), _ptr(bmax), _ptr(codes))
+            return codes
+        F = _c(F, np.uint32)
+        codes = np.zeros(len(F), np.uint32)
+        fn = self.lib.or_morton_tri if kind == "tri" else self.lib.or_morton_tet
+        fn(_ptr(V), _ptr(F), _i64(len(F)), _ptr(bmin), _ptr(bmax), _ptr(codes))
+        return codes
+
+    def morton_point(self, p, bmin, inv):
+        p = _c(p, np.float32)
+        bmin = _c(bmin, np.float32)
+        return int(self.lib.or_morton30_point(_ptr(p), _ptr(bmin), C.c_float(inv[0]),
+                                              C.c_float(inv[1]), C.c_float(inv[2])))
+
+    def sort_keys(self, codes):
+        codes = _c(codes, np.uint32)
+        sc = np.zeros_like(codes)
+        si = np.zeros_like(codes)
+        self.lib.or_sort_keys(_ptr(codes), _i64(len(codes)), _ptr(sc), _ptr(si))
+        return sc, si
+
+    def radix_tree(self, sorted_codes):
+        sc = _c(sorted_codes, np.uint32)
+        out = np.zeros((max(len(sc) - 1, 0), 3), np.uint32)
+        self.lib.or_radix_tree30(_ptr(sc), _i64(len(sc)), _ptr(out))
+        return out
+
+    # ---- builds ----
+    def build_tri(self, V, F):
+        V = _c(V, np.float32)
+        F = _c(F, np.uint32)
+        n = len(F)
+        nodes = np.zeros(2 * n, TRI_NODE)
+        idx = np.zeros(n, np.uint32)
+        bounds = np.zeros(6, np.float64)
+        es = C.c_double()
+        m = self.lib.or_build_tri(_ptr(V), _ptr(F), _i64(n), _ptr(nodes), _ptr(idx),
+                                  _ptr(bounds), C.byref(es))
+        return dict(nodes=nodes[:m].copy(), indices=idx, bounds=bounds,
+                    eps_scale=es.value)
+
+    def build_tet(self, V, F):
+        V = _c(V, np.float32)
+        F = _c(F, np.uint32)
+        n = len(F)
+        nodes = np.zeros(2 * n, TET_NODE)
+        idx = np.zeros(n, np.uint32)
+        bounds = np.zeros(6, np.float64)
+        m = self.lib.or_build_tet(_ptr(V), _ptr(F), _i64(n), _ptr(nodes), _ptr(idx), _ptr(bounds))
+        return dict(nodes=nodes[:m].copy(), indices=idx, bounds=bounds)
+
+    def build_sph(self, P, R=None, rconst=0.0):
+        P = _c(P, np.float32)
+        R = None if R is None else _c(R, np.float32)
+        n = len(P)
+        nodes = np.zeros(2 * n, SPH_NODE)
+        idx = np.zeros(n, np.uint32)
+        bounds = np.zeros(6, np.float64)
+        m = self.lib.or_build_sph(_ptr(P), _ptr(R), C.c_float(rconst), _i64(n), _ptr(nodes),
+                                  _ptr(idx), _ptr(bounds))
+        return dict(nodes=nodes[:m].copy(), indices=idx, bounds=bounds)
+
+    # ---- traversal ----
+    def trace_tri(self, tree, V, F, rays, threads=1, stats=False):
+        from surface_amd.workloads import HIT_DTYPE
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        st = np.zeros((len(rays), 2), np.uint32) if stats else None
+        self.lib.or_trace_tri(_ptr(tree["nodes"]), _ptr(tree["indices"]), _ptr(_c(V, np.float32)),
+                              _ptr(_c(F, np.uint32)), C.c_double(tree["eps_scale"]),
+                              _ptr(rays), _i64(len(rays)), _ptr(hits), _ptr(st), C.c_int(threads))
+        return (hits, st) if stats else hits
+
+    def trace_tet(self, tree, V, F, rays, scalars=None, threads=1, stats=False):
+        from surface_amd.workloads import HIT_DTYPE, TET_EXT_DTYPE
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        ext = np.zeros(len(rays), TET_EXT_DTYPE)
+        st = np.zeros((len(rays), 2), np.uint32) if stats else None
+        sc = None if scalars is None else _c(scalars, np.float32)
+        self.lib.or_trace_tet(_ptr(tree["nodes"]), _ptr(tree["indices"]), _ptr(_c(V, np.float32)),
+                              _ptr(_c(F, np.uint32)), _ptr(sc), _ptr(rays), _i64(len(rays)),
+                              _ptr(hits), _ptr(ext), _ptr(st), C.c_int(threads))
+        return (hits, ext, st) if stats else (hits, ext)
+
+    def trace_sph(self, tree, P, R, rconst, rays, prev_t=None, prev_n=None, threads=1,
+                  stats=False):
+        from surface_amd.workloads import HIT_DTYPE
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        st = np.zeros((len(rays), 2), np.uint32) if stats else None
+        Rr = None if R is None else _c(R, np.float32)
+        pt = None if prev_t is None else _c(prev_t, np.float64)
+        pn = None if prev_n is None else _c(prev_n, np.float32)
+        self.lib.or_trace_sph(_ptr(tree["nodes"]), _ptr(tree["indices"]), _ptr(_c(P, np.float32)),
+                              _ptr(Rr), C.c_float(rconst), _ptr(rays), _ptr(pt), _ptr(pn),
+                              _i64(len(rays)), _ptr(hits), _ptr(st), C.c_int(threads))
+        return (hits, st) if stats else hits
+
+    # ---- volume ----
+    def tex3d(self, vox, dims, clamp, uvw):
+        vox = _c(vox, np.float32)
+        uvw = _c(uvw, np.float32).reshape(-1, 3)
+        out = np.zeros(len(uvw), np.float32)
+        d = _c(dims, np.int32)
+        cl = _c(clamp, np.int32)
+        self.lib.or_tex3d_sample(_ptr(vox), _ptr(d), _ptr(cl), _ptr(uvw), _i64(len(uvw)), _ptr(out))
+        return out
+
+    @staticmethod
+    def _vparams(dims, clamp, tf_size, step):
+        class P(C.Structure):
+            _fields_ = [("dims", C.c_int * 3), ("clamp", C.c_int * 3),
+                        ("tf_size", C.c_int), ("step", C.c_float)]
+        p = P()
+        p.dims[:] = [int(x) for x in dims]
+        p.clamp[:] = [int(x) for x in clamp]
+        p.tf_size = int(tf_size)
+        p.step = float(step)
+        return p
+
+    def volume_fwd(self, vox, dims, clamp, tf, step, rays, threads=1):
+        p = self._vparams(dims, clamp, len(tf), step)
+        rgba = np.zeros((len(rays), 4), np.float32)
+        ns = np.zeros(len(rays), np.uint32)
+        self.lib.or_volume_fwd_f32(C.byref(p), _ptr(_c(vox, np.float32)), _ptr(_c(tf, np.float32)),
+                                   _ptr(rays), _i64(len(rays)), _ptr(rgba), _ptr(ns), C.c_int(threads))
+        return rgba, ns
+
+    def volume_fwd64(self, vox, dims, clamp, tf, step, rays, threads=1):
+        p = self._vparams(dims, clamp, len(tf), step)
+        rgba = np.zeros((len(rays), 4), np.float64)
+        self.lib.or_volume_fwd_f64(C.byref(p), _ptr(_c(vox, np.float64)), _ptr(_c(tf, np.float64)),
+                                   _ptr(rays), _i64(len(rays)), _ptr(rgba), C.c_int(threads))
+        return rgba
+
+    def volume_bwd64(self, vox, dims, clamp, tf, step, rays, grad_rgba):
+        p = self._vparams(dims, clamp, len(tf), step)
+        gv = np.zeros(len(vox), np.float64)
+        gt = np.zeros(np.shape(tf), np.float64)
+        self.lib.or_volume_bwd_f64(C.byref(p), _ptr(_c(vox, np.float64)), _ptr(_c(tf, np.float64)),
+                                   _ptr(rays), _i64(len(rays)), _ptr(_c(grad_rgba, np.float64)),
+                                   _ptr(gv), _ptr(gt))
+        return gv, gt
+
+
+def ref_available(variant="serial"):
+    return os.path.exists(os.path.join(HERE, "_ref", f"libsrfref_{variant}.so"))
+
+
+class Reference:
+    """The reference's own render/ code (oracle/_ref), via ref_driver.cc."""
+
+    def __init__(self, variant="serial"):
+        path = os.path.join(HERE, "_ref", f"libsrfref_{variant}.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle ref`")
+        L = self.lib = C.CDLL(path)
+        for name in ("ref_build_tri", "ref_build_tet", "ref_build_sph"):
+            getattr(L, name).restype = _p
+        for name in ("ref_morton_tri", "ref_morton_tet", "ref_morton_pts", "ref_sort_keys",
+                     "ref_radix_tree30", "ref_bounds", "ref_free", "ref_trace",
+                     "ref_tex3d_sample"):
+            getattr(L, name).restype = None
+        L.ref_dump.restype = C.c_int
+        L.ref_has_openmp.restype = C.c_int
+        for name in ("ref_free", "ref_bounds", "ref_dump", "ref_trace"):
+            getattr(L, name).argtypes = None
+
+    def morton(self, kind, V, F, bmin, bmax, simd=True):
+        V = _c(V, np.float32)
+        bmin = _c(bmin, np.float32)
+        bmax = _c(bmax, np.float32)
+        if kind == "pts":
+            codes = np.zeros(len(V), np.uint32)
+            self.lib.ref_morton_pts(_ptr(V), _i64(len(V)), _ptr(bmin), _ptr(bmax), _ptr(codes),
+                                    C.c_int(1 if simd else 0))
+            return codes
+        F = _c(F, np.uint32)
+        codes = np.zeros(len(F), np.uint32)
+        fn = self.lib.ref_morton_tri if kind == "tri" else self.lib.ref_morton_tet
+        fn(_ptr(V), _ptr(F), _i64(len(F)), _ptr(bmin), _ptr(bmax), _ptr(codes))
+        return codes
+
+    def sort_keys(self, codes):
+        codes = _c(codes, np.uint32)
+        sc = np.zeros_like(codes)
+        si = np.zeros_like(codes)
+        self.lib.ref_sort_keys(_ptr(codes), _i64(len(codes)), _ptr(sc), _ptr(si))
+        return sc, si
+
+    def radix_tree(self, sorted_codes):
+        sc = _c(sorted_codes, np.uint32)
+        out = np.zeros((max(len(sc) - 1, 0), 3), np.uint32)
+        self.lib.ref_radix_tree30(_ptr(sc), _i64(len(sc)), _ptr(out))
+        return out
+
+    def _read_dump(self, h, dtype):
+        fd, path = tempfile.mkstemp(suffix=".bvh")
+        os.close(fd)
+        try:
+            if self.lib.ref_dump(_p(h), path.encode()) != 0:
+                raise RuntimeError("reference Dump failed")
+            raw = np.fromfile(path, np.uint8)
+        finally:
+            os.unlink(path)
+        nn = int(raw[:8].view(np.uint64)[0])
+        off = 8 + nn * dtype.itemsize
+        nodes = raw[8:off].view(dtype).copy()
+        ni = int(raw[off:off + 8].view(np.uint64)[0])
+        idx = raw[off + 8:off + 8 + 4 * ni].view(np.uint32).copy()
+        return nodes, idx
+
+    def _bounds(self, h):
+        bmin = np.zeros(3, np.float64)
+        bmax = np.zeros(3, np.float64)
+        self.lib.ref_bounds(_p(h), _ptr(bmin), _ptr(bmax))
+        return np.concatenate([bmin, bmax])
+
+    def build(self, kind, V, F=None, R=None, rconst=0.0, keep=False):
+        """Run Build32; return nodes (reference layout), indices and bounds.
+        With keep=True the handle stays alive for trace() (call free())."""
+        V = _c(V, np.float32)
+        keepalive = [V]
+        if kind == "tri":
+            F = _c(F, np.uint32)
+            keepalive.append(F)
+            h = self.lib.ref_build_tri(_ptr(V), _i64(len(V)), _ptr(F), _i64(len(F)))
+            dtype = TRI_NODE
+        elif kind == "tet":
+            F = _c(F, np.uint32)
+            keepalive.append(F)
+            h = self.lib.ref_build_tet(_ptr(V), _ptr(F), _i64(len(F)))
+            dtype = TET_NODE
+        else:
+            R = None if R is None else _c(R, np.float32)
+            keepalive.append(R)
+            h = self.lib.ref_build_sph(_ptr(V), _ptr(R), C.c_float(rconst), _i64(len(V)))
+            dtype = SPH_NODE
+        nodes, idx = self._read_dump(h, dtype)
+        out = dict(nodes=nodes, indices=idx, bounds=self._bounds(h))
+        if keep:
+            out["handle"] = h
+            out["_keep"] = keepalive
+        else:
+            self.lib.ref_free(_p(h))
+        return out
+
+    def trace(self, tree, rays, width=0, scalars=None, threads=0):
+        from surface_amd.workloads import HIT_DTYPE, TET_EXT_DTYPE
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        ext = np.zeros(len(rays), TET_EXT_DTYPE)
+        sc = None if scalars is None else _c(scalars, np.float32)
+        self.lib.ref_trace(_p(tree["handle"]), _ptr(rays), _i64(len(rays)), _i64(width),
+                           _ptr(hits), _ptr(ext), _ptr(sc), C.c_int(threads))
+        return hits, ext
+
+    def free(self, tree):
+        if tree.get("handle"):
+            self.lib.ref_free(_p(tree["handle"]))
+            tree["handle"] = None
+
+    def tex3d(self, vox, dims, clamp, uvw):
+        vox = _c(vox, np.float32)
+        uvw = _c(uvw, np.float32).reshape(-1, 3)
+        out = np.zeros(len(uvw), np.float32)
+        self.lib.ref_tex3d_sample(_ptr(vox), _ptr(_c(dims, np.int32)), _ptr(_c(clamp, np.int32)),
+                                  _ptr(uvw), _i64(len(uvw)), _ptr(out))
+        return out
+
+
+def canonical_tri(nodes):
+    """Zero the fields the reference leaves uninitialised (leaf axis and the
+    unused second box slot of a leaf) so trees compare byte for byte."""
+    n = nodes.copy()
+    leaf = n["flag"] == 1
+    n["axis"][leaf] = 0
+    n["bmin"][leaf, 1] = 0
+    n["bmax"][leaf, 1] = 0
+    return n
+
+
+def canonical_tet(nodes):
+    n = nodes.copy()
+    n["axis"][n["flag"] == 1] = 0
+    return n
+
+
+canonical_sph = canonical_tri

@@ -1,0 +1,294 @@
+// ref_driver.cc -- a thin extern "C" shim over the SURFACE reference's own
+// render/ sources, compiled together with them (from where they lie in the
+// reference checkout) by oracle/Makefile into oracle/_ref/. It exists only
+// to pin the oracle: tests and the bench's cpu_baseline leg call it through
+// ctypes. It contains no algorithm of its own -- every result comes from the
+// reference's functions named below.
+//
+//   Morton        CalculateMortonCodes{TriangleFloat,TetraFloat}30,
+//                 CalculateMortonCodes30{,SIMD}   render/render_prefix_tree_util.cc
+//   sort          RadixSort30 / RadixSort30OMP   render/render_prefix_tree_util.h:296-457
+//   Karras        ConstructBinaryRadixTree30     render/render_prefix_tree_util.cc:562-649
+//   build         {Triangle,Tetra,Particle}Accel::Build32 + Dump
+//   trace         {Triangle,Tetra,Particle}Accel::Traverse, OpenMP
+//                 schedule(dynamic,1) over image rows as in
+//                 gles/gles_raytrace_engine.cc:662-665
+//   texture       FilterTexture3DFloat           render/render_texture.cc:946-1058
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <unistd.h>
+#include <fcntl.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "render_accel_particle.h"
+#include "render_accel_tetra.h"
+#include "render_accel_triangle.h"
+#include "render_prefix_tree_util.h"
+#include "render_texture.h"
+
+using namespace lsgl::render;
+
+namespace {
+
+// Build32 prints its stage timings; keep the caller's stdout clean.
+struct QuietStdout {
+  int saved;
+  QuietStdout() {
+    fflush(stdout);
+    saved = dup(1);
+    int devnull = open("/dev/null", O_WRONLY);
+    dup2(devnull, 1);
+    close(devnull);
+  }
+  ~QuietStdout() {
+    fflush(stdout);
+    dup2(saved, 1);
+    close(saved);
+  }
+};
+
+struct RayIn {
+  float org[3];
+  unsigned int flags;
+  float dir[3];
+  unsigned int prev_prim_id;
+};
+struct HitOut {
+  float t, u, v;
+  unsigned int prim_id;
+};
+struct TetExt {
+  float d[4];
+  float scalar;
+  unsigned int subface;
+  float t_leave;
+  float pad;
+};
+
+struct RefAccel {
+  int kind;  // 0 tri, 1 tet, 2 sphere
+  Mesh mesh;
+  Tetrahedron tet;
+  Particles part;
+  TriangleAccel tri_accel;
+  TetraAccel tet_accel;
+  ParticleAccel sph_accel;
+  ~RefAccel() {
+    // Mesh::~Mesh deletes its arrays; they belong to the caller.
+    mesh.vertices = 0;
+    mesh.faces = 0;
+    mesh.dvertices = 0;
+    mesh.normals = 0;
+  }
+};
+
+Ray make_ray(const RayIn &r) {
+  real3 o(r.org[0], r.org[1], r.org[2]);
+  real3 d(r.dir[0], r.dir[1], r.dir[2]);
+  Ray ray(o, d);
+  ray.double_sided = (int)(r.flags & 1u);
+  ray.prev_prim_id = r.prev_prim_id;
+  ray.prev_hit_t = 0.0;
+  ray.prev_hit_normal = real3(0.0f, 0.0f, 0.0f);
+  return ray;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_has_openmp(void) {
+#ifdef _OPENMP
+  return 1;
+#else
+  return 0;
+#endif
+}
+
+void ref_morton_tri(const float *V, const unsigned *F, long long n,
+                    const float bmin[3], const float bmax[3], unsigned *codes) {
+  real3 mn(bmin[0], bmin[1], bmin[2]), mx(bmax[0], bmax[1], bmax[2]);
+  CalculateMortonCodesTriangleFloat30(codes, V, F, mn, mx, 0, n);
+}
+
+void ref_morton_tet(const float *V, const unsigned *F, long long n,
+                    const float bmin[3], const float bmax[3], unsigned *codes) {
+  real3 mn(bmin[0], bmin[1], bmin[2]), mx(bmax[0], bmax[1], bmax[2]);
+  CalculateMortonCodesTetraFloat30(codes, V, F, mn, mx, 0, n);
+}
+
+void ref_morton_pts(const float *P, long long n, const float bmin[3],
+                    const float bmax[3], unsigned *codes, int simd) {
+  real3 mn(bmin[0], bmin[1], bmin[2]), mx(bmax[0], bmax[1], bmax[2]);
+  if (simd)
+    CalculateMortonCodes30SIMD(codes, P, mn, mx, 0, n);
+  else
+    CalculateMortonCodes30(codes, P, mn, mx, 0, n);
+}
+
+void ref_sort_keys(const unsigned *codes, long long n, unsigned *out_codes,
+                   unsigned *out_index) {
+  std::vector<IndexKey30> keys(n);
+  for (long long i = 0; i < n; i++) {
+    keys[i].index = (uint32_t)i;
+    keys[i].code = codes[i];
+  }
+#ifdef _OPENMP
+  RadixSort30OMP(&keys[0], &keys[0] + n);
+#else
+  RadixSort30(&keys[0], &keys[0] + n);
+#endif
+  for (long long i = 0; i < n; i++) {
+    out_codes[i] = keys[i].code;
+    out_index[i] = keys[i].index;
+  }
+}
+
+void ref_radix_tree30(const unsigned *sorted_codes, long long n,
+                      unsigned *out /* 3 words per node */) {
+  std::vector<IndexKey30> keys(n);
+  for (long long i = 0; i < n; i++) {
+    keys[i].index = (uint32_t)i;
+    keys[i].code = sorted_codes[i];
+  }
+  for (long long i = 0; i + 1 < n; i++) {
+    NodeInfo32 ni = ConstructBinaryRadixTree30(&keys[0], (int)i, (uint32_t)n);
+    out[3 * i + 0] = ni.childIndex;
+    out[3 * i + 1] = (unsigned)ni.leftType;
+    out[3 * i + 2] = (unsigned)ni.rightType;
+  }
+}
+
+void *ref_build_tri(const float *V, long long nv, unsigned *F, long long nf) {
+  RefAccel *a = new RefAccel();
+  a->kind = 0;
+  a->mesh.nvertices = nv;
+  a->mesh.vertices = V;
+  a->mesh.nfaces = nf;
+  a->mesh.faces = F;
+  QuietStdout q;
+  a->tri_accel.Build32(&a->mesh, TriangleBuildOptions());
+  return a;
+}
+
+void *ref_build_tet(const float *V, unsigned *F, long long n) {
+  RefAccel *a = new RefAccel();
+  a->kind = 1;
+  memset(&a->tet, 0, sizeof(a->tet));
+  a->tet.numTetrahedrons = n;
+  a->tet.vertices = V;
+  a->tet.faces = F;
+  QuietStdout q;
+  a->tet_accel.Build32(&a->tet, TetraBuildOptions());
+  return a;
+}
+
+void *ref_build_sph(const float *P, float *R, float rconst, long long n) {
+  RefAccel *a = new RefAccel();
+  a->kind = 2;
+  memset(&a->part, 0, sizeof(a->part));
+  a->part.numParticles = n;
+  a->part.positions = P;
+  a->part.radius = R;
+  a->part.constantRadius = rconst;
+  QuietStdout q;
+  a->sph_accel.Build32(&a->part, ParticleBuildOptions());
+  return a;
+}
+
+int ref_dump(void *h, const char *path) {
+  RefAccel *a = (RefAccel *)h;
+  if (a->kind == 0) return a->tri_accel.Dump(path) ? 0 : 1;
+  if (a->kind == 1) return a->tet_accel.Dump(path) ? 0 : 1;
+  return a->sph_accel.Dump(path) ? 0 : 1;
+}
+
+void ref_bounds(void *h, double bmin[3], double bmax[3]) {
+  RefAccel *a = (RefAccel *)h;
+  if (a->kind == 0)
+    a->tri_accel.BoundingBox(bmin, bmax);
+  else if (a->kind == 1)
+    a->tet_accel.BoundingBox(bmin, bmax);
+  else
+    a->sph_accel.BoundingBox(bmin, bmax);
+}
+
+void ref_free(void *h) { delete (RefAccel *)h; }
+
+// Trace n rays as rows of `width` rays, OpenMP dynamic over rows. For tetra,
+// ext receives InterpolateTetra weights and, given scalars, the varying
+// f0*d0 + f1*d1 + f2*d2 + f3*d3 (gles/gles_common.cc:1074-1076).
+void ref_trace(void *h, const void *rays_in, long long n, long long width,
+               void *hits_out, void *ext_out, const float *scalars,
+               int threads) {
+  RefAccel *a = (RefAccel *)h;
+  const RayIn *rays = (const RayIn *)rays_in;
+  HitOut *hits = (HitOut *)hits_out;
+  TetExt *ext = (TetExt *)ext_out;
+  if (width <= 0) width = n;
+  long long rows = (n + width - 1) / width;
+  (void)threads;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (long long y = 0; y < rows; y++) {
+    long long end = (y + 1) * width < n ? (y + 1) * width : n;
+    for (long long i = y * width; i < end; i++) {
+      Ray ray = make_ray(rays[i]);
+      Intersection isect;
+      isect.subface_id = 0;
+      isect.d0 = isect.d1 = isect.d2 = isect.d3 = 0.0f;
+      bool hit = false;
+      if (a->kind == 0)
+        hit = a->tri_accel.Traverse(isect, ray);
+      else if (a->kind == 1)
+        hit = a->tet_accel.Traverse(isect, ray);
+      else
+        hit = a->sph_accel.Traverse(isect, ray);
+      hits[i].t = isect.t;
+      hits[i].u = isect.u;
+      hits[i].v = isect.v;
+      hits[i].prim_id = isect.prim_id;
+      if (ext) {
+        memset(&ext[i], 0, sizeof(TetExt));
+        if (hit && a->kind == 1) {
+          ext[i].d[0] = isect.d0;
+          ext[i].d[1] = isect.d1;
+          ext[i].d[2] = isect.d2;
+          ext[i].d[3] = isect.d3;
+          ext[i].subface = isect.subface_id;
+          if (scalars)
+            ext[i].scalar = scalars[isect.f0] * isect.d0 +
+                            scalars[isect.f1] * isect.d1 +
+                            scalars[isect.f2] * isect.d2 +
+                            scalars[isect.f3] * isect.d3;
+        }
+      }
+    }
+  }
+}
+
+void ref_tex3d_sample(float *vox, const int dims[3], const int clamp[3],
+                      const float *uvw, long long n, float *out) {
+  Texture3D tex;
+  memset(&tex, 0, sizeof(tex));
+  tex.image = (unsigned char *)vox;
+  tex.width = dims[0];
+  tex.height = dims[1];
+  tex.depth = dims[2];
+  tex.components = 1;
+  tex.data_type = LSGL_RENDER_TEXTURE3D_FORMAT_FLOAT;
+  for (long long i = 0; i < n; i++) {
+    float rgba[4];
+    FilterTexture3DFloat(rgba, &tex, uvw[3 * i], uvw[3 * i + 1], uvw[3 * i + 2],
+                         clamp[0] != 0, clamp[1] != 0, clamp[2] != 0);
+    out[i] = rgba[0];
+  }
+}
+
+}  // extern "C"

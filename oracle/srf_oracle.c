@@ -1,0 +1,1627 @@
+/*
+ * srf_oracle.c -- CPU restatement of SURFACE's acceleration-structure build,
+ * closest-hit traversal and 3D-texture sampling.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP path. The product
+ * never links or calls it; tests/, the smoke check and the bench's
+ * cpu_baseline leg load it through oracle/oracle.py.
+ *
+ * Each function cites the reference file:line whose arithmetic it follows
+ * (paths relative to the SURFACE checkout). The arithmetic is restated
+ * operation by operation, with the same float/double mixing and evaluation
+ * order, and is compiled with -ffp-contract=off and without -ffast-math, as
+ * the reference's x86-64 Release build (no -march, hence no FMA) is.
+ *
+ * Pins (see tests/test_oracle_*.py): the known-answer Morton vector of
+ * test/cctest/test-prefix-tree.cc:27-41, the Karras invariants of
+ * test-prefix-tree.cc:418-510,622-625, byte comparison with trees and hits
+ * produced by the reference itself (oracle/_ref, built from the reference
+ * sources by oracle/Makefile) and committed golden fixtures, brute-force
+ * closest hit over all primitives, analytic sphere/tetra/trilinear cases.
+ * The emission-absorption volume march has no reference counterpart (the
+ * reference leaves marching to user GLSL); its f64 adjoint is pinned by
+ * central finite differences of the f64 forward.
+ */
+#include "srf_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define TRI_MAX_LEAF 16 /* render/render_accel_triangle.cc:34 */
+#define TET_MAX_LEAF 4  /* render/render_accel_tetra.cc:34 */
+#define SPH_MAX_LEAF 16 /* render/render_accel_particle.cc:51 */
+#define MAX_DEPTH32 22  /* render/render_accel_triangle.cc:35 */
+#define STACK_MAX 512   /* render/render_accel_triangle.cc:1723 */
+
+/* ------------------------------------------------------------------------ */
+/* Morton codes                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* PartBy2_30, render/render_prefix_tree_util.h:113-120 */
+static uint32_t part_by2_30(uint32_t n) {
+  n &= 0x000003ffu;
+  n = (n ^ (n << 16)) & 0xff0000ffu;
+  n = (n ^ (n << 8)) & 0x0300f00fu;
+  n = (n ^ (n << 4)) & 0x030c30c3u;
+  n = (n ^ (n << 2)) & 0x09249249u;
+  return n;
+}
+
+/* float -> uint32_t as x86-64 g++ emits it (cvttss2si to 64 bit, keep the
+ * low word); identical to a plain truncation for |x| < 2^31. */
+static uint32_t f2u_trunc(float x) { return (uint32_t)(int64_t)x; }
+
+/* MortionCode30, render/render_prefix_tree_util.h:143-157 */
+uint32_t or_morton30_point(const float p[3], const float bmin[3], float invx,
+                           float invy, float invz) {
+  uint32_t ix = f2u_trunc((p[0] - bmin[0]) * invx);
+  uint32_t iy = f2u_trunc((p[1] - bmin[1]) * invy);
+  uint32_t iz = f2u_trunc((p[2] - bmin[2]) * invz);
+  return (part_by2_30(ix) << 2) | (part_by2_30(iy) << 1) | part_by2_30(iz);
+}
+
+/* kDIV / (bmax - bmin): int over float, i.e. a float division.
+ * render/render_prefix_tree_util.cc:242-245 */
+static void morton_inv(const float bmin[3], const float bmax[3], float inv[3]) {
+  for (int k = 0; k < 3; k++) inv[k] = (float)1024 / (bmax[k] - bmin[k]);
+}
+
+/* CalculateMortonCodesTriangleFloat30, render/render_prefix_tree_util.cc:230-265 */
+void or_morton_tri(const float *V, const uint32_t *F, int64_t n,
+                   const float bmin[3], const float bmax[3], uint32_t *codes) {
+  float inv[3];
+  morton_inv(bmin, bmax, inv);
+  const float one_third = 1.0f / 3.0f;
+  for (int64_t i = 0; i < n; i++) {
+    const float *p0 = V + 3 * (size_t)F[3 * i + 0];
+    const float *p1 = V + 3 * (size_t)F[3 * i + 1];
+    const float *p2 = V + 3 * (size_t)F[3 * i + 2];
+    float c[3];
+    for (int k = 0; k < 3; k++) c[k] = one_third * ((p0[k] + p1[k]) + p2[k]);
+    codes[i] = or_morton30_point(c, bmin, inv[0], inv[1], inv[2]);
+  }
+}
+
+/* CalculateMortonCodesTetraFloat30, render/render_prefix_tree_util.cc:267-304 */
+void or_morton_tet(const float *V, const uint32_t *F, int64_t n,
+                   const float bmin[3], const float bmax[3], uint32_t *codes) {
+  float inv[3];
+  morton_inv(bmin, bmax, inv);
+  const float one_fourth = 1.0f / 4.0f;
+  for (int64_t i = 0; i < n; i++) {
+    const float *p0 = V + 3 * (size_t)F[4 * i + 0];
+    const float *p1 = V + 3 * (size_t)F[4 * i + 1];
+    const float *p2 = V + 3 * (size_t)F[4 * i + 2];
+    const float *p3 = V + 3 * (size_t)F[4 * i + 3];
+    float c[3];
+    for (int k = 0; k < 3; k++)
+      c[k] = one_fourth * (((p0[k] + p1[k]) + p2[k]) + p3[k]);
+    codes[i] = or_morton30_point(c, bmin, inv[0], inv[1], inv[2]);
+  }
+}
+
+/* CalculateMortonCodes30SIMD, render/render_prefix_tree_util.cc:176-228: the
+ * SSE2 lanes do the same sub/mul/truncate as the scalar MortionCode30, and
+ * the missing 10-bit mask of vpartby2 only matters for ix > 1024, which the
+ * scene box (radius + eps padded) rules out. */
+void or_morton_pts(const float *P, int64_t n, const float bmin[3],
+                   const float bmax[3], uint32_t *codes) {
+  float inv[3];
+  morton_inv(bmin, bmax, inv);
+  for (int64_t i = 0; i < n; i++)
+    codes[i] = or_morton30_point(P + 3 * i, bmin, inv[0], inv[1], inv[2]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Scene bounds                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* ComputeBoundingBoxOMP<float> with epsScale = 0 (kEPS = 0):
+ * render/render_accel_triangle.cc:711-797 called at :1438 with :1430. */
+void or_scene_bounds_tri(const float *V, const uint32_t *F, int64_t n,
+                         float bmin[3], float bmax[3]) {
+  const float kEPS = FLT_EPSILON * 0.0f;
+  const float *v0 = V + 3 * (size_t)F[0];
+  for (int k = 0; k < 3; k++) {
+    bmin[k] = v0[k] - kEPS;
+    bmax[k] = v0[k] + kEPS;
+  }
+  for (int64_t i = 0; i < n; i++)
+    for (int j = 0; j < 3; j++) {
+      const float *v = V + 3 * (size_t)F[3 * i + j];
+      for (int k = 0; k < 3; k++) {
+        float mn = v[k] - kEPS, mx = v[k] + kEPS;
+        if (mn < bmin[k]) bmin[k] = mn; /* std::min(a, b) keeps a unless b < a */
+        if (mx > bmax[k]) bmax[k] = mx;
+      }
+    }
+}
+
+/* ComputeBoundingBoxOMP (tetra), render/render_accel_tetra.cc:971-1055:
+ * kEPS = FLT_EPSILON * 16. */
+void or_scene_bounds_tet(const float *V, const uint32_t *F, int64_t n,
+                         float bmin[3], float bmax[3]) {
+  const float kEPS = FLT_EPSILON * 16;
+  const float *v0 = V + 3 * (size_t)F[0];
+  for (int k = 0; k < 3; k++) {
+    bmin[k] = v0[k] - kEPS;
+    bmax[k] = v0[k] + kEPS;
+  }
+  for (int64_t i = 0; i < n; i++)
+    for (int j = 0; j < 4; j++) {
+      const float *v = V + 3 * (size_t)F[4 * i + j];
+      for (int k = 0; k < 3; k++) {
+        float mn = v[k] - kEPS, mx = v[k] + kEPS;
+        if (mn < bmin[k]) bmin[k] = mn;
+        if (mx > bmax[k]) bmax[k] = mx;
+      }
+    }
+}
+
+/* GetRadius, render/render_accel_particle.cc:272-278 */
+static float sph_radius(const float *R, float rconst, int64_t i) {
+  return R ? R[i] : rconst;
+}
+
+/* ComputeBoundingBoxOMP (particles), render/render_accel_particle.cc:303-378:
+ * pos - radius - kEPS with kEPS = (float)(FLT_EPSILON * 16.0). */
+void or_scene_bounds_sph(const float *P, const float *R, float rconst,
+                         int64_t n, float bmin[3], float bmax[3]) {
+  const float kEPS = (float)(FLT_EPSILON * 16.0);
+  float r = sph_radius(R, rconst, 0);
+  for (int k = 0; k < 3; k++) {
+    bmin[k] = (P[k] - r) - kEPS;
+    bmax[k] = (P[k] + r) + kEPS;
+  }
+  for (int64_t i = 0; i < n; i++) {
+    r = sph_radius(R, rconst, i);
+    for (int k = 0; k < 3; k++) {
+      float mn = (P[3 * i + k] - r) - kEPS, mx = (P[3 * i + k] + r) + kEPS;
+      if (mn < bmin[k]) bmin[k] = mn;
+      if (mx > bmax[k]) bmax[k] = mx;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Stable key sort and Karras radix tree                                    */
+/* ------------------------------------------------------------------------ */
+
+/* RadixSort30 / RadixSort30OMP, render/render_prefix_tree_util.h:296-457:
+ * four stable 8-bit LSD passes over {index, code}; the permutation equals a
+ * stable sort by code of the identity index sequence. */
+void or_sort_keys(const uint32_t *codes, int64_t n, uint32_t *sorted_codes,
+                  uint32_t *sorted_index) {
+  uint32_t *kc = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+  uint32_t *ki = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+  uint32_t *ac = sorted_codes, *ai = sorted_index;
+  for (int64_t i = 0; i < n; i++) {
+    ac[i] = codes[i];
+    ai[i] = (uint32_t)i;
+  }
+  uint32_t *src_c = ac, *src_i = ai, *dst_c = kc, *dst_i = ki;
+  for (unsigned shift = 0; shift < 32; shift += 8) {
+    size_t count[257];
+    memset(count, 0, sizeof(count));
+    for (int64_t i = 0; i < n; i++) count[((src_c[i] >> shift) & 0xff) + 1]++;
+    for (int b = 0; b < 256; b++) count[b + 1] += count[b];
+    for (int64_t i = 0; i < n; i++) {
+      size_t d = count[(src_c[i] >> shift) & 0xff]++;
+      dst_c[d] = src_c[i];
+      dst_i[d] = src_i[i];
+    }
+    uint32_t *t;
+    t = src_c; src_c = dst_c; dst_c = t;
+    t = src_i; src_i = dst_i; dst_i = t;
+  }
+  /* four passes: the result is back in the caller's arrays */
+  free(kc);
+  free(ki);
+}
+
+/* CountLeadingZeros32, render/render_prefix_tree_util.cc:447-483 */
+static int clz32(uint32_t x) {
+  if (x == 0) return 32;
+  int n = 0;
+  while (!(x & 0x80000000u)) {
+    n++;
+    x <<= 1;
+  }
+  return n;
+}
+
+/* Delta30, render/render_prefix_tree_util.cc:501-526: equal codes fall back to
+ * comparing the sorted positions, offset by 32. */
+static int delta30(const uint32_t *c, int32_t i, int32_t j, int32_t n) {
+  if (j < 0 || j >= n) return -1;
+  uint32_t ki = c[i], kj = c[j];
+  int off = 0;
+  if (ki == kj) {
+    ki = (uint32_t)i;
+    kj = (uint32_t)j;
+    off = 32;
+  }
+  return clz32(ki ^ kj) + off;
+}
+
+/* ConstructBinaryRadixTree30, render/render_prefix_tree_util.cc:562-649
+ * (Karras 2012, Fig. 4). */
+static or_nodeinfo radix_node30(const uint32_t *c, int i, int n, int *jout) {
+  int dp = delta30(c, i, i + 1, n) - delta30(c, i, i - 1, n);
+  int d = dp >= 0 ? 1 : -1;
+  int dmin = delta30(c, i, i - d, n);
+  int lmax = 2;
+  while (delta30(c, i, i + lmax * d, n) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (delta30(c, i, i + (l + t) * d, n) > dmin) l += t;
+  int j = i + l * d;
+  int dnode = delta30(c, i, j, n);
+  int s = 0;
+  int k = 2;
+  int t = (l + k - 1) / k; /* (int)ceil((double)l / k) for l >= 0 */
+  if (t > 0) {
+    for (;;) {
+      if (delta30(c, i, i + (s + t) * d, n) > dnode) s += t;
+      if (t == 1) break;
+      k *= 2;
+      t = (l + k - 1) / k;
+    }
+  }
+  int r = i + s * d + (d < 0 ? d : 0);
+  or_nodeinfo ni;
+  ni.child = (uint32_t)r;
+  ni.left_type = ((i < j ? i : j) == r) ? 1 : 0;
+  ni.right_type = ((i > j ? i : j) == r + 1) ? 1 : 0;
+  if (jout) *jout = j;
+  return ni;
+}
+
+void or_radix_tree30(const uint32_t *sorted_codes, int64_t n,
+                     or_nodeinfo *out) {
+  for (int64_t i = 0; i + 1 < n; i++)
+    out[i] = radix_node30(sorted_codes, (int)i, (int)n, NULL);
+}
+
+/* GetSplitAxis, render/render_accel_triangle.cc:854-863 (asserts compiled out:
+ * code 0 gives (32 - 2) % 3 = 0). */
+static int split_axis(uint32_t code) { return (clz32(code) - 2) % 3; }
+
+/* ------------------------------------------------------------------------ */
+/* Collapse + boxes: BuildTreeRecursive32 (serial), which emits DFS preorder */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int kind; /* 0 tri, 1 tet, 2 sphere */
+  const float *V;
+  const uint32_t *F;
+  const float *R;
+  float rconst;
+  float kEPS;
+  const uint32_t *sidx;  /* sorted primitive index */
+  const uint32_t *scode; /* sorted codes */
+  const or_nodeinfo *info;
+  int max_leaf;
+  void *nodes;
+  int64_t count;
+} build_ctx;
+
+/* Leaf box over prims [l, r) in sorted order.
+ * tri: ComputeBoundingBox30 render/render_accel_triangle.cc:660-709
+ * tet: ComputeBoundingBox30 render/render_accel_tetra.cc:921-969
+ * sph: ComputeBoundingBox30 render/render_accel_particle.cc:424-462 */
+static void leaf_box(const build_ctx *b, uint32_t l, uint32_t r, float mn[3],
+                     float mx[3]) {
+  const float e = b->kEPS;
+  if (b->kind == 2) {
+    uint32_t idx = b->sidx[l];
+    float rad = sph_radius(b->R, b->rconst, idx);
+    for (int k = 0; k < 3; k++) {
+      mn[k] = (b->V[3 * idx + k] - rad) - e;
+      mx[k] = (b->V[3 * idx + k] + rad) + e;
+    }
+    for (uint32_t i = l; i < r; i++) {
+      idx = b->sidx[i];
+      rad = sph_radius(b->R, b->rconst, idx);
+      for (int k = 0; k < 3; k++) {
+        float lo = (b->V[3 * (size_t)idx + k] - rad) - e;
+        float hi = (b->V[3 * (size_t)idx + k] + rad) + e;
+        if (mn[k] > lo) mn[k] = lo;
+        if (mx[k] < hi) mx[k] = hi;
+      }
+    }
+    return;
+  }
+  const int nv = b->kind == 0 ? 3 : 4;
+  uint32_t idx = b->sidx[l];
+  const float *v0 = b->V + 3 * (size_t)b->F[(size_t)nv * idx];
+  for (int k = 0; k < 3; k++) {
+    mn[k] = v0[k] - e;
+    mx[k] = v0[k] + e;
+  }
+  for (uint32_t i = l; i < r; i++) {
+    idx = b->sidx[i];
+    for (int j = 0; j < nv; j++) {
+      const float *v = b->V + 3 * (size_t)b->F[(size_t)nv * idx + j];
+      for (int k = 0; k < 3; k++) {
+        float lo = v[k] - e, hi = v[k] + e;
+        if (mn[k] > lo) mn[k] = lo;
+        if (mx[k] < hi) mx[k] = hi;
+      }
+    }
+  }
+}
+
+/* MergeBoundingBox, render/render_accel_triangle.cc:804-814 */
+static void merge_box(float mn[3], float mx[3], const float lmn[3],
+                      const float lmx[3], const float rmn[3],
+                      const float rmx[3]) {
+  for (int k = 0; k < 3; k++) {
+    mn[k] = (rmn[k] < lmn[k]) ? rmn[k] : lmn[k];
+    mx[k] = (lmx[k] < rmx[k]) ? rmx[k] : lmx[k];
+  }
+}
+
+static int64_t push_node(build_ctx *b) { return b->count++; }
+
+static void set_leaf(build_ctx *b, int64_t at, uint32_t l, uint32_t end,
+                     const float mn[3], const float mx[3]) {
+  if (b->kind == 1) {
+    or_tet_node *nd = (or_tet_node *)b->nodes + at;
+    memset(nd, 0, sizeof(*nd));
+    memcpy(nd->bmin, mn, 12);
+    memcpy(nd->bmax, mx, 12);
+    nd->flag = 1;
+    nd->data[0] = end - l;
+    nd->data[1] = l;
+  } else if (b->kind == 0) {
+    or_tri_node *nd = (or_tri_node *)b->nodes + at;
+    memset(nd, 0, sizeof(*nd));
+    memcpy(nd->bmin[0], mn, 12);
+    memcpy(nd->bmax[0], mx, 12);
+    nd->flag = 1;
+    nd->data[0] = end - l;
+    nd->data[1] = l;
+  } else {
+    or_sph_node *nd = (or_sph_node *)b->nodes + at;
+    memset(nd, 0, sizeof(*nd));
+    memcpy(nd->bmin[0], mn, 12);
+    memcpy(nd->bmax[0], mx, 12);
+    nd->flag = 1;
+    nd->data[0] = end - l;
+    nd->data[1] = l;
+  }
+}
+
+static void set_branch(build_ctx *b, int64_t at, int axis, int64_t lc,
+                       int64_t rc, const float lmn[3], const float lmx[3],
+                       const float rmn[3], const float rmx[3],
+                       const float mn[3], const float mx[3]) {
+  if (b->kind == 1) {
+    /* TetraNode keeps its own (merged) box: render/render_accel_tetra.cc:1225-1235 */
+    or_tet_node *nd = (or_tet_node *)b->nodes + at;
+    memset(nd, 0, sizeof(*nd));
+    memcpy(nd->bmin, mn, 12);
+    memcpy(nd->bmax, mx, 12);
+    nd->flag = 0;
+    nd->axis = axis;
+    nd->data[0] = (uint32_t)lc;
+    nd->data[1] = (uint32_t)rc;
+  } else if (b->kind == 0) {
+    or_tri_node *nd = (or_tri_node *)b->nodes + at;
+    memset(nd, 0, sizeof(*nd));
+    memcpy(nd->bmin[0], lmn, 12);
+    memcpy(nd->bmax[0], lmx, 12);
+    memcpy(nd->bmin[1], rmn, 12);
+    memcpy(nd->bmax[1], rmx, 12);
+    nd->flag = 0;
+    nd->axis = axis;
+    nd->data[0] = (uint32_t)lc;
+    nd->data[1] = (uint32_t)rc;
+  } else {
+    or_sph_node *nd = (or_sph_node *)b->nodes + at;
+    memset(nd, 0, sizeof(*nd));
+    memcpy(nd->bmin[0], lmn, 12);
+    memcpy(nd->bmax[0], lmx, 12);
+    memcpy(nd->bmin[1], rmn, 12);
+    memcpy(nd->bmax[1], rmx, 12);
+    nd->flag = 0;
+    nd->axis = (int16_t)axis;
+    nd->data[0] = (uint32_t)lc;
+    nd->data[1] = (uint32_t)rc;
+  }
+}
+
+/* BuildTreeRecursive32, render/render_accel_triangle.cc:895-992 (tetra
+ * render/render_accel_tetra.cc:1154-1240, particle
+ * render/render_accel_particle.cc:559-661). Range [l, r] is inclusive. */
+static int64_t build_rec(build_ctx *b, uint32_t root, uint32_t l, uint32_t r,
+                         int is_leaf, int depth, float mn[3], float mx[3]) {
+  uint32_t n = r - l;
+  if (is_leaf || n <= (uint32_t)b->max_leaf || depth > MAX_DEPTH32) {
+    leaf_box(b, l, r + 1, mn, mx);
+    int64_t at = push_node(b);
+    set_leaf(b, at, l, r + 1, mn, mx);
+    return at;
+  }
+  uint32_t mid = b->info[root].child;
+  int ll = b->info[root].left_type == 1, rl = b->info[root].right_type == 1;
+  int axis = split_axis(b->scode[root]);
+  int64_t at = push_node(b);
+  float lmn[3], lmx[3], rmn[3], rmx[3];
+  int64_t lc = build_rec(b, mid, l, mid, ll, depth + 1, lmn, lmx);
+  int64_t rc = build_rec(b, mid + 1, mid + 1, r, rl, depth + 1, rmn, rmx);
+  merge_box(mn, mx, lmn, lmx, rmn, rmx);
+  set_branch(b, at, axis, lc, rc, lmn, lmx, rmn, rmx, mn, mx);
+  return at;
+}
+
+/* Root handling of Build32: the root is always a branch over [0, n-1] whose
+ * children start at depth 0. render/render_accel_triangle.cc:1547-1642 */
+static int64_t build_common(build_ctx *b, int64_t n, double bounds[6]) {
+  uint32_t *codes = (uint32_t *)malloc(4 * (size_t)n);
+  uint32_t *scode = (uint32_t *)malloc(4 * (size_t)n);
+  or_nodeinfo *info = (or_nodeinfo *)malloc(sizeof(or_nodeinfo) * (size_t)n);
+  float bmin[3], bmax[3];
+  if (b->kind == 0) {
+    or_scene_bounds_tri(b->V, b->F, n, bmin, bmax);
+    or_morton_tri(b->V, b->F, n, bmin, bmax, codes);
+  } else if (b->kind == 1) {
+    or_scene_bounds_tet(b->V, b->F, n, bmin, bmax);
+    or_morton_tet(b->V, b->F, n, bmin, bmax, codes);
+  } else {
+    or_scene_bounds_sph(b->V, b->R, b->rconst, n, bmin, bmax);
+    or_morton_pts(b->V, n, bmin, bmax, codes);
+  }
+  or_sort_keys(codes, n, scode, (uint32_t *)b->sidx);
+  or_radix_tree30(scode, n, info);
+  b->scode = scode;
+  b->info = info;
+  b->count = 0;
+  int64_t root = push_node(b);
+  uint32_t mid = info[0].child;
+  float lmn[3], lmx[3], rmn[3], rmx[3], mn[3], mx[3];
+  int64_t lc = build_rec(b, mid, 0, mid, info[0].left_type == 1, 0, lmn, lmx);
+  int64_t rc = build_rec(b, mid + 1, mid + 1, (uint32_t)(n - 1),
+                         info[0].right_type == 1, 0, rmn, rmx);
+  merge_box(mn, mx, lmn, lmx, rmn, rmx);
+  set_branch(b, root, split_axis(scode[0]), lc, rc, lmn, lmx, rmn, rmx, mn, mx);
+  for (int k = 0; k < 3; k++) {
+    bounds[k] = mn[k];
+    bounds[3 + k] = mx[k];
+  }
+  free(codes);
+  free(scode);
+  free(info);
+  return b->count;
+}
+
+/* TriangleAccel::Build32, render/render_accel_triangle.cc:1389-1690 */
+int64_t or_build_tri(const float *V, const uint32_t *F, int64_t n,
+                     or_tri_node *nodes, uint32_t *indices, double bounds[6],
+                     double *eps_scale) {
+  float bmin[3], bmax[3];
+  or_scene_bounds_tri(V, F, n, bmin, bmax);
+  /* epsScale = max extent, each extent a float subtraction held in double:
+   * render/render_accel_triangle.cc:1451-1463 */
+  double bs[3];
+  for (int k = 0; k < 3; k++) bs[k] = (double)(bmax[k] - bmin[k]);
+  double es = bs[0];
+  if (es < bs[1]) es = bs[1];
+  if (es < bs[2]) es = bs[2];
+  if (eps_scale) *eps_scale = es;
+  build_ctx b;
+  memset(&b, 0, sizeof(b));
+  b.kind = 0;
+  b.V = V;
+  b.F = F;
+  b.kEPS = FLT_EPSILON * (float)es; /* ComputeBoundingBox30 real kEPS */
+  b.sidx = indices;
+  b.max_leaf = TRI_MAX_LEAF;
+  b.nodes = nodes;
+  return build_common(&b, n, bounds);
+}
+
+/* TetraAccel::Build32, render/render_accel_tetra.cc:1579-1898 */
+int64_t or_build_tet(const float *V, const uint32_t *F, int64_t n,
+                     or_tet_node *nodes, uint32_t *indices, double bounds[6]) {
+  build_ctx b;
+  memset(&b, 0, sizeof(b));
+  b.kind = 1;
+  b.V = V;
+  b.F = F;
+  b.kEPS = FLT_EPSILON * 16;
+  b.sidx = indices;
+  b.max_leaf = TET_MAX_LEAF;
+  b.nodes = nodes;
+  return build_common(&b, n, bounds);
+}
+
+/* ParticleAccel::Build32, render/render_accel_particle.cc:1560-1866 */
+int64_t or_build_sph(const float *P, const float *R, float rconst, int64_t n,
+                     or_sph_node *nodes, uint32_t *indices, double bounds[6]) {
+  build_ctx b;
+  memset(&b, 0, sizeof(b));
+  b.kind = 2;
+  b.V = P;
+  b.R = R;
+  b.rconst = rconst;
+  b.kEPS = (float)(FLT_EPSILON * 16.0);
+  b.sidx = indices;
+  b.max_leaf = SPH_MAX_LEAF;
+  b.nodes = nodes;
+  return build_common(&b, n, bounds);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Traversal                                                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  float o[3], d[3], inv[3];
+  int sign[3];
+} ray_setup;
+
+/* Traverse prologue, render/render_accel_triangle.cc:2523-2538:
+ * rayInvDir = (float)(1.0 / dir) (no safe invert). */
+static void setup_ray(const or_ray *r, ray_setup *s) {
+  for (int k = 0; k < 3; k++) {
+    s->o[k] = r->org[k];
+    s->d[k] = r->dir[k];
+    s->sign[k] = r->dir[k] < 0.0 ? 1 : 0;
+    s->inv[k] = (float)(1.0 / (double)r->dir[k]);
+  }
+}
+
+/* IntersectRayAABB, render/render_accel_triangle.cc:1815-1861 (two-box nodes:
+ * tmax clamped by maxT). clamp_tmax = 0 selects the tetra variant
+ * render/render_accel_tetra.cc:1934-1978 (tmin <= maxT instead). */
+static int ray_aabb(const ray_setup *s, float maxT, const float bmin[3],
+                    const float bmax[3], int clamp_tmax) {
+  float mnx = s->sign[0] ? bmax[0] : bmin[0];
+  float mny = s->sign[1] ? bmax[1] : bmin[1];
+  float mnz = s->sign[2] ? bmax[2] : bmin[2];
+  float mxx = s->sign[0] ? bmin[0] : bmax[0];
+  float mxy = s->sign[1] ? bmin[1] : bmax[1];
+  float mxz = s->sign[2] ? bmin[2] : bmax[2];
+  float tminx = (mnx - s->o[0]) * s->inv[0];
+  float tmaxx = (mxx - s->o[0]) * s->inv[0];
+  float tminy = (mny - s->o[1]) * s->inv[1];
+  float tmaxy = (mxy - s->o[1]) * s->inv[1];
+  float tmin = (tminx > tminy) ? tminx : tminy;
+  float tmax = (tmaxx < tmaxy) ? tmaxx : tmaxy;
+  float tminz = (mnz - s->o[2]) * s->inv[2];
+  float tmaxz = (mxz - s->o[2]) * s->inv[2];
+  tmin = (tmin > tminz) ? tmin : tminz;
+  tmax = (tmax < tmaxz) ? tmax : tmaxz;
+  if (clamp_tmax) {
+    tmax = (maxT < tmax) ? maxT : tmax; /* std::min(tmax, maxT) */
+    return (tmax > 0.0f) && (tmin <= tmax);
+  }
+  return (tmax > 0.0f) && (tmin <= tmax) && (tmin <= maxT);
+}
+
+/* TriangleIsect (Moller-Trumbore), render/render_accel_triangle.cc:1887-1938 */
+static int tri_isect(float *tio, float *uo, float *vo, const float *p0,
+                     const float *p1, const float *p2, const ray_setup *s,
+                     int double_sided, float eps_scale) {
+  const float kEPS = FLT_EPSILON * eps_scale;
+  float e1[3], e2[3], p[3], q[3], sv[3];
+  for (int k = 0; k < 3; k++) {
+    e1[k] = p1[k] - p0[k];
+    e2[k] = p2[k] - p0[k];
+  }
+  const float *d = s->d;
+  p[0] = d[1] * e2[2] - d[2] * e2[1];
+  p[1] = d[2] * e2[0] - d[0] * e2[2];
+  p[2] = d[0] * e2[1] - d[1] * e2[0];
+  float det = (e1[0] * p[0] + e1[1] * p[1]) + e1[2] * p[2];
+  if (double_sided) {
+    if (fabsf(det) < kEPS) return 0;
+  } else {
+    if (det < kEPS) return 0;
+  }
+  float invDet = (float)(1.0 / (double)det);
+  for (int k = 0; k < 3; k++) sv[k] = s->o[k] - p0[k];
+  q[0] = sv[1] * e1[2] - sv[2] * e1[1];
+  q[1] = sv[2] * e1[0] - sv[0] * e1[2];
+  q[2] = sv[0] * e1[1] - sv[1] * e1[0];
+  float u = ((sv[0] * p[0] + sv[1] * p[1]) + sv[2] * p[2]) * invDet;
+  float v = ((q[0] * d[0] + q[1] * d[1]) + q[2] * d[2]) * invDet;
+  float t = ((e2[0] * q[0] + e2[1] * q[1]) + e2[2] * q[2]) * invDet;
+  if (u < 0.0 || u > 1.0) return 0;
+  if (v < 0.0 || u + v > 1.0f) return 0;
+  if (t < 0.0 || t > *tio) return 0;
+  *tio = t;
+  *uo = u;
+  *vo = v;
+  return 1;
+}
+
+/* TriangleAccel::Traverse, render/render_accel_triangle.cc:2511-2669, with
+ * TestLeafNode (scalar path) :2386-2433. */
+static void trace_tri_one(const or_tri_node *nodes, const uint32_t *indices,
+                          const float *V, const uint32_t *F, float eps_scale,
+                          const or_ray *r, or_hit *h, uint32_t *st) {
+  ray_setup s;
+  setup_ray(r, &s);
+  float hitT = FLT_MAX;
+  h->t = hitT;
+  h->u = 0.0f;
+  h->v = 0.0f;
+  h->prim_id = 0xffffffffu;
+  int stack[STACK_MAX];
+  int sp = 0;
+  stack[0] = 0;
+  uint32_t nvisit = 0, ntest = 0;
+  const int ds = (int)(r->flags & 1u);
+  while (sp >= 0) {
+    const or_tri_node *nd = nodes + stack[sp--];
+    nvisit++;
+    if (nd->flag == 0) {
+      int hit0 = ray_aabb(&s, hitT, nd->bmin[0], nd->bmax[0], 1);
+      int hit1 = ray_aabb(&s, hitT, nd->bmin[1], nd->bmax[1], 1);
+      int hits[2] = {hit0, hit1};
+      if (hit0 || hit1) {
+        int nearc = s.sign[nd->axis], farc = 1 - nearc;
+        if (hits[farc]) stack[++sp] = (int)nd->data[farc];
+        if (hits[nearc]) stack[++sp] = (int)nd->data[nearc];
+      }
+    } else {
+      float t = h->t;
+      int any = 0;
+      for (uint32_t i = 0; i < nd->data[0]; i++) {
+        uint32_t fi = indices[i + nd->data[1]];
+        if (fi == r->prev_prim_id) continue;
+        ntest++;
+        const float *p0 = V + 3 * (size_t)F[3 * (size_t)fi + 0];
+        const float *p1 = V + 3 * (size_t)F[3 * (size_t)fi + 1];
+        const float *p2 = V + 3 * (size_t)F[3 * (size_t)fi + 2];
+        float u, v;
+        if (tri_isect(&t, &u, &v, p0, p1, p2, &s, ds, eps_scale)) {
+          h->t = t;
+          h->u = u;
+          h->v = v;
+          h->prim_id = fi;
+          any = 1;
+        }
+      }
+      if (any) hitT = h->t;
+    }
+  }
+  if (st) {
+    st[0] = nvisit;
+    st[1] = ntest;
+  }
+}
+
+void or_trace_tri(const or_tri_node *nodes, const uint32_t *indices,
+                  const float *V, const uint32_t *F, double eps_scale,
+                  const or_ray *rays, int64_t nrays, or_hit *hits,
+                  uint32_t *stats, int threads) {
+  const float es = (float)eps_scale; /* TriangleIsect takes real epsScale */
+  (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+#endif
+  for (int64_t i = 0; i < nrays; i++)
+    trace_tri_one(nodes, indices, V, F, es, rays + i, hits + i,
+                  stats ? stats + 2 * i : NULL);
+}
+
+/* ---- tetra ---- */
+
+typedef struct {
+  double x, y, z;
+} d3;
+
+static d3 d3sub(d3 a, d3 b) { d3 c = {a.x - b.x, a.y - b.y, a.z - b.z}; return c; }
+static d3 d3add(d3 a, d3 b) { d3 c = {a.x + b.x, a.y + b.y, a.z + b.z}; return c; }
+/* operator*(double f, const double3 &v) = (v.x * f, ...), render/render_common.h:172-174 */
+static d3 d3scale(double f, d3 v) { d3 c = {v.x * f, v.y * f, v.z * f}; return c; }
+/* vcrossd / vdotd, render/render_accel_tetra.cc:75-86 */
+static d3 d3cross(d3 a, d3 b) {
+  d3 c = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+  return c;
+}
+static double d3dot(d3 a, d3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+/* fsign, render/render_accel_tetra.cc:89-93 */
+static int fsign(double x) {
+  const double eps = DBL_EPSILON;
+  return x > eps ? 1 : (x < -eps ? -1 : 0);
+}
+
+/* Pluecker(v0, v1) = {d = v1 - v0, c = v1 x v0} and its inner product,
+ * render/render_accel_tetra.cc:98-115 */
+typedef struct {
+  d3 d, c;
+} pluecker;
+static pluecker mk_pl(d3 v0, d3 v1) {
+  pluecker p;
+  p.d = d3sub(v1, v0);
+  p.c = d3cross(v1, v0);
+  return p;
+}
+static double pl_dot(pluecker a, pluecker b) {
+  return d3dot(a.d, b.c) + d3dot(b.d, a.c);
+}
+
+/* ComputeParametricDist, render/render_accel_tetra.cc:123-139 */
+static void param_dist(d3 o, d3 d, d3 ep, d3 lp, double *te, double *tl) {
+  if (d.x != 0.0) {
+    double inv = 1.0 / d.x;
+    *te = (ep.x - o.x) * inv;
+    *tl = (lp.x - o.x) * inv;
+  } else if (d.y != 0.0) {
+    double inv = 1.0 / d.y;
+    *te = (ep.y - o.y) * inv;
+    *tl = (lp.y - o.y) * inv;
+  } else {
+    double inv = 1.0 / d.z;
+    *te = (ep.z - o.z) * inv;
+    *tl = (lp.z - o.z) * inv;
+  }
+}
+
+/* (1 - a - b) * A + a * B + b * C as the double3 operators evaluate it. */
+static d3 bary3(double a, double b, d3 A, d3 B, d3 C) {
+  return d3add(d3add(d3scale((1 - a) - b, A), d3scale(a, B)), d3scale(b, C));
+}
+
+/* RayTetraPluecker (Platis & Theoharis 2003), render/render_accel_tetra.cc:243-567 */
+static int ray_tet_pluecker(d3 orig, d3 dir, const d3 vert[4], int *enterFace,
+                            int *leaveFace, double *uE1, double *uE2,
+                            double *uL1, double *uL2, double *tE,
+                            double *tL) {
+  *enterFace = -1;
+  *leaveFace = -1;
+  double uAB = 0, uAC = 0, uDB = 0, uDC = 0, uBC = 0, uAD = 0;
+  int signAB = -2, signAC = -2, signDB = -2, signDC = -2, signBC = -2,
+      signAD = -2;
+  d3 dest = d3add(orig, dir);
+  pluecker plRay = mk_pl(orig, dest);
+  int nextSign = 0;
+  d3 ep = {0, 0, 0}, lp = {0, 0, 0};
+
+  uAB = pl_dot(plRay, mk_pl(vert[0], vert[1]));
+  signAB = fsign(uAB);
+  uAC = pl_dot(plRay, mk_pl(vert[0], vert[2]));
+  signAC = fsign(uAC);
+
+  if ((signAC == -signAB) || (signAC == 0) || (signAB == 0)) {
+    uBC = pl_dot(plRay, mk_pl(vert[1], vert[2]));
+    signBC = fsign(uBC);
+    int signABC = signAB;
+    if (signABC == 0) {
+      signABC = -signAC;
+      if (signABC == 0) signABC = signBC;
+    }
+    if ((signABC != 0) && ((signBC == signABC) || (signBC == 0))) {
+      double invVolABC = 1.0 / ((uAB + uBC) - uAC);
+      if (signABC == 1) {
+        *enterFace = 3;
+        *uE1 = -uAC * invVolABC;
+        *uE2 = uAB * invVolABC;
+        ep = bary3(*uE1, *uE2, vert[0], vert[1], vert[2]);
+        nextSign = -1;
+      } else {
+        *leaveFace = 3;
+        *uL1 = -uAC * invVolABC;
+        *uL2 = uAB * invVolABC;
+        lp = bary3(*uL1, *uL2, vert[0], vert[1], vert[2]);
+        nextSign = 1;
+      }
+      uAD = pl_dot(plRay, mk_pl(vert[0], vert[3]));
+      signAD = fsign(uAD);
+      if ((signAD == nextSign) || (signAD == 0)) {
+        uDB = pl_dot(plRay, mk_pl(vert[3], vert[1]));
+        signDB = fsign(uDB);
+        if ((signDB == nextSign) ||
+            ((signDB == 0) && ((signAD != 0) || (signAB != 0)))) {
+          double invVolBAD = 1.0 / ((uAD + uDB) - uAB);
+          if (nextSign == 1) {
+            *enterFace = 2;
+            *uE1 = uDB * invVolBAD;
+            *uE2 = -uAB * invVolBAD;
+            ep = bary3(*uE1, *uE2, vert[1], vert[0], vert[3]);
+          } else {
+            *leaveFace = 2;
+            *uL1 = uDB * invVolBAD;
+            *uL2 = -uAB * invVolBAD;
+            lp = bary3(*uL1, *uL2, vert[1], vert[0], vert[3]);
+          }
+          param_dist(orig, dir, ep, lp, tE, tL);
+          return 1;
+        }
+      }
+      uDC = pl_dot(plRay, mk_pl(vert[3], vert[2]));
+      signDC = fsign(uDC);
+      if ((signDC == -nextSign) ||
+          ((signDC == 0) && ((signAD != 0) || (signAC != 0)))) {
+        double invVolCDA = 1.0 / ((uAC - uDC) - uAD);
+        if (nextSign == 1) {
+          *enterFace = 1;
+          *uE1 = uAC * invVolCDA;
+          *uE2 = -uDC * invVolCDA;
+          ep = bary3(*uE1, *uE2, vert[2], vert[3], vert[0]);
+        } else {
+          *leaveFace = 1;
+          *uL1 = uAC * invVolCDA;
+          *uL2 = -uDC * invVolCDA;
+          lp = bary3(*uL1, *uL2, vert[2], vert[3], vert[0]);
+        }
+        param_dist(orig, dir, ep, lp, tE, tL);
+        return 1;
+      } else {
+        if (signDB == -2) uDB = pl_dot(plRay, mk_pl(vert[3], vert[1]));
+        double invVolDCB = 1.0 / ((uDC - uBC) - uDB);
+        if (nextSign == 1) {
+          *enterFace = 0;
+          *uE1 = -uDB * invVolDCB;
+          *uE2 = uDC * invVolDCB;
+          ep = bary3(*uE1, *uE2, vert[3], vert[2], vert[1]);
+        } else {
+          *leaveFace = 0;
+          *uL1 = -uDB * invVolDCB;
+          *uL2 = uDC * invVolDCB;
+          lp = bary3(*uL1, *uL2, vert[3], vert[2], vert[1]);
+        }
+        param_dist(orig, dir, ep, lp, tE, tL);
+        return 1;
+      }
+    }
+  }
+
+  uAD = pl_dot(plRay, mk_pl(vert[0], vert[3]));
+  signAD = fsign(uAD);
+  if ((signAD == -signAB) || (signAB == 0) || (signAD == 0)) {
+    uDB = pl_dot(plRay, mk_pl(vert[3], vert[1]));
+    signDB = fsign(uDB);
+    int signBAD = -signAB;
+    if (signBAD == 0) {
+      signBAD = signAD;
+      if (signBAD == 0) signBAD = signDB;
+    }
+    if ((signBAD != 0) && ((signDB == signBAD) || (signDB == 0))) {
+      double invVolBAD = 1.0 / ((uAD + uDB) - uAB);
+      if (signBAD == 1) {
+        *enterFace = 2;
+        *uE1 = uDB * invVolBAD;
+        *uE2 = -uAB * invVolBAD;
+        ep = bary3(*uE1, *uE2, vert[1], vert[0], vert[3]);
+        nextSign = -1;
+      } else {
+        *leaveFace = 2;
+        *uL1 = uDB * invVolBAD;
+        *uL2 = -uAB * invVolBAD;
+        lp = bary3(*uL1, *uL2, vert[1], vert[0], vert[3]);
+        nextSign = 1;
+      }
+      uDC = pl_dot(plRay, mk_pl(vert[3], vert[2]));
+      signDC = fsign(uDC);
+      if ((signDC == -nextSign) ||
+          ((signDC == 0) && ((signAD != 0) || (signAC != 0)))) {
+        double invVolCDA = 1.0 / ((uAC - uDC) - uAD);
+        if (nextSign == 1) {
+          *enterFace = 1;
+          *uE1 = uAC * invVolCDA;
+          *uE2 = -uDC * invVolCDA;
+          ep = bary3(*uE1, *uE2, vert[2], vert[3], vert[0]);
+        } else {
+          *leaveFace = 1;
+          *uL1 = uAC * invVolCDA;
+          *uL2 = -uDC * invVolCDA;
+          lp = bary3(*uL1, *uL2, vert[2], vert[3], vert[0]);
+        }
+        param_dist(orig, dir, ep, lp, tE, tL);
+        return 1;
+      } else {
+        if (signBC == -2) uBC = pl_dot(plRay, mk_pl(vert[1], vert[2]));
+        double invVolDCB = 1.0 / ((uDC - uBC) - uDB);
+        if (nextSign == 1) {
+          *enterFace = 0;
+          *uE1 = -uDB * invVolDCB;
+          *uE2 = uDC * invVolDCB;
+          ep = bary3(*uE1, *uE2, vert[3], vert[2], vert[1]);
+        } else {
+          *leaveFace = 0;
+          *uL1 = -uDB * invVolDCB;
+          *uL2 = uDC * invVolDCB;
+          lp = bary3(*uL1, *uL2, vert[3], vert[2], vert[1]);
+        }
+        param_dist(orig, dir, ep, lp, tE, tL);
+        return 1;
+      }
+    }
+  }
+
+  if ((-signAD == signAC) || (signAC == 0) || (signAD == 0)) {
+    uDC = pl_dot(plRay, mk_pl(vert[3], vert[2]));
+    signDC = fsign(uDC);
+    int signCDA = signAC;
+    if (signCDA == 0) {
+      signCDA = -signAD;
+      if (signCDA == 0) signCDA = -signDC;
+    }
+    if ((signCDA != 0) && ((-signDC == signCDA) || (signDC == 0))) {
+      double invVolCDA = 1.0 / ((uAC - uDC) - uAD);
+      if (signBC == -2) uBC = pl_dot(plRay, mk_pl(vert[1], vert[2]));
+      if (signDB == -2) uDB = pl_dot(plRay, mk_pl(vert[3], vert[1]));
+      double invVolDCB = 1.0 / ((uDC - uBC) - uDB);
+      if (signCDA == 1) {
+        *enterFace = 1;
+        *uE1 = uAC * invVolCDA;
+        *uE2 = -uDC * invVolCDA;
+        ep = bary3(*uE1, *uE2, vert[2], vert[3], vert[0]);
+        *leaveFace = 0;
+        *uL1 = -uDB * invVolDCB;
+        *uL2 = uDC * invVolDCB;
+        lp = bary3(*uL1, *uL2, vert[3], vert[2], vert[1]);
+      } else {
+        *leaveFace = 1;
+        *uL1 = uAC * invVolCDA;
+        *uL2 = -uDC * invVolCDA;
+        lp = bary3(*uL1, *uL2, vert[2], vert[3], vert[0]);
+        *enterFace = 0;
+        *uE1 = -uDB * invVolDCB;
+        *uE2 = uDC * invVolDCB;
+        ep = bary3(*uE1, *uE2, vert[3], vert[2], vert[1]);
+      }
+      param_dist(orig, dir, ep, lp, tE, tL);
+      return 1;
+    }
+  }
+  return 0;
+}
+
+/* float real3 helpers for InterpolateTetra (render/render_common.h:30-160) */
+typedef struct {
+  float v[3];
+} f3;
+static f3 f3sub(f3 a, f3 b) {
+  f3 c = {{a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]}};
+  return c;
+}
+static f3 f3add(f3 a, f3 b) {
+  f3 c = {{a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2]}};
+  return c;
+}
+static f3 f3mul(f3 a, float f) {
+  f3 c = {{a.v[0] * f, a.v[1] * f, a.v[2] * f}};
+  return c;
+}
+static f3 f3div(f3 a, float f) {
+  f3 c = {{a.v[0] / f, a.v[1] / f, a.v[2] / f}};
+  return c;
+}
+static f3 f3cross(f3 l, f3 r) {
+  f3 c = {{l.v[1] * r.v[2] - l.v[2] * r.v[1], l.v[2] * r.v[0] - l.v[0] * r.v[2],
+           l.v[0] * r.v[1] - l.v[1] * r.v[0]}};
+  return c;
+}
+static float f3dot(f3 l, f3 r) {
+  return (l.v[0] * r.v[0] + l.v[1] * r.v[1]) + l.v[2] * r.v[2];
+}
+/* real3::length: sqrt of the float sum of squares (correctly rounded). */
+static float f3len(f3 a) {
+  float s = (a.v[0] * a.v[0] + a.v[1] * a.v[1]) + a.v[2] * a.v[2];
+  return (float)sqrt((double)s);
+}
+
+static int contains3(int v, const int l[3]) {
+  return l[0] == v || l[1] == v || l[2] == v;
+}
+
+/* InterpolateTetra (interpolate_mode 0), render/render_accel_tetra.cc:148-241 */
+static void interpolate_tetra(float d_out[4], f3 pt, const f3 verts[4]) {
+  static const int faces[4][3] = {{0, 2, 1}, {1, 2, 3}, {0, 3, 2}, {0, 1, 3}};
+  const float kEPS = (float)0.001;
+  f3 edges[6];
+  edges[0] = f3sub(verts[1], verts[0]);
+  edges[1] = f3sub(verts[2], verts[1]);
+  edges[2] = f3sub(verts[0], verts[2]);
+  edges[3] = f3sub(verts[3], verts[0]);
+  edges[4] = f3sub(verts[3], verts[1]);
+  edges[5] = f3sub(verts[3], verts[2]);
+  for (int v = 0; v < 4; v++) {
+    f3 rayorg = verts[v];
+    f3 raydir = f3sub(pt, verts[v]);
+    f3 raypc = f3cross(raydir, rayorg);
+    float d = f3len(f3sub(pt, rayorg));
+    float ws[6];
+    int cw0[6], cw1[6];
+    for (int i = 0; i < 6; i++) {
+      ws[i] = f3dot(raydir, f3cross(edges[i], verts[i % 3])) +
+              f3dot(edges[i], raypc);
+      cw0[i] = ws[i] >= -kEPS;
+      cw1[i] = ws[i] <= kEPS;
+    }
+    f3 cp = {{(float)1e16, (float)1e16, (float)1e16}};
+    if (cw0[0] && cw0[1] && cw0[2] && !contains3(v, faces[0]))
+      cp = f3div(f3add(f3add(f3mul(verts[2], ws[0]), f3mul(verts[0], ws[1])),
+                       f3mul(verts[1], ws[2])),
+                 (ws[0] + ws[1]) + ws[2]);
+    else if (cw1[1] && cw0[4] && cw1[5] && !contains3(v, faces[1]))
+      cp = f3div(f3sub(f3add(f3mul(verts[3], ws[1]), f3mul(verts[1], ws[5])),
+                       f3mul(verts[2], ws[4])),
+                 (ws[5] + ws[1]) - ws[4]);
+    else if (cw1[2] && cw1[3] && cw0[5] && !contains3(v, faces[2]))
+      cp = f3div(f3sub(f3add(f3mul(verts[3], ws[2]), f3mul(verts[2], ws[3])),
+                       f3mul(verts[0], ws[5])),
+                 (ws[2] + ws[3]) - ws[5]);
+    else if (cw1[0] && cw0[3] && cw1[4] && !contains3(v, faces[3]))
+      cp = f3div(f3sub(f3add(f3mul(verts[3], ws[0]), f3mul(verts[0], ws[4])),
+                       f3mul(verts[1], ws[3])),
+                 (ws[0] + ws[4]) - ws[3]);
+    float dv = d / f3len(f3sub(cp, rayorg));
+    if (dv < 0)
+      dv = 0;
+    else if (!(dv < 1) && !(dv > 0))
+      dv = 1;
+    d_out[v] = 1 - dv;
+  }
+  float ds = 0, w = 0, dd[4];
+  for (int i = 0; i < 4; i++) ds += f3len(f3sub(pt, verts[i]));
+  for (int i = 0; i < 4; i++) dd[i] = ds / f3len(f3sub(pt, verts[i]));
+  for (int i = 0; i < 4; i++) {
+    d_out[i] *= dd[i];
+    w += d_out[i];
+  }
+  for (int i = 0; i < 4; i++) d_out[i] = d_out[i] / w;
+}
+
+/* TetraAccel::Traverse, render/render_accel_tetra.cc:2200-2289; TestLeafNode
+ * :1980-2122 (float vertices widened to double); BuildIntersection
+ * :2124-2196; shader-side varying f0*d0 + f1*d1 + f2*d2 + f3*d3,
+ * gles/gles_common.cc:1074-1076. */
+static void trace_tet_one(const or_tet_node *nodes, const uint32_t *indices,
+                          const float *V, const uint32_t *F,
+                          const float *scalars, const or_ray *r, or_hit *h,
+                          or_tet_ext *ext, uint32_t *st) {
+  ray_setup s;
+  setup_ray(r, &s);
+  float hitT = FLT_MAX;
+  h->t = hitT;
+  h->u = 0.0f;
+  h->v = 0.0f;
+  h->prim_id = 0xffffffffu;
+  uint32_t subface = 0;
+  double tleave_best = 0.0;
+  int stack[STACK_MAX];
+  int sp = 0;
+  stack[0] = 0;
+  uint32_t nvisit = 0, ntest = 0;
+  d3 ro = {s.o[0], s.o[1], s.o[2]}, rd = {s.d[0], s.d[1], s.d[2]};
+  while (sp >= 0) {
+    const or_tet_node *nd = nodes + stack[sp--];
+    nvisit++;
+    int hit = ray_aabb(&s, hitT, nd->bmin, nd->bmax, 0);
+    if (nd->flag == 0) {
+      if (hit) {
+        int nearc = s.sign[nd->axis], farc = 1 - nearc;
+        stack[++sp] = (int)nd->data[farc];
+        stack[++sp] = (int)nd->data[nearc];
+      }
+    } else if (hit) {
+      float t = h->t;
+      int any = 0;
+      for (uint32_t i = 0; i < nd->data[0]; i++) {
+        uint32_t fi = indices[i + nd->data[1]];
+        if (fi == r->prev_prim_id) continue;
+        ntest++;
+        d3 vtx[4];
+        for (int j = 0; j < 4; j++) {
+          const float *p = V + 3 * (size_t)F[4 * (size_t)fi + j];
+          vtx[j].x = p[0];
+          vtx[j].y = p[1];
+          vtx[j].z = p[2];
+        }
+        int ef, lf;
+        double eu = 0, lu = 0, ev = 0, lv = 0, et = 0, lt = 0;
+        /* argument order of the call site, render/render_accel_tetra.cc:2103:
+         * (enterU, leaveU, enterV, leaveV) land in (uEnter1, uEnter2,
+         * uLeave1, uLeave2), so isect.v = uLeave1. */
+        if (ray_tet_pluecker(ro, rd, vtx, &ef, &lf, &eu, &lu, &ev, &lv, &et,
+                             &lt)) {
+          if ((et >= 0.0) && (et < (double)t)) {
+            h->t = (float)et;
+            h->u = (float)eu;
+            h->v = (float)ev;
+            h->prim_id = fi;
+            subface = (uint32_t)ef;
+            tleave_best = lt;
+            t = (float)et;
+            any = 1;
+          }
+        }
+      }
+      if (any) hitT = h->t;
+    }
+  }
+  if (st) {
+    st[0] = nvisit;
+    st[1] = ntest;
+  }
+  if (ext) {
+    memset(ext, 0, sizeof(*ext));
+    if (h->t < FLT_MAX) {
+      f3 p[4];
+      uint32_t fidx[4];
+      for (int j = 0; j < 4; j++) {
+        fidx[j] = F[4 * (size_t)h->prim_id + j];
+        for (int k = 0; k < 3; k++) p[j].v[k] = V[3 * (size_t)fidx[j] + k];
+      }
+      f3 pos;
+      for (int k = 0; k < 3; k++) pos.v[k] = s.o[k] + s.d[k] * h->t;
+      interpolate_tetra(ext->d, pos, p);
+      ext->subface = subface;
+      ext->t_leave = (float)tleave_best;
+      if (scalars) {
+        float f0 = scalars[fidx[0]], f1 = scalars[fidx[1]];
+        float f2 = scalars[fidx[2]], f3v = scalars[fidx[3]];
+        ext->scalar = ((f0 * ext->d[0] + f1 * ext->d[1]) + f2 * ext->d[2]) +
+                      f3v * ext->d[3];
+      }
+    }
+  }
+}
+
+void or_trace_tet(const or_tet_node *nodes, const uint32_t *indices,
+                  const float *V, const uint32_t *F, const float *scalars,
+                  const or_ray *rays, int64_t nrays, or_hit *hits,
+                  or_tet_ext *ext, uint32_t *stats, int threads) {
+  (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+#endif
+  for (int64_t i = 0; i < nrays; i++)
+    trace_tet_one(nodes, indices, V, F, scalars, rays + i, hits + i,
+                  ext ? ext + i : NULL, stats ? stats + 2 * i : NULL);
+}
+
+/* ---- spheres ---- */
+
+/* SphereIsect, render/render_accel_particle.cc:2423-2512 */
+static int sphere_isect(float *tio, const float *c, float radius,
+                        const ray_setup *s, double prevT, const float *prevN,
+                        int self) {
+  float oc[3];
+  for (int k = 0; k < 3; k++) oc[k] = s->o[k] - c[k];
+  const float *d = s->d;
+  float a = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+  float b = (float)(2.0 * (double)((d[0] * oc[0] + d[1] * oc[1]) + d[2] * oc[2]));
+  float cc = ((oc[0] * oc[0] + oc[1] * oc[1]) + oc[2] * oc[2]) - radius * radius;
+  float disc = (float)((double)(b * b) - (4.0 * (double)a) * (double)cc);
+  float t0, t1;
+  if (disc < 0.0) return 0;
+  if (disc == 0.0) {
+    t0 = t1 = (float)(-0.5 * (double)(b / a));
+  } else {
+    float dsq = (float)sqrt((double)disc);
+    float q;
+    if (b < 0)
+      q = (float)((double)(-b - dsq) / 2.0);
+    else
+      q = (float)((double)(-b + dsq) / 2.0);
+    t0 = q / a;
+    t1 = cc / q;
+  }
+  if (t0 > t1) {
+    float tmp = t0;
+    t0 = t1;
+    t1 = tmp;
+  }
+  if (t1 < 0) return 0;
+  if (self) {
+    float t = t1;
+    float nrm[3];
+    for (int k = 0; k < 3; k++) nrm[k] = (s->o[k] + t * d[k]) - c[k];
+    float nd = (nrm[0] * prevN[0] + nrm[1] * prevN[1]) + nrm[2] * prevN[2];
+    if ((t < *tio) && ((double)t < prevT) && (nd <= 0.0)) {
+      *tio = t;
+      return 1;
+    }
+  } else {
+    float t = (t0 < 0) ? t1 : t0;
+    if (t < *tio) {
+      *tio = t;
+      return 1;
+    }
+  }
+  return 0;
+}
+
+/* ParticleAccel::Traverse, render/render_accel_particle.cc:3053-3218 with the
+ * scalar TestLeafNode :2809-2829; u, v from BuildIntersection :2968-3049. */
+static void trace_sph_one(const or_sph_node *nodes, const uint32_t *indices,
+                          const float *P, const float *R, float rconst,
+                          const or_ray *r, double prevT, const float *prevN,
+                          or_hit *h, uint32_t *st) {
+  static const float zero3[3] = {0, 0, 0};
+  ray_setup s;
+  setup_ray(r, &s);
+  float hitT = FLT_MAX;
+  h->t = hitT;
+  h->u = 0.0f;
+  h->v = 0.0f;
+  h->prim_id = 0xffffffffu;
+  int stack[STACK_MAX];
+  int sp = 0;
+  stack[0] = 0;
+  uint32_t nvisit = 0, ntest = 0;
+  if (!prevN) prevN = zero3;
+  while (sp >= 0) {
+    const or_sph_node *nd = nodes + stack[sp--];
+    nvisit++;
+    if (nd->flag == 0) {
+      int hit0 = ray_aabb(&s, hitT, nd->bmin[0], nd->bmax[0], 1);
+      int hit1 = ray_aabb(&s, hitT, nd->bmin[1], nd->bmax[1], 1);
+      int hits[2] = {hit0, hit1};
+      if (hit0 || hit1) {
+        int nearc = s.sign[nd->axis], farc = 1 - nearc;
+        if (hits[farc]) stack[++sp] = (int)nd->data[farc];
+        if (hits[nearc]) stack[++sp] = (int)nd->data[nearc];
+      }
+    } else {
+      float t = h->t;
+      int any = 0;
+      for (uint32_t i = 0; i < nd->data[0]; i++) {
+        uint32_t pi = indices[i + nd->data[1]];
+        ntest++;
+        if (sphere_isect(&t, P + 3 * (size_t)pi, sph_radius(R, rconst, pi), &s,
+                         prevT, prevN, r->prev_prim_id == pi)) {
+          h->t = t;
+          h->prim_id = pi;
+          any = 1;
+        }
+      }
+      if (any) hitT = h->t;
+    }
+  }
+  if (h->t < FLT_MAX) {
+    /* vnormalize, render/render_accel_particle.cc:2041-2050, then the
+     * spherical u, v of BuildIntersection (libm; tolerance-only parity). */
+    float cp[3];
+    for (int k = 0; k < 3; k++)
+      cp[k] = (s.o[k] + h->t * s.d[k]) - P[3 * (size_t)h->prim_id + k];
+    const float kEPS = (float)(FLT_EPSILON * 16.0);
+    float len = (float)sqrt((double)((cp[0] * cp[0] + cp[1] * cp[1]) + cp[2] * cp[2]));
+    float n[3] = {cp[0], cp[1], cp[2]};
+    if (len > kEPS) {
+      float invlen = (float)(1.0 / (double)len);
+      for (int k = 0; k < 3; k++) n[k] = cp[k] * invlen;
+    }
+    h->u = (float)((atan2((double)n[0], (double)n[2]) + M_PI) * 0.5 * (1.0 / M_PI));
+    h->v = (float)(acos((double)n[1]) / M_PI);
+  }
+  if (st) {
+    st[0] = nvisit;
+    st[1] = ntest;
+  }
+}
+
+void or_trace_sph(const or_sph_node *nodes, const uint32_t *indices,
+                  const float *P, const float *R, float rconst,
+                  const or_ray *rays, const double *prev_hit_t,
+                  const float *prev_hit_n, int64_t nrays, or_hit *hits,
+                  uint32_t *stats, int threads) {
+  (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+#endif
+  for (int64_t i = 0; i < nrays; i++)
+    trace_sph_one(nodes, indices, P, R, rconst, rays + i,
+                  prev_hit_t ? prev_hit_t[i] : 0.0,
+                  prev_hit_n ? prev_hit_n + 3 * i : NULL, hits + i,
+                  stats ? stats + 2 * i : NULL);
+}
+
+/* ------------------------------------------------------------------------ */
+/* 3D texture and volume march                                              */
+/* ------------------------------------------------------------------------ */
+
+/* fasterfloorf, render/render_texture.cc:108-121 */
+static int faster_floorf(float x) {
+  if (x >= 0) return (int)x;
+  int y = (int)x;
+  if (fabsf(x - (float)y) <= FLT_EPSILON) {
+  } else {
+    y = y - 1;
+  }
+  return y;
+}
+
+static int clampi(int v, int lo, int hi) {
+  int a = v < lo ? lo : v; /* std::max(v, low) */
+  return hi < a ? hi : a;  /* std::min(., high) */
+}
+
+/* Geometry of one trilinear fetch: the 8 corner addresses and fractions.
+ * FilterTexture3DFloat, render/render_texture.cc:946-1011 */
+typedef struct {
+  int64_t a[8]; /* 000 100 010 110 001 101 011 111 */
+  float dx, dy, dz;
+} tri_geom;
+
+static void tex3d_geom(const int dims[3], const int clamp[3], float u, float v,
+                       float r, tri_geom *g) {
+  float uu = clamp[0] ? u : u - (float)faster_floorf(u);
+  float vv = clamp[1] ? v : v - (float)faster_floorf(v);
+  float rr = clamp[2] ? r : r - (float)faster_floorf(r);
+  float vox0 = uu * ((float)dims[0] - 1.0f);
+  float vox1 = vv * ((float)dims[1] - 1.0f);
+  float vox2 = rr * ((float)dims[2] - 1.0f);
+  int vx = (int)vox0, vy = (int)vox1, vz = (int)vox2;
+  vx = clampi(vx, 0, dims[0] - 1);
+  vy = clampi(vy, 0, dims[1] - 1);
+  vz = clampi(vz, 0, dims[2] - 1);
+  int vx1 = (vx + 1 < dims[0] - 1) ? vx + 1 : dims[0] - 1;
+  int vy1 = (vy + 1 < dims[1] - 1) ? vy + 1 : dims[1] - 1;
+  int vz1 = (vz + 1 < dims[2] - 1) ? vz + 1 : dims[2] - 1;
+  g->dx = vox0 - (float)vx;
+  g->dy = vox1 - (float)vy;
+  g->dz = vox2 - (float)vz;
+  int64_t sx = 1, sy = dims[0], sz = (int64_t)dims[0] * dims[1];
+  int xs[2] = {vx, vx1}, ys[2] = {vy, vy1}, zs[2] = {vz, vz1};
+  for (int c = 0; c < 8; c++)
+    g->a[c] = zs[(c >> 2) & 1] * sz + ys[(c >> 1) & 1] * sy + xs[c & 1] * sx;
+}
+
+/* Lerp, render/render_texture.cc:97 */
+static float lerpf(float t, float a, float b) { return (1.f - t) * a + t * b; }
+
+static float tex3d_eval(const float *vox, const tri_geom *g) {
+  float d00 = lerpf(g->dx, vox[g->a[0]], vox[g->a[1]]);
+  float d10 = lerpf(g->dx, vox[g->a[2]], vox[g->a[3]]);
+  float d01 = lerpf(g->dx, vox[g->a[4]], vox[g->a[5]]);
+  float d11 = lerpf(g->dx, vox[g->a[6]], vox[g->a[7]]);
+  float d0 = lerpf(g->dy, d00, d10);
+  float d1 = lerpf(g->dy, d01, d11);
+  return lerpf(g->dz, d0, d1);
+}
+
+void or_tex3d_sample(const float *vox, const int dims[3], const int clamp[3],
+                     const float *uvw, int64_t n, float *out) {
+  for (int64_t i = 0; i < n; i++) {
+    tri_geom g;
+    tex3d_geom(dims, clamp, uvw[3 * i], uvw[3 * i + 1], uvw[3 * i + 2], &g);
+    out[i] = tex3d_eval(vox, &g);
+  }
+}
+
+/* Volume march spec (builder-defined; the reference leaves marching to user
+ * GLSL, e.g. examples/sparse_volume_render/input.frag:99-141):
+ *  - the volume fills texture space [0,1]^3, rays are given in that space;
+ *  - entry/exit by a float slab test against [0,1]^3, tnear clamped to 0;
+ *  - sample k at t_k = tnear + ((float)k + 0.5f) * step while t_k < tfar;
+ *  - s_k = FilterTexture3DFloat at o + t_k * d (single component);
+ *  - TF lookup x = clamp(s, 0, 1) * (N - 1), i0 = min((int)x, N - 2),
+ *    f = x - i0, rgba = Lerp(f, tf[i0], tf[i0 + 1]) per channel;
+ *  - front to back: w = T * a; C += w * rgb; T *= (1 - a); out = (C, 1 - T).
+ */
+#define VOL_MAX_SAMPLES 65536
+
+static int vol_span(const or_ray *r, float *tnear, float *tfar) {
+  float tn = 0.0f, tf = FLT_MAX;
+  for (int k = 0; k < 3; k++) {
+    float o = r->org[k], d = r->dir[k];
+    if (d == 0.0f) {
+      if (o < 0.0f || o > 1.0f) return 0;
+      continue;
+    }
+    float inv = 1.0f / d;
+    float t0 = (0.0f - o) * inv, t1 = (1.0f - o) * inv;
+    float lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
+    if (lo > tn) tn = lo;
+    if (hi < tf) tf = hi;
+  }
+  *tnear = tn;
+  *tfar = tf;
+  return tn < tf;
+}
+
+static float vol_sample_t(float tnear, int k, float step) {
+  return tnear + ((float)k + 0.5f) * step;
+}
+
+static void tf_index(float s, int N, int *i0, float *f, int *inside) {
+  float x = s * (float)(N - 1);
+  *inside = (x > 0.0f) && (x < (float)(N - 1));
+  if (!(x > 0.0f)) x = 0.0f;
+  if (x > (float)(N - 1)) x = (float)(N - 1);
+  int i = (int)x;
+  if (i > N - 2) i = N - 2;
+  *i0 = i;
+  *f = x - (float)i;
+}
+
+static void vol_geom_at(const or_vol_params *p, const or_ray *r, float t,
+                        tri_geom *g) {
+  float pos[3];
+  for (int k = 0; k < 3; k++) pos[k] = r->org[k] + t * r->dir[k];
+  tex3d_geom(p->dims, p->clamp, pos[0], pos[1], pos[2], g);
+}
+
+void or_volume_fwd_f32(const or_vol_params *p, const float *vox,
+                       const float *tf, const or_ray *rays, int64_t nrays,
+                       float *rgba, uint32_t *nsamples, int threads) {
+  (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+#endif
+  for (int64_t i = 0; i < nrays; i++) {
+    const or_ray *r = rays + i;
+    float C[3] = {0, 0, 0}, T = 1.0f;
+    float tn, tfar;
+    uint32_t ns = 0;
+    if (vol_span(r, &tn, &tfar)) {
+      for (int k = 0; k < VOL_MAX_SAMPLES; k++) {
+        float t = vol_sample_t(tn, k, p->step);
+        if (!(t < tfar)) break;
+        tri_geom g;
+        vol_geom_at(p, r, t, &g);
+        float s = tex3d_eval(vox, &g);
+        int i0, inside;
+        float f;
+        tf_index(s, p->tf_size, &i0, &f, &inside);
+        float c[4];
+        for (int ch = 0; ch < 4; ch++)
+          c[ch] = lerpf(f, tf[4 * i0 + ch], tf[4 * (i0 + 1) + ch]);
+        float w = T * c[3];
+        for (int ch = 0; ch < 3; ch++) C[ch] = C[ch] + w * c[ch];
+        T = T * (1.0f - c[3]);
+        ns++;
+      }
+    }
+    rgba[4 * i + 0] = C[0];
+    rgba[4 * i + 1] = C[1];
+    rgba[4 * i + 2] = C[2];
+    rgba[4 * i + 3] = 1.0f - T;
+    if (nsamples) nsamples[i] = ns;
+  }
+}
+
+/* The f64 checker: identical sample geometry (computed in float as above),
+ * values, TF and compositing in double. */
+static double lerpd(double t, double a, double b) { return (1.0 - t) * a + t * b; }
+
+static double tex3d_eval_d(const double *vox, const tri_geom *g, double w[8]) {
+  double dx = g->dx, dy = g->dy, dz = g->dz;
+  w[0] = (1 - dx) * (1 - dy) * (1 - dz);
+  w[1] = dx * (1 - dy) * (1 - dz);
+  w[2] = (1 - dx) * dy * (1 - dz);
+  w[3] = dx * dy * (1 - dz);
+  w[4] = (1 - dx) * (1 - dy) * dz;
+  w[5] = dx * (1 - dy) * dz;
+  w[6] = (1 - dx) * dy * dz;
+  w[7] = dx * dy * dz;
+  double d00 = lerpd(dx, vox[g->a[0]], vox[g->a[1]]);
+  double d10 = lerpd(dx, vox[g->a[2]], vox[g->a[3]]);
+  double d01 = lerpd(dx, vox[g->a[4]], vox[g->a[5]]);
+  double d11 = lerpd(dx, vox[g->a[6]], vox[g->a[7]]);
+  return lerpd(dz, lerpd(dy, d00, d10), lerpd(dy, d01, d11));
+}
+
+static void tf_index_d(double s, int N, int *i0, double *f, int *inside) {
+  double x = s * (double)(N - 1);
+  *inside = (x > 0.0) && (x < (double)(N - 1));
+  if (!(x > 0.0)) x = 0.0;
+  if (x > (double)(N - 1)) x = (double)(N - 1);
+  int i = (int)x;
+  if (i > N - 2) i = N - 2;
+  *i0 = i;
+  *f = x - (double)i;
+}
+
+void or_volume_fwd_f64(const or_vol_params *p, const double *vox,
+                       const double *tf, const or_ray *rays, int64_t nrays,
+                       double *rgba, int threads) {
+  (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+#endif
+  for (int64_t i = 0; i < nrays; i++) {
+    const or_ray *r = rays + i;
+    double C[3] = {0, 0, 0}, T = 1.0;
+    float tn, tfar;
+    if (vol_span(r, &tn, &tfar)) {
+      for (int k = 0; k < VOL_MAX_SAMPLES; k++) {
+        float t = vol_sample_t(tn, k, p->step);
+        if (!(t < tfar)) break;
+        tri_geom g;
+        vol_geom_at(p, r, t, &g);
+        double w8[8];
+        double s = tex3d_eval_d(vox, &g, w8);
+        int i0, inside;
+        double f;
+        tf_index_d(s, p->tf_size, &i0, &f, &inside);
+        double c[4];
+        for (int ch = 0; ch < 4; ch++)
+          c[ch] = lerpd(f, tf[4 * i0 + ch], tf[4 * (i0 + 1) + ch]);
+        double w = T * c[3];
+        for (int ch = 0; ch < 3; ch++) C[ch] += w * c[ch];
+        T *= (1.0 - c[3]);
+      }
+    }
+    rgba[4 * i + 0] = C[0];
+    rgba[4 * i + 1] = C[1];
+    rgba[4 * i + 2] = C[2];
+    rgba[4 * i + 3] = 1.0 - T;
+  }
+}
+
+/* Adjoint of or_volume_fwd_f64 for L = sum grad_rgba . rgba, by the reverse
+ * recursion R_{k-1} = a_k c_k + (1 - a_k) R_k, P_{k-1} = (1 - a_k) P_k:
+ *   dL/dc_k = gC T_k a_k,  dL/da_k = T_k (gC . (c_k - R_k) + gA P_k).
+ * Stored forward T_k here (it is the checker; the GPU recomputes them). */
+void or_volume_bwd_f64(const or_vol_params *p, const double *vox,
+                       const double *tf, const or_ray *rays, int64_t nrays,
+                       const double *grad_rgba, double *grad_vox,
+                       double *grad_tf) {
+  const int N = p->tf_size;
+  double *Tk = (double *)malloc(sizeof(double) * VOL_MAX_SAMPLES);
+  for (int64_t i = 0; i < nrays; i++) {
+    const or_ray *r = rays + i;
+    float tn, tfar;
+    if (!vol_span(r, &tn, &tfar)) continue;
+    int ns = 0;
+    double T = 1.0;
+    for (int k = 0; k < VOL_MAX_SAMPLES; k++) {
+      float t = vol_sample_t(tn, k, p->step);
+      if (!(t < tfar)) break;
+      tri_geom g;
+      vol_geom_at(p, r, t, &g);
+      double w8[8];
+      double s = tex3d_eval_d(vox, &g, w8);
+      int i0, inside;
+      double f;
+      tf_index_d(s, N, &i0, &f, &inside);
+      double a = lerpd(f, tf[4 * i0 + 3], tf[4 * (i0 + 1) + 3]);
+      Tk[k] = T;
+      T *= (1.0 - a);
+      ns++;
+    }
+    const double *gr = grad_rgba + 4 * i;
+    double R[3] = {0, 0, 0}, P = 1.0;
+    for (int k = ns - 1; k >= 0; k--) {
+      float t = vol_sample_t(tn, k, p->step);
+      tri_geom g;
+      vol_geom_at(p, r, t, &g);
+      double w8[8];
+      double s = tex3d_eval_d(vox, &g, w8);
+      int i0, inside;
+      double f;
+      tf_index_d(s, N, &i0, &f, &inside);
+      double c[4];
+      for (int ch = 0; ch < 4; ch++)
+        c[ch] = lerpd(f, tf[4 * i0 + ch], tf[4 * (i0 + 1) + ch]);
+      double a = c[3];
+      double gc[4];
+      double dot = 0.0;
+      for (int ch = 0; ch < 3; ch++) {
+        gc[ch] = gr[ch] * Tk[k] * a;
+        dot += gr[ch] * (c[ch] - R[ch]);
+      }
+      gc[3] = Tk[k] * (dot + gr[3] * P);
+      for (int ch = 0; ch < 4; ch++) {
+        grad_tf[4 * i0 + ch] += (1.0 - f) * gc[ch];
+        grad_tf[4 * (i0 + 1) + ch] += f * gc[ch];
+      }
+      if (inside) {
+        double ds = 0.0;
+        for (int ch = 0; ch < 4; ch++)
+          ds += gc[ch] * (tf[4 * (i0 + 1) + ch] - tf[4 * i0 + ch]) * (double)(N - 1);
+        for (int c8 = 0; c8 < 8; c8++) grad_vox[g.a[c8]] += ds * w8[c8];
+      }
+      for (int ch = 0; ch < 3; ch++) R[ch] = a * c[ch] + (1.0 - a) * R[ch];
+      P = (1.0 - a) * P;
+    }
+  }
+  free(Tk);
+}

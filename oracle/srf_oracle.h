@@ -1,0 +1,157 @@
+/*
+ * srf_oracle.h -- CPU restatement of SURFACE's LBVH build, closest-hit
+ * traversal and 3D-texture sampling, used ONLY as the parity checker.
+ *
+ * TEST INFRASTRUCTURE. Nothing in the product (surface_amd/, include/) may
+ * include, link or call this code; only tests/, the smoke check and the
+ * bench's cpu_baseline leg do.
+ *
+ * Layouts below are byte-identical to the reference's node structs so that a
+ * tree emitted here can be compared field by field with one written by the
+ * reference's own Dump():
+ *   or_tri_node  == TriangleNode  render/render_accel_triangle.h:23-42 (64 B)
+ *   or_tet_node  == TetraNode     render/render_accel_tetra.h:24-43    (40 B)
+ *   or_sph_node  == ParticleNode  render/render_accel_particle.h:24-44 (60 B)
+ *   or_nodeinfo  == NodeInfo32    render/render_prefix_tree_util.h:51-61
+ */
+#ifndef SRF_ORACLE_H_
+#define SRF_ORACLE_H_
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  float bmin[2][3];
+  float bmax[2][3];
+  int32_t flag; /* 1 = leaf */
+  int32_t axis;
+  uint32_t data[2]; /* leaf: {count, first}; branch: {left, right} */
+} or_tri_node;
+
+typedef struct {
+  float bmin[3];
+  float bmax[3];
+  int32_t flag;
+  int32_t axis;
+  uint32_t data[2];
+} or_tet_node;
+
+typedef struct {
+  float bmin[2][3];
+  float bmax[2][3];
+  uint32_t data[2];
+  int16_t flag;
+  int16_t axis;
+} or_sph_node;
+
+typedef struct {
+  uint32_t child;
+  int32_t left_type;  /* 1 = leaf */
+  int32_t right_type; /* 1 = leaf */
+} or_nodeinfo;
+
+/* One ray as handed to the tracer: origin, direction, double-sided flag and
+ * the primitive to skip (prev_prim_id, ~0 = none). For spheres the
+ * self-intersection rule also needs prev_hit_t / prev_hit_normal. */
+typedef struct {
+  float org[3];
+  uint32_t flags; /* bit0: double sided */
+  float dir[3];
+  uint32_t prev_prim_id;
+} or_ray;
+
+typedef struct {
+  float t, u, v;
+  uint32_t prim_id;
+} or_hit;
+
+/* Tetra extras: InterpolateTetra weights d0..d3 at the entry point, the
+ * interpolated per-vertex scalar, the entry face and the exit distance. */
+typedef struct {
+  float d[4];
+  float scalar;
+  uint32_t subface;
+  float t_leave;
+  float pad;
+} or_tet_ext;
+
+/* ---- LBVH stages ---- */
+void or_scene_bounds_tri(const float *V, const uint32_t *F, int64_t n,
+                         float bmin[3], float bmax[3]);
+void or_scene_bounds_tet(const float *V, const uint32_t *F, int64_t n,
+                         float bmin[3], float bmax[3]);
+void or_scene_bounds_sph(const float *P, const float *R, float rconst,
+                         int64_t n, float bmin[3], float bmax[3]);
+void or_morton_tri(const float *V, const uint32_t *F, int64_t n,
+                   const float bmin[3], const float bmax[3], uint32_t *codes);
+void or_morton_tet(const float *V, const uint32_t *F, int64_t n,
+                   const float bmin[3], const float bmax[3], uint32_t *codes);
+void or_morton_pts(const float *P, int64_t n, const float bmin[3],
+                   const float bmax[3], uint32_t *codes);
+uint32_t or_morton30_point(const float p[3], const float bmin[3],
+                           float invx, float invy, float invz);
+/* Stable sort of (code, index=0..n-1) by code. */
+void or_sort_keys(const uint32_t *codes, int64_t n, uint32_t *sorted_codes,
+                  uint32_t *sorted_index);
+void or_radix_tree30(const uint32_t *sorted_codes, int64_t n,
+                     or_nodeinfo *out /* n-1 */);
+
+/* ---- Full Build32 sequences. Return the node count; nodes/indices must be
+ * sized 2n and n. bounds[6] = root bmin/bmax as stored by BoundingBox().
+ * eps_scale (triangles only) receives the scene-size epsilon scale. ---- */
+int64_t or_build_tri(const float *V, const uint32_t *F, int64_t n,
+                     or_tri_node *nodes, uint32_t *indices, double bounds[6],
+                     double *eps_scale);
+int64_t or_build_tet(const float *V, const uint32_t *F, int64_t n,
+                     or_tet_node *nodes, uint32_t *indices, double bounds[6]);
+int64_t or_build_sph(const float *P, const float *R, float rconst, int64_t n,
+                     or_sph_node *nodes, uint32_t *indices, double bounds[6]);
+
+/* ---- Closest-hit traversal (one ray after another, or OpenMP over rays
+ * when threads > 1). stats (optional) receives, per ray, {nodes visited,
+ * primitives tested}. ---- */
+void or_trace_tri(const or_tri_node *nodes, const uint32_t *indices,
+                  const float *V, const uint32_t *F, double eps_scale,
+                  const or_ray *rays, int64_t nrays, or_hit *hits,
+                  uint32_t *stats, int threads);
+void or_trace_tet(const or_tet_node *nodes, const uint32_t *indices,
+                  const float *V, const uint32_t *F, const float *scalars,
+                  const or_ray *rays, int64_t nrays, or_hit *hits,
+                  or_tet_ext *ext, uint32_t *stats, int threads);
+void or_trace_sph(const or_sph_node *nodes, const uint32_t *indices,
+                  const float *P, const float *R, float rconst,
+                  const or_ray *rays, const double *prev_hit_t,
+                  const float *prev_hit_n, int64_t nrays, or_hit *hits,
+                  uint32_t *stats, int threads);
+
+/* ---- 3D texture (FilterTexture3DFloat), single component. ---- */
+void or_tex3d_sample(const float *vox, const int dims[3], const int clamp[3],
+                     const float *uvw, int64_t n, float *out);
+
+/* ---- Emission-absorption volume march (builder-defined; FP32 forward as
+ * the GPU computes it, FP64 forward and adjoint as the gradient checker). */
+typedef struct {
+  int dims[3];
+  int clamp[3];
+  int tf_size;
+  float step; /* in texture units */
+} or_vol_params;
+void or_volume_fwd_f32(const or_vol_params *p, const float *vox,
+                       const float *tf, const or_ray *rays, int64_t nrays,
+                       float *rgba, uint32_t *nsamples, int threads);
+void or_volume_fwd_f64(const or_vol_params *p, const double *vox,
+                       const double *tf, const or_ray *rays, int64_t nrays,
+                       double *rgba, int threads);
+void or_volume_bwd_f64(const or_vol_params *p, const double *vox,
+                       const double *tf, const or_ray *rays, int64_t nrays,
+                       const double *grad_rgba, double *grad_vox,
+                       double *grad_tf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,211 @@
+"""Deterministic synthetic workloads for the acceleration-structure path.
+
+Every generator is seeded and uses numpy's PCG64 stream, so the same arrays
+are produced on any host. Geometry is returned as C-contiguous float32
+vertices (V, 3) and uint32 index arrays, the layouts `Mesh`, `Tetrahedron`
+and `Particles` take (render/render_prim_triangle.h:22-56,
+render/render_prim_tetra.h:30-37, render/render_prim_particle.h:19-27).
+
+Workloads (sizes as named):
+  heightfield(512)          512x512 grid, z = sum of 8 seeded sines + N(0, 0.01),
+                            2 triangles per quad -> 522,242 triangles
+  tet_grid(64)              64^3 hexes jittered U(-0.2, 0.2), 6 tets per hex
+                            -> 1,572,864 tets; per-vertex scalar = 16 Gaussians
+  plummer_spheres(n)        Plummer-sphere centres, lognormal(-6, 0.5) radii
+  gaussian_volume(256)      256^3 float32, sum of 64 anisotropic Gaussians in [0,1]
+  transfer_function(256)    256-entry RGBA table, smooth ramps, alpha <= 0.05
+  pinhole_rays(...)         one primary ray per pixel centre, 8x8-tile order
+"""
+from __future__ import annotations
+
+import numpy as np
+
+BASE_SEED = 20261202
+
+RAY_DTYPE = np.dtype([("org", "<f4", (3,)), ("flags", "<u4"),
+                      ("dir", "<f4", (3,)), ("prev_prim_id", "<u4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"),
+                      ("prim_id", "<u4")])
+TET_EXT_DTYPE = np.dtype([("d", "<f4", (4,)), ("scalar", "<f4"),
+                          ("subface", "<u4"), ("t_leave", "<f4"),
+                          ("pad", "<f4")])
+NO_PRIM = np.uint32(0xFFFFFFFF)
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def heightfield(grid=512, seed=BASE_SEED + 1, jitter=0.01):
+    """Grid heightfield over [0,1]^2: V = grid^2, N = 2 (grid-1)^2 triangles."""
+    rng = _rng(seed)
+    amp = rng.uniform(0.02, 0.06, 8)
+    freq = rng.uniform(0.5, 6.0, (8, 2))
+    phase = rng.uniform(0.0, 2.0 * np.pi, 8)
+    xs = np.linspace(0.0, 1.0, grid)
+    x, y = np.meshgrid(xs, xs, indexing="xy")
+    z = np.zeros_like(x)
+    for k in range(8):
+        z += amp[k] * np.sin(2.0 * np.pi * (freq[k, 0] * x + freq[k, 1] * y) + phase[k])
+    z += rng.normal(0.0, jitter, z.shape)
+    V = np.stack([x, y, z], axis=-1).reshape(-1, 3).astype(np.float32)
+    i = np.arange(grid - 1)
+    jj, ii = np.meshgrid(i, i, indexing="ij")
+    v00 = (jj * grid + ii).ravel()
+    v10 = v00 + 1
+    v01 = v00 + grid
+    v11 = v01 + 1
+    t0 = np.stack([v00, v10, v11], axis=-1)
+    t1 = np.stack([v00, v11, v01], axis=-1)
+    F = np.stack([t0, t1], axis=1).reshape(-1, 3).astype(np.uint32)
+    return np.ascontiguousarray(V), np.ascontiguousarray(F)
+
+
+# Kuhn split of the unit cube into 6 tets along the 0-7 diagonal; corner c
+# has offset (c & 1, (c >> 1) & 1, (c >> 2) & 1).
+_KUHN = np.array([[0, 1, 3, 7], [0, 3, 2, 7], [0, 2, 6, 7],
+                  [0, 6, 4, 7], [0, 4, 5, 7], [0, 5, 1, 7]], dtype=np.int64)
+
+
+def tet_grid(cells=64, seed=BASE_SEED + 2, jitter=0.2, n_gauss=16):
+    """Jittered (cells+1)^3 lattice on [0, cells]^3, 6 tets per hex.
+
+    Returns V (float32), F (uint32, 4 per tet, positively oriented) and a
+    per-vertex float32 scalar (sum of `n_gauss` anisotropic Gaussians).
+    """
+    rng = _rng(seed)
+    n1 = cells + 1
+    g = np.arange(n1, dtype=np.float64)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([x, y, z], axis=-1).reshape(-1, 3)
+    P = P + rng.uniform(-jitter, jitter, P.shape)
+    c = np.arange(cells)
+    cz, cy, cx = np.meshgrid(c, c, c, indexing="ij")
+    base = (cz * n1 * n1 + cy * n1 + cx).ravel()
+    corner = np.array([(k & 1) + ((k >> 1) & 1) * n1 + ((k >> 2) & 1) * n1 * n1
+                       for k in range(8)], dtype=np.int64)
+    F = (base[:, None, None] + corner[_KUHN][None, :, :]).reshape(-1, 4)
+    a, b, cc, d = (P[F[:, k]] for k in range(4))
+    vol = np.einsum("ij,ij->i", np.cross(b - a, cc - a), d - a)
+    neg = vol < 0
+    F[neg, 1], F[neg, 2] = F[neg, 2].copy(), F[neg, 1].copy()
+    centres = rng.uniform(0.0, cells, (n_gauss, 3))
+    sig = rng.uniform(0.05, 0.25, (n_gauss, 3)) * cells
+    w = rng.uniform(0.2, 1.0, n_gauss)
+    s = np.zeros(len(P))
+    for k in range(n_gauss):
+        q = (P - centres[k]) / sig[k]
+        s += w[k] * np.exp(-0.5 * np.einsum("ij,ij->i", q, q))
+    return (np.ascontiguousarray(P.astype(np.float32)),
+            np.ascontiguousarray(F.astype(np.uint32)),
+            np.ascontiguousarray(s.astype(np.float32)))
+
+
+def plummer_spheres(n, seed=BASE_SEED + 5, a=1.0, mu=-6.0, sigma=0.5):
+    """Plummer-sphere centres (scale a) and lognormal(mu, sigma) radii."""
+    rng = _rng(seed)
+    m = rng.uniform(1e-6, 0.999, n)
+    r = a / np.sqrt(m ** (-2.0 / 3.0) - 1.0)
+    r = np.minimum(r, 50.0 * a)
+    cth = rng.uniform(-1.0, 1.0, n)
+    phi = rng.uniform(0.0, 2.0 * np.pi, n)
+    sth = np.sqrt(1.0 - cth * cth)
+    P = np.stack([r * sth * np.cos(phi), r * sth * np.sin(phi), r * cth], axis=-1)
+    R = rng.lognormal(mu, sigma, n)
+    return (np.ascontiguousarray(P.astype(np.float32)),
+            np.ascontiguousarray(R.astype(np.float32)))
+
+
+def gaussian_volume(dim=256, seed=BASE_SEED + 4, n_gauss=64):
+    """dim^3 float32 volume (x fastest), sum of anisotropic Gaussians in [0,1]."""
+    rng = _rng(seed)
+    cen = rng.uniform(0.15, 0.85, (n_gauss, 3))
+    sig = rng.uniform(0.03, 0.15, (n_gauss, 3))
+    w = rng.uniform(0.3, 1.0, n_gauss)
+    ax = np.linspace(0.0, 1.0, dim, dtype=np.float32)
+    vol = np.zeros((dim, dim, dim), dtype=np.float32)
+    for k in range(n_gauss):
+        gx = np.exp(-0.5 * ((ax - cen[k, 0]) / sig[k, 0]) ** 2).astype(np.float32)
+        gy = np.exp(-0.5 * ((ax - cen[k, 1]) / sig[k, 1]) ** 2).astype(np.float32)
+        gz = np.exp(-0.5 * ((ax - cen[k, 2]) / sig[k, 2]) ** 2).astype(np.float32)
+        vol += np.float32(w[k]) * (gz[:, None, None] * gy[None, :, None] * gx[None, None, :])
+    vol /= vol.max()
+    return np.ascontiguousarray(vol.reshape(-1))
+
+
+def transfer_function(size=256, seed=BASE_SEED + 6, max_alpha=0.05):
+    """size x RGBA float32 table: smooth colour ramps, alpha in [0, max_alpha]."""
+    rng = _rng(seed)
+    x = np.linspace(0.0, 1.0, size)
+    tf = np.zeros((size, 4))
+    for ch in range(3):
+        ph = rng.uniform(0, 2 * np.pi)
+        tf[:, ch] = 0.5 + 0.5 * np.sin(2 * np.pi * rng.uniform(0.5, 2.0) * x + ph)
+    tf[:, 3] = max_alpha * np.clip(x - 0.1, 0, None) ** 1.5 / 0.9 ** 1.5
+    return np.ascontiguousarray(tf.astype(np.float32))
+
+
+def tile_order(width, height, tile=8):
+    """Pixel index (y * width + x) of each ray when rays are laid out in
+    tile x tile blocks (a 64-ray wavefront covers one 8x8 block)."""
+    ty, tx = np.meshgrid(np.arange(0, height, tile), np.arange(0, width, tile), indexing="ij")
+    oy, ox = np.meshgrid(np.arange(tile), np.arange(tile), indexing="ij")
+    py = (ty.ravel()[:, None] + oy.ravel()[None, :]).ravel()
+    px = (tx.ravel()[:, None] + ox.ravel()[None, :]).ravel()
+    keep = (py < height) & (px < width)
+    return (py[keep] * width + px[keep]).astype(np.int64)
+
+
+def pinhole_rays(eye, target, up, fov_deg, width, height, tiled=True,
+                 double_sided=True):
+    """One ray per pixel centre of a pinhole camera.
+
+    Directions are formed in float64, normalised and rounded to float32. With
+    tiled=True the rays come in 8x8-tile order (see tile_order).
+    """
+    eye = np.asarray(eye, np.float64)
+    fwd = np.asarray(target, np.float64) - eye
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, np.asarray(up, np.float64))
+    right /= np.linalg.norm(right)
+    upv = np.cross(right, fwd)
+    h = np.tan(np.radians(fov_deg) * 0.5)
+    px = (np.arange(width) + 0.5) / width * 2.0 - 1.0
+    py = 1.0 - (np.arange(height) + 0.5) / height * 2.0
+    X, Y = np.meshgrid(px * h * width / height, py * h, indexing="xy")
+    d = fwd[None, None, :] + X[..., None] * right + Y[..., None] * upv
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    d = d.reshape(-1, 3)
+    if tiled:
+        d = d[tile_order(width, height)]
+    rays = np.zeros(len(d), RAY_DTYPE)
+    rays["org"] = eye.astype(np.float32)
+    rays["dir"] = d.astype(np.float32)
+    rays["flags"] = 1 if double_sided else 0
+    rays["prev_prim_id"] = NO_PRIM
+    return rays
+
+
+def heightfield_rays(width=1024, height=1024, tiled=True):
+    """Camera framing the [0,1]^2 heightfield from above at an angle."""
+    return pinhole_rays((0.5, -0.45, 0.95), (0.5, 0.45, 0.0), (0, 0, 1), 50.0,
+                        width, height, tiled)
+
+
+def tet_rays(cells=64, width=1024, height=1024, tiled=True):
+    """Camera looking at the [0, cells]^3 tet grid across a corner."""
+    c = float(cells)
+    return pinhole_rays((1.9 * c, -0.7 * c, 1.6 * c), (0.5 * c, 0.5 * c, 0.5 * c),
+                        (0, 0, 1), 40.0, width, height, tiled)
+
+
+def sphere_rays(width=2048, height=2048, tiled=True):
+    """Camera framing the core of a unit Plummer sphere."""
+    return pinhole_rays((0.0, -6.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 1), 30.0,
+                        width, height, tiled)
+
+
+def volume_rays(width=1024, height=1024, tiled=True):
+    """Camera framing the unit-cube volume (texture space)."""
+    return pinhole_rays((1.8, -1.2, 1.5), (0.5, 0.5, 0.5), (0, 0, 1), 35.0,
+                        width, height, tiled)
