@@ -1,0 +1,278 @@
+/*
+ * srf.h -- C ABI of the MI355X-native acceleration-structure build and ray
+ * intersection library (libsrf.so).
+ *
+ * It replaces SURFACE's CPU accelerators behind gles_accel_builder and
+ * gles_raytrace_engine. Every entry point below names the reference
+ * interface it stands in for (paths relative to the SURFACE checkout).
+ *
+ * Conventions
+ *  - Array arguments are DEVICE pointers (hipMalloc'd or any pointer a HIP
+ *    kernel on the context's device may dereference) unless the parameter
+ *    name ends in `_host`.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream). All
+ *    device work of a call is enqueued on it; calls return once the work is
+ *    enqueued, except where a comment says the call synchronises.
+ *  - A context and everything built from it belong to one device. Calls on
+ *    one context must be serialised by the caller (the build scratch space
+ *    is shared); accelerators may be traced from any number of streams.
+ *  - Errors: every call returns srf_status; srf_last_error_string() gives a
+ *    thread-local message for the last failing call. The reference reports
+ *    failure as `bool` (render/render_accel_triangle.h:78) and asserts.
+ *  - Floating point follows the reference bit for bit: no FMA contraction,
+ *    IEEE division and square root, the reference's float/double mixing.
+ */
+#ifndef SRF_H_
+#define SRF_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SRF_ABI_VERSION 1
+
+typedef enum srf_status {
+  SRF_OK = 0,
+  SRF_ERR_INVALID_ARG = 1,
+  SRF_ERR_OUT_OF_MEMORY = 2,
+  SRF_ERR_HIP = 3,
+  /* fewer than 1024 primitives: the reference's Build32 switches to its
+   * SAH Build for those (render/render_accel_triangle.cc:1394-1397), which
+   * this library does not provide */
+  SRF_ERR_UNSUPPORTED = 4,
+  SRF_ERR_INTERNAL = 5
+} srf_status;
+
+/* Primitive kinds, numbered as the GL modes that select them in
+ * glDrawElements (gles/gles_primitive.cc:84-368; GL_TETRAHEDRONS_EXT is
+ * include/GLES2/gl2.h:63). */
+typedef enum srf_prim_kind {
+  SRF_PRIM_POINTS = 0x0000,       /* GL_POINTS -> ParticleAccel  */
+  SRF_PRIM_TRIANGLES = 0x0004,    /* GL_TRIANGLES -> TriangleAccel */
+  SRF_PRIM_TETRAHEDRONS = 0x0010  /* GL_TETRAHEDRONS_EXT -> TetraAccel */
+} srf_prim_kind;
+
+typedef struct srf_ctx srf_ctx;
+typedef struct srf_accel srf_accel;
+typedef struct srf_volume srf_volume;
+
+/* Ray: the fields of render/render_ray.h:47-151 that traversal reads.
+ * flags bit 0 = double_sided (Ray::double_sided; eye rays use 1,
+ * gles/gles_raytrace_engine.cc:807). prev_prim_id = Ray::prev_prim_id
+ * (0xFFFFFFFF = none). 32 bytes. */
+typedef struct srf_ray {
+  float org[3];
+  uint32_t flags;
+  float dir[3];
+  uint32_t prev_prim_id;
+} srf_ray;
+#define SRF_RAY_DOUBLE_SIDED 1u
+#define SRF_NO_PRIM 0xFFFFFFFFu
+
+/* Closest hit: Intersection::{t,u,v,prim_id} (render/render_intersection.h:18-62).
+ * A miss leaves t = FLT_MAX and prim_id = SRF_NO_PRIM, as Traverse does. */
+typedef struct srf_hit {
+  float t, u, v;
+  uint32_t prim_id;
+} srf_hit;
+
+/* Tetra extras at the entry point: InterpolateTetra weights d0..d3
+ * (render/render_accel_tetra.cc:148-241), the varying f0*d0+f1*d1+f2*d2+f3*d3
+ * (gles/gles_common.cc:1074-1076) when vertex scalars are given,
+ * Intersection::subface_id and the exit distance of the hit tetra. */
+typedef struct srf_tet_hit_ext {
+  float d[4];
+  float scalar;
+  uint32_t subface;
+  float t_leave;
+  float pad;
+} srf_tet_hit_ext;
+
+/* Sphere self-intersection inputs, Ray::prev_hit_t / prev_hit_normal
+ * (render/render_ray.h:137-139), used when prev_prim_id names a sphere. */
+typedef struct srf_sphere_prev {
+  double prev_hit_t;
+  float prev_hit_normal[3];
+  float pad;
+} srf_sphere_prev;
+
+/* Geometry descriptors: the device-side counterparts of Mesh
+ * (render/render_prim_triangle.h:22-56), Tetrahedron
+ * (render/render_prim_tetra.h:30-37) and Particles
+ * (render/render_prim_particle.h:19-27). FP32 positions only. */
+typedef struct srf_tri_desc {
+  const float *vertices; /* xyz per vertex */
+  uint64_t num_vertices;
+  const uint32_t *faces; /* 3 per triangle */
+  uint64_t num_faces;
+} srf_tri_desc;
+
+typedef struct srf_tet_desc {
+  const float *vertices;
+  uint64_t num_vertices;
+  const uint32_t *indices; /* 4 per tetrahedron */
+  uint64_t num_tetras;
+} srf_tet_desc;
+
+typedef struct srf_sphere_desc {
+  const float *positions;
+  const float *radius; /* per particle, or NULL to use constant_radius */
+  float constant_radius;
+  uint64_t num_particles;
+} srf_sphere_desc;
+
+/* Built-tree summary; srf_accel_get_info synchronises the build stream. */
+typedef struct srf_accel_info {
+  uint32_t kind; /* srf_prim_kind */
+  uint64_t num_prims;
+  uint64_t num_nodes;
+  uint64_t num_leaves;
+  uint32_t max_depth;   /* branch levels, root = 1 */
+  double eps_scale;     /* triangles: scene extent (render/render_accel_triangle.cc:1451-1463) */
+  uint64_t node_bytes;  /* bytes per node in srf_accel_export's reference layout */
+} srf_accel_info;
+
+/* NodeInfo32 (render/render_prefix_tree_util.h:51-61): left child =
+ * child_index, right = child_index + 1; *_leaf = NODE_TYPE_LEAF. */
+typedef struct srf_node_info {
+  uint32_t child_index;
+  uint32_t left_leaf;
+  uint32_t right_leaf;
+} srf_node_info;
+
+/* ---------------------------------------------------------------- context */
+const char *srf_last_error_string(void);
+int srf_abi_version(void);
+srf_status srf_ctx_create(int device, srf_ctx **out);
+srf_status srf_ctx_destroy(srf_ctx *ctx);
+
+/* ------------------------------------------------------------------ build */
+/* TriangleAccel::Build32(const Mesh*, const TriangleBuildOptions&)
+ * (render/render_accel_triangle.h:78, .cc:1389-1690), called from
+ * AccelBuilder::AddTriangleData (gles/gles_accel_builder.cc:572). The
+ * accelerator owns device copies of everything traversal needs; the
+ * caller's buffers may be freed once the stream has passed the build. */
+srf_status srf_accel_build_triangles(srf_ctx *ctx, const srf_tri_desc *desc,
+                                     void *stream, srf_accel **out);
+/* TetraAccel::Build32 (render/render_accel_tetra.h:91, .cc:1579-1898),
+ * from AccelBuilder::AddTetraData (gles/gles_accel_builder.cc:867). */
+srf_status srf_accel_build_tetras(srf_ctx *ctx, const srf_tet_desc *desc,
+                                  void *stream, srf_accel **out);
+/* ParticleAccel::Build32 (render/render_accel_particle.h:77,
+ * .cc:1560-1866), from AccelBuilder::AddParticleData
+ * (gles/gles_accel_builder.cc:673). */
+srf_status srf_accel_build_spheres(srf_ctx *ctx, const srf_sphere_desc *desc,
+                                   void *stream, srf_accel **out);
+/* Many meshes in one segmented build: each mesh gets the tree
+ * TriangleAccel::Build32 would give it alone. The reference builds draw
+ * calls one by one (gles/gles_accel_builder.cc:34-117). out[i] receives
+ * mesh i's accelerator. */
+srf_status srf_accel_build_triangles_batch(srf_ctx *ctx, uint32_t num_meshes,
+                                           const srf_tri_desc *descs_host,
+                                           void *stream, srf_accel **out);
+
+/* Build32 called again on an existing accelerator object (the reference
+ * rebuilds in place: nodes_.clear(), render/render_accel_triangle.cc:1552;
+ * AccelBuilder rebuilds dynamic draws every frame,
+ * gles/gles_accel_builder.cc:442-455). desc must point to the descriptor
+ * type of the accelerator's kind; device buffers are reused when the new
+ * geometry fits, so no allocation or synchronisation happens then. Work
+ * that still reads the old tree on another stream must be complete. */
+srf_status srf_accel_rebuild(srf_ctx *ctx, srf_accel *accel, const void *desc,
+                             void *stream);
+
+/* TriangleAccel/TetraAccel/ParticleAccel::BoundingBox(double[3], double[3])
+ * (render/render_accel_triangle.h:90), used for the top-level BVH
+ * (gles/gles_primitive.cc:174,215,271). Synchronises. */
+srf_status srf_accel_bounds(const srf_accel *accel, double bmin_host[3],
+                            double bmax_host[3]);
+srf_status srf_accel_get_info(const srf_accel *accel, srf_accel_info *info_host);
+/* Tree in the reference's node layout and its permuted primitive indices:
+ * the contents TriangleAccel::Dump writes (render/render_accel_triangle.cc:
+ * 1692-1719), nodes in depth-first preorder. nodes_host must hold
+ * info.num_nodes * info.node_bytes bytes, indices_host num_prims words.
+ * Synchronises. */
+srf_status srf_accel_export(const srf_accel *accel, void *nodes_host,
+                            uint32_t *indices_host);
+srf_status srf_accel_destroy(srf_accel *accel);
+
+/* ------------------------------------------------------------------ trace */
+/* {Triangle,Tetra,Particle}Accel::Traverse(Intersection&, Ray&) for a batch
+ * of rays (render/render_accel_triangle.h:87), the loop the reference runs
+ * per pixel under OpenMP (gles/gles_raytrace_engine.cc:662-880). */
+srf_status srf_trace_closest(const srf_accel *accel, const srf_ray *rays,
+                             uint64_t num_rays, srf_hit *hits, void *stream);
+/* TetraAccel::Traverse plus BuildIntersection / InterpolateTetra
+ * (render/render_accel_tetra.cc:2124-2289); vertex_scalars (one float per
+ * vertex of the build's vertex array, may be NULL) gives ext->scalar. */
+srf_status srf_trace_tetra(const srf_accel *accel, const srf_ray *rays,
+                           uint64_t num_rays, const float *vertex_scalars,
+                           srf_hit *hits, srf_tet_hit_ext *ext, void *stream);
+/* ParticleAccel::Traverse with the self-intersection inputs of each ray
+ * (render/render_accel_particle.cc:2423-2512); prev may be NULL. */
+srf_status srf_trace_spheres(const srf_accel *accel, const srf_ray *rays,
+                             const srf_sphere_prev *prev, uint64_t num_rays,
+                             srf_hit *hits, void *stream);
+/* Occlusion: occluded[i] = 1 iff Traverse would report a hit with
+ * t < t_max[i] (t_max NULL = FLT_MAX). The reference answers shadow() with
+ * a full closest-hit Traverse (gles/gles_glsl_runtime.cc:179-217); this
+ * stops at the first qualifying primitive instead. */
+srf_status srf_trace_any(const srf_accel *accel, const srf_ray *rays,
+                         const float *t_max, uint64_t num_rays,
+                         uint32_t *occluded, void *stream);
+
+/* ------------------------------------------------------- LBVH stages */
+/* CalculateMortonCodes{TriangleFloat,TetraFloat}30 / CalculateMortonCodes30SIMD
+ * (render/render_prefix_tree_util.cc:176-304) for the scene box bmin/bmax
+ * (host arrays). desc points to the srf_*_desc matching kind. */
+srf_status srf_lbvh_morton30(srf_ctx *ctx, srf_prim_kind kind, const void *desc,
+                             const float bmin_host[3], const float bmax_host[3],
+                             uint32_t *codes, void *stream);
+/* RadixSort30OMP (render/render_prefix_tree_util.h:296-402): stable sort of
+ * {index = 0..n-1, code} by code. */
+srf_status srf_lbvh_sort30(srf_ctx *ctx, const uint32_t *codes, uint64_t n,
+                           uint32_t *sorted_codes, uint32_t *sorted_index,
+                           void *stream);
+/* ConstructBinaryRadixTree30 for i in [0, n-2]
+ * (render/render_prefix_tree_util.cc:562-649). */
+srf_status srf_lbvh_radix_tree30(srf_ctx *ctx, const uint32_t *sorted_codes,
+                                 uint64_t n, srf_node_info *out, void *stream);
+
+/* ----------------------------------------------------------------- volume */
+/* glTexImage3D / lsglTexImage3DPointer for a single-component FP32 volume
+ * (gles/gles_texture.cc:198-357); wrap_* = 0 REPEAT (the glTexImage3D
+ * default, gles/gles_common.cc:1274-1276) or 1 CLAMP_TO_EDGE
+ * (render/render_texture.h:23-24). voxels (x fastest) are referenced, not
+ * copied, like lsglTexImage3DPointer's retained buffer. */
+srf_status srf_volume_create(srf_ctx *ctx, const float *voxels, int width,
+                             int height, int depth, int wrap_s, int wrap_t,
+                             int wrap_r, srf_volume **out);
+srf_status srf_volume_destroy(srf_volume *vol);
+/* texture3D -> FilterTexture3DFloat (render/render_texture.cc:946-1058) at
+ * n texture coordinates uvw (3 floats each); out[i] = sampled value. */
+srf_status srf_volume_sample(const srf_volume *vol, const float *uvw,
+                             uint64_t n, float *out, void *stream);
+/* Emission-absorption march through the unit cube, step in texture units,
+ * transfer function tf (tf_size RGBA entries); rgba[4*i..] = (C, 1 - T).
+ * The reference marches in user GLSL (examples/sparse_volume_render/
+ * input.frag:99-141); the march itself is defined by this library. */
+srf_status srf_volume_march_fwd(const srf_volume *vol, const float *tf,
+                                int tf_size, float step, const srf_ray *rays,
+                                uint64_t num_rays, float *rgba, void *stream);
+/* Adjoint of srf_volume_march_fwd for L = sum(grad_rgba * rgba): ACCUMULATES
+ * dL/dvoxel into grad_voxels (width*height*depth floats) and dL/dtf into
+ * grad_tf (4*tf_size floats). Replays each ray in reverse with segment
+ * checkpoints of the transmittance; stores no per-sample state. */
+srf_status srf_volume_march_bwd(const srf_volume *vol, const float *tf,
+                                int tf_size, float step, const srf_ray *rays,
+                                uint64_t num_rays, const float *grad_rgba,
+                                float *grad_voxels, float *grad_tf,
+                                void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SRF_H_ */

@@ -1,0 +1,108 @@
+"""ctypes declarations of include/srf.h (libsrf.so).
+
+torch is imported first when it is installed: torch ships its own
+libamdhip64.so.7, and loading it before libsrf.so makes both use that one
+HIP runtime (the SONAMEs are equal, so the dynamic linker reuses it).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+try:  # noqa: SIM105 - see module docstring
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsrf.so")
+
+SRF_OK = 0
+STATUS_NAMES = {0: "SRF_OK", 1: "SRF_ERR_INVALID_ARG", 2: "SRF_ERR_OUT_OF_MEMORY",
+                3: "SRF_ERR_HIP", 4: "SRF_ERR_UNSUPPORTED", 5: "SRF_ERR_INTERNAL"}
+PRIM_POINTS, PRIM_TRIANGLES, PRIM_TETRAHEDRONS = 0x0000, 0x0004, 0x0010
+
+
+class SrfError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class TriDesc(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("num_vertices", C.c_uint64),
+                ("faces", C.c_void_p), ("num_faces", C.c_uint64)]
+
+
+class TetDesc(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("num_vertices", C.c_uint64),
+                ("indices", C.c_void_p), ("num_tetras", C.c_uint64)]
+
+
+class SphereDesc(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("radius", C.c_void_p),
+                ("constant_radius", C.c_float), ("num_particles", C.c_uint64)]
+
+
+class AccelInfo(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("num_prims", C.c_uint64), ("num_nodes", C.c_uint64),
+                ("num_leaves", C.c_uint64), ("max_depth", C.c_uint32),
+                ("eps_scale", C.c_double), ("node_bytes", C.c_uint64)]
+
+
+_vp, _u64, _i = C.c_void_p, C.c_uint64, C.c_int
+SIGNATURES = {
+    "srf_last_error_string": (C.c_char_p, []),
+    "srf_abi_version": (C.c_int, []),
+    "srf_ctx_create": (_i, [_i, C.POINTER(_vp)]),
+    "srf_ctx_destroy": (_i, [_vp]),
+    "srf_accel_build_triangles": (_i, [_vp, C.POINTER(TriDesc), _vp, C.POINTER(_vp)]),
+    "srf_accel_build_tetras": (_i, [_vp, C.POINTER(TetDesc), _vp, C.POINTER(_vp)]),
+    "srf_accel_build_spheres": (_i, [_vp, C.POINTER(SphereDesc), _vp, C.POINTER(_vp)]),
+    "srf_accel_build_triangles_batch": (_i, [_vp, C.c_uint32, _vp, _vp, _vp]),
+    "srf_accel_rebuild": (_i, [_vp, _vp, _vp, _vp]),
+    "srf_accel_bounds": (_i, [_vp, _vp, _vp]),
+    "srf_accel_get_info": (_i, [_vp, C.POINTER(AccelInfo)]),
+    "srf_accel_export": (_i, [_vp, _vp, _vp]),
+    "srf_accel_destroy": (_i, [_vp]),
+    "srf_trace_closest": (_i, [_vp, _vp, _u64, _vp, _vp]),
+    "srf_trace_tetra": (_i, [_vp, _vp, _u64, _vp, _vp, _vp, _vp]),
+    "srf_trace_spheres": (_i, [_vp, _vp, _vp, _u64, _vp, _vp]),
+    "srf_trace_any": (_i, [_vp, _vp, _vp, _u64, _vp, _vp]),
+    "srf_lbvh_morton30": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "srf_lbvh_sort30": (_i, [_vp, _vp, _u64, _vp, _vp, _vp]),
+    "srf_lbvh_radix_tree30": (_i, [_vp, _vp, _u64, _vp, _vp]),
+    "srf_volume_create": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, C.POINTER(_vp)]),
+    "srf_volume_destroy": (_i, [_vp]),
+    "srf_volume_sample": (_i, [_vp, _vp, _u64, _vp, _vp]),
+    "srf_volume_march_fwd": (_i, [_vp, _vp, _i, C.c_float, _vp, _u64, _vp, _vp]),
+    "srf_volume_march_bwd": (_i, [_vp, _vp, _i, C.c_float, _vp, _u64, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def load(path=None):
+    """Load libsrf.so (raises if it has not been built: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing; build it with `python -m surface_amd.build` "
+            "(the HIP path has no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != SRF_OK:
+        msg = load().srf_last_error_string()
+        raise SrfError(status, msg.decode() if msg else "")
+    return status

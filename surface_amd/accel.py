@@ -1,0 +1,276 @@
+"""Python host mirror of SURFACE's accelerator interface over libsrf.so.
+
+The classes keep the reference's names and meaning -- TriangleAccel,
+TetraAccel and ParticleAccel with Build32 / Traverse / BoundingBox
+(render/render_accel_triangle.h:69-105, render/render_accel_tetra.h:82-125,
+render/render_accel_particle.h:68-110) -- with Traverse taking a batch of
+rays. Geometry, rays and hits are torch tensors on the GPU (torch is only the
+allocator and stream provider here); every computation runs in libsrf.so.
+Failures raise SrfError where the reference returns false or asserts.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import AccelInfo, SphereDesc, SrfError, TetDesc, TriDesc, check, load
+from .workloads import HIT_DTYPE, RAY_DTYPE, TET_EXT_DTYPE
+
+try:
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+TRI_NODE_DTYPE = np.dtype([("bmin", "<f4", (2, 3)), ("bmax", "<f4", (2, 3)),
+                           ("flag", "<i4"), ("axis", "<i4"), ("data", "<u4", (2,))])
+TET_NODE_DTYPE = np.dtype([("bmin", "<f4", (3,)), ("bmax", "<f4", (3,)),
+                           ("flag", "<i4"), ("axis", "<i4"), ("data", "<u4", (2,))])
+SPH_NODE_DTYPE = np.dtype([("bmin", "<f4", (2, 3)), ("bmax", "<f4", (2, 3)),
+                           ("data", "<u4", (2,)), ("flag", "<i2"), ("axis", "<i2")])
+
+
+def _stream(stream=None):
+    if stream is not None:
+        return C.c_void_p(stream)
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def device_tensor(arr, device="cuda"):
+    """numpy array (any dtype, C-contiguous) -> torch tensor on the GPU holding
+    the same bytes; structured ray/hit arrays become [n, words] float32."""
+    arr = np.ascontiguousarray(arr)
+    if arr.dtype.fields is not None:
+        words = arr.dtype.itemsize // 4
+        arr = arr.view(np.float32).reshape(len(arr), words)
+    elif arr.dtype == np.uint32:
+        arr = arr.view(np.int32)
+    return torch.from_numpy(arr).to(device)
+
+
+def hits_to_numpy(t, dtype=HIT_DTYPE):
+    a = t.detach().cpu().numpy()
+    return np.ascontiguousarray(a).view(dtype).reshape(-1)
+
+
+class Context:
+    """srf_ctx: one per device; owns the build scratch space."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.srf_ctx_create(int(device), C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            self.lib.srf_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(torch.cuda.current_device())
+    return _default_ctx
+
+
+class _Accel:
+    kind = None
+    node_dtype = None
+
+    def __init__(self, ctx=None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.handle = None
+        self._keep = ()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def release(self):
+        if self.handle:
+            self.lib.srf_accel_destroy(self.handle)
+            self.handle = None
+
+    def _build(self, fn, desc, stream):
+        """First call allocates (srf_accel_build_*); later calls rebuild in
+        place like a second TriangleAccel::Build32 on the same object."""
+        if self.handle:
+            check(self.lib.srf_accel_rebuild(self.ctx.handle, self.handle, C.byref(desc),
+                                             _stream(stream)))
+        else:
+            h = C.c_void_p()
+            check(fn(self.ctx.handle, C.byref(desc), _stream(stream), C.byref(h)))
+            self.handle = h
+        return True
+
+    def _require(self):
+        if not self.handle:
+            raise SrfError(1, "accelerator has not been built")
+
+    def get_info(self):
+        self._require()
+        info = AccelInfo()
+        check(self.lib.srf_accel_get_info(self.handle, C.byref(info)))
+        return {f: getattr(info, f) for f, _ in AccelInfo._fields_}
+
+    def BoundingBox(self):
+        """BoundingBox(double bmin[3], double bmax[3]) -> (bmin, bmax)."""
+        self._require()
+        bmin = (C.c_double * 3)()
+        bmax = (C.c_double * 3)()
+        check(self.lib.srf_accel_bounds(self.handle, bmin, bmax))
+        return np.array(bmin[:]), np.array(bmax[:])
+
+    def export(self):
+        """(nodes in the reference layout, DFS preorder; sorted prim indices)."""
+        info = self.get_info()
+        nodes = np.zeros(info["num_nodes"], self.node_dtype)
+        idx = np.zeros(info["num_prims"], np.uint32)
+        check(self.lib.srf_accel_export(self.handle, nodes.ctypes.data_as(C.c_void_p),
+                                        idx.ctypes.data_as(C.c_void_p)))
+        return nodes, idx
+
+    def Traverse(self, rays, hits=None, stream=None):
+        """Closest hit for every ray. rays: [n, 8] float32 GPU tensor (srf_ray);
+        returns hits [n, 4] float32 (t, u, v, prim_id bits)."""
+        self._require()
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+        check(self.lib.srf_trace_closest(self.handle, _ptr(rays), n, _ptr(hits),
+                                         _stream(stream)))
+        return hits
+
+    def TraverseAny(self, rays, t_max=None, out=None, stream=None):
+        """Occlusion flags (int32 0/1) for every ray."""
+        self._require()
+        n = rays.shape[0]
+        if out is None:
+            out = torch.empty(n, dtype=torch.int32, device=rays.device)
+        check(self.lib.srf_trace_any(self.handle, _ptr(rays), _ptr(t_max), n, _ptr(out),
+                                     _stream(stream)))
+        return out
+
+
+class TriangleAccel(_Accel):
+    """TriangleAccel::Build32 / Traverse / BoundingBox on the GPU."""
+    kind = _lib.PRIM_TRIANGLES
+    node_dtype = TRI_NODE_DTYPE
+
+    def Build32(self, vertices, faces, stream=None):
+        d = TriDesc(vertices.data_ptr(), vertices.numel() // 3, faces.data_ptr(),
+                    faces.numel() // 3)
+        return self._build(self.lib.srf_accel_build_triangles, d, stream)
+
+
+class TetraAccel(_Accel):
+    """TetraAccel::Build32 / Traverse (+ InterpolateTetra) / BoundingBox."""
+    kind = _lib.PRIM_TETRAHEDRONS
+    node_dtype = TET_NODE_DTYPE
+
+    def Build32(self, vertices, indices, stream=None):
+        d = TetDesc(vertices.data_ptr(), vertices.numel() // 3, indices.data_ptr(),
+                    indices.numel() // 4)
+        return self._build(self.lib.srf_accel_build_tetras, d, stream)
+
+    def TraverseTetra(self, rays, scalars=None, hits=None, ext=None, stream=None):
+        """Closest hit + InterpolateTetra weights and the interpolated scalar;
+        ext is [n, 8] float32 (srf_tet_hit_ext)."""
+        self._require()
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+        if ext is None:
+            ext = torch.empty((n, 8), dtype=torch.float32, device=rays.device)
+        check(self.lib.srf_trace_tetra(self.handle, _ptr(rays), n, _ptr(scalars), _ptr(hits),
+                                       _ptr(ext), _stream(stream)))
+        return hits, ext
+
+
+class ParticleAccel(_Accel):
+    """ParticleAccel::Build32 / Traverse / BoundingBox (spheres)."""
+    kind = _lib.PRIM_POINTS
+    node_dtype = SPH_NODE_DTYPE
+
+    def Build32(self, positions, radius=None, constant_radius=0.0, stream=None):
+        d = SphereDesc(positions.data_ptr(), radius.data_ptr() if radius is not None else None,
+                       float(constant_radius), positions.numel() // 3)
+        return self._build(self.lib.srf_accel_build_spheres, d, stream)
+
+    def TraverseSelf(self, rays, prev, hits=None, stream=None):
+        """Traverse with per-ray prev_hit_t / prev_hit_normal ([n, 6] float32
+        tensor holding srf_sphere_prev records)."""
+        self._require()
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+        check(self.lib.srf_trace_spheres(self.handle, _ptr(rays), _ptr(prev), n, _ptr(hits),
+                                         _stream(stream)))
+        return hits
+
+
+# ---- LBVH stage entry points (parity of the intermediate products) ----
+
+def morton30(kind, desc_tensors, bmin, bmax, ctx=None, stream=None):
+    ctx = ctx or default_context()
+    if kind == _lib.PRIM_TRIANGLES:
+        V, F = desc_tensors
+        d = TriDesc(V.data_ptr(), V.numel() // 3, F.data_ptr(), F.numel() // 3)
+        n = F.numel() // 3
+    elif kind == _lib.PRIM_TETRAHEDRONS:
+        V, F = desc_tensors
+        d = TetDesc(V.data_ptr(), V.numel() // 3, F.data_ptr(), F.numel() // 4)
+        n = F.numel() // 4
+    else:
+        P, R = desc_tensors
+        d = SphereDesc(P.data_ptr(), R.data_ptr() if R is not None else None, 0.0, P.numel() // 3)
+        n = P.numel() // 3
+    codes = torch.empty(n, dtype=torch.int32, device="cuda")
+    mn = (C.c_float * 3)(*[float(x) for x in bmin])
+    mx = (C.c_float * 3)(*[float(x) for x in bmax])
+    check(ctx.lib.srf_lbvh_morton30(ctx.handle, kind, C.byref(d), mn, mx, _ptr(codes),
+                                    _stream(stream)))
+    return codes
+
+
+def sort30(codes, ctx=None, stream=None):
+    ctx = ctx or default_context()
+    n = codes.numel()
+    sc = torch.empty(n, dtype=torch.int32, device=codes.device)
+    si = torch.empty(n, dtype=torch.int32, device=codes.device)
+    check(ctx.lib.srf_lbvh_sort30(ctx.handle, _ptr(codes), n, _ptr(sc), _ptr(si), _stream(stream)))
+    return sc, si
+
+
+def radix_tree30(sorted_codes, ctx=None, stream=None):
+    ctx = ctx or default_context()
+    n = sorted_codes.numel()
+    out = torch.empty((max(n - 1, 0), 3), dtype=torch.int32, device=sorted_codes.device)
+    check(ctx.lib.srf_lbvh_radix_tree30(ctx.handle, _ptr(sorted_codes), n, _ptr(out),
+                                        _stream(stream)))
+    return out
+
+
+__all__ = ["Context", "TriangleAccel", "TetraAccel", "ParticleAccel", "SrfError",
+           "device_tensor", "hits_to_numpy", "morton30", "sort30", "radix_tree30",
+           "RAY_DTYPE", "HIT_DTYPE", "TET_EXT_DTYPE", "default_context"]

@@ -1,0 +1,430 @@
+// srf_api.hip -- the extern "C" entry points of include/srf.h.
+#include <float.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+#include "srf_internal.h"
+
+namespace srf {
+
+static thread_local char g_err[512] = "";
+
+srf_status set_error(srf_status s, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return s;
+}
+
+srf_status hip_error(hipError_t e, const char *what) {
+  if (e == hipErrorOutOfMemory)
+    return set_error(SRF_ERR_OUT_OF_MEMORY, "%s: %s", what, hipGetErrorString(e));
+  return set_error(SRF_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+srf_status ensure_scratch(Ctx *ctx, size_t bytes, hipStream_t stream) {
+  if (ctx->scratch_bytes >= bytes) return SRF_OK;
+  // the old scratch may still be read by enqueued work
+  SRF_HIP(hipStreamSynchronize(stream));
+  SRF_HIP(hipDeviceSynchronize());
+  if (ctx->scratch) SRF_HIP(hipFree(ctx->scratch));
+  ctx->scratch = nullptr;
+  ctx->scratch_bytes = 0;
+  size_t sz = bytes + bytes / 4 + (1u << 20);
+  SRF_HIP(hipMalloc(&ctx->scratch, sz));
+  ctx->scratch_bytes = sz;
+  return SRF_OK;
+}
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+size_t node_bytes(Kind k) { return k == kTet ? 32 : 64; }
+size_t prim_bytes(Kind k) { return k == kTri ? 48 : (k == kTet ? 64 : 16); }
+
+void free_buffers(Accel *a);
+
+void free_accel(Accel *a) {
+  if (!a) return;
+  DeviceGuard g(a->device);
+  if (a->built) (void)hipEventSynchronize(a->built);
+  free_buffers(a);
+  (void)hipFree(a->params);
+  if (a->built) (void)hipEventDestroy(a->built);
+  delete a;
+}
+
+hipError_t alloc_buffers(Accel *a, uint64_t n) {
+  hipError_t e = hipSuccess;
+  const Kind kind = a->kind;
+  if (e == hipSuccess) e = hipMalloc(&a->nodes, node_bytes(kind) * (2 * n));
+  if (e == hipSuccess) e = hipMalloc(&a->prims, prim_bytes(kind) * n);
+  if (e == hipSuccess) e = hipMalloc((void **)&a->sorted_index, 4 * n);
+  if (e == hipSuccess && kind == kTet) e = hipMalloc((void **)&a->sorted_faces, 16 * n);
+  if (e == hipSuccess) a->capacity = n;
+  return e;
+}
+
+void free_buffers(Accel *a) {
+  (void)hipFree(a->nodes);
+  (void)hipFree(a->prims);
+  (void)hipFree(a->sorted_index);
+  (void)hipFree(a->sorted_faces);
+  a->nodes = a->prims = nullptr;
+  a->sorted_index = a->sorted_faces = nullptr;
+  a->capacity = 0;
+}
+
+srf_status alloc_accel(Ctx *ctx, Kind kind, uint64_t n, uint64_t nv, Accel **out) {
+  Accel *a = new (std::nothrow) Accel();
+  if (!a) return set_error(SRF_ERR_OUT_OF_MEMORY, "host allocation failed");
+  memset(a, 0, sizeof(*a));
+  a->kind = kind;
+  a->device = ctx->device;
+  a->n = n;
+  a->nv = nv;
+  hipError_t e = alloc_buffers(a, n);
+  if (e == hipSuccess) e = hipMalloc((void **)&a->params, sizeof(BuildParams));
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&a->built, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    free_accel(a);
+    return hip_error(e, "accelerator allocation");
+  }
+  *out = a;
+  return SRF_OK;
+}
+
+srf_status check_count(uint64_t n) {
+  if (n < 1024)
+    return set_error(SRF_ERR_UNSUPPORTED,
+                     "%llu primitives: Build32 needs at least 1024 (the reference uses its "
+                     "SAH Build below that)",
+                     (unsigned long long)n);
+  if (n >= (1ull << 28))
+    return set_error(SRF_ERR_UNSUPPORTED, "%llu primitives exceed the 2^28 limit",
+                     (unsigned long long)n);
+  return SRF_OK;
+}
+
+srf_status build_common(srf_ctx *c, Kind kind, const void *desc, uint64_t n, uint64_t nv,
+                        void *stream, srf_accel **out) {
+  Ctx *ctx = reinterpret_cast<Ctx *>(c);
+  if (!ctx || !desc || !out) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  srf_status cs = check_count(n);
+  if (cs != SRF_OK) return cs;
+  DeviceGuard g(ctx->device);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  Accel *a = nullptr;
+  srf_status st = alloc_accel(ctx, kind, n, nv, &a);
+  if (st != SRF_OK) return st;
+  st = build_lbvh(ctx, kind, desc, s, a);
+  if (st == SRF_OK) {
+    hipError_t e = hipEventRecord(a->built, s);
+    if (e != hipSuccess) st = hip_error(e, "hipEventRecord");
+  }
+  if (st != SRF_OK) {
+    free_accel(a);
+    return st;
+  }
+  *out = reinterpret_cast<srf_accel *>(a);
+  return SRF_OK;
+}
+
+srf_status fetch_params(const Accel *a, BuildParams *p) {
+  DeviceGuard g(a->device);
+  SRF_HIP(hipEventSynchronize(a->built));
+  SRF_HIP(hipMemcpy(p, a->params, sizeof(BuildParams), hipMemcpyDeviceToHost));
+  if (p->error)
+    return set_error(SRF_ERR_INTERNAL, "a bounded device wait gave up during the build");
+  return SRF_OK;
+}
+
+}  // namespace
+}  // namespace srf
+
+using namespace srf;
+
+extern "C" {
+
+const char *srf_last_error_string(void) { return srf::g_err; }
+
+int srf_abi_version(void) { return SRF_ABI_VERSION; }
+
+srf_status srf_ctx_create(int device, srf_ctx **out) {
+  if (!out) return set_error(SRF_ERR_INVALID_ARG, "null out");
+  int count = 0;
+  SRF_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count)
+    return set_error(SRF_ERR_INVALID_ARG, "device %d out of range (%d devices)", device, count);
+  Ctx *c = new (std::nothrow) Ctx();
+  if (!c) return set_error(SRF_ERR_OUT_OF_MEMORY, "host allocation failed");
+  c->device = device;
+  c->scratch = nullptr;
+  c->scratch_bytes = 0;
+  *out = reinterpret_cast<srf_ctx *>(c);
+  return SRF_OK;
+}
+
+srf_status srf_ctx_destroy(srf_ctx *ctx) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c) return SRF_OK;
+  {
+    DeviceGuard g(c->device);
+    (void)hipDeviceSynchronize();
+    if (c->scratch) (void)hipFree(c->scratch);
+  }
+  delete c;
+  return SRF_OK;
+}
+
+srf_status srf_accel_build_triangles(srf_ctx *ctx, const srf_tri_desc *d, void *stream,
+                                     srf_accel **out) {
+  if (!d || !d->vertices || !d->faces) return set_error(SRF_ERR_INVALID_ARG, "null mesh");
+  return build_common(ctx, kTri, d, d->num_faces, d->num_vertices, stream, out);
+}
+
+srf_status srf_accel_build_tetras(srf_ctx *ctx, const srf_tet_desc *d, void *stream,
+                                  srf_accel **out) {
+  if (!d || !d->vertices || !d->indices) return set_error(SRF_ERR_INVALID_ARG, "null tetras");
+  return build_common(ctx, kTet, d, d->num_tetras, d->num_vertices, stream, out);
+}
+
+srf_status srf_accel_build_spheres(srf_ctx *ctx, const srf_sphere_desc *d, void *stream,
+                                   srf_accel **out) {
+  if (!d || !d->positions) return set_error(SRF_ERR_INVALID_ARG, "null particles");
+  return build_common(ctx, kSph, d, d->num_particles, d->num_particles, stream, out);
+}
+
+srf_status srf_accel_build_triangles_batch(srf_ctx *ctx, uint32_t num_meshes,
+                                           const srf_tri_desc *descs, void *stream,
+                                           srf_accel **out) {
+  if (!descs || !out) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  for (uint32_t m = 0; m < num_meshes; m++) out[m] = nullptr;
+  for (uint32_t m = 0; m < num_meshes; m++) {
+    srf_status st = srf_accel_build_triangles(ctx, &descs[m], stream, &out[m]);
+    if (st != SRF_OK) {
+      for (uint32_t k = 0; k < m; k++) srf_accel_destroy(out[k]), out[k] = nullptr;
+      return st;
+    }
+  }
+  return SRF_OK;
+}
+
+srf_status srf_accel_rebuild(srf_ctx *ctx, srf_accel *accel, const void *desc, void *stream) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  Accel *a = reinterpret_cast<Accel *>(accel);
+  if (!c || !a || !desc) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (c->device != a->device) return set_error(SRF_ERR_INVALID_ARG, "context/device mismatch");
+  uint64_t n, nv;
+  if (a->kind == kTri) {
+    const srf_tri_desc *d = (const srf_tri_desc *)desc;
+    if (!d->vertices || !d->faces) return set_error(SRF_ERR_INVALID_ARG, "null mesh");
+    n = d->num_faces;
+    nv = d->num_vertices;
+  } else if (a->kind == kTet) {
+    const srf_tet_desc *d = (const srf_tet_desc *)desc;
+    if (!d->vertices || !d->indices) return set_error(SRF_ERR_INVALID_ARG, "null tetras");
+    n = d->num_tetras;
+    nv = d->num_vertices;
+  } else {
+    const srf_sphere_desc *d = (const srf_sphere_desc *)desc;
+    if (!d->positions) return set_error(SRF_ERR_INVALID_ARG, "null particles");
+    n = d->num_particles;
+    nv = n;
+  }
+  srf_status st = check_count(n);
+  if (st != SRF_OK) return st;
+  DeviceGuard g(a->device);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (n > a->capacity) {
+    SRF_HIP(hipEventSynchronize(a->built));
+    free_buffers(a);
+    SRF_HIP(alloc_buffers(a, n));
+  }
+  a->n = n;
+  a->nv = nv;
+  st = build_lbvh(c, a->kind, desc, s, a);
+  if (st != SRF_OK) return st;
+  SRF_HIP(hipEventRecord(a->built, s));
+  return SRF_OK;
+}
+
+srf_status srf_accel_bounds(const srf_accel *accel, double bmin[3], double bmax[3]) {
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!a || !bmin || !bmax) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  BuildParams p;
+  srf_status st = fetch_params(a, &p);
+  if (st != SRF_OK) return st;
+  for (int k = 0; k < 3; k++) {
+    bmin[k] = p.root_bmin[k];
+    bmax[k] = p.root_bmax[k];
+  }
+  return SRF_OK;
+}
+
+srf_status srf_accel_get_info(const srf_accel *accel, srf_accel_info *info) {
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!a || !info) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  BuildParams p;
+  srf_status st = fetch_params(a, &p);
+  if (st != SRF_OK) return st;
+  info->kind = a->kind == kTri ? SRF_PRIM_TRIANGLES
+                               : (a->kind == kTet ? SRF_PRIM_TETRAHEDRONS : SRF_PRIM_POINTS);
+  info->num_prims = a->n;
+  info->num_nodes = p.num_nodes;
+  info->num_leaves = p.num_leaves;
+  info->max_depth = p.max_steps + 1;
+  info->eps_scale = p.eps_scale;
+  info->node_bytes = a->kind == kTri ? 64 : (a->kind == kTet ? 40 : 60);
+  return SRF_OK;
+}
+
+srf_status srf_accel_export(const srf_accel *accel, void *nodes_host, uint32_t *indices_host) {
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!a || !nodes_host || !indices_host) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  BuildParams p;
+  srf_status st = fetch_params(a, &p);
+  if (st != SRF_OK) return st;
+  DeviceGuard g(a->device);
+  const size_t nn = p.num_nodes;
+  SRF_HIP(hipMemcpy(indices_host, a->sorted_index, 4 * a->n, hipMemcpyDeviceToHost));
+  if (a->kind == kTri) {
+    SRF_HIP(hipMemcpy(nodes_host, a->nodes, 64 * nn, hipMemcpyDeviceToHost));
+    return SRF_OK;
+  }
+  const size_t gb = node_bytes(a->kind);
+  uint32_t *tmp = new (std::nothrow) uint32_t[gb / 4 * nn];
+  if (!tmp) return set_error(SRF_ERR_OUT_OF_MEMORY, "host allocation failed");
+  hipError_t e = hipMemcpy(tmp, a->nodes, gb * nn, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    delete[] tmp;
+    return hip_error(e, "hipMemcpy");
+  }
+  uint8_t *dst = reinterpret_cast<uint8_t *>(nodes_host);
+  for (size_t i = 0; i < nn; i++) {
+    const uint32_t *w = tmp + gb / 4 * i;
+    if (a->kind == kTet) {
+      // TetraNode: bmin[3], bmax[3], flag, axis, data[2] (40 B)
+      uint32_t o[10] = {w[0], w[1], w[2], w[4], w[5], w[6], w[3] >> 31,
+                        (w[3] >> 29) & 3u, w[3] & 0x1fffffffu, w[7]};
+      memcpy(dst + 40 * i, o, 40);
+    } else {
+      // ParticleNode: bmin[2][3], bmax[2][3], data[2], short flag, short axis (60 B)
+      uint32_t o[15];
+      memcpy(o, w, 48);
+      o[12] = w[14];
+      o[13] = w[15];
+      o[14] = (w[12] & 0xffffu) | (w[13] << 16);
+      memcpy(dst + 60 * i, o, 60);
+    }
+  }
+  delete[] tmp;
+  return SRF_OK;
+}
+
+srf_status srf_accel_destroy(srf_accel *accel) {
+  free_accel(reinterpret_cast<Accel *>(accel));
+  return SRF_OK;
+}
+
+srf_status srf_trace_closest(const srf_accel *accel, const srf_ray *rays, uint64_t n,
+                             srf_hit *hits, void *stream) {
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!a || (n && (!rays || !hits))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many rays");
+  DeviceGuard g(a->device);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (a->kind == kTri) return trace_tri(a, rays, n, hits, s);
+  if (a->kind == kTet) return trace_tet(a, rays, n, nullptr, hits, nullptr, s);
+  return trace_sph(a, rays, nullptr, n, hits, s);
+}
+
+srf_status srf_trace_tetra(const srf_accel *accel, const srf_ray *rays, uint64_t n,
+                           const float *scalars, srf_hit *hits, srf_tet_hit_ext *ext,
+                           void *stream) {
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!a || (n && (!rays || !hits))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (a->kind != kTet) return set_error(SRF_ERR_INVALID_ARG, "not a tetrahedron accelerator");
+  if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many rays");
+  DeviceGuard g(a->device);
+  return trace_tet(a, rays, n, scalars, hits, ext, reinterpret_cast<hipStream_t>(stream));
+}
+
+srf_status srf_trace_spheres(const srf_accel *accel, const srf_ray *rays,
+                             const srf_sphere_prev *prev, uint64_t n, srf_hit *hits,
+                             void *stream) {
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!a || (n && (!rays || !hits))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (a->kind != kSph) return set_error(SRF_ERR_INVALID_ARG, "not a sphere accelerator");
+  if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many rays");
+  DeviceGuard g(a->device);
+  return trace_sph(a, rays, prev, n, hits, reinterpret_cast<hipStream_t>(stream));
+}
+
+srf_status srf_trace_any(const srf_accel *accel, const srf_ray *rays, const float *t_max,
+                         uint64_t n, uint32_t *occluded, void *stream) {
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!a || (n && (!rays || !occluded))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many rays");
+  DeviceGuard g(a->device);
+  return trace_any(a, rays, t_max, n, occluded, reinterpret_cast<hipStream_t>(stream));
+}
+
+srf_status srf_lbvh_morton30(srf_ctx *ctx, srf_prim_kind kind, const void *desc,
+                             const float bmin[3], const float bmax[3], uint32_t *codes,
+                             void *stream) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !desc || !bmin || !bmax || !codes)
+    return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  Kind k;
+  if (kind == SRF_PRIM_TRIANGLES)
+    k = kTri;
+  else if (kind == SRF_PRIM_TETRAHEDRONS)
+    k = kTet;
+  else if (kind == SRF_PRIM_POINTS)
+    k = kSph;
+  else
+    return set_error(SRF_ERR_INVALID_ARG, "unknown primitive kind 0x%x", (unsigned)kind);
+  DeviceGuard g(c->device);
+  return launch_morton(k, desc, bmin, bmax, codes, reinterpret_cast<hipStream_t>(stream));
+}
+
+srf_status srf_lbvh_sort30(srf_ctx *ctx, const uint32_t *codes, uint64_t n,
+                           uint32_t *sorted_codes, uint32_t *sorted_index, void *stream) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || (n && (!codes || !sorted_codes || !sorted_index)))
+    return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (n == 0) return SRF_OK;
+  if (n >= (1ull << 30)) return set_error(SRF_ERR_INVALID_ARG, "too many keys");
+  DeviceGuard g(c->device);
+  return sort_pairs30(c, codes, (uint32_t)n, sorted_codes, sorted_index,
+                      reinterpret_cast<hipStream_t>(stream));
+}
+
+srf_status srf_lbvh_radix_tree30(srf_ctx *ctx, const uint32_t *sorted_codes, uint64_t n,
+                                 srf_node_info *out, void *stream) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || (n > 1 && (!sorted_codes || !out)))
+    return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (n < 2) return SRF_OK;
+  if (n >= (1ull << 31)) return set_error(SRF_ERR_INVALID_ARG, "too many keys");
+  DeviceGuard g(c->device);
+  return launch_radix_tree(sorted_codes, (uint32_t)n, out, reinterpret_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

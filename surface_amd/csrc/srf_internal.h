@@ -1,0 +1,123 @@
+// srf_internal.h -- shared definitions of the HIP implementation of srf.h.
+//
+// Device layouts (all in HBM, 16-byte aligned):
+//   TriNode / SphNode (64 B)  the reference's TriangleNode word order
+//       (render/render_accel_triangle.h:23-42): bmin[2][3], bmax[2][3],
+//       flag, axis, data[2]; fetched as four 16-byte loads. Spheres use the
+//       same 64 B layout (ParticleNode, render/render_accel_particle.h:24-44,
+//       carries the same fields in 60 B; export converts).
+//   TetNode (32 B)            own box of the node (TetraNode keeps one box,
+//       render/render_accel_tetra.h:24-43) in two 16-byte words:
+//       (bmin.xyz, w0) (bmax.xyz, w1), w0 = flag<<31 | axis<<29 | data[0],
+//       w1 = data[1].
+//   packed primitives, in sorted (leaf) order:
+//       triangle 48 B  (p0.xyz, id) (e1.xyz, 0) (e2.xyz, 0), e1 = p1 - p0,
+//                      e2 = p2 - p0 -- the same float subtractions
+//                      TriangleIsect performs, so results are unchanged;
+//       tetra    64 B  (v0.xyz, id) (v1.xyz, 0) (v2.xyz, 0) (v3.xyz, 0);
+//       sphere   16 B  (centre.xyz, radius).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/srf.h"
+
+namespace srf {
+
+constexpr int kTriMaxLeaf = 16;  // render/render_accel_triangle.cc:34
+constexpr int kTetMaxLeaf = 4;   // render/render_accel_tetra.cc:34
+constexpr int kSphMaxLeaf = 16;  // render/render_accel_particle.cc:51
+constexpr int kMaxDepth32 = 22;  // render/render_accel_triangle.cc:35
+// Branch levels are bounded: root, then depths 0..22 (a child deeper than
+// 22 is a leaf), so a traversal stack never holds more than 25 entries.
+constexpr int kMaxBranchLevels = 24;
+constexpr int kStackSize = 32;
+
+enum Kind : uint32_t { kTri = 0, kTet = 1, kSph = 2 };
+
+// Per-build scalars produced on the device and read by later kernels.
+struct alignas(16) BuildParams {
+  float bmin[3];       // scene box used for Morton codes
+  float bmax[3];
+  float inv[3];        // (float)1024 / (bmax - bmin)
+  float kEPS;          // leaf-box padding (tri: FLT_EPSILON*epsScale)
+  double eps_scale;    // triangles: max scene extent
+  float root_bmin[3];  // BoundingBox() of the finished tree
+  float root_bmax[3];
+  uint32_t num_leaves;
+  uint32_t num_nodes;
+  uint32_t max_steps;  // deepest branch (root = 0 steps)
+  uint32_t error;      // nonzero: a bounded wait gave up (never expected)
+  uint32_t ticket;     // last-block counter of the bounds reduction
+  uint32_t pad[3];
+};
+
+struct Accel {
+  Kind kind;
+  int device;
+  uint64_t n;            // primitives
+  uint64_t nv;           // vertices (tets: for scalar lookups)
+  uint64_t capacity;     // primitives the buffers can hold
+  void *nodes;           // TriNode / TetNode / SphNode
+  void *prims;           // packed primitives (sorted order)
+  uint32_t *sorted_index;
+  uint32_t *sorted_faces;  // tets: original 4 vertex ids per sorted tet
+  BuildParams *params;   // device
+  hipEvent_t built;      // recorded on the build stream
+};
+
+struct Ctx {
+  int device;
+  void *scratch;
+  size_t scratch_bytes;
+};
+
+// Bump allocator over the context scratch (256-byte aligned slices).
+struct Arena {
+  char *base;
+  size_t size, off;
+  template <class T>
+  T *take(size_t count) {
+    off = (off + 255) & ~size_t(255);
+    T *p = reinterpret_cast<T *>(base + off);
+    off += sizeof(T) * count;
+    return p;
+  }
+};
+
+// Error plumbing shared by the translation units.
+srf_status set_error(srf_status s, const char *fmt, ...);
+srf_status hip_error(hipError_t e, const char *what);
+srf_status ensure_scratch(Ctx *ctx, size_t bytes, hipStream_t stream);
+
+#define SRF_HIP(call)                                  \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return srf::hip_error(e_, #call); \
+  } while (0)
+
+// Build-stage launchers (srf_build.hip).
+srf_status scratch_bytes_for_build(uint64_t n, size_t *bytes);
+srf_status build_lbvh(Ctx *ctx, Kind kind, const void *desc, hipStream_t stream,
+                      Accel *acc);
+srf_status launch_morton(Kind kind, const void *desc, const float bmin[3],
+                         const float bmax[3], uint32_t *codes, hipStream_t s);
+srf_status sort_pairs30(Ctx *ctx, const uint32_t *codes_in, uint32_t n,
+                        uint32_t *codes_out, uint32_t *index_out, hipStream_t s);
+srf_status launch_radix_tree(const uint32_t *sorted_codes, uint32_t n,
+                             srf_node_info *out, hipStream_t s);
+
+// Traversal launchers (srf_trace.hip).
+srf_status trace_tri(const Accel *a, const srf_ray *rays, uint64_t n,
+                     srf_hit *hits, hipStream_t s);
+srf_status trace_tet(const Accel *a, const srf_ray *rays, uint64_t n,
+                     const float *scalars, srf_hit *hits, srf_tet_hit_ext *ext,
+                     hipStream_t s);
+srf_status trace_sph(const Accel *a, const srf_ray *rays,
+                     const srf_sphere_prev *prev, uint64_t n, srf_hit *hits,
+                     hipStream_t s);
+srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax,
+                     uint64_t n, uint32_t *occluded, hipStream_t s);
+
+}  // namespace srf

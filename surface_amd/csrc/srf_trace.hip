@@ -1,0 +1,797 @@
+// srf_trace.hip -- closest-hit and any-hit traversal on gfx950.
+//
+// One lane per ray; 8x8 pixel tiles per wavefront when the caller lays rays
+// out that way. The next node lives in a register; the far children wait in
+// a per-lane LDS stack laid out column-major (lane stride), so stack traffic
+// never bank-conflicts. The stack is bounded by the tree's 24 branch levels
+// (25 entries), well inside its 32 slots.
+//
+// Visit order, box tests and primitive tests reproduce the reference exactly
+// (hit ids bit-exact, t/u/v bit-exact):
+//   triangles  TriangleAccel::Traverse  render/render_accel_triangle.cc:2511-2669
+//   tetras     TetraAccel::Traverse     render/render_accel_tetra.cc:2200-2289
+//   spheres    ParticleAccel::Traverse  render/render_accel_particle.cc:3053-3218
+#include <float.h>
+#include <math.h>
+
+#include "srf_internal.h"
+
+namespace srf {
+namespace {
+
+constexpr int kTraceBlock = 256;
+
+struct RaySetup {
+  float o[3], d[3], inv[3];
+  int sign[3];
+  uint32_t sbits;  // sign[k] << k, so the near child needs no indexed load
+};
+
+__device__ __forceinline__ RaySetup setup_ray(const srf_ray *rays, uint32_t i, uint32_t &flags,
+                                              uint32_t &prev) {
+  const float4 *r4 = reinterpret_cast<const float4 *>(rays + i);
+  float4 a = r4[0], b = r4[1];
+  RaySetup s;
+  s.o[0] = a.x;
+  s.o[1] = a.y;
+  s.o[2] = a.z;
+  s.d[0] = b.x;
+  s.d[1] = b.y;
+  s.d[2] = b.z;
+  flags = __float_as_uint(a.w);
+  prev = __float_as_uint(b.w);
+  for (int k = 0; k < 3; k++) {
+    s.sign[k] = s.d[k] < 0.0f ? 1 : 0;
+    // (float)(1.0 / dir): correctly rounded float division gives the same bits
+    s.inv[k] = 1.0f / s.d[k];
+  }
+  s.sbits = (uint32_t)s.sign[0] | ((uint32_t)s.sign[1] << 1) | ((uint32_t)s.sign[2] << 2);
+  return s;
+}
+
+// IntersectRayAABB, render/render_accel_triangle.cc:1815-1861 (two-box
+// nodes, tmax clamped to maxT) and render/render_accel_tetra.cc:1934-1978
+// (own box, tmin <= maxT).
+template <bool kClampTmax>
+__device__ __forceinline__ bool ray_aabb(const RaySetup &s, float maxT, float bx0, float by0,
+                                         float bz0, float bx1, float by1, float bz1) {
+  float mnx = s.sign[0] ? bx1 : bx0, mxx = s.sign[0] ? bx0 : bx1;
+  float mny = s.sign[1] ? by1 : by0, mxy = s.sign[1] ? by0 : by1;
+  float mnz = s.sign[2] ? bz1 : bz0, mxz = s.sign[2] ? bz0 : bz1;
+  float tminx = (mnx - s.o[0]) * s.inv[0];
+  float tmaxx = (mxx - s.o[0]) * s.inv[0];
+  float tminy = (mny - s.o[1]) * s.inv[1];
+  float tmaxy = (mxy - s.o[1]) * s.inv[1];
+  float tmin = (tminx > tminy) ? tminx : tminy;
+  float tmax = (tmaxx < tmaxy) ? tmaxx : tmaxy;
+  float tminz = (mnz - s.o[2]) * s.inv[2];
+  float tmaxz = (mxz - s.o[2]) * s.inv[2];
+  tmin = (tmin > tminz) ? tmin : tminz;
+  tmax = (tmax < tmaxz) ? tmax : tmaxz;
+  if (kClampTmax) {
+    tmax = (maxT < tmax) ? maxT : tmax;
+    return (tmax > 0.0f) && (tmin <= tmax);
+  }
+  return (tmax > 0.0f) && (tmin <= tmax) && (tmin <= maxT);
+}
+
+// ---------------------------------------------------------------- triangles
+
+// TriangleIsect (Moller-Trumbore), render/render_accel_triangle.cc:1887-1938,
+// on a packed triangle (p0, e1 = p1 - p0, e2 = p2 - p0).
+__device__ __forceinline__ bool tri_isect(const RaySetup &s, float4 A, float4 B, float4 C,
+                                          bool double_sided, float kEPS, float tin,
+                                          float &tout, float &uout, float &vout) {
+  const float e1x = B.x, e1y = B.y, e1z = B.z, e2x = C.x, e2y = C.y, e2z = C.z;
+  const float dx = s.d[0], dy = s.d[1], dz = s.d[2];
+  float px = dy * e2z - dz * e2y;
+  float py = dz * e2x - dx * e2z;
+  float pz = dx * e2y - dy * e2x;
+  float det = (e1x * px + e1y * py) + e1z * pz;
+  if (double_sided) {
+    if (fabsf(det) < kEPS) return false;
+  } else {
+    if (det < kEPS) return false;
+  }
+  float inv_det = 1.0f / det;
+  float sx = s.o[0] - A.x, sy = s.o[1] - A.y, sz = s.o[2] - A.z;
+  float qx = sy * e1z - sz * e1y;
+  float qy = sz * e1x - sx * e1z;
+  float qz = sx * e1y - sy * e1x;
+  float u = ((sx * px + sy * py) + sz * pz) * inv_det;
+  float v = ((qx * dx + qy * dy) + qz * dz) * inv_det;
+  float t = ((e2x * qx + e2y * qy) + e2z * qz) * inv_det;
+  if (u < 0.0f || u > 1.0f) return false;
+  if (v < 0.0f || u + v > 1.0f) return false;
+  if (t < 0.0f || t > tin) return false;
+  tout = t;
+  uout = u;
+  vout = v;
+  return true;
+}
+
+template <bool kAny>
+__global__ void __launch_bounds__(kTraceBlock)
+k_trace_tri(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
+            const BuildParams *__restrict__ params, const srf_ray *__restrict__ rays, uint32_t n,
+            srf_hit *__restrict__ hits, const float *__restrict__ tmax_in,
+            uint32_t *__restrict__ occluded, uint32_t max_nodes, uint32_t nprims) {
+  __shared__ uint32_t stack[kStackSize * kTraceBlock];
+  const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t flags, prev;
+  const RaySetup s = setup_ray(rays, i, flags, prev);
+  const bool ds = (flags & SRF_RAY_DOUBLE_SIDED) != 0;
+  const float kEPS = FLT_EPSILON * (float)params->eps_scale;
+  const float tlimit = kAny ? (tmax_in ? tmax_in[i] : FLT_MAX) : FLT_MAX;
+  float t = kAny ? tlimit : FLT_MAX, u = 0.0f, v = 0.0f;
+  uint32_t prim = SRF_NO_PRIM;
+  uint32_t *st = stack + threadIdx.x;
+  int sp = 0;
+  uint32_t cur = 0;
+  for (;;) {
+    if (cur >= max_nodes) break;
+    const float4 *nd = nodes + 4 * (size_t)cur;
+    const uint4 meta = reinterpret_cast<const uint4 *>(nd)[3];
+    if (meta.x == 0) {
+      const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+      bool h0 = ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
+      bool h1 = ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
+      const int nearc = (s.sbits >> meta.y) & 1u;
+      const bool hn = nearc ? h1 : h0, hf = nearc ? h0 : h1;
+      const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+      if (hn) {
+        if (hf && sp < kStackSize) st[(sp++) * kTraceBlock] = cf;
+        cur = cn;
+        continue;
+      }
+      if (hf) {
+        cur = cf;
+        continue;
+      }
+    } else {
+      const uint32_t first = meta.w, end = min(meta.w + meta.z, nprims);
+      for (uint32_t q = first; q < end; q++) {
+        const float4 *P = prims + 3 * (size_t)q;
+        const float4 A = P[0];
+        if (__float_as_uint(A.w) == prev) continue;
+        const float4 B = P[1], C = P[2];
+        float tt, uu, vv;
+        if (tri_isect(s, A, B, C, ds, kEPS, t, tt, uu, vv)) {
+          if (kAny) {
+            if (tt < tlimit) {
+              occluded[i] = 1u;
+              return;
+            }
+          } else {
+            t = tt;
+            u = uu;
+            v = vv;
+            prim = __float_as_uint(A.w);
+          }
+        }
+      }
+    }
+    if (sp == 0) break;
+    cur = st[(--sp) * kTraceBlock];
+  }
+  if (kAny) {
+    occluded[i] = 0u;
+  } else {
+    reinterpret_cast<float4 *>(hits)[i] = make_float4(t, u, v, __uint_as_float(prim));
+  }
+}
+
+// ------------------------------------------------------------------ spheres
+
+// SphereIsect, render/render_accel_particle.cc:2423-2512
+__device__ __forceinline__ bool sphere_isect(const RaySetup &s, float4 c, float tin,
+                                             double prevT, const float *prevN, bool self,
+                                             float &tout) {
+  const float ocx = s.o[0] - c.x, ocy = s.o[1] - c.y, ocz = s.o[2] - c.z;
+  const float dx = s.d[0], dy = s.d[1], dz = s.d[2];
+  const float radius = c.w;
+  float a = (dx * dx + dy * dy) + dz * dz;
+  float b = (float)(2.0 * (double)((dx * ocx + dy * ocy) + dz * ocz));
+  float cc = ((ocx * ocx + ocy * ocy) + ocz * ocz) - radius * radius;
+  float disc = (float)((double)(b * b) - (4.0 * (double)a) * (double)cc);
+  float t0, t1;
+  if (disc < 0.0f) return false;
+  if (disc == 0.0f) {
+    t0 = t1 = (float)(-0.5 * (double)(b / a));
+  } else {
+    float dsq = sqrtf(disc);
+    float q;
+    if (b < 0.0f)
+      q = (float)((double)(-b - dsq) / 2.0);
+    else
+      q = (float)((double)(-b + dsq) / 2.0);
+    t0 = q / a;
+    t1 = cc / q;
+  }
+  if (t0 > t1) {
+    float tmp = t0;
+    t0 = t1;
+    t1 = tmp;
+  }
+  if (t1 < 0.0f) return false;
+  if (self) {
+    float t = t1;
+    float nx = (s.o[0] + t * dx) - c.x, ny = (s.o[1] + t * dy) - c.y,
+          nz = (s.o[2] + t * dz) - c.z;
+    float nd = (nx * prevN[0] + ny * prevN[1]) + nz * prevN[2];
+    if ((t < tin) && ((double)t < prevT) && (nd <= 0.0f)) {
+      tout = t;
+      return true;
+    }
+    return false;
+  }
+  float t = (t0 < 0.0f) ? t1 : t0;
+  if (t < tin) {
+    tout = t;
+    return true;
+  }
+  return false;
+}
+
+template <bool kAny>
+__global__ void __launch_bounds__(kTraceBlock)
+k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
+            const uint32_t *__restrict__ sorted_index, const srf_ray *__restrict__ rays,
+            const srf_sphere_prev *__restrict__ prevs, uint32_t n, srf_hit *__restrict__ hits,
+            const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded,
+            uint32_t max_nodes, uint32_t nprims) {
+  __shared__ uint32_t stack[kStackSize * kTraceBlock];
+  const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t flags, prev;
+  const RaySetup s = setup_ray(rays, i, flags, prev);
+  double prevT = 0.0;
+  float prevN[3] = {0.0f, 0.0f, 0.0f};
+  if (prevs) {
+    prevT = prevs[i].prev_hit_t;
+    for (int k = 0; k < 3; k++) prevN[k] = prevs[i].prev_hit_normal[k];
+  }
+  const float tlimit = kAny ? (tmax_in ? tmax_in[i] : FLT_MAX) : FLT_MAX;
+  float t = kAny ? tlimit : FLT_MAX;
+  uint32_t best = SRF_NO_PRIM;
+  uint32_t *st = stack + threadIdx.x;
+  int sp = 0;
+  uint32_t cur = 0;
+  for (;;) {
+    if (cur >= max_nodes) break;
+    const float4 *nd = nodes + 4 * (size_t)cur;
+    const uint4 meta = reinterpret_cast<const uint4 *>(nd)[3];
+    if (meta.x == 0) {
+      const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+      bool h0 = ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
+      bool h1 = ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
+      const int nearc = (s.sbits >> meta.y) & 1u;
+      const bool hn = nearc ? h1 : h0, hf = nearc ? h0 : h1;
+      const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+      if (hn) {
+        if (hf && sp < kStackSize) st[(sp++) * kTraceBlock] = cf;
+        cur = cn;
+        continue;
+      }
+      if (hf) {
+        cur = cf;
+        continue;
+      }
+    } else {
+      const uint32_t first = meta.w, end = min(meta.w + meta.z, nprims);
+      for (uint32_t q = first; q < end; q++) {
+        const float4 c = prims[q];
+        const bool self = (prev != SRF_NO_PRIM) && (sorted_index[q] == prev);
+        float tt;
+        if (sphere_isect(s, c, t, prevT, prevN, self, tt)) {
+          if (kAny) {
+            if (tt < tlimit) {
+              occluded[i] = 1u;
+              return;
+            }
+          } else {
+            t = tt;
+            best = q;
+          }
+        }
+      }
+    }
+    if (sp == 0) break;
+    cur = st[(--sp) * kTraceBlock];
+  }
+  if (kAny) {
+    occluded[i] = 0u;
+    return;
+  }
+  float u = 0.0f, v = 0.0f;
+  uint32_t prim = SRF_NO_PRIM;
+  if (best != SRF_NO_PRIM) {
+    prim = sorted_index[best];
+    // BuildIntersection (render/render_accel_particle.cc:2968-3049) with
+    // vnormalize (:2041-2056); u, v through double atan2/acos.
+    const float4 c = prims[best];
+    float cx = (s.o[0] + t * s.d[0]) - c.x;
+    float cy = (s.o[1] + t * s.d[1]) - c.y;
+    float cz = (s.o[2] + t * s.d[2]) - c.z;
+    const float kEPS = (float)(FLT_EPSILON * 16.0);
+    float len = sqrtf((cx * cx + cy * cy) + cz * cz);
+    float nx = cx, ny = cy, nz = cz;
+    if (len > kEPS) {
+      float il = 1.0f / len;
+      nx = cx * il;
+      ny = cy * il;
+      nz = cz * il;
+    }
+    u = (float)((atan2((double)nx, (double)nz) + M_PI) * 0.5 * (1.0 / M_PI));
+    v = (float)(acos((double)ny) / M_PI);
+  }
+  reinterpret_cast<float4 *>(hits)[i] = make_float4(t, u, v, __uint_as_float(prim));
+}
+
+// ------------------------------------------------------------------- tetras
+
+struct D3 {
+  double x, y, z;
+};
+__device__ __forceinline__ D3 d3sub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 d3add(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 d3scale(double f, D3 v) { return {v.x * f, v.y * f, v.z * f}; }
+__device__ __forceinline__ D3 d3cross(D3 a, D3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double d3dot(D3 a, D3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ int fsign(double x) {
+  // render/render_accel_tetra.cc:89-93
+  return x > DBL_EPSILON ? 1 : (x < -DBL_EPSILON ? -1 : 0);
+}
+struct Pl {
+  D3 d, c;
+};
+__device__ __forceinline__ Pl mk_pl(D3 v0, D3 v1) { return {d3sub(v1, v0), d3cross(v1, v0)}; }
+__device__ __forceinline__ double pl_dot(const Pl &a, const Pl &b) {
+  return d3dot(a.d, b.c) + d3dot(b.d, a.c);
+}
+__device__ __forceinline__ D3 bary3(double a, double b, D3 A, D3 B, D3 C) {
+  return d3add(d3add(d3scale((1 - a) - b, A), d3scale(a, B)), d3scale(b, C));
+}
+__device__ __forceinline__ void param_dist(D3 o, D3 d, D3 ep, D3 lp, double &te, double &tl) {
+  // ComputeParametricDist, render/render_accel_tetra.cc:123-139
+  if (d.x != 0.0) {
+    double inv = 1.0 / d.x;
+    te = (ep.x - o.x) * inv;
+    tl = (lp.x - o.x) * inv;
+  } else if (d.y != 0.0) {
+    double inv = 1.0 / d.y;
+    te = (ep.y - o.y) * inv;
+    tl = (lp.y - o.y) * inv;
+  } else {
+    double inv = 1.0 / d.z;
+    te = (ep.z - o.z) * inv;
+    tl = (lp.z - o.z) * inv;
+  }
+}
+
+// RayTetraPluecker (Platis & Theoharis), render/render_accel_tetra.cc:243-567.
+// Outputs the entry face, uEnter1, uLeave1 (the call site's argument order
+// puts these into isect.u and isect.v, :2103-2111) and both distances.
+__device__ bool ray_tet(D3 orig, D3 dir, const D3 vert[4], int &enterFace, double &uE1,
+                        double &uL1, double &tE, double &tL) {
+  enterFace = -1;
+  double uE2 = 0, uL2 = 0;
+  uE1 = 0;
+  uL1 = 0;
+  double uAB = 0, uAC = 0, uDB = 0, uDC = 0, uBC = 0, uAD = 0;
+  int signAB = -2, signAC = -2, signDB = -2, signDC = -2, signBC = -2, signAD = -2;
+  const Pl plRay = mk_pl(orig, d3add(orig, dir));
+  int nextSign = 0;
+  D3 ep = {0, 0, 0}, lp = {0, 0, 0};
+
+  uAB = pl_dot(plRay, mk_pl(vert[0], vert[1]));
+  signAB = fsign(uAB);
+  uAC = pl_dot(plRay, mk_pl(vert[0], vert[2]));
+  signAC = fsign(uAC);
+
+  if ((signAC == -signAB) || (signAC == 0) || (signAB == 0)) {
+    uBC = pl_dot(plRay, mk_pl(vert[1], vert[2]));
+    signBC = fsign(uBC);
+    int signABC = signAB;
+    if (signABC == 0) {
+      signABC = -signAC;
+      if (signABC == 0) signABC = signBC;
+    }
+    if ((signABC != 0) && ((signBC == signABC) || (signBC == 0))) {
+      double invVolABC = 1.0 / ((uAB + uBC) - uAC);
+      if (signABC == 1) {
+        enterFace = 3;
+        uE1 = -uAC * invVolABC;
+        uE2 = uAB * invVolABC;
+        ep = bary3(uE1, uE2, vert[0], vert[1], vert[2]);
+        nextSign = -1;
+      } else {
+        uL1 = -uAC * invVolABC;
+        uL2 = uAB * invVolABC;
+        lp = bary3(uL1, uL2, vert[0], vert[1], vert[2]);
+        nextSign = 1;
+      }
+      uAD = pl_dot(plRay, mk_pl(vert[0], vert[3]));
+      signAD = fsign(uAD);
+      if ((signAD == nextSign) || (signAD == 0)) {
+        uDB = pl_dot(plRay, mk_pl(vert[3], vert[1]));
+        signDB = fsign(uDB);
+        if ((signDB == nextSign) || ((signDB == 0) && ((signAD != 0) || (signAB != 0)))) {
+          double invVolBAD = 1.0 / ((uAD + uDB) - uAB);
+          if (nextSign == 1) {
+            enterFace = 2;
+            uE1 = uDB * invVolBAD;
+            uE2 = -uAB * invVolBAD;
+            ep = bary3(uE1, uE2, vert[1], vert[0], vert[3]);
+          } else {
+            uL1 = uDB * invVolBAD;
+            uL2 = -uAB * invVolBAD;
+            lp = bary3(uL1, uL2, vert[1], vert[0], vert[3]);
+          }
+          param_dist(orig, dir, ep, lp, tE, tL);
+          return true;
+        }
+      }
+      uDC = pl_dot(plRay, mk_pl(vert[3], vert[2]));
+      signDC = fsign(uDC);
+      if ((signDC == -nextSign) || ((signDC == 0) && ((signAD != 0) || (signAC != 0)))) {
+        double invVolCDA = 1.0 / ((uAC - uDC) - uAD);
+        if (nextSign == 1) {
+          enterFace = 1;
+          uE1 = uAC * invVolCDA;
+          uE2 = -uDC * invVolCDA;
+          ep = bary3(uE1, uE2, vert[2], vert[3], vert[0]);
+        } else {
+          uL1 = uAC * invVolCDA;
+          uL2 = -uDC * invVolCDA;
+          lp = bary3(uL1, uL2, vert[2], vert[3], vert[0]);
+        }
+        param_dist(orig, dir, ep, lp, tE, tL);
+        return true;
+      } else {
+        if (signDB == -2) uDB = pl_dot(plRay, mk_pl(vert[3], vert[1]));
+        double invVolDCB = 1.0 / ((uDC - uBC) - uDB);
+        if (nextSign == 1) {
+          enterFace = 0;
+          uE1 = -uDB * invVolDCB;
+          uE2 = uDC * invVolDCB;
+          ep = bary3(uE1, uE2, vert[3], vert[2], vert[1]);
+        } else {
+          uL1 = -uDB * invVolDCB;
+          uL2 = uDC * invVolDCB;
+          lp = bary3(uL1, uL2, vert[3], vert[2], vert[1]);
+        }
+        param_dist(orig, dir, ep, lp, tE, tL);
+        return true;
+      }
+    }
+  }
+
+  uAD = pl_dot(plRay, mk_pl(vert[0], vert[3]));
+  signAD = fsign(uAD);
+  if ((signAD == -signAB) || (signAB == 0) || (signAD == 0)) {
+    uDB = pl_dot(plRay, mk_pl(vert[3], vert[1]));
+    signDB = fsign(uDB);
+    int signBAD = -signAB;
+    if (signBAD == 0) {
+      signBAD = signAD;
+      if (signBAD == 0) signBAD = signDB;
+    }
+    if ((signBAD != 0) && ((signDB == signBAD) || (signDB == 0))) {
+      double invVolBAD = 1.0 / ((uAD + uDB) - uAB);
+      if (signBAD == 1) {
+        enterFace = 2;
+        uE1 = uDB * invVolBAD;
+        uE2 = -uAB * invVolBAD;
+        ep = bary3(uE1, uE2, vert[1], vert[0], vert[3]);
+        nextSign = -1;
+      } else {
+        uL1 = uDB * invVolBAD;
+        uL2 = -uAB * invVolBAD;
+        lp = bary3(uL1, uL2, vert[1], vert[0], vert[3]);
+        nextSign = 1;
+      }
+      uDC = pl_dot(plRay, mk_pl(vert[3], vert[2]));
+      signDC = fsign(uDC);
+      if ((signDC == -nextSign) || ((signDC == 0) && ((signAD != 0) || (signAC != 0)))) {
+        double invVolCDA = 1.0 / ((uAC - uDC) - uAD);
+        if (nextSign == 1) {
+          enterFace = 1;
+          uE1 = uAC * invVolCDA;
+          uE2 = -uDC * invVolCDA;
+          ep = bary3(uE1, uE2, vert[2], vert[3], vert[0]);
+        } else {
+          uL1 = uAC * invVolCDA;
+          uL2 = -uDC * invVolCDA;
+          lp = bary3(uL1, uL2, vert[2], vert[3], vert[0]);
+        }
+        param_dist(orig, dir, ep, lp, tE, tL);
+        return true;
+      } else {
+        if (signBC == -2) uBC = pl_dot(plRay, mk_pl(vert[1], vert[2]));
+        double invVolDCB = 1.0 / ((uDC - uBC) - uDB);
+        if (nextSign == 1) {
+          enterFace = 0;
+          uE1 = -uDB * invVolDCB;
+          uE2 = uDC * invVolDCB;
+          ep = bary3(uE1, uE2, vert[3], vert[2], vert[1]);
+        } else {
+          uL1 = -uDB * invVolDCB;
+          uL2 = uDC * invVolDCB;
+          lp = bary3(uL1, uL2, vert[3], vert[2], vert[1]);
+        }
+        param_dist(orig, dir, ep, lp, tE, tL);
+        return true;
+      }
+    }
+  }
+
+  if ((-signAD == signAC) || (signAC == 0) || (signAD == 0)) {
+    uDC = pl_dot(plRay, mk_pl(vert[3], vert[2]));
+    signDC = fsign(uDC);
+    int signCDA = signAC;
+    if (signCDA == 0) {
+      signCDA = -signAD;
+      if (signCDA == 0) signCDA = -signDC;
+    }
+    if ((signCDA != 0) && ((-signDC == signCDA) || (signDC == 0))) {
+      double invVolCDA = 1.0 / ((uAC - uDC) - uAD);
+      if (signBC == -2) uBC = pl_dot(plRay, mk_pl(vert[1], vert[2]));
+      if (signDB == -2) uDB = pl_dot(plRay, mk_pl(vert[3], vert[1]));
+      double invVolDCB = 1.0 / ((uDC - uBC) - uDB);
+      if (signCDA == 1) {
+        enterFace = 1;
+        uE1 = uAC * invVolCDA;
+        uE2 = -uDC * invVolCDA;
+        ep = bary3(uE1, uE2, vert[2], vert[3], vert[0]);
+        uL1 = -uDB * invVolDCB;
+        uL2 = uDC * invVolDCB;
+        lp = bary3(uL1, uL2, vert[3], vert[2], vert[1]);
+      } else {
+        uL1 = uAC * invVolCDA;
+        uL2 = -uDC * invVolCDA;
+        lp = bary3(uL1, uL2, vert[2], vert[3], vert[0]);
+        enterFace = 0;
+        uE1 = -uDB * invVolDCB;
+        uE2 = uDC * invVolDCB;
+        ep = bary3(uE1, uE2, vert[3], vert[2], vert[1]);
+      }
+      param_dist(orig, dir, ep, lp, tE, tL);
+      return true;
+    }
+  }
+  return false;
+}
+
+// InterpolateTetra, interpolate_mode 0 (render/render_accel_tetra.cc:148-241),
+// in float with the real3 operators' evaluation order.
+struct F3 {
+  float x, y, z;
+};
+__device__ __forceinline__ F3 f3sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 f3add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 f3mul(F3 a, float f) { return {a.x * f, a.y * f, a.z * f}; }
+__device__ __forceinline__ F3 f3div(F3 a, float f) { return {a.x / f, a.y / f, a.z / f}; }
+__device__ __forceinline__ F3 f3cross(F3 l, F3 r) {
+  return {l.y * r.z - l.z * r.y, l.z * r.x - l.x * r.z, l.x * r.y - l.y * r.x};
+}
+__device__ __forceinline__ float f3dot(F3 l, F3 r) { return (l.x * r.x + l.y * r.y) + l.z * r.z; }
+__device__ __forceinline__ float f3len(F3 a) { return sqrtf((a.x * a.x + a.y * a.y) + a.z * a.z); }
+__device__ __forceinline__ bool contains3(int v, int a, int b, int c) {
+  return v == a || v == b || v == c;
+}
+
+__device__ void interpolate_tetra(float d_out[4], F3 pt, const F3 verts[4]) {
+  const float kEPS = (float)0.001;
+  F3 edges[6];
+  edges[0] = f3sub(verts[1], verts[0]);
+  edges[1] = f3sub(verts[2], verts[1]);
+  edges[2] = f3sub(verts[0], verts[2]);
+  edges[3] = f3sub(verts[3], verts[0]);
+  edges[4] = f3sub(verts[3], verts[1]);
+  edges[5] = f3sub(verts[3], verts[2]);
+  for (int v = 0; v < 4; v++) {
+    F3 rayorg = verts[v];
+    F3 raydir = f3sub(pt, verts[v]);
+    F3 raypc = f3cross(raydir, rayorg);
+    float d = f3len(f3sub(pt, rayorg));
+    float ws[6];
+    bool cw0[6], cw1[6];
+    for (int i = 0; i < 6; i++) {
+      ws[i] = f3dot(raydir, f3cross(edges[i], verts[i % 3])) + f3dot(edges[i], raypc);
+      cw0[i] = ws[i] >= -kEPS;
+      cw1[i] = ws[i] <= kEPS;
+    }
+    F3 cp = {(float)1e16, (float)1e16, (float)1e16};
+    // face vertex lists {0,2,1}, {1,2,3}, {0,3,2}, {0,1,3}
+    if (cw0[0] && cw0[1] && cw0[2] && !contains3(v, 0, 2, 1))
+      cp = f3div(f3add(f3add(f3mul(verts[2], ws[0]), f3mul(verts[0], ws[1])),
+                       f3mul(verts[1], ws[2])),
+                 (ws[0] + ws[1]) + ws[2]);
+    else if (cw1[1] && cw0[4] && cw1[5] && !contains3(v, 1, 2, 3))
+      cp = f3div(f3sub(f3add(f3mul(verts[3], ws[1]), f3mul(verts[1], ws[5])),
+                       f3mul(verts[2], ws[4])),
+                 (ws[5] + ws[1]) - ws[4]);
+    else if (cw1[2] && cw1[3] && cw0[5] && !contains3(v, 0, 3, 2))
+      cp = f3div(f3sub(f3add(f3mul(verts[3], ws[2]), f3mul(verts[2], ws[3])),
+                       f3mul(verts[0], ws[5])),
+                 (ws[2] + ws[3]) - ws[5]);
+    else if (cw1[0] && cw0[3] && cw1[4] && !contains3(v, 0, 1, 3))
+      cp = f3div(f3sub(f3add(f3mul(verts[3], ws[0]), f3mul(verts[0], ws[4])),
+                       f3mul(verts[1], ws[3])),
+                 (ws[0] + ws[4]) - ws[3]);
+    float dv = d / f3len(f3sub(cp, rayorg));
+    if (dv < 0.0f)
+      dv = 0.0f;
+    else if (!(dv < 1.0f) && !(dv > 0.0f))
+      dv = 1.0f;
+    d_out[v] = 1.0f - dv;
+  }
+  float ds = 0.0f, w = 0.0f, dd[4];
+  for (int i = 0; i < 4; i++) ds += f3len(f3sub(pt, verts[i]));
+  for (int i = 0; i < 4; i++) dd[i] = ds / f3len(f3sub(pt, verts[i]));
+  for (int i = 0; i < 4; i++) {
+    d_out[i] *= dd[i];
+    w += d_out[i];
+  }
+  for (int i = 0; i < 4; i++) d_out[i] = d_out[i] / w;
+}
+
+template <bool kAny>
+__global__ void __launch_bounds__(kTraceBlock)
+k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
+            const uint4 *__restrict__ sorted_faces, const srf_ray *__restrict__ rays, uint32_t n,
+            const float *__restrict__ scalars, srf_hit *__restrict__ hits,
+            srf_tet_hit_ext *__restrict__ ext, const float *__restrict__ tmax_in,
+            uint32_t *__restrict__ occluded, uint32_t max_nodes, uint32_t nprims) {
+  __shared__ uint32_t stack[kStackSize * kTraceBlock];
+  const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t flags, prev;
+  const RaySetup s = setup_ray(rays, i, flags, prev);
+  const float tlimit = kAny ? (tmax_in ? tmax_in[i] : FLT_MAX) : FLT_MAX;
+  float t = kAny ? tlimit : FLT_MAX, u = 0.0f, v = 0.0f;
+  uint32_t best = SRF_NO_PRIM, subface = 0;
+  double tleave = 0.0;
+  const D3 ro = {s.o[0], s.o[1], s.o[2]}, rd = {s.d[0], s.d[1], s.d[2]};
+  uint32_t *st = stack + threadIdx.x;
+  int sp = 0;
+  uint32_t cur = 0;
+  for (;;) {
+    if (cur >= max_nodes) break;
+    const float4 a = nodes[2 * (size_t)cur], b = nodes[2 * (size_t)cur + 1];
+    const uint32_t w0 = __float_as_uint(a.w), w1 = __float_as_uint(b.w);
+    const bool hit = ray_aabb<false>(s, t, a.x, a.y, a.z, b.x, b.y, b.z);
+    if (hit) {
+      if (!(w0 >> 31)) {
+        const int nearc = (s.sbits >> ((w0 >> 29) & 3u)) & 1u;
+        const uint32_t c0 = w0 & 0x1fffffffu, c1 = w1;
+        if (sp < kStackSize) st[(sp++) * kTraceBlock] = nearc ? c0 : c1;
+        cur = nearc ? c1 : c0;
+        continue;
+      }
+      const uint32_t count = w0 & 0x1fffffffu, first = w1;
+      for (uint32_t q = first; q < min(first + count, nprims); q++) {
+        const float4 *P = prims + 4 * (size_t)q;
+        const float4 p0 = P[0];
+        const uint32_t id = __float_as_uint(p0.w);
+        if (id == prev) continue;
+        const float4 p1 = P[1], p2 = P[2], p3 = P[3];
+        const D3 vtx[4] = {{p0.x, p0.y, p0.z}, {p1.x, p1.y, p1.z}, {p2.x, p2.y, p2.z},
+                           {p3.x, p3.y, p3.z}};
+        int ef;
+        double eu, lu, et = 0.0, lt = 0.0;
+        if (ray_tet(ro, rd, vtx, ef, eu, lu, et, lt)) {
+          if ((et >= 0.0) && (et < (double)t)) {
+            if (kAny) {
+              if ((float)et < tlimit) {
+                occluded[i] = 1u;
+                return;
+              }
+            } else {
+              t = (float)et;
+              u = (float)eu;
+              v = (float)lu;
+              best = q;
+              subface = (uint32_t)ef;
+              tleave = lt;
+            }
+          }
+        }
+      }
+    }
+    if (sp == 0) break;
+    cur = st[(--sp) * kTraceBlock];
+  }
+  if (kAny) {
+    occluded[i] = 0u;
+    return;
+  }
+  uint32_t prim = SRF_NO_PRIM;
+  srf_tet_hit_ext e = {};
+  if (best != SRF_NO_PRIM) {
+    const float4 *P = prims + 4 * (size_t)best;
+    prim = __float_as_uint(P[0].w);
+    if (ext) {
+      F3 verts[4];
+      for (int k = 0; k < 4; k++) verts[k] = {P[k].x, P[k].y, P[k].z};
+      F3 pos = {s.o[0] + s.d[0] * t, s.o[1] + s.d[1] * t, s.o[2] + s.d[2] * t};
+      interpolate_tetra(e.d, pos, verts);
+      e.subface = subface;
+      e.t_leave = (float)tleave;
+      if (scalars) {
+        const uint4 f = sorted_faces[best];
+        e.scalar = ((scalars[f.x] * e.d[0] + scalars[f.y] * e.d[1]) + scalars[f.z] * e.d[2]) +
+                   scalars[f.w] * e.d[3];
+      }
+    }
+  }
+  reinterpret_cast<float4 *>(hits)[i] = make_float4(t, u, v, __uint_as_float(prim));
+  if (ext) {
+    float4 *e4 = reinterpret_cast<float4 *>(ext + i);
+    e4[0] = make_float4(e.d[0], e.d[1], e.d[2], e.d[3]);
+    e4[1] = make_float4(e.scalar, __uint_as_float(e.subface), e.t_leave, 0.0f);
+  }
+}
+
+inline uint32_t trace_blocks(uint64_t n) { return (uint32_t)((n + kTraceBlock - 1) / kTraceBlock); }
+inline uint32_t max_nodes(const Accel *a) { return (uint32_t)(2 * a->n - 1); }
+
+}  // namespace
+
+srf_status trace_tri(const Accel *a, const srf_ray *rays, uint64_t n, srf_hit *hits,
+                     hipStream_t s) {
+  if (n == 0) return SRF_OK;
+  hipLaunchKernelGGL(k_trace_tri<false>, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
+                     (const float4 *)a->nodes, (const float4 *)a->prims, a->params, rays,
+                     (uint32_t)n, hits, nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
+  SRF_HIP(hipGetLastError());
+  return SRF_OK;
+}
+
+srf_status trace_tet(const Accel *a, const srf_ray *rays, uint64_t n, const float *scalars,
+                     srf_hit *hits, srf_tet_hit_ext *ext, hipStream_t s) {
+  if (n == 0) return SRF_OK;
+  hipLaunchKernelGGL(k_trace_tet<false>, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
+                     (const float4 *)a->nodes, (const float4 *)a->prims,
+                     (const uint4 *)a->sorted_faces, rays, (uint32_t)n, scalars, hits, ext,
+                     nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
+  SRF_HIP(hipGetLastError());
+  return SRF_OK;
+}
+
+srf_status trace_sph(const Accel *a, const srf_ray *rays, const srf_sphere_prev *prev,
+                     uint64_t n, srf_hit *hits, hipStream_t s) {
+  if (n == 0) return SRF_OK;
+  hipLaunchKernelGGL(k_trace_sph<false>, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
+                     (const float4 *)a->nodes, (const float4 *)a->prims, a->sorted_index, rays,
+                     prev, (uint32_t)n, hits, nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
+  SRF_HIP(hipGetLastError());
+  return SRF_OK;
+}
+
+srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax, uint64_t n,
+                     uint32_t *occluded, hipStream_t s) {
+  if (n == 0) return SRF_OK;
+  const dim3 g(trace_blocks(n)), b(kTraceBlock);
+  if (a->kind == kTri)
+    hipLaunchKernelGGL(k_trace_tri<true>, g, b, 0, s, (const float4 *)a->nodes,
+                       (const float4 *)a->prims, a->params, rays, (uint32_t)n, nullptr, tmax,
+                       occluded, max_nodes(a), (uint32_t)a->n);
+  else if (a->kind == kTet)
+    hipLaunchKernelGGL(k_trace_tet<true>, g, b, 0, s, (const float4 *)a->nodes,
+                       (const float4 *)a->prims, (const uint4 *)a->sorted_faces, rays,
+                       (uint32_t)n, nullptr, nullptr, nullptr, tmax, occluded, max_nodes(a),
+                       (uint32_t)a->n);
+  else
+    hipLaunchKernelGGL(k_trace_sph<true>, g, b, 0, s, (const float4 *)a->nodes,
+                       (const float4 *)a->prims, a->sorted_index, rays, nullptr, (uint32_t)n,
+                       nullptr, tmax, occluded, max_nodes(a), (uint32_t)a->n);
+  SRF_HIP(hipGetLastError());
+  return SRF_OK;
+}
+
+}  // namespace srf

@@ -70,8 +70,11 @@ _KUHN = np.array([[0, 1, 3, 7], [0, 3, 2, 7], [0, 2, 6, 7],
 def tet_grid(cells=64, seed=BASE_SEED + 2, jitter=0.2, n_gauss=16):
     """Jittered (cells+1)^3 lattice on [0, cells]^3, 6 tets per hex.
 
-    Returns V (float32), F (uint32, 4 per tet, positively oriented) and a
-    per-vertex float32 scalar (sum of `n_gauss` anisotropic Gaussians).
+    Returns V (float32), F (uint32, 4 per tet) and a per-vertex float32
+    scalar (sum of `n_gauss` anisotropic Gaussians). Tets are oriented with
+    det(v1 - v0, v2 - v0, v3 - v0) < 0, the orientation for which
+    RayTetraPluecker (render/render_accel_tetra.cc:243-567) labels the face
+    a ray crosses first as the entry face.
     """
     rng = _rng(seed)
     n1 = cells + 1
@@ -87,8 +90,8 @@ def tet_grid(cells=64, seed=BASE_SEED + 2, jitter=0.2, n_gauss=16):
     F = (base[:, None, None] + corner[_KUHN][None, :, :]).reshape(-1, 4)
     a, b, cc, d = (P[F[:, k]] for k in range(4))
     vol = np.einsum("ij,ij->i", np.cross(b - a, cc - a), d - a)
-    neg = vol < 0
-    F[neg, 1], F[neg, 2] = F[neg, 2].copy(), F[neg, 1].copy()
+    pos = vol > 0
+    F[pos, 1], F[pos, 2] = F[pos, 2].copy(), F[pos, 1].copy()
     centres = rng.uniform(0.0, cells, (n_gauss, 3))
     sig = rng.uniform(0.05, 0.25, (n_gauss, 3)) * cells
     w = rng.uniform(0.2, 1.0, n_gauss)
@@ -188,15 +191,15 @@ def pinhole_rays(eye, target, up, fov_deg, width, height, tiled=True,
 
 def heightfield_rays(width=1024, height=1024, tiled=True):
     """Camera framing the [0,1]^2 heightfield from above at an angle."""
-    return pinhole_rays((0.5, -0.45, 0.95), (0.5, 0.45, 0.0), (0, 0, 1), 50.0,
+    return pinhole_rays((0.5, -0.1, 1.0), (0.5, 0.45, 0.0), (0, 0, 1), 48.0,
                         width, height, tiled)
 
 
 def tet_rays(cells=64, width=1024, height=1024, tiled=True):
     """Camera looking at the [0, cells]^3 tet grid across a corner."""
     c = float(cells)
-    return pinhole_rays((1.9 * c, -0.7 * c, 1.6 * c), (0.5 * c, 0.5 * c, 0.5 * c),
-                        (0, 0, 1), 40.0, width, height, tiled)
+    return pinhole_rays((1.6 * c, -0.6 * c, 1.3 * c), (0.5 * c, 0.5 * c, 0.5 * c),
+                        (0, 0, 1), 36.0, width, height, tiled)
 
 
 def sphere_rays(width=2048, height=2048, tiled=True):
@@ -209,3 +212,19 @@ def volume_rays(width=1024, height=1024, tiled=True):
     """Camera framing the unit-cube volume (texture space)."""
     return pinhole_rays((1.8, -1.2, 1.5), (0.5, 0.5, 0.5), (0, 0, 1), 35.0,
                         width, height, tiled)
+
+
+def deep_chain_points(dup=40, radius=1e-3):
+    """Particles whose Morton codes run 1, 2, 4, ..., 2^29 (each position
+    repeated `dup` times): the radix tree is a 30-deep caterpillar whose
+    subtrees exceed the depth cap 22, so Build32 emits depth-capped leaves
+    (render/render_accel_triangle.cc:909)."""
+    pts = [(0.0, 0.0, 0.0), (1023.0, 1023.0, 1023.0)]
+    for p in range(30):
+        b, axis = p // 3, 2 - (p % 3)
+        q = [0.0, 0.0, 0.0]
+        q[axis] = float(1 << b)
+        pts.append(tuple(q))
+    P = np.repeat(np.array(pts, np.float32), dup, axis=0)
+    R = np.full(len(P), radius, np.float32)
+    return np.ascontiguousarray(P), R

@@ -1,0 +1,143 @@
+"""Write the golden fixtures of tests/golden/ from the reference itself.
+
+Runs the reference's own render/ code (oracle/_ref/libsrfref_serial.so,
+built from the reference sources by oracle/Makefile) on small deterministic
+inputs and stores inputs + outputs as .npz fixtures; for the full-size
+heightfield (522,242 triangles, 1024^2 rays) and tetra grid (1,572,864 tets,
+1024^2 rays) it stores SHA-256 digests of the tree, the permutation and the
+per-ray hits. Run in the container that has the reference checkout:
+
+    make -C oracle && python tests/golden/gen_golden.py [--full]
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle.oracle import Reference, canonical_sph, canonical_tet, canonical_tri  # noqa: E402
+from surface_amd import workloads as W  # noqa: E402
+
+
+def sha(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def tri_case(ref, grid, res):
+    V, F = W.heightfield(grid)
+    bmin = V[F.reshape(-1)].min(0)
+    bmax = V[F.reshape(-1)].max(0)
+    codes = ref.morton("tri", V, F, bmin, bmax)
+    sc, si = ref.sort_keys(codes)
+    info = ref.radix_tree(sc)
+    tree = ref.build("tri", V, F, keep=True)
+    rays = W.heightfield_rays(res, res)
+    hits, _ = ref.trace(tree, rays)
+    ref.free(tree)
+    return dict(V=V, F=F, bmin=bmin, bmax=bmax, codes=codes, sorted_codes=sc,
+                sorted_index=si, nodeinfo=info, nodes=canonical_tri(tree["nodes"]),
+                indices=tree["indices"], bounds=tree["bounds"], rays=rays, hits=hits)
+
+
+def tet_case(ref, cells, res):
+    V, F, S = W.tet_grid(cells)
+    tree = ref.build("tet", V, F, keep=True)
+    rays = W.tet_rays(cells, res, res)
+    hits, ext = ref.trace(tree, rays, scalars=S)
+    ref.free(tree)
+    return dict(V=V, F=F, S=S, nodes=canonical_tet(tree["nodes"]), indices=tree["indices"],
+                bounds=tree["bounds"], rays=rays, hits=hits, ext_d=ext["d"],
+                ext_scalar=ext["scalar"], ext_subface=ext["subface"])
+
+
+def sph_case(ref, n, res):
+    P, R = W.plummer_spheres(n)
+    tree = ref.build("sph", P, R=R, keep=True)
+    rays = W.sphere_rays(res, res)
+    hits, _ = ref.trace(tree, rays)
+    ref.free(tree)
+    codes_simd = ref.morton("pts", P, None, P.min(0) - 1e-3, P.max(0) + 1e-3, simd=True)
+    codes_scalar = ref.morton("pts", P, None, P.min(0) - 1e-3, P.max(0) + 1e-3, simd=False)
+    assert np.array_equal(codes_simd, codes_scalar)
+    return dict(P=P, R=R, nodes=canonical_sph(tree["nodes"]), indices=tree["indices"],
+                bounds=tree["bounds"], rays=rays, hits=hits)
+
+
+def tex_case(ref):
+    rng = np.random.Generator(np.random.PCG64(W.BASE_SEED + 9))
+    dims = np.array([9, 7, 5], np.int32)
+    vox = rng.uniform(-1, 1, int(np.prod(dims))).astype(np.float32)
+    uvw = rng.uniform(-1.5, 2.5, (4000, 3)).astype(np.float32)
+    edge = np.array([[0, 0, 0], [1, 1, 1], [0.5, 0.5, 0.5], [-0.0, 1.0, 0.25],
+                     [-0.01, 1.01, 0.999], [1e-8, -1e-8, 2.0]], np.float32)
+    uvw = np.concatenate([uvw, edge])
+    out = {"dims": dims, "vox": vox, "uvw": uvw}
+    for name, clamp in (("repeat", (0, 0, 0)), ("clamp", (1, 1, 1)), ("mixed", (1, 0, 1))):
+        out[f"out_{name}"] = ref.tex3d(vox, dims, np.array(clamp, np.int32), uvw)
+    return out
+
+
+def karras_dup_case(ref):
+    rng = np.random.Generator(np.random.PCG64(W.BASE_SEED + 10))
+    codes = np.sort(rng.integers(0, 64, 5000).astype(np.uint32))
+    same = np.full(3000, 12345, np.uint32)
+    return {"codes_dup": codes, "info_dup": ref.radix_tree(codes),
+            "codes_same": same, "info_same": ref.radix_tree(same)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--full", action="store_true", help="also write full-size digests")
+    args = ap.parse_args()
+    ref = Reference("serial")
+    t0 = time.time()
+    np.savez_compressed(os.path.join(HERE, "tri_small.npz"), **tri_case(ref, 48, 64))
+    np.savez_compressed(os.path.join(HERE, "tet_small.npz"), **tet_case(ref, 6, 64))
+    np.savez_compressed(os.path.join(HERE, "sph_small.npz"), **sph_case(ref, 6000, 64))
+    np.savez_compressed(os.path.join(HERE, "tex3d.npz"), **tex_case(ref))
+    np.savez_compressed(os.path.join(HERE, "karras.npz"), **karras_dup_case(ref))
+    print(f"small fixtures written in {time.time() - t0:.1f}s", flush=True)
+    if not args.full:
+        return
+    digests = {}
+    V, F = W.heightfield(512)
+    tree = ref.build("tri", V, F, keep=True)
+    rays = W.heightfield_rays(1024, 1024)
+    hits, _ = ref.trace(tree, rays)
+    ref.free(tree)
+    digests["heightfield512"] = {
+        "num_tris": int(len(F)), "num_nodes": int(len(tree["nodes"])),
+        "nodes": sha(canonical_tri(tree["nodes"])), "indices": sha(tree["indices"]),
+        "bounds": [float(x) for x in tree["bounds"]], "hits": sha(hits),
+        "hit_fraction": float((hits["prim_id"] != W.NO_PRIM).mean())}
+    print("heightfield512 done", time.time() - t0, flush=True)
+    V, F, S = W.tet_grid(64)
+    tree = ref.build("tet", V, F, keep=True)
+    rays = W.tet_rays(64, 1024, 1024)
+    hits, ext = ref.trace(tree, rays, scalars=S)
+    ref.free(tree)
+    digests["tetgrid64"] = {
+        "num_tets": int(len(F)), "num_nodes": int(len(tree["nodes"])),
+        "nodes": sha(canonical_tet(tree["nodes"])), "indices": sha(tree["indices"]),
+        "bounds": [float(x) for x in tree["bounds"]], "hits": sha(hits),
+        "ext": sha(ext["d"], ext["scalar"], ext["subface"]),
+        "hit_fraction": float((hits["prim_id"] != W.NO_PRIM).mean())}
+    print("tetgrid64 done", time.time() - t0, flush=True)
+    with open(os.path.join(HERE, "full_digests.json"), "w") as f:
+        json.dump(digests, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

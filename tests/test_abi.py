@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every entry point include/srf.h
+declares (no device calls: this runs without a GPU)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "srf.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(srf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ("srf_accel_build_triangles", "srf_accel_build_tetras", "srf_accel_build_spheres",
+                 "srf_trace_closest", "srf_trace_any", "srf_accel_bounds",
+                 "srf_volume_march_fwd", "srf_volume_march_bwd", "srf_lbvh_sort30"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from surface_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libsrf.so is not built: run `python -m surface_amd.build`")
+    lib = _lib.load()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_lib.SIGNATURES) == set(declared_functions())
+    assert lib.srf_abi_version() == 1
+    assert lib.srf_last_error_string() is not None
+
+
+def test_wrapper_fails_loudly_without_library(tmp_path):
+    from surface_amd import _lib
+    saved = _lib._lib
+    try:
+        _lib._lib = None
+        with pytest.raises(FileNotFoundError):
+            _lib.load(str(tmp_path / "missing.so"))
+    finally:
+        _lib._lib = saved

@@ -1,0 +1,488 @@
+"""Pins of the CPU oracle (oracle/srf_oracle.c) -- runs without a GPU.
+
+Each check is chosen so a plausible slip in the restatement (a dropped term,
+a swapped operand, a wrong tie rule or index) fails it:
+  * the reference's only known-answer vector (test/cctest/test-prefix-tree.cc:27-41)
+    and an independent bit-by-bit interleave;
+  * the reference's Karras invariants (test-prefix-tree.cc:418-510,622-625)
+    plus a brute-force top-down radix tree, incl. all-equal keys (:732-826);
+  * stable-sort equality with numpy's stable argsort (stability :127-129);
+  * byte equality with fixtures the reference itself produced
+    (tests/golden/gen_golden.py) and, where oracle/_ref is built, with the
+    reference run live on fresh inputs;
+  * brute-force closest hit over every primitive, analytic sphere/tetra
+    cases, exact trilinear cases, and central finite differences of the
+    volume adjoint.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import (Oracle, Reference, canonical_sph, canonical_tet, canonical_tri,
+                           ref_available)
+from surface_amd import workloads as W
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+# ------------------------------------------------------------------ Morton
+
+def test_morton_known_answer(orc):
+    # test/cctest/test-prefix-tree.cc:27-41
+    code = orc.morton_point(np.array([0.5, 0.5, 0.5], np.float32), np.zeros(3, np.float32),
+                            (1024.0, 1024.0, 1024.0))
+    assert code == (1 << 29) | (1 << 28) | (1 << 27)
+
+
+def _interleave_bits(ix, iy, iz):
+    code = np.zeros_like(ix, dtype=np.uint64)
+    for b in range(10):
+        code |= ((ix >> b) & 1).astype(np.uint64) << np.uint64(3 * b + 2)
+        code |= ((iy >> b) & 1).astype(np.uint64) << np.uint64(3 * b + 1)
+        code |= ((iz >> b) & 1).astype(np.uint64) << np.uint64(3 * b + 0)
+    return code.astype(np.uint32)
+
+
+def test_morton_matches_bit_interleave(orc):
+    rng = np.random.default_rng(1)
+    V = rng.uniform(-3, 5, (3000, 3)).astype(np.float32)
+    F = rng.integers(0, 3000, (5000, 3)).astype(np.uint32)
+    bmin, bmax = orc.scene_bounds("tri", V, F)
+    codes = orc.morton("tri", V, F, bmin, bmax)
+    inv = np.float32(1024) / (bmax - bmin)
+    c = np.float32(1.0 / 3.0) * ((V[F[:, 0]] + V[F[:, 1]]) + V[F[:, 2]])
+    q = ((c - bmin) * inv).astype(np.int64).astype(np.uint32) & 0x3FF
+    assert np.array_equal(codes, _interleave_bits(q[:, 0], q[:, 1], q[:, 2]))
+
+
+def test_morton_special_cases(orc):
+    bmin = np.zeros(3, np.float32)
+    inv = (1024.0, 1024.0, 1024.0)
+    assert orc.morton_point(np.zeros(3, np.float32), bmin, inv) == 0
+    # quantised 1024 on an axis wraps to 0 (PartBy2_30 masks 10 bits)
+    assert orc.morton_point(np.array([1.0, 0, 0], np.float32), bmin, inv) == 0
+    top = orc.morton_point(np.array([1023.5 / 1024, 1023.5 / 1024, 1023.5 / 1024], np.float32),
+                           bmin, inv)
+    assert top == (1 << 30) - 1
+
+
+# -------------------------------------------------------------------- sort
+
+def test_sort_is_stable_sort(orc):
+    rng = np.random.default_rng(2)
+    for n in (1, 7, 1024, 50_000):
+        codes = rng.integers(0, 1 << 30, n).astype(np.uint32)
+        codes[: n // 3] = codes[0]  # long runs of equal keys
+        rng.shuffle(codes)
+        sc, si = orc.sort_keys(codes)
+        order = np.argsort(codes, kind="stable")
+        assert np.array_equal(si, order) and np.array_equal(sc, codes[order])
+
+
+# ------------------------------------------------------------------ Karras
+
+def _delta(c, i, j):
+    n = len(c)
+    if j < 0 or j >= n:
+        return -1
+    a, b = int(c[i]), int(c[j])
+    off = 0
+    if a == b:
+        a, b, off = i, j, 32
+    x = a ^ b
+    return (32 - x.bit_length()) + off
+
+
+def _topdown(codes):
+    """Binary radix tree top down: the split of [l, r] is where the adjacent
+    common prefix is shortest (unique once equal codes are told apart by
+    position). Returns {node index: (child, left_leaf, right_leaf)}."""
+    n = len(codes)
+    adj = np.array([_delta(codes, k, k + 1) for k in range(n - 1)])
+    out = {}
+    todo = [(0, n - 1, 0)]
+    while todo:
+        l, r, idx = todo.pop()
+        g = l + int(np.argmin(adj[l:r]))
+        assert np.sum(adj[l:r] == adj[g]) == 1
+        out[idx] = (g, int(l == g), int(g + 1 == r))
+        if g > l:
+            todo.append((l, g, g))
+        if g + 1 < r:
+            todo.append((g + 1, r, g + 1))
+    return out
+
+
+@pytest.mark.parametrize("kind", ["random", "dups", "same"])
+def test_karras_matches_topdown_bruteforce(orc, kind):
+    rng = np.random.default_rng(3)
+    n = 3000
+    if kind == "random":
+        codes = np.sort(rng.integers(0, 1 << 30, n).astype(np.uint32))
+    elif kind == "dups":
+        codes = np.sort(rng.integers(0, 40, n).astype(np.uint32))
+    else:
+        codes = np.full(n, 777, np.uint32)
+    info = orc.radix_tree(codes)
+    ref = _topdown(codes)
+    assert len(ref) == n - 1
+    for i in range(n - 1):
+        assert tuple(int(x) for x in info[i]) == ref[i], i
+
+
+def _check_karras_invariants(info, n):
+    # CountLeafs32 / VisitAndCountLeafs32: test/cctest/test-prefix-tree.cc:418-510
+    seen_leaf = np.zeros(n, np.int64)
+    seen_node = np.zeros(n - 1, np.int64)
+    for i in range(n - 1):
+        c, ll, rl = (int(x) for x in info[i])
+        (seen_leaf if ll else seen_node)[c] += 1
+        (seen_leaf if rl else seen_node)[c + 1] += 1
+    assert np.all(seen_leaf == 1)
+    assert seen_node[0] == 0 and np.all(seen_node[1:] == 1)
+    depth, stack, reached = 0, [(0, 0)], np.zeros(n, np.int64)
+    while stack:
+        i, d = stack.pop()
+        depth = max(depth, d)
+        c, ll, rl = (int(x) for x in info[i])
+        for child, leaf in ((c, ll), (c + 1, rl)):
+            if leaf:
+                reached[child] += 1
+            else:
+                stack.append((child, d + 1))
+    assert np.all(reached == 1)
+    return depth
+
+
+def test_karras_invariants_and_fixtures(orc):
+    g = gold("karras.npz")
+    for key in ("dup", "same"):
+        codes = g[f"codes_{key}"]
+        info = orc.radix_tree(codes)
+        assert np.array_equal(info, g[f"info_{key}"])
+        _check_karras_invariants(info, len(codes))
+    rng = np.random.default_rng(4)
+    codes = np.sort(rng.integers(0, 1 << 30, 20000).astype(np.uint32))
+    assert _check_karras_invariants(orc.radix_tree(codes), 20000) < 60
+
+
+# -------------------------------------------------------- builds vs fixtures
+
+def _tree_invariants(nodes, indices, n, max_leaf, two_box):
+    assert sorted(indices.tolist()) == list(range(n))
+    covered = []
+    stack = [(0, 0)]
+    while stack:
+        i, depth = stack.pop()
+        nd = nodes[i]
+        if nd["flag"] == 1:
+            cnt, first = (int(x) for x in nd["data"])
+            covered.append((first, cnt))
+            assert cnt <= max_leaf + 1 or depth > 23
+        else:
+            assert depth <= 23
+            stack.append((int(nd["data"][1]), depth + 1))
+            stack.append((int(nd["data"][0]), depth + 1))
+    # DFS visits leaves left to right and they tile [0, n)
+    pos = 0
+    for first, cnt in covered:
+        assert first == pos
+        pos += cnt
+    assert pos == n
+    assert len(nodes) == 2 * len(covered) - 1
+
+
+def test_triangle_build_matches_reference_fixture(orc):
+    g = gold("tri_small.npz")
+    bmin, bmax = orc.scene_bounds("tri", g["V"], g["F"])
+    assert np.array_equal(orc.morton("tri", g["V"], g["F"], bmin, bmax), g["codes"])
+    sc, si = orc.sort_keys(g["codes"])
+    assert np.array_equal(sc, g["sorted_codes"]) and np.array_equal(si, g["sorted_index"])
+    assert np.array_equal(orc.radix_tree(sc), g["nodeinfo"])
+    t = orc.build_tri(g["V"], g["F"])
+    assert canonical_tri(t["nodes"]).tobytes() == g["nodes"].tobytes()
+    assert np.array_equal(t["indices"], g["indices"])
+    assert np.array_equal(t["bounds"], g["bounds"])
+    _tree_invariants(t["nodes"], t["indices"], len(g["F"]), 16, True)
+    hits = orc.trace_tri(t, g["V"], g["F"], g["rays"])
+    assert hits.tobytes() == g["hits"].tobytes()
+
+
+def test_tetra_build_and_trace_match_reference_fixture(orc):
+    g = gold("tet_small.npz")
+    t = orc.build_tet(g["V"], g["F"])
+    assert canonical_tet(t["nodes"]).tobytes() == g["nodes"].tobytes()
+    assert np.array_equal(t["indices"], g["indices"])
+    assert np.array_equal(t["bounds"], g["bounds"])
+    _tree_invariants(t["nodes"], t["indices"], len(g["F"]), 4, False)
+    hits, ext = orc.trace_tet(t, g["V"], g["F"], g["rays"], g["S"])
+    assert hits.tobytes() == g["hits"].tobytes()
+    assert np.array_equal(ext["d"], g["ext_d"])
+    assert np.array_equal(ext["scalar"], g["ext_scalar"])
+    assert np.array_equal(ext["subface"], g["ext_subface"])
+
+
+def test_sphere_build_and_trace_match_reference_fixture(orc):
+    g = gold("sph_small.npz")
+    t = orc.build_sph(g["P"], g["R"])
+    assert canonical_sph(t["nodes"]).tobytes() == g["nodes"].tobytes()
+    assert np.array_equal(t["indices"], g["indices"])
+    _tree_invariants(t["nodes"], t["indices"], len(g["P"]), 16, True)
+    hits = orc.trace_sph(t, g["P"], g["R"], 0.0, g["rays"])
+    assert hits.tobytes() == g["hits"].tobytes()
+
+
+def test_texture_matches_reference_fixture(orc):
+    g = gold("tex3d.npz")
+    for name, clamp in (("repeat", (0, 0, 0)), ("clamp", (1, 1, 1)), ("mixed", (1, 0, 1))):
+        out = orc.tex3d(g["vox"], g["dims"], clamp, g["uvw"])
+        assert out.tobytes() == g[f"out_{name}"].tobytes()
+
+
+# ------------------------------------------------------ live reference runs
+
+needs_ref = pytest.mark.skipif(not ref_available("serial"),
+                               reason="oracle/_ref not built (reference checkout absent)")
+
+
+def _soup(rng, n, spread=1.0):
+    c = rng.uniform(-spread, spread, (n, 3))
+    V = (c[:, None, :] + rng.normal(0, 0.02, (n, 3, 3))).reshape(-1, 3).astype(np.float32)
+    return V, np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+
+
+@needs_ref
+@pytest.mark.parametrize("case", ["soup", "duplicates", "identical", "min1024"])
+def test_triangle_oracle_equals_live_reference(orc, case):
+    ref = Reference("serial")
+    rng = np.random.default_rng(5)
+    if case == "soup":
+        V, F = _soup(rng, 20000)
+    elif case == "duplicates":
+        V, F = _soup(rng, 300)
+        F = np.tile(F, (30, 1))  # each triangle 30 times -> equal Morton codes
+    elif case == "identical":
+        V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0.2]], np.float32)
+        F = np.zeros((5000, 3), np.uint32)
+        F[:] = [0, 1, 2]
+    else:
+        V, F = _soup(rng, 1024)
+    t = orc.build_tri(V, F)
+    r = ref.build("tri", V, F, keep=True)
+    assert canonical_tri(t["nodes"]).tobytes() == canonical_tri(r["nodes"]).tobytes()
+    assert np.array_equal(t["indices"], r["indices"])
+    rays = W.pinhole_rays((0.2, -3.0, 0.5), (0, 0, 0), (0, 0, 1), 50.0, 48, 48)
+    rays["flags"][::3] = 0  # some single-sided rays
+    rays["prev_prim_id"][::5] = np.arange(0, len(rays), 5) % len(F)
+    h1 = orc.trace_tri(t, V, F, rays)
+    h2, _ = ref.trace(r, rays)
+    ref.free(r)
+    assert h1.tobytes() == h2.tobytes()
+
+
+@needs_ref
+def test_tetra_and_sphere_oracle_equal_live_reference(orc):
+    ref = Reference("serial")
+    V, F, S = W.tet_grid(7, seed=11)
+    t = orc.build_tet(V, F)
+    r = ref.build("tet", V, F, keep=True)
+    assert canonical_tet(t["nodes"]).tobytes() == canonical_tet(r["nodes"]).tobytes()
+    rays = W.tet_rays(7, 40, 40)
+    h1, e1 = orc.trace_tet(t, V, F, rays, S)
+    h2, e2 = ref.trace(r, rays, scalars=S)
+    ref.free(r)
+    assert h1.tobytes() == h2.tobytes()
+    assert np.array_equal(e1["d"], e2["d"]) and np.array_equal(e1["scalar"], e2["scalar"])
+    P, R = W.plummer_spheres(30000, seed=12)
+    t = orc.build_sph(P, None, 0.02)
+    r = ref.build("sph", P, R=None, rconst=0.02, keep=True)
+    assert canonical_sph(t["nodes"]).tobytes() == canonical_sph(r["nodes"]).tobytes()
+    rays = W.sphere_rays(40, 40)
+    h1 = orc.trace_sph(t, P, None, 0.02, rays)
+    h2, _ = ref.trace(r, rays)
+    ref.free(r)
+    assert h1.tobytes() == h2.tobytes()
+
+
+@needs_ref
+def test_depth_capped_leaves_equal_live_reference(orc):
+    # a 30-deep caterpillar: leaves below depth 22 are cut (render/render_accel_particle.cc:578)
+    ref = Reference("serial")
+    P, R = W.deep_chain_points()
+    t = orc.build_sph(P, R)
+    r = ref.build("sph", P, R=R)
+    assert canonical_sph(t["nodes"]).tobytes() == canonical_sph(r["nodes"]).tobytes()
+    assert t["nodes"]["data"][t["nodes"]["flag"] == 1][:, 0].max() > 17
+
+
+@needs_ref
+def test_texture_oracle_equals_live_reference(orc):
+    ref = Reference("serial")
+    rng = np.random.default_rng(6)
+    dims = np.array([17, 5, 33], np.int32)
+    vox = rng.normal(size=int(np.prod(dims))).astype(np.float32)
+    uvw = rng.uniform(-3, 3, (20000, 3)).astype(np.float32)
+    for clamp in ((0, 0, 0), (1, 1, 1), (0, 1, 0)):
+        a = orc.tex3d(vox, dims, clamp, uvw)
+        b = ref.tex3d(vox, dims, np.array(clamp, np.int32), uvw)
+        assert a.tobytes() == b.tobytes()
+
+
+# -------------------------------------------------- brute force / analytic
+
+def _mt_all(V, F, ray, eps_scale):
+    """Moller-Trumbore of one ray against every triangle, float32 numpy."""
+    f = np.float32
+    o, d = ray["org"].astype(f), ray["dir"].astype(f)
+    p0, p1, p2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    e1, e2 = p1 - p0, p2 - p0
+    p = np.stack([d[1] * e2[:, 2] - d[2] * e2[:, 1], d[2] * e2[:, 0] - d[0] * e2[:, 2],
+                  d[0] * e2[:, 1] - d[1] * e2[:, 0]], -1)
+    det = (e1[:, 0] * p[:, 0] + e1[:, 1] * p[:, 1]) + e1[:, 2] * p[:, 2]
+    keps = f(np.finfo(f).eps) * f(eps_scale)
+    ok = np.abs(det) >= keps
+    with np.errstate(all="ignore"):
+        inv = f(1.0) / det
+        s = o - p0
+        q = np.stack([s[:, 1] * e1[:, 2] - s[:, 2] * e1[:, 1], s[:, 2] * e1[:, 0] - s[:, 0] * e1[:, 2],
+                      s[:, 0] * e1[:, 1] - s[:, 1] * e1[:, 0]], -1)
+        u = ((s[:, 0] * p[:, 0] + s[:, 1] * p[:, 1]) + s[:, 2] * p[:, 2]) * inv
+        v = ((q[:, 0] * d[0] + q[:, 1] * d[1]) + q[:, 2] * d[2]) * inv
+        t = ((e2[:, 0] * q[:, 0] + e2[:, 1] * q[:, 1]) + e2[:, 2] * q[:, 2]) * inv
+    ok &= (u >= 0) & (u <= 1) & (v >= 0) & (u + v <= 1) & (t >= 0)
+    return np.where(ok, t, np.inf)
+
+
+def test_triangle_trace_equals_bruteforce_closest_hit(orc):
+    V, F = W.heightfield(24, seed=99)
+    t = orc.build_tri(V, F)
+    rays = W.heightfield_rays(40, 40)
+    hits = orc.trace_tri(t, V, F, rays)
+    nhit = 0
+    for k, ray in enumerate(rays):
+        tt = _mt_all(V, F, ray, t["eps_scale"])
+        best = tt.min()
+        if np.isinf(best):
+            assert hits["prim_id"][k] == W.NO_PRIM
+            continue
+        nhit += 1
+        assert hits["t"][k] == np.float32(best)
+        if np.sum(tt == best) == 1:
+            assert hits["prim_id"][k] == np.argmin(tt)
+    assert nhit > 1000
+
+
+def test_sphere_trace_analytic(orc):
+    P = np.zeros((2000, 3), np.float32)
+    P[:, 0] = np.arange(2000, dtype=np.float32) * 3.0
+    t = orc.build_sph(P, None, 1.0)
+    rays = np.zeros(50, W.RAY_DTYPE)
+    rays["org"][:, 0] = np.arange(50) * 3.0
+    rays["org"][:, 1] = -10.0
+    rays["dir"][:, 1] = 1.0
+    rays["prev_prim_id"] = W.NO_PRIM
+    hits = orc.trace_sph(t, P, None, 1.0, rays)
+    assert np.array_equal(hits["prim_id"], np.arange(50))
+    assert np.allclose(hits["t"], 9.0, rtol=0, atol=1e-6)
+    assert np.allclose(hits["u"], 0.5, atol=1e-6)  # normal (0,-1,0): atan2(0, 0)+pi / 2pi
+    assert np.allclose(hits["v"], 1.0, atol=1e-6)  # acos(-1) / pi
+
+
+def test_tetra_trace_analytic(orc):
+    # 2000 disjoint unit right tetras along x; a ray along +z through (x+0.2, 0.2).
+    # RayTetraPluecker's enter/leave labels assume det(v1-v0, v2-v0, v3-v0) < 0.
+    n = 2000
+    base = np.array([[0, 0, 0], [0, 1, 0], [1, 0, 0], [0, 0, 1]], np.float32)
+    V = (base[None] + np.stack([np.arange(n) * 3.0, np.zeros(n), np.zeros(n)], -1)[:, None]
+         ).reshape(-1, 3).astype(np.float32)
+    F = np.arange(4 * n, dtype=np.uint32).reshape(n, 4)
+    t = orc.build_tet(V, F)
+    rays = np.zeros(20, W.RAY_DTYPE)
+    rays["org"] = np.stack([np.arange(20) * 3.0 + 0.2, np.full(20, 0.2), np.full(20, -5.0)], -1)
+    rays["dir"][:, 2] = 1.0
+    rays["prev_prim_id"] = W.NO_PRIM
+    hits, ext = orc.trace_tet(t, V, F, rays, np.ones(len(V), np.float32))
+    assert np.array_equal(hits["prim_id"], np.arange(20))
+    assert np.allclose(hits["t"], 5.0, atol=1e-5)  # enters through the z = 0 face
+    assert np.allclose(ext["t_leave"], 5.6, atol=1e-5)  # leaves through x + y + z = 1
+    assert np.allclose(ext["d"].sum(1), 1.0, atol=1e-5)  # weights of a partition of unity
+    assert np.allclose(ext["scalar"], 1.0, atol=1e-5)
+
+
+def test_trilinear_exact_cases(orc):
+    dims = (6, 5, 4)
+    n = int(np.prod(dims))
+    const = np.full(n, 2.5, np.float32)
+    rng = np.random.default_rng(7)
+    uvw = rng.uniform(0, 1, (500, 3)).astype(np.float32)
+    # Lerp (1 - t) a + t b of equal values is the value to within rounding
+    assert np.allclose(orc.tex3d(const, dims, (1, 1, 1), uvw), 2.5, rtol=0, atol=1e-6)
+    z, y, x = np.meshgrid(np.arange(dims[2]), np.arange(dims[1]), np.arange(dims[0]), indexing="ij")
+    lin = (0.5 * x + 0.25 * y - 0.125 * z).astype(np.float32).ravel()
+    want = 0.5 * uvw[:, 0] * 5 + 0.25 * uvw[:, 1] * 4 - 0.125 * uvw[:, 2] * 3
+    assert np.allclose(orc.tex3d(lin, dims, (1, 1, 1), uvw), want, atol=1e-5)
+    corners = np.array([[i / 5, j / 4, k / 3] for i in (0, 5) for j in (0, 4) for k in (0, 3)],
+                       np.float32)
+    got = orc.tex3d(lin, dims, (1, 1, 1), corners)
+    idx = [(k * 5 + j) * 6 + i for i in (0, 5) for j in (0, 4) for k in (0, 3)]
+    assert np.array_equal(got, lin[idx])
+
+
+# ------------------------------------------------------------------ volume
+
+def _small_volume():
+    dims = (12, 10, 9)
+    rng = np.random.default_rng(8)
+    vox = rng.uniform(0, 1, int(np.prod(dims)))
+    tf = np.clip(rng.uniform(0, 1, (16, 4)), 0, 1)
+    tf[:, 3] *= 0.3
+    rays = W.pinhole_rays((1.7, -1.1, 1.4), (0.5, 0.5, 0.5), (0, 0, 1), 35.0, 12, 12)
+    return dims, vox, tf, rays
+
+
+def test_volume_f32_and_f64_forwards_agree(orc):
+    dims, vox, tf, rays = _small_volume()
+    step = 0.5 / 11
+    a, ns = orc.volume_fwd(vox.astype(np.float32), dims, (1, 1, 1), tf.astype(np.float32), step, rays)
+    b = orc.volume_fwd64(vox.astype(np.float32).astype(np.float64), dims, (1, 1, 1),
+                         tf.astype(np.float32).astype(np.float64), step, rays)
+    assert ns.max() > 10
+    assert np.abs(a - b).max() < 1e-5
+
+
+def test_volume_adjoint_matches_central_differences(orc):
+    dims, vox, tf, rays = _small_volume()
+    step = 0.5 / 11
+    rng = np.random.default_rng(9)
+    g = rng.normal(size=(len(rays), 4))
+    gv, gt = orc.volume_bwd64(vox, dims, (1, 1, 1), tf, step, rays, g)
+
+    def loss(v, t):
+        return float(np.sum(g * orc.volume_fwd64(v, dims, (1, 1, 1), t, step, rays)))
+
+    h = 1e-6
+    for k in np.argsort(-np.abs(gv))[:12]:
+        vp, vm = vox.copy(), vox.copy()
+        vp[k] += h
+        vm[k] -= h
+        fd = (loss(vp, tf) - loss(vm, tf)) / (2 * h)
+        assert abs(fd - gv[k]) <= 1e-6 * max(1.0, abs(fd)) + 1e-4 * abs(fd), (k, fd, gv[k])
+    for k in range(tf.size):
+        i, c = divmod(k, 4)
+        tp, tm = tf.copy(), tf.copy()
+        tp[i, c] += h
+        tm[i, c] -= h
+        fd = (loss(vox, tp) - loss(vox, tm)) / (2 * h)
+        assert abs(fd - gt[i, c]) <= 1e-6 * max(1.0, abs(fd)) + 1e-4 * abs(fd), (i, c, fd)
