@@ -53,7 +53,7 @@ struct DeviceGuard {
   }
 };
 
-size_t node_bytes(Kind k) { return k == kTet ? 32 : 64; }
+size_t node_bytes(Kind) { return 64; }
 size_t prim_bytes(Kind k) { return k == kTri ? 48 : (k == kTet ? 64 : 16); }
 
 void free_buffers(Accel *a);
@@ -307,24 +307,45 @@ srf_status srf_accel_export(const srf_accel *accel, void *nodes_host, uint32_t *
     SRF_HIP(hipMemcpy(nodes_host, a->nodes, 64 * nn, hipMemcpyDeviceToHost));
     return SRF_OK;
   }
-  const size_t gb = node_bytes(a->kind);
-  uint32_t *tmp = new (std::nothrow) uint32_t[gb / 4 * nn];
+  uint32_t *tmp = new (std::nothrow) uint32_t[16 * nn];
   if (!tmp) return set_error(SRF_ERR_OUT_OF_MEMORY, "host allocation failed");
-  hipError_t e = hipMemcpy(tmp, a->nodes, gb * nn, hipMemcpyDeviceToHost);
+  hipError_t e = hipMemcpy(tmp, a->nodes, 64 * nn, hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
     delete[] tmp;
     return hip_error(e, "hipMemcpy");
   }
   uint8_t *dst = reinterpret_cast<uint8_t *>(nodes_host);
-  for (size_t i = 0; i < nn; i++) {
-    const uint32_t *w = tmp + gb / 4 * i;
-    if (a->kind == kTet) {
-      // TetraNode: bmin[3], bmax[3], flag, axis, data[2] (40 B)
-      uint32_t o[10] = {w[0], w[1], w[2], w[4], w[5], w[6], w[3] >> 31,
-                        (w[3] >> 29) & 3u, w[3] & 0x1fffffffu, w[7]};
-      memcpy(dst + 40 * i, o, 40);
-    } else {
-      // ParticleNode: bmin[2][3], bmax[2][3], data[2], short flag, short axis (60 B)
+  if (a->kind == kTet) {
+    // TetraNode: bmin[3], bmax[3], flag, axis, data[2] (40 B); a node's own
+    // box is the one its parent holds for it, the root's is BoundingBox()
+    float rb[6];
+    for (int k = 0; k < 3; k++) {
+      rb[k] = p.root_bmin[k];
+      rb[3 + k] = p.root_bmax[k];
+    }
+    memcpy(dst, rb, 24);
+    for (size_t i = 0; i < nn; i++) {
+      const uint32_t *w = tmp + 16 * i;
+      uint32_t meta[4] = {w[12], w[13], w[14], w[15]};
+      memcpy(dst + 40 * i + 24, meta, 16);
+      if (w[12] == 0) {
+        for (int side = 0; side < 2; side++) {
+          uint32_t c = w[14 + side];
+          if (c >= nn) {
+            delete[] tmp;
+            return set_error(SRF_ERR_INTERNAL, "child index out of range");
+          }
+          float box[6];
+          memcpy(box, w + 3 * side, 12);
+          memcpy(box + 3, w + 6 + 3 * side, 12);
+          memcpy(dst + 40 * (size_t)c, box, 24);
+        }
+      }
+    }
+  } else {
+    // ParticleNode: bmin[2][3], bmax[2][3], data[2], short flag, short axis (60 B)
+    for (size_t i = 0; i < nn; i++) {
+      const uint32_t *w = tmp + 16 * i;
       uint32_t o[15];
       memcpy(o, w, 48);
       o[12] = w[14];

@@ -667,18 +667,11 @@ __device__ __forceinline__ int split_axis(uint32_t code) {
   return (clz32(code) - 2) % 3;
 }
 
-template <int K>
 __device__ __forceinline__ void write_meta(void *nodes, uint32_t pos, bool leaf, int axis,
                                            uint32_t d0, uint32_t d1) {
-  if (K == kTet) {
-    uint32_t *w = reinterpret_cast<uint32_t *>(nodes) + 8 * (size_t)pos;
-    w[3] = (leaf ? 0x80000000u : 0u) | ((uint32_t)axis << 29) | d0;
-    w[7] = d1;
-  } else {
-    // words 12..15 of the 64-byte node: flag, axis, data[0], data[1]
-    uint4 *w = reinterpret_cast<uint4 *>(nodes) + 4 * (size_t)pos + 3;
-    *w = make_uint4(leaf ? 1u : 0u, (uint32_t)axis, d0, d1);
-  }
+  // words 12..15 of the 64-byte node: flag, axis, data[0], data[1]
+  uint4 *w = reinterpret_cast<uint4 *>(nodes) + 4 * (size_t)pos + 3;
+  *w = make_uint4(leaf ? 1u : 0u, (uint32_t)axis, d0, d1);
 }
 
 template <int K>
@@ -699,7 +692,7 @@ __global__ void __launch_bounds__(256) k_emit(uint32_t n, int max_leaf, const ui
   uint32_t posR = steps + 2 * leaf_rank[g + 1] - turns;
   const uint32_t max_nodes = 2 * n - 1;
   if (pos >= max_nodes || posL >= max_nodes || posR >= max_nodes || g + 1 >= n) return;
-  write_meta<K>(o.nodes, pos, false, split_axis(codes[i]), posL, posR);
+  write_meta(o.nodes, pos, false, split_axis(codes[i]), posL, posR);
   o.counter[pos] = 0;
   o.parent_link[posL] = pos << 1;
   o.parent_link[posR] = (pos << 1) | 1u;
@@ -709,14 +702,14 @@ __global__ void __launch_bounds__(256) k_emit(uint32_t n, int max_leaf, const ui
     o.leaf_pos[li] = posL;
     o.leaf_first[li] = l;
     o.leaf_count[li] = g + 1 - l;
-    write_meta<K>(o.nodes, posL, true, 0, g + 1 - l, l);
+    write_meta(o.nodes, posL, true, 0, g + 1 - l, l);
   }
   if ((lb & 2u) || (r - (g + 1) <= (uint32_t)max_leaf) || deep) {
     uint32_t li = leaf_rank[g + 1];
     o.leaf_pos[li] = posR;
     o.leaf_first[li] = g + 1;
     o.leaf_count[li] = r - g;
-    write_meta<K>(o.nodes, posR, true, 0, r - g, g + 1);
+    write_meta(o.nodes, posR, true, 0, r - g, g + 1);
   }
 }
 
@@ -800,21 +793,6 @@ __device__ __forceinline__ void load_box_slot(float *node_words, int slot, Box &
   }
 }
 
-__device__ __forceinline__ void store_box_own(float *w, const Box &b) {
-  // TetNode: (bmin.xyz, w0) (bmax.xyz, w1)
-  for (int k = 0; k < 3; k++) {
-    __hip_atomic_store(w + k, b.mn[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(w + 4 + k, b.mx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-__device__ __forceinline__ void load_box_own(float *w, Box &b) {
-  for (int k = 0; k < 3; k++) {
-    b.mn[k] = __hip_atomic_load(w + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    b.mx[k] = __hip_atomic_load(w + 4 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
 // MergeBoundingBox(left, right), render/render_accel_triangle.cc:804-814
 __device__ __forceinline__ Box merge_lr(const Box &l, const Box &r) {
   Box m;
@@ -843,11 +821,9 @@ __global__ void __launch_bounds__(256) k_leaves(PrimSrc s, const uint32_t *sidx,
   leaf_box_pack<K>(s, sidx, first, count, e, prims, sorted_faces, b);
   uint32_t node = o.leaf_pos[li];
   float *words = reinterpret_cast<float *>(o.nodes);
-  if (K == kTet) {
-    store_box_own(words + 8 * (size_t)node, b);
-  } else {
-    // a leaf TriangleNode/ParticleNode keeps its own box in slot 0 and
-    // leaves slot 1 unused (MakeLeaf32, render/render_accel_triangle.cc:865-890)
+  {
+    // a leaf keeps its own box in slot 0 and leaves slot 1 unused
+    // (MakeLeaf32, render/render_accel_triangle.cc:865-890)
     float4 *w4 = reinterpret_cast<float4 *>(o.nodes) + 4 * (size_t)node;
     w4[0] = make_float4(b.mn[0], b.mn[1], b.mn[2], 0.0f);
     w4[1] = make_float4(0.0f, 0.0f, b.mx[0], b.mx[1]);
@@ -864,26 +840,18 @@ __global__ void __launch_bounds__(256) k_leaves(PrimSrc s, const uint32_t *sidx,
       atomicOr(&params->error, 4u);
       return;
     }
-    if (K != kTet) store_box_slot(words + 16 * (size_t)p, (int)side, b);
+    // every branch keeps both child boxes (TriangleNode / ParticleNode
+    // layout); a tetra branch's own box is its parent's slot, and the
+    // root's own box is BoundingBox()
+    store_box_slot(words + 16 * (size_t)p, (int)side, b);
     __threadfence();
     uint32_t old = atomicAdd(&o.counter[p], 1u);
     if (old == 0) return;  // the sibling's thread carries on
     __threadfence();
     Box other;
-    if (K == kTet) {
-      const uint32_t *pw = reinterpret_cast<const uint32_t *>(o.nodes) + 8 * (size_t)p;
-      uint32_t sib = side ? (pw[3] & 0x1fffffffu) : pw[7];
-      if (sib >= 2 * s.n - 1) {
-        atomicOr(&params->error, 4u);
-        return;
-      }
-      load_box_own(words + 8 * (size_t)sib, other);
-    } else {
-      load_box_slot(words + 16 * (size_t)p, (int)(1u - side), other);
-    }
+    load_box_slot(words + 16 * (size_t)p, (int)(1u - side), other);
     b = side ? merge_lr(other, b) : merge_lr(b, other);
     node = p;
-    if (K == kTet) store_box_own(words + 8 * (size_t)node, b);
   }
   for (int k = 0; k < 3; k++) {
     params->root_bmin[k] = b.mn[k];

@@ -1,15 +1,15 @@
 // srf_internal.h -- shared definitions of the HIP implementation of srf.h.
 //
 // Device layouts (all in HBM, 16-byte aligned):
-//   TriNode / SphNode (64 B)  the reference's TriangleNode word order
-//       (render/render_accel_triangle.h:23-42): bmin[2][3], bmax[2][3],
-//       flag, axis, data[2]; fetched as four 16-byte loads. Spheres use the
-//       same 64 B layout (ParticleNode, render/render_accel_particle.h:24-44,
-//       carries the same fields in 60 B; export converts).
-//   TetNode (32 B)            own box of the node (TetraNode keeps one box,
-//       render/render_accel_tetra.h:24-43) in two 16-byte words:
-//       (bmin.xyz, w0) (bmax.xyz, w1), w0 = flag<<31 | axis<<29 | data[0],
-//       w1 = data[1].
+//   node (64 B, every primitive kind)  the reference's TriangleNode word
+//       order (render/render_accel_triangle.h:23-42): bmin[2][3], bmax[2][3],
+//       flag, axis, data[2], fetched as four 16-byte loads. A branch holds
+//       both child boxes; a leaf holds its own box in slot 0. ParticleNode
+//       (render/render_accel_particle.h:24-44) has the same fields; TetraNode
+//       (render/render_accel_tetra.h:24-43) keeps one box per node, which is
+//       the box its parent holds for it (the root's is BoundingBox()), so a
+//       tetra branch can cull both children before they are pushed.
+//       srf_accel_export converts to the reference layouts.
 //   packed primitives, in sorted (leaf) order:
 //       triangle 48 B  (p0.xyz, id) (e1.xyz, 0) (e2.xyz, 0), e1 = p1 - p0,
 //                      e2 = p2 - p0 -- the same float subtractions

@@ -3,8 +3,25 @@
 // One lane per ray; 8x8 pixel tiles per wavefront when the caller lays rays
 // out that way. The next node lives in a register; the far children wait in
 // a per-lane LDS stack laid out column-major (lane stride), so stack traffic
-// never bank-conflicts. The stack is bounded by the tree's 24 branch levels
-// (25 entries), well inside its 32 slots.
+// never bank-conflicts. A far child is pushed by its parent branch and each
+// root-to-node path holds at most 24 branches (root + depths 0..22), so 24
+// slots always suffice. Each wavefront descends through branches until
+// every active lane holds a leaf (or is done) and only then runs the leaf
+// loops, so leaf work is not interleaved with branch work lane by lane.
+//
+// Work the reference performs and then rejects is skipped when its outcome
+// is certain, never otherwise:
+//  * a triangle whose u, v or t, estimated with the hardware reciprocal, lies
+//    outside the accepting range by a margin (1e-5 relative, ~30x the
+//    estimate's error) is rejected before the exact IEEE division;
+//  * a tetra child box the ray misses (independent of the current hit
+//    distance) is never pushed; the distance test is made at pop time with
+//    the stored entry distance, exactly as the reference tests the popped
+//    node's own box;
+//  * a tetra whose padded float bounding box the ray misses, or which lies
+//    wholly behind the current hit by a margin, is not handed to the FP64
+//    Pluecker test (an intersection it reports lies inside the tetra to
+//    within double rounding, far inside the padding).
 //
 // Visit order, box tests and primitive tests reproduce the reference exactly
 // (hit ids bit-exact, t/u/v bit-exact):
@@ -19,7 +36,9 @@
 namespace srf {
 namespace {
 
-constexpr int kTraceBlock = 256;
+constexpr int kTraceBlock = 128;
+constexpr int kTraceStack = kMaxBranchLevels;  // see above
+constexpr uint32_t kDone = 0xffffffffu;
 
 struct RaySetup {
   float o[3], d[3], inv[3];
@@ -78,10 +97,14 @@ __device__ __forceinline__ bool ray_aabb(const RaySetup &s, float maxT, float bx
 // ---------------------------------------------------------------- triangles
 
 // TriangleIsect (Moller-Trumbore), render/render_accel_triangle.cc:1887-1938,
-// on a packed triangle (p0, e1 = p1 - p0, e2 = p2 - p0).
+// on a packed triangle (p0, e1 = p1 - p0, e2 = p2 - p0). u, v and t are the
+// reference's expressions; the estimates ue/ve/te (numerator x hardware
+// reciprocal, within ~4e-7 relative of the exact values) only reject
+// triangles the exact comparisons would reject as well. kEPS >= 1e-30 keeps
+// det normal, so the reciprocal estimate is accurate.
 __device__ __forceinline__ bool tri_isect(const RaySetup &s, float4 A, float4 B, float4 C,
-                                          bool double_sided, float kEPS, float tin,
-                                          float &tout, float &uout, float &vout) {
+                                          bool double_sided, float kEPS, bool filter,
+                                          float tin, float &tout, float &uout, float &vout) {
   const float e1x = B.x, e1y = B.y, e1z = B.z, e2x = C.x, e2y = C.y, e2z = C.z;
   const float dx = s.d[0], dy = s.d[1], dz = s.d[2];
   float px = dy * e2z - dz * e2y;
@@ -93,14 +116,24 @@ __device__ __forceinline__ bool tri_isect(const RaySetup &s, float4 A, float4 B,
   } else {
     if (det < kEPS) return false;
   }
-  float inv_det = 1.0f / det;
   float sx = s.o[0] - A.x, sy = s.o[1] - A.y, sz = s.o[2] - A.z;
+  float un = (sx * px + sy * py) + sz * pz;
+  const float rc = __builtin_amdgcn_rcpf(det);
+  const float ue = un * rc;
+  if (filter && (ue < -1e-5f || ue > 1.00001f)) return false;
   float qx = sy * e1z - sz * e1y;
   float qy = sz * e1x - sx * e1z;
   float qz = sx * e1y - sy * e1x;
-  float u = ((sx * px + sy * py) + sz * pz) * inv_det;
-  float v = ((qx * dx + qy * dy) + qz * dz) * inv_det;
-  float t = ((e2x * qx + e2y * qy) + e2z * qz) * inv_det;
+  float vn = (qx * dx + qy * dy) + qz * dz;
+  const float ve = vn * rc;
+  if (filter && (ve < -1e-5f || ue + ve > 1.00001f)) return false;
+  float tn = (e2x * qx + e2y * qy) + e2z * qz;
+  const float te = tn * rc;
+  if (filter && (te < -1e-30f || te > tin * 1.00001f)) return false;
+  float inv_det = 1.0f / det;  // correctly rounded: equals (float)(1.0 / det)
+  float u = un * inv_det;
+  float v = vn * inv_det;
+  float t = tn * inv_det;
   if (u < 0.0f || u > 1.0f) return false;
   if (v < 0.0f || u + v > 1.0f) return false;
   if (t < 0.0f || t > tin) return false;
@@ -116,59 +149,69 @@ k_trace_tri(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             const BuildParams *__restrict__ params, const srf_ray *__restrict__ rays, uint32_t n,
             srf_hit *__restrict__ hits, const float *__restrict__ tmax_in,
             uint32_t *__restrict__ occluded, uint32_t max_nodes, uint32_t nprims) {
-  __shared__ uint32_t stack[kStackSize * kTraceBlock];
+  __shared__ uint32_t stack[kTraceStack * kTraceBlock];
   const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
   const RaySetup s = setup_ray(rays, i, flags, prev);
   const bool ds = (flags & SRF_RAY_DOUBLE_SIDED) != 0;
   const float kEPS = FLT_EPSILON * (float)params->eps_scale;
+  const bool filter = kEPS >= 1e-30f;
   const float tlimit = kAny ? (tmax_in ? tmax_in[i] : FLT_MAX) : FLT_MAX;
   float t = kAny ? tlimit : FLT_MAX, u = 0.0f, v = 0.0f;
   uint32_t prim = SRF_NO_PRIM;
   uint32_t *st = stack + threadIdx.x;
   int sp = 0;
   uint32_t cur = 0;
+  uint4 meta;
   for (;;) {
-    if (cur >= max_nodes) break;
-    const float4 *nd = nodes + 4 * (size_t)cur;
-    const uint4 meta = reinterpret_cast<const uint4 *>(nd)[3];
-    if (meta.x == 0) {
+    // descend until this lane holds a leaf (or has nothing left)
+    for (;;) {
+      if (cur >= max_nodes) {
+        cur = kDone;
+        break;
+      }
+      const float4 *nd = nodes + 4 * (size_t)cur;
+      meta = reinterpret_cast<const uint4 *>(nd)[3];
+      if (meta.x != 0) break;
       const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
-      bool h0 = ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
-      bool h1 = ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
+      const bool h0 = ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
+      const bool h1 = ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
       const int nearc = (s.sbits >> meta.y) & 1u;
       const bool hn = nearc ? h1 : h0, hf = nearc ? h0 : h1;
       const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
       if (hn) {
-        if (hf && sp < kStackSize) st[(sp++) * kTraceBlock] = cf;
+        if (hf && sp < kTraceStack) st[(sp++) * kTraceBlock] = cf;
         cur = cn;
-        continue;
-      }
-      if (hf) {
+      } else if (hf) {
         cur = cf;
-        continue;
+      } else if (sp == 0) {
+        cur = kDone;
+        break;
+      } else {
+        cur = st[(--sp) * kTraceBlock];
       }
-    } else {
-      const uint32_t first = meta.w, end = min(meta.w + meta.z, nprims);
-      for (uint32_t q = first; q < end; q++) {
-        const float4 *P = prims + 3 * (size_t)q;
-        const float4 A = P[0];
-        if (__float_as_uint(A.w) == prev) continue;
-        const float4 B = P[1], C = P[2];
-        float tt, uu, vv;
-        if (tri_isect(s, A, B, C, ds, kEPS, t, tt, uu, vv)) {
-          if (kAny) {
-            if (tt < tlimit) {
-              occluded[i] = 1u;
-              return;
-            }
-          } else {
-            t = tt;
-            u = uu;
-            v = vv;
-            prim = __float_as_uint(A.w);
+    }
+    if (cur == kDone) break;
+    // leaf: TestLeafNode, render/render_accel_triangle.cc:2386-2433
+    const uint32_t first = meta.w, end = min(meta.w + meta.z, nprims);
+    for (uint32_t q = first; q < end; q++) {
+      const float4 *P = prims + 3 * (size_t)q;
+      const float4 A = P[0];
+      if (__float_as_uint(A.w) == prev) continue;
+      const float4 B = P[1], C = P[2];
+      float tt, uu, vv;
+      if (tri_isect(s, A, B, C, ds, kEPS, filter, t, tt, uu, vv)) {
+        if (kAny) {
+          if (tt < tlimit) {
+            occluded[i] = 1u;
+            return;
           }
+        } else {
+          t = tt;
+          u = uu;
+          v = vv;
+          prim = __float_as_uint(A.w);
         }
       }
     }
@@ -241,7 +284,7 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             const srf_sphere_prev *__restrict__ prevs, uint32_t n, srf_hit *__restrict__ hits,
             const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded,
             uint32_t max_nodes, uint32_t nprims) {
-  __shared__ uint32_t stack[kStackSize * kTraceBlock];
+  __shared__ uint32_t stack[kTraceStack * kTraceBlock];
   const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -258,42 +301,49 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   uint32_t *st = stack + threadIdx.x;
   int sp = 0;
   uint32_t cur = 0;
+  uint4 meta;
   for (;;) {
-    if (cur >= max_nodes) break;
-    const float4 *nd = nodes + 4 * (size_t)cur;
-    const uint4 meta = reinterpret_cast<const uint4 *>(nd)[3];
-    if (meta.x == 0) {
+    for (;;) {
+      if (cur >= max_nodes) {
+        cur = kDone;
+        break;
+      }
+      const float4 *nd = nodes + 4 * (size_t)cur;
+      meta = reinterpret_cast<const uint4 *>(nd)[3];
+      if (meta.x != 0) break;
       const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
-      bool h0 = ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
-      bool h1 = ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
+      const bool h0 = ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
+      const bool h1 = ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
       const int nearc = (s.sbits >> meta.y) & 1u;
       const bool hn = nearc ? h1 : h0, hf = nearc ? h0 : h1;
       const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
       if (hn) {
-        if (hf && sp < kStackSize) st[(sp++) * kTraceBlock] = cf;
+        if (hf && sp < kTraceStack) st[(sp++) * kTraceBlock] = cf;
         cur = cn;
-        continue;
-      }
-      if (hf) {
+      } else if (hf) {
         cur = cf;
-        continue;
+      } else if (sp == 0) {
+        cur = kDone;
+        break;
+      } else {
+        cur = st[(--sp) * kTraceBlock];
       }
-    } else {
-      const uint32_t first = meta.w, end = min(meta.w + meta.z, nprims);
-      for (uint32_t q = first; q < end; q++) {
-        const float4 c = prims[q];
-        const bool self = (prev != SRF_NO_PRIM) && (sorted_index[q] == prev);
-        float tt;
-        if (sphere_isect(s, c, t, prevT, prevN, self, tt)) {
-          if (kAny) {
-            if (tt < tlimit) {
-              occluded[i] = 1u;
-              return;
-            }
-          } else {
-            t = tt;
-            best = q;
+    }
+    if (cur == kDone) break;
+    const uint32_t first = meta.w, end = min(meta.w + meta.z, nprims);
+    for (uint32_t q = first; q < end; q++) {
+      const float4 c = prims[q];
+      const bool self = (prev != SRF_NO_PRIM) && (sorted_index[q] == prev);
+      float tt;
+      if (sphere_isect(s, c, t, prevT, prevN, self, tt)) {
+        if (kAny) {
+          if (tt < tlimit) {
+            occluded[i] = 1u;
+            return;
           }
+        } else {
+          t = tt;
+          best = q;
         }
       }
     }
@@ -375,15 +425,14 @@ __device__ __forceinline__ void param_dist(D3 o, D3 d, D3 ep, D3 lp, double &te,
 // RayTetraPluecker (Platis & Theoharis), render/render_accel_tetra.cc:243-567.
 // Outputs the entry face, uEnter1, uLeave1 (the call site's argument order
 // puts these into isect.u and isect.v, :2103-2111) and both distances.
-__device__ bool ray_tet(D3 orig, D3 dir, const D3 vert[4], int &enterFace, double &uE1,
-                        double &uL1, double &tE, double &tL) {
+__device__ bool ray_tet(D3 orig, D3 dir, const Pl &plRay, const D3 vert[4], int &enterFace,
+                        double &uE1, double &uL1, double &tE, double &tL) {
   enterFace = -1;
   double uE2 = 0, uL2 = 0;
   uE1 = 0;
   uL1 = 0;
   double uAB = 0, uAC = 0, uDB = 0, uDC = 0, uBC = 0, uAD = 0;
   int signAB = -2, signAC = -2, signDB = -2, signDC = -2, signBC = -2, signAD = -2;
-  const Pl plRay = mk_pl(orig, d3add(orig, dir));
   int nextSign = 0;
   D3 ep = {0, 0, 0}, lp = {0, 0, 0};
 
@@ -640,14 +689,39 @@ __device__ void interpolate_tetra(float d_out[4], F3 pt, const F3 verts[4]) {
   for (int i = 0; i < 4; i++) d_out[i] = d_out[i] / w;
 }
 
+// Entry distance of a box the ray crosses (t-independent part of
+// IntersectRayAABB, render/render_accel_tetra.cc:1934-1978): returns false
+// when tmax <= 0 or tmin > tmax; the caller compares tmin with the current
+// hit distance when the node is popped, as the reference does.
+__device__ __forceinline__ bool box_span(const RaySetup &s, float bx0, float by0, float bz0,
+                                         float bx1, float by1, float bz1, float &tmin_out) {
+  float mnx = s.sign[0] ? bx1 : bx0, mxx = s.sign[0] ? bx0 : bx1;
+  float mny = s.sign[1] ? by1 : by0, mxy = s.sign[1] ? by0 : by1;
+  float mnz = s.sign[2] ? bz1 : bz0, mxz = s.sign[2] ? bz0 : bz1;
+  float tminx = (mnx - s.o[0]) * s.inv[0];
+  float tmaxx = (mxx - s.o[0]) * s.inv[0];
+  float tminy = (mny - s.o[1]) * s.inv[1];
+  float tmaxy = (mxy - s.o[1]) * s.inv[1];
+  float tmin = (tminx > tminy) ? tminx : tminy;
+  float tmax = (tmaxx < tmaxy) ? tmaxx : tmaxy;
+  float tminz = (mnz - s.o[2]) * s.inv[2];
+  float tmaxz = (mxz - s.o[2]) * s.inv[2];
+  tmin = (tmin > tminz) ? tmin : tminz;
+  tmax = (tmax < tmaxz) ? tmax : tmaxz;
+  tmin_out = tmin;
+  return (tmax > 0.0f) && (tmin <= tmax);
+}
+
 template <bool kAny>
 __global__ void __launch_bounds__(kTraceBlock)
 k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
-            const uint4 *__restrict__ sorted_faces, const srf_ray *__restrict__ rays, uint32_t n,
-            const float *__restrict__ scalars, srf_hit *__restrict__ hits,
-            srf_tet_hit_ext *__restrict__ ext, const float *__restrict__ tmax_in,
-            uint32_t *__restrict__ occluded, uint32_t max_nodes, uint32_t nprims) {
-  __shared__ uint32_t stack[kStackSize * kTraceBlock];
+            const uint4 *__restrict__ sorted_faces, const BuildParams *__restrict__ params,
+            const srf_ray *__restrict__ rays, uint32_t n, const float *__restrict__ scalars,
+            srf_hit *__restrict__ hits, srf_tet_hit_ext *__restrict__ ext,
+            const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded,
+            uint32_t max_nodes, uint32_t nprims) {
+  __shared__ uint32_t st_node[kTraceStack * kTraceBlock];
+  __shared__ float st_tmin[kTraceStack * kTraceBlock];
   const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -657,54 +731,123 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   uint32_t best = SRF_NO_PRIM, subface = 0;
   double tleave = 0.0;
   const D3 ro = {s.o[0], s.o[1], s.o[2]}, rd = {s.d[0], s.d[1], s.d[2]};
-  uint32_t *st = stack + threadIdx.x;
+  const Pl plRay = mk_pl(ro, d3add(ro, rd));
+  // prefilter scales: float slab rounding is ~6e-8 of the coordinates
+  // involved, the Pluecker entry distance's error ~1e-16 of them over the
+  // direction component ComputeParametricDist divides by
+  const BuildParams &P = *params;
+  float cext = 0.0f;
+  for (int k = 0; k < 3; k++)
+    cext = fmaxf(cext, fmaxf(fabsf(P.root_bmin[k]), fabsf(P.root_bmax[k])));
+  const float omax = fmaxf(fabsf(s.o[0]), fmaxf(fabsf(s.o[1]), fabsf(s.o[2])));
+  const float pad = 1e-5f * (2.0f * cext + omax);
+  const float dfirst = s.d[0] != 0.0f ? s.d[0] : (s.d[1] != 0.0f ? s.d[1] : s.d[2]);
+  const float tslack = 1e-6f * (cext + omax) / fabsf(dfirst);
+  uint32_t *sn = st_node + threadIdx.x;
+  float *stm = st_tmin + threadIdx.x;
   int sp = 0;
-  uint32_t cur = 0;
-  for (;;) {
-    if (cur >= max_nodes) break;
-    const float4 a = nodes[2 * (size_t)cur], b = nodes[2 * (size_t)cur + 1];
-    const uint32_t w0 = __float_as_uint(a.w), w1 = __float_as_uint(b.w);
-    const bool hit = ray_aabb<false>(s, t, a.x, a.y, a.z, b.x, b.y, b.z);
-    if (hit) {
-      if (!(w0 >> 31)) {
-        const int nearc = (s.sbits >> ((w0 >> 29) & 3u)) & 1u;
-        const uint32_t c0 = w0 & 0x1fffffffu, c1 = w1;
-        if (sp < kStackSize) st[(sp++) * kTraceBlock] = nearc ? c0 : c1;
-        cur = nearc ? c1 : c0;
+  uint32_t cur = kDone;
+  {
+    // the root's own box is tested first (TetraAccel::Traverse pops node 0)
+    float tm;
+    if (box_span(s, P.root_bmin[0], P.root_bmin[1], P.root_bmin[2], P.root_bmax[0],
+                 P.root_bmax[1], P.root_bmax[2], tm) &&
+        tm <= t)
+      cur = 0;
+  }
+  uint4 meta;
+  while (cur != kDone) {
+    // descend: `cur` passed its own box test
+    for (;;) {
+      if (cur >= max_nodes) {
+        cur = kDone;
+        break;
+      }
+      const float4 *nd = nodes + 4 * (size_t)cur;
+      meta = reinterpret_cast<const uint4 *>(nd)[3];
+      if (meta.x != 0) break;
+      const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+      float tm0, tm1;
+      const bool g0 = box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm0);
+      const bool g1 = box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm1);
+      const int nearc = (s.sbits >> meta.y) & 1u;
+      const bool gn = nearc ? g1 : g0, gf = nearc ? g0 : g1;
+      const float tn = nearc ? tm1 : tm0, tf = nearc ? tm0 : tm1;
+      const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+      // the reference pushes far then near and pops near first
+      if (gn && tn <= t) {
+        if (gf && sp < kTraceStack) {
+          sn[sp * kTraceBlock] = cf;
+          stm[sp * kTraceBlock] = tf;
+          sp++;
+        }
+        cur = cn;
         continue;
       }
-      const uint32_t count = w0 & 0x1fffffffu, first = w1;
-      for (uint32_t q = first; q < min(first + count, nprims); q++) {
-        const float4 *P = prims + 4 * (size_t)q;
-        const float4 p0 = P[0];
-        const uint32_t id = __float_as_uint(p0.w);
-        if (id == prev) continue;
-        const float4 p1 = P[1], p2 = P[2], p3 = P[3];
-        const D3 vtx[4] = {{p0.x, p0.y, p0.z}, {p1.x, p1.y, p1.z}, {p2.x, p2.y, p2.z},
-                           {p3.x, p3.y, p3.z}};
-        int ef;
-        double eu, lu, et = 0.0, lt = 0.0;
-        if (ray_tet(ro, rd, vtx, ef, eu, lu, et, lt)) {
-          if ((et >= 0.0) && (et < (double)t)) {
-            if (kAny) {
-              if ((float)et < tlimit) {
-                occluded[i] = 1u;
-                return;
-              }
-            } else {
-              t = (float)et;
-              u = (float)eu;
-              v = (float)lu;
-              best = q;
-              subface = (uint32_t)ef;
-              tleave = lt;
+      if (gf && tf <= t) {
+        cur = cf;
+        continue;
+      }
+      cur = kDone;
+      while (sp > 0) {
+        --sp;
+        if (stm[sp * kTraceBlock] <= t) {
+          cur = sn[sp * kTraceBlock];
+          break;
+        }
+      }
+      if (cur == kDone) break;
+    }
+    if (cur == kDone) break;
+    // leaf: TestLeafNode, render/render_accel_tetra.cc:1980-2122
+    const uint32_t count = meta.z, first = meta.w;
+    for (uint32_t q = first; q < min(first + count, nprims); q++) {
+      const float4 *Pp = prims + 4 * (size_t)q;
+      const float4 p0 = Pp[0];
+      const uint32_t id = __float_as_uint(p0.w);
+      if (id == prev) continue;
+      const float4 p1 = Pp[1], p2 = Pp[2], p3 = Pp[3];
+      {
+        const float bx0 = fminf(fminf(p0.x, p1.x), fminf(p2.x, p3.x)) - pad;
+        const float by0 = fminf(fminf(p0.y, p1.y), fminf(p2.y, p3.y)) - pad;
+        const float bz0 = fminf(fminf(p0.z, p1.z), fminf(p2.z, p3.z)) - pad;
+        const float bx1 = fmaxf(fmaxf(p0.x, p1.x), fmaxf(p2.x, p3.x)) + pad;
+        const float by1 = fmaxf(fmaxf(p0.y, p1.y), fmaxf(p2.y, p3.y)) + pad;
+        const float bz1 = fmaxf(fmaxf(p0.z, p1.z), fmaxf(p2.z, p3.z)) + pad;
+        float tm;
+        if (!box_span(s, bx0, by0, bz0, bx1, by1, bz1, tm)) continue;
+        if (tm > t + 1e-5f * t + tslack) continue;
+      }
+      const D3 vtx[4] = {{p0.x, p0.y, p0.z}, {p1.x, p1.y, p1.z}, {p2.x, p2.y, p2.z},
+                         {p3.x, p3.y, p3.z}};
+      int ef;
+      double eu, lu, et = 0.0, lt = 0.0;
+      if (ray_tet(ro, rd, plRay, vtx, ef, eu, lu, et, lt)) {
+        if ((et >= 0.0) && (et < (double)t)) {
+          if (kAny) {
+            if ((float)et < tlimit) {
+              occluded[i] = 1u;
+              return;
             }
+          } else {
+            t = (float)et;
+            u = (float)eu;
+            v = (float)lu;
+            best = q;
+            subface = (uint32_t)ef;
+            tleave = lt;
           }
         }
       }
     }
-    if (sp == 0) break;
-    cur = st[(--sp) * kTraceBlock];
+    cur = kDone;
+    while (sp > 0) {
+      --sp;
+      if (stm[sp * kTraceBlock] <= t) {
+        cur = sn[sp * kTraceBlock];
+        break;
+      }
+    }
   }
   if (kAny) {
     occluded[i] = 0u;
@@ -713,11 +856,11 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   uint32_t prim = SRF_NO_PRIM;
   srf_tet_hit_ext e = {};
   if (best != SRF_NO_PRIM) {
-    const float4 *P = prims + 4 * (size_t)best;
-    prim = __float_as_uint(P[0].w);
+    const float4 *Pp = prims + 4 * (size_t)best;
+    prim = __float_as_uint(Pp[0].w);
     if (ext) {
       F3 verts[4];
-      for (int k = 0; k < 4; k++) verts[k] = {P[k].x, P[k].y, P[k].z};
+      for (int k = 0; k < 4; k++) verts[k] = {Pp[k].x, Pp[k].y, Pp[k].z};
       F3 pos = {s.o[0] + s.d[0] * t, s.o[1] + s.d[1] * t, s.o[2] + s.d[2] * t};
       interpolate_tetra(e.d, pos, verts);
       e.subface = subface;
@@ -757,8 +900,8 @@ srf_status trace_tet(const Accel *a, const srf_ray *rays, uint64_t n, const floa
   if (n == 0) return SRF_OK;
   hipLaunchKernelGGL(k_trace_tet<false>, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
                      (const float4 *)a->nodes, (const float4 *)a->prims,
-                     (const uint4 *)a->sorted_faces, rays, (uint32_t)n, scalars, hits, ext,
-                     nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
+                     (const uint4 *)a->sorted_faces, a->params, rays, (uint32_t)n, scalars, hits,
+                     ext, nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
   SRF_HIP(hipGetLastError());
   return SRF_OK;
 }
@@ -783,7 +926,7 @@ srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax, uin
                        occluded, max_nodes(a), (uint32_t)a->n);
   else if (a->kind == kTet)
     hipLaunchKernelGGL(k_trace_tet<true>, g, b, 0, s, (const float4 *)a->nodes,
-                       (const float4 *)a->prims, (const uint4 *)a->sorted_faces, rays,
+                       (const float4 *)a->prims, (const uint4 *)a->sorted_faces, a->params, rays,
                        (uint32_t)n, nullptr, nullptr, nullptr, tmax, occluded, max_nodes(a),
                        (uint32_t)a->n);
   else
