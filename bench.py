@@ -168,12 +168,14 @@ def main():
         gv = torch.zeros_like(vox)
         gt = torch.zeros_like(tf)
         grad = torch.ones((n_rays, 4), dtype=torch.float32, device="cuda")
-        vol.march(tf, step_len, vr)
-        fwd = timed_events(torch, lambda: vol.march(tf, step_len, vr), 3)
+        ck = torch.empty((n_rays, vol.checkpoint_slots(step_len)), dtype=torch.float32,
+                         device="cuda")
+        rgba = vol.march(tf, step_len, vr)
+        fwd = timed_events(torch, lambda: vol.march(tf, step_len, vr, rgba=rgba), 3)
 
         def fwd_bwd():
-            vol.march(tf, step_len, vr)
-            vol.march_backward(tf, step_len, vr, grad, gv, gt)
+            vol.march(tf, step_len, vr, ck, rgba)
+            vol.march_backward(tf, step_len, vr, grad, gv, gt, ck)
 
         fwd_bwd()
         fb = timed_events(torch, fwd_bwd, 2)

@@ -258,19 +258,30 @@ srf_status srf_volume_sample(const srf_volume *vol, const float *uvw,
 /* Emission-absorption march through the unit cube, step in texture units,
  * transfer function tf (tf_size RGBA entries); rgba[4*i..] = (C, 1 - T).
  * The reference marches in user GLSL (examples/sparse_volume_render/
- * input.frag:99-141); the march itself is defined by this library. */
+ * input.frag:99-141); the march itself is defined by this library.
+ * checkpoints (optional, NULL = none): num_rays * slots floats, slots from
+ * srf_volume_checkpoint_slots; receives each ray's transmittance at the
+ * start of every 16-sample segment and after the last sample, for
+ * srf_volume_march_bwd. */
 srf_status srf_volume_march_fwd(const srf_volume *vol, const float *tf,
                                 int tf_size, float step, const srf_ray *rays,
-                                uint64_t num_rays, float *rgba, void *stream);
+                                uint64_t num_rays, float *rgba,
+                                float *checkpoints, void *stream);
+/* Checkpoint floats per ray srf_volume_march_fwd writes for this step. */
+srf_status srf_volume_checkpoint_slots(const srf_volume *vol, float step,
+                                       uint32_t *slots_host);
 /* Adjoint of srf_volume_march_fwd for L = sum(grad_rgba * rgba): ACCUMULATES
  * dL/dvoxel into grad_voxels (width*height*depth floats) and dL/dtf into
- * grad_tf (4*tf_size floats). Replays each ray in reverse with segment
- * checkpoints of the transmittance; stores no per-sample state. */
+ * grad_tf (4*tf_size floats). Each ray is replayed in reverse from its
+ * segment checkpoints (T_k = T_{k+1} / (1 - a_k), re-marched forward from
+ * the segment start where 1 - a_k is small); no per-sample state is
+ * stored. checkpoints: the forward pass's (same step, tf and rays), or NULL
+ * to recompute them with one extra forward replay. */
 srf_status srf_volume_march_bwd(const srf_volume *vol, const float *tf,
                                 int tf_size, float step, const srf_ray *rays,
-                                uint64_t num_rays, const float *grad_rgba,
-                                float *grad_voxels, float *grad_tf,
-                                void *stream);
+                                uint64_t num_rays, const float *checkpoints,
+                                const float *grad_rgba, float *grad_voxels,
+                                float *grad_tf, void *stream);
 
 #ifdef __cplusplus
 }

@@ -75,8 +75,10 @@ SIGNATURES = {
     "srf_volume_create": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, C.POINTER(_vp)]),
     "srf_volume_destroy": (_i, [_vp]),
     "srf_volume_sample": (_i, [_vp, _vp, _u64, _vp, _vp]),
-    "srf_volume_march_fwd": (_i, [_vp, _vp, _i, C.c_float, _vp, _u64, _vp, _vp]),
-    "srf_volume_march_bwd": (_i, [_vp, _vp, _i, C.c_float, _vp, _u64, _vp, _vp, _vp, _vp]),
+    "srf_volume_march_fwd": (_i, [_vp, _vp, _i, C.c_float, _vp, _u64, _vp, _vp, _vp]),
+    "srf_volume_checkpoint_slots": (_i, [_vp, C.c_float, C.POINTER(C.c_uint32)]),
+    "srf_volume_march_bwd": (_i, [_vp, _vp, _i, C.c_float, _vp, _u64, _vp, _vp, _vp, _vp,
+                                  _vp]),
 }
 
 _lib = None

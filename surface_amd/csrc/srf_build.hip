@@ -518,8 +518,9 @@ __global__ void __launch_bounds__(256) k_karras(const uint32_t *codes, uint32_t 
   kchild[i] = child;
   kleaf[i] = leafbits;
   kother[i] = (uint32_t)j;
-  if (!(leafbits & 1u)) kparent[child] = i;
-  if (!(leafbits & 2u)) kparent[child + 1] = i;
+  // parent link of an internal child: parent << 1 | (1 for the right child)
+  if (!(leafbits & 1u)) kparent[child] = i << 1;
+  if (!(leafbits & 2u)) kparent[child + 1] = (i << 1) | 1u;
 }
 
 __global__ void __launch_bounds__(256) k_karras_nodeinfo(const uint32_t *codes, uint32_t n,
@@ -544,42 +545,44 @@ __global__ void __launch_bounds__(256) k_classify(uint32_t n, int max_leaf,
                                                  const uint32_t *kother, const uint32_t *kparent,
                                                  uint32_t *cls, uint32_t *leaf_start,
                                                  BuildParams *params) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i + 1 >= n) return;
-  uint32_t j = kother[i];
-  uint32_t l = i < j ? i : j, r = i < j ? j : i;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t steps = 0, turns = 0;
-  bool branch;
-  if (i == 0) {
-    branch = true;
-  } else if (r - l <= (uint32_t)max_leaf) {
-    branch = false;
-  } else {
-    uint32_t c = i;
-    while (c != 0 && steps <= (uint32_t)(kMaxDepth32 + 1)) {
-      uint32_t p = kparent[c];
-      if (p + 1 >= n) {  // malformed parent link: never expected, stop safely
-        atomicOr(&params->error, 8u);
-        break;
+  bool branch = false;
+  uint32_t l = 0, r = 0;
+  if (i + 1 < n) {
+    const uint32_t j = kother[i];
+    l = i < j ? i : j;
+    r = i < j ? j : i;
+    if (i == 0) {
+      branch = true;
+    } else if (r - l > (uint32_t)max_leaf) {
+      uint32_t c = i;
+      while (c != 0 && steps <= (uint32_t)(kMaxDepth32 + 1)) {
+        const uint32_t link = kparent[c];
+        const uint32_t p = link >> 1;
+        if (p + 1 >= n) {  // malformed parent link: never expected, stop safely
+          atomicOr(&params->error, 8u);
+          break;
+        }
+        turns += link & 1u;
+        c = p;
+        steps++;
       }
-      if (c == kchild[p] + 1) turns++;
-      c = p;
-      steps++;
+      branch = (c == 0) && steps <= (uint32_t)(kMaxDepth32 + 1);
     }
-    branch = (c == 0) && steps <= (uint32_t)(kMaxDepth32 + 1);
+    cls[i] = branch ? (1u | (steps << 1) | (turns << 8)) : 0u;
   }
-  if (!branch) {
-    cls[i] = 0;
-    return;
-  }
-  cls[i] = 1u | (steps << 1) | (turns << 8);
-  atomicMax(&params->max_steps, steps);
+  // deepest branch (one atomic per wavefront)
+  uint32_t m = branch ? steps : 0u;
+  for (int off = 32; off >= 1; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(&params->max_steps, m);
+  if (!branch) return;
   // children of this branch sit at depth `steps`
-  uint32_t g = kchild[i];
-  uint32_t lb = kleaf[i];
-  bool deep = steps > (uint32_t)kMaxDepth32;
-  bool left_leaf = (lb & 1u) || (g - l <= (uint32_t)max_leaf) || deep;
-  bool right_leaf = (lb & 2u) || (r - (g + 1) <= (uint32_t)max_leaf) || deep;
+  const uint32_t g = kchild[i];
+  const uint32_t lb = kleaf[i];
+  const bool deep = steps > (uint32_t)kMaxDepth32;
+  const bool left_leaf = (lb & 1u) || (g - l <= (uint32_t)max_leaf) || deep;
+  const bool right_leaf = (lb & 2u) || (r - (g + 1) <= (uint32_t)max_leaf) || deep;
   if (left_leaf) leaf_start[l] = 1;
   if (right_leaf) leaf_start[g + 1] = 1;
 }
@@ -656,7 +659,7 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_downsweep(const uint32_t *i
 struct NodeOut {
   void *nodes;
   uint32_t *parent_link;  // per node: parent pos << 1 | side
-  uint32_t *counter;      // per node: refit arrivals
+  uint32_t *branch_pos;   // per Karras internal node that is a branch: its node
   uint32_t *leaf_pos;     // per leaf: node position
   uint32_t *leaf_first;   // per leaf: first sorted primitive
   uint32_t *leaf_count;
@@ -693,7 +696,7 @@ __global__ void __launch_bounds__(256) k_emit(uint32_t n, int max_leaf, const ui
   const uint32_t max_nodes = 2 * n - 1;
   if (pos >= max_nodes || posL >= max_nodes || posR >= max_nodes || g + 1 >= n) return;
   write_meta(o.nodes, pos, false, split_axis(codes[i]), posL, posR);
-  o.counter[pos] = 0;
+  o.branch_pos[i] = pos;
   o.parent_link[posL] = pos << 1;
   o.parent_link[posR] = (pos << 1) | 1u;
   bool deep = steps > (uint32_t)kMaxDepth32;
@@ -777,19 +780,8 @@ __device__ __forceinline__ void leaf_box_pack(const PrimSrc &s, const uint32_t *
 __device__ __forceinline__ void store_box_slot(float *node_words, int slot, const Box &b) {
   // TriangleNode: bmin[slot][k] at word 3*slot+k, bmax[slot][k] at 6+3*slot+k
   for (int k = 0; k < 3; k++) {
-    __hip_atomic_store(node_words + 3 * slot + k, b.mn[k], __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(node_words + 6 + 3 * slot + k, b.mx[k], __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-__device__ __forceinline__ void load_box_slot(float *node_words, int slot, Box &b) {
-  for (int k = 0; k < 3; k++) {
-    b.mn[k] = __hip_atomic_load(node_words + 3 * slot + k, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
-    b.mx[k] = __hip_atomic_load(node_words + 6 + 3 * slot + k, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
+    node_words[3 * slot + k] = b.mn[k];
+    node_words[6 + 3 * slot + k] = b.mx[k];
   }
 }
 
@@ -829,33 +821,55 @@ __global__ void __launch_bounds__(256) k_leaves(PrimSrc s, const uint32_t *sidx,
     w4[1] = make_float4(0.0f, 0.0f, b.mx[0], b.mx[1]);
     w4[2] = make_float4(b.mx[2], 0.0f, 0.0f, 0.0f);
   }
-  for (int level = 0; node != 0; level++) {
-    if (level > kMaxBranchLevels + 1 || node >= 2 * s.n - 1) {
-      atomicOr(&params->error, 4u);
-      return;
-    }
-    uint32_t link = o.parent_link[node];
-    uint32_t p = link >> 1, side = link & 1u;
-    if (p >= 2 * s.n - 1) {
-      atomicOr(&params->error, 4u);
-      return;
-    }
-    // every branch keeps both child boxes (TriangleNode / ParticleNode
-    // layout); a tetra branch's own box is its parent's slot, and the
-    // root's own box is BoundingBox()
-    store_box_slot(words + 16 * (size_t)p, (int)side, b);
-    __threadfence();
-    uint32_t old = atomicAdd(&o.counter[p], 1u);
-    if (old == 0) return;  // the sibling's thread carries on
-    __threadfence();
-    Box other;
-    load_box_slot(words + 16 * (size_t)p, (int)(1u - side), other);
-    b = side ? merge_lr(other, b) : merge_lr(b, other);
-    node = p;
+  // every branch keeps both child boxes (TriangleNode / ParticleNode layout;
+  // a tetra node's own box is the one its parent keeps for it)
+  uint32_t link = o.parent_link[node];
+  uint32_t p = link >> 1;
+  if (node == 0 || node >= 2 * s.n - 1 || p >= 2 * s.n - 1) {
+    atomicOr(&params->error, 4u);
+    return;
   }
-  for (int k = 0; k < 3; k++) {
-    params->root_bmin[k] = b.mn[k];
-    params->root_bmax[k] = b.mx[k];
+  store_box_slot(words + 16 * (size_t)p, (int)(link & 1u), b);
+}
+
+// One level of the bottom-up refit: every branch whose children already hold
+// their boxes (deeper levels ran in earlier launches) merges them,
+// MergeBoundingBox(left, right) (render/render_accel_triangle.cc:804-814),
+// into its parent's slot; the root's merge is BoundingBox().
+__global__ void __launch_bounds__(256) k_refit_level(uint32_t n, uint32_t level,
+                                                    const uint32_t *cls, NodeOut o,
+                                                    BuildParams *params) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t c = cls[i];
+    if (!(c & 1u) || ((c >> 1) & 127u) != level) continue;
+    const uint32_t pos = o.branch_pos[i];
+    if (pos >= 2 * n - 1) {
+      atomicOr(&params->error, 16u);
+      continue;
+    }
+    const float4 *w4 = reinterpret_cast<const float4 *>(o.nodes) + 4 * (size_t)pos;
+    const float4 a = w4[0], b = w4[1], cc = w4[2];
+    Box l, r;
+    l.mn[0] = a.x; l.mn[1] = a.y; l.mn[2] = a.z;
+    r.mn[0] = a.w; r.mn[1] = b.x; r.mn[2] = b.y;
+    l.mx[0] = b.z; l.mx[1] = b.w; l.mx[2] = cc.x;
+    r.mx[0] = cc.y; r.mx[1] = cc.z; r.mx[2] = cc.w;
+    const Box m = merge_lr(l, r);
+    if (pos == 0) {
+      for (int k = 0; k < 3; k++) {
+        params->root_bmin[k] = m.mn[k];
+        params->root_bmax[k] = m.mx[k];
+      }
+      continue;
+    }
+    const uint32_t link = o.parent_link[pos];
+    const uint32_t p = link >> 1;
+    if (p >= 2 * n - 1) {
+      atomicOr(&params->error, 16u);
+      continue;
+    }
+    store_box_slot(reinterpret_cast<float *>(o.nodes) + 16 * (size_t)p, (int)(link & 1u), m);
   }
 }
 
@@ -963,7 +977,7 @@ srf_status build_lbvh(Ctx *ctx, Kind kind, const void *desc, hipStream_t stream,
   NodeOut o;
   o.nodes = acc->nodes;
   o.parent_link = ar.take<uint32_t>(2 * (size_t)n);
-  o.counter = ar.take<uint32_t>(2 * (size_t)n);
+  o.branch_pos = ar.take<uint32_t>(n);
   o.leaf_pos = ar.take<uint32_t>(n);
   o.leaf_first = ar.take<uint32_t>(n);
   o.leaf_count = ar.take<uint32_t>(n);
@@ -1016,6 +1030,13 @@ srf_status build_lbvh(Ctx *ctx, Kind kind, const void *desc, hipStream_t stream,
                kother, cls, leaf_rank, o);
   SRF_DISPATCH(k_leaves, dim3(nblocks), dim3(256), 0, stream, src, acc->sorted_index, o,
                acc->prims, acc->sorted_faces, acc->params);
+  // bottom-up refit, one level per launch (levels are branch steps 23..0)
+  const uint32_t rblocks = blocks_for(n, 256, 2048);
+  for (int level = kMaxBranchLevels - 1; level >= 0; level--) {
+    hipLaunchKernelGGL(k_refit_level, dim3(rblocks), dim3(256), 0, stream, n, (uint32_t)level,
+                       cls, o, acc->params);
+  }
+  SRF_HIP(hipGetLastError());
 #undef SRF_DISPATCH
   return SRF_OK;
 }

@@ -13,15 +13,20 @@
 //   i0 = min((int)x, N - 2), f = x - i0, rgba_k = Lerp(f, tf[i0], tf[i0 + 1]);
 //   front to back: w = T a_k; C += w rgb_k; T *= (1 - a_k); out = (C, 1 - T).
 // Adjoint (L = sum grad . out), replayed backwards without per-sample
-// state: a forward replay stores the transmittance at the start of every
-// S-sample segment (LDS); segments are then revisited last to first, each
-// re-marched forward from its checkpoint to recover T_k, and walked back
-// with the division-free recursions
+// state. The forward pass can leave each ray's transmittance at the start of
+// every 16-sample segment and after its last sample (without them the
+// adjoint first re-marches the ray once to make them). Each segment is then
+// walked from its end to its start recovering T_k = T_{k+1} / (1 - a_k); a
+// sample with 1 - a_k < 1/4 (where that division would amplify rounding)
+// gets T_k by re-marching forward from the segment's checkpoint instead.
+// With T_k in hand the division-free recursions give the gradients:
 //   R_{k-1} = a_k c_k + (1 - a_k) R_k,  P_{k-1} = (1 - a_k) P_k,
 //   dL/dc_k = g_C T_k a_k,  dL/da_k = T_k (g_C . (c_k - R_k) + g_A P_k).
-// TF gradients gather in LDS per block; voxel gradients go out as float
-// atomics, merged across consecutive samples that share a cell.
+// TF gradients collect in registers while a ray stays in one TF bin, then
+// in LDS per block; voxel gradients go out as float atomics, merged across
+// consecutive samples that share a cell.
 #include <float.h>
+#include <math.h>
 
 #include "srf_internal.h"
 
@@ -38,8 +43,8 @@ namespace {
 
 constexpr int kVolBlock = 128;
 constexpr int kMaxTf = 1024;  // entries; forward keeps the table in LDS
-constexpr int kMaxSeg = 64;
-constexpr int kSegMax = 32;
+constexpr int kSegLen = 16;     // samples per checkpointed segment
+constexpr int kMaxSeg = 64;     // LDS checkpoints per ray when recomputing
 constexpr int kMaxSamples = 65536;
 
 __device__ __forceinline__ int faster_floorf(float x) {
@@ -155,10 +160,26 @@ __global__ void __launch_bounds__(256) k_vol_sample(Volume V, const float *__res
   out[i] = tex_eval(V.vox, g);
 }
 
+// Segment length for a ray of ns samples so that its checkpoints (segment
+// starts plus the final transmittance) fit in `slots`: 16, doubled as needed
+// (only rays longer than the unit-direction bound ever need that).
+__device__ __forceinline__ int seg_len_for(int ns, uint32_t slots) {
+  int s = kSegLen;
+  while ((uint32_t)((ns + s - 1) / s + 1) > slots) s *= 2;
+  return s;
+}
+
+__device__ __forceinline__ int count_samples(float tn, float tfar, float step) {
+  int ns = 0;
+  while (ns < kMaxSamples && (tn + ((float)ns + 0.5f) * step) < tfar) ns++;
+  return ns;
+}
+
 __global__ void __launch_bounds__(kVolBlock) k_vol_fwd(Volume V, const float *__restrict__ tf,
                                                       int N, float step,
                                                       const srf_ray *__restrict__ rays,
-                                                      uint32_t n, float *__restrict__ rgba) {
+                                                      uint32_t n, float *__restrict__ rgba,
+                                                      float *__restrict__ ckpt, uint32_t slots) {
   __shared__ float4 s_tf[kMaxTf];
   for (int k = threadIdx.x; k < N; k += kVolBlock)
     s_tf[k] = reinterpret_cast<const float4 *>(tf)[k];
@@ -168,11 +189,15 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_fwd(Volume V, const float *__
   float o[3], d[3];
   load_ray(rays, i, o, d);
   float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, T = 1.0f;
+  float *ck = ckpt ? ckpt + (size_t)i * slots : nullptr;
   float tn, tfar;
+  int k = 0, seg = kSegLen;
   if (vol_span(o, d, tn, tfar)) {
-    for (int k = 0; k < kMaxSamples; k++) {
+    if (ck) seg = seg_len_for(count_samples(tn, tfar, step), slots);
+    for (; k < kMaxSamples; k++) {
       float t = tn + ((float)k + 0.5f) * step;
       if (!(t < tfar)) break;
+      if (ck && (k % seg) == 0) ck[k / seg] = T;
       Geom g;
       tex_geom(V, o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2], g);
       float s = tex_eval(V.vox, g);
@@ -190,6 +215,8 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_fwd(Volume V, const float *__
       T = T * (1.0f - ca);
     }
   }
+  // transmittance after the last sample closes the last segment
+  if (ck) ck[(k + seg - 1) / seg] = T;
   reinterpret_cast<float4 *>(rgba)[i] = make_float4(C0, C1, C2, 1.0f - T);
 }
 
@@ -218,20 +245,33 @@ __device__ __forceinline__ float sample_alpha(const Volume &V, const float4 *s_t
   return lerpf(f, s_tf[i0].w, s_tf[i0 + 1].w);
 }
 
-// Dynamic LDS: tf[N] (float4), tf gradient [4N], checkpoints [kMaxSeg][B],
-// per-sample transmittance of the chunk being walked back [kSegMax][B].
+struct TfAcc {
+  int bin;
+  float g[8];
+};
+
+__device__ __forceinline__ void flush_tf(TfAcc &acc, float *s_gtf) {
+  if (acc.bin < 0) return;
+  for (int c = 0; c < 8; c++)
+    if (acc.g[c] != 0.0f) atomicAdd(&s_gtf[4 * acc.bin + c], acc.g[c]);
+  acc.bin = -1;
+}
+
+// Dynamic LDS: tf[N] (float4), tf gradient [4N], and, when the forward pass
+// left no checkpoints, [kMaxSeg][block] recomputed ones.
+template <bool kHaveCkpt>
 __global__ void __launch_bounds__(kVolBlock) k_vol_bwd(Volume V, const float *__restrict__ tf,
                                                       int N, float step,
                                                       const srf_ray *__restrict__ rays,
-                                                      uint32_t n,
+                                                      uint32_t n, const float *__restrict__ ckpt,
+                                                      uint32_t slots,
                                                       const float *__restrict__ grad_rgba,
                                                       float *__restrict__ grad_vox,
                                                       float *__restrict__ grad_tf) {
   extern __shared__ float4 smem4[];
   float4 *s_tf = smem4;
   float *s_gtf = reinterpret_cast<float *>(smem4 + N);
-  float *s_ckpt = s_gtf + 4 * N;
-  float *s_tk = s_ckpt + kMaxSeg * kVolBlock;
+  float *s_ck = s_gtf + 4 * N;
   for (int k = threadIdx.x; k < N; k += kVolBlock) {
     s_tf[k] = reinterpret_cast<const float4 *>(tf)[k];
     s_gtf[4 * k] = s_gtf[4 * k + 1] = s_gtf[4 * k + 2] = s_gtf[4 * k + 3] = 0.0f;
@@ -247,92 +287,109 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_bwd(Volume V, const float *__
   if (live) {
     const float4 gr = reinterpret_cast<const float4 *>(grad_rgba)[i];
     // sample count (same float arithmetic as the march)
-    int ns = 0;
-    while (ns < kMaxSamples && (tn + ((float)ns + 0.5f) * step) < tfar) ns++;
-    // segment length: at most kMaxSeg checkpoints per ray
-    const int seg_len = ns <= kMaxSeg ? 1 : (ns + kMaxSeg - 1) / kMaxSeg;
-    // forward replay: transmittance at the start of every segment
-    float T = 1.0f;
-    for (int k = 0; k < ns; k++) {
-      if (k % seg_len == 0) s_ckpt[(k / seg_len) * kVolBlock + threadIdx.x] = T;
-      T = T * (1.0f - sample_alpha(V, s_tf, N, o, d, tn + ((float)k + 0.5f) * step));
+    const int ns = count_samples(tn, tfar, step);
+    const int seg_len = seg_len_for(ns, kHaveCkpt ? slots : (uint32_t)(kMaxSeg + 1));
+    const float *ck;
+    int ck_stride;
+    if (kHaveCkpt) {
+      ck = ckpt + (size_t)i * slots;
+      ck_stride = 1;
+    } else {
+      // one forward replay leaves at most kMaxSeg + 1 checkpoints in LDS
+      float T = 1.0f;
+      float *w = s_ck + threadIdx.x;
+      for (int k = 0; k < ns; k++) {
+        if (k % seg_len == 0) w[(k / seg_len) * kVolBlock] = T;
+        T = T * (1.0f - sample_alpha(V, s_tf, N, o, d, tn + ((float)k + 0.5f) * step));
+      }
+      w[((ns + seg_len - 1) / seg_len) * kVolBlock] = T;
+      ck = w;
+      ck_stride = kVolBlock;
     }
     float R0 = 0.0f, R1 = 0.0f, R2 = 0.0f, P = 1.0f;
-    VoxelAcc acc;
-    acc.live = false;
+    VoxelAcc vacc;
+    vacc.live = false;
+    TfAcc tacc;
+    tacc.bin = -1;
     const int nseg = (ns + seg_len - 1) / seg_len;
     for (int sg = nseg - 1; sg >= 0; sg--) {
       const int k0 = sg * seg_len;
       const int k1 = (k0 + seg_len < ns) ? k0 + seg_len : ns;
-      // walk the segment back in chunks of kSegMax samples; a chunk is
-      // re-marched from the segment checkpoint to recover its T_k
-      for (int c1 = k1; c1 > k0;) {
-        const int c0 = (c1 - kSegMax > k0) ? c1 - kSegMax : k0;
-        float Tk = s_ckpt[sg * kVolBlock + threadIdx.x];
-        for (int k = k0; k < c0; k++)
-          Tk = Tk * (1.0f - sample_alpha(V, s_tf, N, o, d, tn + ((float)k + 0.5f) * step));
-        for (int k = c0; k < c1; k++) {
-          s_tk[(k - c0) * kVolBlock + threadIdx.x] = Tk;
-          Tk = Tk * (1.0f - sample_alpha(V, s_tf, N, o, d, tn + ((float)k + 0.5f) * step));
+      float Tnext = ck[(sg + 1) * ck_stride];  // T after sample k1 - 1
+      const float Tstart = ck[sg * ck_stride];
+      for (int k = k1 - 1; k >= k0; k--) {
+        float t = tn + ((float)k + 0.5f) * step;
+        Geom g;
+        tex_geom(V, o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2], g);
+        float s = tex_eval(V.vox, g);
+        int i0;
+        float f;
+        bool inside;
+        tf_index(s, N, i0, f, inside);
+        const float4 ta = s_tf[i0], tb = s_tf[i0 + 1];
+        float c0r = lerpf(f, ta.x, tb.x), c1g = lerpf(f, ta.y, tb.y), c2b = lerpf(f, ta.z, tb.z);
+        float a = lerpf(f, ta.w, tb.w);
+        // transmittance in front of sample k
+        float Tk;
+        if (k == k0) {
+          Tk = Tstart;
+        } else if (1.0f - a >= 0.25f) {
+          Tk = Tnext / (1.0f - a);
+        } else {
+          Tk = Tstart;
+          for (int j = k0; j < k; j++)
+            Tk = Tk * (1.0f - sample_alpha(V, s_tf, N, o, d, tn + ((float)j + 0.5f) * step));
         }
-        for (int k = c1 - 1; k >= c0; k--) {
-          const float Tkk = s_tk[(k - c0) * kVolBlock + threadIdx.x];
-          float t = tn + ((float)k + 0.5f) * step;
-          Geom g;
-          tex_geom(V, o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2], g);
-          float s = tex_eval(V.vox, g);
-          int i0;
-          float f;
-          bool inside;
-          tf_index(s, N, i0, f, inside);
-          const float4 ta = s_tf[i0], tb = s_tf[i0 + 1];
-          float c0r = lerpf(f, ta.x, tb.x), c1g = lerpf(f, ta.y, tb.y), c2b = lerpf(f, ta.z, tb.z);
-          float a = lerpf(f, ta.w, tb.w);
-          float wa = Tkk * a;
-          float g0 = gr.x * wa, g1 = gr.y * wa, g2 = gr.z * wa;
-          float ga = Tkk * ((gr.x * (c0r - R0) + gr.y * (c1g - R1) + gr.z * (c2b - R2)) + gr.w * P);
-          float fo = 1.0f - f;
-          atomicAdd(&s_gtf[4 * i0 + 0], fo * g0);
-          atomicAdd(&s_gtf[4 * i0 + 1], fo * g1);
-          atomicAdd(&s_gtf[4 * i0 + 2], fo * g2);
-          atomicAdd(&s_gtf[4 * i0 + 3], fo * ga);
-          atomicAdd(&s_gtf[4 * i0 + 4], f * g0);
-          atomicAdd(&s_gtf[4 * i0 + 5], f * g1);
-          atomicAdd(&s_gtf[4 * i0 + 6], f * g2);
-          atomicAdd(&s_gtf[4 * i0 + 7], f * ga);
-          if (inside) {
-            float ds = ((g0 * (tb.x - ta.x) + g1 * (tb.y - ta.y)) +
-                        (g2 * (tb.z - ta.z) + ga * (tb.w - ta.w))) *
-                       (float)(N - 1);
-            if (ds != 0.0f) {
-              if (acc.live && (acc.a[0] != g.a[0] || acc.a[7] != g.a[7])) flush_vox(acc, grad_vox);
-              if (!acc.live) {
-                for (int c = 0; c < 8; c++) {
-                  acc.a[c] = g.a[c];
-                  acc.g[c] = 0.0f;
-                }
-                acc.live = true;
+        Tnext = Tk;
+        float wa = Tk * a;
+        float g0 = gr.x * wa, g1 = gr.y * wa, g2 = gr.z * wa;
+        float ga = Tk * ((gr.x * (c0r - R0) + gr.y * (c1g - R1) + gr.z * (c2b - R2)) + gr.w * P);
+        float fo = 1.0f - f;
+        if (tacc.bin != i0) {
+          flush_tf(tacc, s_gtf);
+          tacc.bin = i0;
+          for (int c = 0; c < 8; c++) tacc.g[c] = 0.0f;
+        }
+        tacc.g[0] += fo * g0;
+        tacc.g[1] += fo * g1;
+        tacc.g[2] += fo * g2;
+        tacc.g[3] += fo * ga;
+        tacc.g[4] += f * g0;
+        tacc.g[5] += f * g1;
+        tacc.g[6] += f * g2;
+        tacc.g[7] += f * ga;
+        if (inside) {
+          float ds = ((g0 * (tb.x - ta.x) + g1 * (tb.y - ta.y)) +
+                      (g2 * (tb.z - ta.z) + ga * (tb.w - ta.w))) *
+                     (float)(N - 1);
+          if (ds != 0.0f) {
+            if (vacc.live && (vacc.a[0] != g.a[0] || vacc.a[7] != g.a[7])) flush_vox(vacc, grad_vox);
+            if (!vacc.live) {
+              for (int c = 0; c < 8; c++) {
+                vacc.a[c] = g.a[c];
+                vacc.g[c] = 0.0f;
               }
-              const float ux = 1.0f - g.dx, uy = 1.0f - g.dy, uz = 1.0f - g.dz;
-              acc.g[0] += ds * (ux * uy * uz);
-              acc.g[1] += ds * (g.dx * uy * uz);
-              acc.g[2] += ds * (ux * g.dy * uz);
-              acc.g[3] += ds * (g.dx * g.dy * uz);
-              acc.g[4] += ds * (ux * uy * g.dz);
-              acc.g[5] += ds * (g.dx * uy * g.dz);
-              acc.g[6] += ds * (ux * g.dy * g.dz);
-              acc.g[7] += ds * (g.dx * g.dy * g.dz);
+              vacc.live = true;
             }
+            const float ux = 1.0f - g.dx, uy = 1.0f - g.dy, uz = 1.0f - g.dz;
+            vacc.g[0] += ds * (ux * uy * uz);
+            vacc.g[1] += ds * (g.dx * uy * uz);
+            vacc.g[2] += ds * (ux * g.dy * uz);
+            vacc.g[3] += ds * (g.dx * g.dy * uz);
+            vacc.g[4] += ds * (ux * uy * g.dz);
+            vacc.g[5] += ds * (g.dx * uy * g.dz);
+            vacc.g[6] += ds * (ux * g.dy * g.dz);
+            vacc.g[7] += ds * (g.dx * g.dy * g.dz);
           }
-          R0 = a * c0r + (1.0f - a) * R0;
-          R1 = a * c1g + (1.0f - a) * R1;
-          R2 = a * c2b + (1.0f - a) * R2;
-          P = (1.0f - a) * P;
         }
-        c1 = c0;
+        R0 = a * c0r + (1.0f - a) * R0;
+        R1 = a * c1g + (1.0f - a) * R1;
+        R2 = a * c2b + (1.0f - a) * R2;
+        P = (1.0f - a) * P;
       }
     }
-    flush_vox(acc, grad_vox);
+    flush_vox(vacc, grad_vox);
+    flush_tf(tacc, s_gtf);
   }
   __syncthreads();
   for (int k = threadIdx.x; k < 4 * N; k += kVolBlock) {
@@ -400,8 +457,24 @@ static srf_status check_march(const Volume *v, const float *tf, int tf_size, flo
   return SRF_OK;
 }
 
+static uint32_t checkpoint_slots(float step) {
+  // longest unit-cube chord sqrt(3) for unit directions, plus rounding slack
+  double max_samples = 1.7320508075688772 / (double)step + 2.0;
+  double segs = ceil(max_samples / kSegLen);
+  if (segs > 1e8) segs = 1e8;
+  return (uint32_t)segs + 1;
+}
+
+srf_status srf_volume_checkpoint_slots(const srf_volume *vol, float step, uint32_t *slots) {
+  if (!vol || !slots) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (!(step > 0.0f)) return set_error(SRF_ERR_INVALID_ARG, "step must be > 0");
+  *slots = checkpoint_slots(step);
+  return SRF_OK;
+}
+
 srf_status srf_volume_march_fwd(const srf_volume *vol, const float *tf, int tf_size, float step,
-                                const srf_ray *rays, uint64_t n, float *rgba, void *stream) {
+                                const srf_ray *rays, uint64_t n, float *rgba, float *ckpt,
+                                void *stream) {
   const Volume *v = reinterpret_cast<const Volume *>(vol);
   srf_status st = check_march(v, tf, tf_size, step, rays, n);
   if (st != SRF_OK) return st;
@@ -409,25 +482,34 @@ srf_status srf_volume_march_fwd(const srf_volume *vol, const float *tf, int tf_s
   if (n == 0) return SRF_OK;
   hipLaunchKernelGGL(k_vol_fwd, dim3(vblocks(n, kVolBlock)), dim3(kVolBlock), 0,
                      reinterpret_cast<hipStream_t>(stream), *v, tf, tf_size, step, rays,
-                     (uint32_t)n, rgba);
+                     (uint32_t)n, rgba, ckpt, checkpoint_slots(step));
   SRF_HIP(hipGetLastError());
   return SRF_OK;
 }
 
 srf_status srf_volume_march_bwd(const srf_volume *vol, const float *tf, int tf_size, float step,
-                                const srf_ray *rays, uint64_t n, const float *grad_rgba,
-                                float *grad_voxels, float *grad_tf, void *stream) {
+                                const srf_ray *rays, uint64_t n, const float *ckpt,
+                                const float *grad_rgba, float *grad_voxels, float *grad_tf,
+                                void *stream) {
   const Volume *v = reinterpret_cast<const Volume *>(vol);
   srf_status st = check_march(v, tf, tf_size, step, rays, n);
   if (st != SRF_OK) return st;
   if (n && (!grad_rgba || !grad_voxels || !grad_tf))
     return set_error(SRF_ERR_INVALID_ARG, "null gradient buffer");
   if (n == 0) return SRF_OK;
-  const size_t lds = 16 * (size_t)tf_size + 16 * (size_t)tf_size +
-                    4 * (size_t)(kMaxSeg + kSegMax) * kVolBlock;
-  hipLaunchKernelGGL(k_vol_bwd, dim3(vblocks(n, kVolBlock)), dim3(kVolBlock), lds,
-                     reinterpret_cast<hipStream_t>(stream), *v, tf, tf_size, step, rays,
-                     (uint32_t)n, grad_rgba, grad_voxels, grad_tf);
+  const uint32_t slots = checkpoint_slots(step);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const size_t lds_tf = 32 * (size_t)tf_size;
+  if (ckpt) {
+    hipLaunchKernelGGL(k_vol_bwd<true>, dim3(vblocks(n, kVolBlock)), dim3(kVolBlock), lds_tf, s,
+                       *v, tf, tf_size, step, rays, (uint32_t)n, ckpt, slots, grad_rgba,
+                       grad_voxels, grad_tf);
+  } else {
+    const size_t lds = lds_tf + 4 * (size_t)(kMaxSeg + 1) * kVolBlock;
+    hipLaunchKernelGGL(k_vol_bwd<false>, dim3(vblocks(n, kVolBlock)), dim3(kVolBlock), lds, s,
+                       *v, tf, tf_size, step, rays, (uint32_t)n, nullptr, 0u, grad_rgba,
+                       grad_voxels, grad_tf);
+  }
   SRF_HIP(hipGetLastError());
   return SRF_OK;
 }

@@ -46,18 +46,29 @@ class Volume:
         check(self.lib.srf_volume_sample(self.handle, _ptr(uvw), n, _ptr(out), _stream(stream)))
         return out
 
-    def march(self, tf, step, rays, stream=None):
+    def checkpoint_slots(self, step):
+        slots = C.c_uint32()
+        check(self.lib.srf_volume_checkpoint_slots(self.handle, float(step), C.byref(slots)))
+        return slots.value
+
+    def march(self, tf, step, rays, checkpoints=None, rgba=None, stream=None):
+        """Forward march; with checkpoints ([n, checkpoint_slots(step)] float32)
+        the per-segment transmittance is kept for march_backward."""
         n = rays.shape[0]
-        rgba = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+        if rgba is None:
+            rgba = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
         check(self.lib.srf_volume_march_fwd(self.handle, _ptr(tf), tf.shape[0], float(step),
-                                            _ptr(rays), n, _ptr(rgba), _stream(stream)))
+                                            _ptr(rays), n, _ptr(rgba), _ptr(checkpoints),
+                                            _stream(stream)))
         return rgba
 
-    def march_backward(self, tf, step, rays, grad_rgba, grad_vox, grad_tf, stream=None):
+    def march_backward(self, tf, step, rays, grad_rgba, grad_vox, grad_tf, checkpoints=None,
+                       stream=None):
+        """Accumulates dL/dvoxels and dL/dtf (the adjoint kernel)."""
         n = rays.shape[0]
         check(self.lib.srf_volume_march_bwd(self.handle, _ptr(tf), tf.shape[0], float(step),
-                                            _ptr(rays), n, _ptr(grad_rgba), _ptr(grad_vox),
-                                            _ptr(grad_tf), _stream(stream)))
+                                            _ptr(rays), n, _ptr(checkpoints), _ptr(grad_rgba),
+                                            _ptr(grad_vox), _ptr(grad_tf), _stream(stream)))
 
 
 class VolumeRender(torch.autograd.Function):
@@ -65,24 +76,34 @@ class VolumeRender(torch.autograd.Function):
     kernel (no per-sample state kept between forward and backward)."""
 
     @staticmethod
-    def forward(ctx, voxels, tf, dims, wrap, step, rays):
+    def forward(ctx, voxels, tf, dims, wrap, step, rays, keep_checkpoints=True):
         vol = Volume(voxels.detach().contiguous(), dims, wrap)
         tf_c = tf.detach().contiguous()
-        rgba = vol.march(tf_c, step, rays)
-        ctx.save_for_backward(voxels, tf, rays)
+        ck = None
+        if keep_checkpoints:
+            ck = torch.empty((rays.shape[0], vol.checkpoint_slots(step)), dtype=torch.float32,
+                             device=rays.device)
+        rgba = vol.march(tf_c, step, rays, ck)
+        if ck is None:
+            ctx.save_for_backward(voxels, tf, rays)
+        else:
+            ctx.save_for_backward(voxels, tf, rays, ck)
         ctx.dims, ctx.wrap, ctx.step = dims, wrap, step
         return rgba
 
     @staticmethod
     def backward(ctx, grad_rgba):
-        voxels, tf, rays = ctx.saved_tensors
+        saved = ctx.saved_tensors
+        voxels, tf, rays = saved[:3]
+        ck = saved[3] if len(saved) > 3 else None
         vol = Volume(voxels.detach().contiguous(), ctx.dims, ctx.wrap)
         gv = torch.zeros_like(voxels)
         gt = torch.zeros_like(tf)
         vol.march_backward(tf.detach().contiguous(), ctx.step, rays,
-                           grad_rgba.contiguous(), gv, gt)
-        return gv, gt, None, None, None, None
+                           grad_rgba.contiguous(), gv, gt, ck)
+        return gv, gt, None, None, None, None, None
 
 
-def render(voxels, tf, dims, step, rays, wrap=(CLAMP_TO_EDGE,) * 3):
-    return VolumeRender.apply(voxels, tf, tuple(dims), tuple(wrap), float(step), rays)
+def render(voxels, tf, dims, step, rays, wrap=(CLAMP_TO_EDGE,) * 3, keep_checkpoints=True):
+    return VolumeRender.apply(voxels, tf, tuple(dims), tuple(wrap), float(step), rays,
+                              keep_checkpoints)

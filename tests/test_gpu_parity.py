@@ -327,6 +327,7 @@ def test_texture_sample_bit_exact(S, torch):
 def _vol_case(dim=24):
     vox = W.gaussian_volume(dim, n_gauss=8)
     tf = W.transfer_function(32, max_alpha=0.2)
+    tf[20:24, 3] = 0.9  # a few near-opaque bins exercise the re-march fallback
     rays = W.volume_rays(40, 40)
     return vox, tf, rays
 
@@ -344,8 +345,9 @@ def test_volume_forward_matches_oracle(S, orc, torch):
     assert np.abs(got - want).max() <= 1e-4
 
 
-def test_volume_adjoint_matches_f64_adjoint_and_fd(S, orc, torch):
-    say("volume adjoint")
+@pytest.mark.parametrize("keep", [True, False], ids=["checkpointed", "replayed"])
+def test_volume_adjoint_matches_f64_adjoint_and_fd(S, orc, torch, keep):
+    say(f"volume adjoint keep_checkpoints={keep}")
     from surface_amd.volume import render
     vox, tf, rays = _vol_case()
     dims = (24, 24, 24)
@@ -356,7 +358,7 @@ def test_volume_adjoint_matches_f64_adjoint_and_fd(S, orc, torch):
                                   step, rays, g.astype(np.float64))
     vt = torch.from_numpy(vox).cuda().requires_grad_(True)
     tt = torch.from_numpy(tf).cuda().requires_grad_(True)
-    out = render(vt, tt, dims, step, dev(S, rays))
+    out = render(vt, tt, dims, step, dev(S, rays), keep_checkpoints=keep)
     (out * torch.from_numpy(g).cuda()).sum().backward()
     gv = vt.grad.cpu().numpy()
     gt = tt.grad.cpu().numpy()
