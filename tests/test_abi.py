@@ -47,3 +47,22 @@ def test_wrapper_fails_loudly_without_library(tmp_path):
             _lib.load(str(tmp_path / "missing.so"))
     finally:
         _lib._lib = saved
+
+
+def test_cpp_host_classes_compile_against_the_header(tmp_path):
+    """include/srf_accel.hpp (the C++ mirror of TriangleAccel/TetraAccel/
+    ParticleAccel) is plain C++17 over srf.h: it compiles with the host
+    compiler alone, with no HIP headers."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "srf_accel.hpp"\n'
+                   "int main() { srf::Context c(0); srf::TriangleAccel t(c);"
+                   " srf::TetraAccel k(c); srf::ParticleAccel p(c); return c.ok() ? 0 : 1; }\n")
+    inc = os.path.join(ROOT, "include")
+    r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", inc,
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -1,0 +1,18 @@
+"""The C++ drop-in classes (include/srf_accel.hpp) on the GPU, through the
+C ABI, checked bit for bit against the oracle by tests/cpp/test_accel_cpp."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(os.path.dirname(__file__), "cpp", "test_accel_cpp")
+
+
+def test_cpp_host_classes_bit_exact():
+    assert os.path.exists(EXE), "tests/cpp/test_accel_cpp not built (run __graft_entry__.build())"
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    assert "PASS" in r.stdout

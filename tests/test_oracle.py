@@ -486,3 +486,47 @@ def test_volume_adjoint_matches_central_differences(orc):
         tm[i, c] -= h
         fd = (loss(vox, tp) - loss(vox, tm)) / (2 * h)
         assert abs(fd - gt[i, c]) <= 1e-6 * max(1.0, abs(fd)) + 1e-4 * abs(fd), (i, c, fd)
+
+
+def _ramp_case(tf_size=256):
+    """A volume whose value is its x texture coordinate (x-linear ramp), a TF
+    whose alpha and red are linear in the value, and +x rays: the march has a
+    closed form computed here independently of the oracle's code."""
+    dims = (12, 7, 5)
+    x = np.arange(dims[0]) / (dims[0] - 1)
+    vox = np.broadcast_to(x[None, None, :], dims[::-1]).astype(np.float32).ravel()
+    s = np.arange(tf_size) / (tf_size - 1)
+    tf = np.stack([s, np.ones_like(s), np.full_like(s, 0.5), 0.02 * s], 1).astype(np.float32)
+    rays = np.zeros(4, W.RAY_DTYPE)
+    rays["org"] = [(-0.5, 0.3, 0.6), (-0.25, 0.8, 0.1), (-0.5, 1.5, 0.5), (0.5, 0.5, 2.0)]
+    rays["dir"] = [(1, 0, 0), (1, 0, 0), (1, 0, 0), (1, 0, 0)]
+    rays["prev_prim_id"] = W.NO_PRIM
+    return dims, vox, tf, rays
+
+
+def test_volume_forward_closed_form(orc):
+    dims, vox, tf, rays = _ramp_case()
+    step = 0.5 / (dims[0] - 1)
+    want = np.zeros((len(rays), 4))
+    nwant = np.zeros(len(rays), int)
+    for i, r in enumerate(rays):
+        o = r["org"].astype(np.float64)
+        if not (0 <= o[1] <= 1 and 0 <= o[2] <= 1):
+            continue  # misses the unit cube
+        tn, tf_ = -o[0], 1.0 - o[0]
+        C, T = np.zeros(3), 1.0
+        k = 0
+        while tn + (k + 0.5) * step < tf_:
+            sx = o[0] + tn + (k + 0.5) * step  # sample's x = its value
+            a = 0.02 * sx
+            C += T * a * np.array([sx, 1.0, 0.5])
+            T *= 1 - a
+            k += 1
+        want[i, :3], want[i, 3], nwant[i] = C, 1 - T, k
+    got, ns = orc.volume_fwd(vox, dims, (1, 1, 1), tf, step, rays)
+    got64 = orc.volume_fwd64(vox.astype(np.float64), dims, (1, 1, 1), tf.astype(np.float64),
+                             step, rays)
+    assert list(ns) == list(nwant) and nwant[0] == 22
+    assert np.abs(got - want).max() < 2e-6
+    # f64 path: float sample geometry and float32-rounded voxels/TF entries
+    assert np.abs(got64 - want).max() < 1e-7
