@@ -41,6 +41,10 @@ TET_TESTS_PER_RAY = 60.3673
 TRI_NODES_PER_RAY = 64.3038    # heightfield(512), heightfield_rays(1024, 1024)
 TRI_TESTS_PER_RAY = 94.4435
 TET_BYTES_PER_RAY = 40 + 40 * TET_NODES_PER_RAY + 80 * TET_TESTS_PER_RAY
+# HBM bytes per k_trace_tet launch on the bench rays: rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (KiB/dispatch), each scaled by the same counter on a 1 GiB copy
+# (fetch x 2.000, write x 1.000); profiles/traffic_k_trace_tet_tet64_k_trace_tri_heightfield512.md
+TET_TRACE_HBM_BYTES = (2.000 * 54177.9 + 1.000 * 49152.0) * 1024
 TRI_BYTES_PER_RAY = 40 + 64 * TRI_NODES_PER_RAY + 48 * TRI_TESTS_PER_RAY
 
 
@@ -88,6 +92,28 @@ def cpu_baseline(V, F, S, rays):
             "build_mprims_per_s": round(len(F) / build_s / 1e6, 3)}
 
 
+def traffic_probe(torch, S, W, build, trace):
+    """Kernels for the FETCH_SIZE / WRITE_SIZE counter passes: the tetra build
+    and trace (each once after a warm-up), the heightfield trace, and a plain
+    1 GiB device-to-device copy whose counters give the scale factor."""
+    build()
+    trace()
+    torch.cuda.synchronize()
+    build()
+    trace()
+    Vh, Fh = W.heightfield(512)
+    tri = S.TriangleAccel()
+    tri.Build32(S.device_tensor(Vh), S.device_tensor(Fh))
+    rh = S.device_tensor(W.heightfield_rays(1024, 1024))
+    hh = torch.empty((len(rh), 4), dtype=torch.float32, device="cuda")
+    tri.Traverse(rh, hh)
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"traffic probe done: copy of {src.numel() * 4} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +121,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="tetra step only")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="one launch of each traversal kernel plus a 1 GiB device copy "
+                         "(the calibration for HBM-counter runs); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); rays shard by "
@@ -126,6 +155,10 @@ def main():
     def step():
         build()
         trace()
+
+    if args.traffic_probe:
+        traffic_probe(torch, S, W, build, trace)
+        return
 
     log(f"warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -208,7 +241,7 @@ def main():
         "wall_s_timed": round(wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(tet_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(tet_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_trace_tet",
+                     "traffic": round(TET_TRACE_HBM_BYTES), "kernel": "k_trace_tet",
                      "bytes_per_launch": round(TET_BYTES_PER_RAY * n_rays)},
         "cpu_baseline": cpu,
         "extras": extras,

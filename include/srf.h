@@ -167,12 +167,19 @@ srf_status srf_accel_build_tetras(srf_ctx *ctx, const srf_tet_desc *desc,
 srf_status srf_accel_build_spheres(srf_ctx *ctx, const srf_sphere_desc *desc,
                                    void *stream, srf_accel **out);
 /* Many meshes in one segmented build: each mesh gets the tree
- * TriangleAccel::Build32 would give it alone. The reference builds draw
- * calls one by one (gles/gles_accel_builder.cc:34-117). out[i] receives
- * mesh i's accelerator. */
+ * TriangleAccel::Build32 would give it alone, but every stage is one launch
+ * over all meshes. The reference builds draw calls one by one
+ * (gles/gles_accel_builder.cc:34-117, one Build32 per AddTriangleData).
+ * out[i] receives mesh i's accelerator; on failure none is returned. */
 srf_status srf_accel_build_triangles_batch(srf_ctx *ctx, uint32_t num_meshes,
                                            const srf_tri_desc *descs_host,
                                            void *stream, srf_accel **out);
+/* The batch rebuilt in place (accels from srf_accel_build_triangles_batch or
+ * srf_accel_build_triangles, all distinct), as AccelBuilder's per-frame
+ * rebuild of dynamic draws does one by one (gles/gles_accel_builder.cc:442-455). */
+srf_status srf_accel_rebuild_triangles_batch(srf_ctx *ctx, uint32_t num_meshes,
+                                             srf_accel *const *accels,
+                                             const srf_tri_desc *descs_host, void *stream);
 
 /* Build32 called again on an existing accelerator object (the reference
  * rebuilds in place: nodes_.clear(), render/render_accel_triangle.cc:1552;

@@ -14,6 +14,9 @@
 //                 schedule(dynamic,1) over image rows as in
 //                 gles/gles_raytrace_engine.cc:662-665
 //   texture       FilterTexture3DFloat           render/render_texture.cc:946-1058
+//   scene         RenderElement (Update: world box, inverse matrices),
+//                 RenderGraph::Build / Trace    gles/gles_render_graph.cc:19-314,
+//                 BVHTree::BuildTree            render/render_bvh_tree.cc:319-502
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -29,6 +32,8 @@
 #include "render_accel_triangle.h"
 #include "render_prefix_tree_util.h"
 #include "render_texture.h"
+#include "render_bvh_tree.h"
+#include "gles_render_graph.h"
 
 using namespace lsgl::render;
 
@@ -288,6 +293,104 @@ void ref_tex3d_sample(float *vox, const int dims[3], const int clamp[3],
     FilterTexture3DFloat(rgba, &tex, uvw[3 * i], uvw[3 * i + 1], uvw[3 * i + 2],
                          clamp[0] != 0, clamp[1] != 0, clamp[2] != 0);
     out[i] = rgba[0];
+  }
+}
+
+
+// ---- two-level scenes (RenderGraph over triangle accelerators) ----
+struct RefScene {
+  lsgl::RenderGraph graph;
+};
+
+// Builds a RenderGraph of n triangle elements (accels from ref_build_tri,
+// T[i] = 16 doubles, row-major T_[4][4]); writes each element's inverse
+// matrices and world box as RenderElement::Update computes them, and the
+// top-level BVHTree over those boxes (nodes: 6 doubles box + child[2] +
+// axis + isLeaf + nodeID each, see BVHNode).
+void *ref_scene_build(int n, void **accels, const double *T, double *invT_out,
+                      double *invT33_out, double *xbox_out, double *tlas_box,
+                      int *tlas_meta, int *tlas_count, int tlas_cap) {
+  RefScene *s = new RefScene;
+  BVHTree tree;
+  for (int i = 0; i < n; i++) {
+    RefAccel *a = (RefAccel *)accels[i];
+    double bmin[3], bmax[3], M[4][4];
+    a->tri_accel.BoundingBox(bmin, bmax);
+    memcpy(M, T + 16 * i, sizeof(M));
+    lsgl::RenderElement e(lsgl::AccelBuilder::PRIMITIVE_TRIANGLES, GL_TRIANGLES,
+                          reinterpret_cast<unsigned char *>(&a->tri_accel), bmin, bmax, M, 0,
+                          NULL, FragmentState(), lsgl::ShadingState());
+    s->graph.AddRenderElement(e);
+    memcpy(invT_out + 16 * i, e.invT_, sizeof(M));
+    memcpy(invT33_out + 16 * i, e.invT33_, sizeof(M));
+    memcpy(xbox_out + 6 * i, e.xbmin_, 3 * sizeof(double));
+    memcpy(xbox_out + 6 * i + 3, e.xbmax_, 3 * sizeof(double));
+    BVHData d;
+    memcpy(d.bmin, e.xbmin_, sizeof(d.bmin));
+    memcpy(d.bmax, e.xbmax_, sizeof(d.bmax));
+    d.nodeID = (unsigned)i;
+    tree.AddNode(d);
+  }
+  {
+    QuietStdout q;
+    s->graph.Build();
+    tree.BuildTree();
+  }
+  const std::vector<BVHNode> &tn = tree.GetNodesTree();
+  *tlas_count = (int)tn.size();
+  for (size_t k = 0; k < tn.size() && (int)k < tlas_cap; k++) {
+    memcpy(tlas_box + 6 * k, tn[k].bmin, 3 * sizeof(double));
+    memcpy(tlas_box + 6 * k + 3, tn[k].bmax, 3 * sizeof(double));
+    tlas_meta[5 * k + 0] = tn[k].child[0];
+    tlas_meta[5 * k + 1] = tn[k].child[1];
+    tlas_meta[5 * k + 2] = tn[k].axis;
+    tlas_meta[5 * k + 3] = tn[k].isLeaf ? 1 : 0;
+    tlas_meta[5 * k + 4] = (int)tn[k].nodeID;
+  }
+  return s;
+}
+
+void ref_scene_free(void *h) { delete (RefScene *)h; }
+
+// RenderGraph::Trace per ray (OpenMP dynamic over rows of `width`): hit t is
+// the world distance, u, v, prim_id the element's; inst = element index.
+void ref_scene_trace(void *h, const void *rays_in, long long n, long long width,
+                     void *hits_out, unsigned *inst_out, int threads) {
+  RefScene *s = (RefScene *)h;
+  const RayIn *rays = (const RayIn *)rays_in;
+  HitOut *hits = (HitOut *)hits_out;
+  if (width <= 0) width = n;
+  long long rows = (n + width - 1) / width;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (long long y = 0; y < rows; y++) {
+    for (long long x = 0; x < width; x++) {
+      long long i = y * width + x;
+      if (i >= n) break;
+      Ray ray = make_ray(rays[i]);
+      ray.prev_prim_id = (unsigned int)(-1);
+      ray.prev_node = NULL;
+      Intersection isect;
+      isect.t = 0.0f;
+      isect.u = isect.v = 0.0f;
+      isect.prim_id = (unsigned int)(-1);
+      isect.renderElement = NULL;
+      bool hit = s->graph.Trace(isect, ray);
+      if (hit) {
+        hits[i].t = isect.t;
+        hits[i].u = isect.u;
+        hits[i].v = isect.v;
+        hits[i].prim_id = isect.prim_id;
+        inst_out[i] = (unsigned)((const lsgl::RenderElement *)isect.renderElement)->nodeID_;
+      } else {
+        hits[i].t = 3.402823466e+38f;
+        hits[i].u = hits[i].v = 0.0f;
+        hits[i].prim_id = (unsigned int)(-1);
+        inst_out[i] = (unsigned int)(-1);
+      }
+    }
   }
 }
 

@@ -60,6 +60,7 @@ SIGNATURES = {
     "srf_accel_build_tetras": (_i, [_vp, C.POINTER(TetDesc), _vp, C.POINTER(_vp)]),
     "srf_accel_build_spheres": (_i, [_vp, C.POINTER(SphereDesc), _vp, C.POINTER(_vp)]),
     "srf_accel_build_triangles_batch": (_i, [_vp, C.c_uint32, _vp, _vp, _vp]),
+    "srf_accel_rebuild_triangles_batch": (_i, [_vp, C.c_uint32, _vp, _vp, _vp]),
     "srf_accel_rebuild": (_i, [_vp, _vp, _vp, _vp]),
     "srf_accel_bounds": (_i, [_vp, _vp, _vp]),
     "srf_accel_get_info": (_i, [_vp, C.POINTER(AccelInfo)]),

@@ -183,6 +183,32 @@ class TriangleAccel(_Accel):
         return self._build(self.lib.srf_accel_build_triangles, d, stream)
 
 
+def build_triangles_batch(meshes, accels=None, ctx=None, stream=None):
+    """Builds many triangle meshes with one launch per build stage
+    (srf_accel_build_triangles_batch); meshes is a list of (vertices, faces)
+    GPU tensors. With `accels` (from an earlier call) they are rebuilt in
+    place. Returns one TriangleAccel per mesh, each the tree Build32 of that
+    mesh alone gives."""
+    ctx = ctx or default_context()
+    lib = ctx.lib
+    n = len(meshes)
+    descs = (TriDesc * n)()
+    for i, (V, F) in enumerate(meshes):
+        descs[i] = TriDesc(V.data_ptr(), V.numel() // 3, F.data_ptr(), F.numel() // 3)
+    if accels is None:
+        out = (C.c_void_p * n)()
+        check(lib.srf_accel_build_triangles_batch(ctx.handle, n, descs, _stream(stream), out))
+        accels = []
+        for i in range(n):
+            a = TriangleAccel(ctx)
+            a.handle = C.c_void_p(out[i])
+            accels.append(a)
+        return accels
+    hs = (C.c_void_p * n)(*[a.handle.value for a in accels])
+    check(lib.srf_accel_rebuild_triangles_batch(ctx.handle, n, hs, descs, _stream(stream)))
+    return accels
+
+
 class TetraAccel(_Accel):
     """TetraAccel::Build32 / Traverse (+ InterpolateTetra) / BoundingBox."""
     kind = _lib.PRIM_TETRAHEDRONS

@@ -6,6 +6,8 @@
 
 #include <new>
 
+#include <vector>
+
 #include "srf_internal.h"
 
 namespace srf {
@@ -37,6 +39,18 @@ srf_status ensure_scratch(Ctx *ctx, size_t bytes, hipStream_t stream) {
   size_t sz = bytes + bytes / 4 + (1u << 20);
   SRF_HIP(hipMalloc(&ctx->scratch, sz));
   ctx->scratch_bytes = sz;
+  return SRF_OK;
+}
+
+srf_status ensure_host_stage(Ctx *ctx, size_t bytes) {
+  SRF_HIP(hipEventSynchronize(ctx->stage_done));
+  if (ctx->host_stage_bytes >= bytes) return SRF_OK;
+  if (ctx->host_stage) SRF_HIP(hipHostFree(ctx->host_stage));
+  ctx->host_stage = nullptr;
+  ctx->host_stage_bytes = 0;
+  size_t sz = bytes + 4096;
+  SRF_HIP(hipHostMalloc(&ctx->host_stage, sz, hipHostMallocDefault));
+  ctx->host_stage_bytes = sz;
   return SRF_OK;
 }
 
@@ -176,6 +190,16 @@ srf_status srf_ctx_create(int device, srf_ctx **out) {
   c->device = device;
   c->scratch = nullptr;
   c->scratch_bytes = 0;
+  c->host_stage = nullptr;
+  c->host_stage_bytes = 0;
+  {
+    DeviceGuard g(device);
+    hipError_t e = hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      delete c;
+      return hip_error(e, "hipEventCreate");
+    }
+  }
   *out = reinterpret_cast<srf_ctx *>(c);
   return SRF_OK;
 }
@@ -187,6 +211,8 @@ srf_status srf_ctx_destroy(srf_ctx *ctx) {
     DeviceGuard g(c->device);
     (void)hipDeviceSynchronize();
     if (c->scratch) (void)hipFree(c->scratch);
+    if (c->host_stage) (void)hipHostFree(c->host_stage);
+    (void)hipEventDestroy(c->stage_done);
   }
   delete c;
   return SRF_OK;
@@ -213,15 +239,77 @@ srf_status srf_accel_build_spheres(srf_ctx *ctx, const srf_sphere_desc *d, void 
 srf_status srf_accel_build_triangles_batch(srf_ctx *ctx, uint32_t num_meshes,
                                            const srf_tri_desc *descs, void *stream,
                                            srf_accel **out) {
-  if (!descs || !out) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !descs || !out || num_meshes == 0)
+    return set_error(SRF_ERR_INVALID_ARG, "null argument or empty batch");
   for (uint32_t m = 0; m < num_meshes; m++) out[m] = nullptr;
   for (uint32_t m = 0; m < num_meshes; m++) {
-    srf_status st = srf_accel_build_triangles(ctx, &descs[m], stream, &out[m]);
-    if (st != SRF_OK) {
-      for (uint32_t k = 0; k < m; k++) srf_accel_destroy(out[k]), out[k] = nullptr;
-      return st;
-    }
+    if (!descs[m].vertices || !descs[m].faces)
+      return set_error(SRF_ERR_INVALID_ARG, "mesh %u: null arrays", m);
+    srf_status cs = check_count(descs[m].num_faces);
+    if (cs != SRF_OK) return cs;
   }
+  DeviceGuard g(c->device);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  std::vector<Accel *> accs(num_meshes, nullptr);
+  std::vector<const void *> dp(num_meshes);
+  srf_status st = SRF_OK;
+  for (uint32_t m = 0; m < num_meshes && st == SRF_OK; m++) {
+    st = alloc_accel(c, kTri, descs[m].num_faces, descs[m].num_vertices, &accs[m]);
+    dp[m] = &descs[m];
+  }
+  if (st == SRF_OK) st = build_lbvh_segments(c, kTri, dp.data(), accs.data(), num_meshes, s);
+  for (uint32_t m = 0; m < num_meshes && st == SRF_OK; m++) {
+    hipError_t e = hipEventRecord(accs[m]->built, s);
+    if (e != hipSuccess) st = hip_error(e, "hipEventRecord");
+  }
+  if (st != SRF_OK) {
+    (void)hipStreamSynchronize(s);
+    for (Accel *a : accs)
+      if (a) free_accel(a);
+    return st;
+  }
+  for (uint32_t m = 0; m < num_meshes; m++) out[m] = reinterpret_cast<srf_accel *>(accs[m]);
+  return SRF_OK;
+}
+
+srf_status srf_accel_rebuild_triangles_batch(srf_ctx *ctx, uint32_t num_meshes,
+                                             srf_accel *const *accels,
+                                             const srf_tri_desc *descs, void *stream) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !descs || !accels || num_meshes == 0)
+    return set_error(SRF_ERR_INVALID_ARG, "null argument or empty batch");
+  std::vector<Accel *> accs(num_meshes);
+  std::vector<const void *> dp(num_meshes);
+  for (uint32_t m = 0; m < num_meshes; m++) {
+    Accel *a = reinterpret_cast<Accel *>(accels[m]);
+    if (!a || a->kind != kTri || a->device != c->device)
+      return set_error(SRF_ERR_INVALID_ARG, "accelerator %u: null, not triangles, or on "
+                                            "another device", m);
+    if (!descs[m].vertices || !descs[m].faces)
+      return set_error(SRF_ERR_INVALID_ARG, "mesh %u: null arrays", m);
+    srf_status cs = check_count(descs[m].num_faces);
+    if (cs != SRF_OK) return cs;
+    for (uint32_t k = 0; k < m; k++)
+      if (accs[k] == a) return set_error(SRF_ERR_INVALID_ARG, "accelerator %u repeated", m);
+    accs[m] = a;
+    dp[m] = &descs[m];
+  }
+  DeviceGuard g(c->device);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  for (uint32_t m = 0; m < num_meshes; m++) {
+    Accel *a = accs[m];
+    if (descs[m].num_faces > a->capacity) {
+      SRF_HIP(hipEventSynchronize(a->built));
+      free_buffers(a);
+      SRF_HIP(alloc_buffers(a, descs[m].num_faces));
+    }
+    a->n = descs[m].num_faces;
+    a->nv = descs[m].num_vertices;
+  }
+  srf_status st = build_lbvh_segments(c, kTri, dp.data(), accs.data(), num_meshes, s);
+  if (st != SRF_OK) return st;
+  for (uint32_t m = 0; m < num_meshes; m++) SRF_HIP(hipEventRecord(accs[m]->built, s));
   return SRF_OK;
 }
 

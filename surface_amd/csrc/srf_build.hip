@@ -21,9 +21,16 @@
 //            preorder position pos = ancestors + 2*leaves_left - right_turns,
 //            which is where the reference's serial builder puts it, and the
 //            leaf records of its leaf children.
-//   leaves   one thread per leaf: leaf box + packed primitives, then an
-//            atomic bottom-up refit (second arrival merges and climbs).
+//   leaves   one thread per leaf: leaf box + packed primitives, written to
+//            the leaf's own slot and its parent's slot for it.
+//   refit    one launch per branch level (23..0): each branch merges its two
+//            child boxes into its parent's slot.
+// A batch of meshes runs the same launches over all meshes at once (see the
+// segment notes below).
 #include <float.h>
+#include <string.h>
+
+#include <vector>
 
 #include "srf_internal.h"
 
@@ -112,6 +119,61 @@ struct PrimSrc {
   uint32_t n;
 };
 
+// ---- segments: many meshes in one launch sequence ----
+// A build covers nseg meshes laid end to end in one primitive index space
+// (codes, sort keys, Karras links, per-leaf and per-node scratch). Mesh m
+// owns primitives [prim_off, prim_off + n), sort tiles [tile_off, ...),
+// bounds/Morton blocks [bblk_off, ...) / [mblk_off, ...), global leaves
+// [leaf_base, ...), and per-node scratch [2 prim_off, 2 prim_off + 2n - 1).
+// Every kernel maps its global index to (mesh, local index) and works in
+// local indices, so each mesh gets exactly the tree a build of it alone
+// gives. M = false is the one-mesh build: the segment travels in the kernel
+// arguments and all offsets are zero.
+struct Seg {
+  PrimSrc src;
+  void *nodes;
+  void *prims;
+  uint32_t *sorted_index;
+  uint32_t *sorted_faces;
+  BuildParams *params;
+};
+
+struct SegView {
+  Seg one;                      // M = false
+  const Seg *seg;               // M = true: nseg entries
+  const uint32_t *prim_off;     // nseg + 1 prefix arrays
+  const uint32_t *tile_off;
+  const uint32_t *bblk_off;
+  const uint32_t *mblk_off;
+  const uint32_t *leaf_base;    // written by k_seg_leaves
+  uint32_t nseg;
+};
+
+// largest m with off[m] <= x (off[0] = 0, off ascending)
+template <bool M>
+__device__ __forceinline__ uint32_t seg_find(const uint32_t *off, uint32_t nseg, uint32_t x) {
+  if (!M) return 0;
+  uint32_t lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <bool M>
+__device__ __forceinline__ Seg seg_get(const SegView &v, uint32_t m) {
+  return M ? v.seg[m] : v.one;
+}
+
+template <bool M>
+__device__ __forceinline__ uint32_t seg_off(const uint32_t *off, uint32_t m) {
+  return M ? off[m] : 0u;
+}
+
 // Per-primitive contribution to the scene box:
 //  tri  ComputeBoundingBoxOMP, epsScale 0  render/render_accel_triangle.cc:711-797,1430-1443
 //  tet  kEPS = 16*FLT_EPSILON               render/render_accel_tetra.cc:971-1055
@@ -152,14 +214,20 @@ __device__ __forceinline__ void box_wave_reduce(Box &b) {
   }
 }
 
-template <int K>
-__global__ void __launch_bounds__(256) k_bounds(PrimSrc s, Box *partials,
-                                               BuildParams *params,
+template <int K, bool M>
+__global__ void __launch_bounds__(256) k_bounds(SegView v, Box *partials,
                                                uint32_t *clear, uint32_t nclear) {
+  // blocks [bblk_off[m], bblk_off[m+1]) reduce mesh m
+  const uint32_t m = seg_find<M>(v.bblk_off, v.nseg, blockIdx.x);
+  const uint32_t b0 = seg_off<M>(v.bblk_off, m);
+  const uint32_t nblk = M ? v.bblk_off[m + 1] - b0 : gridDim.x;
+  const Seg sg = seg_get<M>(v, m);
+  const PrimSrc &s = sg.src;
+  BuildParams *params = sg.params;
   Box b;
   box_init(b);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n;
-       i += gridDim.x * blockDim.x)
+  for (uint32_t i = (blockIdx.x - b0) * blockDim.x + threadIdx.x; i < s.n;
+       i += nblk * blockDim.x)
     scene_add<K>(b, s, i);
   box_wave_reduce(b);
   __shared__ Box wbox[4];
@@ -173,7 +241,7 @@ __global__ void __launch_bounds__(256) k_bounds(PrimSrc s, Box *partials,
     partials[blockIdx.x] = r;
     __threadfence();
     uint32_t t = atomicAdd(&params->ticket, 1u);
-    last = (t == gridDim.x - 1);
+    last = (t == nblk - 1);
   }
   // clear the histogram / tile counters the next stages accumulate into
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nclear;
@@ -183,7 +251,7 @@ __global__ void __launch_bounds__(256) k_bounds(PrimSrc s, Box *partials,
   if (!last) return;
   __threadfence();
   box_init(b);
-  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+  for (uint32_t i = b0 + threadIdx.x; i < b0 + nblk; i += blockDim.x) {
     Box o;
     for (int k = 0; k < 3; k++) {
       o.mn[k] = __hip_atomic_load(&partials[i].mn[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -260,15 +328,22 @@ __device__ __forceinline__ uint32_t prim_code(const PrimSrc &s, uint32_t i,
   }
 }
 
-template <int K>
-__global__ void __launch_bounds__(256) k_morton(PrimSrc s, const BuildParams *params,
-                                               uint32_t *codes, uint32_t *hist) {
+template <int K, bool M>
+__global__ void __launch_bounds__(256) k_morton(SegView v, uint32_t *codes, uint32_t *hist) {
+  // blocks [mblk_off[m], mblk_off[m+1]) code mesh m; histograms per mesh
+  const uint32_t m = seg_find<M>(v.mblk_off, v.nseg, blockIdx.x);
+  const uint32_t b0 = seg_off<M>(v.mblk_off, m);
+  const uint32_t nblk = M ? v.mblk_off[m + 1] - b0 : gridDim.x;
+  const Seg sg = seg_get<M>(v, m);
+  const PrimSrc &s = sg.src;
+  codes += seg_off<M>(v.prim_off, m);
+  hist += 1024 * (size_t)m;
   __shared__ uint32_t h[4][256];
   for (int i = threadIdx.x; i < 4 * 256; i += blockDim.x) (&h[0][0])[i] = 0;
   __syncthreads();
-  const BuildParams p = *params;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n;
-       i += gridDim.x * blockDim.x) {
+  const BuildParams p = *sg.params;
+  for (uint32_t i = (blockIdx.x - b0) * blockDim.x + threadIdx.x; i < s.n;
+       i += nblk * blockDim.x) {
     uint32_t c = prim_code<K>(s, i, p);
     codes[i] = c;
     atomicAdd(&h[0][c & 255], 1u);
@@ -340,12 +415,17 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t *tmp
   return base + x - v;
 }
 
+// One LSD pass over 8 bits of the keys. With M, tiles [tile_off[m],
+// tile_off[m+1]) sort mesh m's keys among themselves (digit totals from that
+// mesh's histogram, look-back stopping at its first tile); values are local
+// primitive indices.
+template <bool M>
 __global__ void __launch_bounds__(kSortBlock)
 k_onesweep(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
-           uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, uint32_t n,
-           int shift, const uint32_t *__restrict__ hist /* this pass */,
+           uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, uint32_t n_all,
+           int shift, int pass, const uint32_t *__restrict__ hist_all,
            uint32_t *status /* [tiles][256] */, uint32_t *tile_counter,
-           uint32_t *error) {
+           uint32_t *error, SegView v) {
   __shared__ uint32_t s_keys[kSortTile];
   __shared__ uint32_t s_vals[kSortTile];
   __shared__ uint32_t s_whist[4][256];   // per-wave digit counts, then offsets
@@ -359,7 +439,17 @@ k_onesweep(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ va
   for (int i = threadIdx.x; i < 4 * 256; i += kSortBlock) (&s_whist[0][0])[i] = 0;
   __syncthreads();
   const uint32_t tile = s_tile;
-  const uint32_t tile_base = tile * kSortTile;
+  const uint32_t m = seg_find<M>(v.tile_off, v.nseg, tile);
+  const uint32_t t0 = seg_off<M>(v.tile_off, m);
+  const uint32_t base = seg_off<M>(v.prim_off, m);
+  const uint32_t n = M ? v.prim_off[m + 1] - base : n_all;
+  const uint32_t *hist = hist_all + 1024 * (size_t)m + 256 * pass;
+  keys_in += base;
+  keys_out += base;
+  if (vals_in) vals_in += base;
+  vals_out += base;
+  const uint32_t ltile = tile - t0;
+  const uint32_t tile_base = ltile * kSortTile;
   const uint32_t seg = tile_base + wave * (kSortItems * 64);
 
   uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
@@ -379,8 +469,8 @@ k_onesweep(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ va
     uint64_t peers = __ballot(ok);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
-      uint64_t m = __ballot((d >> b) & 1u);
-      peers &= ((d >> b) & 1u) ? m : ~m;
+      uint64_t bm = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bm : ~bm;
     }
     uint32_t before = __popcll(peers & lt_mask);
     uint32_t cnt = __popcll(peers);
@@ -403,16 +493,16 @@ k_onesweep(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ va
   const uint32_t tile_count = run;
   // publish this tile's aggregate, then look back for the prefix
   uint32_t *my = status + (size_t)tile * 256 + d;
-  if (tile == 0) {
+  if (ltile == 0) {
     __hip_atomic_store(my, kFlagInc | tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else {
     __hip_atomic_store(my, kFlagAgg | tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   uint32_t excl = 0;
-  if (tile > 0) {
+  if (ltile > 0) {
     int32_t j = (int32_t)tile - 1;
     uint32_t spins = 0;
-    while (j >= 0) {
+    while (j >= (int32_t)t0) {
       uint32_t s = __hip_atomic_load(status + (size_t)j * 256 + d, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
       uint32_t flag = s & ~kValueMask;
@@ -507,20 +597,35 @@ __device__ __forceinline__ void radix_node30(const uint32_t *c, int i, int n, ui
   jout = j;
 }
 
-__global__ void __launch_bounds__(256) k_karras(const uint32_t *codes, uint32_t n,
+// Maps a global primitive-space index to (mesh, local index, mesh size).
+template <bool M>
+__device__ __forceinline__ void seg_local(const SegView &v, uint32_t n_all, uint32_t x,
+                                          uint32_t &m, uint32_t &base, uint32_t &n) {
+  m = seg_find<M>(v.prim_off, v.nseg, x);
+  base = seg_off<M>(v.prim_off, m);
+  n = M ? v.prim_off[m + 1] - base : n_all;
+}
+
+template <bool M>
+__global__ void __launch_bounds__(256) k_karras(const uint32_t *codes, uint32_t n_all,
                                                uint32_t *kchild, uint32_t *kleaf,
-                                               uint32_t *kother, uint32_t *kparent) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                               uint32_t *kother, uint32_t *kparent,
+                                               SegView v) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n_all) return;
+  uint32_t m, base, n;
+  seg_local<M>(v, n_all, x, m, base, n);
+  const uint32_t i = x - base;
   if (i + 1 >= n) return;
   uint32_t child, leafbits;
   int j;
-  radix_node30(codes, (int)i, (int)n, child, leafbits, j);
-  kchild[i] = child;
-  kleaf[i] = leafbits;
-  kother[i] = (uint32_t)j;
+  radix_node30(codes + base, (int)i, (int)n, child, leafbits, j);
+  kchild[x] = child;
+  kleaf[x] = leafbits;
+  kother[x] = (uint32_t)j;
   // parent link of an internal child: parent << 1 | (1 for the right child)
-  if (!(leafbits & 1u)) kparent[child] = i << 1;
-  if (!(leafbits & 2u)) kparent[child + 1] = (i << 1) | 1u;
+  if (!(leafbits & 1u)) kparent[base + child] = i << 1;
+  if (!(leafbits & 2u)) kparent[base + child + 1] = (i << 1) | 1u;
 }
 
 __global__ void __launch_bounds__(256) k_karras_nodeinfo(const uint32_t *codes, uint32_t n,
@@ -540,17 +645,23 @@ __global__ void __launch_bounds__(256) k_karras_nodeinfo(const uint32_t *codes, 
 // [l, r] becomes a leaf iff it is a Karras leaf, r - l <= MAX_LEAF, or its
 // depth (root's children = 0) exceeds 22.
 // cls[i] = 0 (not a branch) or 1 | steps << 1 | right_turns << 8.
-__global__ void __launch_bounds__(256) k_classify(uint32_t n, int max_leaf,
+template <bool M>
+__global__ void __launch_bounds__(256) k_classify(uint32_t n_all, int max_leaf,
                                                  const uint32_t *kchild, const uint32_t *kleaf,
                                                  const uint32_t *kother, const uint32_t *kparent,
                                                  uint32_t *cls, uint32_t *leaf_start,
-                                                 BuildParams *params) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                 SegView v) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t steps = 0, turns = 0;
   bool branch = false;
-  uint32_t l = 0, r = 0;
-  if (i + 1 < n) {
-    const uint32_t j = kother[i];
+  uint32_t l = 0, r = 0, m = 0, base = 0, n = 0, i = 0;
+  if (x < n_all) {
+    seg_local<M>(v, n_all, x, m, base, n);
+    i = x - base;
+  }
+  BuildParams *params = (x < n_all) ? seg_get<M>(v, m).params : nullptr;
+  if (x < n_all && i + 1 < n) {
+    const uint32_t j = kother[x];
     l = i < j ? i : j;
     r = i < j ? j : i;
     if (i == 0) {
@@ -558,7 +669,7 @@ __global__ void __launch_bounds__(256) k_classify(uint32_t n, int max_leaf,
     } else if (r - l > (uint32_t)max_leaf) {
       uint32_t c = i;
       while (c != 0 && steps <= (uint32_t)(kMaxDepth32 + 1)) {
-        const uint32_t link = kparent[c];
+        const uint32_t link = kparent[base + c];
         const uint32_t p = link >> 1;
         if (p + 1 >= n) {  // malformed parent link: never expected, stop safely
           atomicOr(&params->error, 8u);
@@ -570,21 +681,28 @@ __global__ void __launch_bounds__(256) k_classify(uint32_t n, int max_leaf,
       }
       branch = (c == 0) && steps <= (uint32_t)(kMaxDepth32 + 1);
     }
-    cls[i] = branch ? (1u | (steps << 1) | (turns << 8)) : 0u;
+    cls[x] = branch ? (1u | (steps << 1) | (turns << 8)) : 0u;
   }
-  // deepest branch (one atomic per wavefront)
-  uint32_t m = branch ? steps : 0u;
-  for (int off = 32; off >= 1; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
-  if ((threadIdx.x & 63) == 0 && m) atomicMax(&params->max_steps, m);
+  // deepest branch per mesh: one atomic per wavefront when the whole wave
+  // lies in one mesh, else one per branch
+  uint32_t mx = branch ? steps : 0u;
+  const uint32_t m0 = __builtin_amdgcn_readfirstlane(m);
+  const bool one_mesh = !M || __ballot(m != m0) == 0ull;
+  if (one_mesh) {
+    for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+    if ((threadIdx.x & 63) == 0 && mx && params) atomicMax(&params->max_steps, mx);
+  } else if (branch && mx) {
+    atomicMax(&params->max_steps, mx);
+  }
   if (!branch) return;
   // children of this branch sit at depth `steps`
-  const uint32_t g = kchild[i];
-  const uint32_t lb = kleaf[i];
+  const uint32_t g = kchild[x];
+  const uint32_t lb = kleaf[x];
   const bool deep = steps > (uint32_t)kMaxDepth32;
   const bool left_leaf = (lb & 1u) || (g - l <= (uint32_t)max_leaf) || deep;
   const bool right_leaf = (lb & 2u) || (r - (g + 1) <= (uint32_t)max_leaf) || deep;
-  if (left_leaf) leaf_start[l] = 1;
-  if (right_leaf) leaf_start[g + 1] = 1;
+  if (left_leaf) leaf_start[base + l] = 1;
+  if (right_leaf) leaf_start[base + g + 1] = 1;
 }
 
 // ---- exclusive scan (3 launches) ----
@@ -628,9 +746,11 @@ __global__ void __launch_bounds__(1024) k_scan_partials(uint32_t *partial, uint3
   }
 }
 
+// Exclusive scan of `in` into `out`; the total goes to *total and, for a
+// one-mesh build (params != null), to num_leaves / num_nodes.
 __global__ void __launch_bounds__(kScanBlock) k_scan_downsweep(const uint32_t *in, uint32_t n,
                                                               const uint32_t *partial,
-                                                              uint32_t *out,
+                                                              uint32_t *out, uint32_t *total_out,
                                                               BuildParams *params) {
   // blocked arrangement: thread t owns items [t*kScanItems, (t+1)*kScanItems)
   uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
@@ -649,15 +769,37 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_downsweep(const uint32_t *i
     if (idx < n) out[idx] = pre;
     pre += v[k];
     if (idx == n - 1) {
-      params->num_leaves = pre;
-      params->num_nodes = 2 * pre - 1;
+      *total_out = pre;
+      if (params) {
+        params->num_leaves = pre;
+        params->num_nodes = 2 * pre - 1;
+      }
     }
   }
 }
 
+// Per-mesh leaf counts of a many-mesh build: mesh m's leaves are the global
+// leaves [leaf_rank[prim_off[m]], leaf_rank[prim_off[m+1]]).
+__global__ void __launch_bounds__(256) k_seg_leaves(SegView v, const uint32_t *leaf_rank,
+                                                   const uint32_t *total, uint32_t *leaf_base) {
+  for (uint32_t m = threadIdx.x; m < v.nseg; m += blockDim.x) {
+    const uint32_t b = leaf_rank[v.prim_off[m]];
+    const uint32_t e = (m + 1 < v.nseg) ? leaf_rank[v.prim_off[m + 1]] : *total;
+    leaf_base[m] = b;
+    if (m + 1 == v.nseg) leaf_base[v.nseg] = *total;
+    BuildParams *p = v.seg[m].params;
+    p->num_leaves = e - b;
+    p->num_nodes = 2 * (e - b) - 1;
+  }
+}
+
+// Clears the bounds tickets of every mesh (fresh parameter blocks).
+__global__ void __launch_bounds__(64) k_seg_reset(SegView v) {
+  for (uint32_t m = threadIdx.x; m < v.nseg; m += blockDim.x) v.seg[m].params->ticket = 0;
+}
+
 // ---- node emission ----
 struct NodeOut {
-  void *nodes;
   uint32_t *parent_link;  // per node: parent pos << 1 | side
   uint32_t *branch_pos;   // per Karras internal node that is a branch: its node
   uint32_t *leaf_pos;     // per leaf: node position
@@ -677,42 +819,51 @@ __device__ __forceinline__ void write_meta(void *nodes, uint32_t pos, bool leaf,
   *w = make_uint4(leaf ? 1u : 0u, (uint32_t)axis, d0, d1);
 }
 
-template <int K>
-__global__ void __launch_bounds__(256) k_emit(uint32_t n, int max_leaf, const uint32_t *codes,
-                                             const uint32_t *kchild, const uint32_t *kleaf,
-                                             const uint32_t *kother, const uint32_t *cls,
-                                             const uint32_t *leaf_rank, NodeOut o) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i + 1 >= n) return;
-  uint32_t c = cls[i];
+template <int K, bool M>
+__global__ void __launch_bounds__(256) k_emit(uint32_t n_all, int max_leaf,
+                                             const uint32_t *codes, const uint32_t *kchild,
+                                             const uint32_t *kleaf, const uint32_t *kother,
+                                             const uint32_t *cls, const uint32_t *leaf_rank,
+                                             NodeOut o, SegView v) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n_all) return;
+  uint32_t c = cls[x];
   if (!(c & 1u)) return;
+  uint32_t m, base, n;
+  seg_local<M>(v, n_all, x, m, base, n);
+  const uint32_t i = x - base;
+  if (i + 1 >= n) return;
+  void *nodes = seg_get<M>(v, m).nodes;
+  const uint32_t lbase = seg_off<M>(v.leaf_base, m);
+  uint32_t *parent_link = o.parent_link + 2 * (size_t)base;
   uint32_t steps = (c >> 1) & 127u, turns = c >> 8;
-  uint32_t j = kother[i];
+  uint32_t j = kother[x];
   uint32_t l = i < j ? i : j, r = i < j ? j : i;
-  uint32_t g = kchild[i], lb = kleaf[i];
-  uint32_t pos = steps + 2 * leaf_rank[l] - turns;
+  uint32_t g = kchild[x], lb = kleaf[x];
+  if (g + 1 >= n) return;
+  // global leaf ids; preorder positions use the mesh-local leaf ranks
+  const uint32_t yl = leaf_rank[base + l], yr = leaf_rank[base + g + 1];
+  uint32_t pos = steps + 2 * (yl - lbase) - turns;
   uint32_t posL = pos + 1;
-  uint32_t posR = steps + 2 * leaf_rank[g + 1] - turns;
+  uint32_t posR = steps + 2 * (yr - lbase) - turns;
   const uint32_t max_nodes = 2 * n - 1;
-  if (pos >= max_nodes || posL >= max_nodes || posR >= max_nodes || g + 1 >= n) return;
-  write_meta(o.nodes, pos, false, split_axis(codes[i]), posL, posR);
-  o.branch_pos[i] = pos;
-  o.parent_link[posL] = pos << 1;
-  o.parent_link[posR] = (pos << 1) | 1u;
+  if (pos >= max_nodes || posL >= max_nodes || posR >= max_nodes) return;
+  write_meta(nodes, pos, false, split_axis(codes[base + i]), posL, posR);
+  o.branch_pos[x] = pos;
+  parent_link[posL] = pos << 1;
+  parent_link[posR] = (pos << 1) | 1u;
   bool deep = steps > (uint32_t)kMaxDepth32;
   if ((lb & 1u) || (g - l <= (uint32_t)max_leaf) || deep) {
-    uint32_t li = leaf_rank[l];
-    o.leaf_pos[li] = posL;
-    o.leaf_first[li] = l;
-    o.leaf_count[li] = g + 1 - l;
-    write_meta(o.nodes, posL, true, 0, g + 1 - l, l);
+    o.leaf_pos[yl] = posL;
+    o.leaf_first[yl] = l;
+    o.leaf_count[yl] = g + 1 - l;
+    write_meta(nodes, posL, true, 0, g + 1 - l, l);
   }
   if ((lb & 2u) || (r - (g + 1) <= (uint32_t)max_leaf) || deep) {
-    uint32_t li = leaf_rank[g + 1];
-    o.leaf_pos[li] = posR;
-    o.leaf_first[li] = g + 1;
-    o.leaf_count[li] = r - g;
-    write_meta(o.nodes, posR, true, 0, r - g, g + 1);
+    o.leaf_pos[yr] = posR;
+    o.leaf_first[yr] = g + 1;
+    o.leaf_count[yr] = r - g;
+    write_meta(nodes, posR, true, 0, r - g, g + 1);
   }
 }
 
@@ -795,37 +946,52 @@ __device__ __forceinline__ Box merge_lr(const Box &l, const Box &r) {
   return m;
 }
 
-template <int K>
-__global__ void __launch_bounds__(256) k_leaves(PrimSrc s, const uint32_t *sidx, NodeOut o,
-                                               void *prims, uint32_t *sorted_faces,
-                                               BuildParams *params) {
-  uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t nleaves = params->num_leaves;
-  if (li >= nleaves || li >= s.n) return;
+// One thread per leaf (global leaf id y): leaf box + packed primitives, the
+// leaf's own slot 0 and its parent's slot for it. With M the sorted index
+// comes from the shared scratch and is copied into the mesh's accelerator.
+template <int K, bool M>
+__global__ void __launch_bounds__(256) k_leaves(uint32_t n_all, const uint32_t *sidx_all,
+                                               NodeOut o, const uint32_t *total_leaves,
+                                               SegView v) {
+  const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= *total_leaves || y >= n_all) return;
+  const uint32_t m = seg_find<M>(v.leaf_base, v.nseg, y);
+  const Seg sg = seg_get<M>(v, m);
+  const PrimSrc &s = sg.src;
+  const uint32_t base = seg_off<M>(v.prim_off, m);
+  BuildParams *params = sg.params;
   const float e = params->kEPS;
-  uint32_t first = o.leaf_first[li], count = o.leaf_count[li];
+  uint32_t first = o.leaf_first[y], count = o.leaf_count[y];
   if (first >= s.n || count > s.n - first) {
     atomicOr(&params->error, 2u);
     return;
   }
+  const uint32_t *sidx = M ? sidx_all + base : sg.sorted_index;
+  if (M) {
+    for (uint32_t q = first; q < first + count; q++) sg.sorted_index[q] = sidx[q];
+  }
   Box b;
   box_init(b);
-  leaf_box_pack<K>(s, sidx, first, count, e, prims, sorted_faces, b);
-  uint32_t node = o.leaf_pos[li];
-  float *words = reinterpret_cast<float *>(o.nodes);
+  leaf_box_pack<K>(s, sidx, first, count, e, sg.prims, sg.sorted_faces, b);
+  uint32_t node = o.leaf_pos[y];
+  float *words = reinterpret_cast<float *>(sg.nodes);
   {
     // a leaf keeps its own box in slot 0 and leaves slot 1 unused
     // (MakeLeaf32, render/render_accel_triangle.cc:865-890)
-    float4 *w4 = reinterpret_cast<float4 *>(o.nodes) + 4 * (size_t)node;
+    float4 *w4 = reinterpret_cast<float4 *>(sg.nodes) + 4 * (size_t)node;
     w4[0] = make_float4(b.mn[0], b.mn[1], b.mn[2], 0.0f);
     w4[1] = make_float4(0.0f, 0.0f, b.mx[0], b.mx[1]);
     w4[2] = make_float4(b.mx[2], 0.0f, 0.0f, 0.0f);
   }
   // every branch keeps both child boxes (TriangleNode / ParticleNode layout;
   // a tetra node's own box is the one its parent keeps for it)
-  uint32_t link = o.parent_link[node];
+  if (node == 0 || node >= 2 * s.n - 1) {
+    atomicOr(&params->error, 4u);
+    return;
+  }
+  uint32_t link = o.parent_link[2 * (size_t)base + node];
   uint32_t p = link >> 1;
-  if (node == 0 || node >= 2 * s.n - 1 || p >= 2 * s.n - 1) {
+  if (p >= 2 * s.n - 1) {
     atomicOr(&params->error, 4u);
     return;
   }
@@ -836,40 +1002,45 @@ __global__ void __launch_bounds__(256) k_leaves(PrimSrc s, const uint32_t *sidx,
 // their boxes (deeper levels ran in earlier launches) merges them,
 // MergeBoundingBox(left, right) (render/render_accel_triangle.cc:804-814),
 // into its parent's slot; the root's merge is BoundingBox().
-__global__ void __launch_bounds__(256) k_refit_level(uint32_t n, uint32_t level,
+template <bool M>
+__global__ void __launch_bounds__(256) k_refit_level(uint32_t n_all, uint32_t level,
                                                     const uint32_t *cls, NodeOut o,
-                                                    BuildParams *params) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n;
-       i += gridDim.x * blockDim.x) {
-    const uint32_t c = cls[i];
+                                                    SegView v) {
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n_all;
+       x += gridDim.x * blockDim.x) {
+    const uint32_t c = cls[x];
     if (!(c & 1u) || ((c >> 1) & 127u) != level) continue;
-    const uint32_t pos = o.branch_pos[i];
+    uint32_t m, base, n;
+    seg_local<M>(v, n_all, x, m, base, n);
+    if (x - base + 1 >= n) continue;
+    const Seg sg = seg_get<M>(v, m);
+    const uint32_t pos = o.branch_pos[x];
     if (pos >= 2 * n - 1) {
-      atomicOr(&params->error, 16u);
+      atomicOr(&sg.params->error, 16u);
       continue;
     }
-    const float4 *w4 = reinterpret_cast<const float4 *>(o.nodes) + 4 * (size_t)pos;
+    const float4 *w4 = reinterpret_cast<const float4 *>(sg.nodes) + 4 * (size_t)pos;
     const float4 a = w4[0], b = w4[1], cc = w4[2];
     Box l, r;
     l.mn[0] = a.x; l.mn[1] = a.y; l.mn[2] = a.z;
     r.mn[0] = a.w; r.mn[1] = b.x; r.mn[2] = b.y;
     l.mx[0] = b.z; l.mx[1] = b.w; l.mx[2] = cc.x;
     r.mx[0] = cc.y; r.mx[1] = cc.z; r.mx[2] = cc.w;
-    const Box m = merge_lr(l, r);
+    const Box mb = merge_lr(l, r);
     if (pos == 0) {
       for (int k = 0; k < 3; k++) {
-        params->root_bmin[k] = m.mn[k];
-        params->root_bmax[k] = m.mx[k];
+        sg.params->root_bmin[k] = mb.mn[k];
+        sg.params->root_bmax[k] = mb.mx[k];
       }
       continue;
     }
-    const uint32_t link = o.parent_link[pos];
+    const uint32_t link = o.parent_link[2 * (size_t)base + pos];
     const uint32_t p = link >> 1;
     if (p >= 2 * n - 1) {
-      atomicOr(&params->error, 16u);
+      atomicOr(&sg.params->error, 16u);
       continue;
     }
-    store_box_slot(reinterpret_cast<float *>(o.nodes) + 16 * (size_t)p, (int)(link & 1u), m);
+    store_box_slot(reinterpret_cast<float *>(sg.nodes) + 16 * (size_t)p, (int)(link & 1u), mb);
   }
 }
 
@@ -907,18 +1078,18 @@ struct SortScratch {
   uint32_t *keys_b, *vals_b, *vals_a, *status, *counters, *hist;
 };
 
-// Runs the 4 onesweep passes; keys start in keys_a (vals implicit identity),
-// the sorted pairs end in (keys_a, vals_a). hist/counters must be zero
-// before the histogram was accumulated.
-srf_status run_sort_passes(uint32_t *keys_a, uint32_t n, const SortScratch &ss,
-                           hipStream_t stream) {
-  const uint32_t tiles = sort_tiles(n);
+// Runs the 4 onesweep passes over `tiles` tiles; keys start in keys_a (vals
+// implicit local identity), the sorted pairs end in (keys_a, vals_a).
+// hist/counters/status must be zero before the histogram was accumulated.
+template <bool M>
+srf_status run_sort_passes(uint32_t *keys_a, uint32_t n, uint32_t tiles, const SortScratch &ss,
+                           const SegView &v, hipStream_t stream) {
   uint32_t *src_k = keys_a, *src_v = nullptr, *dst_k = ss.keys_b, *dst_v = ss.vals_b;
   for (int pass = 0; pass < 4; pass++) {
-    hipLaunchKernelGGL(k_onesweep, dim3(tiles), dim3(kSortBlock), 0, stream, src_k, src_v,
-                       dst_k, dst_v, n, pass * 8, ss.hist + 256 * pass,
+    hipLaunchKernelGGL(k_onesweep<M>, dim3(tiles), dim3(kSortBlock), 0, stream, src_k, src_v,
+                       dst_k, dst_v, n, pass * 8, pass, ss.hist,
                        ss.status + (size_t)pass * tiles * 256, ss.counters + pass,
-                       ss.counters + 4);
+                       ss.counters + 4, v);
     SRF_HIP(hipGetLastError());
     uint32_t *tk = src_k, *tv = src_v;
     src_k = dst_k;
@@ -929,116 +1100,211 @@ srf_status run_sort_passes(uint32_t *keys_a, uint32_t n, const SortScratch &ss,
   return SRF_OK;
 }
 
-size_t sort_scratch_words(uint32_t n) {
-  return 3 * (size_t)n + 4 * (size_t)sort_tiles(n) * 256 + 8 + 1024;
+// Layout of one build's scratch (one-mesh or many-mesh) in the context arena.
+struct BuildScratch {
+  uint32_t *codes;
+  SortScratch ss;
+  uint32_t *kchild, *kleaf, *kother, *kparent, *cls, *leaf_start, *leaf_rank, *partial;
+  uint32_t *total;
+  NodeOut o;
+  Box *partials;
+  // many-mesh only
+  Seg *seg;
+  uint32_t *offs;       // prim_off, tile_off, bblk_off, mblk_off (nseg + 1 each)
+  uint32_t *leaf_base;  // nseg + 1
+  uint32_t *sidx;       // sorted local indices before they move to each mesh
+};
+
+// Takes every slice from the arena; returns the bytes used (the arena may be
+// a null sizing pass).
+size_t layout_build(Arena &ar, uint32_t n, uint32_t tiles, uint32_t bblocks, uint32_t nseg,
+                    bool multi, BuildScratch &b) {
+  b.codes = ar.take<uint32_t>(n);
+  b.ss.keys_b = ar.take<uint32_t>(n);
+  b.ss.vals_b = ar.take<uint32_t>(n);
+  b.ss.vals_a = nullptr;
+  b.ss.status = ar.take<uint32_t>((size_t)4 * tiles * 256);
+  b.ss.counters = ar.take<uint32_t>(8);
+  b.ss.hist = ar.take<uint32_t>((size_t)1024 * nseg);
+  b.kchild = ar.take<uint32_t>(n);
+  b.kleaf = ar.take<uint32_t>(n);
+  b.kother = ar.take<uint32_t>(n);
+  b.kparent = ar.take<uint32_t>(n);
+  b.cls = ar.take<uint32_t>(n);
+  b.leaf_start = ar.take<uint32_t>(n);
+  b.leaf_rank = ar.take<uint32_t>(n);
+  b.partial = ar.take<uint32_t>(blocks_for(n, kScanTile) + 1);
+  b.total = ar.take<uint32_t>(64);
+  b.o.parent_link = ar.take<uint32_t>(2 * (size_t)n);
+  b.o.branch_pos = ar.take<uint32_t>(n);
+  b.o.leaf_pos = ar.take<uint32_t>(n);
+  b.o.leaf_first = ar.take<uint32_t>(n);
+  b.o.leaf_count = ar.take<uint32_t>(n);
+  b.partials = ar.take<Box>(bblocks);
+  b.seg = nullptr;
+  b.offs = b.leaf_base = b.sidx = nullptr;
+  if (multi) {
+    b.seg = ar.take<Seg>(nseg);
+    b.offs = ar.take<uint32_t>(4 * ((size_t)nseg + 1));
+    b.leaf_base = ar.take<uint32_t>((size_t)nseg + 1);
+    b.sidx = ar.take<uint32_t>(n);
+  }
+  return ar.off;
 }
 
 }  // namespace
 
 // ------------------------------------------------------------------ host
 
-srf_status scratch_bytes_for_build(uint64_t n, size_t *bytes) {
-  // codes + sort scratch + Karras (child, leaf, other, parent) + cls +
-  // leaf_start + leaf_rank + scan partials + per-node link/counter +
-  // per-leaf pos/first/count + bounds partials, each slice 256-B aligned.
-  size_t words = n + sort_scratch_words((uint32_t)n) + 4 * n + n + 2 * n +
-                 (n / kScanTile + 2) + 2 * (2 * n) + 3 * n;
-  *bytes = 4 * words + sizeof(Box) * kBoundsBlocks + 64 * 256;
-  return SRF_OK;
-}
-
-srf_status build_lbvh(Ctx *ctx, Kind kind, const void *desc, hipStream_t stream,
-                      Accel *acc) {
-  const PrimSrc src = make_src(kind, desc);
-  const uint32_t n = src.n;
-  size_t need = 0;
-  scratch_bytes_for_build(n, &need);
+// Builds nseg meshes of one kind with one launch per stage. nseg == 1 is the
+// plain Build32; nseg > 1 uploads the segment table first (through the
+// context's pinned staging buffer) and gives every mesh its own tree.
+srf_status build_lbvh_segments(Ctx *ctx, Kind kind, const void *const *descs,
+                               Accel *const *accs, uint32_t nseg, hipStream_t stream) {
+  if (nseg == 0) return SRF_OK;
+  const bool multi = nseg > 1;
+  std::vector<PrimSrc> src(nseg);
+  std::vector<uint32_t> offs(4 * ((size_t)nseg + 1), 0u);
+  uint32_t *prim_off = offs.data(), *tile_off = prim_off + nseg + 1,
+           *bblk_off = tile_off + nseg + 1, *mblk_off = bblk_off + nseg + 1;
+  uint64_t total = 0;
+  for (uint32_t m = 0; m < nseg; m++) {
+    src[m] = make_src(kind, descs[m]);
+    const uint32_t n = src[m].n;
+    total += n;
+    if (total >= (1ull << 30))
+      return set_error(SRF_ERR_UNSUPPORTED, "batch of %llu primitives exceeds 2^30",
+                       (unsigned long long)total);
+    prim_off[m + 1] = prim_off[m] + n;
+    tile_off[m + 1] = tile_off[m] + sort_tiles(n);
+    bblk_off[m + 1] = bblk_off[m] + blocks_for(n, 256, kBoundsBlocks);
+    mblk_off[m + 1] = mblk_off[m] + blocks_for(n, 256, 4096);
+  }
+  const uint32_t N = prim_off[nseg], tiles = tile_off[nseg];
+  const uint32_t bblocks = bblk_off[nseg], mblocks = mblk_off[nseg];
+  BuildScratch b;
+  Arena sizing{nullptr, 0, 0};
+  const size_t need = layout_build(sizing, N, tiles, bblocks, nseg, multi, b) + 256;
   srf_status st = ensure_scratch(ctx, need, stream);
   if (st != SRF_OK) return st;
   Arena ar{(char *)ctx->scratch, ctx->scratch_bytes, 0};
-  uint32_t *codes = ar.take<uint32_t>(n);
-  SortScratch ss;
-  ss.keys_b = ar.take<uint32_t>(n);
-  ss.vals_b = ar.take<uint32_t>(n);
-  ss.vals_a = acc->sorted_index;  // final sorted index lands in the accel
-  const uint32_t tiles = sort_tiles(n);
-  ss.status = ar.take<uint32_t>((size_t)4 * tiles * 256);
-  ss.counters = ar.take<uint32_t>(8);
-  ss.hist = ar.take<uint32_t>(1024);
-  uint32_t *kchild = ar.take<uint32_t>(n);
-  uint32_t *kleaf = ar.take<uint32_t>(n);
-  uint32_t *kother = ar.take<uint32_t>(n);
-  uint32_t *kparent = ar.take<uint32_t>(n);
-  uint32_t *cls = ar.take<uint32_t>(n);
-  uint32_t *leaf_start = ar.take<uint32_t>(n);
-  uint32_t *leaf_rank = ar.take<uint32_t>(n);
-  const uint32_t scan_blocks = blocks_for(n, kScanTile);
-  uint32_t *partial = ar.take<uint32_t>(scan_blocks + 1);
-  NodeOut o;
-  o.nodes = acc->nodes;
-  o.parent_link = ar.take<uint32_t>(2 * (size_t)n);
-  o.branch_pos = ar.take<uint32_t>(n);
-  o.leaf_pos = ar.take<uint32_t>(n);
-  o.leaf_first = ar.take<uint32_t>(n);
-  o.leaf_count = ar.take<uint32_t>(n);
-  Box *partials = ar.take<Box>(kBoundsBlocks);
+  layout_build(ar, N, tiles, bblocks, nseg, multi, b);
   if (ar.off > ar.size) return set_error(SRF_ERR_INTERNAL, "scratch layout overflow");
 
-  // zero the ticket (params is fully rewritten by the last bounds block)
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream,
-                     reinterpret_cast<uint32_t *>(acc->params), 0u,
-                     sizeof(BuildParams) / 4);
-  // the region [status, hist] is contiguous: clear it inside k_bounds
-  uint32_t *clear_begin = ss.status;
-  uint32_t nclear = (uint32_t)((ss.hist + 1024) - clear_begin);
-  const uint32_t bblocks = blocks_for(n, 256, kBoundsBlocks);
-  const uint32_t pblocks = blocks_for(n, 256, 4096);
-  const uint32_t nblocks = blocks_for(n, 256);
+  SegView v{};
+  v.nseg = nseg;
+  std::vector<Seg> segs(nseg);
+  for (uint32_t m = 0; m < nseg; m++) {
+    Accel *a = accs[m];
+    segs[m] = Seg{src[m], a->nodes, a->prims, a->sorted_index, a->sorted_faces, a->params};
+  }
+  v.one = segs[0];
+  if (multi) {
+    // table bytes: Seg[nseg] then the four prefix arrays
+    const size_t seg_bytes = sizeof(Seg) * nseg, off_bytes = 4 * offs.size();
+    st = ensure_host_stage(ctx, seg_bytes + off_bytes);
+    if (st != SRF_OK) return st;
+    memcpy(ctx->host_stage, segs.data(), seg_bytes);
+    memcpy((char *)ctx->host_stage + seg_bytes, offs.data(), off_bytes);
+    SRF_HIP(hipMemcpyAsync(b.seg, ctx->host_stage, seg_bytes, hipMemcpyHostToDevice, stream));
+    SRF_HIP(hipMemcpyAsync(b.offs, (char *)ctx->host_stage + seg_bytes, off_bytes,
+                           hipMemcpyHostToDevice, stream));
+    SRF_HIP(hipEventRecord(ctx->stage_done, stream));
+    v.seg = b.seg;
+    v.prim_off = b.offs;
+    v.tile_off = b.offs + (nseg + 1);
+    v.bblk_off = b.offs + 2 * (nseg + 1);
+    v.mblk_off = b.offs + 3 * (nseg + 1);
+    v.leaf_base = b.leaf_base;
+    b.ss.vals_a = b.sidx;
+  } else {
+    b.ss.vals_a = accs[0]->sorted_index;  // the sorted index lands in the accel
+  }
   int max_leaf = kind == kTri ? kTriMaxLeaf : (kind == kTet ? kTetMaxLeaf : kSphMaxLeaf);
+
+  // zero the bounds tickets (the last bounds block rewrites each params block)
+  if (multi) {
+    hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(64), 0, stream, v);
+  } else {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream,
+                       reinterpret_cast<uint32_t *>(accs[0]->params), 0u,
+                       sizeof(BuildParams) / 4);
+  }
+  SRF_HIP(hipGetLastError());
+  // the region [status, hist] is contiguous: clear it inside k_bounds
+  uint32_t *clear_begin = b.ss.status;
+  uint32_t nclear = (uint32_t)((b.ss.hist + 1024 * (size_t)nseg) - clear_begin);
+  const uint32_t nblocks = blocks_for(N, 256);
 
 #define SRF_DISPATCH(KERNEL, ...)                                                      \
   do {                                                                                 \
-    if (kind == kTri)                                                                  \
-      hipLaunchKernelGGL(KERNEL<kTri>, __VA_ARGS__);                                   \
-    else if (kind == kTet)                                                             \
-      hipLaunchKernelGGL(KERNEL<kTet>, __VA_ARGS__);                                   \
+    if (multi) {                                                                       \
+      if (kind == kTri)                                                                \
+        hipLaunchKernelGGL((KERNEL<kTri, true>), __VA_ARGS__);                           \
+      else if (kind == kTet)                                                           \
+        hipLaunchKernelGGL((KERNEL<kTet, true>), __VA_ARGS__);                           \
+      else                                                                             \
+        hipLaunchKernelGGL((KERNEL<kSph, true>), __VA_ARGS__);                           \
+    } else {                                                                           \
+      if (kind == kTri)                                                                \
+        hipLaunchKernelGGL((KERNEL<kTri, false>), __VA_ARGS__);                          \
+      else if (kind == kTet)                                                           \
+        hipLaunchKernelGGL((KERNEL<kTet, false>), __VA_ARGS__);                          \
+      else                                                                             \
+        hipLaunchKernelGGL((KERNEL<kSph, false>), __VA_ARGS__);                          \
+    }                                                                                  \
+    SRF_HIP(hipGetLastError());                                                        \
+  } while (0)
+#define SRF_DISPATCH_M(KERNEL, ...)                                                    \
+  do {                                                                                 \
+    if (multi)                                                                         \
+      hipLaunchKernelGGL(KERNEL<true>, __VA_ARGS__);                                   \
     else                                                                               \
-      hipLaunchKernelGGL(KERNEL<kSph>, __VA_ARGS__);                                   \
+      hipLaunchKernelGGL(KERNEL<false>, __VA_ARGS__);                                  \
     SRF_HIP(hipGetLastError());                                                        \
   } while (0)
 
-  SRF_DISPATCH(k_bounds, dim3(bblocks), dim3(256), 0, stream, src, partials, acc->params,
-               clear_begin, nclear);
-  SRF_DISPATCH(k_morton, dim3(pblocks), dim3(256), 0, stream, src, acc->params, codes,
-               ss.hist);
-  st = run_sort_passes(codes, n, ss, stream);
+  SRF_DISPATCH(k_bounds, dim3(bblocks), dim3(256), 0, stream, v, b.partials, clear_begin,
+               nclear);
+  SRF_DISPATCH(k_morton, dim3(mblocks), dim3(256), 0, stream, v, b.codes, b.ss.hist);
+  st = multi ? run_sort_passes<true>(b.codes, N, tiles, b.ss, v, stream)
+             : run_sort_passes<false>(b.codes, N, tiles, b.ss, v, stream);
   if (st != SRF_OK) return st;
-  // sorted codes in `codes`, sorted index in acc->sorted_index
-  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, stream,
-                     leaf_start, 0u, (size_t)n);
-  hipLaunchKernelGGL(k_karras, dim3(nblocks), dim3(256), 0, stream, codes, n, kchild, kleaf,
-                     kother, kparent);
-  hipLaunchKernelGGL(k_classify, dim3(nblocks), dim3(256), 0, stream, n, max_leaf, kchild,
-                     kleaf, kother, kparent, cls, leaf_start, acc->params);
+  // sorted codes in `codes`, sorted local indices in ss.vals_a
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(N, 256, 2048)), dim3(256), 0, stream,
+                     b.leaf_start, 0u, (size_t)N);
+  SRF_DISPATCH_M(k_karras, dim3(nblocks), dim3(256), 0, stream, b.codes, N, b.kchild, b.kleaf,
+                 b.kother, b.kparent, v);
+  SRF_DISPATCH_M(k_classify, dim3(nblocks), dim3(256), 0, stream, N, max_leaf, b.kchild,
+                 b.kleaf, b.kother, b.kparent, b.cls, b.leaf_start, v);
+  const uint32_t scan_blocks = blocks_for(N, kScanTile);
   hipLaunchKernelGGL(k_scan_reduce, dim3(scan_blocks), dim3(kScanBlock), 0, stream,
-                     leaf_start, n, partial);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, stream, partial, scan_blocks);
+                     b.leaf_start, N, b.partial);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, stream, b.partial, scan_blocks);
   hipLaunchKernelGGL(k_scan_downsweep, dim3(scan_blocks), dim3(kScanBlock), 0, stream,
-                     leaf_start, n, partial, leaf_rank, acc->params);
+                     b.leaf_start, N, b.partial, b.leaf_rank, b.total,
+                     multi ? nullptr : accs[0]->params);
   SRF_HIP(hipGetLastError());
-  SRF_DISPATCH(k_emit, dim3(nblocks), dim3(256), 0, stream, n, max_leaf, codes, kchild, kleaf,
-               kother, cls, leaf_rank, o);
-  SRF_DISPATCH(k_leaves, dim3(nblocks), dim3(256), 0, stream, src, acc->sorted_index, o,
-               acc->prims, acc->sorted_faces, acc->params);
-  // bottom-up refit, one level per launch (levels are branch steps 23..0)
-  const uint32_t rblocks = blocks_for(n, 256, 2048);
-  for (int level = kMaxBranchLevels - 1; level >= 0; level--) {
-    hipLaunchKernelGGL(k_refit_level, dim3(rblocks), dim3(256), 0, stream, n, (uint32_t)level,
-                       cls, o, acc->params);
+  if (multi) {
+    hipLaunchKernelGGL(k_seg_leaves, dim3(1), dim3(256), 0, stream, v, b.leaf_rank, b.total,
+                       b.leaf_base);
+    SRF_HIP(hipGetLastError());
   }
-  SRF_HIP(hipGetLastError());
+  SRF_DISPATCH(k_emit, dim3(nblocks), dim3(256), 0, stream, N, max_leaf, b.codes, b.kchild,
+               b.kleaf, b.kother, b.cls, b.leaf_rank, b.o, v);
+  SRF_DISPATCH(k_leaves, dim3(nblocks), dim3(256), 0, stream, N, b.ss.vals_a, b.o, b.total, v);
+  // bottom-up refit, one level per launch (levels are branch steps 23..0)
+  const uint32_t rblocks = blocks_for(N, 256, 2048);
+  for (int level = kMaxBranchLevels - 1; level >= 0; level--)
+    SRF_DISPATCH_M(k_refit_level, dim3(rblocks), dim3(256), 0, stream, N, (uint32_t)level,
+                   b.cls, b.o, v);
 #undef SRF_DISPATCH
+#undef SRF_DISPATCH_M
   return SRF_OK;
+}
+
+srf_status build_lbvh(Ctx *ctx, Kind kind, const void *desc, hipStream_t stream, Accel *acc) {
+  return build_lbvh_segments(ctx, kind, &desc, &acc, 1, stream);
 }
 
 srf_status launch_morton(Kind kind, const void *desc, const float bmin[3],
@@ -1063,7 +1329,8 @@ srf_status launch_morton(Kind kind, const void *desc, const float bmin[3],
 
 srf_status sort_pairs30(Ctx *ctx, const uint32_t *codes_in, uint32_t n, uint32_t *codes_out,
                         uint32_t *index_out, hipStream_t stream) {
-  size_t need = 4 * (2 * (size_t)n + sort_scratch_words(n)) + 8 * 256;
+  const uint32_t tiles = sort_tiles(n);
+  size_t need = 4 * (3 * (size_t)n + 4 * (size_t)tiles * 256 + 8 + 1024) + 8 * 256;
   srf_status st = ensure_scratch(ctx, need, stream);
   if (st != SRF_OK) return st;
   Arena ar{(char *)ctx->scratch, ctx->scratch_bytes, 0};
@@ -1071,7 +1338,6 @@ srf_status sort_pairs30(Ctx *ctx, const uint32_t *codes_in, uint32_t n, uint32_t
   ss.keys_b = ar.take<uint32_t>(n);
   ss.vals_b = ar.take<uint32_t>(n);
   ss.vals_a = index_out;
-  const uint32_t tiles = sort_tiles(n);
   ss.status = ar.take<uint32_t>((size_t)4 * tiles * 256);
   ss.counters = ar.take<uint32_t>(8);
   ss.hist = ar.take<uint32_t>(1024);
@@ -1083,7 +1349,9 @@ srf_status sort_pairs30(Ctx *ctx, const uint32_t *codes_in, uint32_t n, uint32_t
   hipLaunchKernelGGL(k_histogram, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, stream,
                      codes_out, n, ss.hist);
   SRF_HIP(hipGetLastError());
-  return run_sort_passes(codes_out, n, ss, stream);
+  SegView v{};
+  v.nseg = 1;
+  return run_sort_passes<false>(codes_out, n, tiles, ss, v, stream);
 }
 
 srf_status launch_radix_tree(const uint32_t *sorted_codes, uint32_t n, srf_node_info *out,

@@ -71,6 +71,9 @@ struct Ctx {
   int device;
   void *scratch;
   size_t scratch_bytes;
+  void *host_stage;       // pinned staging for segment tables (many-mesh builds)
+  size_t host_stage_bytes;
+  hipEvent_t stage_done;  // the last copy out of host_stage
 };
 
 // Bump allocator over the context scratch (256-byte aligned slices).
@@ -90,6 +93,8 @@ struct Arena {
 srf_status set_error(srf_status s, const char *fmt, ...);
 srf_status hip_error(hipError_t e, const char *what);
 srf_status ensure_scratch(Ctx *ctx, size_t bytes, hipStream_t stream);
+// Waits for the previous copy out of the staging buffer, then grows it.
+srf_status ensure_host_stage(Ctx *ctx, size_t bytes);
 
 #define SRF_HIP(call)                                  \
   do {                                                 \
@@ -98,9 +103,10 @@ srf_status ensure_scratch(Ctx *ctx, size_t bytes, hipStream_t stream);
   } while (0)
 
 // Build-stage launchers (srf_build.hip).
-srf_status scratch_bytes_for_build(uint64_t n, size_t *bytes);
 srf_status build_lbvh(Ctx *ctx, Kind kind, const void *desc, hipStream_t stream,
                       Accel *acc);
+srf_status build_lbvh_segments(Ctx *ctx, Kind kind, const void *const *descs,
+                               Accel *const *accs, uint32_t nseg, hipStream_t stream);
 srf_status launch_morton(Kind kind, const void *desc, const float bmin[3],
                          const float bmax[3], uint32_t *codes, hipStream_t s);
 srf_status sort_pairs30(Ctx *ctx, const uint32_t *codes_in, uint32_t n,

@@ -15,6 +15,9 @@ Workloads (sizes as named):
   gaussian_volume(256)      256^3 float32, sum of 64 anisotropic Gaussians in [0,1]
   transfer_function(256)    256-entry RGBA table, smooth ramps, alpha <= 0.05
   pinhole_rays(...)         one primary ray per pixel centre, 8x8-tile order
+  batch_scene(64)           64 meshes, log-uniform 10k..2M triangles, alternating
+                            perturbed geodesic spheres and heightfields, each
+                            with a random rigid transform; 16 orbit cameras
 """
 from __future__ import annotations
 
@@ -228,3 +231,107 @@ def deep_chain_points(dup=40, radius=1e-3):
     P = np.repeat(np.array(pts, np.float32), dup, axis=0)
     R = np.full(len(P), radius, np.float32)
     return np.ascontiguousarray(P), R
+
+
+_ICO_V = None
+
+
+def _icosahedron():
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    V = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+                  [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    F = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+                  [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+                  [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+                  [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    return V, F
+
+
+def geodesic_sphere(freq, seed=BASE_SEED + 7, radius=0.5, bumps=6, amp=0.06):
+    """Icosahedron with every face split into freq^2 triangles, projected to a
+    sphere and perturbed radially by seeded spherical bumps: 20 freq^2
+    triangles (face vertices are not shared across icosahedron edges)."""
+    rng = _rng(seed)
+    IV, IF = _icosahedron()
+    i, j = np.meshgrid(np.arange(freq + 1), np.arange(freq + 1), indexing="ij")
+    keep = (i + j) <= freq
+    bi, bj = i[keep], j[keep]                      # barycentric grid of one face
+    local = {(a, b): k for k, (a, b) in enumerate(zip(bi.tolist(), bj.tolist()))}
+    tris = []
+    for a in range(freq):
+        for b in range(freq - a):
+            tris.append((local[(a, b)], local[(a + 1, b)], local[(a, b + 1)]))
+            if a + b + 1 < freq:
+                tris.append((local[(a + 1, b)], local[(a + 1, b + 1)], local[(a, b + 1)]))
+    tris = np.array(tris, np.int64)
+    nloc = len(bi)
+    w1, w2 = bi / freq, bj / freq
+    w0 = 1.0 - w1 - w2
+    P = (w0[None, :, None] * IV[IF[:, 0]][:, None, :] + w1[None, :, None] * IV[IF[:, 1]][:, None, :]
+         + w2[None, :, None] * IV[IF[:, 2]][:, None, :]).reshape(-1, 3)
+    P /= np.linalg.norm(P, axis=1, keepdims=True)
+    centres = rng.normal(size=(bumps, 3))
+    centres /= np.linalg.norm(centres, axis=1, keepdims=True)
+    r = np.ones(len(P))
+    for c in centres:
+        r += amp * rng.uniform(-1, 1) * np.exp(-8.0 * np.sum((P - c) ** 2, axis=1))
+    V = (radius * r[:, None] * P).astype(np.float32)
+    F = (tris[None, :, :] + nloc * np.arange(20)[:, None, None]).reshape(-1, 3).astype(np.uint32)
+    return np.ascontiguousarray(V), np.ascontiguousarray(F)
+
+
+def _rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def batch_scene(count=64, lo=10_000, hi=2_000_000, seed=BASE_SEED + 3, spacing=1.5):
+    """Many meshes for the batched build and the two-level trace.
+
+    Sizes N_i = floor(exp(U(ln lo, ln hi))); even i is a perturbed geodesic
+    sphere with 20 f^2 ~ N_i triangles (radius 0.5), odd i a heightfield with
+    2 (g-1)^2 ~ N_i triangles centred on the origin. Each mesh gets a random
+    rotation and a translation on a jittered cube lattice of the given
+    spacing. Returns (meshes [(V, F)], T) with T[i] the 4x4 float64
+    object-to-world matrix in the row-vector convention p' = p T
+    (render/render_matrix.h:165-173: rows 0-2 the linear part, row 3 the
+    translation)."""
+    rng = _rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
+    side = int(np.ceil(count ** (1.0 / 3.0)))
+    meshes, T = [], []
+    for i, n in enumerate(sizes):
+        if i % 2 == 0:
+            f = max(1, int(round((n / 20.0) ** 0.5)))
+            V, F = geodesic_sphere(f, seed=seed + 1000 + i)
+        else:
+            g = max(2, int(round((n / 2.0) ** 0.5)) + 1)
+            V, F = heightfield(g, seed=seed + 1000 + i)
+            V = V - np.array([0.5, 0.5, 0.0], np.float32)
+        meshes.append((np.ascontiguousarray(V), np.ascontiguousarray(F)))
+        cell = np.array([i % side, (i // side) % side, i // (side * side)], np.float64)
+        M = np.eye(4)
+        M[:3, :3] = _rotation(rng).T  # row-vector form of a rotation
+        M[3, :3] = spacing * (cell + rng.uniform(-0.15, 0.15, 3))
+        T.append(M)
+    return meshes, np.array(T)
+
+
+def scene_cameras(T, count=16, width=1024, height=1024, fov=40.0, tiled=True):
+    """`count` pinhole cameras on a horizontal orbit around the scene's
+    centre, all looking at it; returns a list of ray arrays."""
+    centre = T[:, 3, :3].mean(axis=0)
+    extent = np.ptp(T[:, 3, :3], axis=0).max() + 1.0
+    out = []
+    for k in range(count):
+        a = 2.0 * np.pi * k / count
+        eye = centre + extent * np.array([1.4 * np.cos(a), 1.4 * np.sin(a), 0.6])
+        out.append(pinhole_rays(tuple(eye), tuple(centre), (0, 0, 1), fov, width, height,
+                                tiled=tiled))
+    return out
