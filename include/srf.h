@@ -58,6 +58,7 @@ typedef enum srf_prim_kind {
 typedef struct srf_ctx srf_ctx;
 typedef struct srf_accel srf_accel;
 typedef struct srf_volume srf_volume;
+typedef struct srf_scene srf_scene;
 
 /* Ray: the fields of render/render_ray.h:47-151 that traversal reads.
  * flags bit 0 = double_sided (Ray::double_sided; eye rays use 1,
@@ -142,6 +143,32 @@ typedef struct srf_node_info {
   uint32_t left_leaf;
   uint32_t right_leaf;
 } srf_node_info;
+
+/* One element of a two-level scene: RenderElement's accelerator and
+ * matrices (gles/gles_render_graph.h:26-118). All three matrices are
+ * row-major [4][4] in the reference's row-vector convention
+ * (render/render_matrix.h:165-183: p' = p * M, row 3 = translation):
+ * T local->world, invT = Inverse(T), invT33 = Inverse(T with row 3
+ * cleared), as RenderElement::Update computes them
+ * (gles/gles_render_graph.cc:81-101). */
+typedef struct {
+  const srf_accel *accel; /* triangles */
+  double T[16];
+  double invT[16];
+  double invT33[16];
+} srf_instance;
+
+/* A top-level node as BVHTree keeps it (BVHNode, render/render_bvh_tree.h):
+ * box, children (0 = none), split axis, leaf flag, element index. 72 B. */
+typedef struct {
+  double bmin[3];
+  double bmax[3];
+  int32_t child[2];
+  int32_t axis;
+  int32_t is_leaf;
+  uint32_t node_id;
+  uint32_t pad;
+} srf_tlas_node;
 
 /* ---------------------------------------------------------------- context */
 const char *srf_last_error_string(void);
@@ -249,6 +276,31 @@ srf_status srf_lbvh_radix_tree30(srf_ctx *ctx, const uint32_t *sorted_codes,
                                  uint64_t n, srf_node_info *out, void *stream);
 
 /* ----------------------------------------------------------------- volume */
+/* ---- two-level scenes (RenderGraph) ---- */
+/* RenderGraph::Build (gles/gles_render_graph.cc:103-143): world boxes of the
+ * elements (TransformBoundingBox of each accelerator's BoundingBox(), :19-79)
+ * and the top-level BVHTree over them (render/render_bvh_tree.cc:319-502),
+ * built on the host; waits for the accelerators' builds. The scene keeps the
+ * accelerators' device arrays: rebuilding one of them requires a new scene,
+ * as the reference rebuilds its graph. */
+srf_status srf_scene_create(srf_ctx *ctx, uint32_t num_instances,
+                            const srf_instance *instances_host, srf_scene **out);
+srf_status srf_scene_destroy(srf_scene *scene);
+/* BVHTree::BoundingBox (render/render_bvh_tree.cc:656-669). */
+srf_status srf_scene_bounds(const srf_scene *scene, double bmin_host[3], double bmax_host[3]);
+/* The top-level BVH: *count nodes; nodes_host (NULL to query the count)
+ * receives them in BVHTree order. */
+srf_status srf_scene_export(const srf_scene *scene, srf_tlas_node *nodes_host, uint32_t *count);
+/* RenderGraph::Trace (gles/gles_render_graph.cc:145-314) for a batch of
+ * rays: hits[i] = {world distance, u, v, prim_id} of the nearest element
+ * hit, instance_ids[i] its element (~0 on a miss; may be NULL). The ray's
+ * prev_prim_id is not used (no element is the ray's previous one).
+ * persistent != 0 runs a persistent grid that hands out 64-ray tiles from a
+ * work counter (graph-capturable: the counter is reset by a kernel). */
+srf_status srf_scene_trace(const srf_scene *scene, const srf_ray *rays, uint64_t n,
+                           srf_hit *hits, uint32_t *instance_ids, int persistent,
+                           void *stream);
+
 /* glTexImage3D / lsglTexImage3DPointer for a single-component FP32 volume
  * (gles/gles_texture.cc:198-357); wrap_* = 0 REPEAT (the glTexImage3D
  * default, gles/gles_common.cc:1274-1276) or 1 CLAMP_TO_EDGE

@@ -21,7 +21,16 @@ TET_NODE = np.dtype([("bmin", "<f4", (3,)), ("bmax", "<f4", (3,)),
                      ("flag", "<i4"), ("axis", "<i4"), ("data", "<u4", (2,))])
 SPH_NODE = np.dtype([("bmin", "<f4", (2, 3)), ("bmax", "<f4", (2, 3)),
                      ("data", "<u4", (2,)), ("flag", "<i2"), ("axis", "<i2")])
+TLAS_NODE = np.dtype([("bmin", "<f8", (3,)), ("bmax", "<f8", (3,)), ("child", "<i4", (2,)),
+                      ("axis", "<i4"), ("is_leaf", "<i4"), ("node_id", "<u4"), ("pad", "<u4")])
 assert TRI_NODE.itemsize == 64 and TET_NODE.itemsize == 40 and SPH_NODE.itemsize == 60
+assert TLAS_NODE.itemsize == 72
+
+
+class TriElement(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("indices", C.c_void_p), ("V", C.c_void_p),
+                ("F", C.c_void_p), ("eps_scale", C.c_double), ("T", C.c_double * 16),
+                ("invT", C.c_double * 16), ("invT33", C.c_double * 16)]
 
 _p = C.c_void_p
 _i64 = C.c_int64
@@ -48,8 +57,9 @@ class Oracle:
                      "or_radix_tree30", "or_trace_tri", "or_trace_tet",
                      "or_trace_sph", "or_tex3d_sample", "or_volume_fwd_f32",
                      "or_volume_fwd_f64", "or_volume_bwd_f64",
-                     "or_scene_bounds_sph"):
+                     "or_scene_bounds_sph", "or_transform_box", "or_scene_trace"):
             getattr(L, name).restype = None
+        L.or_tlas_build.restype = _i64
         for name in ("or_build_tri", "or_build_tet", "or_build_sph"):
             getattr(L, name).restype = _i64
         L.or_morton30_point.restype = C.c_uint32
@@ -220,6 +230,44 @@ class Oracle:
         return gv, gt
 
 
+    # ---- two-level scene ----
+    def transform_box(self, bmin, bmax, T):
+        out = np.zeros(6, np.float64)
+        self.lib.or_transform_box(_ptr(_c(bmin, np.float64)), _ptr(_c(bmax, np.float64)),
+                                  _ptr(_c(T, np.float64).reshape(16)), _ptr(out))
+        return out
+
+    def tlas_build(self, boxes):
+        boxes = _c(boxes, np.float64).reshape(-1, 6)
+        nodes = np.zeros(2 * len(boxes), TLAS_NODE)
+        m = self.lib.or_tlas_build(_ptr(boxes), _i64(len(boxes)), _ptr(nodes))
+        return nodes[:m].copy()
+
+    def scene_trace(self, tlas, elements, rays, threads=1):
+        """elements: list of dicts with the element's tree (build_tri), V, F,
+        T, invT, invT33 (4x4 float64, row-vector convention)."""
+        from surface_amd.workloads import HIT_DTYPE
+        keep = []
+        arr = (TriElement * len(elements))()
+        for k, e in enumerate(elements):
+            nodes = _c(e["tree"]["nodes"], TRI_NODE)
+            idx = _c(e["tree"]["indices"], np.uint32)
+            V = _c(e["V"], np.float32)
+            F = _c(e["F"], np.uint32)
+            keep += [nodes, idx, V, F]
+            arr[k].nodes, arr[k].indices = _ptr(nodes).value, _ptr(idx).value
+            arr[k].V, arr[k].F = _ptr(V).value, _ptr(F).value
+            arr[k].eps_scale = e["tree"]["eps_scale"]
+            for name in ("T", "invT", "invT33"):
+                getattr(arr[k], name)[:] = list(np.asarray(e[name], np.float64).reshape(16))
+        tl = _c(tlas, TLAS_NODE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        inst = np.zeros(len(rays), np.uint32)
+        self.lib.or_scene_trace(_ptr(tl), arr, _ptr(rays), _i64(len(rays)), _ptr(hits),
+                                _ptr(inst), C.c_int(threads))
+        return hits, inst
+
+
 def ref_available(variant="serial"):
     return os.path.exists(os.path.join(HERE, "_ref", f"libsrfref_{variant}.so"))
 
@@ -336,6 +384,47 @@ class Reference:
             self.lib.ref_free(_p(tree["handle"]))
             tree["handle"] = None
 
+    def scene_build(self, trees, T):
+        """RenderGraph over triangle accelerators (trees from build(..., keep=True))
+        with object-to-world matrices T [n, 4, 4]; returns the reference's
+        invT, invT33, world boxes, top-level BVH nodes and the graph handle."""
+        n = len(trees)
+        T = _c(T, np.float64).reshape(n, 16)
+        hs = (C.c_void_p * n)(*[t["handle"] for t in trees])
+        invT = np.zeros((n, 16), np.float64)
+        invT33 = np.zeros((n, 16), np.float64)
+        xbox = np.zeros((n, 6), np.float64)
+        cap = 2 * n + 2
+        tbox = np.zeros((cap, 6), np.float64)
+        tmeta = np.zeros((cap, 5), np.int32)
+        cnt = C.c_int()
+        self.lib.ref_scene_build.restype = _p
+        h = self.lib.ref_scene_build(C.c_int(n), hs, _ptr(T), _ptr(invT), _ptr(invT33),
+                                     _ptr(xbox), _ptr(tbox), _ptr(tmeta), C.byref(cnt),
+                                     C.c_int(cap))
+        m = cnt.value
+        tl = np.zeros(m, TLAS_NODE)
+        tl["bmin"], tl["bmax"] = tbox[:m, :3], tbox[:m, 3:]
+        tl["child"] = tmeta[:m, :2]
+        tl["axis"], tl["is_leaf"], tl["node_id"] = tmeta[:m, 2], tmeta[:m, 3], tmeta[:m, 4]
+        return dict(handle=h, invT=invT.reshape(n, 4, 4), invT33=invT33.reshape(n, 4, 4),
+                    xbox=xbox, tlas=tl)
+
+    def scene_trace(self, scene, rays, width=0, threads=0):
+        from surface_amd.workloads import HIT_DTYPE
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        inst = np.zeros(len(rays), np.uint32)
+        self.lib.ref_scene_trace.restype = None
+        self.lib.ref_scene_trace(_p(scene["handle"]), _ptr(rays), _i64(len(rays)), _i64(width),
+                                 _ptr(hits), _ptr(inst), C.c_int(threads))
+        return hits, inst
+
+    def scene_free(self, scene):
+        if scene.get("handle"):
+            self.lib.ref_scene_free.restype = None
+            self.lib.ref_scene_free(_p(scene["handle"]))
+            scene["handle"] = None
+
     def tex3d(self, vox, dims, clamp, uvw):
         vox = _c(vox, np.float32)
         uvw = _c(uvw, np.float32).reshape(-1, 3)
@@ -363,3 +452,13 @@ def canonical_tet(nodes):
 
 
 canonical_sph = canonical_tri
+
+
+def canonical_tlas(nodes):
+    """Zero what BVHTree leaves uninitialised: a branch's node id, a leaf's
+    axis, and the padding."""
+    n = nodes.copy()
+    n["node_id"][n["is_leaf"] == 0] = 0
+    n["axis"][n["is_leaf"] == 1] = 0
+    n["pad"] = 0
+    return n

@@ -1625,3 +1625,360 @@ void or_volume_bwd_f64(const or_vol_params *p, const double *vox,
   }
   free(Tk);
 }
+
+
+/* ===================================================================
+ * Two-level scene
+ * =================================================================== */
+
+/* Matrixd::MultV, render/render_matrix.h:165-183: row vector times the
+ * matrix, sums left to right. */
+static void mult_v(const double m[16], const double v[3], double out[3]) {
+  for (int k = 0; k < 3; k++)
+    out[k] = ((m[0 * 4 + k] * v[0] + m[1 * 4 + k] * v[1]) + m[2 * 4 + k] * v[2]) + m[3 * 4 + k];
+}
+
+/* std::min / std::max: keep the first unless the second is strictly
+ * smaller / larger. */
+static double dmin_ref(double a, double b) { return (b < a) ? b : a; }
+static double dmax_ref(double a, double b) { return (a < b) ? b : a; }
+
+/* TransformBoundingBox, gles/gles_render_graph.cc:19-79: the 8 corners
+ * (x fastest, then y, then z) transformed, min/max from corner 0 on. */
+void or_transform_box(const double bmin[3], const double bmax[3], const double T[16],
+                      double out[6]) {
+  double xb[8][3];
+  for (int c = 0; c < 8; c++) {
+    double p[3] = {(c & 1) ? bmax[0] : bmin[0], (c & 2) ? bmax[1] : bmin[1],
+                   (c & 4) ? bmax[2] : bmin[2]};
+    mult_v(T, p, xb[c]);
+  }
+  for (int k = 0; k < 3; k++) {
+    out[k] = xb[0][k];
+    out[3 + k] = xb[0][k];
+  }
+  for (int c = 1; c < 8; c++)
+    for (int k = 0; k < 3; k++) {
+      out[k] = dmin_ref(xb[c][k], out[k]);
+      out[3 + k] = dmax_ref(xb[c][k], out[3 + k]);
+    }
+}
+
+typedef struct {
+  double bmin[3], bmax[3];
+  uint32_t id;
+} tlas_item; /* BVHData */
+
+/* CalculateBoundingBox over items [lo, hi), render/render_bvh_tree.cc:129-147 */
+static void items_box(const tlas_item *it, int64_t lo, int64_t hi, double mn[3], double mx[3]) {
+  for (int k = 0; k < 3; k++) {
+    mn[k] = it[lo].bmin[k];
+    mx[k] = it[lo].bmax[k];
+  }
+  for (int64_t i = lo + 1; i < hi; i++)
+    for (int k = 0; k < 3; k++) {
+      mn[k] = dmin_ref(mn[k], it[i].bmin[k]);
+      mx[k] = dmax_ref(mx[k], it[i].bmax[k]);
+    }
+}
+
+/* PartitionNodes, render/render_bvh_tree.cc:83-127: items after the first
+ * whose bmax on the axis lies below the split move forward (swaps), the
+ * first item is exchanged with the last moved one, and the split index
+ * counts every item of the range below the split. */
+static int64_t partition_items(tlas_item *it, int64_t lo, int64_t hi, double pos, int axis) {
+  tlas_item first = it[lo];
+  int64_t pivot = lo;
+  for (int64_t i = lo + 1; i < hi; i++) {
+    if (it[i].bmax[axis] < pos) {
+      pivot++;
+      tlas_item t = it[pivot];
+      it[pivot] = it[i];
+      it[i] = t;
+    }
+  }
+  it[lo] = it[pivot];
+  it[pivot] = first;
+  int64_t split = lo;
+  for (int64_t i = lo; i < hi; i++)
+    if (it[i].bmax[axis] < pos) split++;
+  return split;
+}
+
+typedef struct {
+  or_tlas_node *nodes;
+  int64_t count;
+  tlas_item *items;
+} tlas_builder;
+
+static void pad_box(double mn[3], double mx[3]) {
+  const double eps = 1.0e-6; /* render/render_bvh_tree.cc:404-410 */
+  for (int k = 0; k < 3; k++) {
+    double ext = mx[k] - mn[k];
+    mn[k] = mn[k] - ext * eps;
+    mx[k] = mx[k] + ext * eps;
+  }
+}
+
+/* ConstructTree, render/render_bvh_tree.cc:319-472: spatial median of the
+ * node box along the longest axis, then the next two axes in turn; object
+ * median when no axis separates; children appended depth first. */
+static void tlas_construct(tlas_builder *b, int64_t root, int64_t lo, int64_t hi) {
+  static const int order[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+  const int64_t n = hi - lo;
+  or_tlas_node *nd = b->nodes + root;
+  if (n == 1) {
+    nd->is_leaf = 1;
+    nd->child[0] = nd->child[1] = 0;
+    nd->node_id = b->items[lo].id;
+    return;
+  }
+  double size[3], mn[3];
+  for (int k = 0; k < 3; k++) {
+    size[k] = nd->bmax[k] - nd->bmin[k];
+    mn[k] = nd->bmin[k];
+  }
+  double longest = size[0];
+  int first_axis = 0;
+  if (longest < size[1]) {
+    longest = size[1];
+    first_axis = 1;
+  }
+  if (longest < size[2]) first_axis = 2;
+  int64_t split = lo, nleft = 0, nright = n;
+  int axis = 0;
+  for (int i = 0; i < 3; i++) {
+    axis = order[first_axis][i];
+    double pos = mn[axis] + size[axis] * 0.5;
+    split = partition_items(b->items, lo, hi, pos, axis);
+    if (split != lo && split != hi) {
+      nleft = split - lo;
+      nright = n - nleft;
+      break;
+    }
+  }
+  if (split == lo || split == hi) {
+    nleft = n / 2;
+    nright = n - nleft;
+    split = lo + (hi - lo) / 2;
+  }
+  b->nodes[root].axis = axis;
+  b->nodes[root].is_leaf = 0;
+  if (nleft > 0) {
+    int64_t c = b->count++;
+    or_tlas_node *cn = b->nodes + c;
+    items_box(b->items, lo, split, cn->bmin, cn->bmax);
+    pad_box(cn->bmin, cn->bmax);
+    b->nodes[root].child[0] = (int32_t)c;
+    tlas_construct(b, c, lo, split);
+  } else {
+    b->nodes[root].child[0] = 0;
+  }
+  if (nright > 0) {
+    int64_t c = b->count++;
+    or_tlas_node *cn = b->nodes + c;
+    items_box(b->items, split, hi, cn->bmin, cn->bmax);
+    pad_box(cn->bmin, cn->bmax);
+    b->nodes[root].child[1] = (int32_t)c;
+    tlas_construct(b, c, split, hi);
+  } else {
+    b->nodes[root].child[1] = 0;
+  }
+}
+
+/* BVHTree::BuildTree, render/render_bvh_tree.cc:478-502: root = unpadded
+ * scene box (CalcSceneBBox). */
+int64_t or_tlas_build(const double *boxes, int64_t n, or_tlas_node *nodes) {
+  if (n <= 0) return 0;
+  tlas_item *items = (tlas_item *)malloc(sizeof(tlas_item) * (size_t)n);
+  for (int64_t i = 0; i < n; i++) {
+    for (int k = 0; k < 3; k++) {
+      items[i].bmin[k] = boxes[6 * i + k];
+      items[i].bmax[k] = boxes[6 * i + 3 + k];
+    }
+    items[i].id = (uint32_t)i;
+  }
+  memset(nodes, 0, sizeof(or_tlas_node) * (size_t)(2 * n));
+  tlas_builder b = {nodes, 1, items};
+  items_box(items, 0, n, nodes[0].bmin, nodes[0].bmax);
+  tlas_construct(&b, 0, 0, n);
+  free(items);
+  return b.count;
+}
+
+typedef struct {
+  int sign[3];
+  double org[3], inv[3];
+} tlas_ray;
+
+/* SetupRay, render/render_bvh_tree.cc:40-71 (kEPS = 2e-15) */
+static void tlas_setup(const double org[3], const double dir[3], tlas_ray *r) {
+  for (int k = 0; k < 3; k++) {
+    r->org[k] = org[k];
+    r->sign[k] = dir[k] < 0.0 ? 1 : 0;
+    if (fabs(dir[k]) > 2.0e-15)
+      r->inv[k] = 1.0 / dir[k];
+    else
+      r->inv[k] = dir[k] < 0.0 ? -FLT_MAX : FLT_MAX;
+  }
+}
+
+/* TestRayAABB, render/render_bvh_tree.cc:198-255 */
+static int tlas_box(const tlas_ray *r, const double mn[3], const double mx[3], double *tmin_out,
+                    double *tmax_out) {
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; k++) {
+    lo[k] = r->sign[k] ? mx[k] : mn[k];
+    hi[k] = r->sign[k] ? mn[k] : mx[k];
+  }
+  double tminx = (lo[0] - r->org[0]) * r->inv[0], tmaxx = (hi[0] - r->org[0]) * r->inv[0];
+  double tminy = (lo[1] - r->org[1]) * r->inv[1], tmaxy = (hi[1] - r->org[1]) * r->inv[1];
+  double tmin = (tminx > tminy) ? tminx : tminy;
+  double tmax = (tmaxx < tmaxy) ? tmaxx : tmaxy;
+  double tminz = (lo[2] - r->org[2]) * r->inv[2], tmaxz = (hi[2] - r->org[2]) * r->inv[2];
+  tmin = (tmin > tminz) ? tmin : tminz;
+  tmax = (tmax < tmaxz) ? tmax : tmaxz;
+  if (tmax > 0.0 && tmin <= tmax) {
+    *tmin_out = tmin;
+    *tmax_out = tmax;
+    return 1;
+  }
+  return 0;
+}
+
+typedef struct {
+  double tmin;
+  uint32_t id;
+} tlas_isect;
+
+/* BVHTree::Trace, render/render_bvh_tree.cc:504-586: every leaf whose box the
+ * ray enters (maxdist = DBL_MAX), in traversal order, then ordered by tMin.
+ * The reference's std::sort is unstable in general; it is an insertion sort
+ * (stable) for up to 16 entries, and a stable order is used here. Returns
+ * the count (capacity: number of leaves). */
+static int64_t tlas_trace(const or_tlas_node *tl, const tlas_ray *r, tlas_isect *out) {
+  int64_t cnt = 0;
+  int64_t stack[256];
+  int sp = 0;
+  const or_tlas_node *nd = tl;
+  for (;;) {
+    if (nd->is_leaf) {
+      double t0, t1;
+      if (tlas_box(r, nd->bmin, nd->bmax, &t0, &t1) && t0 <= DBL_MAX) {
+        out[cnt].tmin = t0;
+        out[cnt].id = nd->node_id;
+        cnt++;
+      }
+      if (sp < 1) break;
+      nd = tl + stack[--sp];
+    } else {
+      double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+      int hl = nd->child[0] > 0 && tlas_box(r, tl[nd->child[0]].bmin, tl[nd->child[0]].bmax, &a0, &a1);
+      int hr = nd->child[1] > 0 && tlas_box(r, tl[nd->child[1]].bmin, tl[nd->child[1]].bmax, &b0, &b1);
+      if (!hl && !hr) {
+        if (sp < 1) break;
+        nd = tl + stack[--sp];
+      } else if (hl && !hr) {
+        nd = tl + nd->child[0];
+      } else if (!hl && hr) {
+        nd = tl + nd->child[1];
+      } else {
+        int o = r->sign[nd->axis];
+        stack[sp++] = nd->child[1 - o];
+        nd = tl + nd->child[o];
+      }
+    }
+  }
+  for (int64_t i = 1; i < cnt; i++) { /* stable insertion sort by tmin */
+    tlas_isect v = out[i];
+    int64_t j = i;
+    while (j > 0 && v.tmin < out[j - 1].tmin) {
+      out[j] = out[j - 1];
+      j--;
+    }
+    out[j] = v;
+  }
+  return cnt;
+}
+
+static int64_t tlas_leaves(const or_tlas_node *tl) {
+  /* leaves reachable from the root */
+  int64_t stack[256], cnt = 0;
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const or_tlas_node *nd = tl + stack[--sp];
+    if (nd->is_leaf) {
+      cnt++;
+      continue;
+    }
+    if (nd->child[0] > 0) stack[sp++] = nd->child[0];
+    if (nd->child[1] > 0) stack[sp++] = nd->child[1];
+  }
+  return cnt;
+}
+
+/* RenderGraph::Trace, gles/gles_render_graph.cc:145-314, triangle elements */
+static void scene_trace_one(const or_tlas_node *tl, const or_tri_element *el, tlas_isect *list,
+                            const or_ray *r, or_hit *h, uint32_t *inst) {
+  double ro[3] = {r->org[0], r->org[1], r->org[2]};
+  double rd[3] = {r->dir[0], r->dir[1], r->dir[2]};
+  tlas_ray tr;
+  tlas_setup(ro, rd, &tr);
+  h->t = FLT_MAX;
+  h->u = h->v = 0.0f;
+  h->prim_id = 0xffffffffu;
+  *inst = 0xffffffffu;
+  int64_t cnt = tlas_trace(tl, &tr, list);
+  double nearest = DBL_MAX;
+  for (int64_t i = 0; i < cnt; i++) {
+    if (nearest < list[i].tmin) continue;
+    const or_tri_element *e = el + list[i].id;
+    double lo[3], ld[3];
+    mult_v(e->invT, ro, lo);
+    mult_v(e->invT33, rd, ld);
+    or_ray lr;
+    for (int k = 0; k < 3; k++) {
+      lr.org[k] = (float)lo[k];
+      lr.dir[k] = (float)ld[k];
+    }
+    lr.flags = r->flags & 1u;
+    lr.prev_prim_id = 0xffffffffu;
+    or_hit bh;
+    trace_tri_one(e->nodes, e->indices, e->V, e->F, (float)e->eps_scale, &lr, &bh, NULL);
+    if (bh.prim_id == 0xffffffffu) continue;
+    /* world distance of the local hit point */
+    float lp[3], wp[3];
+    for (int k = 0; k < 3; k++) lp[k] = (float)(lo[k] + (double)bh.t * ld[k]);
+    double lpd[3] = {lp[0], lp[1], lp[2]}, wpd[3];
+    mult_v(e->T, lpd, wpd);
+    for (int k = 0; k < 3; k++) wp[k] = (float)wpd[k];
+    float px = wp[0] - r->org[0], py = wp[1] - r->org[1], pz = wp[2] - r->org[2];
+    float tw = sqrtf((px * px + py * py) + pz * pz);
+    if (tw < nearest) {
+      nearest = tw;
+      *h = bh;
+      h->t = tw;
+      *inst = list[i].id;
+    }
+  }
+}
+
+void or_scene_trace(const or_tlas_node *tlas, const or_tri_element *elems,
+                    const or_ray *rays, int64_t nrays, or_hit *hits, uint32_t *inst,
+                    int threads) {
+  const int64_t nleaves = tlas_leaves(tlas);
+  (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+#endif
+  {
+    tlas_isect *list = (tlas_isect *)malloc(sizeof(tlas_isect) * (size_t)(nleaves + 1));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+    for (int64_t i = 0; i < nrays; i++)
+      scene_trace_one(tlas, elems, list, rays + i, hits + i, inst + i);
+    free(list);
+  }
+}

@@ -151,6 +151,42 @@ void or_volume_bwd_f64(const or_vol_params *p, const double *vox,
                        const double *grad_rgba, double *grad_vox,
                        double *grad_tf);
 
+/* ---- Two-level scene: the top-level BVHTree over element world boxes and
+ * RenderGraph::Trace over triangle elements (gles/gles_render_graph.cc). */
+typedef struct {
+  double bmin[3];
+  double bmax[3];
+  int32_t child[2]; /* 0 = none */
+  int32_t axis;
+  int32_t is_leaf;
+  uint32_t node_id; /* leaves: element index */
+} or_tlas_node;     /* BVHNode, render/render_bvh_tree.h */
+
+/* TransformBoundingBox: world box of a local box under T (row-vector
+ * convention, p' = p T), gles/gles_render_graph.cc:19-79. out = bmin, bmax. */
+void or_transform_box(const double bmin[3], const double bmax[3], const double T[16],
+                      double out[6]);
+/* BVHTree::BuildTree over n boxes (box i = bmin[3], bmax[3], node id i).
+ * nodes must hold 2n entries; returns the node count. */
+int64_t or_tlas_build(const double *boxes, int64_t n, or_tlas_node *nodes);
+
+typedef struct {
+  const or_tri_node *nodes;
+  const uint32_t *indices;
+  const float *V;
+  const uint32_t *F;
+  double eps_scale;
+  double T[16];      /* local -> world */
+  double invT[16];   /* world -> local */
+  double invT33[16]; /* inverse of T with the translation cleared */
+} or_tri_element;
+
+/* RenderGraph::Trace per ray: hit t = world distance, u/v/prim_id of the
+ * element's triangle, inst = element index (~0 on a miss). */
+void or_scene_trace(const or_tlas_node *tlas, const or_tri_element *elems,
+                    const or_ray *rays, int64_t nrays, or_hit *hits, uint32_t *inst,
+                    int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,19 @@ class AccelInfo(C.Structure):
                 ("eps_scale", C.c_double), ("node_bytes", C.c_uint64)]
 
 
+class Instance(C.Structure):
+    """srf_instance: accelerator + T, invT, invT33 (row-major 4x4 doubles)."""
+    _fields_ = [("accel", C.c_void_p), ("T", C.c_double * 16), ("invT", C.c_double * 16),
+                ("invT33", C.c_double * 16)]
+
+
+class TlasNode(C.Structure):
+    """srf_tlas_node (72 bytes)."""
+    _fields_ = [("bmin", C.c_double * 3), ("bmax", C.c_double * 3), ("child", C.c_int32 * 2),
+                ("axis", C.c_int32), ("is_leaf", C.c_int32), ("node_id", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
 _vp, _u64, _i = C.c_void_p, C.c_uint64, C.c_int
 SIGNATURES = {
     "srf_last_error_string": (C.c_char_p, []),
@@ -73,6 +86,11 @@ SIGNATURES = {
     "srf_lbvh_morton30": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
     "srf_lbvh_sort30": (_i, [_vp, _vp, _u64, _vp, _vp, _vp]),
     "srf_lbvh_radix_tree30": (_i, [_vp, _vp, _u64, _vp, _vp]),
+    "srf_scene_create": (_i, [_vp, C.c_uint32, _vp, C.POINTER(_vp)]),
+    "srf_scene_destroy": (_i, [_vp]),
+    "srf_scene_bounds": (_i, [_vp, _vp, _vp]),
+    "srf_scene_export": (_i, [_vp, _vp, C.POINTER(C.c_uint32)]),
+    "srf_scene_trace": (_i, [_vp, _vp, _u64, _vp, _vp, _i, _vp]),
     "srf_volume_create": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, C.POINTER(_vp)]),
     "srf_volume_destroy": (_i, [_vp]),
     "srf_volume_sample": (_i, [_vp, _vp, _u64, _vp, _vp]),

@@ -300,3 +300,75 @@ def radix_tree30(sorted_codes, ctx=None, stream=None):
 __all__ = ["Context", "TriangleAccel", "TetraAccel", "ParticleAccel", "SrfError",
            "device_tensor", "hits_to_numpy", "morton30", "sort30", "radix_tree30",
            "RAY_DTYPE", "HIT_DTYPE", "TET_EXT_DTYPE", "default_context"]
+
+
+def rigid_inverses(T):
+    """invT and invT33 of object-to-world matrices T [n, 4, 4] (row-vector
+    convention) computed with numpy. The reference's RenderElement::Update
+    computes them with its own Cramer-rule inverse; pass those matrices to
+    Scene to reproduce its results bit for bit."""
+    T = np.asarray(T, np.float64)
+    invT = np.linalg.inv(T)
+    T33 = T.copy()
+    T33[:, 3, :3] = 0.0
+    return invT, np.linalg.inv(T33)
+
+
+class Scene:
+    """RenderGraph over triangle accelerators: Build (top-level BVH over the
+    elements' world boxes) at construction, Trace for batches of rays."""
+
+    def __init__(self, accels, T, invT=None, invT33=None, ctx=None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.accels = list(accels)  # keeps the elements alive
+        T = np.asarray(T, np.float64).reshape(-1, 4, 4)
+        if invT is None or invT33 is None:
+            invT, invT33 = rigid_inverses(T)
+        n = len(self.accels)
+        arr = (_lib.Instance * n)()
+        for i, a in enumerate(self.accels):
+            a._require()
+            arr[i].accel = a.handle.value
+            arr[i].T[:] = list(T[i].reshape(16))
+            arr[i].invT[:] = list(np.asarray(invT[i], np.float64).reshape(16))
+            arr[i].invT33[:] = list(np.asarray(invT33[i], np.float64).reshape(16))
+        h = C.c_void_p()
+        check(self.lib.srf_scene_create(self.ctx.handle, n, arr, C.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.lib.srf_scene_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def BoundingBox(self):
+        bmin = (C.c_double * 3)()
+        bmax = (C.c_double * 3)()
+        check(self.lib.srf_scene_bounds(self.handle, bmin, bmax))
+        return np.array(bmin[:]), np.array(bmax[:])
+
+    def export_tlas(self):
+        cnt = C.c_uint32()
+        check(self.lib.srf_scene_export(self.handle, None, C.byref(cnt)))
+        arr = (_lib.TlasNode * cnt.value)()
+        check(self.lib.srf_scene_export(self.handle, arr, C.byref(cnt)))
+        return np.frombuffer(bytes(arr), dtype=np.dtype([
+            ("bmin", "<f8", (3,)), ("bmax", "<f8", (3,)), ("child", "<i4", (2,)),
+            ("axis", "<i4"), ("is_leaf", "<i4"), ("node_id", "<u4"), ("pad", "<u4")])).copy()
+
+    def Trace(self, rays, hits=None, instance_ids=None, persistent=True, stream=None):
+        """Closest hit of every ray over all elements: hits [n, 4] (world t,
+        u, v, prim id bits) and instance_ids [n] int32 (-1 on a miss)."""
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+        if instance_ids is None:
+            instance_ids = torch.empty(n, dtype=torch.int32, device=rays.device)
+        check(self.lib.srf_scene_trace(self.handle, _ptr(rays), n, _ptr(hits),
+                                       _ptr(instance_ids), 1 if persistent else 0,
+                                       _stream(stream)))
+        return hits, instance_ids

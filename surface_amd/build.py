@@ -16,7 +16,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsrf.so")
-SOURCES = ["srf_api.hip", "srf_build.hip", "srf_trace.hip", "srf_volume.hip"]
+SOURCES = ["srf_api.hip", "srf_build.hip", "srf_trace.hip", "srf_scene.hip", "srf_volume.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

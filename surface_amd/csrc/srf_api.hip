@@ -169,6 +169,18 @@ srf_status fetch_params(const Accel *a, BuildParams *p) {
 }
 
 }  // namespace
+
+srf_status fetch_params_for(const Accel *a, BuildParams *p) { return fetch_params(a, p); }
+
+struct Scene;
+srf_status scene_create(Ctx *ctx, uint32_t n, const srf_instance *inst, Scene **out);
+srf_status scene_trace(const Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
+                       uint32_t *inst_out, int persistent, hipStream_t stream);
+void scene_free(Scene *sc);
+srf_status scene_query(const Scene *sc, double bmin[3], double bmax[3], srf_tlas_node *nodes,
+                       uint32_t *count);
+int scene_device(const Scene *sc);
+
 }  // namespace srf
 
 using namespace srf;
@@ -534,6 +546,51 @@ srf_status srf_lbvh_radix_tree30(srf_ctx *ctx, const uint32_t *sorted_codes, uin
   if (n >= (1ull << 31)) return set_error(SRF_ERR_INVALID_ARG, "too many keys");
   DeviceGuard g(c->device);
   return launch_radix_tree(sorted_codes, (uint32_t)n, out, reinterpret_cast<hipStream_t>(stream));
+}
+
+srf_status srf_scene_create(srf_ctx *ctx, uint32_t n, const srf_instance *inst,
+                            srf_scene **out) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !inst || !out || n == 0)
+    return set_error(SRF_ERR_INVALID_ARG, "null argument or empty scene");
+  *out = nullptr;
+  DeviceGuard g(c->device);
+  Scene *sc = nullptr;
+  srf_status st = scene_create(c, n, inst, &sc);
+  if (st != SRF_OK) return st;
+  *out = reinterpret_cast<srf_scene *>(sc);
+  return SRF_OK;
+}
+
+srf_status srf_scene_destroy(srf_scene *scene) {
+  Scene *sc = reinterpret_cast<Scene *>(scene);
+  if (!sc) return SRF_OK;
+  DeviceGuard g(scene_device(sc));
+  (void)hipDeviceSynchronize();
+  scene_free(sc);
+  return SRF_OK;
+}
+
+srf_status srf_scene_bounds(const srf_scene *scene, double bmin[3], double bmax[3]) {
+  const Scene *sc = reinterpret_cast<const Scene *>(scene);
+  if (!sc || !bmin || !bmax) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  return scene_query(sc, bmin, bmax, nullptr, nullptr);
+}
+
+srf_status srf_scene_export(const srf_scene *scene, srf_tlas_node *nodes, uint32_t *count) {
+  const Scene *sc = reinterpret_cast<const Scene *>(scene);
+  if (!sc || !count) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  return scene_query(sc, nullptr, nullptr, nodes, count);
+}
+
+srf_status srf_scene_trace(const srf_scene *scene, const srf_ray *rays, uint64_t n,
+                           srf_hit *hits, uint32_t *inst_out, int persistent, void *stream) {
+  const Scene *sc = reinterpret_cast<const Scene *>(scene);
+  if (!sc || (n && (!rays || !hits))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many rays in one call");
+  DeviceGuard g(scene_device(sc));
+  return scene_trace(sc, rays, n, hits, inst_out, persistent,
+                     reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

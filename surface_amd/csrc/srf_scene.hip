@@ -1,0 +1,574 @@
+// srf_scene.hip -- two-level scenes: many triangle accelerators placed by
+// transforms, a top-level BVH over their world boxes, and one kernel that
+// traces rays through both levels.
+//
+// The top level follows the reference's RenderGraph exactly:
+//   world box    TransformBoundingBox          gles/gles_render_graph.cc:19-79
+//   top BVH      BVHTree::BuildTree / ConstructTree
+//                                               render/render_bvh_tree.cc:319-502
+//   trace        BVHTree::Trace (double slabs)  render/render_bvh_tree.cc:504-586
+//                RenderGraph::Trace             gles/gles_render_graph.cc:145-314
+// The top-level build runs on the host (a handful of boxes, as in the
+// reference); the trace is one lane per ray on the GPU:
+//   1. walk the top BVH in the reference's order and pick, among the leaves
+//      whose box the ray enters, the next one by (entry t, visit order) --
+//      the order std::sort leaves them in for up to 16 entries (insertion
+//      sort, stable); the walk is repeated per element, so no per-ray list
+//      is kept;
+//   2. stop once the nearest hit so far lies before that entry t (the
+//      reference skips every later element in that case);
+//   3. move the ray into the element's space with its invT / invT33 (double),
+//      trace the element's LBVH with the single-level traversal, and keep
+//      the hit if its world distance |T (o + t d) - o| is nearer.
+// With many more than 16 entries the reference's introsort may order equal
+// entry distances differently; only exact ties of entry t are affected.
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "srf_trace_dev.h"
+
+namespace srf {
+
+// 64-byte top-level node (BVHNode, render/render_bvh_tree.h): box in double,
+// children (0 = none), split axis, leaf flag, element id.
+struct alignas(16) TlasNode {
+  double bmin[3];
+  double bmax[3];
+  int32_t child[2];
+  uint32_t axis_leaf;  // axis | (is_leaf << 8)
+  uint32_t id;
+};
+static_assert(sizeof(TlasNode) == 64, "TlasNode is 64 bytes");
+
+// Per element on the device: its accelerator and the rows 0..3, columns
+// 0..2 of T, invT and invT33 (MultV reads nothing else).
+struct alignas(16) SceneInst {
+  const float4 *nodes;
+  const float4 *prims;
+  uint32_t max_nodes;
+  uint32_t nprims;
+  float kEPS;
+  uint32_t pad;
+  double T[12];
+  double invT[12];
+  double invT33[12];
+};
+
+struct Scene {
+  int device;
+  uint32_t count;       // elements
+  uint32_t num_nodes;   // top-level nodes
+  uint32_t depth;       // most branches on a root-to-leaf path
+  double bmin[3], bmax[3];
+  TlasNode *d_nodes;
+  SceneInst *d_inst;
+  uint32_t *d_counter;  // work counter of the persistent launch
+  std::vector<TlasNode> h_nodes;
+};
+
+namespace {
+
+constexpr int kSceneBlock = 128;
+constexpr uint32_t kChunk = 64;  // rays a wavefront takes per grab
+
+// ------------------------------------------------------------------ host
+
+// Matrixd::MultV (render/render_matrix.h:165-183), row vector times m.
+void mult_v_host(const double m[16], const double v[3], double out[3]) {
+  for (int k = 0; k < 3; k++)
+    out[k] = ((m[k] * v[0] + m[4 + k] * v[1]) + m[8 + k] * v[2]) + m[12 + k];
+}
+
+double dmin_ref(double a, double b) { return (b < a) ? b : a; }
+double dmax_ref(double a, double b) { return (a < b) ? b : a; }
+
+// TransformBoundingBox: corners x fastest, min/max starting from corner 0.
+void world_box(const double bmin[3], const double bmax[3], const double T[16], double mn[3],
+               double mx[3]) {
+  for (int c = 0; c < 8; c++) {
+    const double p[3] = {(c & 1) ? bmax[0] : bmin[0], (c & 2) ? bmax[1] : bmin[1],
+                         (c & 4) ? bmax[2] : bmin[2]};
+    double q[3];
+    mult_v_host(T, p, q);
+    for (int k = 0; k < 3; k++) {
+      mn[k] = c ? dmin_ref(q[k], mn[k]) : q[k];
+      mx[k] = c ? dmax_ref(q[k], mx[k]) : q[k];
+    }
+  }
+}
+
+struct Item {  // BVHData
+  double bmin[3], bmax[3];
+  uint32_t id;
+};
+
+class TlasBuilder {
+ public:
+  explicit TlasBuilder(std::vector<Item> items) : items_(std::move(items)) {}
+
+  std::vector<TlasNode> build() {
+    nodes_.assign(2 * items_.size(), TlasNode{});
+    count_ = 1;
+    span_box(0, items_.size(), nodes_[0].bmin, nodes_[0].bmax);  // root: unpadded
+    construct(0, 0, items_.size());
+    nodes_.resize(count_);
+    return nodes_;
+  }
+
+ private:
+  // CalculateBoundingBox over [lo, hi)
+  void span_box(size_t lo, size_t hi, double mn[3], double mx[3]) const {
+    for (int k = 0; k < 3; k++) {
+      mn[k] = items_[lo].bmin[k];
+      mx[k] = items_[lo].bmax[k];
+    }
+    for (size_t i = lo + 1; i < hi; i++)
+      for (int k = 0; k < 3; k++) {
+        mn[k] = dmin_ref(mn[k], items_[i].bmin[k]);
+        mx[k] = dmax_ref(mx[k], items_[i].bmax[k]);
+      }
+  }
+
+  // PartitionNodes: items after the first whose bmax on the axis lies below
+  // the split are swapped forward; the first item trades places with the
+  // last of them; the returned split counts every item below the split.
+  size_t partition(size_t lo, size_t hi, double pos, int axis) {
+    const Item head = items_[lo];
+    size_t pivot = lo;
+    for (size_t i = lo + 1; i < hi; i++)
+      if (items_[i].bmax[axis] < pos) std::swap(items_[++pivot], items_[i]);
+    items_[lo] = items_[pivot];
+    items_[pivot] = head;
+    size_t split = lo;
+    for (size_t i = lo; i < hi; i++)
+      if (items_[i].bmax[axis] < pos) split++;
+    return split;
+  }
+
+  uint32_t add_child(size_t lo, size_t hi) {
+    const uint32_t c = count_++;
+    TlasNode &nd = nodes_[c];
+    span_box(lo, hi, nd.bmin, nd.bmax);
+    for (int k = 0; k < 3; k++) {  // 1e-6 of the extent on each side
+      const double ext = nd.bmax[k] - nd.bmin[k];
+      nd.bmin[k] = nd.bmin[k] - ext * 1.0e-6;
+      nd.bmax[k] = nd.bmax[k] + ext * 1.0e-6;
+    }
+    return c;
+  }
+
+  // ConstructTree: spatial median along the longest axis of the node box,
+  // then the other two axes in turn; object median if none separates.
+  void construct(uint32_t node, size_t lo, size_t hi) {
+    static const int kOrder[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+    const size_t n = hi - lo;
+    if (n == 1) {
+      nodes_[node].axis_leaf = 1u << 8;
+      nodes_[node].child[0] = nodes_[node].child[1] = 0;
+      nodes_[node].id = items_[lo].id;
+      return;
+    }
+    double size[3], base[3];
+    for (int k = 0; k < 3; k++) {
+      size[k] = nodes_[node].bmax[k] - nodes_[node].bmin[k];
+      base[k] = nodes_[node].bmin[k];
+    }
+    int first = 0;
+    double longest = size[0];
+    if (longest < size[1]) {
+      longest = size[1];
+      first = 1;
+    }
+    if (longest < size[2]) first = 2;
+    size_t split = lo, nleft = 0, nright = n;
+    int axis = 0;
+    for (int i = 0; i < 3; i++) {
+      axis = kOrder[first][i];
+      split = partition(lo, hi, base[axis] + size[axis] * 0.5, axis);
+      if (split != lo && split != hi) {
+        nleft = split - lo;
+        nright = n - nleft;
+        break;
+      }
+    }
+    if (split == lo || split == hi) {
+      nleft = n / 2;
+      nright = n - nleft;
+      split = lo + (hi - lo) / 2;
+    }
+    nodes_[node].axis_leaf = (uint32_t)axis;
+    if (nleft > 0) {
+      const uint32_t c = add_child(lo, split);
+      nodes_[node].child[0] = (int32_t)c;
+      construct(c, lo, split);
+    }
+    if (nright > 0) {
+      const uint32_t c = add_child(split, hi);
+      nodes_[node].child[1] = (int32_t)c;
+      construct(c, split, hi);
+    }
+  }
+
+  std::vector<Item> items_;
+  std::vector<TlasNode> nodes_;
+  uint32_t count_ = 0;
+};
+
+uint32_t branch_depth(const std::vector<TlasNode> &nodes, uint32_t node) {
+  const TlasNode &nd = nodes[node];
+  if (nd.axis_leaf >> 8) return 0;
+  uint32_t d = 0;
+  for (int c = 0; c < 2; c++)
+    if (nd.child[c] > 0) d = std::max(d, branch_depth(nodes, (uint32_t)nd.child[c]));
+  return d + 1;
+}
+
+// ---------------------------------------------------------------- device
+
+struct TlasRay {
+  double o[3], inv[3];
+  int sign[3];
+};
+
+// SetupRay, render/render_bvh_tree.cc:40-71 (directions below 2e-15 get
+// +-FLT_MAX as inverse)
+__device__ __forceinline__ void tlas_setup(const double o[3], const double d[3], TlasRay &r) {
+  for (int k = 0; k < 3; k++) {
+    r.o[k] = o[k];
+    r.sign[k] = d[k] < 0.0 ? 1 : 0;
+    r.inv[k] = fabs(d[k]) > 2.0e-15 ? 1.0 / d[k] : (d[k] < 0.0 ? -FLT_MAX : FLT_MAX);
+  }
+}
+
+// TestRayAABB, render/render_bvh_tree.cc:198-255
+__device__ __forceinline__ bool tlas_box(const TlasRay &r, const TlasNode &nd, double &tmin) {
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; k++) {
+    lo[k] = r.sign[k] ? nd.bmax[k] : nd.bmin[k];
+    hi[k] = r.sign[k] ? nd.bmin[k] : nd.bmax[k];
+  }
+  const double tminx = (lo[0] - r.o[0]) * r.inv[0], tmaxx = (hi[0] - r.o[0]) * r.inv[0];
+  const double tminy = (lo[1] - r.o[1]) * r.inv[1], tmaxy = (hi[1] - r.o[1]) * r.inv[1];
+  double t0 = (tminx > tminy) ? tminx : tminy;
+  double t1 = (tmaxx < tmaxy) ? tmaxx : tmaxy;
+  const double tminz = (lo[2] - r.o[2]) * r.inv[2], tmaxz = (hi[2] - r.o[2]) * r.inv[2];
+  t0 = (t0 > tminz) ? t0 : tminz;
+  t1 = (t1 < tmaxz) ? t1 : tmaxz;
+  tmin = t0;
+  return (t1 > 0.0) && (t0 <= t1);
+}
+
+__device__ __forceinline__ TlasNode load_tlas(const TlasNode *tl, uint32_t i) {
+  const double2 *p = reinterpret_cast<const double2 *>(tl + i);
+  const double2 a = p[0], b = p[1], c = p[2];
+  const uint4 m = reinterpret_cast<const uint4 *>(tl + i)[3];
+  TlasNode nd;
+  nd.bmin[0] = a.x;
+  nd.bmin[1] = a.y;
+  nd.bmin[2] = b.x;
+  nd.bmax[0] = b.y;
+  nd.bmax[1] = c.x;
+  nd.bmax[2] = c.y;
+  nd.child[0] = (int32_t)m.x;
+  nd.child[1] = (int32_t)m.y;
+  nd.axis_leaf = m.z;
+  nd.id = m.w;
+  return nd;
+}
+
+// One pass over the top BVH in the reference's visit order: the leaf with
+// the smallest key (entry t, visit rank) above the previous key. Returns
+// false when none is left.
+template <int kBlock>
+__device__ __forceinline__ bool next_element(const TlasNode *__restrict__ tl, uint32_t nnodes,
+                                             const TlasRay &r, uint32_t *tst, uint32_t tdepth,
+                                             bool have_prev, double prev_t, uint32_t prev_rank,
+                                             double &next_t, uint32_t &next_rank,
+                                             uint32_t &next_id) {
+  bool found = false;
+  uint32_t rank = 0, sp = 0, node = 0;
+  for (;;) {
+    if (node >= nnodes) break;
+    const TlasNode nd = load_tlas(tl, node);
+    if (nd.axis_leaf >> 8) {
+      double t0;
+      if (tlas_box(r, nd, t0)) {
+        const bool after = !have_prev || t0 > prev_t || (t0 == prev_t && rank > prev_rank);
+        const bool before = !found || t0 < next_t || (t0 == next_t && rank < next_rank);
+        if (after && before) {
+          found = true;
+          next_t = t0;
+          next_rank = rank;
+          next_id = nd.id;
+        }
+        rank++;
+      }
+      if (sp == 0) break;
+      node = tst[(--sp) * kBlock];
+    } else {
+      double a, b;
+      const bool hl = nd.child[0] > 0 && (uint32_t)nd.child[0] < nnodes &&
+                      tlas_box(r, load_tlas(tl, (uint32_t)nd.child[0]), a);
+      const bool hr = nd.child[1] > 0 && (uint32_t)nd.child[1] < nnodes &&
+                      tlas_box(r, load_tlas(tl, (uint32_t)nd.child[1]), b);
+      if (hl && hr) {
+        const int o = r.sign[nd.axis_leaf & 3u];
+        if (sp < tdepth) tst[(sp++) * kBlock] = (uint32_t)nd.child[1 - o];
+        node = (uint32_t)nd.child[o];
+      } else if (hl) {
+        node = (uint32_t)nd.child[0];
+      } else if (hr) {
+        node = (uint32_t)nd.child[1];
+      } else {
+        if (sp == 0) break;
+        node = tst[(--sp) * kBlock];
+      }
+    }
+  }
+  return found;
+}
+
+__device__ __forceinline__ void mult_rows(const double *m, const double v[3], double out[3]) {
+  for (int k = 0; k < 3; k++)
+    out[k] = ((m[k] * v[0] + m[3 + k] * v[1]) + m[6 + k] * v[2]) + m[9 + k];
+}
+
+// RenderGraph::Trace for one ray.
+template <int kBlock>
+__device__ __forceinline__ void scene_ray(const TlasNode *__restrict__ tl, uint32_t nnodes,
+                                          const SceneInst *__restrict__ inst, uint32_t ninst,
+                                          const srf_ray *__restrict__ rays, uint32_t i,
+                                          uint32_t *bst, uint32_t *tst, uint32_t tdepth,
+                                          srf_hit *__restrict__ hits,
+                                          uint32_t *__restrict__ inst_out) {
+  const float4 *r4 = reinterpret_cast<const float4 *>(rays + i);
+  const float4 ra = r4[0], rb = r4[1];
+  const float wo[3] = {ra.x, ra.y, ra.z};
+  const bool ds = (__float_as_uint(ra.w) & SRF_RAY_DOUBLE_SIDED) != 0;
+  const double ro[3] = {ra.x, ra.y, ra.z}, rd[3] = {rb.x, rb.y, rb.z};
+  TlasRay tr;
+  tlas_setup(ro, rd, tr);
+  double nearest = DBL_MAX;
+  float bt = FLT_MAX, bu = 0.0f, bv = 0.0f;
+  uint32_t bprim = SRF_NO_PRIM, binst = SRF_NO_PRIM;
+  bool have_prev = false;
+  double prev_t = 0.0;
+  uint32_t prev_rank = 0;
+  for (uint32_t pass = 0; pass < ninst; pass++) {
+    double nt = 0.0;
+    uint32_t nr = 0, nid = 0;
+    if (!next_element<kBlock>(tl, nnodes, tr, tst, tdepth, have_prev, prev_t, prev_rank, nt, nr,
+                              nid))
+      break;
+    if (nearest < nt) break;  // every later element is skipped as well
+    have_prev = true;
+    prev_t = nt;
+    prev_rank = nr;
+    if (nid >= ninst) continue;
+    const SceneInst *e = inst + nid;
+    double lo[3], ld[3];
+    mult_rows(e->invT, ro, lo);
+    mult_rows(e->invT33, rd, ld);
+    RaySetup s;
+    for (int k = 0; k < 3; k++) {
+      s.o[k] = (float)lo[k];
+      s.d[k] = (float)ld[k];
+      s.sign[k] = s.d[k] < 0.0f ? 1 : 0;
+      s.inv[k] = 1.0f / s.d[k];
+    }
+    s.sbits = (uint32_t)s.sign[0] | ((uint32_t)s.sign[1] << 1) | ((uint32_t)s.sign[2] << 2);
+    float t, u, v;
+    uint32_t prim;
+    tri_traverse<false, kBlock>(e->nodes, e->prims, e->max_nodes, e->nprims, s, ds, e->kEPS,
+                                SRF_NO_PRIM, FLT_MAX, bst, t, u, v, prim);
+    if (prim == SRF_NO_PRIM) continue;
+    // world distance of the local hit point (real3 arithmetic in float)
+    double lp[3], wp[3];
+    for (int k = 0; k < 3; k++) lp[k] = (double)(float)(lo[k] + (double)t * ld[k]);
+    mult_rows(e->T, lp, wp);
+    const float px = (float)wp[0] - wo[0], py = (float)wp[1] - wo[1], pz = (float)wp[2] - wo[2];
+    const float tw = sqrtf((px * px + py * py) + pz * pz);
+    if ((double)tw < nearest) {
+      nearest = tw;
+      bt = tw;
+      bu = u;
+      bv = v;
+      bprim = prim;
+      binst = nid;
+    }
+  }
+  reinterpret_cast<float4 *>(hits)[i] = make_float4(bt, bu, bv, __uint_as_float(bprim));
+  if (inst_out) inst_out[i] = binst;
+}
+
+// One lane per ray; with `counter` the launch is persistent: each wavefront
+// takes kChunk consecutive rays (one 8x8 tile) at a time until none is left.
+template <int kBlock>
+__global__ void __launch_bounds__(kBlock)
+k_scene_tri(const TlasNode *__restrict__ tl, uint32_t nnodes, const SceneInst *__restrict__ inst,
+            uint32_t ninst, const srf_ray *__restrict__ rays, uint32_t n,
+            srf_hit *__restrict__ hits, uint32_t *__restrict__ inst_out, uint32_t tdepth,
+            uint32_t *counter) {
+  extern __shared__ uint32_t smem[];
+  uint32_t *bst = smem + threadIdx.x;
+  uint32_t *tst = smem + kTraceStack * kBlock + threadIdx.x;
+  if (!counter) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) scene_ray<kBlock>(tl, nnodes, inst, ninst, rays, i, bst, tst, tdepth, hits, inst_out);
+    return;
+  }
+  const uint32_t lane = threadIdx.x & 63u;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(counter, kChunk);
+    base = __shfl(base, 0, 64);
+    if (base >= n) break;
+    const uint32_t i = base + lane;
+    if (i < n) scene_ray<kBlock>(tl, nnodes, inst, ninst, rays, i, bst, tst, tdepth, hits, inst_out);
+  }
+}
+
+__global__ void k_zero_counter(uint32_t *c) { *c = 0u; }
+
+}  // namespace
+
+// ------------------------------------------------------------------ API
+
+srf_status scene_create(Ctx *ctx, uint32_t n, const srf_instance *inst, Scene **out);
+srf_status scene_trace(const Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
+                       uint32_t *inst_out, int persistent, hipStream_t stream);
+void scene_free(Scene *sc);
+int scene_device(const Scene *sc);
+srf_status scene_query(const Scene *sc, double bmin[3], double bmax[3], srf_tlas_node *nodes,
+                       uint32_t *count);
+srf_status fetch_params_for(const Accel *a, BuildParams *p);
+
+srf_status scene_create(Ctx *ctx, uint32_t n, const srf_instance *inst, Scene **out) {
+  std::vector<Item> items(n);
+  std::vector<SceneInst> hinst(n);
+  for (uint32_t i = 0; i < n; i++) {
+    const Accel *a = reinterpret_cast<const Accel *>(inst[i].accel);
+    if (!a) return set_error(SRF_ERR_INVALID_ARG, "instance %u: null accelerator", i);
+    if (a->kind != kTri)
+      return set_error(SRF_ERR_UNSUPPORTED, "instance %u: scenes hold triangle accelerators", i);
+    if (a->device != ctx->device)
+      return set_error(SRF_ERR_INVALID_ARG, "instance %u: accelerator on another device", i);
+    BuildParams p;
+    srf_status st = fetch_params_for(a, &p);
+    if (st != SRF_OK) return st;
+    // BoundingBox() of the element, then its world box (RenderElement::Update)
+    const double bmin[3] = {p.root_bmin[0], p.root_bmin[1], p.root_bmin[2]};
+    const double bmax[3] = {p.root_bmax[0], p.root_bmax[1], p.root_bmax[2]};
+    world_box(bmin, bmax, inst[i].T, items[i].bmin, items[i].bmax);
+    items[i].id = i;
+    SceneInst &d = hinst[i];
+    d.nodes = reinterpret_cast<const float4 *>(a->nodes);
+    d.prims = reinterpret_cast<const float4 *>(a->prims);
+    d.max_nodes = (uint32_t)(2 * a->n - 1);
+    d.nprims = (uint32_t)a->n;
+    d.kEPS = FLT_EPSILON * (float)p.eps_scale;
+    d.pad = 0;
+    for (int r = 0; r < 4; r++)
+      for (int c = 0; c < 3; c++) {
+        d.T[3 * r + c] = inst[i].T[4 * r + c];
+        d.invT[3 * r + c] = inst[i].invT[4 * r + c];
+        d.invT33[3 * r + c] = inst[i].invT33[4 * r + c];
+      }
+  }
+  Scene *sc = new (std::nothrow) Scene();
+  if (!sc) return set_error(SRF_ERR_OUT_OF_MEMORY, "host allocation failed");
+  sc->device = ctx->device;
+  sc->count = n;
+  sc->h_nodes = TlasBuilder(items).build();
+  sc->num_nodes = (uint32_t)sc->h_nodes.size();
+  sc->depth = branch_depth(sc->h_nodes, 0);
+  for (int k = 0; k < 3; k++) {
+    sc->bmin[k] = sc->h_nodes[0].bmin[k];
+    sc->bmax[k] = sc->h_nodes[0].bmax[k];
+  }
+  sc->d_nodes = nullptr;
+  sc->d_inst = nullptr;
+  sc->d_counter = nullptr;
+  hipError_t e = hipMalloc(&sc->d_nodes, sizeof(TlasNode) * sc->num_nodes);
+  if (e == hipSuccess) e = hipMalloc(&sc->d_inst, sizeof(SceneInst) * n);
+  if (e == hipSuccess) e = hipMalloc(&sc->d_counter, 256);
+  if (e == hipSuccess)
+    e = hipMemcpy(sc->d_nodes, sc->h_nodes.data(), sizeof(TlasNode) * sc->num_nodes,
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(sc->d_inst, hinst.data(), sizeof(SceneInst) * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    scene_free(sc);
+    return hip_error(e, "scene upload");
+  }
+  *out = sc;
+  return SRF_OK;
+}
+
+int scene_device(const Scene *sc) { return sc->device; }
+
+srf_status scene_query(const Scene *sc, double bmin[3], double bmax[3], srf_tlas_node *nodes,
+                       uint32_t *count) {
+  if (bmin && bmax)
+    for (int k = 0; k < 3; k++) {
+      bmin[k] = sc->bmin[k];
+      bmax[k] = sc->bmax[k];
+    }
+  if (count) *count = sc->num_nodes;
+  if (nodes)
+    for (uint32_t i = 0; i < sc->num_nodes; i++) {
+      const TlasNode &a = sc->h_nodes[i];
+      srf_tlas_node &b = nodes[i];
+      for (int k = 0; k < 3; k++) {
+        b.bmin[k] = a.bmin[k];
+        b.bmax[k] = a.bmax[k];
+      }
+      b.child[0] = a.child[0];
+      b.child[1] = a.child[1];
+      b.is_leaf = (int32_t)(a.axis_leaf >> 8);
+      b.axis = b.is_leaf ? 0 : (int32_t)(a.axis_leaf & 3u);
+      b.node_id = b.is_leaf ? a.id : 0u;
+      b.pad = 0;
+    }
+  return SRF_OK;
+}
+
+void scene_free(Scene *sc) {
+  if (!sc) return;
+  if (sc->d_nodes) (void)hipFree(sc->d_nodes);
+  if (sc->d_inst) (void)hipFree(sc->d_inst);
+  if (sc->d_counter) (void)hipFree(sc->d_counter);
+  delete sc;
+}
+
+srf_status scene_trace(const Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
+                       uint32_t *inst_out, int persistent, hipStream_t stream) {
+  if (n == 0) return SRF_OK;
+  const uint32_t tdepth = std::max<uint32_t>(sc->depth, 1u);
+  const size_t lds = sizeof(uint32_t) * (size_t)(kTraceStack + tdepth) * kSceneBlock;
+  if (lds > 64 * 1024)
+    return set_error(SRF_ERR_UNSUPPORTED, "top-level BVH %u levels deep exceeds the LDS stack",
+                     sc->depth);
+  uint32_t *counter = nullptr;
+  uint32_t blocks = (uint32_t)((n + kSceneBlock - 1) / kSceneBlock);
+  if (persistent) {
+    int dev = 0, cus = 0;
+    SRF_HIP(hipGetDevice(&dev));
+    SRF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t waves = (uint32_t)((n + kChunk - 1) / kChunk);
+    blocks = std::min<uint32_t>(blocks, (uint32_t)cus * 8u);
+    blocks = std::min<uint32_t>(blocks, (waves + (kSceneBlock / 64) - 1) / (kSceneBlock / 64));
+    counter = sc->d_counter;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, counter);
+  }
+  hipLaunchKernelGGL((k_scene_tri<kSceneBlock>), dim3(blocks), dim3(kSceneBlock), lds, stream,
+                     sc->d_nodes, sc->num_nodes, sc->d_inst, sc->count, rays, (uint32_t)n, hits,
+                     inst_out, tdepth, counter);
+  SRF_HIP(hipGetLastError());
+  return SRF_OK;
+}
+
+}  // namespace srf

@@ -28,120 +28,10 @@
 //   triangles  TriangleAccel::Traverse  render/render_accel_triangle.cc:2511-2669
 //   tetras     TetraAccel::Traverse     render/render_accel_tetra.cc:2200-2289
 //   spheres    ParticleAccel::Traverse  render/render_accel_particle.cc:3053-3218
-#include <float.h>
-#include <math.h>
-
-#include "srf_internal.h"
+#include "srf_trace_dev.h"
 
 namespace srf {
 namespace {
-
-constexpr int kTraceBlock = 128;
-constexpr int kTraceStack = kMaxBranchLevels;  // see above
-constexpr uint32_t kDone = 0xffffffffu;
-
-struct RaySetup {
-  float o[3], d[3], inv[3];
-  int sign[3];
-  uint32_t sbits;  // sign[k] << k, so the near child needs no indexed load
-};
-
-__device__ __forceinline__ RaySetup setup_ray(const srf_ray *rays, uint32_t i, uint32_t &flags,
-                                              uint32_t &prev) {
-  const float4 *r4 = reinterpret_cast<const float4 *>(rays + i);
-  float4 a = r4[0], b = r4[1];
-  RaySetup s;
-  s.o[0] = a.x;
-  s.o[1] = a.y;
-  s.o[2] = a.z;
-  s.d[0] = b.x;
-  s.d[1] = b.y;
-  s.d[2] = b.z;
-  flags = __float_as_uint(a.w);
-  prev = __float_as_uint(b.w);
-  for (int k = 0; k < 3; k++) {
-    s.sign[k] = s.d[k] < 0.0f ? 1 : 0;
-    // (float)(1.0 / dir): correctly rounded float division gives the same bits
-    s.inv[k] = 1.0f / s.d[k];
-  }
-  s.sbits = (uint32_t)s.sign[0] | ((uint32_t)s.sign[1] << 1) | ((uint32_t)s.sign[2] << 2);
-  return s;
-}
-
-// IntersectRayAABB, render/render_accel_triangle.cc:1815-1861 (two-box
-// nodes, tmax clamped to maxT) and render/render_accel_tetra.cc:1934-1978
-// (own box, tmin <= maxT).
-template <bool kClampTmax>
-__device__ __forceinline__ bool ray_aabb(const RaySetup &s, float maxT, float bx0, float by0,
-                                         float bz0, float bx1, float by1, float bz1) {
-  float mnx = s.sign[0] ? bx1 : bx0, mxx = s.sign[0] ? bx0 : bx1;
-  float mny = s.sign[1] ? by1 : by0, mxy = s.sign[1] ? by0 : by1;
-  float mnz = s.sign[2] ? bz1 : bz0, mxz = s.sign[2] ? bz0 : bz1;
-  float tminx = (mnx - s.o[0]) * s.inv[0];
-  float tmaxx = (mxx - s.o[0]) * s.inv[0];
-  float tminy = (mny - s.o[1]) * s.inv[1];
-  float tmaxy = (mxy - s.o[1]) * s.inv[1];
-  float tmin = (tminx > tminy) ? tminx : tminy;
-  float tmax = (tmaxx < tmaxy) ? tmaxx : tmaxy;
-  float tminz = (mnz - s.o[2]) * s.inv[2];
-  float tmaxz = (mxz - s.o[2]) * s.inv[2];
-  tmin = (tmin > tminz) ? tmin : tminz;
-  tmax = (tmax < tmaxz) ? tmax : tmaxz;
-  if (kClampTmax) {
-    tmax = (maxT < tmax) ? maxT : tmax;
-    return (tmax > 0.0f) && (tmin <= tmax);
-  }
-  return (tmax > 0.0f) && (tmin <= tmax) && (tmin <= maxT);
-}
-
-// ---------------------------------------------------------------- triangles
-
-// TriangleIsect (Moller-Trumbore), render/render_accel_triangle.cc:1887-1938,
-// on a packed triangle (p0, e1 = p1 - p0, e2 = p2 - p0). u, v and t are the
-// reference's expressions; the estimates ue/ve/te (numerator x hardware
-// reciprocal, within ~4e-7 relative of the exact values) only reject
-// triangles the exact comparisons would reject as well. kEPS >= 1e-30 keeps
-// det normal, so the reciprocal estimate is accurate.
-__device__ __forceinline__ bool tri_isect(const RaySetup &s, float4 A, float4 B, float4 C,
-                                          bool double_sided, float kEPS, bool filter,
-                                          float tin, float &tout, float &uout, float &vout) {
-  const float e1x = B.x, e1y = B.y, e1z = B.z, e2x = C.x, e2y = C.y, e2z = C.z;
-  const float dx = s.d[0], dy = s.d[1], dz = s.d[2];
-  float px = dy * e2z - dz * e2y;
-  float py = dz * e2x - dx * e2z;
-  float pz = dx * e2y - dy * e2x;
-  float det = (e1x * px + e1y * py) + e1z * pz;
-  if (double_sided) {
-    if (fabsf(det) < kEPS) return false;
-  } else {
-    if (det < kEPS) return false;
-  }
-  float sx = s.o[0] - A.x, sy = s.o[1] - A.y, sz = s.o[2] - A.z;
-  float un = (sx * px + sy * py) + sz * pz;
-  const float rc = __builtin_amdgcn_rcpf(det);
-  const float ue = un * rc;
-  if (filter && (ue < -1e-5f || ue > 1.00001f)) return false;
-  float qx = sy * e1z - sz * e1y;
-  float qy = sz * e1x - sx * e1z;
-  float qz = sx * e1y - sy * e1x;
-  float vn = (qx * dx + qy * dy) + qz * dz;
-  const float ve = vn * rc;
-  if (filter && (ve < -1e-5f || ue + ve > 1.00001f)) return false;
-  float tn = (e2x * qx + e2y * qy) + e2z * qz;
-  const float te = tn * rc;
-  if (filter && (te < -1e-30f || te > tin * 1.00001f)) return false;
-  float inv_det = 1.0f / det;  // correctly rounded: equals (float)(1.0 / det)
-  float u = un * inv_det;
-  float v = vn * inv_det;
-  float t = tn * inv_det;
-  if (u < 0.0f || u > 1.0f) return false;
-  if (v < 0.0f || u + v > 1.0f) return false;
-  if (t < 0.0f || t > tin) return false;
-  tout = t;
-  uout = u;
-  vout = v;
-  return true;
-}
 
 template <bool kAny>
 __global__ void __launch_bounds__(kTraceBlock)
@@ -156,70 +46,14 @@ k_trace_tri(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   const RaySetup s = setup_ray(rays, i, flags, prev);
   const bool ds = (flags & SRF_RAY_DOUBLE_SIDED) != 0;
   const float kEPS = FLT_EPSILON * (float)params->eps_scale;
-  const bool filter = kEPS >= 1e-30f;
   const float tlimit = kAny ? (tmax_in ? tmax_in[i] : FLT_MAX) : FLT_MAX;
-  float t = kAny ? tlimit : FLT_MAX, u = 0.0f, v = 0.0f;
-  uint32_t prim = SRF_NO_PRIM;
-  uint32_t *st = stack + threadIdx.x;
-  int sp = 0;
-  uint32_t cur = 0;
-  uint4 meta;
-  for (;;) {
-    // descend until this lane holds a leaf (or has nothing left)
-    for (;;) {
-      if (cur >= max_nodes) {
-        cur = kDone;
-        break;
-      }
-      const float4 *nd = nodes + 4 * (size_t)cur;
-      meta = reinterpret_cast<const uint4 *>(nd)[3];
-      if (meta.x != 0) break;
-      const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
-      const bool h0 = ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
-      const bool h1 = ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
-      const int nearc = (s.sbits >> meta.y) & 1u;
-      const bool hn = nearc ? h1 : h0, hf = nearc ? h0 : h1;
-      const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
-      if (hn) {
-        if (hf && sp < kTraceStack) st[(sp++) * kTraceBlock] = cf;
-        cur = cn;
-      } else if (hf) {
-        cur = cf;
-      } else if (sp == 0) {
-        cur = kDone;
-        break;
-      } else {
-        cur = st[(--sp) * kTraceBlock];
-      }
-    }
-    if (cur == kDone) break;
-    // leaf: TestLeafNode, render/render_accel_triangle.cc:2386-2433
-    const uint32_t first = meta.w, end = min(meta.w + meta.z, nprims);
-    for (uint32_t q = first; q < end; q++) {
-      const float4 *P = prims + 3 * (size_t)q;
-      const float4 A = P[0];
-      if (__float_as_uint(A.w) == prev) continue;
-      const float4 B = P[1], C = P[2];
-      float tt, uu, vv;
-      if (tri_isect(s, A, B, C, ds, kEPS, filter, t, tt, uu, vv)) {
-        if (kAny) {
-          if (tt < tlimit) {
-            occluded[i] = 1u;
-            return;
-          }
-        } else {
-          t = tt;
-          u = uu;
-          v = vv;
-          prim = __float_as_uint(A.w);
-        }
-      }
-    }
-    if (sp == 0) break;
-    cur = st[(--sp) * kTraceBlock];
-  }
+  float t, u, v;
+  uint32_t prim;
+  const bool occ = tri_traverse<kAny, kTraceBlock>(nodes, prims, max_nodes, nprims, s, ds, kEPS,
+                                                   prev, tlimit, stack + threadIdx.x, t, u, v,
+                                                   prim);
   if (kAny) {
-    occluded[i] = 0u;
+    occluded[i] = occ ? 1u : 0u;
   } else {
     reinterpret_cast<float4 *>(hits)[i] = make_float4(t, u, v, __uint_as_float(prim));
   }

@@ -24,7 +24,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from oracle.oracle import Reference, canonical_sph, canonical_tet, canonical_tri  # noqa: E402
+from oracle.oracle import (Reference, canonical_sph, canonical_tet, canonical_tlas,  # noqa: E402
+                           canonical_tri)
 from surface_amd import workloads as W  # noqa: E402
 
 
@@ -97,17 +98,43 @@ def karras_dup_case(ref):
             "codes_same": same, "info_same": ref.radix_tree(same)}
 
 
+SCENE_ARGS = dict(count=12, lo=1100, hi=6000, seed=W.BASE_SEED + 21)
+
+
+def scene_case(ref):
+    """RenderGraph of 12 triangle elements (batch_scene(**SCENE_ARGS)): the
+    reference's inverse matrices, world boxes, top-level BVH, and the hits of
+    two orbit cameras (64x64)."""
+    meshes, T = W.batch_scene(**SCENE_ARGS)
+    trees = [ref.build("tri", V, F, keep=True) for V, F in meshes]
+    sc = ref.scene_build(trees, T)
+    cams = W.scene_cameras(T, count=2, width=64, height=64)
+    rays = np.concatenate(cams)
+    hits, inst = ref.scene_trace(sc, rays)
+    ref.scene_free(sc)
+    for t in trees:
+        ref.free(t)
+    return dict(T=T, invT=sc["invT"], invT33=sc["invT33"], xbox=sc["xbox"],
+                tlas=canonical_tlas(sc["tlas"]), rays=rays, hits=hits, inst=inst,
+                meshes_sha=np.array(sha(*[a for m in meshes for a in m])))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--full", action="store_true", help="also write full-size digests")
+    ap.add_argument("--only", choices=["scene"], help="write one fixture only")
     args = ap.parse_args()
     ref = Reference("serial")
     t0 = time.time()
+    if args.only == "scene":
+        np.savez_compressed(os.path.join(HERE, "scene_small.npz"), **scene_case(ref))
+        return
     np.savez_compressed(os.path.join(HERE, "tri_small.npz"), **tri_case(ref, 48, 64))
     np.savez_compressed(os.path.join(HERE, "tet_small.npz"), **tet_case(ref, 6, 64))
     np.savez_compressed(os.path.join(HERE, "sph_small.npz"), **sph_case(ref, 6000, 64))
     np.savez_compressed(os.path.join(HERE, "tex3d.npz"), **tex_case(ref))
     np.savez_compressed(os.path.join(HERE, "karras.npz"), **karras_dup_case(ref))
+    np.savez_compressed(os.path.join(HERE, "scene_small.npz"), **scene_case(ref))
     print(f"small fixtures written in {time.time() - t0:.1f}s", flush=True)
     if not args.full:
         return

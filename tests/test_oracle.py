@@ -530,3 +530,129 @@ def test_volume_forward_closed_form(orc):
     assert np.abs(got - want).max() < 2e-6
     # f64 path: float sample geometry and float32-rounded voxels/TF entries
     assert np.abs(got64 - want).max() < 1e-7
+
+
+# ------------------------------------------------------------------ scenes
+
+def _scene_elements(orc, meshes, T, invT, invT33):
+    trees = [orc.build_tri(V, F) for V, F in meshes]
+    els = [dict(tree=trees[i], V=meshes[i][0], F=meshes[i][1], T=T[i], invT=invT[i],
+                invT33=invT33[i]) for i in range(len(meshes))]
+    return trees, els
+
+
+def test_scene_oracle_matches_reference_fixture(orc):
+    from oracle.oracle import canonical_tlas
+    import hashlib
+    g = np.load(os.path.join(GOLD, "scene_small.npz"), allow_pickle=False)
+    meshes, T = W.batch_scene(count=12, lo=1100, hi=6000, seed=W.BASE_SEED + 21)
+    h = hashlib.sha256()
+    for m in meshes:
+        for a in m:
+            h.update(np.ascontiguousarray(a).tobytes())
+    assert h.hexdigest() == str(g["meshes_sha"]), "scene generator drifted from the fixture"
+    assert np.array_equal(T, g["T"])
+    trees, els = _scene_elements(orc, meshes, T, g["invT"], g["invT33"])
+    xb = np.array([orc.transform_box(t["bounds"][:3], t["bounds"][3:], T[i])
+                   for i, t in enumerate(trees)])
+    assert np.array_equal(xb, g["xbox"])
+    tl = orc.tlas_build(xb)
+    assert canonical_tlas(tl).tobytes() == g["tlas"].tobytes()
+    hits, inst = orc.scene_trace(tl, els, g["rays"])
+    assert hits.tobytes() == g["hits"].tobytes()
+    assert np.array_equal(inst, g["inst"])
+
+
+def test_scene_transform_box_closed_forms(orc):
+    bmin, bmax = np.array([-1.0, 0.5, 2.0]), np.array([3.0, 1.5, 2.25])
+    I = np.eye(4)
+    assert np.array_equal(orc.transform_box(bmin, bmax, I), np.concatenate([bmin, bmax]))
+    M = np.eye(4)
+    M[3, :3] = [10.0, -4.0, 0.5]  # translation row (row-vector convention)
+    assert np.array_equal(orc.transform_box(bmin, bmax, M),
+                          np.concatenate([bmin + M[3, :3], bmax + M[3, :3]]))
+    R = np.zeros((4, 4))  # p' = p R: x -> y, y -> -x, z -> z (90 degrees about z)
+    R[0, 1], R[1, 0], R[2, 2], R[3, 3] = 1.0, -1.0, 1.0, 1.0
+    out = orc.transform_box(bmin, bmax, R)
+    assert np.array_equal(out, [-1.5, -1.0, 2.0, -0.5, 3.0, 2.25])
+
+
+def test_tlas_invariants(orc):
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 3, 17, 64):
+        lo = rng.uniform(-5, 5, (n, 3))
+        boxes = np.concatenate([lo, lo + rng.uniform(0.1, 2, (n, 3))], 1)
+        if n == 17:
+            boxes[5:9] = boxes[4]  # identical boxes force the object-median fallback
+        tl = orc.tlas_build(boxes)
+        leaves = tl[tl["is_leaf"] == 1]
+        assert sorted(leaves["node_id"].tolist()) == list(range(n))
+        assert len(tl) == 2 * n - 1
+        # each leaf's box contains its element's box; children inside parents
+        for k, nd in enumerate(tl):
+            if nd["is_leaf"]:
+                b = boxes[nd["node_id"]]
+                assert np.all(nd["bmin"] <= b[:3]) and np.all(nd["bmax"] >= b[3:])
+            else:
+                for c in nd["child"]:
+                    if c > 0 and k > 0:
+                        assert np.all(tl[c]["bmin"] >= nd["bmin"]) and np.all(tl[c]["bmax"] <= nd["bmax"])
+
+
+def test_scene_trace_equals_bruteforce_over_elements(orc):
+    """The ordered, culled visit of elements gives the nearest world hit of
+    all elements, each traced on its own."""
+    meshes, T = W.batch_scene(count=10, lo=1100, hi=5000, seed=W.BASE_SEED + 22)
+    invT = np.linalg.inv(T)
+    invT33 = np.array([np.linalg.inv(np.block([[M[:3, :3], np.zeros((3, 1))],
+                                               [np.zeros((1, 3)), np.ones((1, 1))]]))
+                       for M in T])
+    trees, els = _scene_elements(orc, meshes, T, invT, invT33)
+    xb = np.array([orc.transform_box(t["bounds"][:3], t["bounds"][3:], T[i])
+                   for i, t in enumerate(trees)])
+    tl = orc.tlas_build(xb)
+    rays = W.scene_cameras(T, count=1, width=48, height=48)[0]
+    hits, inst = orc.scene_trace(tl, els, rays)
+    best = np.full(len(rays), np.inf)
+    for i, e in enumerate(els):
+        lr = rays.copy()
+        o = rays["org"].astype(np.float64)
+        d = rays["dir"].astype(np.float64)
+        lo = o @ e["invT"][:3, :3] + e["invT"][3, :3]
+        ld = d @ e["invT33"][:3, :3] + e["invT33"][3, :3]
+        lr["org"], lr["dir"] = lo.astype(np.float32), ld.astype(np.float32)
+        lr["prev_prim_id"] = W.NO_PRIM
+        h = orc.trace_tri(e["tree"], e["V"], e["F"], lr)
+        ok = h["prim_id"] != W.NO_PRIM
+        lp = (lo + h["t"].astype(np.float64)[:, None] * ld).astype(np.float32).astype(np.float64)
+        wp = (lp @ e["T"][:3, :3] + e["T"][3, :3]).astype(np.float32)
+        with np.errstate(over="ignore", invalid="ignore"):  # misses carry t = FLT_MAX
+            tw = np.sqrt(np.sum((wp - rays["org"]) ** 2, axis=1, dtype=np.float32))
+        best = np.where(ok & (tw < best), tw, best)
+    hit = inst != W.NO_PRIM
+    assert hit.mean() > 0.2
+    assert np.array_equal(np.isfinite(best), hit)
+    assert np.allclose(hits["t"][hit], best[hit], rtol=1e-6, atol=0)
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference build not present")
+def test_scene_oracle_equals_live_reference(orc):
+    from oracle.oracle import Reference, canonical_tlas
+    ref = Reference("serial")
+    meshes, T = W.batch_scene(count=64, lo=1100, hi=4000, seed=W.BASE_SEED + 23)
+    rtrees = [ref.build("tri", V, F, keep=True) for V, F in meshes]
+    sc = ref.scene_build(rtrees, T)
+    trees, els = _scene_elements(orc, meshes, T, sc["invT"], sc["invT33"])
+    xb = np.array([orc.transform_box(t["bounds"][:3], t["bounds"][3:], T[i])
+                   for i, t in enumerate(trees)])
+    assert np.array_equal(xb, sc["xbox"])
+    tl = orc.tlas_build(xb)
+    assert canonical_tlas(tl).tobytes() == canonical_tlas(sc["tlas"]).tobytes()
+    rays = W.scene_cameras(T, count=2, width=96, height=96)
+    for r in rays:
+        h1, i1 = ref.scene_trace(sc, r)
+        h2, i2 = orc.scene_trace(tl, els, r, threads=4)
+        assert h1.tobytes() == h2.tobytes() and np.array_equal(i1, i2)
+    ref.scene_free(sc)
+    for t in rtrees:
+        ref.free(t)
