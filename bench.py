@@ -114,6 +114,70 @@ def traffic_probe(torch, S, W, build, trace):
     log(f"traffic probe done: copy of {src.numel() * 4} bytes")
 
 
+def batched_scene(torch, S, W):
+    """64 meshes (log-uniform 10k..2M triangles) built by one segmented LBVH
+    build, placed by rigid transforms under a top-level BVH, traced from 16
+    orbit cameras of 1024x1024 rays: plain, persistent and autotuned
+    launches over all 16M rays, the tuned launch captured in a hipGraph, and
+    one launch per camera for comparison."""
+    log("batched scene: 64 meshes, 16 cameras")
+    meshes, T = W.batch_scene(64)
+    ntris = int(sum(len(F) for _, F in meshes))
+    dm = [(S.device_tensor(V), S.device_tensor(F)) for V, F in meshes]
+    accs = S.build_triangles_batch(dm)
+    torch.cuda.synchronize()
+    batch_ms = timed_events(torch, lambda: S.build_triangles_batch(dm, accels=accs), 5)
+
+    def loop_build():
+        for a, (Vd, Fd) in zip(accs, dm):
+            a.Build32(Vd, Fd)
+
+    loop_build()
+    loop_ms = timed_events(torch, loop_build, 3)
+    S.build_triangles_batch(dm, accels=accs)
+    t0 = time.perf_counter()
+    scene = S.Scene(accs, T)
+    scene_ms = (time.perf_counter() - t0) * 1e3
+    cams = W.scene_cameras(T, count=16, width=1024, height=1024)
+    rays = [S.device_tensor(c) for c in cams]
+    nr = sum(len(c) for c in cams)
+    allrays = torch.cat(rays)
+    hits = torch.empty((nr, 4), dtype=torch.float32, device="cuda")
+    ids = torch.empty(nr, dtype=torch.int32, device="cuda")
+    out = {"meshes": len(meshes), "triangles": ntris, "rays": nr,
+           "batch_build_ms": round(batch_ms, 3),
+           "batch_build_mprims_per_s": round(ntris / batch_ms / 1e3, 1),
+           "per_mesh_build_loop_ms": round(loop_ms, 3), "scene_create_ms": round(scene_ms, 3)}
+    out["autotuned"] = scene.autotune(allrays, hits, ids)
+    for mode in ("plain", "persistent", "tuned"):
+        scene.Trace(allrays, hits, ids, mode=mode)
+        ms = timed_events(torch, lambda: scene.Trace(allrays, hits, ids, mode=mode), 3)
+        out[f"trace_{mode}_ms"] = round(ms, 3)
+        out[f"trace_{mode}_mrays_per_s"] = round(nr / ms / 1e3, 1)
+    # all 16 cameras in one tuned launch captured in a hipGraph and replayed
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        scene.Trace(allrays, hits, ids, mode="tuned")
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        scene.Trace(allrays, hits, ids, mode="tuned")
+    graph.replay()
+    torch.cuda.synchronize()
+    ms = timed_events(torch, graph.replay, 3)
+    out["trace_graph_16_cameras_ms"] = round(ms, 3)
+    out["trace_graph_mrays_per_s"] = round(nr / ms / 1e3, 1)
+    # one camera per launch, for comparison
+    ms = timed_events(torch, lambda: [scene.Trace(r, hits[:len(r)], ids[:len(r)], mode="tuned")
+                                      for r in rays], 2)
+    out["trace_per_camera_launches_ms"] = round(ms, 3)
+    out["hit_fraction"] = round(float((ids != -1).float().mean().item()), 4)
+    del graph, scene, accs
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +278,9 @@ def main():
         fb = timed_events(torch, fwd_bwd, 2)
         extras["volume_256cubed_1M_rays"] = {"fwd_ms": round(fwd, 3), "fwd_bwd_ms": round(fb, 3),
                                              "fwd_bwd_over_fwd": round(fb / fwd, 3)}
+
+    if not args.no_extras:
+        extras["batched_scene_64_meshes_16_cameras"] = batched_scene(torch, S, W)
 
     cpu = None if args.no_cpu else cpu_baseline(V, F, Sc, rays_np)
 

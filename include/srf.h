@@ -291,15 +291,38 @@ srf_status srf_scene_bounds(const srf_scene *scene, double bmin_host[3], double 
 /* The top-level BVH: *count nodes; nodes_host (NULL to query the count)
  * receives them in BVHTree order. */
 srf_status srf_scene_export(const srf_scene *scene, srf_tlas_node *nodes_host, uint32_t *count);
+/* Launch shape of the scene trace. PLAIN: one lane per ray, one block per
+ * block_size rays. PERSISTENT: blocks_per_cu resident blocks per compute
+ * unit, each wavefront taking 64-ray tiles from a work counter until none is
+ * left (the counter is reset by a kernel, so the launch can be captured in a
+ * hipGraph). TUNED: the configuration srf_scene_autotune picked (before any
+ * tuning: persistent, 128 threads, 8 blocks per CU). */
+typedef enum {
+  SRF_LAUNCH_TUNED = 0,
+  SRF_LAUNCH_PLAIN = 1,
+  SRF_LAUNCH_PERSISTENT = 2
+} srf_launch_mode;
+
+typedef struct {
+  uint32_t block_size;    /* 64, 128 or 256 */
+  uint32_t persistent;    /* 0 or 1 */
+  uint32_t blocks_per_cu; /* persistent grid: resident blocks per CU */
+  float ms;               /* measured time of the tuning batch */
+} srf_launch_config;
+
 /* RenderGraph::Trace (gles/gles_render_graph.cc:145-314) for a batch of
  * rays: hits[i] = {world distance, u, v, prim_id} of the nearest element
  * hit, instance_ids[i] its element (~0 on a miss; may be NULL). The ray's
  * prev_prim_id is not used (no element is the ray's previous one).
- * persistent != 0 runs a persistent grid that hands out 64-ray tiles from a
- * work counter (graph-capturable: the counter is reset by a kernel). */
+ * mode: srf_launch_mode. */
 srf_status srf_scene_trace(const srf_scene *scene, const srf_ray *rays, uint64_t n,
-                           srf_hit *hits, uint32_t *instance_ids, int persistent,
-                           void *stream);
+                           srf_hit *hits, uint32_t *instance_ids, int mode, void *stream);
+/* Autotuner: traces the given rays with every launch configuration (block
+ * size 64/128/256, plain or persistent with 2/4/8/16 blocks per CU), keeps
+ * the fastest for SRF_LAUNCH_TUNED and reports it. Synchronises. No
+ * counterpart in the reference (its CPU loop has no launch shape). */
+srf_status srf_scene_autotune(srf_scene *scene, const srf_ray *rays, uint64_t n, srf_hit *hits,
+                              uint32_t *instance_ids, void *stream, srf_launch_config *best);
 
 /* glTexImage3D / lsglTexImage3DPointer for a single-component FP32 volume
  * (gles/gles_texture.cc:198-357); wrap_* = 0 REPEAT (the glTexImage3D

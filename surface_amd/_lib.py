@@ -56,6 +56,15 @@ class Instance(C.Structure):
                 ("invT33", C.c_double * 16)]
 
 
+class LaunchConfig(C.Structure):
+    """srf_launch_config."""
+    _fields_ = [("block_size", C.c_uint32), ("persistent", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32), ("ms", C.c_float)]
+
+
+LAUNCH_TUNED, LAUNCH_PLAIN, LAUNCH_PERSISTENT = 0, 1, 2
+
+
 class TlasNode(C.Structure):
     """srf_tlas_node (72 bytes)."""
     _fields_ = [("bmin", C.c_double * 3), ("bmax", C.c_double * 3), ("child", C.c_int32 * 2),
@@ -91,6 +100,7 @@ SIGNATURES = {
     "srf_scene_bounds": (_i, [_vp, _vp, _vp]),
     "srf_scene_export": (_i, [_vp, _vp, C.POINTER(C.c_uint32)]),
     "srf_scene_trace": (_i, [_vp, _vp, _u64, _vp, _vp, _i, _vp]),
+    "srf_scene_autotune": (_i, [_vp, _vp, _u64, _vp, _vp, _vp, _vp]),
     "srf_volume_create": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, C.POINTER(_vp)]),
     "srf_volume_destroy": (_i, [_vp]),
     "srf_volume_sample": (_i, [_vp, _vp, _u64, _vp, _vp]),

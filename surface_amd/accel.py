@@ -360,15 +360,30 @@ class Scene:
             ("bmin", "<f8", (3,)), ("bmax", "<f8", (3,)), ("child", "<i4", (2,)),
             ("axis", "<i4"), ("is_leaf", "<i4"), ("node_id", "<u4"), ("pad", "<u4")])).copy()
 
-    def Trace(self, rays, hits=None, instance_ids=None, persistent=True, stream=None):
+    def Trace(self, rays, hits=None, instance_ids=None, mode="tuned", stream=None):
         """Closest hit of every ray over all elements: hits [n, 4] (world t,
-        u, v, prim id bits) and instance_ids [n] int32 (-1 on a miss)."""
+        u, v, prim id bits) and instance_ids [n] int32 (-1 on a miss).
+        mode: "tuned" (the autotuned launch), "plain" or "persistent"."""
         n = rays.shape[0]
         if hits is None:
             hits = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
         if instance_ids is None:
             instance_ids = torch.empty(n, dtype=torch.int32, device=rays.device)
+        m = {"tuned": _lib.LAUNCH_TUNED, "plain": _lib.LAUNCH_PLAIN,
+             "persistent": _lib.LAUNCH_PERSISTENT}[mode]
         check(self.lib.srf_scene_trace(self.handle, _ptr(rays), n, _ptr(hits),
-                                       _ptr(instance_ids), 1 if persistent else 0,
-                                       _stream(stream)))
+                                       _ptr(instance_ids), m, _stream(stream)))
         return hits, instance_ids
+
+    def autotune(self, rays, hits=None, instance_ids=None, stream=None):
+        """Picks the fastest launch configuration on these rays; returns it."""
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+        if instance_ids is None:
+            instance_ids = torch.empty(n, dtype=torch.int32, device=rays.device)
+        cfg = _lib.LaunchConfig()
+        check(self.lib.srf_scene_autotune(self.handle, _ptr(rays), n, _ptr(hits),
+                                          _ptr(instance_ids), _stream(stream), C.byref(cfg)))
+        return {"block_size": cfg.block_size, "persistent": bool(cfg.persistent),
+                "blocks_per_cu": cfg.blocks_per_cu, "ms": cfg.ms}

@@ -175,7 +175,9 @@ srf_status fetch_params_for(const Accel *a, BuildParams *p) { return fetch_param
 struct Scene;
 srf_status scene_create(Ctx *ctx, uint32_t n, const srf_instance *inst, Scene **out);
 srf_status scene_trace(const Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
-                       uint32_t *inst_out, int persistent, hipStream_t stream);
+                       uint32_t *inst_out, int mode, hipStream_t stream);
+srf_status scene_autotune(Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
+                          uint32_t *inst_out, hipStream_t stream, srf_launch_config *best);
 void scene_free(Scene *sc);
 srf_status scene_query(const Scene *sc, double bmin[3], double bmax[3], srf_tlas_node *nodes,
                        uint32_t *count);
@@ -584,13 +586,22 @@ srf_status srf_scene_export(const srf_scene *scene, srf_tlas_node *nodes, uint32
 }
 
 srf_status srf_scene_trace(const srf_scene *scene, const srf_ray *rays, uint64_t n,
-                           srf_hit *hits, uint32_t *inst_out, int persistent, void *stream) {
+                           srf_hit *hits, uint32_t *inst_out, int mode, void *stream) {
   const Scene *sc = reinterpret_cast<const Scene *>(scene);
   if (!sc || (n && (!rays || !hits))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
   if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many rays in one call");
   DeviceGuard g(scene_device(sc));
-  return scene_trace(sc, rays, n, hits, inst_out, persistent,
-                     reinterpret_cast<hipStream_t>(stream));
+  return scene_trace(sc, rays, n, hits, inst_out, mode, reinterpret_cast<hipStream_t>(stream));
+}
+
+srf_status srf_scene_autotune(srf_scene *scene, const srf_ray *rays, uint64_t n, srf_hit *hits,
+                              uint32_t *inst_out, void *stream, srf_launch_config *best) {
+  Scene *sc = reinterpret_cast<Scene *>(scene);
+  if (!sc || !rays || !hits || n == 0) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many rays in one call");
+  DeviceGuard g(scene_device(sc));
+  return scene_autotune(sc, rays, n, hits, inst_out, reinterpret_cast<hipStream_t>(stream),
+                        best);
 }
 
 }  // extern "C"

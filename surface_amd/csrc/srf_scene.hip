@@ -67,11 +67,12 @@ struct Scene {
   SceneInst *d_inst;
   uint32_t *d_counter;  // work counter of the persistent launch
   std::vector<TlasNode> h_nodes;
+  int cus;              // compute units of the device
+  srf_launch_config tuned;
 };
 
 namespace {
 
-constexpr int kSceneBlock = 128;
 constexpr uint32_t kChunk = 64;  // rays a wavefront takes per grab
 
 // ------------------------------------------------------------------ host
@@ -231,6 +232,7 @@ uint32_t branch_depth(const std::vector<TlasNode> &nodes, uint32_t node) {
 struct TlasRay {
   double o[3], inv[3];
   int sign[3];
+  uint32_t sbits;  // sign[k] << k: the near child without an indexed load
 };
 
 // SetupRay, render/render_bvh_tree.cc:40-71 (directions below 2e-15 get
@@ -241,6 +243,7 @@ __device__ __forceinline__ void tlas_setup(const double o[3], const double d[3],
     r.sign[k] = d[k] < 0.0 ? 1 : 0;
     r.inv[k] = fabs(d[k]) > 2.0e-15 ? 1.0 / d[k] : (d[k] < 0.0 ? -FLT_MAX : FLT_MAX);
   }
+  r.sbits = (uint32_t)r.sign[0] | ((uint32_t)r.sign[1] << 1) | ((uint32_t)r.sign[2] << 2);
 }
 
 // TestRayAABB, render/render_bvh_tree.cc:198-255
@@ -315,9 +318,10 @@ __device__ __forceinline__ bool next_element(const TlasNode *__restrict__ tl, ui
       const bool hr = nd.child[1] > 0 && (uint32_t)nd.child[1] < nnodes &&
                       tlas_box(r, load_tlas(tl, (uint32_t)nd.child[1]), b);
       if (hl && hr) {
-        const int o = r.sign[nd.axis_leaf & 3u];
-        if (sp < tdepth) tst[(sp++) * kBlock] = (uint32_t)nd.child[1 - o];
-        node = (uint32_t)nd.child[o];
+        const int o = (int)((r.sbits >> (nd.axis_leaf & 3u)) & 1u);
+        const uint32_t c0 = (uint32_t)nd.child[0], c1 = (uint32_t)nd.child[1];
+        if (sp < tdepth) tst[(sp++) * kBlock] = o ? c0 : c1;
+        node = o ? c1 : c0;
       } else if (hl) {
         node = (uint32_t)nd.child[0];
       } else if (hr) {
@@ -439,7 +443,7 @@ __global__ void k_zero_counter(uint32_t *c) { *c = 0u; }
 
 srf_status scene_create(Ctx *ctx, uint32_t n, const srf_instance *inst, Scene **out);
 srf_status scene_trace(const Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
-                       uint32_t *inst_out, int persistent, hipStream_t stream);
+                       uint32_t *inst_out, int mode, hipStream_t stream);
 void scene_free(Scene *sc);
 int scene_device(const Scene *sc);
 srf_status scene_query(const Scene *sc, double bmin[3], double bmax[3], srf_tlas_node *nodes,
@@ -492,7 +496,11 @@ srf_status scene_create(Ctx *ctx, uint32_t n, const srf_instance *inst, Scene **
   sc->d_nodes = nullptr;
   sc->d_inst = nullptr;
   sc->d_counter = nullptr;
-  hipError_t e = hipMalloc(&sc->d_nodes, sizeof(TlasNode) * sc->num_nodes);
+  sc->cus = 256;
+  sc->tuned = srf_launch_config{128u, 1u, 8u, 0.0f};
+  hipError_t e = hipDeviceGetAttribute(&sc->cus, hipDeviceAttributeMultiprocessorCount,
+                                       ctx->device);
+  if (e == hipSuccess) e = hipMalloc(&sc->d_nodes, sizeof(TlasNode) * sc->num_nodes);
   if (e == hipSuccess) e = hipMalloc(&sc->d_inst, sizeof(SceneInst) * n);
   if (e == hipSuccess) e = hipMalloc(&sc->d_counter, 256);
   if (e == hipSuccess)
@@ -544,30 +552,101 @@ void scene_free(Scene *sc) {
   delete sc;
 }
 
-srf_status scene_trace(const Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
-                       uint32_t *inst_out, int persistent, hipStream_t stream) {
+srf_status scene_launch(const Scene *sc, const srf_launch_config &cfg, const srf_ray *rays,
+                        uint64_t n, srf_hit *hits, uint32_t *inst_out, hipStream_t stream) {
   if (n == 0) return SRF_OK;
+  const uint32_t block = cfg.block_size;
+  if (block != 64 && block != 128 && block != 256)
+    return set_error(SRF_ERR_INVALID_ARG, "block size %u (64, 128 or 256)", block);
   const uint32_t tdepth = std::max<uint32_t>(sc->depth, 1u);
-  const size_t lds = sizeof(uint32_t) * (size_t)(kTraceStack + tdepth) * kSceneBlock;
+  const size_t lds = sizeof(uint32_t) * (size_t)(kTraceStack + tdepth) * block;
   if (lds > 64 * 1024)
     return set_error(SRF_ERR_UNSUPPORTED, "top-level BVH %u levels deep exceeds the LDS stack",
                      sc->depth);
   uint32_t *counter = nullptr;
-  uint32_t blocks = (uint32_t)((n + kSceneBlock - 1) / kSceneBlock);
-  if (persistent) {
-    int dev = 0, cus = 0;
-    SRF_HIP(hipGetDevice(&dev));
-    SRF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  uint32_t blocks = (uint32_t)((n + block - 1) / block);
+  if (cfg.persistent) {
     const uint32_t waves = (uint32_t)((n + kChunk - 1) / kChunk);
-    blocks = std::min<uint32_t>(blocks, (uint32_t)cus * 8u);
-    blocks = std::min<uint32_t>(blocks, (waves + (kSceneBlock / 64) - 1) / (kSceneBlock / 64));
+    const uint32_t per_cu = std::max<uint32_t>(cfg.blocks_per_cu, 1u);
+    blocks = std::min<uint32_t>(blocks, (uint32_t)sc->cus * per_cu);
+    blocks = std::min<uint32_t>(blocks, (waves + (block / 64) - 1) / (block / 64));
     counter = sc->d_counter;
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, counter);
   }
-  hipLaunchKernelGGL((k_scene_tri<kSceneBlock>), dim3(blocks), dim3(kSceneBlock), lds, stream,
-                     sc->d_nodes, sc->num_nodes, sc->d_inst, sc->count, rays, (uint32_t)n, hits,
-                     inst_out, tdepth, counter);
+#define SRF_SCENE_LAUNCH(B)                                                               \
+  hipLaunchKernelGGL((k_scene_tri<B>), dim3(blocks), dim3(B), lds, stream, sc->d_nodes,    \
+                     sc->num_nodes, sc->d_inst, sc->count, rays, (uint32_t)n, hits, inst_out, \
+                     tdepth, counter)
+  if (block == 64)
+    SRF_SCENE_LAUNCH(64);
+  else if (block == 128)
+    SRF_SCENE_LAUNCH(128);
+  else
+    SRF_SCENE_LAUNCH(256);
+#undef SRF_SCENE_LAUNCH
   SRF_HIP(hipGetLastError());
+  return SRF_OK;
+}
+
+srf_status scene_trace(const Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
+                       uint32_t *inst_out, int mode, hipStream_t stream) {
+  srf_launch_config cfg = sc->tuned;
+  if (mode == SRF_LAUNCH_PLAIN) {
+    cfg.persistent = 0;
+  } else if (mode == SRF_LAUNCH_PERSISTENT && !cfg.persistent) {
+    cfg.persistent = 1;
+    cfg.blocks_per_cu = 8;
+  } else if (mode != SRF_LAUNCH_TUNED && mode != SRF_LAUNCH_PERSISTENT) {
+    return set_error(SRF_ERR_INVALID_ARG, "launch mode %d", mode);
+  }
+  return scene_launch(sc, cfg, rays, n, hits, inst_out, stream);
+}
+
+// Times every launch configuration on the caller's rays (HIP events on the
+// stream, best of two runs after a warm-up) and keeps the fastest for
+// SRF_LAUNCH_TUNED.
+srf_status scene_autotune(Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
+                          uint32_t *inst_out, hipStream_t stream, srf_launch_config *best) {
+  hipEvent_t e0, e1;
+  SRF_HIP(hipEventCreate(&e0));
+  hipError_t he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return hip_error(he, "hipEventCreate");
+  }
+  srf_launch_config win{};
+  win.ms = FLT_MAX;
+  srf_status st = SRF_OK;
+  for (uint32_t block : {64u, 128u, 256u}) {
+    for (uint32_t per_cu : {0u, 2u, 4u, 8u, 16u}) {
+      srf_launch_config cfg{block, per_cu ? 1u : 0u, per_cu, 0.0f};
+      if (per_cu * block > 2048u) continue;  // more lanes than a CU holds
+      st = scene_launch(sc, cfg, rays, n, hits, inst_out, stream);  // warm-up
+      if (st != SRF_OK) break;
+      float t = FLT_MAX;
+      for (int rep = 0; rep < 2 && st == SRF_OK; rep++) {
+        he = hipEventRecord(e0, stream);
+        if (he == hipSuccess) st = scene_launch(sc, cfg, rays, n, hits, inst_out, stream);
+        if (he == hipSuccess && st == SRF_OK) he = hipEventRecord(e1, stream);
+        if (he == hipSuccess && st == SRF_OK) he = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (he == hipSuccess && st == SRF_OK) he = hipEventElapsedTime(&ms, e0, e1);
+        if (he != hipSuccess) st = hip_error(he, "autotune timing");
+        t = std::min(t, ms);
+      }
+      if (st != SRF_OK) break;
+      if (t < win.ms) {
+        win = cfg;
+        win.ms = t;
+      }
+    }
+    if (st != SRF_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != SRF_OK) return st;
+  sc->tuned = win;
+  if (best) *best = win;
   return SRF_OK;
 }
 

@@ -60,7 +60,7 @@ def test_scene_matches_reference_fixture(S):
                           np.concatenate([g["tlas"]["bmin"][0], g["tlas"]["bmax"][0]]))
     rays = S.device_tensor(g["rays"])
     for persistent in (False, True):
-        hits, inst = _hits(S, *sc.Trace(rays, persistent=persistent))
+        hits, inst = _hits(S, *sc.Trace(rays, mode=("persistent" if persistent else "plain")))
         assert hits.tobytes() == g["hits"].tobytes(), persistent
         assert np.array_equal(inst, g["inst"]), persistent
 
@@ -82,9 +82,16 @@ def test_scene_matches_oracle(S, orc, count):
     assert canonical_tlas(sc.export_tlas()).tobytes() == canonical_tlas(tl).tobytes()
     els = [dict(tree=trees[i], V=meshes[i][0], F=meshes[i][1], T=T[i], invT=invT[i],
                 invT33=invT33[i]) for i in range(count)]
-    for cam in W.scene_cameras(T, count=2, width=128, height=128):
+    for k, cam in enumerate(W.scene_cameras(T, count=2, width=128, height=128)):
         want_h, want_i = orc.scene_trace(tl, els, cam, threads=8)
-        got_h, got_i = _hits(S, *sc.Trace(S.device_tensor(cam), persistent=True))
+        rays = S.device_tensor(cam)
+        if k == 1:
+            # every launch shape the autotuner tries must give the same hits
+            cfg = sc.autotune(rays)
+            say(f"autotuned: {cfg}")
+            got_h, got_i = _hits(S, *sc.Trace(rays, mode="tuned"))
+        else:
+            got_h, got_i = _hits(S, *sc.Trace(rays, mode="persistent"))
         assert got_h.tobytes() == want_h.tobytes()
         assert np.array_equal(got_i, want_i)
         assert (want_i != W.NO_PRIM).mean() > 0.05
@@ -97,3 +104,38 @@ def test_scene_rejects_non_triangle_elements(S):
     tet.Build32(S.device_tensor(V), S.device_tensor(F))
     with pytest.raises(SrfError):
         S.Scene([tet], np.eye(4)[None])
+
+
+def test_scene_trace_in_hip_graph_replays_match_eager(S):
+    """Persistent scene traces captured in a hipGraph (the work counter is
+    cleared by a kernel, not a memset): every replay equals the eager run."""
+    import torch
+    meshes, T = W.batch_scene(count=8, lo=1100, hi=20000, seed=W.BASE_SEED + 40)
+    accs = S.build_triangles_batch([(S.device_tensor(V), S.device_tensor(F)) for V, F in meshes])
+    sc = S.Scene(accs, T)
+    cams = [S.device_tensor(c) for c in W.scene_cameras(T, count=3, width=128, height=128)]
+    eager = [sc.Trace(r, mode="persistent") for r in cams]
+    want = [(h.clone(), i.clone()) for h, i in eager]
+    outs = [(torch.full_like(h, -1.0), torch.full_like(i, -7)) for h, i in eager]
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for r, (h, i) in zip(cams, outs):
+            sc.Trace(r, h, i, mode="persistent")
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for r, (h, i) in zip(cams, outs):
+            sc.Trace(r, h, i, mode="persistent")
+    for rep in range(4):
+        for h, i in outs:
+            h.fill_(-1.0)
+            i.fill_(-7)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for (h, i), (wh, wi) in zip(outs, want):
+            assert torch.equal(h.view(torch.int32), wh.view(torch.int32)), rep
+            assert torch.equal(i, wi), rep
+        say(f"graph replay {rep} equals the eager trace")
