@@ -178,6 +178,48 @@ def batched_scene(torch, S, W):
     return out
 
 
+def particles_ao(torch, S, W):
+    """16,777,216 Plummer spheres (lognormal radii): build, 2048x2048 primary
+    closest hits, 8 cosine-weighted AO rays per hit, occlusion traced
+    unsorted (plain and persistent) and with the autotuned ray sort."""
+    log("particles: 16M spheres, 2048^2 primary + 8 AO rays per hit")
+    P, R = W.plummer_spheres(1 << 24)
+    Pd, Rd = S.device_tensor(P), S.device_tensor(R)
+    del P, R
+    acc = S.ParticleAccel()
+    acc.Build32(Pd, Rd, 0.0)
+    torch.cuda.synchronize()
+    build_ms = timed_events(torch, lambda: acc.Build32(Pd, Rd, 0.0), 3)
+    prim = S.device_tensor(W.sphere_rays(2048, 2048))
+    np_ = prim.shape[0]
+    hits = torch.empty((np_, 4), dtype=torch.float32, device="cuda")
+    acc.Traverse(prim, hits)
+    prim_ms = timed_events(torch, lambda: acc.Traverse(prim, hits), 3)
+    rays, tmax, pix = S.ao_rays_spheres(prim, hits, Pd, Rd, 0.0, 8, 1e-3, 0.05, 7)
+    gen_ms = timed_events(torch, lambda: S.ao_rays_spheres(prim, hits, Pd, Rd, 0.0, 8, 1e-3,
+                                                           0.05, 7), 3)
+    na = rays.shape[0]
+    occ = torch.empty(na, dtype=torch.int32, device="cuda")
+    out = {"spheres": 1 << 24, "primary_rays": np_, "ao_rays": na,
+           "build_ms": round(build_ms, 3), "build_mprims_per_s": round((1 << 24) / build_ms / 1e3, 1),
+           "primary_ms": round(prim_ms, 3), "primary_mrays_per_s": round(np_ / prim_ms / 1e3, 1),
+           "primary_hit_fraction": round(na / 8 / np_, 4), "ao_gen_ms": round(gen_ms, 3)}
+    for name, bits, mode in (("unsorted_plain", 0, "plain"), ("unsorted_persistent", 0, "persistent")):
+        ms = timed_events(torch, lambda: acc.TraverseAnySorted(rays, tmax, bits, mode, occ), 3)
+        out[f"ao_{name}_ms"] = round(ms, 3)
+        out[f"ao_{name}_mrays_per_s"] = round(na / ms / 1e3, 1)
+    cfg = acc.autotune_any(rays, tmax, occ)
+    out["autotuned"] = cfg
+    mode = "persistent" if cfg["persistent"] else "plain"
+    ms = timed_events(torch, lambda: acc.TraverseAnySorted(rays, tmax, cfg["key_bits"], mode, occ), 3)
+    out["ao_tuned_ms"] = round(ms, 3)
+    out["ao_tuned_mrays_per_s"] = round(na / ms / 1e3, 1)
+    out["ao_occluded_fraction"] = round(float(occ.float().mean().item()), 4)
+    del acc, rays, tmax, pix, occ, hits, prim, Pd, Rd
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +323,7 @@ def main():
 
     if not args.no_extras:
         extras["batched_scene_64_meshes_16_cameras"] = batched_scene(torch, S, W)
+        extras["particles_16M_ao"] = particles_ao(torch, S, W)
 
     cpu = None if args.no_cpu else cpu_baseline(V, F, Sc, rays_np)
 

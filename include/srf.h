@@ -259,6 +259,46 @@ srf_status srf_trace_any(const srf_accel *accel, const srf_ray *rays,
                          uint32_t *occluded, void *stream);
 
 /* ------------------------------------------------------- LBVH stages */
+/* Occlusion with ray sorting: the rays are traced in the order of a sort
+ * key of key_bits bits (0: in the given order) -- the direction octant over
+ * the 3D Morton code of the origin in the accelerator's scene box -- and
+ * each flag goes back to its ray's own slot. mode: SRF_LAUNCH_PLAIN or
+ * SRF_LAUNCH_PERSISTENT (a persistent grid taking 64 sorted rays per
+ * wavefront from a work counter). Keys, order and counter live in ctx's
+ * scratch. Triangles and spheres. Each flag equals srf_trace_any's. */
+srf_status srf_trace_any_sorted(srf_ctx *ctx, const srf_accel *accel, const srf_ray *rays,
+                                const float *t_max, uint64_t n, uint32_t key_bits, int mode,
+                                uint32_t *occluded, void *stream);
+
+typedef struct {
+  uint32_t key_bits;   /* 0 = no sort */
+  uint32_t persistent; /* 0 or 1 */
+  float ms;            /* measured time of the tuning batch */
+} srf_any_config;
+
+/* Autotuner for srf_trace_any_sorted: times key widths {0, 9, 15, 21, 27,
+ * 30} with plain and persistent launches on the given rays and returns the
+ * fastest. Synchronises. No counterpart in the reference (its rays are
+ * traced one at a time). */
+srf_status srf_autotune_any(srf_ctx *ctx, const srf_accel *accel, const srf_ray *rays,
+                            const float *t_max, uint64_t n, uint32_t *occluded, void *stream,
+                            srf_any_config *best);
+
+/* Ambient-occlusion rays as a fragment shader issues them through shadow()
+ * (gles/gles_glsl_runtime.cc:179-217), for a batch of sphere hits: for
+ * every primary ray i with a hit on sphere c, per_hit cosine-weighted
+ * directions about the outward normal n = normalize(p - c), p = o + t d,
+ * drawn from a PCG32 stream seeded with (seed, i); origin p + eps r n,
+ * t_max = dist, prev_prim_id = c (so the sphere itself is never reported).
+ * Up to n * per_hit rays are written (out_pixel[j] = i); the count lands in
+ * *out_count (device memory). Slots are handed out per wavefront, so the
+ * order of the groups is not fixed. */
+srf_status srf_ao_rays_spheres(const srf_ray *primary, const srf_hit *hits, uint64_t n,
+                               const float *positions, const float *radius,
+                               float constant_radius, uint32_t per_hit, float eps, float dist,
+                               uint64_t seed, srf_ray *out_rays, float *out_tmax,
+                               uint32_t *out_pixel, uint32_t *out_count, void *stream);
+
 /* CalculateMortonCodes{TriangleFloat,TetraFloat}30 / CalculateMortonCodes30SIMD
  * (render/render_prefix_tree_util.cc:176-304) for the scene box bmin/bmax
  * (host arrays). desc points to the srf_*_desc matching kind. */

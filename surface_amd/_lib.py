@@ -65,6 +65,11 @@ class LaunchConfig(C.Structure):
 LAUNCH_TUNED, LAUNCH_PLAIN, LAUNCH_PERSISTENT = 0, 1, 2
 
 
+class AnyConfig(C.Structure):
+    """srf_any_config."""
+    _fields_ = [("key_bits", C.c_uint32), ("persistent", C.c_uint32), ("ms", C.c_float)]
+
+
 class TlasNode(C.Structure):
     """srf_tlas_node (72 bytes)."""
     _fields_ = [("bmin", C.c_double * 3), ("bmax", C.c_double * 3), ("child", C.c_int32 * 2),
@@ -101,6 +106,10 @@ SIGNATURES = {
     "srf_scene_export": (_i, [_vp, _vp, C.POINTER(C.c_uint32)]),
     "srf_scene_trace": (_i, [_vp, _vp, _u64, _vp, _vp, _i, _vp]),
     "srf_scene_autotune": (_i, [_vp, _vp, _u64, _vp, _vp, _vp, _vp]),
+    "srf_trace_any_sorted": (_i, [_vp, _vp, _vp, _vp, _u64, C.c_uint32, _i, _vp, _vp]),
+    "srf_autotune_any": (_i, [_vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp]),
+    "srf_ao_rays_spheres": (_i, [_vp, _vp, _u64, _vp, _vp, C.c_float, C.c_uint32, C.c_float,
+                                 C.c_float, C.c_uint64, _vp, _vp, _vp, _vp, _vp]),
     "srf_volume_create": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, C.POINTER(_vp)]),
     "srf_volume_destroy": (_i, [_vp]),
     "srf_volume_sample": (_i, [_vp, _vp, _u64, _vp, _vp]),

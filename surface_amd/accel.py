@@ -172,6 +172,51 @@ class _Accel:
         return out
 
 
+    def TraverseAnySorted(self, rays, t_max=None, key_bits=0, mode="plain", out=None,
+                          stream=None):
+        """Occlusion flags with the rays traced in sorted order (key of
+        key_bits bits; 0 = given order); mode "plain" or "persistent"."""
+        self._require()
+        n = rays.shape[0]
+        if out is None:
+            out = torch.empty(n, dtype=torch.int32, device=rays.device)
+        m = {"plain": _lib.LAUNCH_PLAIN, "persistent": _lib.LAUNCH_PERSISTENT}[mode]
+        check(self.lib.srf_trace_any_sorted(self.ctx.handle, self.handle, _ptr(rays), _ptr(t_max),
+                                            n, int(key_bits), m, _ptr(out), _stream(stream)))
+        return out
+
+    def autotune_any(self, rays, t_max=None, out=None, stream=None):
+        """Fastest (key_bits, persistent) for TraverseAnySorted on these rays."""
+        self._require()
+        n = rays.shape[0]
+        if out is None:
+            out = torch.empty(n, dtype=torch.int32, device=rays.device)
+        cfg = _lib.AnyConfig()
+        check(self.lib.srf_autotune_any(self.ctx.handle, self.handle, _ptr(rays), _ptr(t_max), n,
+                                        _ptr(out), _stream(stream), C.byref(cfg)))
+        return {"key_bits": cfg.key_bits, "persistent": bool(cfg.persistent), "ms": cfg.ms}
+
+
+def ao_rays_spheres(primary, hits, positions, radius=None, constant_radius=0.0, per_hit=8,
+                    eps=1e-3, dist=0.05, seed=1, stream=None):
+    """Cosine-weighted ambient-occlusion rays from sphere hits
+    (srf_ao_rays_spheres). Returns (rays [m, 8] float32, t_max [m],
+    pixel [m] int32) with m the number written (hits x per_hit)."""
+    lib = load()
+    n = primary.shape[0]
+    cap = n * per_hit
+    rays = torch.empty((cap, 8), dtype=torch.float32, device=primary.device)
+    tmax = torch.empty(cap, dtype=torch.float32, device=primary.device)
+    pixel = torch.empty(cap, dtype=torch.int32, device=primary.device)
+    count = torch.zeros(1, dtype=torch.int32, device=primary.device)
+    check(lib.srf_ao_rays_spheres(_ptr(primary), _ptr(hits), n, _ptr(positions), _ptr(radius),
+                                  float(constant_radius), int(per_hit), float(eps), float(dist),
+                                  int(seed), _ptr(rays), _ptr(tmax), _ptr(pixel), _ptr(count),
+                                  _stream(stream)))
+    m = int(count.item())
+    return rays[:m], tmax[:m], pixel[:m]
+
+
 class TriangleAccel(_Accel):
     """TriangleAccel::Build32 / Traverse / BoundingBox on the GPU."""
     kind = _lib.PRIM_TRIANGLES

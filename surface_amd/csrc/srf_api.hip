@@ -604,4 +604,79 @@ srf_status srf_scene_autotune(srf_scene *scene, const srf_ray *rays, uint64_t n,
                         best);
 }
 
+srf_status srf_trace_any_sorted(srf_ctx *ctx, const srf_accel *accel, const srf_ray *rays,
+                                const float *t_max, uint64_t n, uint32_t key_bits, int mode,
+                                uint32_t *occluded, void *stream) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!c || !a || (n && (!rays || !occluded))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (c->device != a->device) return set_error(SRF_ERR_INVALID_ARG, "context/device mismatch");
+  if (n >= (1ull << 31)) return set_error(SRF_ERR_INVALID_ARG, "too many rays in one call");
+  if (mode != SRF_LAUNCH_PLAIN && mode != SRF_LAUNCH_PERSISTENT)
+    return set_error(SRF_ERR_INVALID_ARG, "mode must be SRF_LAUNCH_PLAIN or _PERSISTENT");
+  DeviceGuard g(a->device);
+  return trace_any_sorted(c, a, rays, t_max, n, key_bits, mode, occluded,
+                          reinterpret_cast<hipStream_t>(stream));
+}
+
+srf_status srf_autotune_any(srf_ctx *ctx, const srf_accel *accel, const srf_ray *rays,
+                            const float *t_max, uint64_t n, uint32_t *occluded, void *stream,
+                            srf_any_config *best) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!c || !a || !rays || !occluded || n == 0)
+    return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (n >= (1ull << 31)) return set_error(SRF_ERR_INVALID_ARG, "too many rays in one call");
+  DeviceGuard g(a->device);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipEvent_t e0, e1;
+  SRF_HIP(hipEventCreate(&e0));
+  hipError_t he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return hip_error(he, "hipEventCreate");
+  }
+  srf_any_config win{0u, 0u, FLT_MAX};
+  srf_status st = SRF_OK;
+  for (uint32_t bits : {0u, 9u, 15u, 21u, 27u, 30u}) {
+    for (int mode : {(int)SRF_LAUNCH_PLAIN, (int)SRF_LAUNCH_PERSISTENT}) {
+      st = trace_any_sorted(c, a, rays, t_max, n, bits, mode, occluded, s);  // warm-up
+      float t = FLT_MAX;
+      for (int rep = 0; rep < 2 && st == SRF_OK; rep++) {
+        he = hipEventRecord(e0, s);
+        if (he == hipSuccess) st = trace_any_sorted(c, a, rays, t_max, n, bits, mode, occluded, s);
+        if (he == hipSuccess && st == SRF_OK) he = hipEventRecord(e1, s);
+        if (he == hipSuccess && st == SRF_OK) he = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (he == hipSuccess && st == SRF_OK) he = hipEventElapsedTime(&ms, e0, e1);
+        if (he != hipSuccess && st == SRF_OK) st = hip_error(he, "autotune timing");
+        t = ms < t ? ms : t;
+      }
+      if (st != SRF_OK) break;
+      if (t < win.ms) win = srf_any_config{bits, mode == SRF_LAUNCH_PERSISTENT ? 1u : 0u, t};
+    }
+    if (st != SRF_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != SRF_OK) return st;
+  if (best) *best = win;
+  return SRF_OK;
+}
+
+srf_status srf_ao_rays_spheres(const srf_ray *primary, const srf_hit *hits, uint64_t n,
+                               const float *positions, const float *radius, float rconst,
+                               uint32_t per_hit, float eps, float dist, uint64_t seed,
+                               srf_ray *out, float *out_tmax, uint32_t *out_pixel,
+                               uint32_t *out_count, void *stream) {
+  if (!out_count || (n && (!primary || !hits || !positions || !out || !out_tmax || !out_pixel)))
+    return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (per_hit == 0 || per_hit > 64) return set_error(SRF_ERR_INVALID_ARG, "per_hit in [1, 64]");
+  if (n * (uint64_t)per_hit >= (1ull << 32))
+    return set_error(SRF_ERR_INVALID_ARG, "too many secondary rays in one call");
+  return ao_rays_spheres(primary, hits, n, positions, radius, rconst, per_hit, eps, dist, seed,
+                         out, out_tmax, out_pixel, out_count,
+                         reinterpret_cast<hipStream_t>(stream));
+}
+
 }  // extern "C"

@@ -49,16 +49,6 @@ constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
-__device__ __forceinline__ uint32_t part_by2_30(uint32_t n) {
-  // PartBy2_30, render/render_prefix_tree_util.h:113-120
-  n &= 0x000003ffu;
-  n = (n ^ (n << 16)) & 0xff0000ffu;
-  n = (n ^ (n << 8)) & 0x0300f00fu;
-  n = (n ^ (n << 4)) & 0x030c30c3u;
-  n = (n ^ (n << 2)) & 0x09249249u;
-  return n;
-}
-
 // (uint32_t)(float): x86-64 truncates through int64; for |x| < 2^31 and NaN
 // (both give 0) this equals the 32-bit truncation.
 __device__ __forceinline__ uint32_t f2u_trunc(float x) { return (uint32_t)(int)x; }
@@ -1327,13 +1317,17 @@ srf_status launch_morton(Kind kind, const void *desc, const float bmin[3],
   return SRF_OK;
 }
 
-srf_status sort_pairs30(Ctx *ctx, const uint32_t *codes_in, uint32_t n, uint32_t *codes_out,
-                        uint32_t *index_out, hipStream_t stream) {
+size_t sort_pairs30_bytes(uint32_t n) {
   const uint32_t tiles = sort_tiles(n);
-  size_t need = 4 * (3 * (size_t)n + 4 * (size_t)tiles * 256 + 8 + 1024) + 8 * 256;
-  srf_status st = ensure_scratch(ctx, need, stream);
+  return 4 * (3 * (size_t)n + 4 * (size_t)tiles * 256 + 8 + 1024) + 8 * 256;
+}
+
+srf_status sort_pairs30_at(Ctx *ctx, size_t arena_off, const uint32_t *codes_in, uint32_t n,
+                           uint32_t *codes_out, uint32_t *index_out, hipStream_t stream) {
+  const uint32_t tiles = sort_tiles(n);
+  srf_status st = ensure_scratch(ctx, arena_off + sort_pairs30_bytes(n), stream);
   if (st != SRF_OK) return st;
-  Arena ar{(char *)ctx->scratch, ctx->scratch_bytes, 0};
+  Arena ar{(char *)ctx->scratch, ctx->scratch_bytes, arena_off};
   SortScratch ss;
   ss.keys_b = ar.take<uint32_t>(n);
   ss.vals_b = ar.take<uint32_t>(n);
@@ -1345,13 +1339,19 @@ srf_status sort_pairs30(Ctx *ctx, const uint32_t *codes_in, uint32_t n, uint32_t
   size_t nclear = (size_t)((ss.hist + 1024) - ss.status);
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nclear, 256, 2048)), dim3(256), 0, stream,
                      ss.status, 0u, nclear);
-  SRF_HIP(hipMemcpyAsync(codes_out, codes_in, 4 * (size_t)n, hipMemcpyDeviceToDevice, stream));
+  if (codes_out != codes_in)
+    SRF_HIP(hipMemcpyAsync(codes_out, codes_in, 4 * (size_t)n, hipMemcpyDeviceToDevice, stream));
   hipLaunchKernelGGL(k_histogram, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, stream,
                      codes_out, n, ss.hist);
   SRF_HIP(hipGetLastError());
   SegView v{};
   v.nseg = 1;
   return run_sort_passes<false>(codes_out, n, tiles, ss, v, stream);
+}
+
+srf_status sort_pairs30(Ctx *ctx, const uint32_t *codes_in, uint32_t n, uint32_t *codes_out,
+                        uint32_t *index_out, hipStream_t stream) {
+  return sort_pairs30_at(ctx, 0, codes_in, n, codes_out, index_out, stream);
 }
 
 srf_status launch_radix_tree(const uint32_t *sorted_codes, uint32_t n, srf_node_info *out,

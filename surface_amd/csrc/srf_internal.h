@@ -36,6 +36,17 @@ constexpr int kStackSize = 32;
 
 enum Kind : uint32_t { kTri = 0, kTet = 1, kSph = 2 };
 
+// PartBy2_30 (render/render_prefix_tree_util.h:113-120): the low 10 bits of
+// n spread to every third bit.
+__device__ __forceinline__ uint32_t part_by2_30(uint32_t n) {
+  n &= 0x000003ffu;
+  n = (n ^ (n << 16)) & 0xff0000ffu;
+  n = (n ^ (n << 8)) & 0x0300f00fu;
+  n = (n ^ (n << 4)) & 0x030c30c3u;
+  n = (n ^ (n << 2)) & 0x09249249u;
+  return n;
+}
+
 // Per-build scalars produced on the device and read by later kernels.
 struct alignas(16) BuildParams {
   float bmin[3];       // scene box used for Morton codes
@@ -125,5 +136,18 @@ srf_status trace_sph(const Accel *a, const srf_ray *rays,
                      hipStream_t s);
 srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax,
                      uint64_t n, uint32_t *occluded, hipStream_t s);
+srf_status trace_any_sorted(Ctx *ctx, const Accel *a, const srf_ray *rays, const float *tmax,
+                            uint64_t n, uint32_t key_bits, int mode, uint32_t *occluded,
+                            hipStream_t s);
+srf_status ao_rays_spheres(const srf_ray *primary, const srf_hit *hits, uint64_t n,
+                           const float *positions, const float *radius, float rconst,
+                           uint32_t per_hit, float eps, float dist, uint64_t seed,
+                           srf_ray *out, float *out_tmax, uint32_t *out_pixel,
+                           uint32_t *count, hipStream_t s);
+// Stable sort of 30-bit codes with their indices, using the arena from
+// `arena_off` on (the caller's own slices come before it).
+srf_status sort_pairs30_at(Ctx *ctx, size_t arena_off, const uint32_t *codes_in, uint32_t n,
+                           uint32_t *codes_out, uint32_t *index_out, hipStream_t stream);
+size_t sort_pairs30_bytes(uint32_t n);
 
 }  // namespace srf

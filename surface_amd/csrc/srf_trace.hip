@@ -28,6 +28,8 @@
 //   triangles  TriangleAccel::Traverse  render/render_accel_triangle.cc:2511-2669
 //   tetras     TetraAccel::Traverse     render/render_accel_tetra.cc:2200-2289
 //   spheres    ParticleAccel::Traverse  render/render_accel_particle.cc:3053-3218
+#include <algorithm>
+
 #include "srf_trace_dev.h"
 
 namespace srf {
@@ -111,28 +113,20 @@ __device__ __forceinline__ bool sphere_isect(const RaySetup &s, float4 c, float 
   return false;
 }
 
-template <bool kAny>
-__global__ void __launch_bounds__(kTraceBlock)
-k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
-            const uint32_t *__restrict__ sorted_index, const srf_ray *__restrict__ rays,
-            const srf_sphere_prev *__restrict__ prevs, uint32_t n, srf_hit *__restrict__ hits,
-            const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded,
-            uint32_t max_nodes, uint32_t nprims) {
-  __shared__ uint32_t stack[kTraceStack * kTraceBlock];
-  const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
-  if (i >= n) return;
-  uint32_t flags, prev;
-  const RaySetup s = setup_ray(rays, i, flags, prev);
-  double prevT = 0.0;
-  float prevN[3] = {0.0f, 0.0f, 0.0f};
-  if (prevs) {
-    prevT = prevs[i].prev_hit_t;
-    for (int k = 0; k < 3; k++) prevN[k] = prevs[i].prev_hit_normal[k];
-  }
-  const float tlimit = kAny ? (tmax_in ? tmax_in[i] : FLT_MAX) : FLT_MAX;
-  float t = kAny ? tlimit : FLT_MAX;
-  uint32_t best = SRF_NO_PRIM;
-  uint32_t *st = stack + threadIdx.x;
+// Closest (kAny: any closer than tlimit) sphere of one ray,
+// ParticleAccel::Traverse (render/render_accel_particle.cc:3053-3218).
+// `best` receives the sorted position of the hit sphere (SRF_NO_PRIM on a
+// miss), t its distance.
+template <bool kAny, int kStride>
+__device__ __forceinline__ bool sph_traverse(const float4 *__restrict__ nodes,
+                                             const float4 *__restrict__ prims,
+                                             const uint32_t *__restrict__ sorted_index,
+                                             uint32_t max_nodes, uint32_t nprims,
+                                             const RaySetup &s, uint32_t prev, double prevT,
+                                             const float prevN[3], float tlimit, uint32_t *st,
+                                             float &t, uint32_t &best) {
+  t = kAny ? tlimit : FLT_MAX;
+  best = SRF_NO_PRIM;
   int sp = 0;
   uint32_t cur = 0;
   uint4 meta;
@@ -152,7 +146,7 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
       const bool hn = nearc ? h1 : h0, hf = nearc ? h0 : h1;
       const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
       if (hn) {
-        if (hf && sp < kTraceStack) st[(sp++) * kTraceBlock] = cf;
+        if (hf && sp < kTraceStack) st[(sp++) * kStride] = cf;
         cur = cn;
       } else if (hf) {
         cur = cf;
@@ -160,7 +154,7 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
         cur = kDone;
         break;
       } else {
-        cur = st[(--sp) * kTraceBlock];
+        cur = st[(--sp) * kStride];
       }
     }
     if (cur == kDone) break;
@@ -171,10 +165,7 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
       float tt;
       if (sphere_isect(s, c, t, prevT, prevN, self, tt)) {
         if (kAny) {
-          if (tt < tlimit) {
-            occluded[i] = 1u;
-            return;
-          }
+          if (tt < tlimit) return true;
         } else {
           t = tt;
           best = q;
@@ -182,10 +173,37 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
       }
     }
     if (sp == 0) break;
-    cur = st[(--sp) * kTraceBlock];
+    cur = st[(--sp) * kStride];
   }
+  return false;
+}
+
+template <bool kAny>
+__global__ void __launch_bounds__(kTraceBlock)
+k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
+            const uint32_t *__restrict__ sorted_index, const srf_ray *__restrict__ rays,
+            const srf_sphere_prev *__restrict__ prevs, uint32_t n, srf_hit *__restrict__ hits,
+            const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded,
+            uint32_t max_nodes, uint32_t nprims) {
+  __shared__ uint32_t stack[kTraceStack * kTraceBlock];
+  const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t flags, prev;
+  const RaySetup s = setup_ray(rays, i, flags, prev);
+  double prevT = 0.0;
+  float prevN[3] = {0.0f, 0.0f, 0.0f};
+  if (prevs) {
+    prevT = prevs[i].prev_hit_t;
+    for (int k = 0; k < 3; k++) prevN[k] = prevs[i].prev_hit_normal[k];
+  }
+  const float tlimit = kAny ? (tmax_in ? tmax_in[i] : FLT_MAX) : FLT_MAX;
+  float t;
+  uint32_t best;
+  const bool occ = sph_traverse<kAny, kTraceBlock>(nodes, prims, sorted_index, max_nodes, nprims,
+                                                   s, prev, prevT, prevN, tlimit,
+                                                   stack + threadIdx.x, t, best);
   if (kAny) {
-    occluded[i] = 0u;
+    occluded[i] = occ ? 1u : 0u;
     return;
   }
   float u = 0.0f, v = 0.0f;
@@ -714,6 +732,171 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   }
 }
 
+// ------------------------------------------------- sorted / persistent any-hit
+
+// Occlusion for rays taken in `order` (order[j] is the j-th ray to trace,
+// the flag goes back to its own slot). With `counter` the grid is
+// persistent: each wavefront takes 64 consecutive positions of the order at
+// a time until none is left.
+template <int K>
+__global__ void __launch_bounds__(kTraceBlock)
+k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
+              const uint32_t *__restrict__ sorted_index, const BuildParams *__restrict__ params,
+              const srf_ray *__restrict__ rays, const float *__restrict__ tmax_in,
+              const uint32_t *__restrict__ order, uint32_t n, uint32_t *__restrict__ occluded,
+              uint32_t max_nodes, uint32_t nprims, uint32_t *counter) {
+  __shared__ uint32_t stack[kTraceStack * kTraceBlock];
+  uint32_t *st = stack + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t base = blockIdx.x * kTraceBlock + (threadIdx.x & ~63u);
+  for (;;) {
+    if (counter) {
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(counter, 64u);
+      base = __shfl(b, 0, 64);
+    }
+    if (base >= n) break;
+    const uint32_t j = base + lane;
+    if (j < n) {
+      const uint32_t i = order ? order[j] : j;
+      uint32_t flags, prev;
+      const RaySetup s = setup_ray(rays, i, flags, prev);
+      const float tlimit = tmax_in ? tmax_in[i] : FLT_MAX;
+      bool occ;
+      if (K == kTri) {
+        const float kEPS = FLT_EPSILON * (float)params->eps_scale;
+        float t, u, v;
+        uint32_t prim;
+        occ = tri_traverse<true, kTraceBlock>(nodes, prims, max_nodes, nprims, s,
+                                              (flags & SRF_RAY_DOUBLE_SIDED) != 0, kEPS, prev,
+                                              tlimit, st, t, u, v, prim);
+      } else {
+        const float zero[3] = {0.0f, 0.0f, 0.0f};
+        float t;
+        uint32_t best;
+        occ = sph_traverse<true, kTraceBlock>(nodes, prims, sorted_index, max_nodes, nprims, s,
+                                              prev, 0.0, zero, tlimit, st, t, best);
+      }
+      occluded[i] = occ ? 1u : 0u;
+    }
+    if (!counter) break;
+  }
+}
+
+// Sort key of a ray: direction octant over the 3D Morton code of its origin
+// in the accelerator's scene box (origins outside clamp to the box), the top
+// key_bits of those 30 bits.
+__global__ void __launch_bounds__(256) k_ray_keys(const srf_ray *__restrict__ rays, uint32_t n,
+                                                 const BuildParams *__restrict__ params,
+                                                 uint32_t key_bits, uint32_t *__restrict__ keys) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float4 *r4 = reinterpret_cast<const float4 *>(rays + i);
+  const float4 a = r4[0], b = r4[1];
+  const float o[3] = {a.x, a.y, a.z};
+  uint32_t q[3];
+  for (int k = 0; k < 3; k++) {
+    float x = (o[k] - params->bmin[k]) * params->inv[k];
+    x = x > 0.0f ? (x < 1023.0f ? x : 1023.0f) : 0.0f;  // NaN goes to 0
+    q[k] = (uint32_t)x;
+  }
+  const uint32_t m = (part_by2_30(q[0]) << 2) | (part_by2_30(q[1]) << 1) | part_by2_30(q[2]);
+  const uint32_t oct = (b.x < 0.0f ? 4u : 0u) | (b.y < 0.0f ? 2u : 0u) | (b.z < 0.0f ? 1u : 0u);
+  const uint32_t key30 = (oct << 27) | (m >> 3);
+  keys[i] = key_bits >= 30 ? key30 : (key_bits == 0 ? 0u : key30 >> (30 - key_bits));
+}
+
+__global__ void k_zero_word(uint32_t *c) { *c = 0u; }
+
+// PCG32 (O'Neill 2014, pcg32_srandom / pcg32_random): one stream per pixel.
+struct Pcg32 {
+  uint64_t state, inc;
+  __device__ __forceinline__ uint32_t next() {
+    const uint64_t old = state;
+    state = old * 6364136223846793005ull + inc;
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+  }
+  __device__ __forceinline__ void seed(uint64_t initstate, uint64_t initseq) {
+    state = 0u;
+    inc = (initseq << 1u) | 1u;
+    next();
+    state += initstate;
+    next();
+  }
+  __device__ __forceinline__ float uniform() { return (float)(next() >> 8) * (1.0f / 16777216.0f); }
+};
+
+// Ambient-occlusion rays from sphere hits: for each primary ray i that hit
+// sphere c, `per_hit` cosine-weighted directions about the outward normal
+// n = normalize(p - c) at p = o + t d, from p + eps r n, with t_max = dist
+// and prev_prim_id = the sphere (never reported again). Output slots are
+// handed out per wavefront (the order of wavefronts is not fixed; every
+// ray carries its pixel in out_pixel).
+__global__ void __launch_bounds__(256)
+k_ao_rays_sph(const srf_ray *__restrict__ primary, const srf_hit *__restrict__ hits, uint32_t n,
+              const float *__restrict__ P, const float *__restrict__ R, float rconst,
+              uint32_t per_hit, float eps, float dist, uint64_t seed, srf_ray *__restrict__ out,
+              float *__restrict__ out_tmax, uint32_t *__restrict__ out_pixel, uint32_t *count) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t prim = SRF_NO_PRIM;
+  float t = 0.0f;
+  if (i < n) {
+    const float4 h = reinterpret_cast<const float4 *>(hits)[i];
+    prim = __float_as_uint(h.w);
+    t = h.x;
+  }
+  const bool hit = prim != SRF_NO_PRIM;
+  const uint64_t mask = __ballot(hit);
+  uint32_t base = 0;
+  if (lane == 0 && mask) base = atomicAdd(count, (uint32_t)__popcll(mask) * per_hit);
+  base = __shfl(base, 0, 64);
+  if (!hit) return;
+  const uint64_t below = lane ? (mask & (~0ull >> (64 - lane))) : 0ull;
+  const uint32_t slot = base + (uint32_t)__popcll(below) * per_hit;
+  const float4 *r4 = reinterpret_cast<const float4 *>(primary + i);
+  const float4 ra = r4[0], rb = r4[1];
+  const float px = ra.x + t * rb.x, py = ra.y + t * rb.y, pz = ra.z + t * rb.z;
+  const float cx = P[3 * (size_t)prim], cy = P[3 * (size_t)prim + 1], cz = P[3 * (size_t)prim + 2];
+  const float r = R ? R[prim] : rconst;
+  float nx = px - cx, ny = py - cy, nz = pz - cz;
+  const float len = sqrtf((nx * nx + ny * ny) + nz * nz);
+  if (len > 0.0f) {
+    nx = nx / len;
+    ny = ny / len;
+    nz = nz / len;
+  } else {
+    nx = 0.0f;
+    ny = 0.0f;
+    nz = 1.0f;
+  }
+  // orthonormal basis about n (Duff et al. 2017)
+  const float sg = nz >= 0.0f ? 1.0f : -1.0f;
+  const float ia = -1.0f / (sg + nz);
+  const float bb = nx * ny * ia;
+  const float b1x = 1.0f + sg * nx * nx * ia, b1y = sg * bb, b1z = -sg * nx;
+  const float b2x = bb, b2y = sg + ny * ny * ia, b2z = -ny;
+  const float ox = px + eps * r * nx, oy = py + eps * r * ny, oz = pz + eps * r * nz;
+  Pcg32 rng;
+  rng.seed(seed, (uint64_t)i);
+  for (uint32_t k = 0; k < per_hit; k++) {
+    const float u1 = rng.uniform(), u2 = rng.uniform();
+    const float rr = sqrtf(u1), phi = 6.2831853071795864769f * u2;
+    const float lx = rr * cosf(phi), ly = rr * sinf(phi);
+    const float lz = sqrtf(fmaxf(0.0f, 1.0f - u1));
+    const float dx = (lx * b1x + ly * b2x) + lz * nx;
+    const float dy = (lx * b1y + ly * b2y) + lz * ny;
+    const float dz = (lx * b1z + ly * b2z) + lz * nz;
+    float4 *o4 = reinterpret_cast<float4 *>(out + slot + k);
+    o4[0] = make_float4(ox, oy, oz, __uint_as_float(0u));
+    o4[1] = make_float4(dx, dy, dz, __uint_as_float(prim));
+    out_tmax[slot + k] = dist;
+    out_pixel[slot + k] = i;
+  }
+}
+
 inline uint32_t trace_blocks(uint64_t n) { return (uint32_t)((n + kTraceBlock - 1) / kTraceBlock); }
 inline uint32_t max_nodes(const Accel *a) { return (uint32_t)(2 * a->n - 1); }
 
@@ -767,6 +950,72 @@ srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax, uin
     hipLaunchKernelGGL(k_trace_sph<true>, g, b, 0, s, (const float4 *)a->nodes,
                        (const float4 *)a->prims, a->sorted_index, rays, nullptr, (uint32_t)n,
                        nullptr, tmax, occluded, max_nodes(a), (uint32_t)a->n);
+  SRF_HIP(hipGetLastError());
+  return SRF_OK;
+}
+
+srf_status trace_any_sorted(Ctx *ctx, const Accel *a, const srf_ray *rays, const float *tmax,
+                            uint64_t n, uint32_t key_bits, int mode, uint32_t *occluded,
+                            hipStream_t s) {
+  if (n == 0) return SRF_OK;
+  if (a->kind == kTet)
+    return set_error(SRF_ERR_UNSUPPORTED, "sorted any-hit takes triangles or spheres");
+  if (key_bits > 30) return set_error(SRF_ERR_INVALID_ARG, "key_bits must be <= 30");
+  const uint32_t nn = (uint32_t)n;
+  // scratch: [counter][keys][sorted keys][order] then the sort's own slices
+  const size_t head = 256 + 3 * (((size_t)4 * nn + 255) & ~(size_t)255);
+  const uint32_t *order = nullptr;
+  uint32_t *counter = nullptr;
+  if (key_bits > 0 || mode == SRF_LAUNCH_PERSISTENT) {
+    srf_status st = ensure_scratch(ctx, head + sort_pairs30_bytes(nn), s);
+    if (st != SRF_OK) return st;
+    Arena ar{(char *)ctx->scratch, ctx->scratch_bytes, 0};
+    counter = ar.take<uint32_t>(64);
+    uint32_t *keys = ar.take<uint32_t>(nn);
+    uint32_t *skeys = ar.take<uint32_t>(nn);
+    uint32_t *ord = ar.take<uint32_t>(nn);
+    if (key_bits > 0) {
+      hipLaunchKernelGGL(k_ray_keys, dim3((nn + 255) / 256), dim3(256), 0, s, rays, nn, a->params,
+                         key_bits, keys);
+      SRF_HIP(hipGetLastError());
+      st = sort_pairs30_at(ctx, head, keys, nn, skeys, ord, s);
+      if (st != SRF_OK) return st;
+      order = ord;
+    }
+    if (mode != SRF_LAUNCH_PERSISTENT) counter = nullptr;
+  }
+  uint32_t blocks = trace_blocks(n);
+  if (counter) {
+    int dev = 0, cus = 0;
+    SRF_HIP(hipGetDevice(&dev));
+    SRF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    blocks = std::min<uint32_t>(blocks, (uint32_t)cus * 16u);
+    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, s, counter);
+  }
+  if (a->kind == kTri)
+    hipLaunchKernelGGL(k_any_ordered<kTri>, dim3(blocks), dim3(kTraceBlock), 0, s,
+                       (const float4 *)a->nodes, (const float4 *)a->prims, a->sorted_index,
+                       a->params, rays, tmax, order, nn, occluded, max_nodes(a), (uint32_t)a->n,
+                       counter);
+  else
+    hipLaunchKernelGGL(k_any_ordered<kSph>, dim3(blocks), dim3(kTraceBlock), 0, s,
+                       (const float4 *)a->nodes, (const float4 *)a->prims, a->sorted_index,
+                       a->params, rays, tmax, order, nn, occluded, max_nodes(a), (uint32_t)a->n,
+                       counter);
+  SRF_HIP(hipGetLastError());
+  return SRF_OK;
+}
+
+srf_status ao_rays_spheres(const srf_ray *primary, const srf_hit *hits, uint64_t n,
+                           const float *positions, const float *radius, float rconst,
+                           uint32_t per_hit, float eps, float dist, uint64_t seed,
+                           srf_ray *out, float *out_tmax, uint32_t *out_pixel,
+                           uint32_t *count, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, s, count);
+  if (n == 0) return SRF_OK;
+  hipLaunchKernelGGL(k_ao_rays_sph, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, primary,
+                     hits, (uint32_t)n, positions, radius, rconst, per_hit, eps, dist, seed, out,
+                     out_tmax, out_pixel, count);
   SRF_HIP(hipGetLastError());
   return SRF_OK;
 }
