@@ -48,6 +48,9 @@ constexpr uint32_t kSpinLimit = 1u << 26;
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
+// Leaves with more primitives than this are boxed and packed by a whole block
+// (k_leaves_big); depth-capped leaves of clustered data hold 10^5 and more.
+constexpr uint32_t kBigLeaf = 64;
 
 // (uint32_t)(float): x86-64 truncates through int64; for |x| < 2^31 and NaN
 // (both give 0) this equals the 32-bit truncation.
@@ -599,8 +602,7 @@ __device__ __forceinline__ void seg_local(const SegView &v, uint32_t n_all, uint
 template <bool M>
 __global__ void __launch_bounds__(256) k_karras(const uint32_t *codes, uint32_t n_all,
                                                uint32_t *kchild, uint32_t *kleaf,
-                                               uint32_t *kother, uint32_t *kparent,
-                                               SegView v) {
+                                               uint32_t *kother, SegView v) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n_all) return;
   uint32_t m, base, n;
@@ -613,9 +615,6 @@ __global__ void __launch_bounds__(256) k_karras(const uint32_t *codes, uint32_t 
   kchild[x] = child;
   kleaf[x] = leafbits;
   kother[x] = (uint32_t)j;
-  // parent link of an internal child: parent << 1 | (1 for the right child)
-  if (!(leafbits & 1u)) kparent[base + child] = i << 1;
-  if (!(leafbits & 2u)) kparent[base + child + 1] = (i << 1) | 1u;
 }
 
 __global__ void __launch_bounds__(256) k_karras_nodeinfo(const uint32_t *codes, uint32_t n,
@@ -633,66 +632,87 @@ __global__ void __launch_bounds__(256) k_karras_nodeinfo(const uint32_t *codes, 
 // ---- collapse: which Karras nodes become branches ----
 // BuildTreeRecursive32, render/render_accel_triangle.cc:895-992: a child range
 // [l, r] becomes a leaf iff it is a Karras leaf, r - l <= MAX_LEAF, or its
-// depth (root's children = 0) exceeds 22.
-// cls[i] = 0 (not a branch) or 1 | steps << 1 | right_turns << 8.
+// depth (root's children = 0) exceeds 22. The branches are found top down,
+// one launch per depth: a branch at depth L (L ancestors) records its class
+// cls = 1 | L << 1 | right_turns << 8, marks where its leaf children start,
+// and hands its internal children that are branches too to depth L + 1.
+// Work is proportional to the branches, whatever the Karras tree's depth.
 template <bool M>
-__global__ void __launch_bounds__(256) k_classify(uint32_t n_all, int max_leaf,
-                                                 const uint32_t *kchild, const uint32_t *kleaf,
-                                                 const uint32_t *kother, const uint32_t *kparent,
-                                                 uint32_t *cls, uint32_t *leaf_start,
-                                                 SegView v) {
-  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t steps = 0, turns = 0;
-  bool branch = false;
-  uint32_t l = 0, r = 0, m = 0, base = 0, n = 0, i = 0;
-  if (x < n_all) {
-    seg_local<M>(v, n_all, x, m, base, n);
-    i = x - base;
-  }
-  BuildParams *params = (x < n_all) ? seg_get<M>(v, m).params : nullptr;
-  if (x < n_all && i + 1 < n) {
-    const uint32_t j = kother[x];
-    l = i < j ? i : j;
-    r = i < j ? j : i;
-    if (i == 0) {
-      branch = true;
-    } else if (r - l > (uint32_t)max_leaf) {
-      uint32_t c = i;
-      while (c != 0 && steps <= (uint32_t)(kMaxDepth32 + 1)) {
-        const uint32_t link = kparent[base + c];
-        const uint32_t p = link >> 1;
-        if (p + 1 >= n) {  // malformed parent link: never expected, stop safely
-          atomicOr(&params->error, 8u);
-          break;
-        }
-        turns += link & 1u;
-        c = p;
-        steps++;
+__global__ void __launch_bounds__(64) k_classify_roots(SegView v, uint2 *front,
+                                                      uint32_t *counts) {
+  for (uint32_t m = threadIdx.x; m < v.nseg; m += blockDim.x)
+    front[m] = make_uint2(seg_off<M>(v.prim_off, m), 0u);
+  if (threadIdx.x == 0) counts[0] = v.nseg;
+}
+
+template <bool M>
+__global__ void __launch_bounds__(256) k_classify_level(uint32_t n_all, int max_leaf,
+                                                       uint32_t level, const uint32_t *kchild,
+                                                       const uint32_t *kleaf,
+                                                       const uint32_t *kother,
+                                                       const uint2 *front_in, uint2 *front_out,
+                                                       uint32_t *counts, uint32_t *cls,
+                                                       uint32_t *leaf_start, SegView v) {
+  const uint32_t count = counts[level];
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool deep = level > (uint32_t)kMaxDepth32;
+  // whole wavefronts step through the frontier, so the appends below can be
+  // counted with one atomic per wavefront
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); j0 < count; j0 += stride) {
+    const uint32_t j = j0 + lane;
+    uint32_t push_l = 0, push_r = 0, cl = 0, cr = 0, turns = 0;
+    BuildParams *params = nullptr;
+    if (j < count) {
+      const uint2 e = front_in[j];
+      const uint32_t x = e.x;
+      turns = e.y;
+      uint32_t m, base, n;
+      seg_local<M>(v, n_all, x, m, base, n);
+      const uint32_t i = x - base;
+      params = seg_get<M>(v, m).params;
+      if (i + 1 < n) {
+        cls[x] = 1u | (level << 1) | (turns << 8);
+        const uint32_t jo = kother[x];
+        const uint32_t l = i < jo ? i : jo, r = i < jo ? jo : i;
+        const uint32_t g = kchild[x], lb = kleaf[x];
+        const bool left_leaf = (lb & 1u) || (g - l <= (uint32_t)max_leaf) || deep;
+        const bool right_leaf = (lb & 2u) || (r - (g + 1) <= (uint32_t)max_leaf) || deep;
+        if (left_leaf)
+          leaf_start[base + l] = 1;
+        else
+          push_l = 1, cl = base + g;
+        if (right_leaf)
+          leaf_start[base + g + 1] = 1;
+        else
+          push_r = 1, cr = base + g + 1;
+      } else {
+        atomicOr(&params->error, 8u);  // a leaf handed over as a branch: never expected
       }
-      branch = (c == 0) && steps <= (uint32_t)(kMaxDepth32 + 1);
     }
-    cls[x] = branch ? (1u | (steps << 1) | (turns << 8)) : 0u;
+    // deepest branch per mesh: one atomic per wavefront and mesh present
+    {
+      const bool has = params != nullptr;
+      uint64_t pending = __ballot(has);
+      while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint64_t pv = (uint64_t)__shfl((long long)(uintptr_t)params, leader, 64);
+        const bool mine = has && (uint64_t)(uintptr_t)params == pv;
+        if ((int)lane == leader) atomicMax(&params->max_steps, level);
+        pending &= ~__ballot(mine);
+      }
+    }
+    const uint64_t ml = __ballot(push_l), mr = __ballot(push_r);
+    const uint32_t total = (uint32_t)(__popcll(ml) + __popcll(mr));
+    uint32_t base_out = 0;
+    if (lane == 0 && total) base_out = atomicAdd(&counts[level + 1], total);
+    base_out = __shfl(base_out, 0, 64);
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const uint32_t ol = base_out + (uint32_t)__popcll(ml & below);
+    const uint32_t orr = base_out + (uint32_t)__popcll(ml) + (uint32_t)__popcll(mr & below);
+    if (push_l && ol < n_all) front_out[ol] = make_uint2(cl, turns);
+    if (push_r && orr < n_all) front_out[orr] = make_uint2(cr, turns + 1);
   }
-  // deepest branch per mesh: one atomic per wavefront when the whole wave
-  // lies in one mesh, else one per branch
-  uint32_t mx = branch ? steps : 0u;
-  const uint32_t m0 = __builtin_amdgcn_readfirstlane(m);
-  const bool one_mesh = !M || __ballot(m != m0) == 0ull;
-  if (one_mesh) {
-    for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
-    if ((threadIdx.x & 63) == 0 && mx && params) atomicMax(&params->max_steps, mx);
-  } else if (branch && mx) {
-    atomicMax(&params->max_steps, mx);
-  }
-  if (!branch) return;
-  // children of this branch sit at depth `steps`
-  const uint32_t g = kchild[x];
-  const uint32_t lb = kleaf[x];
-  const bool deep = steps > (uint32_t)kMaxDepth32;
-  const bool left_leaf = (lb & 1u) || (g - l <= (uint32_t)max_leaf) || deep;
-  const bool right_leaf = (lb & 2u) || (r - (g + 1) <= (uint32_t)max_leaf) || deep;
-  if (left_leaf) leaf_start[base + l] = 1;
-  if (right_leaf) leaf_start[base + g + 1] = 1;
 }
 
 // ---- exclusive scan (3 launches) ----
@@ -942,6 +962,7 @@ __device__ __forceinline__ Box merge_lr(const Box &l, const Box &r) {
 template <int K, bool M>
 __global__ void __launch_bounds__(256) k_leaves(uint32_t n_all, const uint32_t *sidx_all,
                                                NodeOut o, const uint32_t *total_leaves,
+                                               uint32_t *big, uint32_t *big_count,
                                                SegView v) {
   const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
   if (y >= *total_leaves || y >= n_all) return;
@@ -954,6 +975,10 @@ __global__ void __launch_bounds__(256) k_leaves(uint32_t n_all, const uint32_t *
   uint32_t first = o.leaf_first[y], count = o.leaf_count[y];
   if (first >= s.n || count > s.n - first) {
     atomicOr(&params->error, 2u);
+    return;
+  }
+  if (count > kBigLeaf) {  // a whole block does this one (k_leaves_big)
+    big[atomicAdd(big_count, 1u)] = y;
     return;
   }
   const uint32_t *sidx = M ? sidx_all + base : sg.sorted_index;
@@ -986,6 +1011,140 @@ __global__ void __launch_bounds__(256) k_leaves(uint32_t n_all, const uint32_t *
     return;
   }
   store_box_slot(words + 16 * (size_t)p, (int)(link & 1u), b);
+}
+
+// One primitive's packed record and its box candidates: vertices (tri 3,
+// tet 4) or the sphere, each -+ e, in the order the sequential leaf box
+// visits them.
+template <int K>
+__device__ __forceinline__ int pack_prim(const PrimSrc &s, const uint32_t *sidx, uint32_t q,
+                                         float e, void *prims, uint32_t *sorted_faces,
+                                         float lo[4][3], float hi[4][3]) {
+  const uint32_t id = sidx[q];
+  if (K == kTri) {
+    float3 v[3];
+    for (int k = 0; k < 3; k++) v[k] = load_v3(s.V, s.F[3 * (size_t)id + k]);
+    float4 *P = reinterpret_cast<float4 *>(prims) + 3 * (size_t)q;
+    P[0] = make_float4(v[0].x, v[0].y, v[0].z, __uint_as_float(id));
+    P[1] = make_float4(v[1].x - v[0].x, v[1].y - v[0].y, v[1].z - v[0].z, 0.0f);
+    P[2] = make_float4(v[2].x - v[0].x, v[2].y - v[0].y, v[2].z - v[0].z, 0.0f);
+    for (int k = 0; k < 3; k++) {
+      lo[k][0] = v[k].x - e; lo[k][1] = v[k].y - e; lo[k][2] = v[k].z - e;
+      hi[k][0] = v[k].x + e; hi[k][1] = v[k].y + e; hi[k][2] = v[k].z + e;
+    }
+    return 3;
+  } else if (K == kTet) {
+    uint32_t f[4];
+    float3 v[4];
+    for (int k = 0; k < 4; k++) {
+      f[k] = s.F[4 * (size_t)id + k];
+      v[k] = load_v3(s.V, f[k]);
+    }
+    float4 *P = reinterpret_cast<float4 *>(prims) + 4 * (size_t)q;
+    P[0] = make_float4(v[0].x, v[0].y, v[0].z, __uint_as_float(id));
+    P[1] = make_float4(v[1].x, v[1].y, v[1].z, 0.0f);
+    P[2] = make_float4(v[2].x, v[2].y, v[2].z, 0.0f);
+    P[3] = make_float4(v[3].x, v[3].y, v[3].z, 0.0f);
+    reinterpret_cast<uint4 *>(sorted_faces)[q] = make_uint4(f[0], f[1], f[2], f[3]);
+    for (int k = 0; k < 4; k++) {
+      lo[k][0] = v[k].x - e; lo[k][1] = v[k].y - e; lo[k][2] = v[k].z - e;
+      hi[k][0] = v[k].x + e; hi[k][1] = v[k].y + e; hi[k][2] = v[k].z + e;
+    }
+    return 4;
+  } else {
+    const float3 p = load_v3(s.V, id);
+    const float r = s.R ? s.R[id] : s.rconst;
+    reinterpret_cast<float4 *>(prims)[q] = make_float4(p.x, p.y, p.z, r);
+    lo[0][0] = (p.x - r) - e; lo[0][1] = (p.y - r) - e; lo[0][2] = (p.z - r) - e;
+    hi[0][0] = (p.x + r) + e; hi[0][1] = (p.y + r) + e; hi[0][2] = (p.z + r) + e;
+    return 1;
+  }
+}
+
+// A running extreme that remembers where it was seen: the sequential box
+// keeps the first candidate on ties (std::min / std::max), so a parallel
+// reduction breaks ties by the lower candidate index. Only +0 / -0 can tie
+// with different bits.
+struct Ext {
+  float v;
+  uint32_t at;
+};
+__device__ __forceinline__ Ext ext_min(Ext a, Ext b) {
+  return (b.v < a.v || (b.v == a.v && b.at < a.at)) ? b : a;
+}
+__device__ __forceinline__ Ext ext_max(Ext a, Ext b) {
+  return (b.v > a.v || (b.v == a.v && b.at < a.at)) ? b : a;
+}
+
+// One block per big leaf: primitives packed in parallel, the leaf box
+// reduced with the sequential tie rule, then the leaf's own slot and its
+// parent's slot written as k_leaves does for small leaves.
+template <int K, bool M>
+__global__ void __launch_bounds__(256) k_leaves_big(const uint32_t *sidx_all, NodeOut o,
+                                                   const uint32_t *big, const uint32_t *big_count,
+                                                   SegView v) {
+  __shared__ Ext red[6][4];
+  const uint32_t nbig = *big_count;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t j = blockIdx.x; j < nbig; j += gridDim.x) {
+    const uint32_t y = big[j];
+    const uint32_t m = seg_find<M>(v.leaf_base, v.nseg, y);
+    const Seg sg = seg_get<M>(v, m);
+    const PrimSrc &s = sg.src;
+    const uint32_t base = seg_off<M>(v.prim_off, m);
+    const float e = sg.params->kEPS;
+    const uint32_t first = o.leaf_first[y], count = o.leaf_count[y];
+    const uint32_t *sidx = M ? sidx_all + base : sg.sorted_index;
+    Ext ex[6];
+    for (int c = 0; c < 3; c++) {
+      ex[c] = Ext{FLT_MAX, 0xffffffffu};
+      ex[3 + c] = Ext{-FLT_MAX, 0xffffffffu};
+    }
+    for (uint32_t q = first + threadIdx.x; q < first + count; q += blockDim.x) {
+      if (M) sg.sorted_index[q] = sidx[q];
+      float lo[4][3], hi[4][3];
+      const int nc = pack_prim<K>(s, sidx, q, e, sg.prims, sg.sorted_faces, lo, hi);
+      for (int k = 0; k < nc; k++) {
+        const uint32_t at = (q - first) * 4u + (uint32_t)k;
+        for (int c = 0; c < 3; c++) {
+          ex[c] = ext_min(ex[c], Ext{lo[k][c], at});
+          ex[3 + c] = ext_max(ex[3 + c], Ext{hi[k][c], at});
+        }
+      }
+    }
+    for (int c = 0; c < 6; c++) {
+      for (int off = 32; off >= 1; off >>= 1) {
+        Ext o2{__shfl_xor(ex[c].v, off, 64), (uint32_t)__shfl_xor((int)ex[c].at, off, 64)};
+        ex[c] = c < 3 ? ext_min(ex[c], o2) : ext_max(ex[c], o2);
+      }
+      if (lane == 0) red[c][wave] = ex[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      Box b;
+      for (int c = 0; c < 3; c++) {
+        Ext mn = red[c][0], mx = red[3 + c][0];
+        for (int w = 1; w < (int)(blockDim.x / 64); w++) {
+          mn = ext_min(mn, red[c][w]);
+          mx = ext_max(mx, red[3 + c][w]);
+        }
+        b.mn[c] = mn.v;
+        b.mx[c] = mx.v;
+      }
+      const uint32_t node = o.leaf_pos[y];
+      float4 *w4 = reinterpret_cast<float4 *>(sg.nodes) + 4 * (size_t)node;
+      w4[0] = make_float4(b.mn[0], b.mn[1], b.mn[2], 0.0f);
+      w4[1] = make_float4(0.0f, 0.0f, b.mx[0], b.mx[1]);
+      w4[2] = make_float4(b.mx[2], 0.0f, 0.0f, 0.0f);
+      const uint32_t link = o.parent_link[2 * (size_t)base + node];
+      const uint32_t p = link >> 1;
+      if (node == 0 || node >= 2 * s.n - 1 || p >= 2 * s.n - 1)
+        atomicOr(&sg.params->error, 4u);
+      else
+        store_box_slot(reinterpret_cast<float *>(sg.nodes) + 16 * (size_t)p, (int)(link & 1u), b);
+    }
+    __syncthreads();
+  }
 }
 
 // One level of the bottom-up refit: every branch whose children already hold
@@ -1094,7 +1253,10 @@ srf_status run_sort_passes(uint32_t *keys_a, uint32_t n, uint32_t tiles, const S
 struct BuildScratch {
   uint32_t *codes;
   SortScratch ss;
-  uint32_t *kchild, *kleaf, *kother, *kparent, *cls, *leaf_start, *leaf_rank, *partial;
+  uint32_t *kchild, *kleaf, *kother, *cls, *leaf_start, *leaf_rank, *partial;
+  uint2 *front[2];
+  uint32_t *level_counts;  // [0, 25): branches per depth; [32]: big leaves
+  uint32_t *big;
   uint32_t *total;
   NodeOut o;
   Box *partials;
@@ -1119,7 +1281,10 @@ size_t layout_build(Arena &ar, uint32_t n, uint32_t tiles, uint32_t bblocks, uin
   b.kchild = ar.take<uint32_t>(n);
   b.kleaf = ar.take<uint32_t>(n);
   b.kother = ar.take<uint32_t>(n);
-  b.kparent = ar.take<uint32_t>(n);
+  b.front[0] = ar.take<uint2>(n);
+  b.front[1] = ar.take<uint2>(n);
+  b.level_counts = ar.take<uint32_t>(64);
+  b.big = ar.take<uint32_t>(n);
   b.cls = ar.take<uint32_t>(n);
   b.leaf_start = ar.take<uint32_t>(n);
   b.leaf_rank = ar.take<uint32_t>(n);
@@ -1264,9 +1429,17 @@ srf_status build_lbvh_segments(Ctx *ctx, Kind kind, const void *const *descs,
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(N, 256, 2048)), dim3(256), 0, stream,
                      b.leaf_start, 0u, (size_t)N);
   SRF_DISPATCH_M(k_karras, dim3(nblocks), dim3(256), 0, stream, b.codes, N, b.kchild, b.kleaf,
-                 b.kother, b.kparent, v);
-  SRF_DISPATCH_M(k_classify, dim3(nblocks), dim3(256), 0, stream, N, max_leaf, b.kchild,
-                 b.kleaf, b.kother, b.kparent, b.cls, b.leaf_start, v);
+                 b.kother, v);
+  // top-down branch classification, one launch per depth
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(N, 256, 2048)), dim3(256), 0, stream, b.cls,
+                     0u, (size_t)N);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, b.level_counts, 0u, (size_t)64);
+  SRF_DISPATCH_M(k_classify_roots, dim3(1), dim3(64), 0, stream, v, b.front[0], b.level_counts);
+  const uint32_t cblocks = blocks_for(N, 256, 1024);
+  for (uint32_t level = 0; level < (uint32_t)kMaxBranchLevels; level++)
+    SRF_DISPATCH_M(k_classify_level, dim3(cblocks), dim3(256), 0, stream, N, max_leaf, level,
+                   b.kchild, b.kleaf, b.kother, b.front[level & 1], b.front[(level + 1) & 1],
+                   b.level_counts, b.cls, b.leaf_start, v);
   const uint32_t scan_blocks = blocks_for(N, kScanTile);
   hipLaunchKernelGGL(k_scan_reduce, dim3(scan_blocks), dim3(kScanBlock), 0, stream,
                      b.leaf_start, N, b.partial);
@@ -1282,7 +1455,10 @@ srf_status build_lbvh_segments(Ctx *ctx, Kind kind, const void *const *descs,
   }
   SRF_DISPATCH(k_emit, dim3(nblocks), dim3(256), 0, stream, N, max_leaf, b.codes, b.kchild,
                b.kleaf, b.kother, b.cls, b.leaf_rank, b.o, v);
-  SRF_DISPATCH(k_leaves, dim3(nblocks), dim3(256), 0, stream, N, b.ss.vals_a, b.o, b.total, v);
+  SRF_DISPATCH(k_leaves, dim3(nblocks), dim3(256), 0, stream, N, b.ss.vals_a, b.o, b.total,
+               b.big, b.level_counts + 32, v);
+  SRF_DISPATCH(k_leaves_big, dim3(blocks_for(N, 256 * (int)kBigLeaf, 1024)), dim3(256), 0, stream,
+               b.ss.vals_a, b.o, b.big, b.level_counts + 32, v);
   // bottom-up refit, one level per launch (levels are branch steps 23..0)
   const uint32_t rblocks = blocks_for(N, 256, 2048);
   for (int level = kMaxBranchLevels - 1; level >= 0; level--)
