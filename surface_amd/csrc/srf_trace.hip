@@ -137,9 +137,11 @@ __device__ __forceinline__ bool sph_traverse(const float4 *__restrict__ nodes,
         break;
       }
       const float4 *nd = nodes + 4 * (size_t)cur;
-      meta = reinterpret_cast<const uint4 *>(nd)[3];
-      if (meta.x != 0) break;
+      // the whole 64-byte node in one round trip (a leaf ignores the boxes)
       const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+      meta = reinterpret_cast<const uint4 *>(nd)[3];
+      keep_loaded(n0, n1, n2);
+      if (meta.x != 0) break;
       const bool h0 = ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
       const bool h1 = ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
       const int nearc = (s.sbits >> meta.y) & 1u;
@@ -616,9 +618,11 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
         break;
       }
       const float4 *nd = nodes + 4 * (size_t)cur;
-      meta = reinterpret_cast<const uint4 *>(nd)[3];
-      if (meta.x != 0) break;
+      // the whole 64-byte node in one round trip (a leaf ignores the boxes)
       const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+      meta = reinterpret_cast<const uint4 *>(nd)[3];
+      keep_loaded(n0, n1, n2);
+      if (meta.x != 0) break;
       float tm0, tm1;
       const bool g0 = box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm0);
       const bool g1 = box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm1);

@@ -123,6 +123,14 @@ __device__ __forceinline__ bool tri_isect(const RaySetup &s, float4 A, float4 B,
 // the LDS stack (entries kStride words apart). Returns whether an occluder
 // was found (kAny); t/u/v/prim receive the closest hit (t = FLT_MAX, prim =
 // SRF_NO_PRIM on a miss).
+// Empty asm that takes the three values as operands: their loads are issued
+// here, together, instead of being sunk past the branch that follows (one
+// memory round trip instead of two).
+__device__ __forceinline__ void keep_loaded(const float4 &a, const float4 &b, const float4 &c) {
+  asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z),
+               "v"(b.w), "v"(c.x), "v"(c.y), "v"(c.z), "v"(c.w));
+}
+
 template <bool kAny, int kStride>
 __device__ __forceinline__ bool tri_traverse(const float4 *__restrict__ nodes,
                                              const float4 *__restrict__ prims,
@@ -146,9 +154,11 @@ __device__ __forceinline__ bool tri_traverse(const float4 *__restrict__ nodes,
         break;
       }
       const float4 *nd = nodes + 4 * (size_t)cur;
-      meta = reinterpret_cast<const uint4 *>(nd)[3];
-      if (meta.x != 0) break;
+      // the whole 64-byte node in one round trip (a leaf ignores the boxes)
       const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+      meta = reinterpret_cast<const uint4 *>(nd)[3];
+      keep_loaded(n0, n1, n2);
+      if (meta.x != 0) break;
       const bool h0 = ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
       const bool h1 = ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
       const int nearc = (s.sbits >> meta.y) & 1u;
@@ -167,23 +177,37 @@ __device__ __forceinline__ bool tri_traverse(const float4 *__restrict__ nodes,
       }
     }
     if (cur == kDone) break;
-    // leaf: TestLeafNode, render/render_accel_triangle.cc:2386-2433
+    // leaf: TestLeafNode, render/render_accel_triangle.cc:2386-2433; the
+    // next triangle's 48 bytes are in flight while this one is tested
     const uint32_t first = meta.w, end = min(meta.w + meta.z, nprims);
-    for (uint32_t q = first; q < end; q++) {
-      const float4 *P = prims + 3 * (size_t)q;
-      const float4 A = P[0];
-      if (__float_as_uint(A.w) == prev) continue;
-      const float4 B = P[1], C = P[2];
-      float tt, uu, vv;
-      if (tri_isect(s, A, B, C, ds, kEPS, filter, t, tt, uu, vv)) {
-        if (kAny) {
-          if (tt < tlimit) return true;
-        } else {
-          t = tt;
-          u = uu;
-          v = vv;
-          prim = __float_as_uint(A.w);
+    if (first < end) {
+      const float4 *P = prims + 3 * (size_t)first;
+      float4 A = P[0], B = P[1], C = P[2];
+      for (uint32_t q = first; q < end; q++) {
+        float4 nA = A, nB = B, nC = C;
+        if (q + 1 < end) {
+          P += 3;
+          nA = P[0];
+          nB = P[1];
+          nC = P[2];
         }
+        keep_loaded(nA, nB, nC);
+        if (__float_as_uint(A.w) != prev) {
+          float tt, uu, vv;
+          if (tri_isect(s, A, B, C, ds, kEPS, filter, t, tt, uu, vv)) {
+            if (kAny) {
+              if (tt < tlimit) return true;
+            } else {
+              t = tt;
+              u = uu;
+              v = vv;
+              prim = __float_as_uint(A.w);
+            }
+          }
+        }
+        A = nA;
+        B = nB;
+        C = nC;
       }
     }
     if (sp == 0) break;
