@@ -47,6 +47,29 @@ __device__ __forceinline__ uint32_t part_by2_30(uint32_t n) {
   return n;
 }
 
+// Workgroups go to the 8 XCDs round-robin (block b runs on XCD b % 8), each
+// XCD with its own L2. Renumbering the blocks so that consecutive blocks run
+// on the same XCD keeps neighbouring ray tiles, which touch the same nodes or
+// voxels, on the same L2.
+constexpr uint32_t kXcds = 8;
+// Runs of SRF_XCD_GROUP consecutive blocks go to one XCD in turn: locality
+// within a run, balance across the grid (0 = one contiguous eighth per XCD,
+// 1 = the hardware's own round robin). Measured on 1024^2 ray tiles (128
+// rays per block): 128 gave the tetra trace 7.2 -> 5.4 ms and the volume
+// march 2.4 -> 2.05 ms, the triangle trace within 2 %; 0 lost 25 % on
+// triangles to imbalance between the eighths.
+#ifndef SRF_XCD_GROUP
+#define SRF_XCD_GROUP 128
+#endif
+// grids using xcd_block are padded to a multiple of this
+constexpr uint32_t kXcdPad = kXcds * (SRF_XCD_GROUP > 0 ? SRF_XCD_GROUP : 1);
+template <uint32_t kGroup = SRF_XCD_GROUP>
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+  if (kGroup == 0) return x * (gridDim.x / kXcds) + k;
+  return (k / kGroup) * (kXcds * kGroup) + x * kGroup + (k % kGroup);
+}
+
 // Per-build scalars produced on the device and read by later kernels.
 struct alignas(16) BuildParams {
   float bmin[3];       // scene box used for Morton codes

@@ -420,7 +420,7 @@ k_scene_tri(const TlasNode *__restrict__ tl, uint32_t nnodes, const SceneInst *_
   uint32_t *bst = smem + threadIdx.x;
   uint32_t *tst = smem + kTraceStack * kBlock + threadIdx.x;
   if (!counter) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i < n) scene_ray<kBlock>(tl, nnodes, inst, ninst, rays, i, bst, tst, tdepth, hits, inst_out);
     return;
   }
@@ -565,6 +565,7 @@ srf_status scene_launch(const Scene *sc, const srf_launch_config &cfg, const srf
                      sc->depth);
   uint32_t *counter = nullptr;
   uint32_t blocks = (uint32_t)((n + block - 1) / block);
+  blocks = (blocks + kXcdPad - 1) / kXcdPad * kXcdPad;  // see xcd_block
   if (cfg.persistent) {
     const uint32_t waves = (uint32_t)((n + kChunk - 1) / kChunk);
     const uint32_t per_cu = std::max<uint32_t>(cfg.blocks_per_cu, 1u);

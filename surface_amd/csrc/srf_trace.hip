@@ -42,7 +42,7 @@ k_trace_tri(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             srf_hit *__restrict__ hits, const float *__restrict__ tmax_in,
             uint32_t *__restrict__ occluded, uint32_t max_nodes, uint32_t nprims) {
   __shared__ uint32_t stack[kTraceStack * kTraceBlock];
-  const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+  const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
   const RaySetup s = setup_ray(rays, i, flags, prev);
@@ -188,7 +188,7 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded,
             uint32_t max_nodes, uint32_t nprims) {
   __shared__ uint32_t stack[kTraceStack * kTraceBlock];
-  const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+  const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
   const RaySetup s = setup_ray(rays, i, flags, prev);
@@ -574,9 +574,11 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             srf_hit *__restrict__ hits, srf_tet_hit_ext *__restrict__ ext,
             const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded,
             uint32_t max_nodes, uint32_t nprims) {
+  // pending far children as (parent << 1 | slot): the child's box and entry
+  // distance are recomputed from the parent's slot when it is popped (the
+  // same arithmetic as at push time), which halves this stack's LDS
   __shared__ uint32_t st_node[kTraceStack * kTraceBlock];
-  __shared__ float st_tmin[kTraceStack * kTraceBlock];
-  const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+  const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
   const RaySetup s = setup_ray(rays, i, flags, prev);
@@ -598,8 +600,22 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   const float dfirst = s.d[0] != 0.0f ? s.d[0] : (s.d[1] != 0.0f ? s.d[1] : s.d[2]);
   const float tslack = 1e-6f * (cext + omax) / fabsf(dfirst);
   uint32_t *sn = st_node + threadIdx.x;
-  float *stm = st_tmin + threadIdx.x;
   int sp = 0;
+  // pops entries until one whose box the ray enters no later than t
+  auto pop_next = [&]() -> uint32_t {
+    while (sp > 0) {
+      const uint32_t e = sn[(--sp) * kTraceBlock];
+      const uint32_t par = e >> 1;
+      const float4 *pd = nodes + 4 * (size_t)par;
+      const float4 a0 = pd[0], a1 = pd[1], a2 = pd[2];
+      const uint4 pm = reinterpret_cast<const uint4 *>(pd)[3];
+      float tm;
+      const bool g = (e & 1u) ? box_span(s, a0.w, a1.x, a1.y, a2.y, a2.z, a2.w, tm)
+                              : box_span(s, a0.x, a0.y, a0.z, a1.z, a1.w, a2.x, tm);
+      if (g && tm <= t) return (e & 1u) ? pm.w : pm.z;
+    }
+    return kDone;
+  };
   uint32_t cur = kDone;
   {
     // the root's own box is tested first (TetraAccel::Traverse pops node 0)
@@ -632,11 +648,7 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
       const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
       // the reference pushes far then near and pops near first
       if (gn && tn <= t) {
-        if (gf && sp < kTraceStack) {
-          sn[sp * kTraceBlock] = cf;
-          stm[sp * kTraceBlock] = tf;
-          sp++;
-        }
+        if (gf && sp < kTraceStack) sn[(sp++) * kTraceBlock] = (cur << 1) | (uint32_t)(1 - nearc);
         cur = cn;
         continue;
       }
@@ -644,14 +656,7 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
         cur = cf;
         continue;
       }
-      cur = kDone;
-      while (sp > 0) {
-        --sp;
-        if (stm[sp * kTraceBlock] <= t) {
-          cur = sn[sp * kTraceBlock];
-          break;
-        }
-      }
+      cur = pop_next();
       if (cur == kDone) break;
     }
     if (cur == kDone) break;
@@ -659,10 +664,10 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
     const uint32_t count = meta.z, first = meta.w;
     for (uint32_t q = first; q < min(first + count, nprims); q++) {
       const float4 *Pp = prims + 4 * (size_t)q;
-      const float4 p0 = Pp[0];
+      const float4 p0 = Pp[0], p1 = Pp[1], p2 = Pp[2], p3 = Pp[3];  // one round trip
+      keep_loaded(p1, p2, p3);
       const uint32_t id = __float_as_uint(p0.w);
       if (id == prev) continue;
-      const float4 p1 = Pp[1], p2 = Pp[2], p3 = Pp[3];
       {
         const float bx0 = fminf(fminf(p0.x, p1.x), fminf(p2.x, p3.x)) - pad;
         const float by0 = fminf(fminf(p0.y, p1.y), fminf(p2.y, p3.y)) - pad;
@@ -696,14 +701,7 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
         }
       }
     }
-    cur = kDone;
-    while (sp > 0) {
-      --sp;
-      if (stm[sp * kTraceBlock] <= t) {
-        cur = sn[sp * kTraceBlock];
-        break;
-      }
-    }
+    cur = pop_next();
   }
   if (kAny) {
     occluded[i] = 0u;
@@ -752,7 +750,7 @@ k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims
   __shared__ uint32_t stack[kTraceStack * kTraceBlock];
   uint32_t *st = stack + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
-  uint32_t base = blockIdx.x * kTraceBlock + (threadIdx.x & ~63u);
+  uint32_t base = (counter ? blockIdx.x : xcd_block()) * kTraceBlock + (threadIdx.x & ~63u);
   for (;;) {
     if (counter) {
       uint32_t b = 0;
@@ -901,7 +899,11 @@ k_ao_rays_sph(const srf_ray *__restrict__ primary, const srf_hit *__restrict__ h
   }
 }
 
-inline uint32_t trace_blocks(uint64_t n) { return (uint32_t)((n + kTraceBlock - 1) / kTraceBlock); }
+// blocks for n rays, padded to a multiple of kXcds (see xcd_block)
+inline uint32_t trace_blocks(uint64_t n) {
+  const uint64_t b = (n + kTraceBlock - 1) / kTraceBlock;
+  return (uint32_t)((b + kXcdPad - 1) / kXcdPad * kXcdPad);
+}
 inline uint32_t max_nodes(const Accel *a) { return (uint32_t)(2 * a->n - 1); }
 
 }  // namespace

@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_fwd(Volume V, const float *__
   for (int k = threadIdx.x; k < N; k += kVolBlock)
     s_tf[k] = reinterpret_cast<const float4 *>(tf)[k];
   __syncthreads();
-  uint32_t i = blockIdx.x * kVolBlock + threadIdx.x;
+  uint32_t i = xcd_block() * kVolBlock + threadIdx.x;
   if (i >= n) return;
   float o[3], d[3];
   load_ray(rays, i, o, d);
@@ -469,7 +469,7 @@ k_vol_bwd(Volume V, const float *__restrict__ tf, int N, float step,
   win.set = false;
   win.bx = win.by = win.bz = 0;
 
-  const uint32_t i = blockIdx.x * kVolBwdBlock + threadIdx.x;
+  const uint32_t i = xcd_block() * kVolBwdBlock + threadIdx.x;
   float o[3] = {0.0f, 0.0f, 0.0f}, d[3] = {0.0f, 0.0f, 1.0f}, tn = 0.0f, tfar = 0.0f;
   bool live = i < n;
   if (live) {
@@ -620,7 +620,11 @@ k_vol_bwd(Volume V, const float *__restrict__ tf, int N, float step,
   }
 }
 
-inline uint32_t vblocks(uint64_t n, int b) { return (uint32_t)((n + b - 1) / b); }
+// blocks for n rays, padded to a multiple of kXcds (see xcd_block)
+inline uint32_t vblocks(uint64_t n, int b) {
+  const uint64_t g = (n + b - 1) / b;
+  return (uint32_t)((g + kXcdPad - 1) / kXcdPad * kXcdPad);
+}
 
 }  // namespace
 }  // namespace srf
