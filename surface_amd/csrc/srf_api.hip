@@ -54,6 +54,19 @@ srf_status ensure_host_stage(Ctx *ctx, size_t bytes) {
   return SRF_OK;
 }
 
+void free_sub_buffers(Accel *a) {
+  (void)hipFree(a->sub_nodes);
+  (void)hipFree(a->sub_prims);
+  (void)hipFree(a->sub_sidx);
+  (void)hipFree(a->sub_params);
+  a->sub_nodes = a->sub_prims = nullptr;
+  a->sub_sidx = nullptr;
+  a->sub_params = nullptr;
+  a->sub_cap_prims = a->sub_cap_nodes = a->sub_cap_trees = 0;
+  a->sub_nodes_used = a->sub_prims_used = 0;
+  a->nbig = 0;
+}
+
 namespace {
 
 struct DeviceGuard {
@@ -89,6 +102,12 @@ hipError_t alloc_buffers(Accel *a, uint64_t n) {
   if (e == hipSuccess) e = hipMalloc(&a->prims, prim_bytes(kind) * n);
   if (e == hipSuccess) e = hipMalloc((void **)&a->sorted_index, 4 * n);
   if (e == hipSuccess && kind == kTet) e = hipMalloc((void **)&a->sorted_faces, 16 * n);
+  if (e == hipSuccess && kind == kSph) {
+    // every big leaf holds more than kBigLeaf spheres
+    a->big_cap = n / (kBigLeaf + 1) + 1;
+    e = hipMalloc((void **)&a->big_info, 16 * a->big_cap);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->big_count, 4);
+  }
   if (e == hipSuccess) a->capacity = n;
   return e;
 }
@@ -98,9 +117,15 @@ void free_buffers(Accel *a) {
   (void)hipFree(a->prims);
   (void)hipFree(a->sorted_index);
   (void)hipFree(a->sorted_faces);
+  (void)hipFree(a->big_info);
+  (void)hipFree(a->big_count);
   a->nodes = a->prims = nullptr;
   a->sorted_index = a->sorted_faces = nullptr;
+  a->big_info = nullptr;
+  a->big_count = nullptr;
+  a->big_cap = 0;
   a->capacity = 0;
+  free_sub_buffers(a);
 }
 
 srf_status alloc_accel(Ctx *ctx, Kind kind, uint64_t n, uint64_t nv, Accel **out) {
@@ -450,6 +475,10 @@ srf_status srf_accel_export(const srf_accel *accel, void *nodes_host, uint32_t *
       const uint32_t *w = tmp + 16 * i;
       uint32_t o[15];
       memcpy(o, w, 48);
+      // a leaf's second box slot is unused in the reference (left
+      // uninitialised); here words 3 and 4 link a big leaf to its
+      // secondary tree, which is not part of the exported layout
+      if (w[12] != 0) o[3] = o[4] = 0u;
       o[12] = w[14];
       o[13] = w[15];
       o[14] = (w[12] & 0xffffu) | (w[13] << 16);

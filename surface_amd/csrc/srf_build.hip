@@ -30,6 +30,7 @@
 #include <float.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "srf_internal.h"
@@ -48,9 +49,6 @@ constexpr uint32_t kSpinLimit = 1u << 26;
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
-// Leaves with more primitives than this are boxed and packed by a whole block
-// (k_leaves_big); depth-capped leaves of clustered data hold 10^5 and more.
-constexpr uint32_t kBigLeaf = 64;
 
 // (uint32_t)(float): x86-64 truncates through int64; for |x| < 2^31 and NaN
 // (both give 0) this equals the 32-bit truncation.
@@ -110,7 +108,18 @@ struct PrimSrc {
   const float *R;
   float rconst;
   uint32_t n;
+  // spheres: floats from one centre (radius) to the next -- 3 (1) for the
+  // caller's arrays, 4 (4) for packed (centre, radius) records
+  uint32_t vstride, rstride;
 };
+
+__device__ __forceinline__ float3 sph_pos(const PrimSrc &s, uint32_t i) {
+  const float *p = s.V + (size_t)s.vstride * i;
+  return make_float3(p[0], p[1], p[2]);
+}
+__device__ __forceinline__ float sph_rad(const PrimSrc &s, uint32_t i) {
+  return s.R ? s.R[(size_t)s.rstride * i] : s.rconst;
+}
 
 // ---- segments: many meshes in one launch sequence ----
 // A build covers nseg meshes laid end to end in one primitive index space
@@ -184,8 +193,8 @@ __device__ __forceinline__ void scene_add(Box &b, const PrimSrc &s, uint32_t i) 
       box_add(b, v.x, v.y, v.z, FLT_EPSILON * 16);
     }
   } else {
-    float3 p = load_v3(s.V, i);
-    float r = s.R ? s.R[i] : s.rconst;
+    float3 p = sph_pos(s, i);
+    float r = sph_rad(s, i);
     const float e = (float)(FLT_EPSILON * 16.0);
     b.mn[0] = fmin_ref(b.mn[0], (p.x - r) - e);
     b.mn[1] = fmin_ref(b.mn[1], (p.y - r) - e);
@@ -316,7 +325,7 @@ __device__ __forceinline__ uint32_t prim_code(const PrimSrc &s, uint32_t i,
     return morton30(q * (((a.x + b.x) + c.x) + d.x), q * (((a.y + b.y) + c.y) + d.y),
                     q * (((a.z + b.z) + c.z) + d.z), p);
   } else {
-    float3 a = load_v3(s.V, i);
+    float3 a = sph_pos(s, i);
     return morton30(a.x, a.y, a.z, p);
   }
 }
@@ -921,8 +930,8 @@ __device__ __forceinline__ void leaf_box_pack(const PrimSrc &s, const uint32_t *
       P[3] = make_float4(v[3].x, v[3].y, v[3].z, 0.0f);
       reinterpret_cast<uint4 *>(sorted_faces)[q] = make_uint4(f[0], f[1], f[2], f[3]);
     } else {
-      float3 p = load_v3(s.V, id);
-      float r = s.R ? s.R[id] : s.rconst;
+      float3 p = sph_pos(s, id);
+      float r = sph_rad(s, id);
       float lo0 = (p.x - r) - e, lo1 = (p.y - r) - e, lo2 = (p.z - r) - e;
       float hi0 = (p.x + r) + e, hi1 = (p.y + r) + e, hi2 = (p.z + r) + e;
       if (q == first) {
@@ -1052,8 +1061,8 @@ __device__ __forceinline__ int pack_prim(const PrimSrc &s, const uint32_t *sidx,
     }
     return 4;
   } else {
-    const float3 p = load_v3(s.V, id);
-    const float r = s.R ? s.R[id] : s.rconst;
+    const float3 p = sph_pos(s, id);
+    const float r = sph_rad(s, id);
     reinterpret_cast<float4 *>(prims)[q] = make_float4(p.x, p.y, p.z, r);
     lo[0][0] = (p.x - r) - e; lo[0][1] = (p.y - r) - e; lo[0][2] = (p.z - r) - e;
     hi[0][0] = (p.x + r) + e; hi[0][1] = (p.y + r) + e; hi[0][2] = (p.z + r) + e;
@@ -1147,6 +1156,42 @@ __global__ void __launch_bounds__(256) k_leaves_big(const uint32_t *sidx_all, No
   }
 }
 
+// The big leaves of a one-mesh sphere build as (node, first, count, 0); the
+// count goes to out_count (the host checks it against cap).
+__global__ void __launch_bounds__(256) k_big_export(const uint32_t *big,
+                                                   const uint32_t *big_count, NodeOut o,
+                                                   uint4 *out, uint32_t *out_count,
+                                                   uint32_t cap) {
+  const uint32_t nbig = *big_count;
+  const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t j = i0; j < nbig && j < cap; j += gridDim.x * blockDim.x) {
+    const uint32_t y = big[j];
+    out[j] = make_uint4(o.leaf_pos[y], o.leaf_first[y], o.leaf_count[y], 0u);
+  }
+  if (i0 == 0) *out_count = nbig;
+}
+
+// Links each big sphere leaf to its secondary tree: words 3 and 4 of the
+// leaf (its unused second box slot) take the offsets of the tree's nodes
+// and spheres; the secondary builds' error flags join the main one.
+__global__ void __launch_bounds__(256) k_sub_link(float4 *nodes, uint32_t nnodes,
+                                                 const uint4 *link, uint32_t nbig,
+                                                 const BuildParams *sub_params,
+                                                 BuildParams *params) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nbig;
+       j += gridDim.x * blockDim.x) {
+    const uint4 l = link[j];
+    if (sub_params[j].error) atomicOr(&params->error, 64u);
+    if (l.x >= nnodes) {
+      atomicOr(&params->error, 32u);
+      continue;
+    }
+    float *w = reinterpret_cast<float *>(nodes + 4 * (size_t)l.x);
+    w[3] = __uint_as_float(l.y);
+    w[4] = __uint_as_float(l.z);
+  }
+}
+
 // One level of the bottom-up refit: every branch whose children already hold
 // their boxes (deeper levels ran in earlier launches) merges them,
 // MergeBoundingBox(left, right) (render/render_accel_triangle.cc:804-814),
@@ -1218,6 +1263,8 @@ PrimSrc make_src(Kind kind, const void *desc) {
     s.rconst = d->constant_radius;
     s.n = (uint32_t)d->num_particles;
   }
+  s.vstride = 3;
+  s.rstride = 1;
   return s;
 }
 
@@ -1307,25 +1354,25 @@ size_t layout_build(Arena &ar, uint32_t n, uint32_t tiles, uint32_t bblocks, uin
   return ar.off;
 }
 
-}  // namespace
-
 // ------------------------------------------------------------------ host
 
-// Builds nseg meshes of one kind with one launch per stage. nseg == 1 is the
-// plain Build32; nseg > 1 uploads the segment table first (through the
-// context's pinned staging buffer) and gives every mesh its own tree.
-srf_status build_lbvh_segments(Ctx *ctx, Kind kind, const void *const *descs,
-                               Accel *const *accs, uint32_t nseg, hipStream_t stream) {
+// Builds one tree per segment (segs[m].src into segs[m]'s buffers) with one
+// launch per stage. One segment is the plain Build32; more upload the
+// segment table first (through the context's pinned staging buffer) and
+// give every mesh its own tree. With big_out (one-segment sphere builds)
+// the leaves of more than kBigLeaf spheres are listed in
+// big_out->big_info / big_count.
+srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t stream,
+                      Accel *big_out) {
+  const uint32_t nseg = (uint32_t)segs.size();
   if (nseg == 0) return SRF_OK;
   const bool multi = nseg > 1;
-  std::vector<PrimSrc> src(nseg);
   std::vector<uint32_t> offs(4 * ((size_t)nseg + 1), 0u);
   uint32_t *prim_off = offs.data(), *tile_off = prim_off + nseg + 1,
            *bblk_off = tile_off + nseg + 1, *mblk_off = bblk_off + nseg + 1;
   uint64_t total = 0;
   for (uint32_t m = 0; m < nseg; m++) {
-    src[m] = make_src(kind, descs[m]);
-    const uint32_t n = src[m].n;
+    const uint32_t n = segs[m].src.n;
     total += n;
     if (total >= (1ull << 30))
       return set_error(SRF_ERR_UNSUPPORTED, "batch of %llu primitives exceeds 2^30",
@@ -1348,11 +1395,6 @@ srf_status build_lbvh_segments(Ctx *ctx, Kind kind, const void *const *descs,
 
   SegView v{};
   v.nseg = nseg;
-  std::vector<Seg> segs(nseg);
-  for (uint32_t m = 0; m < nseg; m++) {
-    Accel *a = accs[m];
-    segs[m] = Seg{src[m], a->nodes, a->prims, a->sorted_index, a->sorted_faces, a->params};
-  }
   v.one = segs[0];
   if (multi) {
     // table bytes: Seg[nseg] then the four prefix arrays
@@ -1373,7 +1415,7 @@ srf_status build_lbvh_segments(Ctx *ctx, Kind kind, const void *const *descs,
     v.leaf_base = b.leaf_base;
     b.ss.vals_a = b.sidx;
   } else {
-    b.ss.vals_a = accs[0]->sorted_index;  // the sorted index lands in the accel
+    b.ss.vals_a = segs[0].sorted_index;  // the sorted index lands in the accel
   }
   int max_leaf = kind == kTri ? kTriMaxLeaf : (kind == kTet ? kTetMaxLeaf : kSphMaxLeaf);
 
@@ -1382,7 +1424,7 @@ srf_status build_lbvh_segments(Ctx *ctx, Kind kind, const void *const *descs,
     hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(64), 0, stream, v);
   } else {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream,
-                       reinterpret_cast<uint32_t *>(accs[0]->params), 0u,
+                       reinterpret_cast<uint32_t *>(segs[0].params), 0u,
                        sizeof(BuildParams) / 4);
   }
   SRF_HIP(hipGetLastError());
@@ -1446,7 +1488,7 @@ srf_status build_lbvh_segments(Ctx *ctx, Kind kind, const void *const *descs,
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, stream, b.partial, scan_blocks);
   hipLaunchKernelGGL(k_scan_downsweep, dim3(scan_blocks), dim3(kScanBlock), 0, stream,
                      b.leaf_start, N, b.partial, b.leaf_rank, b.total,
-                     multi ? nullptr : accs[0]->params);
+                     multi ? nullptr : segs[0].params);
   SRF_HIP(hipGetLastError());
   if (multi) {
     hipLaunchKernelGGL(k_seg_leaves, dim3(1), dim3(256), 0, stream, v, b.leaf_rank, b.total,
@@ -1464,9 +1506,105 @@ srf_status build_lbvh_segments(Ctx *ctx, Kind kind, const void *const *descs,
   for (int level = kMaxBranchLevels - 1; level >= 0; level--)
     SRF_DISPATCH_M(k_refit_level, dim3(rblocks), dim3(256), 0, stream, N, (uint32_t)level,
                    b.cls, b.o, v);
+  if (big_out && !multi && kind == kSph) {
+    hipLaunchKernelGGL(k_big_export, dim3(blocks_for(big_out->big_cap, 256, 1024)), dim3(256), 0,
+                       stream, b.big, b.level_counts + 32, b.o, big_out->big_info,
+                       big_out->big_count, (uint32_t)big_out->big_cap);
+    SRF_HIP(hipGetLastError());
+  }
 #undef SRF_DISPATCH
 #undef SRF_DISPATCH_M
   return SRF_OK;
+}
+
+// Secondary trees over the big leaves of a finished sphere build: the big
+// leaves' records come to the host (the build's only synchronisation), each
+// leaf becomes one segment of a batched sphere build over its packed
+// spheres, and each big leaf's unused second box slot receives the offsets
+// of its tree's nodes and spheres.
+srf_status build_sph_subtrees(Ctx *ctx, Accel *a, hipStream_t stream) {
+  a->nbig = 0;
+  a->sub_nodes_used = a->sub_prims_used = 0;
+  SRF_HIP(hipStreamSynchronize(stream));
+  uint32_t nbig = 0;
+  SRF_HIP(hipMemcpy(&nbig, a->big_count, 4, hipMemcpyDeviceToHost));
+  if (nbig == 0) return SRF_OK;
+  if (nbig > a->big_cap) return set_error(SRF_ERR_INTERNAL, "big-leaf list overflow");
+  std::vector<uint4> info(nbig);
+  SRF_HIP(hipMemcpy(info.data(), a->big_info, 16 * (size_t)nbig, hipMemcpyDeviceToHost));
+  // a fixed order (by first sphere) makes the layout independent of the
+  // order the build's atomics listed the leaves in
+  std::sort(info.begin(), info.end(), [](const uint4 &x, const uint4 &y) { return x.y < y.y; });
+  uint64_t nodes_total = 0, prims_total = 0;
+  std::vector<uint4> link(nbig);
+  for (uint32_t m = 0; m < nbig; m++) {
+    const uint4 r = info[m];
+    if (r.z <= kBigLeaf || (uint64_t)r.y + r.z > a->n)
+      return set_error(SRF_ERR_INTERNAL, "inconsistent big-leaf record");
+    link[m] = make_uint4(r.x, (uint32_t)nodes_total, (uint32_t)prims_total, r.z);
+    nodes_total += 2 * (uint64_t)r.z - 1;
+    prims_total += r.z;
+  }
+  if (nodes_total >= (1ull << 32))
+    return set_error(SRF_ERR_UNSUPPORTED, "secondary trees exceed 2^32 nodes");
+  // buffers grow only
+  if (prims_total > a->sub_cap_prims || nodes_total > a->sub_cap_nodes || nbig > a->sub_cap_trees) {
+    free_sub_buffers(a);
+    SRF_HIP(hipMalloc(&a->sub_nodes, 64 * nodes_total));
+    a->sub_cap_nodes = nodes_total;
+    SRF_HIP(hipMalloc(&a->sub_prims, 16 * prims_total));
+    SRF_HIP(hipMalloc((void **)&a->sub_sidx, 4 * prims_total));
+    a->sub_cap_prims = prims_total;
+    SRF_HIP(hipMalloc((void **)&a->sub_params, sizeof(BuildParams) * (size_t)nbig));
+    a->sub_cap_trees = nbig;
+  }
+  std::vector<Seg> segs(nbig);
+  const float *packed = reinterpret_cast<const float *>(a->prims);
+  for (uint32_t m = 0; m < nbig; m++) {
+    const uint32_t first = info[m].y, count = info[m].z;
+    PrimSrc s{};
+    s.V = packed + 4 * (size_t)first;
+    s.R = s.V + 3;
+    s.n = count;
+    s.vstride = 4;
+    s.rstride = 4;
+    segs[m] = Seg{s, (char *)a->sub_nodes + 64 * (size_t)link[m].y,
+                  (char *)a->sub_prims + 16 * (size_t)link[m].z, a->sub_sidx + link[m].z,
+                  nullptr, a->sub_params + m};
+  }
+  srf_status st = build_segs(ctx, kSph, segs, stream, nullptr);
+  if (st != SRF_OK) return st;
+  // the link table travels through big_info (no longer needed as such)
+  st = ensure_host_stage(ctx, 16 * (size_t)nbig);
+  if (st != SRF_OK) return st;
+  memcpy(ctx->host_stage, link.data(), 16 * (size_t)nbig);
+  SRF_HIP(hipMemcpyAsync(a->big_info, ctx->host_stage, 16 * (size_t)nbig, hipMemcpyHostToDevice,
+                         stream));
+  SRF_HIP(hipEventRecord(ctx->stage_done, stream));
+  hipLaunchKernelGGL(k_sub_link, dim3(blocks_for(nbig, 256, 1024)), dim3(256), 0, stream,
+                     reinterpret_cast<float4 *>(a->nodes), (uint32_t)(2 * a->n - 1), a->big_info,
+                     nbig, a->sub_params, a->params);
+  SRF_HIP(hipGetLastError());
+  a->nbig = nbig;
+  a->sub_nodes_used = nodes_total;
+  a->sub_prims_used = prims_total;
+  return SRF_OK;
+}
+
+}  // namespace
+
+srf_status build_lbvh_segments(Ctx *ctx, Kind kind, const void *const *descs,
+                               Accel *const *accs, uint32_t nseg, hipStream_t stream) {
+  std::vector<Seg> segs(nseg);
+  for (uint32_t m = 0; m < nseg; m++) {
+    Accel *a = accs[m];
+    segs[m] = Seg{make_src(kind, descs[m]), a->nodes, a->prims, a->sorted_index,
+                  a->sorted_faces, a->params};
+  }
+  srf_status st = build_segs(ctx, kind, segs, stream,
+                             (nseg == 1 && kind == kSph) ? accs[0] : nullptr);
+  if (st != SRF_OK || nseg != 1 || kind != kSph) return st;
+  return build_sph_subtrees(ctx, accs[0], stream);
 }
 
 srf_status build_lbvh(Ctx *ctx, Kind kind, const void *desc, hipStream_t stream, Accel *acc) {

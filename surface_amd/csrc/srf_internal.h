@@ -33,6 +33,10 @@ constexpr int kMaxDepth32 = 22;  // render/render_accel_triangle.cc:35
 // 22 is a leaf), so a traversal stack never holds more than 25 entries.
 constexpr int kMaxBranchLevels = 24;
 constexpr int kStackSize = 32;
+// Leaves with more primitives than this are packed by a whole block during
+// the build; for spheres each such leaf (depth-capped leaves of clustered
+// particles hold 10^5 and more) also gets a secondary tree over its spheres.
+constexpr uint32_t kBigLeaf = 64;
 
 enum Kind : uint32_t { kTri = 0, kTet = 1, kSph = 2 };
 
@@ -99,6 +103,27 @@ struct Accel {
   uint32_t *sorted_faces;  // tets: original 4 vertex ids per sorted tet
   BuildParams *params;   // device
   hipEvent_t built;      // recorded on the build stream
+  // spheres: the leaves of more than kBigLeaf spheres and their secondary
+  // trees (one Build32 tree per such leaf, built over its packed spheres;
+  // the leaf's unused second box slot links to it)
+  uint4 *big_info;       // per big leaf: (node, first, count, 0) after the build
+  uint32_t *big_count;   // device counter, written by the build
+  uint64_t big_cap;      // big_info entries
+  uint32_t nbig;         // big leaves of the last build (host)
+  void *sub_nodes;       // 64-byte nodes of all secondary trees
+  void *sub_prims;       // their packed spheres (16 B)
+  uint32_t *sub_sidx;    // a secondary tree's sphere -> position in its leaf
+  BuildParams *sub_params;
+  uint64_t sub_cap_prims, sub_cap_nodes, sub_cap_trees;
+  uint64_t sub_nodes_used, sub_prims_used;
+};
+
+// Device view of the secondary sphere trees handed to the traversal.
+struct SphSub {
+  const float4 *nodes;
+  const float4 *prims;
+  const uint32_t *sidx;
+  uint32_t nnodes, nprims;  // valid extents of nodes / prims
 };
 
 struct Ctx {
@@ -141,6 +166,8 @@ srf_status build_lbvh(Ctx *ctx, Kind kind, const void *desc, hipStream_t stream,
                       Accel *acc);
 srf_status build_lbvh_segments(Ctx *ctx, Kind kind, const void *const *descs,
                                Accel *const *accs, uint32_t nseg, hipStream_t stream);
+// Frees a sphere accelerator's secondary-tree buffers (srf_api.hip).
+void free_sub_buffers(Accel *a);
 srf_status launch_morton(Kind kind, const void *desc, const float bmin[3],
                          const float bmax[3], uint32_t *codes, hipStream_t s);
 srf_status sort_pairs30(Ctx *ctx, const uint32_t *codes_in, uint32_t n,

@@ -113,18 +113,128 @@ __device__ __forceinline__ bool sphere_isect(const RaySetup &s, float4 c, float 
   return false;
 }
 
+// Entry distance bound of a secondary-tree box for the sphere test: false
+// when no sphere inside can report a hit (for any t); else `lo` is at most
+// the distance any sphere inside can report. The box holds the spheres'
+// extents. SphereIsect's float arithmetic moves its roots away from the exact
+// ones: with u = 2^-24, its discriminant carries an error of at most about
+// 90 u |d|^2 (|o - c|^2 + r^2), so a reported root lies no nearer than the
+// entry into the ball of radius r + 1.2e-3 sqrt(|o - c|^2 + r^2) around the
+// centre (plus terms of order u |o - c|). The box is grown by 2e-3 of the
+// farthest distance from the origin to it, which bounds that for every
+// sphere inside. A ray whose origin is inside the grown box gets lo < 0.
+__device__ __forceinline__ bool sub_box(const RaySetup &s, float x0, float y0, float z0,
+                                        float x1, float y1, float z1, float &lo) {
+  const float fx = fmaxf(fabsf(s.o[0] - x0), fabsf(s.o[0] - x1));
+  const float fy = fmaxf(fabsf(s.o[1] - y0), fabsf(s.o[1] - y1));
+  const float fz = fmaxf(fabsf(s.o[2] - z0), fabsf(s.o[2] - z1));
+  const float pad = 2e-3f * sqrtf((fx * fx + fy * fy) + fz * fz);
+  const float b0[3] = {x0 - pad, y0 - pad, z0 - pad}, b1[3] = {x1 + pad, y1 + pad, z1 + pad};
+  float tn = -FLT_MAX, tf = FLT_MAX;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (!(fabsf(s.inv[k]) <= FLT_MAX)) {
+      // direction component zero (or too small for a finite reciprocal):
+      // the ray stays in its plane
+      if (s.o[k] < b0[k] || s.o[k] > b1[k]) return false;
+    } else {
+      const float a = (b0[k] - s.o[k]) * s.inv[k], b = (b1[k] - s.o[k]) * s.inv[k];
+      tn = fmaxf(tn, fminf(a, b));
+      tf = fminf(tf, fmaxf(a, b));
+    }
+  }
+  lo = tn;
+  return tn <= tf && tf >= 0.0f;
+}
+
+// One big leaf through its secondary tree (the leaf's spheres, built as a
+// tree of their own). The reference scans the leaf in order and a sphere
+// replaces the kept one only when strictly closer, so the leaf yields the
+// smallest (t, position) among the spheres whose test reports t below the
+// distance on entry; the secondary tree finds that pair in any order,
+// skipping boxes whose bound lies beyond the kept distance (kAny: at or
+// beyond tlimit). `st` holds kTraceStack pending nodes (stride kStride).
+template <bool kAny, int kStride>
+__device__ __forceinline__ bool sph_big_leaf(const SphSub &sub, uint32_t nbase, uint32_t pbase,
+                                             uint32_t first, uint32_t count,
+                                             const uint32_t *__restrict__ sorted_index,
+                                             const RaySetup &s, uint32_t prev, double prevT,
+                                             const float prevN[3], float tlimit, uint32_t *st,
+                                             float &t, uint32_t &best) {
+  bool kept = false;  // `best` is one of this leaf's spheres
+  int sp = 0;
+  uint32_t cur = 0;
+  const uint32_t nnodes = 2 * count - 1;
+  for (;;) {
+    uint4 meta;
+    for (;;) {
+      if (cur >= nnodes || nbase + cur >= sub.nnodes) {
+        cur = kDone;
+        break;
+      }
+      const float4 *nd = sub.nodes + 4 * (size_t)(nbase + cur);
+      const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+      meta = reinterpret_cast<const uint4 *>(nd)[3];
+      keep_loaded(n0, n1, n2);
+      if (meta.x != 0) break;
+      float l0, l1;
+      bool h0 = sub_box(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, l0);
+      bool h1 = sub_box(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, l1);
+      h0 = h0 && (kAny ? l0 < tlimit : l0 <= t);
+      h1 = h1 && (kAny ? l1 < tlimit : l1 <= t);
+      const bool one_first = h1 && (!h0 || l1 < l0);
+      const uint32_t cn = one_first ? meta.w : meta.z, cf = one_first ? meta.z : meta.w;
+      if (h0 && h1) {
+        if (sp < kTraceStack) st[(sp++) * kStride] = cf;
+        cur = cn;
+      } else if (h0 || h1) {
+        cur = cn;
+      } else if (sp == 0) {
+        cur = kDone;
+        break;
+      } else {
+        cur = st[(--sp) * kStride];
+      }
+    }
+    if (cur == kDone) break;
+    const uint32_t f0 = meta.w, fend = min(meta.w + meta.z, count);
+    for (uint32_t k = f0; k < fend; k++) {
+      const uint32_t j = pbase + k;
+      if (j >= sub.nprims) break;
+      const float4 c = sub.prims[j];
+      const uint32_t loc = sub.sidx[j];
+      if (loc >= count) continue;
+      const uint32_t q = first + loc;
+      const bool self = (prev != SRF_NO_PRIM) && (sorted_index[q] == prev);
+      float tt;
+      if (!sphere_isect(s, c, INFINITY, prevT, prevN, self, tt)) continue;
+      if (kAny) {
+        if (tt < tlimit) return true;
+      } else if (tt < t || (tt == t && kept && q < best)) {
+        t = tt;
+        best = q;
+        kept = true;
+      }
+    }
+    if (sp == 0) break;
+    cur = st[(--sp) * kStride];
+  }
+  return false;
+}
+
 // Closest (kAny: any closer than tlimit) sphere of one ray,
 // ParticleAccel::Traverse (render/render_accel_particle.cc:3053-3218).
 // `best` receives the sorted position of the hit sphere (SRF_NO_PRIM on a
-// miss), t its distance.
+// miss), t its distance. `st` holds 2 kTraceStack entries (stride kStride):
+// the tree's pending nodes, then a big leaf's.
 template <bool kAny, int kStride>
 __device__ __forceinline__ bool sph_traverse(const float4 *__restrict__ nodes,
                                              const float4 *__restrict__ prims,
                                              const uint32_t *__restrict__ sorted_index,
                                              uint32_t max_nodes, uint32_t nprims,
-                                             const RaySetup &s, uint32_t prev, double prevT,
-                                             const float prevN[3], float tlimit, uint32_t *st,
-                                             float &t, uint32_t &best) {
+                                             const SphSub &sub, const RaySetup &s, uint32_t prev,
+                                             double prevT, const float prevN[3], float tlimit,
+                                             uint32_t *st, float &t, uint32_t &best) {
   t = kAny ? tlimit : FLT_MAX;
   best = SRF_NO_PRIM;
   int sp = 0;
@@ -160,6 +270,19 @@ __device__ __forceinline__ bool sph_traverse(const float4 *__restrict__ nodes,
       }
     }
     if (cur == kDone) break;
+    if (meta.z > kBigLeaf && sub.nodes) {
+      // words 3 and 4 of a big leaf: its secondary tree's node and sphere offsets
+      const float4 *nd = nodes + 4 * (size_t)cur;
+      const uint32_t nbase = __float_as_uint(nd[0].w), pbase = __float_as_uint(nd[1].x);
+      if (meta.w < nprims && meta.z <= nprims - meta.w &&
+          sph_big_leaf<kAny, kStride>(sub, nbase, pbase, meta.w, meta.z, sorted_index, s, prev,
+                                      prevT, prevN, tlimit, st + kTraceStack * kStride, t,
+                                      best))
+        return true;
+      if (sp == 0) break;
+      cur = st[(--sp) * kStride];
+      continue;
+    }
     const uint32_t first = meta.w, end = min(meta.w + meta.z, nprims);
     for (uint32_t q = first; q < end; q++) {
       const float4 c = prims[q];
@@ -186,8 +309,8 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             const uint32_t *__restrict__ sorted_index, const srf_ray *__restrict__ rays,
             const srf_sphere_prev *__restrict__ prevs, uint32_t n, srf_hit *__restrict__ hits,
             const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded,
-            uint32_t max_nodes, uint32_t nprims) {
-  __shared__ uint32_t stack[kTraceStack * kTraceBlock];
+            uint32_t max_nodes, uint32_t nprims, SphSub sub) {
+  __shared__ uint32_t stack[2 * kTraceStack * kTraceBlock];
   const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -202,7 +325,7 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   float t;
   uint32_t best;
   const bool occ = sph_traverse<kAny, kTraceBlock>(nodes, prims, sorted_index, max_nodes, nprims,
-                                                   s, prev, prevT, prevN, tlimit,
+                                                   sub, s, prev, prevT, prevN, tlimit,
                                                    stack + threadIdx.x, t, best);
   if (kAny) {
     occluded[i] = occ ? 1u : 0u;
@@ -741,8 +864,8 @@ k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims
               const uint32_t *__restrict__ sorted_index, const BuildParams *__restrict__ params,
               const srf_ray *__restrict__ rays, const float *__restrict__ tmax_in,
               const uint32_t *__restrict__ order, uint32_t n, uint32_t *__restrict__ occluded,
-              uint32_t max_nodes, uint32_t nprims, uint32_t *counter) {
-  __shared__ uint32_t stack[kTraceStack * kTraceBlock];
+              uint32_t max_nodes, uint32_t nprims, uint32_t *counter, SphSub sub) {
+  __shared__ uint32_t stack[(K == kSph ? 2 : 1) * kTraceStack * kTraceBlock];
   uint32_t *st = stack + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t base = (counter ? blockIdx.x : xcd_block()) * kTraceBlock + (threadIdx.x & ~63u);
@@ -771,8 +894,8 @@ k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims
         const float zero[3] = {0.0f, 0.0f, 0.0f};
         float t;
         uint32_t best;
-        occ = sph_traverse<true, kTraceBlock>(nodes, prims, sorted_index, max_nodes, nprims, s,
-                                              prev, 0.0, zero, tlimit, st, t, best);
+        occ = sph_traverse<true, kTraceBlock>(nodes, prims, sorted_index, max_nodes, nprims,
+                                              sub, s, prev, 0.0, zero, tlimit, st, t, best);
       }
       occluded[i] = occ ? 1u : 0u;
     }
@@ -900,6 +1023,17 @@ inline uint32_t trace_blocks(uint64_t n) {
   return (uint32_t)((b + kXcdPad - 1) / kXcdPad * kXcdPad);
 }
 inline uint32_t max_nodes(const Accel *a) { return (uint32_t)(2 * a->n - 1); }
+inline SphSub sph_sub(const Accel *a) {
+  SphSub s{};
+  if (a->kind == kSph && a->nbig) {
+    s.nodes = (const float4 *)a->sub_nodes;
+    s.prims = (const float4 *)a->sub_prims;
+    s.sidx = a->sub_sidx;
+    s.nnodes = (uint32_t)a->sub_nodes_used;
+    s.nprims = (uint32_t)a->sub_prims_used;
+  }
+  return s;
+}
 
 }  // namespace
 
@@ -929,7 +1063,8 @@ srf_status trace_sph(const Accel *a, const srf_ray *rays, const srf_sphere_prev 
   if (n == 0) return SRF_OK;
   hipLaunchKernelGGL(k_trace_sph<false>, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
                      (const float4 *)a->nodes, (const float4 *)a->prims, a->sorted_index, rays,
-                     prev, (uint32_t)n, hits, nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
+                     prev, (uint32_t)n, hits, nullptr, nullptr, max_nodes(a), (uint32_t)a->n,
+                     sph_sub(a));
   SRF_HIP(hipGetLastError());
   return SRF_OK;
 }
@@ -950,7 +1085,7 @@ srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax, uin
   else
     hipLaunchKernelGGL(k_trace_sph<true>, g, b, 0, s, (const float4 *)a->nodes,
                        (const float4 *)a->prims, a->sorted_index, rays, nullptr, (uint32_t)n,
-                       nullptr, tmax, occluded, max_nodes(a), (uint32_t)a->n);
+                       nullptr, tmax, occluded, max_nodes(a), (uint32_t)a->n, sph_sub(a));
   SRF_HIP(hipGetLastError());
   return SRF_OK;
 }
@@ -997,12 +1132,12 @@ srf_status trace_any_sorted(Ctx *ctx, const Accel *a, const srf_ray *rays, const
     hipLaunchKernelGGL(k_any_ordered<kTri>, dim3(blocks), dim3(kTraceBlock), 0, s,
                        (const float4 *)a->nodes, (const float4 *)a->prims, a->sorted_index,
                        a->params, rays, tmax, order, nn, occluded, max_nodes(a), (uint32_t)a->n,
-                       counter);
+                       counter, SphSub{});
   else
     hipLaunchKernelGGL(k_any_ordered<kSph>, dim3(blocks), dim3(kTraceBlock), 0, s,
                        (const float4 *)a->nodes, (const float4 *)a->prims, a->sorted_index,
                        a->params, rays, tmax, order, nn, occluded, max_nodes(a), (uint32_t)a->n,
-                       counter);
+                       counter, sph_sub(a));
   SRF_HIP(hipGetLastError());
   return SRF_OK;
 }

@@ -236,6 +236,30 @@ def deep_chain_points(dup=40, radius=1e-3):
 _ICO_V = None
 
 
+def clustered_spheres(n_bg=20_000, clusters=6, per_cluster=3_000, spread=2e-3,
+                      seed=BASE_SEED + 8):
+    """Background Plummer spheres plus dense clusters whose centres share one
+    30-bit Morton cell, so Build32 makes leaves of thousands of spheres; each
+    cluster also holds exact duplicates (same centre and radius) and shared
+    centres with different radii, which tie in t."""
+    rng = _rng(seed)
+    P, R = plummer_spheres(n_bg, seed=seed + 1, mu=-7.0)
+    cs = rng.uniform(-0.6, 0.6, (clusters, 3))
+    parts_p, parts_r = [P], [R]
+    for k in range(clusters):
+        p = cs[k] + rng.uniform(-spread, spread, (per_cluster, 3))
+        r = rng.lognormal(np.log(spread * 0.05), 0.6, per_cluster)
+        dup = rng.choice(per_cluster, per_cluster // 10, replace=False)
+        p[dup[1::2]] = p[dup[0::2]][: len(dup[1::2])]
+        r[dup[1::4]] = r[dup[0::4]][: len(dup[1::4])]
+        parts_p.append(p)
+        parts_r.append(r)
+    P = np.concatenate(parts_p).astype(np.float32)
+    R = np.concatenate(parts_r).astype(np.float32)
+    perm = rng.permutation(len(P))
+    return np.ascontiguousarray(P[perm]), np.ascontiguousarray(R[perm]), cs.astype(np.float32)
+
+
 def _icosahedron():
     t = (1.0 + 5.0 ** 0.5) / 2.0
     V = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
