@@ -237,7 +237,9 @@ __global__ void __launch_bounds__(256) k_bounds(SegView v, Box *partials,
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) wbox[wave] = b;
   __syncthreads();
-  if (threadIdx.x == 0) {
+  // a mesh with one block needs no partials, fence or ticket (many small
+  // meshes: each device-scope fence writes back the L2)
+  if (threadIdx.x == 0 && nblk > 1) {
     Box r = wbox[0];
     for (int w = 1; w < 4; w++) box_merge(r, wbox[w]);
     partials[blockIdx.x] = r;
@@ -250,20 +252,25 @@ __global__ void __launch_bounds__(256) k_bounds(SegView v, Box *partials,
        i += gridDim.x * blockDim.x)
     clear[i] = 0;
   __syncthreads();
-  if (!last) return;
-  __threadfence();
-  box_init(b);
-  for (uint32_t i = b0 + threadIdx.x; i < b0 + nblk; i += blockDim.x) {
-    Box o;
-    for (int k = 0; k < 3; k++) {
-      o.mn[k] = __hip_atomic_load(&partials[i].mn[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      o.mx[k] = __hip_atomic_load(&partials[i].mx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (nblk > 1) {
+    if (!last) return;
+    __threadfence();
+    box_init(b);
+    for (uint32_t i = b0 + threadIdx.x; i < b0 + nblk; i += blockDim.x) {
+      Box o;
+      for (int k = 0; k < 3; k++) {
+        o.mn[k] =
+            __hip_atomic_load(&partials[i].mn[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        o.mx[k] =
+            __hip_atomic_load(&partials[i].mx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      box_merge(b, o);
     }
-    box_merge(b, o);
+    box_wave_reduce(b);
+    __syncthreads();
+    if (lane == 0) wbox[wave] = b;
+    __syncthreads();
   }
-  box_wave_reduce(b);
-  if (lane == 0) wbox[wave] = b;
-  __syncthreads();
   if (threadIdx.x == 0) {
     Box r = wbox[0];
     for (int w = 1; w < 4; w++) box_merge(r, wbox[w]);
@@ -650,8 +657,16 @@ template <bool M>
 __global__ void __launch_bounds__(64) k_classify_roots(SegView v, uint2 *front,
                                                       uint32_t *counts) {
   for (uint32_t m = threadIdx.x; m < v.nseg; m += blockDim.x)
-    front[m] = make_uint2(seg_off<M>(v.prim_off, m), 0u);
+    front[m] = make_uint2(seg_off<M>(v.prim_off, m), m << 8);
   if (threadIdx.x == 0) counts[0] = v.nseg;
+}
+
+// Start of level `level`'s branches in the level-ordered branch list: the
+// levels are stored one after another, level 0 (the roots) first.
+__device__ __forceinline__ uint32_t level_start(const uint32_t *counts, uint32_t level) {
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < level; k++) s += counts[k];
+  return s;
 }
 
 template <bool M>
@@ -659,10 +674,16 @@ __global__ void __launch_bounds__(256) k_classify_level(uint32_t n_all, int max_
                                                        uint32_t level, const uint32_t *kchild,
                                                        const uint32_t *kleaf,
                                                        const uint32_t *kother,
-                                                       const uint2 *front_in, uint2 *front_out,
-                                                       uint32_t *counts, uint32_t *cls,
-                                                       uint32_t *leaf_start, SegView v) {
+                                                       uint2 *levels, uint32_t *counts,
+                                                       uint32_t *cls, uint32_t *leaf_start,
+                                                       SegView v) {
+  // this level's branches and, after them, the next level's: entries are
+  // (Karras index, turns | mesh << 8)
   const uint32_t count = counts[level];
+  const uint32_t in_off = level_start(counts, level);
+  const uint2 *front_in = levels + in_off;
+  uint2 *front_out = levels + in_off + count;
+  const uint32_t out_cap = n_all - in_off - count;
   const uint32_t lane = threadIdx.x & 63u;
   const bool deep = level > (uint32_t)kMaxDepth32;
   // whole wavefronts step through the frontier, so the appends below can be
@@ -672,12 +693,14 @@ __global__ void __launch_bounds__(256) k_classify_level(uint32_t n_all, int max_
     const uint32_t j = j0 + lane;
     uint32_t push_l = 0, push_r = 0, cl = 0, cr = 0, turns = 0;
     BuildParams *params = nullptr;
+    uint32_t m = 0;
     if (j < count) {
       const uint2 e = front_in[j];
       const uint32_t x = e.x;
-      turns = e.y;
-      uint32_t m, base, n;
-      seg_local<M>(v, n_all, x, m, base, n);
+      turns = e.y & 255u;
+      m = e.y >> 8;
+      const uint32_t base = seg_off<M>(v.prim_off, m);
+      const uint32_t n = M ? v.prim_off[m + 1] - base : n_all;
       const uint32_t i = x - base;
       params = seg_get<M>(v, m).params;
       if (i + 1 < n) {
@@ -719,8 +742,8 @@ __global__ void __launch_bounds__(256) k_classify_level(uint32_t n_all, int max_
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     const uint32_t ol = base_out + (uint32_t)__popcll(ml & below);
     const uint32_t orr = base_out + (uint32_t)__popcll(ml) + (uint32_t)__popcll(mr & below);
-    if (push_l && ol < n_all) front_out[ol] = make_uint2(cl, turns);
-    if (push_r && orr < n_all) front_out[orr] = make_uint2(cr, turns + 1);
+    if (push_l && ol < out_cap) front_out[ol] = make_uint2(cl, turns | (m << 8));
+    if (push_r && orr < out_cap) front_out[orr] = make_uint2(cr, (turns + 1) | (m << 8));
   }
 }
 
@@ -1198,15 +1221,20 @@ __global__ void __launch_bounds__(256) k_sub_link(float4 *nodes, uint32_t nnodes
 // into its parent's slot; the root's merge is BoundingBox().
 template <bool M>
 __global__ void __launch_bounds__(256) k_refit_level(uint32_t n_all, uint32_t level,
-                                                    const uint32_t *cls, NodeOut o,
+                                                    const uint2 *levels,
+                                                    const uint32_t *counts, NodeOut o,
                                                     SegView v) {
-  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n_all;
-       x += gridDim.x * blockDim.x) {
-    const uint32_t c = cls[x];
-    if (!(c & 1u) || ((c >> 1) & 127u) != level) continue;
-    uint32_t m, base, n;
-    seg_local<M>(v, n_all, x, m, base, n);
-    if (x - base + 1 >= n) continue;
+  // the branches of this level, as the classification listed them
+  const uint32_t count = counts[level];
+  const uint2 *list = levels + level_start(counts, level);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count;
+       j += gridDim.x * blockDim.x) {
+    const uint2 e = list[j];
+    const uint32_t x = e.x, m = e.y >> 8;
+    if (x >= n_all || (M && m >= v.nseg)) continue;
+    const uint32_t base = seg_off<M>(v.prim_off, m);
+    const uint32_t n = M ? v.prim_off[m + 1] - base : n_all;
+    if (x < base || x - base + 1 >= n) continue;
     const Seg sg = seg_get<M>(v, m);
     const uint32_t pos = o.branch_pos[x];
     if (pos >= 2 * n - 1) {
@@ -1301,7 +1329,7 @@ struct BuildScratch {
   uint32_t *codes;
   SortScratch ss;
   uint32_t *kchild, *kleaf, *kother, *cls, *leaf_start, *leaf_rank, *partial;
-  uint2 *front[2];
+  uint2 *levels;
   uint32_t *level_counts;  // [0, 25): branches per depth; [32]: big leaves
   uint32_t *big;
   uint32_t *total;
@@ -1328,8 +1356,7 @@ size_t layout_build(Arena &ar, uint32_t n, uint32_t tiles, uint32_t bblocks, uin
   b.kchild = ar.take<uint32_t>(n);
   b.kleaf = ar.take<uint32_t>(n);
   b.kother = ar.take<uint32_t>(n);
-  b.front[0] = ar.take<uint2>(n);
-  b.front[1] = ar.take<uint2>(n);
+  b.levels = ar.take<uint2>(n);
   b.level_counts = ar.take<uint32_t>(64);
   b.big = ar.take<uint32_t>(n);
   b.cls = ar.take<uint32_t>(n);
@@ -1379,7 +1406,8 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
                        (unsigned long long)total);
     prim_off[m + 1] = prim_off[m] + n;
     tile_off[m + 1] = tile_off[m] + sort_tiles(n);
-    bblk_off[m + 1] = bblk_off[m] + blocks_for(n, 256, kBoundsBlocks);
+    // batches: a block per 4096 primitives, so most small meshes take one
+    bblk_off[m + 1] = bblk_off[m] + blocks_for(n, multi ? 256 * 16 : 256, kBoundsBlocks);
     mblk_off[m + 1] = mblk_off[m] + blocks_for(n, 256, 4096);
   }
   const uint32_t N = prim_off[nseg], tiles = tile_off[nseg];
@@ -1476,12 +1504,12 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(N, 256, 2048)), dim3(256), 0, stream, b.cls,
                      0u, (size_t)N);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, b.level_counts, 0u, (size_t)64);
-  SRF_DISPATCH_M(k_classify_roots, dim3(1), dim3(64), 0, stream, v, b.front[0], b.level_counts);
+  SRF_DISPATCH_M(k_classify_roots, dim3(1), dim3(64), 0, stream, v, b.levels, b.level_counts);
   const uint32_t cblocks = blocks_for(N, 256, 1024);
   for (uint32_t level = 0; level < (uint32_t)kMaxBranchLevels; level++)
     SRF_DISPATCH_M(k_classify_level, dim3(cblocks), dim3(256), 0, stream, N, max_leaf, level,
-                   b.kchild, b.kleaf, b.kother, b.front[level & 1], b.front[(level + 1) & 1],
-                   b.level_counts, b.cls, b.leaf_start, v);
+                   b.kchild, b.kleaf, b.kother, b.levels, b.level_counts, b.cls, b.leaf_start,
+                   v);
   const uint32_t scan_blocks = blocks_for(N, kScanTile);
   hipLaunchKernelGGL(k_scan_reduce, dim3(scan_blocks), dim3(kScanBlock), 0, stream,
                      b.leaf_start, N, b.partial);
@@ -1505,7 +1533,7 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
   const uint32_t rblocks = blocks_for(N, 256, 2048);
   for (int level = kMaxBranchLevels - 1; level >= 0; level--)
     SRF_DISPATCH_M(k_refit_level, dim3(rblocks), dim3(256), 0, stream, N, (uint32_t)level,
-                   b.cls, b.o, v);
+                   b.levels, b.level_counts, b.o, v);
   if (big_out && !multi && kind == kSph) {
     hipLaunchKernelGGL(k_big_export, dim3(blocks_for(big_out->big_cap, 256, 1024)), dim3(256), 0,
                        stream, b.big, b.level_counts + 32, b.o, big_out->big_info,
