@@ -386,8 +386,8 @@ __device__ __forceinline__ void scene_ray(const TlasNode *__restrict__ tl, uint3
     s.sbits = (uint32_t)s.sign[0] | ((uint32_t)s.sign[1] << 1) | ((uint32_t)s.sign[2] << 2);
     float t, u, v;
     uint32_t prim;
-    tri_traverse<false, kBlock>(e->nodes, e->prims, e->max_nodes, e->nprims, s, ds, e->kEPS,
-                                SRF_NO_PRIM, FLT_MAX, bst, t, u, v, prim);
+    tri_traverse<false>(e->nodes, e->prims, e->max_nodes, e->nprims, s, ds, e->kEPS,
+                        SRF_NO_PRIM, FLT_MAX, Stack32<kBlock>{bst}, t, u, v, prim);
     if (prim == SRF_NO_PRIM) continue;
     // world distance of the local hit point (real3 arithmetic in float)
     double lp[3], wp[3];

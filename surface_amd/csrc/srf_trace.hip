@@ -35,13 +35,16 @@
 namespace srf {
 namespace {
 
-template <bool kAny>
+// kSmall: node indices below 2^24, the 3-byte stack
+template <bool kAny, bool kSmall>
 __global__ void __launch_bounds__(kTraceBlock)
 k_trace_tri(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             const BuildParams *__restrict__ params, const srf_ray *__restrict__ rays, uint32_t n,
             srf_hit *__restrict__ hits, const float *__restrict__ tmax_in,
             uint32_t *__restrict__ occluded, uint32_t max_nodes, uint32_t nprims) {
-  __shared__ uint32_t stack[kTraceStack * kTraceBlock];
+  __shared__ uint32_t stack[kSmall ? 1 : kTraceStack * kTraceBlock];
+  __shared__ uint16_t stack_lo[kSmall ? kTraceStack * kTraceBlock : 1];
+  __shared__ uint8_t stack_hi[kSmall ? kTraceStack * kTraceBlock : 1];
   const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -51,9 +54,14 @@ k_trace_tri(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   const float tlimit = kAny ? (tmax_in ? tmax_in[i] : FLT_MAX) : FLT_MAX;
   float t, u, v;
   uint32_t prim;
-  const bool occ = tri_traverse<kAny, kTraceBlock>(nodes, prims, max_nodes, nprims, s, ds, kEPS,
-                                                   prev, tlimit, stack + threadIdx.x, t, u, v,
-                                                   prim);
+  bool occ;
+  if (kSmall)
+    occ = tri_traverse<kAny>(nodes, prims, max_nodes, nprims, s, ds, kEPS, prev, tlimit,
+                             Stack24<kTraceBlock>{stack_lo + threadIdx.x, stack_hi + threadIdx.x},
+                             t, u, v, prim);
+  else
+    occ = tri_traverse<kAny>(nodes, prims, max_nodes, nprims, s, ds, kEPS, prev, tlimit,
+                             Stack32<kTraceBlock>{stack + threadIdx.x}, t, u, v, prim);
   if (kAny) {
     occluded[i] = occ ? 1u : 0u;
   } else {
@@ -887,9 +895,9 @@ k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims
         const float kEPS = FLT_EPSILON * (float)params->eps_scale;
         float t, u, v;
         uint32_t prim;
-        occ = tri_traverse<true, kTraceBlock>(nodes, prims, max_nodes, nprims, s,
-                                              (flags & SRF_RAY_DOUBLE_SIDED) != 0, kEPS, prev,
-                                              tlimit, st, t, u, v, prim);
+        occ = tri_traverse<true>(nodes, prims, max_nodes, nprims, s,
+                                 (flags & SRF_RAY_DOUBLE_SIDED) != 0, kEPS, prev,
+                                 tlimit, Stack32<kTraceBlock>{st}, t, u, v, prim);
       } else {
         const float zero[3] = {0.0f, 0.0f, 0.0f};
         float t;
@@ -1023,6 +1031,8 @@ inline uint32_t trace_blocks(uint64_t n) {
   return (uint32_t)((b + kXcdPad - 1) / kXcdPad * kXcdPad);
 }
 inline uint32_t max_nodes(const Accel *a) { return (uint32_t)(2 * a->n - 1); }
+// every node index fits the 3-byte stack entries
+inline bool small_nodes(const Accel *a) { return 2 * a->n - 1 <= (1u << 24); }
 inline SphSub sph_sub(const Accel *a) {
   SphSub s{};
   if (a->kind == kSph && a->nbig) {
@@ -1040,9 +1050,14 @@ inline SphSub sph_sub(const Accel *a) {
 srf_status trace_tri(const Accel *a, const srf_ray *rays, uint64_t n, srf_hit *hits,
                      hipStream_t s) {
   if (n == 0) return SRF_OK;
-  hipLaunchKernelGGL(k_trace_tri<false>, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
-                     (const float4 *)a->nodes, (const float4 *)a->prims, a->params, rays,
-                     (uint32_t)n, hits, nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
+  if (small_nodes(a))
+    hipLaunchKernelGGL((k_trace_tri<false, true>), dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
+                       (const float4 *)a->nodes, (const float4 *)a->prims, a->params, rays,
+                       (uint32_t)n, hits, nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
+  else
+    hipLaunchKernelGGL((k_trace_tri<false, false>), dim3(trace_blocks(n)), dim3(kTraceBlock), 0,
+                       s, (const float4 *)a->nodes, (const float4 *)a->prims, a->params, rays,
+                       (uint32_t)n, hits, nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
   SRF_HIP(hipGetLastError());
   return SRF_OK;
 }
@@ -1073,8 +1088,12 @@ srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax, uin
                      uint32_t *occluded, hipStream_t s) {
   if (n == 0) return SRF_OK;
   const dim3 g(trace_blocks(n)), b(kTraceBlock);
-  if (a->kind == kTri)
-    hipLaunchKernelGGL(k_trace_tri<true>, g, b, 0, s, (const float4 *)a->nodes,
+  if (a->kind == kTri && small_nodes(a))
+    hipLaunchKernelGGL((k_trace_tri<true, true>), g, b, 0, s, (const float4 *)a->nodes,
+                       (const float4 *)a->prims, a->params, rays, (uint32_t)n, nullptr, tmax,
+                       occluded, max_nodes(a), (uint32_t)a->n);
+  else if (a->kind == kTri)
+    hipLaunchKernelGGL((k_trace_tri<true, false>), g, b, 0, s, (const float4 *)a->nodes,
                        (const float4 *)a->prims, a->params, rays, (uint32_t)n, nullptr, tmax,
                        occluded, max_nodes(a), (uint32_t)a->n);
   else if (a->kind == kTet)
