@@ -23,8 +23,8 @@
 //   R_{k-1} = a_k c_k + (1 - a_k) R_k,  P_{k-1} = (1 - a_k) P_k,
 //   dL/dc_k = g_C T_k a_k,  dL/da_k = T_k (g_C . (c_k - R_k) + g_A P_k).
 // TF gradients collect in registers while a ray stays in one TF bin, then
-// in LDS per block; voxel gradients go out as float atomics, merged across
-// consecutive samples that share a cell.
+// in LDS per block (four copies, spread over the lanes); voxel gradients go
+// out as float atomics, merged across consecutive samples that share a cell.
 #include <float.h>
 #include <math.h>
 
@@ -48,6 +48,14 @@ constexpr int kFwdLdsEntries = 1024;  // forward LDS reserved per block (float4 
 constexpr int kSegLen = 16;     // samples per checkpointed segment
 constexpr int kMaxSegNoCkpt = 8;  // LDS checkpoints per ray when recomputing
 constexpr int kMaxSamples = 65536;
+// The adjoint's TF-gradient table is kept in this many copies, lane l adding
+// to copy l % kTfCopies: neighbouring rays sit in the same TF bin at the same
+// step, and LDS atomics to one address from several lanes of an instruction
+// serialise (measured on the 256^3 bench: 1, 2, 4, 8 copies -> adjoint 5.9,
+// 5.9, 5.5, 6.3 ms; 8 no longer fit the LDS beside 12^3 voxel windows). Copy
+// strides are 4N + 1 floats, so equal entries of two copies fall in
+// different banks.
+constexpr int kTfCopies = 4;
 
 __device__ __forceinline__ int faster_floorf(float x) {
   // render/render_texture.cc:108-121
@@ -240,6 +248,9 @@ __device__ __forceinline__ float sample_alpha(const Volume &V, const float4 *s_t
   tf_index(s, N, i0, f, inside);
   return lerpf(f, s_tf[i0].w, s_tf[i0 + 1].w);
 }
+
+// floats of the adjoint's TF-gradient copies, rounded up to a float4
+__host__ __device__ inline int tf_grad_floats(int N) { return (kTfCopies * (4 * N + 1) + 3) & ~3; }
 
 struct TfAcc {
   int bin;
@@ -454,16 +465,17 @@ k_vol_bwd(Volume V, const float *__restrict__ tf, int N, float step,
   extern __shared__ float4 smem4[];
   float4 *s_tf = smem4;
   float *s_gtf = reinterpret_cast<float *>(smem4 + N);
-  float *s_win = s_gtf + 4 * N;
+  const int gstride = 4 * N + 1;
+  float *s_win = s_gtf + tf_grad_floats(N);
   float *s_ck = s_win + (kVolBwdBlock / 64) * W * W * W;
-  for (int k = threadIdx.x; k < N; k += kVolBwdBlock) {
+  for (int k = threadIdx.x; k < N; k += kVolBwdBlock)
     s_tf[k] = reinterpret_cast<const float4 *>(tf)[k];
-    s_gtf[4 * k] = s_gtf[4 * k + 1] = s_gtf[4 * k + 2] = s_gtf[4 * k + 3] = 0.0f;
-  }
+  for (int k = threadIdx.x; k < kTfCopies * gstride; k += kVolBwdBlock) s_gtf[k] = 0.0f;
   for (int k = threadIdx.x; k < (kVolBwdBlock / 64) * W * W * W; k += kVolBwdBlock)
     s_win[k] = 0.0f;
   __syncthreads();
   const int lane = threadIdx.x & 63;
+  float *my_gtf = s_gtf + (lane % kTfCopies) * gstride;
   VoxWindow<W> win;
   win.cell = s_win + (threadIdx.x >> 6) * W * W * W;
   win.set = false;
@@ -551,7 +563,7 @@ k_vol_bwd(Volume V, const float *__restrict__ tf, int N, float step,
       const float ga =
           Tk * ((gr.x * (c0r - R0) + gr.y * (c1g - R1) + gr.z * (c2b - R2)) + gr.w * P);
       const float fo = 1.0f - f;
-      tf_move(tacc, i0, s_gtf);
+      tf_move(tacc, i0, my_gtf);
       tacc.g[0] += fo * g0;
       tacc.g[1] += fo * g1;
       tacc.g[2] += fo * g2;
@@ -612,10 +624,11 @@ k_vol_bwd(Volume V, const float *__restrict__ tf, int N, float step,
   }
   cell_flush(cacc, win, V, grad_vox);
   window_flush<W>(win, V, grad_vox, lane);
-  flush_tf(tacc, s_gtf);
+  flush_tf(tacc, my_gtf);
   __syncthreads();
   for (int k = threadIdx.x; k < 4 * N; k += kVolBwdBlock) {
-    float v = s_gtf[k];
+    float v = 0.0f;
+    for (int r = 0; r < kTfCopies; r++) v += s_gtf[r * gstride + k];
     if (v != 0.0f) unsafeAtomicAdd(grad_tf + k, v);
   }
 }
@@ -729,7 +742,7 @@ srf_status srf_volume_march_bwd(const srf_volume *vol, const float *tf, int tf_s
   if (n == 0) return SRF_OK;
   const uint32_t slots = checkpoint_slots(step);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  const size_t lds_tf = 32 * (size_t)tf_size;
+  const size_t lds_tf = 16 * (size_t)tf_size + 4 * (size_t)tf_grad_floats(tf_size);
   const size_t lds_ck = ckpt ? 0 : 4 * (size_t)(kMaxSegNoCkpt + 1) * kVolBwdBlock;
   const size_t waves = kVolBwdBlock / 64;
   const size_t lds12 = lds_tf + lds_ck + 4 * waves * 12 * 12 * 12;
