@@ -231,6 +231,13 @@ srf_status srf_accel_get_info(const srf_accel *accel, srf_accel_info *info_host)
  * Synchronises. */
 srf_status srf_accel_export(const srf_accel *accel, void *nodes_host,
                             uint32_t *indices_host);
+/* {Triangle,Tetra,Particle}Accel::Dump (render/render_accel_triangle.cc:
+ * 1692-1719, render/render_accel_tetra.cc:1900-1928,
+ * render/render_accel_particle.cc:1868-1896): unsigned long long node count,
+ * the nodes in the reference layout (srf_accel_export), unsigned long long
+ * index count, the 32-bit indices. A leaf's unused fields (axis, second box
+ * slot), which the reference leaves uninitialised, are zero. Synchronises. */
+srf_status srf_accel_dump(const srf_accel *accel, const char *path);
 srf_status srf_accel_destroy(srf_accel *accel);
 
 /* ------------------------------------------------------------------ trace */

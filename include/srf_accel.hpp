@@ -75,6 +75,10 @@ class AccelBase {
     return srf_accel_export(accel_, nodes_host, indices_host) == SRF_OK;
   }
 
+  /// TriangleAccel::Dump / TetraAccel::Dump / ParticleAccel::Dump: the tree
+  /// and indices in the reference's file format.
+  bool Dump(const char *filename) { return remember(srf_accel_dump(accel_, filename)); }
+
   srf_accel *handle() const { return accel_; }
   const std::string &LastError() const { return error_; }
 

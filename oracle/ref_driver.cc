@@ -17,6 +17,11 @@
 //   scene         RenderElement (Update: world box, inverse matrices),
 //                 RenderGraph::Build / Trace    gles/gles_render_graph.cc:19-314,
 //                 BVHTree::BuildTree            render/render_bvh_tree.cc:319-502
+//   eye rays      Camera::buildFrame            render/render_camera.cc:17-63, then
+//                 the per-pixel ray expression of RaytraceEngine::OnData and its
+//                 ViewportConversion (gles/gles_raytrace_engine.cc:530-544,
+//                 730-812), written out here in the reference's real3 types
+//                 because that file needs the GL context to compile
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -28,6 +33,7 @@
 #endif
 
 #include "render_accel_particle.h"
+#include "render_camera.h"
 #include "render_accel_tetra.h"
 #include "render_accel_triangle.h"
 #include "render_prefix_tree_util.h"
@@ -392,6 +398,38 @@ void ref_scene_trace(void *h, const void *rays_in, long long n, long long width,
       }
     }
   }
+}
+
+void ref_eye_rays(const float *eye, const float *lookat, const float *up, float fov, int width,
+                  int height, float *org_out, float *dir_out, unsigned char *valid) {
+  using namespace lsgl::render;
+  Camera cam(real3(eye[0], eye[1], eye[2]), real3(lookat[0], lookat[1], lookat[2]),
+             real3(up[0], up[1], up[2]), fov);
+  real3 corner, udir, vdir;
+  cam.buildFrame(corner, udir, vdir, width, height);
+  const float invWidth = 1.0f / width;
+  const float invHeight = 1.0f / height;
+  const int step = 1;
+  for (int y = 0; y < height; y++)
+    for (int x = 0; x < width; x++) {
+      const size_t i = (size_t)y * width + x;
+      float x01 = (x + 0.5f) * invWidth;
+      float y01 = (y + 0.5f) * invHeight;
+      float xnd = 2.0f * (x01 - 0.5f);
+      float ynd = 2.0f * (y01 - 0.5f);
+      int px = (xnd + 1.0f) * (0.5f * width) - 0;
+      int py = (ynd + 1.0f) * (0.5f * height) + 0;
+      valid[i] = !((px < 0) || (py < 0) || (px >= width) || (py >= height));
+      float u = (0 + 0.0f) / (float)1, v = (0 + 0.0f) / (float)1;
+      real3 org = cam.getEye();
+      real3 dir = (corner + ((float)px + u + step / 2.0f) * udir +
+                   ((float)py + v - step / 2.0f + 1.0f) * vdir) - org;
+      dir.normalize();
+      for (int k = 0; k < 3; k++) {
+        org_out[3 * i + k] = valid[i] ? org[k] : 0.0f;
+        dir_out[3 * i + k] = valid[i] ? dir[k] : 0.0f;
+      }
+    }
 }
 
 }  // extern "C"

@@ -92,6 +92,7 @@ SIGNATURES = {
     "srf_accel_bounds": (_i, [_vp, _vp, _vp]),
     "srf_accel_get_info": (_i, [_vp, C.POINTER(AccelInfo)]),
     "srf_accel_export": (_i, [_vp, _vp, _vp]),
+    "srf_accel_dump": (_i, [_vp, C.c_char_p]),
     "srf_accel_destroy": (_i, [_vp]),
     "srf_trace_closest": (_i, [_vp, _vp, _u64, _vp, _vp]),
     "srf_trace_tetra": (_i, [_vp, _vp, _u64, _vp, _vp, _vp, _vp]),

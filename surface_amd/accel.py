@@ -11,6 +11,7 @@ Failures raise SrfError where the reference returns false or asserts.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -149,6 +150,11 @@ class _Accel:
         check(self.lib.srf_accel_export(self.handle, nodes.ctypes.data_as(C.c_void_p),
                                         idx.ctypes.data_as(C.c_void_p)))
         return nodes, idx
+
+    def Dump(self, filename):
+        """The tree and indices in the reference's Dump file format
+        (render/render_accel_triangle.cc:1692-1719)."""
+        check(self.lib.srf_accel_dump(self.handle, os.fsencode(filename)))
 
     def Traverse(self, rays, hits=None, stream=None):
         """Closest hit for every ray. rays: [n, 8] float32 GPU tensor (srf_ray);

@@ -56,23 +56,27 @@ def build(verbose=False, extra=()):
 
 
 def build_cpp_test(verbose=False):
-    """The C++ host test of include/srf_accel.hpp (tests/cpp), linked against
-    libsrf.so and the oracle; run by tests/test_gpu_cpp.py on a GPU."""
+    """The C++ host tests of include/srf_accel.hpp and include/srf_frame.hpp
+    (tests/cpp), linked against libsrf.so and the oracle; run by
+    tests/test_gpu_cpp.py on a GPU."""
     root = os.path.dirname(HERE)
-    src = os.path.join(root, "tests", "cpp", "test_accel_cpp.cpp")
-    out = os.path.join(root, "tests", "cpp", "test_accel_cpp")
     oracle = os.path.join(root, "oracle")
-    cmd = [hipcc(), "--offload-arch=gfx950", "-O2", "-std=c++17", "-o", out, src,
-           "-L" + HERE, "-lsrf", "-L" + oracle, "-loracle",
-           "-Wl,-rpath," + HERE, "-Wl,-rpath," + oracle, "-Wl,-rpath,$ORIGIN/../../surface_amd",
-           "-Wl,-rpath,$ORIGIN/../../oracle"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building tests/cpp/test_accel_cpp")
-    return out
+    outs = []
+    for name, extra in (("test_accel_cpp", []), ("test_frame_cpp", ["-fopenmp", "-pthread"])):
+        src = os.path.join(root, "tests", "cpp", name + ".cpp")
+        out = os.path.join(root, "tests", "cpp", name)
+        cmd = [hipcc(), "--offload-arch=gfx950", "-O2", "-std=c++17", *extra, "-o", out, src,
+               "-L" + HERE, "-lsrf", "-L" + oracle, "-loracle",
+               "-Wl,-rpath," + HERE, "-Wl,-rpath," + oracle,
+               "-Wl,-rpath,$ORIGIN/../../surface_amd", "-Wl,-rpath,$ORIGIN/../../oracle"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc failed building tests/cpp/{name}")
+        outs.append(out)
+    return outs
 
 
 if __name__ == "__main__":

@@ -489,6 +489,29 @@ srf_status srf_accel_export(const srf_accel *accel, void *nodes_host, uint32_t *
   return SRF_OK;
 }
 
+srf_status srf_accel_dump(const srf_accel *accel, const char *path) {
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!a || !path) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  srf_accel_info info;
+  srf_status st = srf_accel_get_info(accel, &info);
+  if (st != SRF_OK) return st;
+  std::vector<uint8_t> nodes(info.num_nodes * info.node_bytes);
+  std::vector<uint32_t> idx(a->n);
+  st = srf_accel_export(accel, nodes.data(), idx.data());
+  if (st != SRF_OK) return st;
+  FILE *fp = fopen(path, "wb");
+  if (!fp) return set_error(SRF_ERR_INVALID_ARG, "cannot write %s", path);
+  // unsigned long long count, node records, count, unsigned int indices
+  const unsigned long long nn = info.num_nodes, ni = a->n;
+  bool ok = fwrite(&nn, sizeof(nn), 1, fp) == 1;
+  ok = ok && fwrite(nodes.data(), 1, nodes.size(), fp) == nodes.size();
+  ok = ok && fwrite(&ni, sizeof(ni), 1, fp) == 1;
+  ok = ok && fwrite(idx.data(), 4, idx.size(), fp) == idx.size();
+  ok = (fclose(fp) == 0) && ok;
+  if (!ok) return set_error(SRF_ERR_INVALID_ARG, "short write to %s", path);
+  return SRF_OK;
+}
+
 srf_status srf_accel_destroy(srf_accel *accel) {
   free_accel(reinterpret_cast<Accel *>(accel));
   return SRF_OK;

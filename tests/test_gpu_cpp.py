@@ -16,3 +16,14 @@ def test_cpp_host_classes_bit_exact():
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stderr
     assert "PASS" in r.stdout
+
+
+FRAME = os.path.join(os.path.dirname(__file__), "cpp", "test_frame_cpp")
+
+
+def test_cpp_frame_loop_and_ray_queues():
+    assert os.path.exists(FRAME), "tests/cpp/test_frame_cpp not built (run __graft_entry__.build())"
+    r = subprocess.run([FRAME], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    assert "PASS" in r.stdout
