@@ -44,7 +44,7 @@ TET_BYTES_PER_RAY = 40 + 40 * TET_NODES_PER_RAY + 80 * TET_TESTS_PER_RAY
 # HBM bytes per k_trace_tet launch on the bench rays: rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (KiB/dispatch), each scaled by the same counter on a 1 GiB copy
 # (fetch x 2.000, write x 1.000); profiles/traffic_k_trace_tet_tet64_k_trace_tri_heightfield512.md
-TET_TRACE_HBM_BYTES = (2.000 * 68078.6 + 1.000 * 49152.0) * 1024
+TET_TRACE_HBM_BYTES = (2.000 * 64221.8 + 1.000 * 49152.0) * 1024
 TRI_BYTES_PER_RAY = 40 + 64 * TRI_NODES_PER_RAY + 48 * TRI_TESTS_PER_RAY
 
 
