@@ -337,6 +337,21 @@ __device__ __forceinline__ uint32_t prim_code(const PrimSrc &s, uint32_t i,
   }
 }
 
+// Counts digit d of every valid lane into the LDS histogram row h: the lanes
+// that share the first valid lane's digit add with one atomic, the others
+// each with their own. Neighbouring primitives (and rays in sorted order)
+// share their high digits, and atomics from many lanes to one LDS address
+// serialise.
+__device__ __forceinline__ void hist_add(uint32_t *h, uint32_t d, bool valid) {
+  const uint64_t act = __ballot(valid);
+  if (act == 0) return;
+  const int leader = __ffsll((long long)act) - 1;
+  const uint32_t v = __shfl(d, leader, 64);
+  const uint64_t same = __ballot(valid && d == v);
+  if ((int)(threadIdx.x & 63u) == leader) atomicAdd(&h[v], (uint32_t)__popcll(same));
+  if (valid && d != v) atomicAdd(&h[d], 1u);
+}
+
 template <int K, bool M>
 __global__ void __launch_bounds__(256) k_morton(SegView v, uint32_t *codes, uint32_t *hist) {
   // blocks [mblk_off[m], mblk_off[m+1]) code mesh m; histograms per mesh
@@ -351,14 +366,20 @@ __global__ void __launch_bounds__(256) k_morton(SegView v, uint32_t *codes, uint
   for (int i = threadIdx.x; i < 4 * 256; i += blockDim.x) (&h[0][0])[i] = 0;
   __syncthreads();
   const BuildParams p = *sg.params;
-  for (uint32_t i = (blockIdx.x - b0) * blockDim.x + threadIdx.x; i < s.n;
-       i += nblk * blockDim.x) {
-    uint32_t c = prim_code<K>(s, i, p);
-    codes[i] = c;
-    atomicAdd(&h[0][c & 255], 1u);
-    atomicAdd(&h[1][(c >> 8) & 255], 1u);
-    atomicAdd(&h[2][(c >> 16) & 255], 1u);
-    atomicAdd(&h[3][c >> 24], 1u);
+  // whole wavefronts step together (hist_add takes ballots)
+  for (uint32_t i0 = (blockIdx.x - b0) * blockDim.x + (threadIdx.x & ~63u); i0 < s.n;
+       i0 += nblk * blockDim.x) {
+    const uint32_t i = i0 + (threadIdx.x & 63u);
+    const bool valid = i < s.n;
+    uint32_t c = 0;
+    if (valid) {
+      c = prim_code<K>(s, i, p);
+      codes[i] = c;
+    }
+    hist_add(h[0], c & 255, valid);
+    hist_add(h[1], (c >> 8) & 255, valid);
+    hist_add(h[2], (c >> 16) & 255, valid);
+    hist_add(h[3], c >> 24, valid);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 4 * 256; i += blockDim.x) {
@@ -380,13 +401,15 @@ __global__ void __launch_bounds__(256) k_histogram(const uint32_t *codes, uint32
   __shared__ uint32_t h[4][256];
   for (int i = threadIdx.x; i < 4 * 256; i += blockDim.x) (&h[0][0])[i] = 0;
   __syncthreads();
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x) {
-    uint32_t c = codes[i];
-    atomicAdd(&h[0][c & 255], 1u);
-    atomicAdd(&h[1][(c >> 8) & 255], 1u);
-    atomicAdd(&h[2][(c >> 16) & 255], 1u);
-    atomicAdd(&h[3][c >> 24], 1u);
+  for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n;
+       i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + (threadIdx.x & 63u);
+    const bool valid = i < n;
+    const uint32_t c = valid ? codes[i] : 0u;
+    hist_add(h[0], c & 255, valid);
+    hist_add(h[1], (c >> 8) & 255, valid);
+    hist_add(h[2], (c >> 16) & 255, valid);
+    hist_add(h[3], c >> 24, valid);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 4 * 256; i += blockDim.x) {
