@@ -52,7 +52,12 @@ typedef enum srf_status {
 typedef enum srf_prim_kind {
   SRF_PRIM_POINTS = 0x0000,       /* GL_POINTS -> ParticleAccel  */
   SRF_PRIM_TRIANGLES = 0x0004,    /* GL_TRIANGLES -> TriangleAccel */
-  SRF_PRIM_TETRAHEDRONS = 0x0010  /* GL_TETRAHEDRONS_EXT -> TetraAccel */
+  SRF_PRIM_TETRAHEDRONS = 0x0010, /* GL_TETRAHEDRONS_EXT -> TetraAccel */
+  /* GL_PYRAMIDS_EXT / GL_PRISMS_EXT / GL_HEXAHEDRONS_EXT -> SolidAccel
+   * (include/GLES2/gl2.h:64-66, gles/gles_accel_builder.cc:880-968) */
+  SRF_PRIM_PYRAMIDS = 0x0011,
+  SRF_PRIM_PRISMS = 0x0012,
+  SRF_PRIM_HEXAHEDRONS = 0x0013
 } srf_prim_kind;
 
 typedef struct srf_ctx srf_ctx;
@@ -92,6 +97,22 @@ typedef struct srf_tet_hit_ext {
   float pad;
 } srf_tet_hit_ext;
 
+/* Solid extras at the reported crossing (SolidAccel::TestLeafNode,
+ * render/render_accel_solid.cc:2206-2333): interpolate() weights d0..d7
+ * of the solid's vertices (:247-435, mode 0; entries past the vertex count
+ * are 0), Intersection::position and ::normal of the crossing (the face
+ * point the Pluecker weights give, not org + t dir; an exit crossing's
+ * normal is negated), the varying sum over k of scalar[nv * prim + k] * d_k
+ * (gles/gles_common.cc:1078-1140) when scalars are given, and side = 0 for
+ * the entry crossing, 1 for the exit (a ray starting inside). 64 bytes. */
+typedef struct srf_solid_hit_ext {
+  float d[8];
+  float position[3];
+  float scalar;
+  float normal[3];
+  uint32_t side;
+} srf_solid_hit_ext;
+
 /* Sphere self-intersection inputs, Ray::prev_hit_t / prev_hit_normal
  * (render/render_ray.h:137-139), used when prev_prim_id names a sphere. */
 typedef struct srf_sphere_prev {
@@ -117,6 +138,17 @@ typedef struct srf_tet_desc {
   const uint32_t *indices; /* 4 per tetrahedron */
   uint64_t num_tetras;
 } srf_tet_desc;
+
+/* Solid (render/render_prim_solid.h:33-40), FP32 positions: vertices_per_solid
+ * = 5 pyramids, 6 prisms, 8 hexahedra, with the vertex order of
+ * kPyramidFaces / kPrismFaces / kHexaFaces (render/render_accel_solid.cc:96-99). */
+typedef struct srf_solid_desc {
+  const float *vertices;
+  uint64_t num_vertices;
+  const uint32_t *indices; /* vertices_per_solid per solid */
+  uint64_t num_solids;
+  uint32_t vertices_per_solid;
+} srf_solid_desc;
 
 typedef struct srf_sphere_desc {
   const float *positions;
@@ -188,6 +220,12 @@ srf_status srf_accel_build_triangles(srf_ctx *ctx, const srf_tri_desc *desc,
  * from AccelBuilder::AddTetraData (gles/gles_accel_builder.cc:867). */
 srf_status srf_accel_build_tetras(srf_ctx *ctx, const srf_tet_desc *desc,
                                   void *stream, srf_accel **out);
+/* SolidAccel::Build32 (render/render_accel_solid.h:91, .cc:1795-2124). The
+ * reference's AccelBuilder::AddSolidData calls the SAH Build instead
+ * (gles/gles_accel_builder.cc:967-968, Build32 left commented out); this is
+ * the LBVH Build32 path, whose tree srf_accel_export reproduces. */
+srf_status srf_accel_build_solids(srf_ctx *ctx, const srf_solid_desc *desc,
+                                  void *stream, srf_accel **out);
 /* ParticleAccel::Build32 (render/render_accel_particle.h:77,
  * .cc:1560-1866), from AccelBuilder::AddParticleData
  * (gles/gles_accel_builder.cc:673). */
@@ -252,6 +290,14 @@ srf_status srf_trace_closest(const srf_accel *accel, const srf_ray *rays,
 srf_status srf_trace_tetra(const srf_accel *accel, const srf_ray *rays,
                            uint64_t num_rays, const float *vertex_scalars,
                            srf_hit *hits, srf_tet_hit_ext *ext, void *stream);
+/* SolidAccel::Traverse (render/render_accel_solid.cc:2345-2435) with its
+ * BuildIntersection; index_scalars (one float per index position, i.e.
+ * num_solids * vertices_per_solid, may be NULL) gives ext->scalar, ext may
+ * be NULL. The reference never writes a solid hit's u and v (they come
+ * from an uninitialised Intersection); they are 0 here. */
+srf_status srf_trace_solids(const srf_accel *accel, const srf_ray *rays,
+                            uint64_t num_rays, const float *index_scalars,
+                            srf_hit *hits, srf_solid_hit_ext *ext, void *stream);
 /* ParticleAccel::Traverse with the self-intersection inputs of each ray
  * (render/render_accel_particle.cc:2423-2512); prev may be NULL. */
 srf_status srf_trace_spheres(const srf_accel *accel, const srf_ray *rays,
@@ -307,8 +353,9 @@ srf_status srf_ao_rays_spheres(const srf_ray *primary, const srf_hit *hits, uint
                                uint32_t *out_pixel, uint32_t *out_count, void *stream);
 
 /* CalculateMortonCodes{TriangleFloat,TetraFloat}30 / CalculateMortonCodes30SIMD
- * (render/render_prefix_tree_util.cc:176-304) for the scene box bmin/bmax
- * (host arrays). desc points to the srf_*_desc matching kind. */
+ * (render/render_prefix_tree_util.cc:176-304) and CalculateMortonCodesSolidFloat30
+ * (:347-382, the three solid kinds) for the scene box bmin/bmax (host
+ * arrays). desc points to the srf_*_desc matching kind. */
 srf_status srf_lbvh_morton30(srf_ctx *ctx, srf_prim_kind kind, const void *desc,
                              const float bmin_host[3], const float bmax_host[3],
                              uint32_t *codes, void *stream);

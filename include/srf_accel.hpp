@@ -6,6 +6,7 @@
 //   lsgl::render::TriangleAccel   render/render_accel_triangle.h:69-105
 //   lsgl::render::TetraAccel      render/render_accel_tetra.h:82-125
 //   lsgl::render::ParticleAccel   render/render_accel_particle.h:68-110
+//   lsgl::render::SolidAccel      render/render_accel_solid.h:82-125
 //
 // Build32 returns false on failure (the reference's contract) and leaves the
 // reason in LastError(). Traverse takes a batch of device-resident rays and
@@ -124,6 +125,24 @@ class TetraAccel : public detail::AccelBase {
   bool TraverseTetra(const srf_ray *rays, uint64_t n, const float *vertex_scalars, srf_hit *hits,
                      srf_tet_hit_ext *ext, void *stream = nullptr) {
     return remember(srf_trace_tetra(accel_, rays, n, vertex_scalars, hits, ext, stream));
+  }
+};
+
+/// SolidAccel: Build32(const Solid*) with device arrays; vertices_per_solid
+/// is Solid::numVertsPerSolid (5 pyramids, 6 prisms, 8 hexahedra).
+class SolidAccel : public detail::AccelBase {
+ public:
+  using AccelBase::AccelBase;
+  bool Build32(const float *vertices, uint64_t num_vertices, const uint32_t *indices,
+               uint64_t num_solids, uint32_t vertices_per_solid, void *stream = nullptr) {
+    srf_solid_desc d{vertices, num_vertices, indices, num_solids, vertices_per_solid};
+    return build(d, srf_accel_build_solids, stream);
+  }
+  /// Traverse + the crossing's position, normal, interpolate() weights and
+  /// the varying sum of index_scalars (one per index position).
+  bool TraverseSolid(const srf_ray *rays, uint64_t n, const float *index_scalars, srf_hit *hits,
+                     srf_solid_hit_ext *ext, void *stream = nullptr) {
+    return remember(srf_trace_solids(accel_, rays, n, index_scalars, hits, ext, stream));
   }
 };
 

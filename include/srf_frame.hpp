@@ -267,8 +267,8 @@ class AccelCache {
 // Shader-issued rays, queued per shading thread and traced in batches.
 class RayQueue {
  public:
-  // kind: SRF_QUEUE_CLOSEST (trace(): t, u, v, prim) or SRF_QUEUE_ANY
-  // (occlusion against t_max).
+  // kind: kClosest (trace(): t, u, v, prim) or kAny (occlusion against
+  // t_max).
   enum Kind { kClosest = 0, kAny = 1 };
 
   RayQueue(const srf_accel *accel, int threads, size_t batch, Kind kind = kClosest)

@@ -57,12 +57,15 @@ class Oracle:
                      "or_radix_tree30", "or_trace_tri", "or_trace_tet",
                      "or_trace_sph", "or_tex3d_sample", "or_volume_fwd_f32",
                      "or_volume_fwd_f64", "or_volume_bwd_f64",
-                     "or_scene_bounds_sph", "or_transform_box", "or_scene_trace"):
+                     "or_scene_bounds_sph", "or_transform_box", "or_scene_trace",
+                     "or_scene_bounds_sol", "or_morton_sol", "or_trace_sol",
+                     "or_interpolate_solid"):
             getattr(L, name).restype = None
         L.or_tlas_build.restype = _i64
-        for name in ("or_build_tri", "or_build_tet", "or_build_sph"):
+        for name in ("or_build_tri", "or_build_tet", "or_build_sph", "or_build_sol"):
             getattr(L, name).restype = _i64
         L.or_morton30_point.restype = C.c_uint32
+        L.or_solid_isect.restype = C.c_int
 
     # ---- stages ----
     def scene_bounds(self, kind, V, F=None, R=None, rconst=0.0):
@@ -138,6 +141,49 @@ class Oracle:
         m = self.lib.or_build_tet(_ptr(V), _ptr(F), _i64(n), _ptr(nodes), _ptr(idx), _ptr(bounds))
         return dict(nodes=nodes[:m].copy(), indices=idx, bounds=bounds)
 
+    def build_sol(self, V, F, nv):
+        """SolidAccel::Build32 over solids of nv vertices (F: nv per solid)."""
+        V = _c(V, np.float32)
+        F = _c(F, np.uint32).reshape(-1, nv)
+        n = len(F)
+        nodes = np.zeros(2 * n, TET_NODE)
+        idx = np.zeros(n, np.uint32)
+        bounds = np.zeros(6, np.float64)
+        m = self.lib.or_build_sol(_ptr(V), _ptr(F), _i64(n), C.c_int(nv), _ptr(nodes), _ptr(idx),
+                                  _ptr(bounds))
+        return dict(nodes=nodes[:m].copy(), indices=idx, bounds=bounds, nv=nv)
+
+    def scene_bounds_sol(self, V, F, nv):
+        bmin = np.zeros(3, np.float32)
+        bmax = np.zeros(3, np.float32)
+        F = _c(F, np.uint32).reshape(-1, nv)
+        self.lib.or_scene_bounds_sol(_ptr(_c(V, np.float32)), _ptr(F), _i64(len(F)), C.c_int(nv),
+                                     _ptr(bmin), _ptr(bmax))
+        return bmin, bmax
+
+    def morton_sol(self, V, F, nv, bmin, bmax):
+        F = _c(F, np.uint32).reshape(-1, nv)
+        codes = np.zeros(len(F), np.uint32)
+        self.lib.or_morton_sol(_ptr(_c(V, np.float32)), _ptr(F), _i64(len(F)), C.c_int(nv),
+                               _ptr(_c(bmin, np.float32)), _ptr(_c(bmax, np.float32)),
+                               _ptr(codes))
+        return codes
+
+    def solid_isect(self, nv, o, d, verts):
+        """One ray against one solid: (status, t[2], pos[2,3], nrm[2,3])."""
+        t = np.zeros(2, np.float32)
+        pos = np.zeros(6, np.float32)
+        nrm = np.zeros(6, np.float32)
+        st = self.lib.or_solid_isect(C.c_int(nv), _ptr(_c(o, np.float64)), _ptr(_c(d, np.float64)),
+                                     _ptr(_c(verts, np.float64)), _ptr(t), _ptr(pos), _ptr(nrm))
+        return st, t, pos.reshape(2, 3), nrm.reshape(2, 3)
+
+    def interpolate_solid(self, nv, p, verts):
+        d = np.zeros(8, np.float32)
+        self.lib.or_interpolate_solid(C.c_int(nv), _ptr(_c(p, np.float32)),
+                                      _ptr(_c(verts, np.float64)), _ptr(d))
+        return d
+
     def build_sph(self, P, R=None, rconst=0.0):
         P = _c(P, np.float32)
         R = None if R is None else _c(R, np.float32)
@@ -168,6 +214,18 @@ class Oracle:
         self.lib.or_trace_tet(_ptr(tree["nodes"]), _ptr(tree["indices"]), _ptr(_c(V, np.float32)),
                               _ptr(_c(F, np.uint32)), _ptr(sc), _ptr(rays), _i64(len(rays)),
                               _ptr(hits), _ptr(ext), _ptr(st), C.c_int(threads))
+        return (hits, ext, st) if stats else (hits, ext)
+
+    def trace_sol(self, tree, V, F, rays, scalars=None, threads=1, stats=False):
+        from surface_amd.workloads import HIT_DTYPE, SOL_EXT_DTYPE
+        nv = tree["nv"]
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        ext = np.zeros(len(rays), SOL_EXT_DTYPE)
+        st = np.zeros((len(rays), 2), np.uint32) if stats else None
+        sc = None if scalars is None else _c(scalars, np.float32)
+        self.lib.or_trace_sol(_ptr(tree["nodes"]), _ptr(tree["indices"]), _ptr(_c(V, np.float32)),
+                              _ptr(_c(F, np.uint32)), C.c_int(nv), _ptr(sc), _ptr(rays),
+                              _i64(len(rays)), _ptr(hits), _ptr(ext), _ptr(st), C.c_int(threads))
         return (hits, ext, st) if stats else (hits, ext)
 
     def trace_sph(self, tree, P, R, rconst, rays, prev_t=None, prev_n=None, threads=1,
@@ -280,7 +338,7 @@ class Reference:
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} missing: run `make -C oracle ref`")
         L = self.lib = C.CDLL(path)
-        for name in ("ref_build_tri", "ref_build_tet", "ref_build_sph"):
+        for name in ("ref_build_tri", "ref_build_tet", "ref_build_sph", "ref_build_sol"):
             getattr(L, name).restype = _p
         for name in ("ref_morton_tri", "ref_morton_tet", "ref_morton_pts", "ref_sort_keys",
                      "ref_radix_tree30", "ref_bounds", "ref_free", "ref_trace",
@@ -341,12 +399,18 @@ class Reference:
         self.lib.ref_bounds(_p(h), _ptr(bmin), _ptr(bmax))
         return np.concatenate([bmin, bmax])
 
-    def build(self, kind, V, F=None, R=None, rconst=0.0, keep=False):
+    def build(self, kind, V, F=None, R=None, rconst=0.0, keep=False, nv=0):
         """Run Build32; return nodes (reference layout), indices and bounds.
-        With keep=True the handle stays alive for trace() (call free())."""
+        With keep=True the handle stays alive for trace() (call free()).
+        kind "sol" builds solids of nv vertices (SolidAccel::Build32)."""
         V = _c(V, np.float32)
         keepalive = [V]
-        if kind == "tri":
+        if kind == "sol":
+            F = _c(F, np.uint32).reshape(-1, nv)
+            keepalive.append(F)
+            h = self.lib.ref_build_sol(_ptr(V), _ptr(F), _i64(len(F)), C.c_int(nv))
+            dtype = TET_NODE
+        elif kind == "tri":
             F = _c(F, np.uint32)
             keepalive.append(F)
             h = self.lib.ref_build_tri(_ptr(V), _i64(len(V)), _ptr(F), _i64(len(F)))
@@ -362,7 +426,7 @@ class Reference:
             h = self.lib.ref_build_sph(_ptr(V), _ptr(R), C.c_float(rconst), _i64(len(V)))
             dtype = SPH_NODE
         nodes, idx = self._read_dump(h, dtype)
-        out = dict(nodes=nodes, indices=idx, bounds=self._bounds(h))
+        out = dict(nodes=nodes, indices=idx, bounds=self._bounds(h), kind=kind, nv=nv)
         if keep:
             out["handle"] = h
             out["_keep"] = keepalive
@@ -371,9 +435,9 @@ class Reference:
         return out
 
     def trace(self, tree, rays, width=0, scalars=None, threads=0):
-        from surface_amd.workloads import HIT_DTYPE, TET_EXT_DTYPE
+        from surface_amd.workloads import HIT_DTYPE, SOL_EXT_DTYPE, TET_EXT_DTYPE
         hits = np.zeros(len(rays), HIT_DTYPE)
-        ext = np.zeros(len(rays), TET_EXT_DTYPE)
+        ext = np.zeros(len(rays), SOL_EXT_DTYPE if tree.get("kind") == "sol" else TET_EXT_DTYPE)
         sc = None if scalars is None else _c(scalars, np.float32)
         self.lib.ref_trace(_p(tree["handle"]), _ptr(rays), _i64(len(rays)), _i64(width),
                            _ptr(hits), _ptr(ext), _ptr(sc), C.c_int(threads))

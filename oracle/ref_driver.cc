@@ -9,8 +9,8 @@
 //                 CalculateMortonCodes30{,SIMD}   render/render_prefix_tree_util.cc
 //   sort          RadixSort30 / RadixSort30OMP   render/render_prefix_tree_util.h:296-457
 //   Karras        ConstructBinaryRadixTree30     render/render_prefix_tree_util.cc:562-649
-//   build         {Triangle,Tetra,Particle}Accel::Build32 + Dump
-//   trace         {Triangle,Tetra,Particle}Accel::Traverse, OpenMP
+//   build         {Triangle,Tetra,Particle,Solid}Accel::Build32 + Dump
+//   trace         {Triangle,Tetra,Particle,Solid}Accel::Traverse, OpenMP
 //                 schedule(dynamic,1) over image rows as in
 //                 gles/gles_raytrace_engine.cc:662-665
 //   texture       FilterTexture3DFloat           render/render_texture.cc:946-1058
@@ -33,6 +33,7 @@
 #endif
 
 #include "render_accel_particle.h"
+#include "render_accel_solid.h"
 #include "render_camera.h"
 #include "render_accel_tetra.h"
 #include "render_accel_triangle.h"
@@ -80,14 +81,24 @@ struct TetExt {
   float pad;
 };
 
+struct SolExt {
+  float d[8];
+  float pos[3];
+  float scalar;
+  float nrm[3];
+  unsigned int side;
+};
+
 struct RefAccel {
-  int kind;  // 0 tri, 1 tet, 2 sphere
+  int kind;  // 0 tri, 1 tet, 2 sphere, 3 solid
   Mesh mesh;
   Tetrahedron tet;
   Particles part;
+  Solid sol;
   TriangleAccel tri_accel;
   TetraAccel tet_accel;
   ParticleAccel sph_accel;
+  SolidAccel sol_accel;
   ~RefAccel() {
     // Mesh::~Mesh deletes its arrays; they belong to the caller.
     mesh.vertices = 0;
@@ -211,10 +222,27 @@ void *ref_build_sph(const float *P, float *R, float rconst, long long n) {
   return a;
 }
 
+// SolidAccel::Build32 (render/render_accel_solid.cc:1795-2124); nv = 5
+// pyramids, 6 prisms, 8 hexahedra.
+void *ref_build_sol(const float *V, unsigned *F, long long n, int nv) {
+  RefAccel *a = new RefAccel();
+  a->kind = 3;
+  memset(&a->sol, 0, sizeof(a->sol));
+  a->sol.numVertsPerSolid = nv;
+  a->sol.numSolids = n;
+  a->sol.vertices = V;
+  a->sol.isDoublePrecisionPos = false;
+  a->sol.indices = F;
+  QuietStdout q;
+  a->sol_accel.Build32(&a->sol, SolidBuildOptions());
+  return a;
+}
+
 int ref_dump(void *h, const char *path) {
   RefAccel *a = (RefAccel *)h;
   if (a->kind == 0) return a->tri_accel.Dump(path) ? 0 : 1;
   if (a->kind == 1) return a->tet_accel.Dump(path) ? 0 : 1;
+  if (a->kind == 3) return a->sol_accel.Dump(path) ? 0 : 1;
   return a->sph_accel.Dump(path) ? 0 : 1;
 }
 
@@ -224,6 +252,8 @@ void ref_bounds(void *h, double bmin[3], double bmax[3]) {
     a->tri_accel.BoundingBox(bmin, bmax);
   else if (a->kind == 1)
     a->tet_accel.BoundingBox(bmin, bmax);
+  else if (a->kind == 3)
+    a->sol_accel.BoundingBox(bmin, bmax);
   else
     a->sph_accel.BoundingBox(bmin, bmax);
 }
@@ -232,7 +262,12 @@ void ref_free(void *h) { delete (RefAccel *)h; }
 
 // Trace n rays as rows of `width` rays, OpenMP dynamic over rows. For tetra,
 // ext receives InterpolateTetra weights and, given scalars, the varying
-// f0*d0 + f1*d1 + f2*d2 + f3*d3 (gles/gles_common.cc:1074-1076).
+// f0*d0 + f1*d1 + f2*d2 + f3*d3 (gles/gles_common.cc:1074-1076); for solids
+// (SolExt) the interpolate() weights of the solid's vertices (the rest 0),
+// the position and normal, and the varying sum over f0..f(nv-1)
+// (gles/gles_common.cc:1078-1140). A solid's Intersection::u/v are never
+// written by the reference (uninitialised); hits[i].u/v carry them as they
+// are.
 void ref_trace(void *h, const void *rays_in, long long n, long long width,
                void *hits_out, void *ext_out, const float *scalars,
                int threads) {
@@ -259,13 +294,35 @@ void ref_trace(void *h, const void *rays_in, long long n, long long width,
         hit = a->tri_accel.Traverse(isect, ray);
       else if (a->kind == 1)
         hit = a->tet_accel.Traverse(isect, ray);
+      else if (a->kind == 3)
+        hit = a->sol_accel.Traverse(isect, ray);
       else
         hit = a->sph_accel.Traverse(isect, ray);
       hits[i].t = isect.t;
       hits[i].u = isect.u;
       hits[i].v = isect.v;
       hits[i].prim_id = isect.prim_id;
-      if (ext) {
+      if (ext && a->kind == 3) {
+        SolExt *se = (SolExt *)ext_out + i;
+        memset(se, 0, sizeof(SolExt));
+        if (hit) {
+          const int nv = a->sol.numVertsPerSolid;
+          const float dd[8] = {isect.d0, isect.d1, isect.d2, isect.d3,
+                               isect.d4, isect.d5, isect.d6, isect.d7};
+          const unsigned ff[8] = {isect.f0, isect.f1, isect.f2, isect.f3,
+                                  isect.f4, isect.f5, isect.f6, isect.f7};
+          for (int k = 0; k < nv; k++) se->d[k] = dd[k];
+          for (int k = 0; k < 3; k++) {
+            se->pos[k] = isect.position[k];
+            se->nrm[k] = isect.normal[k];
+          }
+          if (scalars) {
+            float acc = scalars[ff[0]] * dd[0];
+            for (int k = 1; k < nv; k++) acc = acc + scalars[ff[k]] * dd[k];
+            se->scalar = acc;
+          }
+        }
+      } else if (ext) {
         memset(&ext[i], 0, sizeof(TetExt));
         if (hit && a->kind == 1) {
           ext[i].d[0] = isect.d0;

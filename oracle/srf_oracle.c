@@ -105,6 +105,25 @@ void or_morton_tet(const float *V, const uint32_t *F, int64_t n,
   }
 }
 
+/* CalculateMortonCodesSolidFloat30, render/render_prefix_tree_util.cc:347-382:
+ * the centroid is the float sum of the solid's vertices from (0, 0, 0), in
+ * index order, times the float reciprocal of the vertex count. */
+void or_morton_sol(const float *V, const uint32_t *F, int64_t n, int nv,
+                   const float bmin[3], const float bmax[3], uint32_t *codes) {
+  float inv[3];
+  morton_inv(bmin, bmax, inv);
+  const float inv_div = 1.0f / (float)nv;
+  for (int64_t i = 0; i < n; i++) {
+    float c[3] = {0.0f, 0.0f, 0.0f};
+    for (int j = 0; j < nv; j++) {
+      const float *p = V + 3 * (size_t)F[(size_t)nv * i + j];
+      for (int k = 0; k < 3; k++) c[k] = c[k] + p[k];
+    }
+    for (int k = 0; k < 3; k++) c[k] *= inv_div;
+    codes[i] = or_morton30_point(c, bmin, inv[0], inv[1], inv[2]);
+  }
+}
+
 /* CalculateMortonCodes30SIMD, render/render_prefix_tree_util.cc:176-228: the
  * SSE2 lanes do the same sub/mul/truncate as the scalar MortionCode30, and
  * the missing 10-bit mask of vpartby2 only matters for ix > 1024, which the
@@ -155,6 +174,27 @@ void or_scene_bounds_tet(const float *V, const uint32_t *F, int64_t n,
   for (int64_t i = 0; i < n; i++)
     for (int j = 0; j < 4; j++) {
       const float *v = V + 3 * (size_t)F[4 * i + j];
+      for (int k = 0; k < 3; k++) {
+        float mn = v[k] - kEPS, mx = v[k] + kEPS;
+        if (mn < bmin[k]) bmin[k] = mn;
+        if (mx > bmax[k]) bmax[k] = mx;
+      }
+    }
+}
+
+/* ComputeBoundingBoxOMP (solid), render/render_accel_solid.cc:1172-1262:
+ * every vertex of every solid -+ 16 * FLT_EPSILON. */
+void or_scene_bounds_sol(const float *V, const uint32_t *F, int64_t n, int nv,
+                         float bmin[3], float bmax[3]) {
+  const float kEPS = FLT_EPSILON * 16;
+  const float *v0 = V + 3 * (size_t)F[0];
+  for (int k = 0; k < 3; k++) {
+    bmin[k] = v0[k] - kEPS;
+    bmax[k] = v0[k] + kEPS;
+  }
+  for (int64_t i = 0; i < n; i++)
+    for (int j = 0; j < nv; j++) {
+      const float *v = V + 3 * (size_t)F[(size_t)nv * i + j];
       for (int k = 0; k < 3; k++) {
         float mn = v[k] - kEPS, mx = v[k] + kEPS;
         if (mn < bmin[k]) bmin[k] = mn;
@@ -297,7 +337,8 @@ static int split_axis(uint32_t code) { return (clz32(code) - 2) % 3; }
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
-  int kind; /* 0 tri, 1 tet, 2 sphere */
+  int kind; /* 0 tri, 1 tet, 2 sphere, 3 solid (pyramid / prism / hexahedron) */
+  int nv;   /* solids: vertices per solid (5, 6, 8) */
   const float *V;
   const uint32_t *F;
   const float *R;
@@ -337,7 +378,7 @@ static void leaf_box(const build_ctx *b, uint32_t l, uint32_t r, float mn[3],
     }
     return;
   }
-  const int nv = b->kind == 0 ? 3 : 4;
+  const int nv = b->kind == 0 ? 3 : (b->kind == 1 ? 4 : b->nv);
   uint32_t idx = b->sidx[l];
   const float *v0 = b->V + 3 * (size_t)b->F[(size_t)nv * idx];
   for (int k = 0; k < 3; k++) {
@@ -371,7 +412,7 @@ static int64_t push_node(build_ctx *b) { return b->count++; }
 
 static void set_leaf(build_ctx *b, int64_t at, uint32_t l, uint32_t end,
                      const float mn[3], const float mx[3]) {
-  if (b->kind == 1) {
+  if (b->kind == 1 || b->kind == 3) {
     or_tet_node *nd = (or_tet_node *)b->nodes + at;
     memset(nd, 0, sizeof(*nd));
     memcpy(nd->bmin, mn, 12);
@@ -402,8 +443,10 @@ static void set_branch(build_ctx *b, int64_t at, int axis, int64_t lc,
                        int64_t rc, const float lmn[3], const float lmx[3],
                        const float rmn[3], const float rmx[3],
                        const float mn[3], const float mx[3]) {
-  if (b->kind == 1) {
-    /* TetraNode keeps its own (merged) box: render/render_accel_tetra.cc:1225-1235 */
+  if (b->kind == 1 || b->kind == 3) {
+    /* TetraNode keeps its own (merged) box: render/render_accel_tetra.cc:1225-1235;
+     * SolidNode has the same 40-byte layout and rule,
+     * render/render_accel_solid.h:24-43, render/render_accel_solid.cc:1436-1450 */
     or_tet_node *nd = (or_tet_node *)b->nodes + at;
     memset(nd, 0, sizeof(*nd));
     memcpy(nd->bmin, mn, 12);
@@ -474,6 +517,9 @@ static int64_t build_common(build_ctx *b, int64_t n, double bounds[6]) {
   } else if (b->kind == 1) {
     or_scene_bounds_tet(b->V, b->F, n, bmin, bmax);
     or_morton_tet(b->V, b->F, n, bmin, bmax, codes);
+  } else if (b->kind == 3) {
+    or_scene_bounds_sol(b->V, b->F, n, b->nv, bmin, bmax);
+    or_morton_sol(b->V, b->F, n, b->nv, bmin, bmax, codes);
   } else {
     or_scene_bounds_sph(b->V, b->R, b->rconst, n, bmin, bmax);
     or_morton_pts(b->V, n, bmin, bmax, codes);
@@ -554,6 +600,25 @@ int64_t or_build_sph(const float *P, const float *R, float rconst, int64_t n,
   b.kEPS = (float)(FLT_EPSILON * 16.0);
   b.sidx = indices;
   b.max_leaf = SPH_MAX_LEAF;
+  b.nodes = nodes;
+  return build_common(&b, n, bounds);
+}
+
+/* SolidAccel::Build32, render/render_accel_solid.cc:1795-2124: the tetra
+ * sequence (MAX_LEAF_ELEMENTS 4 and depth cap 22, :34-36; leaf boxes of
+ * vertices -+ 16 * FLT_EPSILON, ComputeBoundingBox30 :1120-1169) over
+ * solids of nv = 5 (pyramid), 6 (prism) or 8 (hexahedron) vertices. */
+int64_t or_build_sol(const float *V, const uint32_t *F, int64_t n, int nv,
+                     or_tet_node *nodes, uint32_t *indices, double bounds[6]) {
+  build_ctx b;
+  memset(&b, 0, sizeof(b));
+  b.kind = 3;
+  b.nv = nv;
+  b.V = V;
+  b.F = F;
+  b.kEPS = FLT_EPSILON * 16;
+  b.sidx = indices;
+  b.max_leaf = TET_MAX_LEAF;
   b.nodes = nodes;
   return build_common(&b, n, bounds);
 }
@@ -1183,6 +1248,424 @@ void or_trace_tet(const or_tet_node *nodes, const uint32_t *indices,
   for (int64_t i = 0; i < nrays; i++)
     trace_tet_one(nodes, indices, V, F, scalars, rays + i, hits + i,
                   ext ? ext + i : NULL, stats ? stats + 2 * i : NULL);
+}
+
+/* ---- solids (pyramid, prism, hexahedron) ---- */
+
+/* GetEdges, render/render_accel_solid.cc:119-154 (double) and :156-191
+ * (float): edge i runs from vertex i % vmod to vertex i % vmod + edge i. */
+static int sol_edge_count(int nv) { return nv == 5 ? 8 : (nv == 6 ? 9 : 12); }
+static int sol_vmod(int nv) { return nv == 5 ? 4 : nv; }
+static const int kSolEdgeEnds[3][12][2] = {
+    /* pyramid */ {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 4}, {2, 4}, {3, 4}},
+    /* prism */ {{0, 1}, {1, 2}, {2, 0}, {3, 4}, {4, 5}, {5, 3}, {0, 3}, {1, 4}, {2, 5}},
+    /* hexahedron */
+    {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6}, {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6},
+     {3, 7}}};
+static int sol_type(int nv) { return nv == 5 ? 0 : (nv == 6 ? 1 : 2); }
+
+/* Result of Intersect{Pyramid,Prism,Hexa}D for one of the two crossings:
+ * Intersection::{t, position, normal}. */
+typedef struct {
+  float t;
+  float pos[3];
+  float nrm[3];
+} sol_cross;
+
+/* toreal3(num) / den: the double numerator rounded to float, divided by the
+ * float-rounded denominator (real3::operator/(real), render/render_common.h:57-59) */
+static void sol_pos_div(float out[3], d3 num, double den) {
+  const float f = (float)den;
+  out[0] = (float)num.x / f;
+  out[1] = (float)num.y / f;
+  out[2] = (float)num.z / f;
+}
+
+/* SetCrossPoint_Sq (double), render/render_accel_solid.cc:438-444 */
+static void sol_pos_sq(float out[3], d3 v0, d3 v1, d3 v2, double ws0, double ws1, double ws2,
+                       double ws3, d3 e2, d3 e3, d3 rd) {
+  const double w = (ws2 + ws3) + d3dot(d3cross(e2, e3), rd);
+  const d3 num = d3add(d3add(d3scale(ws0, v2), d3scale(ws1, v0)), d3scale(w, v1));
+  const double den = (ws0 + ws1) + w;
+  out[0] = (float)(num.x / den);
+  out[1] = (float)(num.y / den);
+  out[2] = (float)(num.z / den);
+}
+
+/* toreal3(normalize(vcrossd(a, b))), render/render_accel_solid.cc:69-85 */
+static void sol_nrm(float out[3], d3 a, d3 b) {
+  const d3 c = d3cross(a, b);
+  const double len = sqrt((c.x * c.x + c.y * c.y) + c.z * c.z);
+  out[0] = (float)(c.x / len);
+  out[1] = (float)(c.y / len);
+  out[2] = (float)(c.z / len);
+}
+
+/* (v4 * -wa + vb * wb + vc * -wc) / (-wa + wb - wc), the pyramid's side
+ * faces, render/render_accel_solid.cc:484-495 */
+static void sol_pos_pyr(float out[3], d3 v4, d3 vb, d3 vc, double wa, double wb, double wc) {
+  const d3 num = d3add(d3add(d3scale(-wa, v4), d3scale(wb, vb)), d3scale(-wc, vc));
+  sol_pos_div(out, num, ((-wa) + wb) - wc);
+}
+
+/* (va * w0 + vb * w1 + vc * w2) / (w0 + w1 + w2), the prism's triangles,
+ * render/render_accel_solid.cc:585-590 */
+static void sol_pos_tri(float out[3], d3 va, d3 vb, d3 vc, double w0, double w1, double w2) {
+  const d3 num = d3add(d3add(d3scale(w0, va), d3scale(w1, vb)), d3scale(w2, vc));
+  sol_pos_div(out, num, (w0 + w1) + w2);
+}
+
+/* Intersect{Pyramid,Prism,Hexa}D, render/render_accel_solid.cc:455-505,
+ * 559-611, 664-716: Pluecker side products of the ray with every edge,
+ * then for each of the two crossings the first face whose edges all lie on
+ * one side (the "<= 0" side for c[0] of prism / hexahedron and for the
+ * pyramid's base, see the loops' index pairs), its cross point and outward
+ * normal; t from the cross point along the last axis with |dir| > 0.1.
+ * Returns 0 when a crossing finds no face, 1 with both crossings, -1 when
+ * no axis qualifies (the reference leaves t unset; treated as no hit). */
+int or_solid_isect(int nv, const double o_in[3], const double d_in[3], const double *vin,
+                   float out_t[2], float out_pos[6], float out_nrm[6]) {
+  d3 v[8], e[12], ro = {o_in[0], o_in[1], o_in[2]}, rd = {d_in[0], d_in[1], d_in[2]};
+  double ws[12];
+  int cw[2][12];
+  const int ty = sol_type(nv), ne = sol_edge_count(nv), vm = sol_vmod(nv);
+  for (int k = 0; k < nv; k++) {
+    v[k].x = vin[3 * k];
+    v[k].y = vin[3 * k + 1];
+    v[k].z = vin[3 * k + 2];
+  }
+  for (int i = 0; i < ne; i++) e[i] = d3sub(v[kSolEdgeEnds[ty][i][1]], v[kSolEdgeEnds[ty][i][0]]);
+  const d3 raypc = d3cross(rd, ro);
+  for (int i = 0; i < ne; i++) {
+    const d3 epc = d3cross(e[i], v[i % vm]);
+    ws[i] = d3dot(rd, epc) + d3dot(e[i], raypc);
+    cw[0][i] = ws[i] >= 0;
+    cw[1][i] = ws[i] <= 0;
+  }
+  sol_cross is[2];
+  for (int pass = 0; pass < 2; pass++) {
+    sol_cross *c;
+    if (nv == 5) {
+      /* for (int i = 1, n = 0; n < 2; i--, n++) fills isects[n] */
+      const int n = pass, i = 1 - pass;
+      c = &is[n];
+      if (cw[i][0] && cw[i][1] && cw[i][2] && cw[i][3]) {
+        sol_pos_sq(c->pos, v[0], v[1], v[2], ws[0], ws[1], ws[2], ws[3], e[2], e[3], rd);
+        sol_nrm(c->nrm, e[1], e[0]);
+      } else if (cw[n][0] && cw[i][4] && cw[n][5]) {
+        sol_pos_pyr(c->pos, v[4], v[1], v[0], ws[0], ws[4], ws[5]);
+        sol_nrm(c->nrm, e[4], e[0]);
+      } else if (cw[n][1] && cw[i][5] && cw[n][6]) {
+        sol_pos_pyr(c->pos, v[4], v[2], v[1], ws[1], ws[5], ws[6]);
+        sol_nrm(c->nrm, e[5], e[1]);
+      } else if (cw[n][2] && cw[i][6] && cw[n][7]) {
+        sol_pos_pyr(c->pos, v[4], v[3], v[2], ws[2], ws[6], ws[7]);
+        sol_nrm(c->nrm, e[6], e[2]);
+      } else if (cw[n][3] && cw[i][7] && cw[n][4]) {
+        sol_pos_pyr(c->pos, v[4], v[0], v[3], ws[3], ws[7], ws[4]);
+        sol_nrm(c->nrm, e[7], e[3]);
+      } else {
+        return 0;
+      }
+    } else if (nv == 6) {
+      /* for (int i = 0, n = 1; i < 2; i++, n--) fills isects[i] */
+      const int i = pass, n = 1 - pass;
+      c = &is[i];
+      if (cw[n][0] && cw[n][1] && cw[n][2]) {
+        sol_pos_tri(c->pos, v[2], v[0], v[1], ws[0], ws[1], ws[2]);
+        sol_nrm(c->nrm, e[0], e[1]);
+      } else if (cw[i][3] && cw[i][4] && cw[i][5]) {
+        sol_pos_tri(c->pos, v[5], v[3], v[4], ws[3], ws[4], ws[5]);
+        sol_nrm(c->nrm, e[3], e[4]);
+      } else if (cw[i][2] && cw[i][6] && cw[n][5] && cw[n][8]) {
+        sol_pos_sq(c->pos, v[2], v[0], v[3], ws[2], ws[6], -ws[5], -ws[8], e[5], e[8], rd);
+        sol_nrm(c->nrm, e[2], e[6]);
+      } else if (cw[i][0] && cw[i][7] && cw[n][3] && cw[n][6]) {
+        sol_pos_sq(c->pos, v[0], v[1], v[4], ws[0], ws[7], -ws[3], -ws[6], e[3], e[6], rd);
+        sol_nrm(c->nrm, e[0], e[7]);
+      } else if (cw[i][1] && cw[i][8] && cw[n][4] && cw[n][7]) {
+        sol_pos_sq(c->pos, v[1], v[2], v[5], ws[1], ws[8], -ws[4], -ws[7], e[4], e[7], rd);
+        sol_nrm(c->nrm, e[1], e[8]);
+      } else {
+        return 0;
+      }
+    } else {
+      const int i = pass, n = 1 - pass;
+      c = &is[i];
+      if (cw[n][0] && cw[n][1] && cw[n][2] && cw[n][3]) {
+        sol_pos_sq(c->pos, v[0], v[1], v[2], ws[0], ws[1], ws[2], ws[3], e[2], e[3], rd);
+        sol_nrm(c->nrm, e[0], e[1]);
+      } else if (cw[i][4] && cw[i][5] && cw[i][6] && cw[i][7]) {
+        sol_pos_sq(c->pos, v[4], v[5], v[6], ws[4], ws[5], ws[6], ws[7], e[6], e[7], rd);
+        sol_nrm(c->nrm, e[4], e[5]);
+      } else if (cw[i][0] && cw[i][9] && cw[n][4] && cw[n][8]) {
+        sol_pos_sq(c->pos, v[0], v[1], v[5], ws[0], ws[9], -ws[4], -ws[8], e[4], e[8], rd);
+        sol_nrm(c->nrm, e[0], e[9]);
+      } else if (cw[i][1] && cw[i][10] && cw[n][5] && cw[n][9]) {
+        sol_pos_sq(c->pos, v[1], v[2], v[6], ws[1], ws[10], -ws[5], -ws[9], e[5], e[9], rd);
+        sol_nrm(c->nrm, e[1], e[10]);
+      } else if (cw[i][2] && cw[i][11] && cw[n][6] && cw[n][10]) {
+        sol_pos_sq(c->pos, v[2], v[3], v[7], ws[2], ws[11], -ws[6], -ws[10], e[6], e[10], rd);
+        sol_nrm(c->nrm, e[2], e[11]);
+      } else if (cw[i][3] && cw[i][8] && cw[n][7] && cw[n][11]) {
+        sol_pos_sq(c->pos, v[3], v[0], v[4], ws[3], ws[8], -ws[7], -ws[11], e[7], e[11], rd);
+        sol_nrm(c->nrm, e[3], e[8]);
+      } else {
+        return 0;
+      }
+    }
+  }
+  int axis = -1;
+  for (int j = 0; j < 3; j++) {
+    const double dj = j == 0 ? rd.x : (j == 1 ? rd.y : rd.z);
+    if (dj > 0.1 || dj < -0.1) axis = j;
+  }
+  if (axis < 0) return -1;
+  {
+    const double oj = axis == 0 ? ro.x : (axis == 1 ? ro.y : ro.z);
+    const double dj = axis == 0 ? rd.x : (axis == 1 ? rd.y : rd.z);
+    for (int k = 0; k < 2; k++) out_t[k] = (float)(((double)is[k].pos[axis] - oj) / dj);
+  }
+  for (int k = 0; k < 3; k++) is[1].nrm[k] = -is[1].nrm[k];
+  for (int k = 0; k < 2; k++)
+    for (int c = 0; c < 3; c++) {
+      out_pos[3 * k + c] = is[k].pos[c];
+      out_nrm[3 * k + c] = is[k].nrm[c];
+    }
+  return 1;
+}
+
+/* getCP_sq (float vertices, double weights), render/render_accel_solid.cc:
+ * 238-244: the double w rounds to float where it scales a vertex and the
+ * denominator rounds to float before the division. */
+static f3 sol_cp_sq_f(f3 v0, f3 v1, f3 v2, double ws0, double ws1, double ws2, double ws3,
+                      f3 e2, f3 e3, f3 rd) {
+  const double w = (ws2 + ws3) + (double)f3dot(f3cross(e2, e3), rd);
+  const f3 num = f3add(f3add(f3mul(v2, (float)ws0), f3mul(v0, (float)ws1)), f3mul(v1, (float)w));
+  return f3div(num, (float)((ws0 + ws1) + w));
+}
+
+/* (vx * -wa + vy * wb + vz * -wc) / (-wa + wb - wc), all float */
+static f3 sol_cp_pyr_f(f3 vx, f3 vy, f3 vz, float wa, float wb, float wc) {
+  const f3 num = f3add(f3add(f3mul(vx, -wa), f3mul(vy, wb)), f3mul(vz, -wc));
+  return f3div(num, ((-wa) + wb) - wc);
+}
+
+static f3 sol_cp_tri_f(f3 va, f3 vb, f3 vc, float w0, float w1, float w2) {
+  const f3 num = f3add(f3add(f3mul(va, w0), f3mul(vb, w1)), f3mul(vc, w2));
+  return f3div(num, (w0 + w1) + w2);
+}
+
+static int contains4(int v, const int l[4]) {
+  return l[0] == v || l[1] == v || l[2] == v || l[3] == v;
+}
+
+/* Initial value of interpolate()'s kEPS accumulator, which the reference
+ * leaves uninitialised (render/render_accel_solid.cc:271-275); 0 here. */
+#ifndef OR_SOLID_KEPS_INIT
+#define OR_SOLID_KEPS_INIT 0.0f
+#endif
+
+/* interpolate() with interpolate_mode 0, render/render_accel_solid.cc:247-435:
+ * per-vertex weights at p from the ray (vertex -> p) crossing the opposite
+ * faces, everything in float after moving vertex 0 to the origin. */
+void or_interpolate_solid(int nv, const float p[3], const double *vin, float d_out[8]) {
+  static const int faces[3][6][4] = {
+      {{0, 3, 2, 1}, {0, 1, 4, -1}, {1, 2, 4, -1}, {2, 3, 4, -1}, {0, 4, 3, -1}},
+      {{0, 2, 1, -1}, {3, 4, 5, -1}, {0, 3, 5, 2}, {0, 1, 4, 3}, {1, 2, 5, 4}},
+      {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 4, 7, 3}}};
+  const int ty = sol_type(nv), ne = sol_edge_count(nv), vm = sol_vmod(nv);
+  f3 pt = {{p[0], p[1], p[2]}}, verts[8], edges[12], edgepc[12];
+  for (int i = 0; i < nv; i++)
+    for (int k = 0; k < 3; k++) verts[i].v[k] = (float)vin[3 * i + k];
+  pt = f3sub(pt, verts[0]);
+  for (int i = nv - 1; i >= 0; i--) verts[i] = f3sub(verts[i], verts[0]);
+  for (int i = 0; i < ne; i++)
+    edges[i] = f3sub(verts[kSolEdgeEnds[ty][i][1]], verts[kSolEdgeEnds[ty][i][0]]);
+  float kEPS = OR_SOLID_KEPS_INIT;
+  for (int i = 0; i < ne; i++) kEPS += f3len(edges[i]);
+  kEPS = (float)(((double)kEPS * 1e-6) / ne);
+  for (int i = 0; i < ne; i++) edgepc[i] = f3cross(edges[i], verts[i % vm]);
+  for (int v = 0; v < nv; v++) {
+    const f3 rayorg = verts[v];
+    const f3 raydir = f3sub(pt, rayorg);
+    const f3 raypc = f3cross(raydir, rayorg);
+    const float d = f3len(f3sub(pt, rayorg));
+    float ws[12];
+    int c0[12], c1[12];
+    for (int i = 0; i < ne; i++) {
+      ws[i] = f3dot(raydir, edgepc[i]) + f3dot(edges[i], raypc);
+      c0[i] = ws[i] >= -kEPS;
+      c1[i] = ws[i] <= kEPS;
+    }
+    const int(*fc)[4] = faces[ty];
+    const f3 *V = verts;
+    f3 cp;
+    int found = 1;
+    if (nv == 5) {
+      if (c1[0] && c1[1] && c1[2] && c1[3] && !contains4(v, fc[0]))
+        cp = sol_cp_sq_f(V[0], V[1], V[2], ws[0], ws[1], ws[2], ws[3], edges[2], edges[3], raydir);
+      else if (c0[0] && c1[4] && c0[5] && !contains4(v, fc[1]))
+        cp = sol_cp_pyr_f(V[4], V[1], V[0], ws[0], ws[4], ws[5]);
+      else if (c0[1] && c1[5] && c0[6] && !contains4(v, fc[2]))
+        cp = sol_cp_pyr_f(V[4], V[2], V[1], ws[1], ws[5], ws[6]);
+      else if (c0[2] && c1[6] && c0[7] && !contains4(v, fc[3]))
+        cp = sol_cp_pyr_f(V[4], V[3], V[2], ws[2], ws[6], ws[7]);
+      else if (c0[3] && c1[7] && c0[4] && !contains4(v, fc[4]))
+        cp = sol_cp_pyr_f(V[4], V[0], V[3], ws[3], ws[7], ws[4]);
+      else
+        found = 0;
+    } else if (nv == 6) {
+      if (c0[0] && c0[1] && c0[2] && !contains4(v, fc[0]))
+        cp = sol_cp_tri_f(V[2], V[0], V[1], ws[0], ws[1], ws[2]);
+      else if (c1[3] && c1[4] && c1[5] && !contains4(v, fc[1]))
+        cp = sol_cp_tri_f(V[5], V[3], V[4], ws[3], ws[4], ws[5]);
+      else if (c1[2] && c1[6] && c0[5] && c0[8] && !contains4(v, fc[2]))
+        cp = sol_cp_sq_f(V[2], V[0], V[3], ws[2], ws[6], -ws[5], -ws[8], edges[5], edges[8],
+                         raydir);
+      else if (c1[0] && c1[7] && c0[3] && c0[6] && !contains4(v, fc[3]))
+        cp = sol_cp_sq_f(V[0], V[1], V[4], ws[0], ws[7], -ws[3], -ws[6], edges[3], edges[6],
+                         raydir);
+      else if (c1[1] && c1[8] && c0[4] && c0[7] && !contains4(v, fc[4]))
+        cp = sol_cp_sq_f(V[1], V[2], V[5], ws[1], ws[8], -ws[4], -ws[7], edges[4], edges[7],
+                         raydir);
+      else
+        found = 0;
+    } else {
+      if (c0[0] && c0[1] && c0[2] && c0[3] && !contains4(v, fc[0]))
+        cp = sol_cp_sq_f(V[0], V[1], V[2], ws[0], ws[1], ws[2], ws[3], edges[2], edges[3], raydir);
+      else if (c1[4] && c1[5] && c1[6] && c1[7] && !contains4(v, fc[1]))
+        cp = sol_cp_sq_f(V[4], V[5], V[6], ws[4], ws[5], ws[6], ws[7], edges[6], edges[7], raydir);
+      else if (c1[0] && c1[9] && c0[4] && c0[8] && !contains4(v, fc[2]))
+        cp = sol_cp_sq_f(V[0], V[1], V[5], ws[0], ws[9], -ws[4], -ws[8], edges[4], edges[8],
+                         raydir);
+      else if (c1[1] && c1[10] && c0[5] && c0[9] && !contains4(v, fc[3]))
+        cp = sol_cp_sq_f(V[1], V[2], V[6], ws[1], ws[10], -ws[5], -ws[9], edges[5], edges[9],
+                         raydir);
+      else if (c1[2] && c1[11] && c0[6] && c0[10] && !contains4(v, fc[4]))
+        cp = sol_cp_sq_f(V[2], V[3], V[7], ws[2], ws[11], -ws[6], -ws[10], edges[6], edges[10],
+                         raydir);
+      else if (c1[3] && c1[8] && c0[7] && c0[11] && !contains4(v, fc[5]))
+        cp = sol_cp_sq_f(V[3], V[0], V[4], ws[3], ws[8], -ws[7], -ws[11], edges[7], edges[11],
+                         raydir);
+      else
+        found = 0;
+    }
+    if (!found) {
+      d_out[v] = 0;
+      continue;
+    }
+    float dv = d / f3len(f3sub(cp, rayorg));
+    if (dv < 0)
+      dv = 0;
+    else if (!(dv < 1) && !(dv > 0))
+      dv = 1;
+    d_out[v] = 1 - dv;
+  }
+  float ds = 0, w = 0, dd[8];
+  for (int i = 0; i < nv; i++) ds += f3len(f3sub(pt, verts[i]));
+  for (int i = 0; i < nv; i++) dd[i] = ds / f3len(f3sub(pt, verts[i]));
+  for (int i = 0; i < nv; i++) {
+    d_out[i] *= dd[i];
+    w += d_out[i];
+  }
+  for (int i = 0; i < nv; i++) d_out[i] = d_out[i] / w;
+  for (int i = nv; i < 8; i++) d_out[i] = 0;
+}
+
+/* SolidAccel::Traverse, render/render_accel_solid.cc:2345-2435, with
+ * TestLeafNode :2206-2333. TestLeafNode always returns false (its `hit` is
+ * shadowed inside the loop), so the search distance never narrows: every
+ * node whose box the ray enters is visited, and the closest crossing wins
+ * (entry before exit of one solid, strict <, first in visit order). u and v
+ * come from never-written Intersection fields in the reference and are 0
+ * here; subface_id is 0. */
+static void trace_sol_one(const or_tet_node *nodes, const uint32_t *indices, const float *V,
+                          const uint32_t *F, int nv, const float *scalars, const or_ray *r,
+                          or_hit *h, or_sol_ext *ext, uint32_t *st) {
+  ray_setup s;
+  setup_ray(r, &s);
+  h->t = FLT_MAX;
+  h->u = 0.0f;
+  h->v = 0.0f;
+  h->prim_id = 0xffffffffu;
+  float best_pos[3] = {0, 0, 0}, best_nrm[3] = {0, 0, 0};
+  uint32_t best_side = 0;
+  double bestv[24];
+  int stack[STACK_MAX];
+  int sp = 0;
+  stack[0] = 0;
+  uint32_t nvisit = 0, ntest = 0;
+  const double ro[3] = {s.o[0], s.o[1], s.o[2]}, rd[3] = {s.d[0], s.d[1], s.d[2]};
+  while (sp >= 0) {
+    const or_tet_node *nd = nodes + stack[sp--];
+    nvisit++;
+    const int hit = ray_aabb(&s, FLT_MAX, nd->bmin, nd->bmax, 0);
+    if (nd->flag == 0) {
+      if (hit) {
+        int nearc = s.sign[nd->axis], farc = 1 - nearc;
+        stack[++sp] = (int)nd->data[farc];
+        stack[++sp] = (int)nd->data[nearc];
+      }
+    } else if (hit) {
+      float t = h->t;
+      for (uint32_t i = 0; i < nd->data[0]; i++) {
+        const uint32_t pi = indices[i + nd->data[1]];
+        if (pi == r->prev_prim_id) continue;
+        ntest++;
+        double vtx[24];
+        for (int j = 0; j < nv; j++) {
+          const float *p = V + 3 * (size_t)F[(size_t)nv * pi + j];
+          for (int k = 0; k < 3; k++) vtx[3 * j + k] = p[k];
+        }
+        float ts[2], pos[6], nrm[6];
+        if (or_solid_isect(nv, ro, rd, vtx, ts, pos, nrm) != 1) continue;
+        for (int k = 0; k < 2; k++) {
+          if ((ts[k] >= 0.0) && (ts[k] < t)) {
+            h->t = ts[k];
+            h->prim_id = pi;
+            memcpy(best_pos, pos + 3 * k, 12);
+            memcpy(best_nrm, nrm + 3 * k, 12);
+            best_side = (uint32_t)k;
+            memcpy(bestv, vtx, sizeof(double) * 3 * nv);
+            t = h->t;
+          }
+        }
+      }
+    }
+  }
+  if (st) {
+    st[0] = nvisit;
+    st[1] = ntest;
+  }
+  if (ext) {
+    memset(ext, 0, sizeof(*ext));
+    if (h->t < FLT_MAX) {
+      or_interpolate_solid(nv, best_pos, bestv, ext->d);
+      memcpy(ext->pos, best_pos, 12);
+      memcpy(ext->nrm, best_nrm, 12);
+      ext->side = best_side;
+      if (scalars) {
+        /* f_k = nv * prim + k (TestLeafNode :2284-2291), the varying sum of
+         * gles/gles_common.cc:1078-1140 from left to right */
+        float acc = scalars[(size_t)nv * h->prim_id] * ext->d[0];
+        for (int k = 1; k < nv; k++) acc = acc + scalars[(size_t)nv * h->prim_id + k] * ext->d[k];
+        ext->scalar = acc;
+      }
+    }
+  }
+}
+
+void or_trace_sol(const or_tet_node *nodes, const uint32_t *indices, const float *V,
+                  const uint32_t *F, int nv, const float *scalars, const or_ray *rays,
+                  int64_t nrays, or_hit *hits, or_sol_ext *ext, uint32_t *stats, int threads) {
+  (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+#endif
+  for (int64_t i = 0; i < nrays; i++)
+    trace_sol_one(nodes, indices, V, F, nv, scalars, rays + i, hits + i, ext ? ext + i : NULL,
+                  stats ? stats + 2 * i : NULL);
 }
 
 /* ---- spheres ---- */

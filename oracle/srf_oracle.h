@@ -11,6 +11,7 @@
  * reference's own Dump():
  *   or_tri_node  == TriangleNode  render/render_accel_triangle.h:23-42 (64 B)
  *   or_tet_node  == TetraNode     render/render_accel_tetra.h:24-43    (40 B)
+ *                   and SolidNode render/render_accel_solid.h:24-43    (40 B)
  *   or_sph_node  == ParticleNode  render/render_accel_particle.h:24-44 (60 B)
  *   or_nodeinfo  == NodeInfo32    render/render_prefix_tree_util.h:51-61
  */
@@ -79,6 +80,19 @@ typedef struct {
   float pad;
 } or_tet_ext;
 
+/* Solid extras at the reported crossing: interpolate() weights d0..d7
+ * (render/render_accel_solid.cc:247-435; unused entries 0), the crossing's
+ * Intersection::position and normal (the exit crossing's normal negated),
+ * the varying sum of scalar[nv * prim + k] * d_k (gles/gles_common.cc:
+ * 1078-1140) and which crossing it is (0 entry, 1 exit). */
+typedef struct {
+  float d[8];
+  float pos[3];
+  float scalar;
+  float nrm[3];
+  uint32_t side;
+} or_sol_ext;
+
 /* ---- LBVH stages ---- */
 void or_scene_bounds_tri(const float *V, const uint32_t *F, int64_t n,
                          float bmin[3], float bmax[3]);
@@ -89,6 +103,10 @@ void or_scene_bounds_sph(const float *P, const float *R, float rconst,
 void or_morton_tri(const float *V, const uint32_t *F, int64_t n,
                    const float bmin[3], const float bmax[3], uint32_t *codes);
 void or_morton_tet(const float *V, const uint32_t *F, int64_t n,
+                   const float bmin[3], const float bmax[3], uint32_t *codes);
+void or_scene_bounds_sol(const float *V, const uint32_t *F, int64_t n, int nv,
+                         float bmin[3], float bmax[3]);
+void or_morton_sol(const float *V, const uint32_t *F, int64_t n, int nv,
                    const float bmin[3], const float bmax[3], uint32_t *codes);
 void or_morton_pts(const float *P, int64_t n, const float bmin[3],
                    const float bmax[3], uint32_t *codes);
@@ -110,6 +128,9 @@ int64_t or_build_tet(const float *V, const uint32_t *F, int64_t n,
                      or_tet_node *nodes, uint32_t *indices, double bounds[6]);
 int64_t or_build_sph(const float *P, const float *R, float rconst, int64_t n,
                      or_sph_node *nodes, uint32_t *indices, double bounds[6]);
+/* solids of nv = 5 (pyramid), 6 (prism) or 8 (hexahedron) vertices */
+int64_t or_build_sol(const float *V, const uint32_t *F, int64_t n, int nv,
+                     or_tet_node *nodes, uint32_t *indices, double bounds[6]);
 
 /* ---- Closest-hit traversal (one ray after another, or OpenMP over rays
  * when threads > 1). stats (optional) receives, per ray, {nodes visited,
@@ -122,6 +143,16 @@ void or_trace_tet(const or_tet_node *nodes, const uint32_t *indices,
                   const float *V, const uint32_t *F, const float *scalars,
                   const or_ray *rays, int64_t nrays, or_hit *hits,
                   or_tet_ext *ext, uint32_t *stats, int threads);
+void or_trace_sol(const or_tet_node *nodes, const uint32_t *indices,
+                  const float *V, const uint32_t *F, int nv, const float *scalars,
+                  const or_ray *rays, int64_t nrays, or_hit *hits, or_sol_ext *ext,
+                  uint32_t *stats, int threads);
+/* One solid against one ray (double origin / direction / vertices):
+ * 1 with both crossings' t, position, normal; 0 no crossing; -1 no axis
+ * with |dir| > 0.1. */
+int or_solid_isect(int nv, const double o[3], const double d[3], const double *verts,
+                   float t[2], float pos[6], float nrm[6]);
+void or_interpolate_solid(int nv, const float p[3], const double *verts, float d_out[8]);
 void or_trace_sph(const or_sph_node *nodes, const uint32_t *indices,
                   const float *P, const float *R, float rconst,
                   const or_ray *rays, const double *prev_hit_t,

@@ -1,9 +1,10 @@
 """Python host mirror of SURFACE's accelerator interface over libsrf.so.
 
 The classes keep the reference's names and meaning -- TriangleAccel,
-TetraAccel and ParticleAccel with Build32 / Traverse / BoundingBox
-(render/render_accel_triangle.h:69-105, render/render_accel_tetra.h:82-125,
-render/render_accel_particle.h:68-110) -- with Traverse taking a batch of
+TetraAccel, ParticleAccel and SolidAccel with Build32 / Traverse /
+BoundingBox (render/render_accel_triangle.h:69-105,
+render/render_accel_tetra.h:82-125, render/render_accel_particle.h:68-110,
+render/render_accel_solid.h:82-125) -- with Traverse taking a batch of
 rays. Geometry, rays and hits are torch tensors on the GPU (torch is only the
 allocator and stream provider here); every computation runs in libsrf.so.
 Failures raise SrfError where the reference returns false or asserts.
@@ -16,7 +17,7 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import AccelInfo, SphereDesc, SrfError, TetDesc, TriDesc, check, load
+from ._lib import AccelInfo, SolidDesc, SphereDesc, SrfError, TetDesc, TriDesc, check, load
 from .workloads import HIT_DTYPE, RAY_DTYPE, TET_EXT_DTYPE
 
 try:
@@ -284,6 +285,34 @@ class TetraAccel(_Accel):
         return hits, ext
 
 
+class SolidAccel(_Accel):
+    """SolidAccel::Build32 / Traverse / BoundingBox over pyramids (5
+    vertices per solid), prisms (6) or hexahedra (8)."""
+    kind = _lib.PRIM_HEXAHEDRONS
+    node_dtype = TET_NODE_DTYPE
+
+    def Build32(self, vertices, indices, vertices_per_solid, stream=None):
+        k = int(vertices_per_solid)
+        d = SolidDesc(vertices.data_ptr(), vertices.numel() // 3, indices.data_ptr(),
+                      indices.numel() // k if k else 0, k)
+        self.kind = _lib.PRIM_SOLID.get(k, _lib.PRIM_HEXAHEDRONS)
+        return self._build(self.lib.srf_accel_build_solids, d, stream)
+
+    def TraverseSolid(self, rays, scalars=None, hits=None, ext=None, stream=None):
+        """Closest crossing + interpolate() weights, position, normal and the
+        varying sum of the per-index-position scalars; ext is [n, 16] float32
+        (srf_solid_hit_ext)."""
+        self._require()
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+        if ext is None:
+            ext = torch.empty((n, 16), dtype=torch.float32, device=rays.device)
+        check(self.lib.srf_trace_solids(self.handle, _ptr(rays), n, _ptr(scalars), _ptr(hits),
+                                        _ptr(ext), _stream(stream)))
+        return hits, ext
+
+
 class ParticleAccel(_Accel):
     """ParticleAccel::Build32 / Traverse / BoundingBox (spheres)."""
     kind = _lib.PRIM_POINTS
@@ -318,6 +347,11 @@ def morton30(kind, desc_tensors, bmin, bmax, ctx=None, stream=None):
         V, F = desc_tensors
         d = TetDesc(V.data_ptr(), V.numel() // 3, F.data_ptr(), F.numel() // 4)
         n = F.numel() // 4
+    elif kind in (_lib.PRIM_PYRAMIDS, _lib.PRIM_PRISMS, _lib.PRIM_HEXAHEDRONS):
+        V, F = desc_tensors
+        k = {_lib.PRIM_PYRAMIDS: 5, _lib.PRIM_PRISMS: 6, _lib.PRIM_HEXAHEDRONS: 8}[kind]
+        d = SolidDesc(V.data_ptr(), V.numel() // 3, F.data_ptr(), F.numel() // k, k)
+        n = F.numel() // k
     else:
         P, R = desc_tensors
         d = SphereDesc(P.data_ptr(), R.data_ptr() if R is not None else None, 0.0, P.numel() // 3)

@@ -81,7 +81,11 @@ struct DeviceGuard {
 };
 
 size_t node_bytes(Kind) { return 64; }
-size_t prim_bytes(Kind k) { return k == kTri ? 48 : (k == kTet ? 64 : 16); }
+// packed primitive record: triangle 48 B, tetra 64 B, sphere 16 B, solid
+// 16 B per vertex
+size_t prim_bytes(Kind k, uint32_t nvps) {
+  return k == kTri ? 48 : (k == kTet ? 64 : (k == kSol ? 16 * (size_t)nvps : 16));
+}
 
 void free_buffers(Accel *a);
 
@@ -99,7 +103,7 @@ hipError_t alloc_buffers(Accel *a, uint64_t n) {
   hipError_t e = hipSuccess;
   const Kind kind = a->kind;
   if (e == hipSuccess) e = hipMalloc(&a->nodes, node_bytes(kind) * (2 * n));
-  if (e == hipSuccess) e = hipMalloc(&a->prims, prim_bytes(kind) * n);
+  if (e == hipSuccess) e = hipMalloc(&a->prims, prim_bytes(kind, a->nvps) * n);
   if (e == hipSuccess) e = hipMalloc((void **)&a->sorted_index, 4 * n);
   if (e == hipSuccess && kind == kTet) e = hipMalloc((void **)&a->sorted_faces, 16 * n);
   if (e == hipSuccess && kind == kSph) {
@@ -128,7 +132,8 @@ void free_buffers(Accel *a) {
   free_sub_buffers(a);
 }
 
-srf_status alloc_accel(Ctx *ctx, Kind kind, uint64_t n, uint64_t nv, Accel **out) {
+srf_status alloc_accel(Ctx *ctx, Kind kind, uint64_t n, uint64_t nv, Accel **out,
+                       uint32_t nvps = 0) {
   Accel *a = new (std::nothrow) Accel();
   if (!a) return set_error(SRF_ERR_OUT_OF_MEMORY, "host allocation failed");
   memset(a, 0, sizeof(*a));
@@ -136,6 +141,7 @@ srf_status alloc_accel(Ctx *ctx, Kind kind, uint64_t n, uint64_t nv, Accel **out
   a->device = ctx->device;
   a->n = n;
   a->nv = nv;
+  a->nvps = nvps;
   hipError_t e = alloc_buffers(a, n);
   if (e == hipSuccess) e = hipMalloc((void **)&a->params, sizeof(BuildParams));
   if (e == hipSuccess) e = hipEventCreateWithFlags(&a->built, hipEventDisableTiming);
@@ -160,7 +166,7 @@ srf_status check_count(uint64_t n) {
 }
 
 srf_status build_common(srf_ctx *c, Kind kind, const void *desc, uint64_t n, uint64_t nv,
-                        void *stream, srf_accel **out) {
+                        void *stream, srf_accel **out, uint32_t nvps = 0) {
   Ctx *ctx = reinterpret_cast<Ctx *>(c);
   if (!ctx || !desc || !out) return set_error(SRF_ERR_INVALID_ARG, "null argument");
   *out = nullptr;
@@ -169,7 +175,7 @@ srf_status build_common(srf_ctx *c, Kind kind, const void *desc, uint64_t n, uin
   DeviceGuard g(ctx->device);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   Accel *a = nullptr;
-  srf_status st = alloc_accel(ctx, kind, n, nv, &a);
+  srf_status st = alloc_accel(ctx, kind, n, nv, &a, nvps);
   if (st != SRF_OK) return st;
   st = build_lbvh(ctx, kind, desc, s, a);
   if (st == SRF_OK) {
@@ -267,6 +273,16 @@ srf_status srf_accel_build_tetras(srf_ctx *ctx, const srf_tet_desc *d, void *str
                                   srf_accel **out) {
   if (!d || !d->vertices || !d->indices) return set_error(SRF_ERR_INVALID_ARG, "null tetras");
   return build_common(ctx, kTet, d, d->num_tetras, d->num_vertices, stream, out);
+}
+
+srf_status srf_accel_build_solids(srf_ctx *ctx, const srf_solid_desc *d, void *stream,
+                                  srf_accel **out) {
+  if (!d || !d->vertices || !d->indices) return set_error(SRF_ERR_INVALID_ARG, "null solids");
+  const uint32_t k = d->vertices_per_solid;
+  if (k != 5 && k != 6 && k != 8)
+    return set_error(SRF_ERR_INVALID_ARG,
+                     "%u vertices per solid: 5 (pyramid), 6 (prism) or 8 (hexahedron)", k);
+  return build_common(ctx, kSol, d, d->num_solids, d->num_vertices, stream, out, k);
 }
 
 srf_status srf_accel_build_spheres(srf_ctx *ctx, const srf_sphere_desc *d, void *stream,
@@ -368,6 +384,14 @@ srf_status srf_accel_rebuild(srf_ctx *ctx, srf_accel *accel, const void *desc, v
     if (!d->vertices || !d->indices) return set_error(SRF_ERR_INVALID_ARG, "null tetras");
     n = d->num_tetras;
     nv = d->num_vertices;
+  } else if (a->kind == kSol) {
+    const srf_solid_desc *d = (const srf_solid_desc *)desc;
+    if (!d->vertices || !d->indices) return set_error(SRF_ERR_INVALID_ARG, "null solids");
+    if (d->vertices_per_solid != a->nvps)
+      return set_error(SRF_ERR_INVALID_ARG, "%u vertices per solid; the accelerator has %u",
+                       d->vertices_per_solid, a->nvps);
+    n = d->num_solids;
+    nv = d->num_vertices;
   } else {
     const srf_sphere_desc *d = (const srf_sphere_desc *)desc;
     if (!d->positions) return set_error(SRF_ERR_INVALID_ARG, "null particles");
@@ -410,14 +434,18 @@ srf_status srf_accel_get_info(const srf_accel *accel, srf_accel_info *info) {
   BuildParams p;
   srf_status st = fetch_params(a, &p);
   if (st != SRF_OK) return st;
-  info->kind = a->kind == kTri ? SRF_PRIM_TRIANGLES
-                               : (a->kind == kTet ? SRF_PRIM_TETRAHEDRONS : SRF_PRIM_POINTS);
+  if (a->kind == kSol)
+    info->kind = a->nvps == 5 ? SRF_PRIM_PYRAMIDS
+                              : (a->nvps == 6 ? SRF_PRIM_PRISMS : SRF_PRIM_HEXAHEDRONS);
+  else
+    info->kind = a->kind == kTri ? SRF_PRIM_TRIANGLES
+                                 : (a->kind == kTet ? SRF_PRIM_TETRAHEDRONS : SRF_PRIM_POINTS);
   info->num_prims = a->n;
   info->num_nodes = p.num_nodes;
   info->num_leaves = p.num_leaves;
   info->max_depth = p.max_steps + 1;
   info->eps_scale = p.eps_scale;
-  info->node_bytes = a->kind == kTri ? 64 : (a->kind == kTet ? 40 : 60);
+  info->node_bytes = a->kind == kTri ? 64 : ((a->kind == kTet || a->kind == kSol) ? 40 : 60);
   return SRF_OK;
 }
 
@@ -442,8 +470,8 @@ srf_status srf_accel_export(const srf_accel *accel, void *nodes_host, uint32_t *
     return hip_error(e, "hipMemcpy");
   }
   uint8_t *dst = reinterpret_cast<uint8_t *>(nodes_host);
-  if (a->kind == kTet) {
-    // TetraNode: bmin[3], bmax[3], flag, axis, data[2] (40 B); a node's own
+  if (a->kind == kTet || a->kind == kSol) {
+    // TetraNode / SolidNode: bmin[3], bmax[3], flag, axis, data[2] (40 B); a node's own
     // box is the one its parent holds for it, the root's is BoundingBox()
     float rb[6];
     for (int k = 0; k < 3; k++) {
@@ -526,7 +554,19 @@ srf_status srf_trace_closest(const srf_accel *accel, const srf_ray *rays, uint64
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (a->kind == kTri) return trace_tri(a, rays, n, hits, s);
   if (a->kind == kTet) return trace_tet(a, rays, n, nullptr, hits, nullptr, s);
+  if (a->kind == kSol) return trace_sol(a, rays, n, nullptr, hits, nullptr, s);
   return trace_sph(a, rays, nullptr, n, hits, s);
+}
+
+srf_status srf_trace_solids(const srf_accel *accel, const srf_ray *rays, uint64_t n,
+                            const float *scalars, srf_hit *hits, srf_solid_hit_ext *ext,
+                            void *stream) {
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!a || (n && (!rays || !hits))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (a->kind != kSol) return set_error(SRF_ERR_INVALID_ARG, "not a solid accelerator");
+  if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many rays");
+  DeviceGuard g(a->device);
+  return trace_sol(a, rays, n, scalars, hits, ext, reinterpret_cast<hipStream_t>(stream));
 }
 
 srf_status srf_trace_tetra(const srf_accel *accel, const srf_ray *rays, uint64_t n,
@@ -573,7 +613,13 @@ srf_status srf_lbvh_morton30(srf_ctx *ctx, srf_prim_kind kind, const void *desc,
     k = kTet;
   else if (kind == SRF_PRIM_POINTS)
     k = kSph;
-  else
+  else if (kind == SRF_PRIM_PYRAMIDS || kind == SRF_PRIM_PRISMS || kind == SRF_PRIM_HEXAHEDRONS) {
+    const uint32_t want = kind == SRF_PRIM_PYRAMIDS ? 5 : (kind == SRF_PRIM_PRISMS ? 6 : 8);
+    if (((const srf_solid_desc *)desc)->vertices_per_solid != want)
+      return set_error(SRF_ERR_INVALID_ARG, "descriptor does not hold %u vertices per solid",
+                       want);
+    k = kSol;
+  } else
     return set_error(SRF_ERR_INVALID_ARG, "unknown primitive kind 0x%x", (unsigned)kind);
   DeviceGuard g(c->device);
   return launch_morton(k, desc, bmin, bmax, codes, reinterpret_cast<hipStream_t>(stream));

@@ -111,6 +111,7 @@ struct PrimSrc {
   // spheres: floats from one centre (radius) to the next -- 3 (1) for the
   // caller's arrays, 4 (4) for packed (centre, radius) records
   uint32_t vstride, rstride;
+  uint32_t nv;  // solids: vertices per solid (5, 6, 8)
 };
 
 __device__ __forceinline__ float3 sph_pos(const PrimSrc &s, uint32_t i) {
@@ -180,9 +181,15 @@ __device__ __forceinline__ uint32_t seg_off(const uint32_t *off, uint32_t m) {
 //  tri  ComputeBoundingBoxOMP, epsScale 0  render/render_accel_triangle.cc:711-797,1430-1443
 //  tet  kEPS = 16*FLT_EPSILON               render/render_accel_tetra.cc:971-1055
 //  sph  pos -+ r -+ 16*FLT_EPSILON          render/render_accel_particle.cc:303-378
+//  sol  kEPS = 16*FLT_EPSILON               render/render_accel_solid.cc:1172-1262
 template <int K>
 __device__ __forceinline__ void scene_add(Box &b, const PrimSrc &s, uint32_t i) {
-  if (K == kTri) {
+  if (K == kSol) {
+    for (uint32_t j = 0; j < s.nv; j++) {
+      float3 v = load_v3(s.V, s.F[(size_t)s.nv * i + j]);
+      box_add(b, v.x, v.y, v.z, FLT_EPSILON * 16);
+    }
+  } else if (K == kTri) {
     for (int j = 0; j < 3; j++) {
       float3 v = load_v3(s.V, s.F[3 * (size_t)i + j]);
       box_add(b, v.x, v.y, v.z, FLT_EPSILON * 0.0f);
@@ -291,7 +298,7 @@ __global__ void __launch_bounds__(256) k_bounds(SegView v, Box *partials,
     p.eps_scale = es;
     if (K == kTri)
       p.kEPS = FLT_EPSILON * (float)es;  // ComputeBoundingBox30's real kEPS
-    else if (K == kTet)
+    else if (K == kTet || K == kSol)
       p.kEPS = FLT_EPSILON * 16;
     else
       p.kEPS = (float)(FLT_EPSILON * 16.0);
@@ -313,10 +320,22 @@ __global__ void __launch_bounds__(256) k_bounds(SegView v, Box *partials,
 // tri CalculateMortonCodesTriangleFloat30 render/render_prefix_tree_util.cc:230-265
 // tet CalculateMortonCodesTetraFloat30    render/render_prefix_tree_util.cc:267-304
 // sph CalculateMortonCodes30SIMD          render/render_prefix_tree_util.cc:176-228
+// sol CalculateMortonCodesSolidFloat30    render/render_prefix_tree_util.cc:347-382
 template <int K>
 __device__ __forceinline__ uint32_t prim_code(const PrimSrc &s, uint32_t i,
                                               const BuildParams &p) {
-  if (K == kTri) {
+  if (K == kSol) {
+    // the vertices summed from (0, 0, 0) in index order, times 1 / nv
+    const float inv_div = 1.0f / (float)s.nv;
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    for (uint32_t j = 0; j < s.nv; j++) {
+      float3 a = load_v3(s.V, s.F[(size_t)s.nv * i + j]);
+      cx = cx + a.x;
+      cy = cy + a.y;
+      cz = cz + a.z;
+    }
+    return morton30(cx * inv_div, cy * inv_div, cz * inv_div, p);
+  } else if (K == kTri) {
     const float one_third = 1.0f / 3.0f;
     float3 a = load_v3(s.V, s.F[3 * (size_t)i]);
     float3 b = load_v3(s.V, s.F[3 * (size_t)i + 1]);
@@ -940,9 +959,22 @@ __device__ __forceinline__ void leaf_box_pack(const PrimSrc &s, const uint32_t *
   // tri ComputeBoundingBox30 render/render_accel_triangle.cc:660-709
   // tet ComputeBoundingBox30 render/render_accel_tetra.cc:921-969
   // sph ComputeBoundingBox30 render/render_accel_particle.cc:417-462
+  // sol ComputeBoundingBox30 render/render_accel_solid.cc:1120-1169
   for (uint32_t q = first; q < first + count; q++) {
     uint32_t id = sidx[q];
-    if (K == kTri) {
+    if (K == kSol) {
+      // nv records per solid: vertex k in xyz, the solid's id in w of the first
+      float4 *P = reinterpret_cast<float4 *>(prims) + (size_t)s.nv * q;
+      for (uint32_t k = 0; k < s.nv; k++) {
+        float3 v = load_v3(s.V, s.F[(size_t)s.nv * id + k]);
+        if (q == first && k == 0) {
+          b.mn[0] = v.x - e; b.mn[1] = v.y - e; b.mn[2] = v.z - e;
+          b.mx[0] = v.x + e; b.mx[1] = v.y + e; b.mx[2] = v.z + e;
+        }
+        box_add(b, v.x, v.y, v.z, e);
+        P[k] = make_float4(v.x, v.y, v.z, k == 0 ? __uint_as_float(id) : 0.0f);
+      }
+    } else if (K == kTri) {
       uint32_t f0 = s.F[3 * (size_t)id], f1 = s.F[3 * (size_t)id + 1],
                f2 = s.F[3 * (size_t)id + 2];
       float3 a = load_v3(s.V, f0), bb = load_v3(s.V, f1), c = load_v3(s.V, f2);
@@ -1074,9 +1106,19 @@ __global__ void __launch_bounds__(256) k_leaves(uint32_t n_all, const uint32_t *
 template <int K>
 __device__ __forceinline__ int pack_prim(const PrimSrc &s, const uint32_t *sidx, uint32_t q,
                                          float e, void *prims, uint32_t *sorted_faces,
-                                         float lo[4][3], float hi[4][3]) {
+                                         float lo[8][3], float hi[8][3]) {
   const uint32_t id = sidx[q];
-  if (K == kTri) {
+  if (K == kSol) {
+    float4 *P = reinterpret_cast<float4 *>(prims) + (size_t)s.nv * q;
+    const int nv = s.nv < 8u ? (int)s.nv : 8;
+    for (int k = 0; k < nv; k++) {
+      float3 v = load_v3(s.V, s.F[(size_t)s.nv * id + k]);
+      P[k] = make_float4(v.x, v.y, v.z, k == 0 ? __uint_as_float(id) : 0.0f);
+      lo[k][0] = v.x - e; lo[k][1] = v.y - e; lo[k][2] = v.z - e;
+      hi[k][0] = v.x + e; hi[k][1] = v.y + e; hi[k][2] = v.z + e;
+    }
+    return nv;
+  } else if (K == kTri) {
     float3 v[3];
     for (int k = 0; k < 3; k++) v[k] = load_v3(s.V, s.F[3 * (size_t)id + k]);
     float4 *P = reinterpret_cast<float4 *>(prims) + 3 * (size_t)q;
@@ -1157,10 +1199,10 @@ __global__ void __launch_bounds__(256) k_leaves_big(const uint32_t *sidx_all, No
     }
     for (uint32_t q = first + threadIdx.x; q < first + count; q += blockDim.x) {
       if (M) sg.sorted_index[q] = sidx[q];
-      float lo[4][3], hi[4][3];
+      float lo[8][3], hi[8][3];
       const int nc = pack_prim<K>(s, sidx, q, e, sg.prims, sg.sorted_faces, lo, hi);
       for (int k = 0; k < nc; k++) {
-        const uint32_t at = (q - first) * 4u + (uint32_t)k;
+        const uint32_t at = (q - first) * 8u + (uint32_t)k;
         for (int c = 0; c < 3; c++) {
           ex[c] = ext_min(ex[c], Ext{lo[k][c], at});
           ex[3 + c] = ext_max(ex[3 + c], Ext{hi[k][c], at});
@@ -1307,6 +1349,12 @@ PrimSrc make_src(Kind kind, const void *desc) {
     s.V = d->vertices;
     s.F = d->indices;
     s.n = (uint32_t)d->num_tetras;
+  } else if (kind == kSol) {
+    const srf_solid_desc *d = (const srf_solid_desc *)desc;
+    s.V = d->vertices;
+    s.F = d->indices;
+    s.n = (uint32_t)d->num_solids;
+    s.nv = d->vertices_per_solid;
   } else {
     const srf_sphere_desc *d = (const srf_sphere_desc *)desc;
     s.V = d->positions;
@@ -1468,7 +1516,9 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
   } else {
     b.ss.vals_a = segs[0].sorted_index;  // the sorted index lands in the accel
   }
-  int max_leaf = kind == kTri ? kTriMaxLeaf : (kind == kTet ? kTetMaxLeaf : kSphMaxLeaf);
+  // SolidAccel's MAX_LEAF_ELEMENTS is the tetra's 4 (render/render_accel_solid.cc:34)
+  int max_leaf =
+      kind == kTri ? kTriMaxLeaf : ((kind == kTet || kind == kSol) ? kTetMaxLeaf : kSphMaxLeaf);
 
   // zero the bounds tickets (the last bounds block rewrites each params block)
   if (multi) {
@@ -1491,6 +1541,8 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
         hipLaunchKernelGGL((KERNEL<kTri, true>), __VA_ARGS__);                           \
       else if (kind == kTet)                                                           \
         hipLaunchKernelGGL((KERNEL<kTet, true>), __VA_ARGS__);                           \
+      else if (kind == kSol)                                                           \
+        hipLaunchKernelGGL((KERNEL<kSol, true>), __VA_ARGS__);                           \
       else                                                                             \
         hipLaunchKernelGGL((KERNEL<kSph, true>), __VA_ARGS__);                           \
     } else {                                                                           \
@@ -1498,6 +1550,8 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
         hipLaunchKernelGGL((KERNEL<kTri, false>), __VA_ARGS__);                          \
       else if (kind == kTet)                                                           \
         hipLaunchKernelGGL((KERNEL<kTet, false>), __VA_ARGS__);                          \
+      else if (kind == kSol)                                                           \
+        hipLaunchKernelGGL((KERNEL<kSol, false>), __VA_ARGS__);                          \
       else                                                                             \
         hipLaunchKernelGGL((KERNEL<kSph, false>), __VA_ARGS__);                          \
     }                                                                                  \
@@ -1676,6 +1730,8 @@ srf_status launch_morton(Kind kind, const void *desc, const float bmin[3],
     hipLaunchKernelGGL(k_morton_box<kTri>, dim3(blocks), dim3(256), 0, stream, s, p, codes);
   else if (kind == kTet)
     hipLaunchKernelGGL(k_morton_box<kTet>, dim3(blocks), dim3(256), 0, stream, s, p, codes);
+  else if (kind == kSol)
+    hipLaunchKernelGGL(k_morton_box<kSol>, dim3(blocks), dim3(256), 0, stream, s, p, codes);
   else
     hipLaunchKernelGGL(k_morton_box<kSph>, dim3(blocks), dim3(256), 0, stream, s, p, codes);
   SRF_HIP(hipGetLastError());

@@ -28,6 +28,7 @@
 //   triangles  TriangleAccel::Traverse  render/render_accel_triangle.cc:2511-2669
 //   tetras     TetraAccel::Traverse     render/render_accel_tetra.cc:2200-2289
 //   spheres    ParticleAccel::Traverse  render/render_accel_particle.cc:3053-3218
+//   solids     SolidAccel::Traverse     render/render_accel_solid.cc:2345-2435
 #include <algorithm>
 
 #include "srf_trace_dev.h"
@@ -860,6 +861,479 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   }
 }
 
+// ------------------------------------------------------------------- solids
+// SolidAccel (pyramids, prisms, hexahedra), render/render_accel_solid.cc.
+// TestLeafNode always returns false there (:2206-2333, its `hit` is shadowed
+// inside the loop), so Traverse never narrows its search: every node whose
+// own box the ray enters is visited, nearer child first, and the closest
+// crossing wins (a solid's entry before its exit, strict <, first in visit
+// order). A crossing's point need not lie inside the solid's box (a warped
+// quad face is crossed on the plane of three of its corners), so no node or
+// solid can be skipped by distance; the kernel visits exactly the
+// reference's nodes in the reference's order.
+
+// Edge i of a solid runs from vertex kSolA[i] (= i % 4 for pyramids, i % nv
+// otherwise) to kSolB[i]: GetEdges, render/render_accel_solid.cc:119-154.
+template <int NV>
+struct SolTopo;
+template <>
+struct SolTopo<5> {
+  static constexpr int kEdges = 8;
+  static __device__ __forceinline__ int a(int i) { return i & 3; }
+  static __device__ __forceinline__ int b(int i) { return i < 4 ? ((i + 1) & 3) : 4; }
+};
+template <>
+struct SolTopo<6> {
+  static constexpr int kEdges = 9;
+  static __device__ __forceinline__ int a(int i) { return i < 6 ? i : i - 6; }
+  static __device__ __forceinline__ int b(int i) {
+    return i < 3 ? (i + 1) % 3 : (i < 6 ? 3 + (i - 2) % 3 : i - 3);
+  }
+};
+template <>
+struct SolTopo<8> {
+  static constexpr int kEdges = 12;
+  static __device__ __forceinline__ int a(int i) { return i < 8 ? i : i - 8; }
+  static __device__ __forceinline__ int b(int i) {
+    return i < 4 ? (i + 1) & 3 : (i < 8 ? 4 + ((i + 1) & 3) : i - 4);
+  }
+};
+
+__device__ __forceinline__ D3 f4d(const float4 &p) { return {p.x, p.y, p.z}; }
+__device__ __forceinline__ double d3c(const D3 &v, int j) {
+  return j == 0 ? v.x : (j == 1 ? v.y : v.z);
+}
+
+// Face f of the crossing each pass reports, from the side masks m0 (edges
+// with side >= 0) and m1 (side <= 0); -1 = none, and the solid is missed.
+// The pass's own mask A and the other B play the roles of the reference's
+// cw_f[i] / cw_f[n]:
+//   pyramid   Intersect{Pyramid}D  :455-505 (isects[n], i = 1 - n)
+//   prism     IntersectPrismD      :559-611 (isects[i], n = 1 - i)
+//   hexahedron IntersectHexaD      :664-716 (isects[i], n = 1 - i)
+template <int NV>
+__device__ __forceinline__ int sol_face(uint32_t A, uint32_t B) {
+  auto has = [](uint32_t m, uint32_t bits) { return (m & bits) == bits; };
+  if (NV == 5) {
+    const uint32_t Mi = B, Mn = A;
+    if (has(Mi, 0xfu)) return 0;
+    if (has(Mn, 0x21u) && has(Mi, 0x10u)) return 1;
+    if (has(Mn, 0x42u) && has(Mi, 0x20u)) return 2;
+    if (has(Mn, 0x84u) && has(Mi, 0x40u)) return 3;
+    if (has(Mn, 0x18u) && has(Mi, 0x80u)) return 4;
+    return -1;
+  } else if (NV == 6) {
+    const uint32_t Mi = A, Mn = B;
+    if (has(Mn, 0x7u)) return 0;
+    if (has(Mi, 0x38u)) return 1;
+    if (has(Mi, 0x44u) && has(Mn, 0x120u)) return 2;
+    if (has(Mi, 0x81u) && has(Mn, 0x48u)) return 3;
+    if (has(Mi, 0x102u) && has(Mn, 0x90u)) return 4;
+    return -1;
+  } else {
+    const uint32_t Mi = A, Mn = B;
+    if (has(Mn, 0xfu)) return 0;
+    if (has(Mi, 0xf0u)) return 1;
+    if (has(Mi, 0x201u) && has(Mn, 0x110u)) return 2;
+    if (has(Mi, 0x402u) && has(Mn, 0x220u)) return 3;
+    if (has(Mi, 0x804u) && has(Mn, 0x440u)) return 4;
+    if (has(Mi, 0x108u) && has(Mn, 0x880u)) return 5;
+    return -1;
+  }
+}
+
+// How face f's crossing point is formed: 0 = SetCrossPoint_Sq (:438-444,
+// all double, rounded to float at the end) over corners (v0, v1, v2), side
+// products (w0, w1, w2, w3) and edges (e2, e3); 1 = the pyramid's side
+// triangles (v4 * -wa + vb * wb + vc * -wc) / (-wa + wb - wc); 2 = the
+// prism's end triangles (va * w0 + vb * w1 + vc * w2) / (w0 + w1 + w2); for
+// 1 and 2 the double numerator and denominator round to float before the
+// float division (real3::operator/(real)). nrm = the edges whose cross
+// product is the face normal. Negative w entries mean -ws[~w].
+struct SolFace {
+  int kind, v0, v1, v2, w0, w1, w2, w3, e2, e3, na, nb;
+};
+template <int NV>
+__device__ __forceinline__ SolFace sol_face_desc(int f) {
+  if (NV == 5) {
+    switch (f) {
+      case 0: return {0, 0, 1, 2, 0, 1, 2, 3, 2, 3, 1, 0};
+      case 1: return {1, 4, 1, 0, 0, 4, 5, 0, 0, 0, 4, 0};
+      case 2: return {1, 4, 2, 1, 1, 5, 6, 0, 0, 0, 5, 1};
+      case 3: return {1, 4, 3, 2, 2, 6, 7, 0, 0, 0, 6, 2};
+      default: return {1, 4, 0, 3, 3, 7, 4, 0, 0, 0, 7, 3};
+    }
+  } else if (NV == 6) {
+    switch (f) {
+      case 0: return {2, 2, 0, 1, 0, 1, 2, 0, 0, 0, 0, 1};
+      case 1: return {2, 5, 3, 4, 3, 4, 5, 0, 0, 0, 3, 4};
+      case 2: return {0, 2, 0, 3, 2, 6, ~5, ~8, 5, 8, 2, 6};
+      case 3: return {0, 0, 1, 4, 0, 7, ~3, ~6, 3, 6, 0, 7};
+      default: return {0, 1, 2, 5, 1, 8, ~4, ~7, 4, 7, 1, 8};
+    }
+  } else {
+    switch (f) {
+      case 0: return {0, 0, 1, 2, 0, 1, 2, 3, 2, 3, 0, 1};
+      case 1: return {0, 4, 5, 6, 4, 5, 6, 7, 6, 7, 4, 5};
+      case 2: return {0, 0, 1, 5, 0, 9, ~4, ~8, 4, 8, 0, 9};
+      case 3: return {0, 1, 2, 6, 1, 10, ~5, ~9, 5, 9, 1, 10};
+      case 4: return {0, 2, 3, 7, 2, 11, ~6, ~10, 6, 10, 2, 11};
+      default: return {0, 3, 0, 4, 3, 8, ~7, ~11, 7, 11, 3, 8};
+    }
+  }
+}
+
+template <int NV>
+__device__ __forceinline__ D3 sol_edge(const float4 *P, int i) {
+  return d3sub(f4d(P[SolTopo<NV>::b(i)]), f4d(P[SolTopo<NV>::a(i)]));
+}
+__device__ __forceinline__ double sol_w(const double *ws, int w) { return w < 0 ? -ws[~w] : ws[w]; }
+
+// Coordinates of face F's crossing point: component j only (mask 1 << j)
+// during the search, all three for the reported hit.
+template <int NV>
+__device__ __forceinline__ void sol_point_of(const float4 *P, const double *ws, const D3 &rd,
+                                             const SolFace F, int jmask, float out[3]) {
+  const D3 v0 = f4d(P[F.v0]), v1 = f4d(P[F.v1]), v2 = f4d(P[F.v2]);
+  const double w0 = sol_w(ws, F.w0), w1 = sol_w(ws, F.w1), w2 = sol_w(ws, F.w2);
+  if (F.kind == 0) {
+    const double w3 = sol_w(ws, F.w3);
+    const double w = (w2 + w3) + d3dot(d3cross(sol_edge<NV>(P, F.e2), sol_edge<NV>(P, F.e3)), rd);
+    const double den = (w0 + w1) + w;
+    for (int j = 0; j < 3; j++)
+      if (jmask & (1 << j))
+        out[j] = (float)((((d3c(v2, j) * w0) + (d3c(v0, j) * w1)) + (d3c(v1, j) * w)) / den);
+  } else if (F.kind == 1) {
+    // (v4 * -wa + vb * wb + vc * -wc) / (-wa + wb - wc) with v4 = v0 here
+    const float den = (float)(((-w0) + w1) - w2);
+    for (int j = 0; j < 3; j++)
+      if (jmask & (1 << j))
+        out[j] = (float)(((d3c(v0, j) * (-w0)) + (d3c(v1, j) * w1)) + (d3c(v2, j) * (-w2))) / den;
+  } else {
+    const float den = (float)((w0 + w1) + w2);
+    for (int j = 0; j < 3; j++)
+      if (jmask & (1 << j))
+        out[j] = (float)(((d3c(v0, j) * w0) + (d3c(v1, j) * w1)) + (d3c(v2, j) * w2)) / den;
+  }
+}
+
+// Face f's point / normal edges with every index a compile-time constant
+// (one case per face), so the side products and vertices stay in registers.
+template <int NV>
+__device__ __forceinline__ void sol_point(const float4 *P, const double *ws, const D3 &rd, int f,
+                                          int jmask, float out[3]) {
+  switch (f) {
+    case 0: sol_point_of<NV>(P, ws, rd, sol_face_desc<NV>(0), jmask, out); break;
+    case 1: sol_point_of<NV>(P, ws, rd, sol_face_desc<NV>(1), jmask, out); break;
+    case 2: sol_point_of<NV>(P, ws, rd, sol_face_desc<NV>(2), jmask, out); break;
+    case 3: sol_point_of<NV>(P, ws, rd, sol_face_desc<NV>(3), jmask, out); break;
+    case 4: sol_point_of<NV>(P, ws, rd, sol_face_desc<NV>(4), jmask, out); break;
+    default: sol_point_of<NV>(P, ws, rd, sol_face_desc<NV>(5), jmask, out); break;
+  }
+}
+template <int NV>
+__device__ __forceinline__ D3 sol_normal_of(const float4 *P, const SolFace F) {
+  return d3cross(sol_edge<NV>(P, F.na), sol_edge<NV>(P, F.nb));
+}
+template <int NV>
+__device__ __forceinline__ D3 sol_normal(const float4 *P, int f) {
+  switch (f) {
+    case 0: return sol_normal_of<NV>(P, sol_face_desc<NV>(0));
+    case 1: return sol_normal_of<NV>(P, sol_face_desc<NV>(1));
+    case 2: return sol_normal_of<NV>(P, sol_face_desc<NV>(2));
+    case 3: return sol_normal_of<NV>(P, sol_face_desc<NV>(3));
+    case 4: return sol_normal_of<NV>(P, sol_face_desc<NV>(4));
+    default: return sol_normal_of<NV>(P, sol_face_desc<NV>(5));
+  }
+}
+
+// Side products of the ray with every edge: dot(d, e x v_a) + dot(e, d x o),
+// render/render_accel_solid.cc:465-474 (and the prism / hexahedron copies).
+template <int NV>
+__device__ __forceinline__ void sol_sides(const float4 *P, const D3 &rd, const D3 &raypc,
+                                          double *ws, uint32_t &m0, uint32_t &m1) {
+  m0 = m1 = 0;
+#pragma unroll
+  for (int i = 0; i < SolTopo<NV>::kEdges; i++) {
+    const D3 e = sol_edge<NV>(P, i);
+    ws[i] = d3dot(rd, d3cross(e, f4d(P[SolTopo<NV>::a(i)]))) + d3dot(e, raypc);
+    m0 |= (ws[i] >= 0 ? 1u : 0u) << i;
+    m1 |= (ws[i] <= 0 ? 1u : 0u) << i;
+  }
+}
+
+// The float real3 side of interpolate() (mode 0), render/render_accel_solid.cc:
+// 247-435: vertex 0 moved to the origin, then per vertex the ray from it
+// through p and the face it leaves by. The reference never initialises
+// the accumulator of its tolerance kEPS; it starts from 0 here.
+__device__ __forceinline__ F3 sol_cp_sq(F3 v0, F3 v1, F3 v2, double ws0, double ws1, double ws2,
+                                        double ws3, F3 e2, F3 e3, F3 rd) {
+  // getCP_sq, render/render_accel_solid.cc:238-244
+  const double w = (ws2 + ws3) + (double)f3dot(f3cross(e2, e3), rd);
+  const F3 num = f3add(f3add(f3mul(v2, (float)ws0), f3mul(v0, (float)ws1)), f3mul(v1, (float)w));
+  return f3div(num, (float)((ws0 + ws1) + w));
+}
+__device__ __forceinline__ F3 sol_cp_pyr(F3 vx, F3 vy, F3 vz, float wa, float wb, float wc) {
+  return f3div(f3add(f3add(f3mul(vx, -wa), f3mul(vy, wb)), f3mul(vz, -wc)), ((-wa) + wb) - wc);
+}
+__device__ __forceinline__ F3 sol_cp_tri(F3 va, F3 vb, F3 vc, float w0, float w1, float w2) {
+  return f3div(f3add(f3add(f3mul(va, w0), f3mul(vb, w1)), f3mul(vc, w2)), (w0 + w1) + w2);
+}
+__device__ __forceinline__ bool has4(int v, int a, int b, int c, int d) {
+  return v == a || v == b || v == c || v == d;
+}
+
+template <int NV>
+__device__ void sol_interpolate(float d_out[8], const float p[3], const float4 *P) {
+  constexpr int NE = SolTopo<NV>::kEdges;
+  F3 V[NV], E[NE], EPC[NE];
+  for (int i = 0; i < NV; i++) V[i] = {P[i].x, P[i].y, P[i].z};
+  F3 pt = f3sub(F3{p[0], p[1], p[2]}, V[0]);
+  for (int i = NV - 1; i >= 0; i--) V[i] = f3sub(V[i], V[0]);
+  float keps = 0.0f;
+  for (int i = 0; i < NE; i++) {
+    E[i] = f3sub(V[SolTopo<NV>::b(i)], V[SolTopo<NV>::a(i)]);
+    keps += f3len(E[i]);
+  }
+  keps = (float)(((double)keps * 1e-6) / NE);
+  for (int i = 0; i < NE; i++) EPC[i] = f3cross(E[i], V[SolTopo<NV>::a(i)]);
+  for (int v = 0; v < NV; v++) {
+    const F3 ro = V[v], rd = f3sub(pt, ro), rpc = f3cross(rd, ro);
+    const float d = f3len(f3sub(pt, ro));
+    float w[NE];
+    uint32_t c0 = 0, c1 = 0;
+    for (int i = 0; i < NE; i++) {
+      w[i] = f3dot(rd, EPC[i]) + f3dot(E[i], rpc);
+      c0 |= (w[i] >= -keps ? 1u : 0u) << i;
+      c1 |= (w[i] <= keps ? 1u : 0u) << i;
+    }
+    auto all = [](uint32_t m, uint32_t bits) { return (m & bits) == bits; };
+    F3 cp;
+    bool found = true;
+    if (NV == 5) {
+      if (all(c1, 0xfu) && !has4(v, 0, 3, 2, 1))
+        cp = sol_cp_sq(V[0], V[1], V[2], w[0], w[1], w[2], w[3], E[2], E[3], rd);
+      else if (all(c0, 0x21u) && all(c1, 0x10u) && !has4(v, 0, 1, 4, -1))
+        cp = sol_cp_pyr(V[4], V[1], V[0], w[0], w[4], w[5]);
+      else if (all(c0, 0x42u) && all(c1, 0x20u) && !has4(v, 1, 2, 4, -1))
+        cp = sol_cp_pyr(V[4], V[2], V[1], w[1], w[5], w[6]);
+      else if (all(c0, 0x84u) && all(c1, 0x40u) && !has4(v, 2, 3, 4, -1))
+        cp = sol_cp_pyr(V[4], V[3], V[2], w[2], w[6], w[7]);
+      else if (all(c0, 0x18u) && all(c1, 0x80u) && !has4(v, 0, 4, 3, -1))
+        cp = sol_cp_pyr(V[4], V[0], V[3], w[3], w[7], w[4]);
+      else
+        found = false;
+    } else if (NV == 6) {
+      if (all(c0, 0x7u) && !has4(v, 0, 2, 1, -1))
+        cp = sol_cp_tri(V[2], V[0], V[1], w[0], w[1], w[2]);
+      else if (all(c1, 0x38u) && !has4(v, 3, 4, 5, -1))
+        cp = sol_cp_tri(V[5], V[3], V[4], w[3], w[4], w[5]);
+      else if (all(c1, 0x44u) && all(c0, 0x120u) && !has4(v, 0, 3, 5, 2))
+        cp = sol_cp_sq(V[2], V[0], V[3], w[2], w[6], -w[5], -w[8], E[5], E[8], rd);
+      else if (all(c1, 0x81u) && all(c0, 0x48u) && !has4(v, 0, 1, 4, 3))
+        cp = sol_cp_sq(V[0], V[1], V[4], w[0], w[7], -w[3], -w[6], E[3], E[6], rd);
+      else if (all(c1, 0x102u) && all(c0, 0x90u) && !has4(v, 1, 2, 5, 4))
+        cp = sol_cp_sq(V[1], V[2], V[5], w[1], w[8], -w[4], -w[7], E[4], E[7], rd);
+      else
+        found = false;
+    } else {
+      if (all(c0, 0xfu) && !has4(v, 0, 3, 2, 1))
+        cp = sol_cp_sq(V[0], V[1], V[2], w[0], w[1], w[2], w[3], E[2], E[3], rd);
+      else if (all(c1, 0xf0u) && !has4(v, 4, 5, 6, 7))
+        cp = sol_cp_sq(V[4], V[5], V[6], w[4], w[5], w[6], w[7], E[6], E[7], rd);
+      else if (all(c1, 0x201u) && all(c0, 0x110u) && !has4(v, 0, 1, 5, 4))
+        cp = sol_cp_sq(V[0], V[1], V[5], w[0], w[9], -w[4], -w[8], E[4], E[8], rd);
+      else if (all(c1, 0x402u) && all(c0, 0x220u) && !has4(v, 1, 2, 6, 5))
+        cp = sol_cp_sq(V[1], V[2], V[6], w[1], w[10], -w[5], -w[9], E[5], E[9], rd);
+      else if (all(c1, 0x804u) && all(c0, 0x440u) && !has4(v, 2, 3, 7, 6))
+        cp = sol_cp_sq(V[2], V[3], V[7], w[2], w[11], -w[6], -w[10], E[6], E[10], rd);
+      else if (all(c1, 0x108u) && all(c0, 0x880u) && !has4(v, 0, 4, 7, 3))
+        cp = sol_cp_sq(V[3], V[0], V[4], w[3], w[8], -w[7], -w[11], E[7], E[11], rd);
+      else
+        found = false;
+    }
+    if (!found) {
+      d_out[v] = 0.0f;
+      continue;
+    }
+    float dv = d / f3len(f3sub(cp, ro));
+    if (dv < 0.0f)
+      dv = 0.0f;
+    else if (!(dv < 1.0f) && !(dv > 0.0f))
+      dv = 1.0f;
+    d_out[v] = 1.0f - dv;
+  }
+  float ds = 0.0f, wsum = 0.0f, dd[NV];
+  for (int i = 0; i < NV; i++) ds += f3len(f3sub(pt, V[i]));
+  for (int i = 0; i < NV; i++) dd[i] = ds / f3len(f3sub(pt, V[i]));
+  for (int i = 0; i < NV; i++) {
+    d_out[i] *= dd[i];
+    wsum += d_out[i];
+  }
+  for (int i = 0; i < NV; i++) d_out[i] = d_out[i] / wsum;
+  for (int i = NV; i < 8; i++) d_out[i] = 0.0f;
+}
+
+template <int NV, bool kAny>
+__global__ void __launch_bounds__(kTraceBlock)
+k_trace_sol(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
+            const BuildParams *__restrict__ params, const srf_ray *__restrict__ rays, uint32_t n,
+            const float *__restrict__ scalars, srf_hit *__restrict__ hits,
+            srf_solid_hit_ext *__restrict__ ext, const float *__restrict__ tmax_in,
+            uint32_t *__restrict__ occluded, uint32_t max_nodes, uint32_t nprims) {
+  constexpr int NE = SolTopo<NV>::kEdges;
+  __shared__ uint32_t st_node[kTraceStack * kTraceBlock];
+  const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t flags, prev;
+  const RaySetup s = setup_ray(rays, i, flags, prev);
+  // t_max for occlusion: the closest crossing is < t_max iff some crossing is
+  const float tlimit = kAny ? fminf(tmax_in ? tmax_in[i] : FLT_MAX, FLT_MAX) : FLT_MAX;
+  float t = FLT_MAX;
+  uint32_t best = SRF_NO_PRIM, side = 0;
+  const D3 ro = {s.o[0], s.o[1], s.o[2]}, rd = {s.d[0], s.d[1], s.d[2]};
+  const D3 raypc = d3cross(rd, ro);
+  // t comes from the crossing point along the last axis with |dir| > 0.1
+  // (:497-501); with none the reference's t is undefined and no solid is hit
+  int axis = -1;
+  for (int j = 0; j < 3; j++)
+    if ((double)s.d[j] > 0.1 || (double)s.d[j] < -0.1) axis = j;
+  const BuildParams &P = *params;
+  uint32_t *sn = st_node + threadIdx.x;
+  int sp = 0;
+  uint32_t cur = kDone;
+  {
+    // node 0's own box is tested first (SolidAccel::Traverse pops node 0)
+    float tm;
+    if (box_span(s, P.root_bmin[0], P.root_bmin[1], P.root_bmin[2], P.root_bmax[0],
+                 P.root_bmax[1], P.root_bmax[2], tm) &&
+        tm <= FLT_MAX && axis >= 0)
+      cur = 0;
+  }
+  const double oj = axis == 0 ? ro.x : (axis == 1 ? ro.y : ro.z);
+  const double dj = axis == 0 ? rd.x : (axis == 1 ? rd.y : rd.z);
+  uint4 meta;
+  while (cur != kDone) {
+    for (;;) {
+      if (cur >= max_nodes) {
+        cur = kDone;
+        break;
+      }
+      const float4 *nd = nodes + 4 * (size_t)cur;
+      const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+      meta = reinterpret_cast<const uint4 *>(nd)[3];
+      keep_loaded(n0, n1, n2);
+      if (meta.x != 0) break;
+      float tm0, tm1;
+      const bool g0 = box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm0) && tm0 <= FLT_MAX;
+      const bool g1 = box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm1) && tm1 <= FLT_MAX;
+      const int nearc = (s.sbits >> meta.y) & 1u;
+      const bool gn = nearc ? g1 : g0, gf = nearc ? g0 : g1;
+      const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+      if (gn) {
+        if (gf && sp < kTraceStack) sn[(sp++) * kTraceBlock] = cf;
+        cur = cn;
+        continue;
+      }
+      if (gf) {
+        cur = cf;
+        continue;
+      }
+      cur = sp > 0 ? sn[(--sp) * kTraceBlock] : kDone;
+      if (cur == kDone) break;
+    }
+    if (cur == kDone) break;
+    // leaf: TestLeafNode, render/render_accel_solid.cc:2206-2333
+    const uint32_t count = meta.z, first = meta.w;
+    for (uint32_t q = first; q < min(first + count, nprims); q++) {
+      float4 V[NV];
+      const float4 *Pq = prims + (size_t)NV * q;
+#pragma unroll
+      for (int k = 0; k < NV; k++) V[k] = Pq[k];
+      const uint32_t id = __float_as_uint(V[0].w);
+      if (id == prev) continue;
+      double ws[NE];
+      uint32_t m0, m1;
+      sol_sides<NV>(V, rd, raypc, ws, m0, m1);
+      const int f0 = sol_face<NV>(m0, m1);
+      if (f0 < 0) continue;
+      const int f1 = sol_face<NV>(m1, m0);
+      if (f1 < 0) continue;
+      float c0[3], c1[3];
+      sol_point<NV>(V, ws, rd, f0, 1 << axis, c0);
+      sol_point<NV>(V, ws, rd, f1, 1 << axis, c1);
+      const float tc[2] = {(float)(((double)c0[axis] - oj) / dj),
+                           (float)(((double)c1[axis] - oj) / dj)};
+      for (int k = 0; k < 2; k++) {
+        if ((tc[k] >= 0.0f) && (tc[k] < t)) {
+          if (kAny) {
+            if (tc[k] < tlimit) {
+              occluded[i] = 1u;
+              return;
+            }
+          }
+          t = tc[k];
+          best = q;
+          side = (uint32_t)k;
+        }
+      }
+    }
+    cur = sp > 0 ? sn[(--sp) * kTraceBlock] : kDone;
+  }
+  if (kAny) {
+    occluded[i] = 0u;
+    return;
+  }
+  uint32_t prim = SRF_NO_PRIM;
+  if (best != SRF_NO_PRIM) {
+    float4 V[NV];
+    const float4 *Pq = prims + (size_t)NV * best;
+    for (int k = 0; k < NV; k++) V[k] = Pq[k];
+    prim = __float_as_uint(V[0].w);
+    if (ext) {
+      // the reported crossing again, in full: point, normal, weights
+      double ws[NE];
+      uint32_t m0, m1;
+      sol_sides<NV>(V, rd, raypc, ws, m0, m1);
+      const int f = side ? sol_face<NV>(m1, m0) : sol_face<NV>(m0, m1);
+      srf_solid_hit_ext e = {};
+      if (f >= 0) {
+        sol_point<NV>(V, ws, rd, f, 7, e.position);
+        // toreal3(normalize(vcrossd(ea, eb))), render/render_accel_solid.cc:69-85
+        const D3 c = sol_normal<NV>(V, f);
+        const double len = sqrt((c.x * c.x + c.y * c.y) + c.z * c.z);
+        const float sg = side ? -1.0f : 1.0f;
+        e.normal[0] = sg * (float)(c.x / len);
+        e.normal[1] = sg * (float)(c.y / len);
+        e.normal[2] = sg * (float)(c.z / len);
+        sol_interpolate<NV>(e.d, e.position, V);
+        e.side = side;
+        if (scalars) {
+          // f_k = nv * prim + k (:2284-2291), summed left to right
+          // (gles/gles_common.cc:1078-1140)
+          const float *sc = scalars + (size_t)NV * prim;
+          float acc = sc[0] * e.d[0];
+          for (int k = 1; k < NV; k++) acc = acc + sc[k] * e.d[k];
+          e.scalar = acc;
+        }
+      }
+      float4 *e4 = reinterpret_cast<float4 *>(ext + i);
+      e4[0] = make_float4(e.d[0], e.d[1], e.d[2], e.d[3]);
+      e4[1] = make_float4(e.d[4], e.d[5], e.d[6], e.d[7]);
+      e4[2] = make_float4(e.position[0], e.position[1], e.position[2], e.scalar);
+      e4[3] = make_float4(e.normal[0], e.normal[1], e.normal[2], __uint_as_float(e.side));
+    }
+  } else if (ext) {
+    float4 *e4 = reinterpret_cast<float4 *>(ext + i);
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    e4[0] = z;
+    e4[1] = z;
+    e4[2] = z;
+    e4[3] = z;
+  }
+  reinterpret_cast<float4 *>(hits)[i] = make_float4(t, 0.0f, 0.0f, __uint_as_float(prim));
+}
+
 // ------------------------------------------------- sorted / persistent any-hit
 
 // Occlusion for rays taken in `order` (order[j] is the j-th ray to trace,
@@ -1073,6 +1547,33 @@ srf_status trace_tet(const Accel *a, const srf_ray *rays, uint64_t n, const floa
   return SRF_OK;
 }
 
+template <bool kAny>
+srf_status launch_sol(const Accel *a, const srf_ray *rays, uint64_t n, const float *scalars,
+                      srf_hit *hits, srf_solid_hit_ext *ext, const float *tmax,
+                      uint32_t *occluded, hipStream_t s) {
+  const dim3 g(trace_blocks(n)), b(kTraceBlock);
+  const float4 *nd = (const float4 *)a->nodes, *pr = (const float4 *)a->prims;
+  if (a->nvps == 5)
+    hipLaunchKernelGGL((k_trace_sol<5, kAny>), g, b, 0, s, nd, pr, a->params, rays, (uint32_t)n,
+                       scalars, hits, ext, tmax, occluded, max_nodes(a), (uint32_t)a->n);
+  else if (a->nvps == 6)
+    hipLaunchKernelGGL((k_trace_sol<6, kAny>), g, b, 0, s, nd, pr, a->params, rays, (uint32_t)n,
+                       scalars, hits, ext, tmax, occluded, max_nodes(a), (uint32_t)a->n);
+  else if (a->nvps == 8)
+    hipLaunchKernelGGL((k_trace_sol<8, kAny>), g, b, 0, s, nd, pr, a->params, rays, (uint32_t)n,
+                       scalars, hits, ext, tmax, occluded, max_nodes(a), (uint32_t)a->n);
+  else
+    return set_error(SRF_ERR_INTERNAL, "solid accelerator with %u vertices per solid", a->nvps);
+  SRF_HIP(hipGetLastError());
+  return SRF_OK;
+}
+
+srf_status trace_sol(const Accel *a, const srf_ray *rays, uint64_t n, const float *scalars,
+                     srf_hit *hits, srf_solid_hit_ext *ext, hipStream_t s) {
+  if (n == 0) return SRF_OK;
+  return launch_sol<false>(a, rays, n, scalars, hits, ext, nullptr, nullptr, s);
+}
+
 srf_status trace_sph(const Accel *a, const srf_ray *rays, const srf_sphere_prev *prev,
                      uint64_t n, srf_hit *hits, hipStream_t s) {
   if (n == 0) return SRF_OK;
@@ -1087,6 +1588,7 @@ srf_status trace_sph(const Accel *a, const srf_ray *rays, const srf_sphere_prev 
 srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax, uint64_t n,
                      uint32_t *occluded, hipStream_t s) {
   if (n == 0) return SRF_OK;
+  if (a->kind == kSol) return launch_sol<true>(a, rays, n, nullptr, nullptr, nullptr, tmax, occluded, s);
   const dim3 g(trace_blocks(n)), b(kTraceBlock);
   if (a->kind == kTri && small_nodes(a))
     hipLaunchKernelGGL((k_trace_tri<true, true>), g, b, 0, s, (const float4 *)a->nodes,
@@ -1113,7 +1615,7 @@ srf_status trace_any_sorted(Ctx *ctx, const Accel *a, const srf_ray *rays, const
                             uint64_t n, uint32_t key_bits, int mode, uint32_t *occluded,
                             hipStream_t s) {
   if (n == 0) return SRF_OK;
-  if (a->kind == kTet)
+  if (a->kind == kTet || a->kind == kSol)
     return set_error(SRF_ERR_UNSUPPORTED, "sorted any-hit takes triangles or spheres");
   if (key_bits > 30) return set_error(SRF_ERR_INVALID_ARG, "key_bits must be <= 30");
   const uint32_t nn = (uint32_t)n;

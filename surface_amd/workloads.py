@@ -32,6 +32,10 @@ HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"),
 TET_EXT_DTYPE = np.dtype([("d", "<f4", (4,)), ("scalar", "<f4"),
                           ("subface", "<u4"), ("t_leave", "<f4"),
                           ("pad", "<f4")])
+# srf_solid_hit_ext: interpolate() weights, crossing position and normal,
+# the varying sum, and which crossing (0 entry, 1 exit) was reported
+SOL_EXT_DTYPE = np.dtype([("d", "<f4", (8,)), ("pos", "<f4", (3,)), ("scalar", "<f4"),
+                          ("nrm", "<f4", (3,)), ("side", "<u4")])
 NO_PRIM = np.uint32(0xFFFFFFFF)
 
 
@@ -105,6 +109,72 @@ def tet_grid(cells=64, seed=BASE_SEED + 2, jitter=0.2, n_gauss=16):
     return (np.ascontiguousarray(P.astype(np.float32)),
             np.ascontiguousarray(F.astype(np.uint32)),
             np.ascontiguousarray(s.astype(np.float32)))
+
+
+# hexahedron faces (outward, counter-clockwise seen from outside), the
+# order of kHexaFaces in render/render_accel_solid.cc:99
+_HEX_FACES = np.array([[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4], [1, 2, 6, 5],
+                       [2, 3, 7, 6], [0, 4, 7, 3]])
+
+
+def solid_grid(cells=32, nv=8, seed=BASE_SEED + 8, jitter=0.2, n_gauss=8):
+    """Jittered (cells+1)^3 lattice on [0, cells]^3 split into solids.
+
+    nv = 8: one hexahedron per cell (vertices 0-3 the lower face
+    counter-clockwise seen from above, 4-7 above them); nv = 6: two prisms
+    per cell (lower triangle 0-2, upper 3-5); nv = 5: six pyramids per cell,
+    one per face with the cell's centroid as apex (a vertex of its own after
+    the lattice). Returns V (float32), F (uint32, nv per solid) and one
+    float32 scalar per index position (the reference's solid varyings are
+    looked up at nv * prim + k).
+    """
+    rng = _rng(seed + nv)
+    n1 = cells + 1
+    g = np.arange(n1, dtype=np.float64)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([x, y, z], axis=-1).reshape(-1, 3)
+    P = P + rng.uniform(-jitter, jitter, P.shape)
+    c = np.arange(cells)
+    cz, cy, cx = np.meshgrid(c, c, c, indexing="ij")
+    base = (cz * n1 * n1 + cy * n1 + cx).ravel()
+    # corners in hexahedron order: (0,0,0) (1,0,0) (1,1,0) (0,1,0), then z+1
+    off = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    corner = np.array([dx + dy * n1 + dz * n1 * n1 for dx, dy, dz in off], dtype=np.int64)
+    H = base[:, None] + corner[None, :]
+    if nv == 8:
+        F = H
+    elif nv == 6:
+        F = np.concatenate([H[:, [0, 1, 2, 4, 5, 6]], H[:, [0, 2, 3, 4, 6, 7]]], axis=1)
+        F = F.reshape(-1, 6)
+    elif nv == 5:
+        centre = P[H].mean(axis=1)
+        cid = len(P) + np.arange(len(H))
+        P = np.concatenate([P, centre])
+        # base (a, d, c, b) for the outward face (a, b, c, d): counter-clockwise
+        # seen from the apex, the base order of kPyramidFaces[0] = {0,3,2,1}
+        fa = H[:, _HEX_FACES]
+        F = np.concatenate([fa[:, :, [0, 3, 2, 1]], np.repeat(cid[:, None, None], 6, 1)], axis=2)
+        F = F.reshape(-1, 5)
+    else:
+        raise ValueError("nv must be 5, 6 or 8")
+    centres = rng.uniform(0.0, cells, (n_gauss, 3))
+    sig = rng.uniform(0.1, 0.3, (n_gauss, 3)) * cells
+    w = rng.uniform(0.2, 1.0, n_gauss)
+    q = P[F.ravel()]
+    s = np.zeros(len(q))
+    for k in range(n_gauss):
+        d = (q - centres[k]) / sig[k]
+        s += w[k] * np.exp(-0.5 * np.einsum("ij,ij->i", d, d))
+    return (np.ascontiguousarray(P.astype(np.float32)),
+            np.ascontiguousarray(F.astype(np.uint32)),
+            np.ascontiguousarray(s.astype(np.float32)))
+
+
+def solid_rays(cells=32, width=256, height=256, tiled=True):
+    """Pinhole rays at a solid grid from outside one corner."""
+    c = float(cells)
+    return pinhole_rays((1.6 * c, -0.7 * c, 1.3 * c), (0.5 * c, 0.5 * c, 0.45 * c), (0, 0, 1),
+                        45.0, width, height, tiled=tiled)
 
 
 def plummer_spheres(n, seed=BASE_SEED + 5, a=1.0, mu=-6.0, sigma=0.5):

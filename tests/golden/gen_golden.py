@@ -7,7 +7,7 @@ heightfield (522,242 triangles, 1024^2 rays) and tetra grid (1,572,864 tets,
 1024^2 rays) it stores SHA-256 digests of the tree, the permutation and the
 per-ray hits. Run in the container that has the reference checkout:
 
-    make -C oracle && python tests/golden/gen_golden.py [--full]
+    make -C oracle && python tests/golden/gen_golden.py [--full] [--only scene|sol]
 """
 from __future__ import annotations
 
@@ -98,6 +98,43 @@ def karras_dup_case(ref):
             "codes_same": same, "info_same": ref.radix_tree(same)}
 
 
+# solid grids of 1331 hexahedra, 1458 prisms and 1296 pyramids (Build32
+# needs 1024 or more), seen from outside and from rays starting inside
+SOL_CELLS = {8: 11, 6: 9, 5: 6}
+
+
+def solid_rays_mixed(cells, res, n_inside, seed):
+    rays = W.solid_rays(cells, res, res)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r2 = np.zeros(n_inside, W.RAY_DTYPE)
+    r2["org"] = rng.uniform(0.3, cells - 0.3, (n_inside, 3)).astype(np.float32)
+    d = rng.normal(size=(n_inside, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r2["dir"] = d.astype(np.float32)
+    r2["flags"] = 1
+    r2["prev_prim_id"] = W.NO_PRIM
+    return np.concatenate([rays, r2])
+
+
+def sol_case(ref):
+    """SolidAccel::Build32 + Traverse for each solid kind. A solid hit's u/v
+    are uninitialised in the reference and are not stored."""
+    out = {}
+    for nv, cells in SOL_CELLS.items():
+        V, F, S = W.solid_grid(cells, nv)
+        tree = ref.build("sol", V, F, keep=True, nv=nv)
+        rays = solid_rays_mixed(cells, 32, 500, W.BASE_SEED + 30 + nv)
+        hits, ext = ref.trace(tree, rays, scalars=S)
+        ref.free(tree)
+        p = f"n{nv}_"
+        out.update({p + "V": V, p + "F": F, p + "S": S, p + "nodes": canonical_tet(tree["nodes"]),
+                    p + "indices": tree["indices"], p + "bounds": tree["bounds"],
+                    p + "rays": rays, p + "t": hits["t"], p + "prim_id": hits["prim_id"],
+                    p + "d": ext["d"], p + "pos": ext["pos"], p + "nrm": ext["nrm"],
+                    p + "scalar": ext["scalar"]})
+    return out
+
+
 SCENE_ARGS = dict(count=12, lo=1100, hi=6000, seed=W.BASE_SEED + 21)
 
 
@@ -122,12 +159,15 @@ def scene_case(ref):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--full", action="store_true", help="also write full-size digests")
-    ap.add_argument("--only", choices=["scene"], help="write one fixture only")
+    ap.add_argument("--only", choices=["scene", "sol"], help="write one fixture only")
     args = ap.parse_args()
     ref = Reference("serial")
     t0 = time.time()
     if args.only == "scene":
         np.savez_compressed(os.path.join(HERE, "scene_small.npz"), **scene_case(ref))
+        return
+    if args.only == "sol":
+        np.savez_compressed(os.path.join(HERE, "sol_small.npz"), **sol_case(ref))
         return
     np.savez_compressed(os.path.join(HERE, "tri_small.npz"), **tri_case(ref, 48, 64))
     np.savez_compressed(os.path.join(HERE, "tet_small.npz"), **tet_case(ref, 6, 64))
@@ -135,6 +175,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "tex3d.npz"), **tex_case(ref))
     np.savez_compressed(os.path.join(HERE, "karras.npz"), **karras_dup_case(ref))
     np.savez_compressed(os.path.join(HERE, "scene_small.npz"), **scene_case(ref))
+    np.savez_compressed(os.path.join(HERE, "sol_small.npz"), **sol_case(ref))
     print(f"small fixtures written in {time.time() - t0:.1f}s", flush=True)
     if not args.full:
         return

@@ -1,8 +1,9 @@
 """srf_accel_dump against the reference's own Dump files.
 
-{Triangle,Tetra,Particle}Accel::Dump (render/render_accel_triangle.cc:
+{Triangle,Tetra,Particle,Solid}Accel::Dump (render/render_accel_triangle.cc:
 1692-1719, render/render_accel_tetra.cc:1900-1928,
-render/render_accel_particle.cc:1868-1896) write an unsigned long long node
+render/render_accel_particle.cc:1868-1896, render/render_accel_solid.cc:
+2126-2154) write an unsigned long long node
 count, the nodes, an unsigned long long index count and the indices. The
 reference build (oracle/_ref, compiled from the reference's sources) dumps its
 tree; the GPU build dumps the same meshes; the files must carry the same
@@ -49,10 +50,11 @@ CASES = {
     "tet": (lambda: W.tet_grid(8)[:2], TET_NODE, canonical_tet),
     "sph": (lambda: W.clustered_spheres(n_bg=4000, clusters=3, per_cluster=500)[:2], SPH_NODE,
             canonical_sph),
+    "sol": (lambda: W.solid_grid(12, 6)[:2], TET_NODE, canonical_tet),
 }
 
 
-@pytest.mark.parametrize("kind", ["tri", "tet", "sph"])
+@pytest.mark.parametrize("kind", ["tri", "tet", "sph", "sol"])
 def test_dump_matches_reference_dump(S, kind, tmp_path):
     if not ref_available("serial"):
         pytest.skip("oracle/_ref not built (the reference checkout was absent at build time)")
@@ -67,6 +69,10 @@ def test_dump_matches_reference_dump(S, kind, tmp_path):
         a.Build32(S.device_tensor(A), S.device_tensor(B), 0.0)
         counts = want["nodes"]["data"][want["nodes"]["flag"] == 1][:, 0]
         assert counts.max() > 64  # secondary trees exist
+    elif kind == "sol":
+        want = ref.build("sol", A, B, nv=6)
+        a = S.SolidAccel()
+        a.Build32(S.device_tensor(A), S.device_tensor(B), 6)
     else:
         want = ref.build(kind, A, B)
         a = S.TriangleAccel() if kind == "tri" else S.TetraAccel()

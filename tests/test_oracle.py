@@ -656,3 +656,138 @@ def test_scene_oracle_equals_live_reference(orc):
     ref.scene_free(sc)
     for t in rtrees:
         ref.free(t)
+
+
+# ------------------------------------------------------------------ solids
+
+SOL_NV = (8, 6, 5)
+
+
+def _eq_nan(a, b):
+    return np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("nv", SOL_NV)
+def test_solid_build_and_trace_match_reference_fixture(orc, nv):
+    g = gold("sol_small.npz")
+    p = f"n{nv}_"
+    V, F, S = g[p + "V"], g[p + "F"], g[p + "S"]
+    t = orc.build_sol(V, F, nv)
+    assert canonical_tet(t["nodes"]).tobytes() == g[p + "nodes"].tobytes()
+    assert np.array_equal(t["indices"], g[p + "indices"])
+    assert np.array_equal(t["bounds"], g[p + "bounds"])
+    _tree_invariants(t["nodes"], t["indices"], len(F), 4, False)
+    hits, ext = orc.trace_sol(t, V, F, g[p + "rays"], S)
+    assert np.array_equal(hits["prim_id"], g[p + "prim_id"])
+    assert np.array_equal(hits["t"], g[p + "t"])
+    assert (hits["prim_id"] != W.NO_PRIM).mean() > 0.5
+    assert _eq_nan(ext["d"], g[p + "d"]) and _eq_nan(ext["scalar"], g[p + "scalar"])
+    assert np.array_equal(ext["pos"], g[p + "pos"]) and np.array_equal(ext["nrm"], g[p + "nrm"])
+
+
+@needs_ref
+@pytest.mark.parametrize("nv", SOL_NV)
+def test_solid_oracle_equals_live_reference(orc, nv):
+    # fresh jitter, self-skipping rays, axis-parallel directions, inside origins
+    ref = Reference("serial")
+    V, F, S = W.solid_grid({8: 12, 6: 10, 5: 7}[nv], nv, seed=W.BASE_SEED + 77, jitter=0.3)
+    t = orc.build_sol(V, F, nv)
+    r = ref.build("sol", V, F, keep=True, nv=nv)
+    assert canonical_tet(t["nodes"]).tobytes() == canonical_tet(r["nodes"]).tobytes()
+    assert np.array_equal(t["indices"], r["indices"])
+    cells = {8: 12, 6: 10, 5: 7}[nv]
+    rays = W.solid_rays(cells, 40, 40)
+    rng = np.random.default_rng(nv)
+    rays["prev_prim_id"][::7] = rng.integers(0, len(F), len(rays[::7]))
+    inner = rays[:800].copy()
+    inner["org"] = rng.uniform(0.2, cells - 0.2, (800, 3)).astype(np.float32)
+    d = rng.normal(size=(800, 3))
+    d[::4, 1] = 0.0
+    d[1::4, 0] = 0.0
+    d[2::8, 2] = 0.0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    inner["dir"] = d.astype(np.float32)
+    rays = np.concatenate([rays, inner])
+    h1, e1 = orc.trace_sol(t, V, F, rays, S, threads=4)
+    h2, e2 = ref.trace(r, rays, scalars=S)
+    ref.free(r)
+    assert np.array_equal(h1["prim_id"], h2["prim_id"]) and np.array_equal(h1["t"], h2["t"])
+    hit = h2["prim_id"] != W.NO_PRIM
+    assert hit.mean() > 0.5
+    assert _eq_nan(e1["d"][hit], e2["d"][hit]) and _eq_nan(e1["scalar"][hit], e2["scalar"][hit])
+    assert np.array_equal(e1["pos"], e2["pos"]) and np.array_equal(e1["nrm"], e2["nrm"])
+
+
+def _unit_solids():
+    c = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1],
+                  [0, 1, 1]], np.float64)
+    return {8: c, 6: c[[0, 1, 2, 4, 5, 6]],
+            5: np.concatenate([c[[0, 1, 2, 3]], [[0.5, 0.5, 1.0]]])}
+
+
+@pytest.mark.parametrize("nv", SOL_NV)
+def test_solid_crossings_analytic(orc, nv):
+    # a vertical ray through each unit solid crosses its base at z = 0 and
+    # its top (face or apex side) at a height the geometry fixes
+    verts = _unit_solids()[nv]
+    x, y = (0.5, 0.5) if nv != 6 else (0.6, 0.3)
+    st, t, pos, nrm = orc.solid_isect(nv, [x, y, -1.0], [0.0, 0.0, 1.0], verts)
+    assert st == 1
+    top = 1.0 if nv != 5 else 1.0 - 2 * max(abs(x - 0.5), abs(y - 0.5))
+    assert sorted(t.tolist()) == [1.0, 1.0 + top]
+    for k in range(2):
+        assert np.array_equal(pos[k][:2], np.float32([x, y]))
+        assert np.abs(np.linalg.norm(nrm[k]) - 1) < 1e-6
+    # the base point's normal is vertical, the entry before the exit
+    base = int(np.argmin(t))
+    assert pos[base][2] == 0.0 and abs(abs(nrm[base][2]) - 1) < 1e-6
+    # a parallel ray beside the solid misses it; one starting inside crosses
+    # its base behind the origin
+    assert orc.solid_isect(nv, [2.5, 0.5, -1.0], [0.0, 0.0, 1.0], verts)[0] == 0
+    st, t, _, _ = orc.solid_isect(nv, [x, y, 0.25], [0.0, 0.0, 1.0], verts)
+    assert st == 1 and min(t) == np.float32(-0.25)
+    # no axis with |dir| > 0.1: the reference leaves t unset
+    assert orc.solid_isect(nv, [x, y, -1.0], [0.0, 0.0, 0.05], verts)[0] == -1
+
+
+def test_solid_weights_analytic(orc):
+    # at the centre of the unit cube's base the four base corners share the
+    # weight and the top corners, whose rays through the point end on it, get 0
+    c = _unit_solids()[8]
+    d = orc.interpolate_solid(8, np.float32([0.5, 0.5, 0.0]), c)
+    assert np.allclose(d[:4], 0.25, atol=1e-6) and np.all(d[4:] == 0)
+    # at a corner of the cube that corner takes all the weight
+    d = orc.interpolate_solid(8, np.float32([1.0, 1.0, 1.0]) - np.float32(1e-6), c)
+    assert d[6] > 0.99 and abs(d.sum() - 1) < 1e-5
+    # the prism's top-face centroid (2/3, 1/3, 1): the bottom corners get 0,
+    # the corners (0,0,1) and (1,1,1), mirror images across the centroid's
+    # line x + y = 1, tie, and the nearer right-angle corner weighs more
+    p = _unit_solids()[6]
+    d = orc.interpolate_solid(6, np.float32([2 / 3, 1 / 3, 1.0]), p)
+    assert np.allclose(d[:3], 0, atol=1e-6) and abs(d[3:6].sum() - 1) < 1e-5
+    assert abs(d[3] - d[5]) < 1e-6 and d[4] > d[3] + 0.1
+
+
+@pytest.mark.parametrize("nv", SOL_NV)
+def test_solid_trace_equals_bruteforce(orc, nv):
+    # the traversal reports the smallest non-negative crossing over all solids
+    V, F, _ = W.solid_grid({8: 11, 6: 9, 5: 6}[nv], nv, seed=W.BASE_SEED + 91)
+    t = orc.build_sol(V, F, nv)
+    rays = W.solid_rays({8: 11, 6: 9, 5: 6}[nv], 12, 12)
+    hits, _ = orc.trace_sol(t, V, F, rays)
+    Fr = F.reshape(-1, nv)
+    for i in range(0, len(rays), 3):
+        o = rays["org"][i].astype(np.float64)
+        d = rays["dir"][i].astype(np.float64)
+        best = np.float32(np.finfo(np.float32).max)
+        for q in range(len(Fr)):
+            st, ts, _, _ = orc.solid_isect(nv, o, d, V[Fr[q]].astype(np.float64))
+            if st == 1:
+                for tk in ts:
+                    if 0 <= tk < best:
+                        best = tk
+        assert hits["t"][i] == best
+        if best < np.finfo(np.float32).max:
+            q = hits["prim_id"][i]
+            st, ts, _, _ = orc.solid_isect(nv, o, d, V[Fr[q]].astype(np.float64))
+            assert st == 1 and best in ts
