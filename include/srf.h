@@ -51,6 +51,7 @@ typedef enum srf_status {
  * include/GLES2/gl2.h:63). */
 typedef enum srf_prim_kind {
   SRF_PRIM_POINTS = 0x0000,       /* GL_POINTS -> ParticleAccel  */
+  SRF_PRIM_LINES = 0x0001,        /* GL_LINES -> LineAccel (gles/gles_accel_builder.cc:686-760) */
   SRF_PRIM_TRIANGLES = 0x0004,    /* GL_TRIANGLES -> TriangleAccel */
   SRF_PRIM_TETRAHEDRONS = 0x0010, /* GL_TETRAHEDRONS_EXT -> TetraAccel */
   /* GL_PYRAMIDS_EXT / GL_PRISMS_EXT / GL_HEXAHEDRONS_EXT -> SolidAccel
@@ -113,6 +114,18 @@ typedef struct srf_solid_hit_ext {
   uint32_t side;
 } srf_solid_hit_ext;
 
+/* Line extras (LineAccel's BuildIntersection, render/render_accel_line.cc:
+ * 654-702): Intersection::position = org + t dir, ::normal (the cap's axis
+ * direction, or from the axis point at v to the hit point), and whether an
+ * end cap was hit. hit.u / hit.v are CylinderIsect's (cap: distance from
+ * the axis and 0 / 1; side: 0 and the axial fraction). 32 bytes. */
+typedef struct srf_line_hit_ext {
+  float position[3];
+  uint32_t hit_cap;
+  float normal[3];
+  float pad;
+} srf_line_hit_ext;
+
 /* Sphere self-intersection inputs, Ray::prev_hit_t / prev_hit_normal
  * (render/render_ray.h:137-139), used when prev_prim_id names a sphere. */
 typedef struct srf_sphere_prev {
@@ -149,6 +162,20 @@ typedef struct srf_solid_desc {
   uint64_t num_solids;
   uint32_t vertices_per_solid;
 } srf_solid_desc;
+
+/* Lines (render/render_prim_line.h:30-38), FP32 positions: segments hold two
+ * vertex ids per line, radius one value per vertex (NULL: constant_radius);
+ * cap = LineBuildOptions::cap (render/render_accel_line.h:52-60, default on:
+ * the end discs are part of the primitive). */
+typedef struct srf_line_desc {
+  const float *positions;
+  uint64_t num_vertices;
+  const uint32_t *segments;
+  uint64_t num_lines;
+  const float *radius;
+  float constant_radius;
+  uint32_t cap;
+} srf_line_desc;
 
 typedef struct srf_sphere_desc {
   const float *positions;
@@ -226,6 +253,16 @@ srf_status srf_accel_build_tetras(srf_ctx *ctx, const srf_tet_desc *desc,
  * the LBVH Build32 path, whose tree srf_accel_export reproduces. */
 srf_status srf_accel_build_solids(srf_ctx *ctx, const srf_solid_desc *desc,
                                   void *stream, srf_accel **out);
+/* Lines on the same LBVH as the other kinds (Morton code of the segment
+ * midpoint, leaves of up to 5 lines). The reference has only a top-down
+ * spatial-median LineAccel::Build (render/render_accel_line.h:79,
+ * .cc:167-341), from AccelBuilder::AddLineData
+ * (gles/gles_accel_builder.cc:686-760); its tree is not reproduced. Traversal
+ * finds the closest crossing over all lines, which is what LineAccel::Traverse
+ * returns whenever its boxes hold the cylinders it tests (always for a
+ * constant radius). */
+srf_status srf_accel_build_lines(srf_ctx *ctx, const srf_line_desc *desc, void *stream,
+                                 srf_accel **out);
 /* ParticleAccel::Build32 (render/render_accel_particle.h:77,
  * .cc:1560-1866), from AccelBuilder::AddParticleData
  * (gles/gles_accel_builder.cc:673). */
@@ -298,6 +335,10 @@ srf_status srf_trace_tetra(const srf_accel *accel, const srf_ray *rays,
 srf_status srf_trace_solids(const srf_accel *accel, const srf_ray *rays,
                             uint64_t num_rays, const float *index_scalars,
                             srf_hit *hits, srf_solid_hit_ext *ext, void *stream);
+/* LineAccel::Traverse (render/render_accel_line.cc:704-774) with
+ * CylinderIsect (:488-577) and BuildIntersection; ext may be NULL. */
+srf_status srf_trace_lines(const srf_accel *accel, const srf_ray *rays, uint64_t num_rays,
+                           srf_hit *hits, srf_line_hit_ext *ext, void *stream);
 /* ParticleAccel::Traverse with the self-intersection inputs of each ray
  * (render/render_accel_particle.cc:2423-2512); prev may be NULL. */
 srf_status srf_trace_spheres(const srf_accel *accel, const srf_ray *rays,

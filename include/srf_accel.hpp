@@ -7,6 +7,7 @@
 //   lsgl::render::TetraAccel      render/render_accel_tetra.h:82-125
 //   lsgl::render::ParticleAccel   render/render_accel_particle.h:68-110
 //   lsgl::render::SolidAccel      render/render_accel_solid.h:82-125
+//   lsgl::render::LineAccel       render/render_accel_line.h:73-103
 //
 // Build32 returns false on failure (the reference's contract) and leaves the
 // reason in LastError(). Traverse takes a batch of device-resident rays and
@@ -143,6 +144,26 @@ class SolidAccel : public detail::AccelBase {
   bool TraverseSolid(const srf_ray *rays, uint64_t n, const float *index_scalars, srf_hit *hits,
                      srf_solid_hit_ext *ext, void *stream = nullptr) {
     return remember(srf_trace_solids(accel_, rays, n, index_scalars, hits, ext, stream));
+  }
+};
+
+/// LineAccel: Build(const Lines*, options) with device arrays; cap is
+/// LineBuildOptions::cap. Built on the LBVH (the reference's own tree is a
+/// top-down spatial-median one); hits are the closest crossing.
+class LineAccel : public detail::AccelBase {
+ public:
+  using AccelBase::AccelBase;
+  bool Build(const float *positions, uint64_t num_vertices, const uint32_t *segments,
+             uint64_t num_lines, const float *radius, float constant_radius, bool cap = true,
+             void *stream = nullptr) {
+    srf_line_desc d{positions, num_vertices, segments, num_lines, radius, constant_radius,
+                    cap ? 1u : 0u};
+    return build(d, srf_accel_build_lines, stream);
+  }
+  /// Traverse + BuildIntersection's position and normal.
+  bool TraverseLines(const srf_ray *rays, uint64_t n, srf_hit *hits, srf_line_hit_ext *ext,
+                     void *stream = nullptr) {
+    return remember(srf_trace_lines(accel_, rays, n, hits, ext, stream));
   }
 };
 

@@ -59,13 +59,16 @@ class Oracle:
                      "or_volume_fwd_f64", "or_volume_bwd_f64",
                      "or_scene_bounds_sph", "or_transform_box", "or_scene_trace",
                      "or_scene_bounds_sol", "or_morton_sol", "or_trace_sol",
-                     "or_interpolate_solid"):
+                     "or_interpolate_solid", "or_scene_bounds_lin", "or_morton_lin",
+                     "or_trace_lin"):
             getattr(L, name).restype = None
         L.or_tlas_build.restype = _i64
-        for name in ("or_build_tri", "or_build_tet", "or_build_sph", "or_build_sol"):
+        for name in ("or_build_tri", "or_build_tet", "or_build_sph", "or_build_sol",
+                     "or_build_lin"):
             getattr(L, name).restype = _i64
         L.or_morton30_point.restype = C.c_uint32
         L.or_solid_isect.restype = C.c_int
+        L.or_line_isect.restype = C.c_int
 
     # ---- stages ----
     def scene_bounds(self, kind, V, F=None, R=None, rconst=0.0):
@@ -183,6 +186,41 @@ class Oracle:
         self.lib.or_interpolate_solid(C.c_int(nv), _ptr(_c(p, np.float32)),
                                       _ptr(_c(verts, np.float64)), _ptr(d))
         return d
+
+    def build_lin(self, P, segs, R=None, rconst=0.0):
+        """The LBVH over line segments (segs: 2 vertex ids per line; radius
+        per vertex R, or rconst)."""
+        P = _c(P, np.float32)
+        segs = _c(segs, np.uint32).reshape(-1, 2)
+        R = None if R is None else _c(R, np.float32)
+        n = len(segs)
+        nodes = np.zeros(2 * n, TET_NODE)
+        idx = np.zeros(n, np.uint32)
+        bounds = np.zeros(6, np.float64)
+        m = self.lib.or_build_lin(_ptr(P), _ptr(segs), _ptr(R), C.c_float(rconst), _i64(n),
+                                  _ptr(nodes), _ptr(idx), _ptr(bounds))
+        return dict(nodes=nodes[:m].copy(), indices=idx, bounds=bounds)
+
+    def line_isect(self, p0, p1, rr, o, d, cap=True, tmax=np.finfo(np.float32).max):
+        out = np.zeros(3, np.float32)
+        hc = C.c_int()
+        h = self.lib.or_line_isect(_ptr(_c(p0, np.float32)), _ptr(_c(p1, np.float32)),
+                                   C.c_float(rr), _ptr(_c(o, np.float32)), _ptr(_c(d, np.float32)),
+                                   C.c_int(1 if cap else 0), C.c_float(tmax), _ptr(out),
+                                   C.byref(hc))
+        return h, out, hc.value
+
+    def trace_lin(self, tree, P, segs, R, rconst, rays, cap=True, threads=1, stats=False):
+        from surface_amd.workloads import HIT_DTYPE, LIN_EXT_DTYPE
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        ext = np.zeros(len(rays), LIN_EXT_DTYPE)
+        st = np.zeros((len(rays), 2), np.uint32) if stats else None
+        Rc = None if R is None else _c(R, np.float32)
+        self.lib.or_trace_lin(_ptr(tree["nodes"]), _ptr(tree["indices"]), _ptr(_c(P, np.float32)),
+                              _ptr(_c(segs, np.uint32)), _ptr(Rc), C.c_float(rconst),
+                              C.c_int(1 if cap else 0), _ptr(rays), _i64(len(rays)), _ptr(hits),
+                              _ptr(ext), _ptr(st), C.c_int(threads))
+        return (hits, ext, st) if stats else (hits, ext)
 
     def build_sph(self, P, R=None, rconst=0.0):
         P = _c(P, np.float32)
@@ -338,7 +376,8 @@ class Reference:
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} missing: run `make -C oracle ref`")
         L = self.lib = C.CDLL(path)
-        for name in ("ref_build_tri", "ref_build_tet", "ref_build_sph", "ref_build_sol"):
+        for name in ("ref_build_tri", "ref_build_tet", "ref_build_sph", "ref_build_sol",
+                     "ref_build_lin"):
             getattr(L, name).restype = _p
         for name in ("ref_morton_tri", "ref_morton_tet", "ref_morton_pts", "ref_sort_keys",
                      "ref_radix_tree30", "ref_bounds", "ref_free", "ref_trace",
@@ -399,13 +438,21 @@ class Reference:
         self.lib.ref_bounds(_p(h), _ptr(bmin), _ptr(bmax))
         return np.concatenate([bmin, bmax])
 
-    def build(self, kind, V, F=None, R=None, rconst=0.0, keep=False, nv=0):
+    def build(self, kind, V, F=None, R=None, rconst=0.0, keep=False, nv=0, cap=True):
         """Run Build32; return nodes (reference layout), indices and bounds.
         With keep=True the handle stays alive for trace() (call free()).
-        kind "sol" builds solids of nv vertices (SolidAccel::Build32)."""
+        kind "sol" builds solids of nv vertices (SolidAccel::Build32); kind
+        "lin" runs LineAccel::Build (F = 2 vertex ids per line)."""
         V = _c(V, np.float32)
         keepalive = [V]
-        if kind == "sol":
+        if kind == "lin":
+            F = _c(F, np.uint32).reshape(-1, 2)
+            R = None if R is None else _c(R, np.float32)
+            keepalive += [F, R]
+            h = self.lib.ref_build_lin(_ptr(V), _ptr(F), _ptr(R), C.c_float(rconst),
+                                       _i64(len(F)), C.c_int(1 if cap else 0))
+            dtype = TET_NODE
+        elif kind == "sol":
             F = _c(F, np.uint32).reshape(-1, nv)
             keepalive.append(F)
             h = self.lib.ref_build_sol(_ptr(V), _ptr(F), _i64(len(F)), C.c_int(nv))
@@ -435,9 +482,10 @@ class Reference:
         return out
 
     def trace(self, tree, rays, width=0, scalars=None, threads=0):
-        from surface_amd.workloads import HIT_DTYPE, SOL_EXT_DTYPE, TET_EXT_DTYPE
+        from surface_amd.workloads import HIT_DTYPE, LIN_EXT_DTYPE, SOL_EXT_DTYPE, TET_EXT_DTYPE
         hits = np.zeros(len(rays), HIT_DTYPE)
-        ext = np.zeros(len(rays), SOL_EXT_DTYPE if tree.get("kind") == "sol" else TET_EXT_DTYPE)
+        ext = np.zeros(len(rays), {"sol": SOL_EXT_DTYPE, "lin": LIN_EXT_DTYPE}.get(
+            tree.get("kind"), TET_EXT_DTYPE))
         sc = None if scalars is None else _c(scalars, np.float32)
         self.lib.ref_trace(_p(tree["handle"]), _ptr(rays), _i64(len(rays)), _i64(width),
                            _ptr(hits), _ptr(ext), _ptr(sc), C.c_int(threads))

@@ -9,8 +9,9 @@
 //                 CalculateMortonCodes30{,SIMD}   render/render_prefix_tree_util.cc
 //   sort          RadixSort30 / RadixSort30OMP   render/render_prefix_tree_util.h:296-457
 //   Karras        ConstructBinaryRadixTree30     render/render_prefix_tree_util.cc:562-649
-//   build         {Triangle,Tetra,Particle,Solid}Accel::Build32 + Dump
-//   trace         {Triangle,Tetra,Particle,Solid}Accel::Traverse, OpenMP
+//   build         {Triangle,Tetra,Particle,Solid}Accel::Build32 + Dump,
+//                 LineAccel::Build (the reference's only line builder)
+//   trace         {Triangle,Tetra,Particle,Solid,Line}Accel::Traverse, OpenMP
 //                 schedule(dynamic,1) over image rows as in
 //                 gles/gles_raytrace_engine.cc:662-665
 //   texture       FilterTexture3DFloat           render/render_texture.cc:946-1058
@@ -22,6 +23,7 @@
 //                 ViewportConversion (gles/gles_raytrace_engine.cc:530-544,
 //                 730-812), written out here in the reference's real3 types
 //                 because that file needs the GL context to compile
+#include <cstdint>  // render_prim_line.h uses uint32_t without including it
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -32,6 +34,7 @@
 #include <omp.h>
 #endif
 
+#include "render_accel_line.h"
 #include "render_accel_particle.h"
 #include "render_accel_solid.h"
 #include "render_camera.h"
@@ -89,12 +92,21 @@ struct SolExt {
   unsigned int side;
 };
 
+struct LinExt {
+  float pos[3];
+  unsigned int hit_cap;
+  float nrm[3];
+  float pad;
+};
+
 struct RefAccel {
-  int kind;  // 0 tri, 1 tet, 2 sphere, 3 solid
+  int kind;  // 0 tri, 1 tet, 2 sphere, 3 solid, 4 line
   Mesh mesh;
   Tetrahedron tet;
   Particles part;
   Solid sol;
+  Lines lin;
+  LineAccel lin_accel;
   TriangleAccel tri_accel;
   TetraAccel tet_accel;
   ParticleAccel sph_accel;
@@ -238,11 +250,32 @@ void *ref_build_sol(const float *V, unsigned *F, long long n, int nv) {
   return a;
 }
 
+// LineAccel::Build (render/render_accel_line.cc:298-341) with
+// LineBuildOptions (minLeafPrimitives 4, maxTreeDepth 512, cap).
+void *ref_build_lin(const float *P, unsigned *segs, const float *R, float rconst, long long n,
+                    int cap) {
+  RefAccel *a = new RefAccel();
+  a->kind = 4;
+  memset(&a->lin, 0, sizeof(a->lin));
+  a->lin.numLines = n;
+  a->lin.positions = P;
+  a->lin.isDoublePrecisionPos = false;
+  a->lin.radius = R;
+  a->lin.constantRadius = rconst;
+  a->lin.segments = segs;
+  LineBuildOptions opt;
+  opt.cap = cap != 0;
+  QuietStdout q;
+  a->lin_accel.Build(&a->lin, opt);
+  return a;
+}
+
 int ref_dump(void *h, const char *path) {
   RefAccel *a = (RefAccel *)h;
   if (a->kind == 0) return a->tri_accel.Dump(path) ? 0 : 1;
   if (a->kind == 1) return a->tet_accel.Dump(path) ? 0 : 1;
   if (a->kind == 3) return a->sol_accel.Dump(path) ? 0 : 1;
+  if (a->kind == 4) return a->lin_accel.Dump(path) ? 0 : 1;
   return a->sph_accel.Dump(path) ? 0 : 1;
 }
 
@@ -254,6 +287,8 @@ void ref_bounds(void *h, double bmin[3], double bmax[3]) {
     a->tet_accel.BoundingBox(bmin, bmax);
   else if (a->kind == 3)
     a->sol_accel.BoundingBox(bmin, bmax);
+  else if (a->kind == 4)
+    a->lin_accel.BoundingBox(bmin, bmax);
   else
     a->sph_accel.BoundingBox(bmin, bmax);
 }
@@ -296,13 +331,32 @@ void ref_trace(void *h, const void *rays_in, long long n, long long width,
         hit = a->tet_accel.Traverse(isect, ray);
       else if (a->kind == 3)
         hit = a->sol_accel.Traverse(isect, ray);
+      else if (a->kind == 4)
+        hit = a->lin_accel.Traverse(isect, ray);
       else
         hit = a->sph_accel.Traverse(isect, ray);
+      if (!hit && a->kind == 4) {
+        // LineAccel::Traverse leaves the caller's Intersection untouched on
+        // a miss (render/render_accel_line.cc:766-773): report the miss
+        isect.t = 3.402823466e+38f;
+        isect.u = isect.v = 0.0f;
+        isect.prim_id = 0xffffffffu;
+      }
       hits[i].t = isect.t;
       hits[i].u = isect.u;
       hits[i].v = isect.v;
       hits[i].prim_id = isect.prim_id;
-      if (ext && a->kind == 3) {
+      if (ext && a->kind == 4) {
+        // position and normal of BuildIntersection (:654-702); whether a
+        // cap was hit is not part of the reference's Intersection
+        LinExt *le = (LinExt *)ext_out + i;
+        memset(le, 0, sizeof(LinExt));
+        if (hit)
+          for (int k = 0; k < 3; k++) {
+            le->pos[k] = isect.position[k];
+            le->nrm[k] = isect.normal[k];
+          }
+      } else if (ext && a->kind == 3) {
         SolExt *se = (SolExt *)ext_out + i;
         memset(se, 0, sizeof(SolExt));
         if (hit) {

@@ -337,7 +337,7 @@ static int split_axis(uint32_t code) { return (clz32(code) - 2) % 3; }
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
-  int kind; /* 0 tri, 1 tet, 2 sphere, 3 solid (pyramid / prism / hexahedron) */
+  int kind; /* 0 tri, 1 tet, 2 sphere, 3 solid (pyramid / prism / hexahedron), 4 line */
   int nv;   /* solids: vertices per solid (5, 6, 8) */
   const float *V;
   const uint32_t *F;
@@ -356,9 +356,24 @@ typedef struct {
  * tri: ComputeBoundingBox30 render/render_accel_triangle.cc:660-709
  * tet: ComputeBoundingBox30 render/render_accel_tetra.cc:921-969
  * sph: ComputeBoundingBox30 render/render_accel_particle.cc:424-462 */
+static void lin_box(const float *V, const uint32_t *F, const float *R, float rconst, uint32_t i,
+                    float mn[3], float mx[3]);
+
 static void leaf_box(const build_ctx *b, uint32_t l, uint32_t r, float mn[3],
                      float mx[3]) {
   const float e = b->kEPS;
+  if (b->kind == 4) {
+    lin_box(b->V, b->F, b->R, b->rconst, b->sidx[l], mn, mx);
+    for (uint32_t i = l; i < r; i++) {
+      float lo[3], hi[3];
+      lin_box(b->V, b->F, b->R, b->rconst, b->sidx[i], lo, hi);
+      for (int k = 0; k < 3; k++) {
+        if (mn[k] > lo[k]) mn[k] = lo[k];
+        if (mx[k] < hi[k]) mx[k] = hi[k];
+      }
+    }
+    return;
+  }
   if (b->kind == 2) {
     uint32_t idx = b->sidx[l];
     float rad = sph_radius(b->R, b->rconst, idx);
@@ -412,7 +427,7 @@ static int64_t push_node(build_ctx *b) { return b->count++; }
 
 static void set_leaf(build_ctx *b, int64_t at, uint32_t l, uint32_t end,
                      const float mn[3], const float mx[3]) {
-  if (b->kind == 1 || b->kind == 3) {
+  if (b->kind == 1 || b->kind == 3 || b->kind == 4) {
     or_tet_node *nd = (or_tet_node *)b->nodes + at;
     memset(nd, 0, sizeof(*nd));
     memcpy(nd->bmin, mn, 12);
@@ -443,7 +458,7 @@ static void set_branch(build_ctx *b, int64_t at, int axis, int64_t lc,
                        int64_t rc, const float lmn[3], const float lmx[3],
                        const float rmn[3], const float rmx[3],
                        const float mn[3], const float mx[3]) {
-  if (b->kind == 1 || b->kind == 3) {
+  if (b->kind == 1 || b->kind == 3 || b->kind == 4) {
     /* TetraNode keeps its own (merged) box: render/render_accel_tetra.cc:1225-1235;
      * SolidNode has the same 40-byte layout and rule,
      * render/render_accel_solid.h:24-43, render/render_accel_solid.cc:1436-1450 */
@@ -520,6 +535,9 @@ static int64_t build_common(build_ctx *b, int64_t n, double bounds[6]) {
   } else if (b->kind == 3) {
     or_scene_bounds_sol(b->V, b->F, n, b->nv, bmin, bmax);
     or_morton_sol(b->V, b->F, n, b->nv, bmin, bmax, codes);
+  } else if (b->kind == 4) {
+    or_scene_bounds_lin(b->V, b->F, b->R, b->rconst, n, bmin, bmax);
+    or_morton_lin(b->V, b->F, n, bmin, bmax, codes);
   } else {
     or_scene_bounds_sph(b->V, b->R, b->rconst, n, bmin, bmax);
     or_morton_pts(b->V, n, bmin, bmax, codes);
@@ -1665,6 +1683,303 @@ void or_trace_sol(const or_tet_node *nodes, const uint32_t *indices, const float
 #endif
   for (int64_t i = 0; i < nrays; i++)
     trace_sol_one(nodes, indices, V, F, nv, scalars, rays + i, hits + i, ext ? ext + i : NULL,
+                  stats ? stats + 2 * i : NULL);
+}
+
+/* ---- lines (capped cylinders) ---- */
+
+/* The reference builds lines only with LineAccel::Build, a top-down
+ * spatial-median tree (render/render_accel_line.cc:167-341); here they go
+ * through the same Karras LBVH as the other kinds: Morton code of the
+ * segment midpoint (p0 + p1) * 0.5f, leaves of r - l <= 4, depth cap 22, and
+ * per-line boxes that hold the whole cylinder CylinderIsect intersects --
+ * radius max(r0, r1) around both endpoints (the reference's
+ * GetBoundingBoxOfLine, :83-121, uses r0 and r1 separately) -- widened by
+ * 16 FLT_EPSILON of each bound's magnitude. */
+static float lin_radius(const float *R, float rconst, uint32_t vtx) {
+  return R ? R[vtx] : rconst;
+}
+
+static void lin_box(const float *V, const uint32_t *F, const float *R, float rconst, uint32_t i,
+                    float mn[3], float mx[3]) {
+  const float kE = FLT_EPSILON * 16.0f;
+  const uint32_t a = F[2 * (size_t)i], b = F[2 * (size_t)i + 1];
+  const float r0 = lin_radius(R, rconst, a), r1 = lin_radius(R, rconst, b);
+  const float rr = r0 > r1 ? r0 : r1;
+  for (int k = 0; k < 3; k++) {
+    const float p = V[3 * (size_t)a + k], q = V[3 * (size_t)b + k];
+    float lo = p - rr, hi = p + rr;
+    const float lo2 = q - rr, hi2 = q + rr;
+    if (lo2 < lo) lo = lo2;
+    if (hi2 > hi) hi = hi2;
+    mn[k] = lo - fabsf(lo) * kE;
+    mx[k] = hi + fabsf(hi) * kE;
+  }
+}
+
+void or_scene_bounds_lin(const float *V, const uint32_t *F, const float *R, float rconst,
+                         int64_t n, float bmin[3], float bmax[3]) {
+  lin_box(V, F, R, rconst, 0, bmin, bmax);
+  for (int64_t i = 0; i < n; i++) {
+    float mn[3], mx[3];
+    lin_box(V, F, R, rconst, (uint32_t)i, mn, mx);
+    for (int k = 0; k < 3; k++) {
+      if (mn[k] < bmin[k]) bmin[k] = mn[k];
+      if (mx[k] > bmax[k]) bmax[k] = mx[k];
+    }
+  }
+}
+
+void or_morton_lin(const float *V, const uint32_t *F, int64_t n, const float bmin[3],
+                   const float bmax[3], uint32_t *codes) {
+  float inv[3];
+  morton_inv(bmin, bmax, inv);
+  for (int64_t i = 0; i < n; i++) {
+    const float *p = V + 3 * (size_t)F[2 * i], *q = V + 3 * (size_t)F[2 * i + 1];
+    float c[3];
+    for (int k = 0; k < 3; k++) c[k] = (p[k] + q[k]) * 0.5f;
+    codes[i] = or_morton30_point(c, bmin, inv[0], inv[1], inv[2]);
+  }
+}
+
+int64_t or_build_lin(const float *V, const uint32_t *F, const float *R, float rconst, int64_t n,
+                     or_tet_node *nodes, uint32_t *indices, double bounds[6]) {
+  build_ctx b;
+  memset(&b, 0, sizeof(b));
+  b.kind = 4;
+  b.V = V;
+  b.F = F;
+  b.R = R;
+  b.rconst = rconst;
+  b.sidx = indices;
+  b.max_leaf = TET_MAX_LEAF;
+  b.nodes = nodes;
+  return build_common(&b, n, bounds);
+}
+
+/* real3 helpers of render/render_accel_line.cc:422-454 (float) */
+static float l3dot(const float a[3], const float b[3]) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static void l3normalize(float v[3], const float a[3]) {
+  const float kE = FLT_EPSILON * 16.0f;
+  const float len = (float)sqrt((double)l3dot(a, a));
+  v[0] = a[0];
+  v[1] = a[1];
+  v[2] = a[2];
+  if (len > kE) {
+    const float inv = (float)(1.0 / len);
+    v[0] *= inv;
+    v[1] *= inv;
+    v[2] *= inv;
+  }
+}
+
+/* solve2e, render/render_accel_line.cc:456-486: A t^2 + 2 B t + C = 0 */
+static int lin_solve2e(float root[2], float A, float B, float C) {
+  if (fabsf(A) <= FLT_EPSILON) {
+    root[0] = -C / B;
+    return 1;
+  }
+  const float D = B * B - A * C;
+  if (D < 0) return 0;
+  if (D == 0) {
+    root[0] = -B / A;
+    return 1;
+  }
+  /* fabs and sqrt are the C library's double functions here, so the sum
+   * and the division run in double and round once (matches the compiled
+   * reference on every tested ray; the float reading differs in 1 ulp) */
+  float x1 = (float)((fabs((double)B) + sqrt((double)D)) / (double)A);
+  if (B >= 0.0) x1 = -x1;
+  float x2 = C / (A * x1);
+  if (x1 > x2) {
+    const float tmp = x1;
+    x1 = x2;
+    x2 = tmp;
+  }
+  root[0] = x1;
+  root[1] = x2;
+  return 2;
+}
+
+/* CylinderIsect, render/render_accel_line.cc:488-577: the end caps (discs of
+ * radius max(r0, r1) on the planes through p0 and p1 normal to the axis,
+ * measured along the normalised direction, strict <), then the side (the
+ * smaller root only, t <= tmax and <= the cap's t, axial fraction in [0, 1]). */
+static int lin_isect(float *pt, float *pu, float *pv, int *hit_cap, const float p0[3],
+                     const float p1[3], float rr, const float o[3], const float dir[3], int cap) {
+  const float kEPSl = 1.0e-6f;
+  const float tmax = *pt;
+  float d[3], m[3];
+  for (int k = 0; k < 3; k++) {
+    d[k] = p1[k] - p0[k];
+    m[k] = o[k] - p0[k];
+  }
+  const float md = l3dot(m, d), nd = l3dot(dir, d), dd = l3dot(d, d);
+  *hit_cap = 0;
+  float capT = FLT_MAX;
+  if (cap) {
+    float dn0[3], dn1[3], rd[3], tmp[3];
+    for (int k = 0; k < 3; k++) tmp[k] = p0[k] - p1[k];
+    l3normalize(dn0, tmp);
+    for (int k = 0; k < 3; k++) dn1[k] = -dn0[k];
+    l3normalize(rd, dir);
+    if (fabsf(l3dot(dir, dn0)) > kEPSl) {
+      const float p0D = -l3dot(p0, dn0), p1D = -l3dot(p1, dn1);
+      const float p0T = -(l3dot(o, dn0) + p0D) / l3dot(rd, dn0);
+      const float p1T = -(l3dot(o, dn1) + p1D) / l3dot(rd, dn1);
+      float q0[3], q1[3], e0[3], e1[3];
+      for (int k = 0; k < 3; k++) {
+        q0[k] = o[k] + p0T * rd[k];
+        q1[k] = o[k] + p1T * rd[k];
+        e0[k] = q0[k] - p0[k];
+        e1[k] = q1[k] - p1[k];
+      }
+      const float qp0 = l3dot(e0, e0), qp1 = l3dot(e1, e1);
+      if (p0T > 0.0 && p0T < tmax && (qp0 < rr * rr)) {
+        *hit_cap = 1;
+        capT = p0T;
+        *pt = capT;
+        *pu = (float)sqrt((double)qp0);
+        *pv = 0;
+      }
+      if (p1T > 0.0 && p1T < tmax && p1T < capT && (qp1 < rr * rr)) {
+        *hit_cap = 1;
+        capT = p1T;
+        *pt = capT;
+        *pu = (float)sqrt((double)qp1);
+        *pv = 1.0f;
+      }
+    }
+  }
+  if (md <= 0.0 && nd <= 0.0) return *hit_cap;
+  if (md >= dd && nd >= 0.0) return *hit_cap;
+  const float nn = l3dot(dir, dir), mn = l3dot(m, dir);
+  const float A = dd * nn - nd * nd;
+  const float kk = l3dot(m, m) - rr * rr;
+  const float C = dd * kk - md * md;
+  const float B = dd * mn - nd * md;
+  float root[2] = {0, 0};
+  if (lin_solve2e(root, A, B, C)) {
+    const float t = root[0];
+    if (0 <= t && t <= tmax && t <= capT) {
+      float s = md + t * nd;
+      s /= dd;
+      if (0 <= s && s <= 1) {
+        *hit_cap = 0;
+        *pt = t;
+        *pu = 0;
+        *pv = s;
+        return 1;
+      }
+    }
+  }
+  return *hit_cap;
+}
+
+int or_line_isect(const float p0[3], const float p1[3], float rr, const float o[3],
+                  const float d[3], int cap, float tmax, float out[3], int *hit_cap) {
+  float t = tmax, u = 0, v = 0;
+  const int h = lin_isect(&t, &u, &v, hit_cap, p0, p1, rr, o, d, cap);
+  out[0] = t;
+  out[1] = u;
+  out[2] = v;
+  return h;
+}
+
+/* LineAccel::Traverse's TestLeafNode (render/render_accel_line.cc:579-633)
+ * and BuildIntersection (:654-702) over the LBVH: every node whose box the
+ * ray enters (IntersectRayAABB, :377-420) is visited, nearer child first,
+ * with no narrowing by the hit distance (a computed crossing of a grazing
+ * ray may lie outside its padded box), so the closest crossing over all
+ * lines is found; ties follow this visit order (caps: first, side: last). */
+static void trace_lin_one(const or_tet_node *nodes, const uint32_t *indices, const float *V,
+                          const uint32_t *F, const float *R, float rconst, int cap,
+                          const or_ray *r, or_hit *h, or_lin_ext *ext, uint32_t *st) {
+  ray_setup s;
+  setup_ray(r, &s);
+  float t = FLT_MAX, bu = 0, bv = 0;
+  uint32_t best = 0xffffffffu;
+  int best_cap = 0;
+  int stack[STACK_MAX];
+  int sp = 0;
+  stack[0] = 0;
+  uint32_t nvisit = 0, ntest = 0;
+  while (sp >= 0) {
+    const or_tet_node *nd = nodes + stack[sp--];
+    nvisit++;
+    if (!ray_aabb(&s, FLT_MAX, nd->bmin, nd->bmax, 0)) continue;
+    if (nd->flag == 0) {
+      int nearc = s.sign[nd->axis], farc = 1 - nearc;
+      stack[++sp] = (int)nd->data[farc];
+      stack[++sp] = (int)nd->data[nearc];
+      continue;
+    }
+    for (uint32_t i = 0; i < nd->data[0]; i++) {
+      const uint32_t li = indices[i + nd->data[1]];
+      ntest++;
+      const uint32_t a = F[2 * (size_t)li], b = F[2 * (size_t)li + 1];
+      const float r0 = lin_radius(R, rconst, a), r1 = lin_radius(R, rconst, b);
+      float u, v;
+      int hc;
+      if (lin_isect(&t, &u, &v, &hc, V + 3 * (size_t)a, V + 3 * (size_t)b, r0 > r1 ? r0 : r1,
+                    s.o, s.d, cap)) {
+        bu = u;
+        bv = v;
+        best = li;
+        best_cap = hc;
+      }
+    }
+  }
+  h->t = t;
+  h->u = best != 0xffffffffu ? bu : 0.0f;
+  h->v = best != 0xffffffffu ? bv : 0.0f;
+  h->prim_id = best;
+  if (st) {
+    st[0] = nvisit;
+    st[1] = ntest;
+  }
+  if (ext) {
+    memset(ext, 0, sizeof(*ext));
+    if (best != 0xffffffffu) {
+      const float *p0 = V + 3 * (size_t)F[2 * (size_t)best];
+      const float *p1 = V + 3 * (size_t)F[2 * (size_t)best + 1];
+      float pos[3], n[3], dl[3];
+      for (int k = 0; k < 3; k++) {
+        pos[k] = r->org[k] + t * r->dir[k];
+        dl[k] = p1[k] - p0[k];
+      }
+      if (best_cap) {
+        float c[3], pc[3];
+        for (int k = 0; k < 3; k++) {
+          c[k] = 0.5f * dl[k] + p0[k];
+          pc[k] = pos[k] - c[k];
+        }
+        l3normalize(n, dl);
+        if (!(l3dot(pc, n) > 0.0))
+          for (int k = 0; k < 3; k++) n[k] = -n[k];
+      } else {
+        float pc[3];
+        for (int k = 0; k < 3; k++) pc[k] = pos[k] - (p0[k] + bv * dl[k]);
+        l3normalize(n, pc);
+      }
+      memcpy(ext->pos, pos, 12);
+      memcpy(ext->nrm, n, 12);
+      ext->hit_cap = (uint32_t)best_cap;
+    }
+  }
+}
+
+void or_trace_lin(const or_tet_node *nodes, const uint32_t *indices, const float *V,
+                  const uint32_t *F, const float *R, float rconst, int cap, const or_ray *rays,
+                  int64_t nrays, or_hit *hits, or_lin_ext *ext, uint32_t *stats, int threads) {
+  (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+#endif
+  for (int64_t i = 0; i < nrays; i++)
+    trace_lin_one(nodes, indices, V, F, R, rconst, cap, rays + i, hits + i, ext ? ext + i : NULL,
                   stats ? stats + 2 * i : NULL);
 }
 

@@ -93,6 +93,15 @@ typedef struct {
   uint32_t side;
 } or_sol_ext;
 
+/* Line extras: Intersection::position (org + t dir) and normal
+ * (render/render_accel_line.cc:654-702) and whether an end cap was hit. */
+typedef struct {
+  float pos[3];
+  uint32_t hit_cap;
+  float nrm[3];
+  float pad;
+} or_lin_ext;
+
 /* ---- LBVH stages ---- */
 void or_scene_bounds_tri(const float *V, const uint32_t *F, int64_t n,
                          float bmin[3], float bmax[3]);
@@ -128,6 +137,15 @@ int64_t or_build_tet(const float *V, const uint32_t *F, int64_t n,
                      or_tet_node *nodes, uint32_t *indices, double bounds[6]);
 int64_t or_build_sph(const float *P, const float *R, float rconst, int64_t n,
                      or_sph_node *nodes, uint32_t *indices, double bounds[6]);
+/* lines: segments F (2 vertex ids per line), radius per vertex R or rconst;
+ * the LBVH over segment midpoints (the reference has only its top-down
+ * LineAccel::Build) */
+void or_scene_bounds_lin(const float *V, const uint32_t *F, const float *R, float rconst,
+                         int64_t n, float bmin[3], float bmax[3]);
+void or_morton_lin(const float *V, const uint32_t *F, int64_t n, const float bmin[3],
+                   const float bmax[3], uint32_t *codes);
+int64_t or_build_lin(const float *V, const uint32_t *F, const float *R, float rconst, int64_t n,
+                     or_tet_node *nodes, uint32_t *indices, double bounds[6]);
 /* solids of nv = 5 (pyramid), 6 (prism) or 8 (hexahedron) vertices */
 int64_t or_build_sol(const float *V, const uint32_t *F, int64_t n, int nv,
                      or_tet_node *nodes, uint32_t *indices, double bounds[6]);
@@ -153,6 +171,13 @@ void or_trace_sol(const or_tet_node *nodes, const uint32_t *indices,
 int or_solid_isect(int nv, const double o[3], const double d[3], const double *verts,
                    float t[2], float pos[6], float nrm[6]);
 void or_interpolate_solid(int nv, const float p[3], const double *verts, float d_out[8]);
+void or_trace_lin(const or_tet_node *nodes, const uint32_t *indices, const float *V,
+                  const uint32_t *F, const float *R, float rconst, int cap, const or_ray *rays,
+                  int64_t nrays, or_hit *hits, or_lin_ext *ext, uint32_t *stats, int threads);
+/* CylinderIsect for one line (radius rr) with the running distance tmax:
+ * out = (t, u, v) as updated, returns the hit flag. */
+int or_line_isect(const float p0[3], const float p1[3], float rr, const float o[3],
+                  const float d[3], int cap, float tmax, float out[3], int *hit_cap);
 void or_trace_sph(const or_sph_node *nodes, const uint32_t *indices,
                   const float *P, const float *R, float rconst,
                   const or_ray *rays, const double *prev_hit_t,

@@ -22,6 +22,7 @@ STATUS_NAMES = {0: "SRF_OK", 1: "SRF_ERR_INVALID_ARG", 2: "SRF_ERR_OUT_OF_MEMORY
                 3: "SRF_ERR_HIP", 4: "SRF_ERR_UNSUPPORTED", 5: "SRF_ERR_INTERNAL"}
 PRIM_POINTS, PRIM_TRIANGLES, PRIM_TETRAHEDRONS = 0x0000, 0x0004, 0x0010
 PRIM_PYRAMIDS, PRIM_PRISMS, PRIM_HEXAHEDRONS = 0x0011, 0x0012, 0x0013
+PRIM_LINES = 0x0001
 PRIM_SOLID = {5: PRIM_PYRAMIDS, 6: PRIM_PRISMS, 8: PRIM_HEXAHEDRONS}
 
 
@@ -45,6 +46,12 @@ class SolidDesc(C.Structure):
     _fields_ = [("vertices", C.c_void_p), ("num_vertices", C.c_uint64),
                 ("indices", C.c_void_p), ("num_solids", C.c_uint64),
                 ("vertices_per_solid", C.c_uint32)]
+
+
+class LineDesc(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("num_vertices", C.c_uint64),
+                ("segments", C.c_void_p), ("num_lines", C.c_uint64),
+                ("radius", C.c_void_p), ("constant_radius", C.c_float), ("cap", C.c_uint32)]
 
 
 class SphereDesc(C.Structure):
@@ -95,6 +102,7 @@ SIGNATURES = {
     "srf_accel_build_tetras": (_i, [_vp, C.POINTER(TetDesc), _vp, C.POINTER(_vp)]),
     "srf_accel_build_spheres": (_i, [_vp, C.POINTER(SphereDesc), _vp, C.POINTER(_vp)]),
     "srf_accel_build_solids": (_i, [_vp, C.POINTER(SolidDesc), _vp, C.POINTER(_vp)]),
+    "srf_accel_build_lines": (_i, [_vp, C.POINTER(LineDesc), _vp, C.POINTER(_vp)]),
     "srf_accel_build_triangles_batch": (_i, [_vp, C.c_uint32, _vp, _vp, _vp]),
     "srf_accel_rebuild_triangles_batch": (_i, [_vp, C.c_uint32, _vp, _vp, _vp]),
     "srf_accel_rebuild": (_i, [_vp, _vp, _vp, _vp]),
@@ -106,6 +114,7 @@ SIGNATURES = {
     "srf_trace_closest": (_i, [_vp, _vp, _u64, _vp, _vp]),
     "srf_trace_tetra": (_i, [_vp, _vp, _u64, _vp, _vp, _vp, _vp]),
     "srf_trace_solids": (_i, [_vp, _vp, _u64, _vp, _vp, _vp, _vp]),
+    "srf_trace_lines": (_i, [_vp, _vp, _u64, _vp, _vp, _vp]),
     "srf_trace_spheres": (_i, [_vp, _vp, _vp, _u64, _vp, _vp]),
     "srf_trace_any": (_i, [_vp, _vp, _vp, _u64, _vp, _vp]),
     "srf_lbvh_morton30": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),

@@ -4,7 +4,8 @@ The classes keep the reference's names and meaning -- TriangleAccel,
 TetraAccel, ParticleAccel and SolidAccel with Build32 / Traverse /
 BoundingBox (render/render_accel_triangle.h:69-105,
 render/render_accel_tetra.h:82-125, render/render_accel_particle.h:68-110,
-render/render_accel_solid.h:82-125) -- with Traverse taking a batch of
+render/render_accel_solid.h:82-125) and LineAccel with Build / Traverse
+(render/render_accel_line.h:73-103) -- with Traverse taking a batch of
 rays. Geometry, rays and hits are torch tensors on the GPU (torch is only the
 allocator and stream provider here); every computation runs in libsrf.so.
 Failures raise SrfError where the reference returns false or asserts.
@@ -17,7 +18,8 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import AccelInfo, SolidDesc, SphereDesc, SrfError, TetDesc, TriDesc, check, load
+from ._lib import (AccelInfo, LineDesc, SolidDesc, SphereDesc, SrfError, TetDesc, TriDesc, check,
+                   load)
 from .workloads import HIT_DTYPE, RAY_DTYPE, TET_EXT_DTYPE
 
 try:
@@ -310,6 +312,33 @@ class SolidAccel(_Accel):
             ext = torch.empty((n, 16), dtype=torch.float32, device=rays.device)
         check(self.lib.srf_trace_solids(self.handle, _ptr(rays), n, _ptr(scalars), _ptr(hits),
                                         _ptr(ext), _stream(stream)))
+        return hits, ext
+
+
+class LineAccel(_Accel):
+    """LineAccel::Build / Traverse / BoundingBox (capped cylinders along line
+    segments), built on the LBVH."""
+    kind = _lib.PRIM_LINES
+    node_dtype = TET_NODE_DTYPE
+
+    def Build(self, positions, segments, radius=None, constant_radius=0.0, cap=True,
+              stream=None):
+        d = LineDesc(positions.data_ptr(), positions.numel() // 3, segments.data_ptr(),
+                     segments.numel() // 2, radius.data_ptr() if radius is not None else None,
+                     float(constant_radius), 1 if cap else 0)
+        return self._build(self.lib.srf_accel_build_lines, d, stream)
+
+    def TraverseLines(self, rays, hits=None, ext=None, stream=None):
+        """Closest hit + position, normal and cap flag; ext is [n, 8] float32
+        (srf_line_hit_ext)."""
+        self._require()
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+        if ext is None:
+            ext = torch.empty((n, 8), dtype=torch.float32, device=rays.device)
+        check(self.lib.srf_trace_lines(self.handle, _ptr(rays), n, _ptr(hits), _ptr(ext),
+                                       _stream(stream)))
         return hits, ext
 
 

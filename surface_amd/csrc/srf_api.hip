@@ -84,6 +84,7 @@ size_t node_bytes(Kind) { return 64; }
 // packed primitive record: triangle 48 B, tetra 64 B, sphere 16 B, solid
 // 16 B per vertex
 size_t prim_bytes(Kind k, uint32_t nvps) {
+  if (k == kLin) return 32;
   return k == kTri ? 48 : (k == kTet ? 64 : (k == kSol ? 16 * (size_t)nvps : 16));
 }
 
@@ -285,6 +286,14 @@ srf_status srf_accel_build_solids(srf_ctx *ctx, const srf_solid_desc *d, void *s
   return build_common(ctx, kSol, d, d->num_solids, d->num_vertices, stream, out, k);
 }
 
+srf_status srf_accel_build_lines(srf_ctx *ctx, const srf_line_desc *d, void *stream,
+                                 srf_accel **out) {
+  if (!d || !d->positions || !d->segments) return set_error(SRF_ERR_INVALID_ARG, "null lines");
+  srf_status st = build_common(ctx, kLin, d, d->num_lines, d->num_vertices, stream, out);
+  if (st == SRF_OK) reinterpret_cast<Accel *>(*out)->line_cap = d->cap ? 1u : 0u;
+  return st;
+}
+
 srf_status srf_accel_build_spheres(srf_ctx *ctx, const srf_sphere_desc *d, void *stream,
                                    srf_accel **out) {
   if (!d || !d->positions) return set_error(SRF_ERR_INVALID_ARG, "null particles");
@@ -384,6 +393,12 @@ srf_status srf_accel_rebuild(srf_ctx *ctx, srf_accel *accel, const void *desc, v
     if (!d->vertices || !d->indices) return set_error(SRF_ERR_INVALID_ARG, "null tetras");
     n = d->num_tetras;
     nv = d->num_vertices;
+  } else if (a->kind == kLin) {
+    const srf_line_desc *d = (const srf_line_desc *)desc;
+    if (!d->positions || !d->segments) return set_error(SRF_ERR_INVALID_ARG, "null lines");
+    n = d->num_lines;
+    nv = d->num_vertices;
+    a->line_cap = d->cap ? 1u : 0u;
   } else if (a->kind == kSol) {
     const srf_solid_desc *d = (const srf_solid_desc *)desc;
     if (!d->vertices || !d->indices) return set_error(SRF_ERR_INVALID_ARG, "null solids");
@@ -434,7 +449,9 @@ srf_status srf_accel_get_info(const srf_accel *accel, srf_accel_info *info) {
   BuildParams p;
   srf_status st = fetch_params(a, &p);
   if (st != SRF_OK) return st;
-  if (a->kind == kSol)
+  if (a->kind == kLin)
+    info->kind = SRF_PRIM_LINES;
+  else if (a->kind == kSol)
     info->kind = a->nvps == 5 ? SRF_PRIM_PYRAMIDS
                               : (a->nvps == 6 ? SRF_PRIM_PRISMS : SRF_PRIM_HEXAHEDRONS);
   else
@@ -445,7 +462,8 @@ srf_status srf_accel_get_info(const srf_accel *accel, srf_accel_info *info) {
   info->num_leaves = p.num_leaves;
   info->max_depth = p.max_steps + 1;
   info->eps_scale = p.eps_scale;
-  info->node_bytes = a->kind == kTri ? 64 : ((a->kind == kTet || a->kind == kSol) ? 40 : 60);
+  info->node_bytes =
+      a->kind == kTri ? 64 : ((a->kind == kTet || a->kind == kSol || a->kind == kLin) ? 40 : 60);
   return SRF_OK;
 }
 
@@ -470,8 +488,8 @@ srf_status srf_accel_export(const srf_accel *accel, void *nodes_host, uint32_t *
     return hip_error(e, "hipMemcpy");
   }
   uint8_t *dst = reinterpret_cast<uint8_t *>(nodes_host);
-  if (a->kind == kTet || a->kind == kSol) {
-    // TetraNode / SolidNode: bmin[3], bmax[3], flag, axis, data[2] (40 B); a node's own
+  if (a->kind == kTet || a->kind == kSol || a->kind == kLin) {
+    // TetraNode / SolidNode / LineNode: bmin[3], bmax[3], flag, axis, data[2] (40 B); a node's own
     // box is the one its parent holds for it, the root's is BoundingBox()
     float rb[6];
     for (int k = 0; k < 3; k++) {
@@ -555,7 +573,18 @@ srf_status srf_trace_closest(const srf_accel *accel, const srf_ray *rays, uint64
   if (a->kind == kTri) return trace_tri(a, rays, n, hits, s);
   if (a->kind == kTet) return trace_tet(a, rays, n, nullptr, hits, nullptr, s);
   if (a->kind == kSol) return trace_sol(a, rays, n, nullptr, hits, nullptr, s);
+  if (a->kind == kLin) return trace_lin(a, rays, n, hits, nullptr, s);
   return trace_sph(a, rays, nullptr, n, hits, s);
+}
+
+srf_status srf_trace_lines(const srf_accel *accel, const srf_ray *rays, uint64_t n,
+                           srf_hit *hits, srf_line_hit_ext *ext, void *stream) {
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!a || (n && (!rays || !hits))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (a->kind != kLin) return set_error(SRF_ERR_INVALID_ARG, "not a line accelerator");
+  if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many rays");
+  DeviceGuard g(a->device);
+  return trace_lin(a, rays, n, hits, ext, reinterpret_cast<hipStream_t>(stream));
 }
 
 srf_status srf_trace_solids(const srf_accel *accel, const srf_ray *rays, uint64_t n,
@@ -613,6 +642,8 @@ srf_status srf_lbvh_morton30(srf_ctx *ctx, srf_prim_kind kind, const void *desc,
     k = kTet;
   else if (kind == SRF_PRIM_POINTS)
     k = kSph;
+  else if (kind == SRF_PRIM_LINES)
+    k = kLin;
   else if (kind == SRF_PRIM_PYRAMIDS || kind == SRF_PRIM_PRISMS || kind == SRF_PRIM_HEXAHEDRONS) {
     const uint32_t want = kind == SRF_PRIM_PYRAMIDS ? 5 : (kind == SRF_PRIM_PRISMS ? 6 : 8);
     if (((const srf_solid_desc *)desc)->vertices_per_solid != want)

@@ -177,6 +177,27 @@ __device__ __forceinline__ uint32_t seg_off(const uint32_t *off, uint32_t m) {
   return M ? off[m] : 0u;
 }
 
+// A line's box: radius max(r0, r1) around both endpoints (the cylinder and
+// end discs CylinderIsect tests, render/render_accel_line.cc:488-577), each
+// bound widened by 16 FLT_EPSILON of its magnitude. The reference's own
+// GetBoundingBoxOfLine (:83-121) pads p0 by r0 and p1 by r1 only.
+__device__ __forceinline__ void line_box(const PrimSrc &s, uint32_t i, float mn[3], float mx[3]) {
+  const float kE = FLT_EPSILON * 16.0f;
+  const uint32_t a = s.F[2 * (size_t)i], b = s.F[2 * (size_t)i + 1];
+  const float r0 = s.R ? s.R[a] : s.rconst, r1 = s.R ? s.R[b] : s.rconst;
+  const float rr = r0 > r1 ? r0 : r1;
+  const float3 p = load_v3(s.V, a), q = load_v3(s.V, b);
+  const float pc[3] = {p.x, p.y, p.z}, qc[3] = {q.x, q.y, q.z};
+  for (int k = 0; k < 3; k++) {
+    float lo = pc[k] - rr, hi = pc[k] + rr;
+    const float lo2 = qc[k] - rr, hi2 = qc[k] + rr;
+    if (lo2 < lo) lo = lo2;
+    if (hi2 > hi) hi = hi2;
+    mn[k] = lo - fabsf(lo) * kE;
+    mx[k] = hi + fabsf(hi) * kE;
+  }
+}
+
 // Per-primitive contribution to the scene box:
 //  tri  ComputeBoundingBoxOMP, epsScale 0  render/render_accel_triangle.cc:711-797,1430-1443
 //  tet  kEPS = 16*FLT_EPSILON               render/render_accel_tetra.cc:971-1055
@@ -184,7 +205,14 @@ __device__ __forceinline__ uint32_t seg_off(const uint32_t *off, uint32_t m) {
 //  sol  kEPS = 16*FLT_EPSILON               render/render_accel_solid.cc:1172-1262
 template <int K>
 __device__ __forceinline__ void scene_add(Box &b, const PrimSrc &s, uint32_t i) {
-  if (K == kSol) {
+  if (K == kLin) {
+    float mn[3], mx[3];
+    line_box(s, i, mn, mx);
+    for (int k = 0; k < 3; k++) {
+      b.mn[k] = fmin_ref(b.mn[k], mn[k]);
+      b.mx[k] = fmax_ref(b.mx[k], mx[k]);
+    }
+  } else if (K == kSol) {
     for (uint32_t j = 0; j < s.nv; j++) {
       float3 v = load_v3(s.V, s.F[(size_t)s.nv * i + j]);
       box_add(b, v.x, v.y, v.z, FLT_EPSILON * 16);
@@ -298,7 +326,7 @@ __global__ void __launch_bounds__(256) k_bounds(SegView v, Box *partials,
     p.eps_scale = es;
     if (K == kTri)
       p.kEPS = FLT_EPSILON * (float)es;  // ComputeBoundingBox30's real kEPS
-    else if (K == kTet || K == kSol)
+    else if (K == kTet || K == kSol || K == kLin)
       p.kEPS = FLT_EPSILON * 16;
     else
       p.kEPS = (float)(FLT_EPSILON * 16.0);
@@ -324,7 +352,11 @@ __global__ void __launch_bounds__(256) k_bounds(SegView v, Box *partials,
 template <int K>
 __device__ __forceinline__ uint32_t prim_code(const PrimSrc &s, uint32_t i,
                                               const BuildParams &p) {
-  if (K == kSol) {
+  if (K == kLin) {
+    // the segment's midpoint (p0 + p1) * 0.5f
+    const float3 a = load_v3(s.V, s.F[2 * (size_t)i]), b = load_v3(s.V, s.F[2 * (size_t)i + 1]);
+    return morton30((a.x + b.x) * 0.5f, (a.y + b.y) * 0.5f, (a.z + b.z) * 0.5f, p);
+  } else if (K == kSol) {
     // the vertices summed from (0, 0, 0) in index order, times 1 / nv
     const float inv_div = 1.0f / (float)s.nv;
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;
@@ -962,7 +994,27 @@ __device__ __forceinline__ void leaf_box_pack(const PrimSrc &s, const uint32_t *
   // sol ComputeBoundingBox30 render/render_accel_solid.cc:1120-1169
   for (uint32_t q = first; q < first + count; q++) {
     uint32_t id = sidx[q];
-    if (K == kSol) {
+    if (K == kLin) {
+      // two records per line: (p0, id) and (p1, max radius)
+      float mn[3], mx[3];
+      line_box(s, id, mn, mx);
+      if (q == first) {
+        for (int k = 0; k < 3; k++) {
+          b.mn[k] = mn[k];
+          b.mx[k] = mx[k];
+        }
+      }
+      for (int k = 0; k < 3; k++) {
+        b.mn[k] = fmin_ref(b.mn[k], mn[k]);
+        b.mx[k] = fmax_ref(b.mx[k], mx[k]);
+      }
+      const uint32_t a = s.F[2 * (size_t)id], c = s.F[2 * (size_t)id + 1];
+      const float r0 = s.R ? s.R[a] : s.rconst, r1 = s.R ? s.R[c] : s.rconst;
+      const float3 p = load_v3(s.V, a), pq = load_v3(s.V, c);
+      float4 *P = reinterpret_cast<float4 *>(prims) + 2 * (size_t)q;
+      P[0] = make_float4(p.x, p.y, p.z, __uint_as_float(id));
+      P[1] = make_float4(pq.x, pq.y, pq.z, r0 > r1 ? r0 : r1);
+    } else if (K == kSol) {
       // nv records per solid: vertex k in xyz, the solid's id in w of the first
       float4 *P = reinterpret_cast<float4 *>(prims) + (size_t)s.nv * q;
       for (uint32_t k = 0; k < s.nv; k++) {
@@ -1108,7 +1160,16 @@ __device__ __forceinline__ int pack_prim(const PrimSrc &s, const uint32_t *sidx,
                                          float e, void *prims, uint32_t *sorted_faces,
                                          float lo[8][3], float hi[8][3]) {
   const uint32_t id = sidx[q];
-  if (K == kSol) {
+  if (K == kLin) {
+    line_box(s, id, lo[0], hi[0]);
+    const uint32_t a = s.F[2 * (size_t)id], c = s.F[2 * (size_t)id + 1];
+    const float r0 = s.R ? s.R[a] : s.rconst, r1 = s.R ? s.R[c] : s.rconst;
+    const float3 p = load_v3(s.V, a), pq = load_v3(s.V, c);
+    float4 *P = reinterpret_cast<float4 *>(prims) + 2 * (size_t)q;
+    P[0] = make_float4(p.x, p.y, p.z, __uint_as_float(id));
+    P[1] = make_float4(pq.x, pq.y, pq.z, r0 > r1 ? r0 : r1);
+    return 1;
+  } else if (K == kSol) {
     float4 *P = reinterpret_cast<float4 *>(prims) + (size_t)s.nv * q;
     const int nv = s.nv < 8u ? (int)s.nv : 8;
     for (int k = 0; k < nv; k++) {
@@ -1355,6 +1416,13 @@ PrimSrc make_src(Kind kind, const void *desc) {
     s.F = d->indices;
     s.n = (uint32_t)d->num_solids;
     s.nv = d->vertices_per_solid;
+  } else if (kind == kLin) {
+    const srf_line_desc *d = (const srf_line_desc *)desc;
+    s.V = d->positions;
+    s.F = d->segments;
+    s.R = d->radius;
+    s.rconst = d->constant_radius;
+    s.n = (uint32_t)d->num_lines;
   } else {
     const srf_sphere_desc *d = (const srf_sphere_desc *)desc;
     s.V = d->positions;
@@ -1517,8 +1585,11 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
     b.ss.vals_a = segs[0].sorted_index;  // the sorted index lands in the accel
   }
   // SolidAccel's MAX_LEAF_ELEMENTS is the tetra's 4 (render/render_accel_solid.cc:34)
-  int max_leaf =
-      kind == kTri ? kTriMaxLeaf : ((kind == kTet || kind == kSol) ? kTetMaxLeaf : kSphMaxLeaf);
+  // lines: the same 4 (LineAccel's own tree stops below 4 lines per leaf,
+  // render/render_accel_line.h:59)
+  int max_leaf = kind == kTri ? kTriMaxLeaf
+                              : ((kind == kTet || kind == kSol || kind == kLin) ? kTetMaxLeaf
+                                                                                : kSphMaxLeaf);
 
   // zero the bounds tickets (the last bounds block rewrites each params block)
   if (multi) {
@@ -1543,6 +1614,8 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
         hipLaunchKernelGGL((KERNEL<kTet, true>), __VA_ARGS__);                           \
       else if (kind == kSol)                                                           \
         hipLaunchKernelGGL((KERNEL<kSol, true>), __VA_ARGS__);                           \
+      else if (kind == kLin)                                                           \
+        hipLaunchKernelGGL((KERNEL<kLin, true>), __VA_ARGS__);                           \
       else                                                                             \
         hipLaunchKernelGGL((KERNEL<kSph, true>), __VA_ARGS__);                           \
     } else {                                                                           \
@@ -1551,7 +1624,9 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
       else if (kind == kTet)                                                           \
         hipLaunchKernelGGL((KERNEL<kTet, false>), __VA_ARGS__);                          \
       else if (kind == kSol)                                                           \
-        hipLaunchKernelGGL((KERNEL<kSol, false>), __VA_ARGS__);                          \
+        hipLaunchKernelGGL((KERNEL<kSol, false>), __VA_ARGS__);                           \
+      else if (kind == kLin)                                                           \
+        hipLaunchKernelGGL((KERNEL<kLin, false>), __VA_ARGS__);                          \
       else                                                                             \
         hipLaunchKernelGGL((KERNEL<kSph, false>), __VA_ARGS__);                          \
     }                                                                                  \
@@ -1732,6 +1807,8 @@ srf_status launch_morton(Kind kind, const void *desc, const float bmin[3],
     hipLaunchKernelGGL(k_morton_box<kTet>, dim3(blocks), dim3(256), 0, stream, s, p, codes);
   else if (kind == kSol)
     hipLaunchKernelGGL(k_morton_box<kSol>, dim3(blocks), dim3(256), 0, stream, s, p, codes);
+  else if (kind == kLin)
+    hipLaunchKernelGGL(k_morton_box<kLin>, dim3(blocks), dim3(256), 0, stream, s, p, codes);
   else
     hipLaunchKernelGGL(k_morton_box<kSph>, dim3(blocks), dim3(256), 0, stream, s, p, codes);
   SRF_HIP(hipGetLastError());

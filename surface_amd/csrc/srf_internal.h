@@ -39,8 +39,8 @@ constexpr int kStackSize = 32;
 constexpr uint32_t kBigLeaf = 64;
 
 // kSol: pyramids, prisms and hexahedra (SolidAccel), Accel::nvps vertices
-// per solid
-enum Kind : uint32_t { kTri = 0, kTet = 1, kSph = 2, kSol = 3 };
+// per solid; kLin: capped cylinders along line segments (LineAccel)
+enum Kind : uint32_t { kTri = 0, kTet = 1, kSph = 2, kSol = 3, kLin = 4 };
 
 // PartBy2_30 (render/render_prefix_tree_util.h:113-120): the low 10 bits of
 // n spread to every third bit.
@@ -99,6 +99,7 @@ struct Accel {
   uint64_t n;            // primitives
   uint64_t nv;           // vertices (tets: for scalar lookups)
   uint32_t nvps;         // solids: vertices per solid (5, 6, 8)
+  uint32_t line_cap;     // lines: LineBuildOptions::cap (end discs tested)
   uint64_t capacity;     // primitives the buffers can hold
   void *nodes;           // TriNode / TetNode / SphNode
   void *prims;           // packed primitives (sorted order)
@@ -186,6 +187,8 @@ srf_status trace_tet(const Accel *a, const srf_ray *rays, uint64_t n,
                      hipStream_t s);
 srf_status trace_sol(const Accel *a, const srf_ray *rays, uint64_t n, const float *scalars,
                      srf_hit *hits, srf_solid_hit_ext *ext, hipStream_t s);
+srf_status trace_lin(const Accel *a, const srf_ray *rays, uint64_t n, srf_hit *hits,
+                     srf_line_hit_ext *ext, hipStream_t s);
 srf_status trace_sph(const Accel *a, const srf_ray *rays,
                      const srf_sphere_prev *prev, uint64_t n, srf_hit *hits,
                      hipStream_t s);

@@ -29,6 +29,7 @@
 //   tetras     TetraAccel::Traverse     render/render_accel_tetra.cc:2200-2289
 //   spheres    ParticleAccel::Traverse  render/render_accel_particle.cc:3053-3218
 //   solids     SolidAccel::Traverse     render/render_accel_solid.cc:2345-2435
+//   lines      LineAccel::Traverse      render/render_accel_line.cc:704-774
 #include <algorithm>
 
 #include "srf_trace_dev.h"
@@ -1334,6 +1335,234 @@ k_trace_sol(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   reinterpret_cast<float4 *>(hits)[i] = make_float4(t, 0.0f, 0.0f, __uint_as_float(prim));
 }
 
+// -------------------------------------------------------------------- lines
+// LineAccel::Traverse (render/render_accel_line.cc:704-774) over the LBVH.
+// Every node whose box the ray enters is visited, nearer child first, with
+// no narrowing by the hit distance: the crossing a float quadratic reports
+// for a grazing ray may lie outside its padded box, so the closest crossing
+// over all lines is found. Running t as TestLeafNode keeps it: a cap
+// replaces it only when strictly nearer, the side when not farther.
+
+__device__ __forceinline__ float l3dot(float ax, float ay, float az, float bx, float by,
+                                       float bz) {
+  return (ax * bx + ay * by) + az * bz;
+}
+// vnormalize, render/render_accel_line.cc:439-454
+__device__ __forceinline__ void l3norm(float &x, float &y, float &z) {
+  const float len = sqrtf(l3dot(x, y, z, x, y, z));
+  if (len > FLT_EPSILON * 16.0f) {
+    const float inv = (float)(1.0 / (double)len);
+    x *= inv;
+    y *= inv;
+    z *= inv;
+  }
+}
+
+// CylinderIsect, render/render_accel_line.cc:488-577 (solve2e :456-486):
+// t / u / v / cap updated in place on a hit.
+__device__ __forceinline__ bool line_isect(float &t, float &u, float &v, bool &hit_cap,
+                                           const float4 &P0, const float4 &P1, const RaySetup &s,
+                                           bool cap) {
+  const float tmax = t, rr = P1.w;
+  const float dx = P1.x - P0.x, dy = P1.y - P0.y, dz = P1.z - P0.z;
+  const float mx = s.o[0] - P0.x, my = s.o[1] - P0.y, mz = s.o[2] - P0.z;
+  const float md = l3dot(mx, my, mz, dx, dy, dz);
+  const float nd = l3dot(s.d[0], s.d[1], s.d[2], dx, dy, dz);
+  const float dd = l3dot(dx, dy, dz, dx, dy, dz);
+  hit_cap = false;
+  float capT = FLT_MAX;
+  bool hit = false;
+  if (cap) {
+    float n0x = P0.x - P1.x, n0y = P0.y - P1.y, n0z = P0.z - P1.z;
+    l3norm(n0x, n0y, n0z);
+    const float n1x = -n0x, n1y = -n0y, n1z = -n0z;
+    float rx = s.d[0], ry = s.d[1], rz = s.d[2];
+    l3norm(rx, ry, rz);
+    if (fabsf(l3dot(s.d[0], s.d[1], s.d[2], n0x, n0y, n0z)) > 1.0e-6f) {
+      const float p0D = -l3dot(P0.x, P0.y, P0.z, n0x, n0y, n0z);
+      const float p1D = -l3dot(P1.x, P1.y, P1.z, n1x, n1y, n1z);
+      const float p0T = -(l3dot(s.o[0], s.o[1], s.o[2], n0x, n0y, n0z) + p0D) /
+                        l3dot(rx, ry, rz, n0x, n0y, n0z);
+      const float p1T = -(l3dot(s.o[0], s.o[1], s.o[2], n1x, n1y, n1z) + p1D) /
+                        l3dot(rx, ry, rz, n1x, n1y, n1z);
+      const float e0x = (s.o[0] + p0T * rx) - P0.x, e0y = (s.o[1] + p0T * ry) - P0.y,
+                  e0z = (s.o[2] + p0T * rz) - P0.z;
+      const float e1x = (s.o[0] + p1T * rx) - P1.x, e1y = (s.o[1] + p1T * ry) - P1.y,
+                  e1z = (s.o[2] + p1T * rz) - P1.z;
+      const float q0 = l3dot(e0x, e0y, e0z, e0x, e0y, e0z);
+      const float q1 = l3dot(e1x, e1y, e1z, e1x, e1y, e1z);
+      if (p0T > 0.0f && p0T < tmax && (q0 < rr * rr)) {
+        hit_cap = hit = true;
+        capT = p0T;
+        t = capT;
+        u = sqrtf(q0);
+        v = 0.0f;
+      }
+      if (p1T > 0.0f && p1T < tmax && p1T < capT && (q1 < rr * rr)) {
+        hit_cap = hit = true;
+        capT = p1T;
+        t = capT;
+        u = sqrtf(q1);
+        v = 1.0f;
+      }
+    }
+  }
+  if (md <= 0.0f && nd <= 0.0f) return hit;
+  if (md >= dd && nd >= 0.0f) return hit;
+  const float nn = l3dot(s.d[0], s.d[1], s.d[2], s.d[0], s.d[1], s.d[2]);
+  const float mn = l3dot(mx, my, mz, s.d[0], s.d[1], s.d[2]);
+  const float A = dd * nn - nd * nd;
+  const float kk = l3dot(mx, my, mz, mx, my, mz) - rr * rr;
+  const float C = dd * kk - md * md;
+  const float B = dd * mn - nd * md;
+  float root;
+  bool ok = true;
+  if (fabsf(A) <= FLT_EPSILON) {
+    root = -C / B;
+  } else {
+    const float D = B * B - A * C;
+    if (D < 0.0f) {
+      ok = false;
+      root = 0.0f;
+    } else if (D == 0.0f) {
+      root = -B / A;
+    } else {
+      // fabs and sqrt there are the double C library functions: the sum and
+      // the division run in double
+      float x1 = (float)((fabs((double)B) + sqrt((double)D)) / (double)A);
+      if (B >= 0.0f) x1 = -x1;
+      const float x2 = C / (A * x1);
+      root = x1 > x2 ? x2 : x1;
+    }
+  }
+  if (ok && 0.0f <= root && root <= tmax && root <= capT) {
+    float sa = md + root * nd;
+    sa /= dd;
+    if (0.0f <= sa && sa <= 1.0f) {
+      hit_cap = false;
+      t = root;
+      u = 0.0f;
+      v = sa;
+      return true;
+    }
+  }
+  return hit;
+}
+
+template <bool kAny>
+__global__ void __launch_bounds__(kTraceBlock)
+k_trace_lin(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
+            const BuildParams *__restrict__ params, const srf_ray *__restrict__ rays, uint32_t n,
+            srf_hit *__restrict__ hits, srf_line_hit_ext *__restrict__ ext,
+            const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded, uint32_t max_nodes,
+            uint32_t nprims, uint32_t cap) {
+  __shared__ uint32_t st_node[kTraceStack * kTraceBlock];
+  const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t flags, prev;
+  const RaySetup s = setup_ray(rays, i, flags, prev);
+  const float tlimit = kAny ? fminf(tmax_in ? tmax_in[i] : FLT_MAX, FLT_MAX) : FLT_MAX;
+  float t = FLT_MAX, u = 0.0f, v = 0.0f;
+  uint32_t best = SRF_NO_PRIM, best_q = 0;
+  bool best_cap = false;
+  const BuildParams &P = *params;
+  uint32_t *sn = st_node + threadIdx.x;
+  int sp = 0;
+  uint32_t cur = kDone;
+  {
+    float tm;
+    if (box_span(s, P.root_bmin[0], P.root_bmin[1], P.root_bmin[2], P.root_bmax[0],
+                 P.root_bmax[1], P.root_bmax[2], tm) &&
+        tm <= FLT_MAX)
+      cur = 0;
+  }
+  uint4 meta;
+  while (cur != kDone) {
+    for (;;) {
+      if (cur >= max_nodes) {
+        cur = kDone;
+        break;
+      }
+      const float4 *nd = nodes + 4 * (size_t)cur;
+      const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+      meta = reinterpret_cast<const uint4 *>(nd)[3];
+      keep_loaded(n0, n1, n2);
+      if (meta.x != 0) break;
+      float tm0, tm1;
+      const bool g0 = box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm0) && tm0 <= FLT_MAX;
+      const bool g1 = box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm1) && tm1 <= FLT_MAX;
+      const int nearc = (s.sbits >> meta.y) & 1u;
+      const bool gn = nearc ? g1 : g0, gf = nearc ? g0 : g1;
+      const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+      if (gn) {
+        if (gf && sp < kTraceStack) sn[(sp++) * kTraceBlock] = cf;
+        cur = cn;
+        continue;
+      }
+      if (gf) {
+        cur = cf;
+        continue;
+      }
+      cur = sp > 0 ? sn[(--sp) * kTraceBlock] : kDone;
+      if (cur == kDone) break;
+    }
+    if (cur == kDone) break;
+    // leaf: TestLeafNode, render/render_accel_line.cc:579-633
+    const uint32_t count = meta.z, first = meta.w;
+    for (uint32_t q = first; q < min(first + count, nprims); q++) {
+      const float4 P0 = prims[2 * (size_t)q], P1 = prims[2 * (size_t)q + 1];
+      bool hc;
+      if (line_isect(t, u, v, hc, P0, P1, s, cap != 0)) {
+        if (kAny && t < tlimit) {
+          occluded[i] = 1u;
+          return;
+        }
+        best = __float_as_uint(P0.w);
+        best_q = q;
+        best_cap = hc;
+      }
+    }
+    cur = sp > 0 ? sn[(--sp) * kTraceBlock] : kDone;
+  }
+  if (kAny) {
+    occluded[i] = 0u;
+    return;
+  }
+  if (best == SRF_NO_PRIM) u = v = 0.0f;
+  if (ext) {
+    // BuildIntersection, render/render_accel_line.cc:654-702
+    float4 e0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), e1 = e0;
+    if (best != SRF_NO_PRIM) {
+      const float4 P0 = prims[2 * (size_t)best_q], P1 = prims[2 * (size_t)best_q + 1];
+      const float px = s.o[0] + t * s.d[0], py = s.o[1] + t * s.d[1], pz = s.o[2] + t * s.d[2];
+      const float dx = P1.x - P0.x, dy = P1.y - P0.y, dz = P1.z - P0.z;
+      float nx, ny, nz;
+      if (best_cap) {
+        const float cx = 0.5f * dx + P0.x, cy = 0.5f * dy + P0.y, cz = 0.5f * dz + P0.z;
+        nx = dx;
+        ny = dy;
+        nz = dz;
+        l3norm(nx, ny, nz);
+        if (!(l3dot(px - cx, py - cy, pz - cz, nx, ny, nz) > 0.0f)) {
+          nx = -nx;
+          ny = -ny;
+          nz = -nz;
+        }
+      } else {
+        nx = px - (P0.x + v * dx);
+        ny = py - (P0.y + v * dy);
+        nz = pz - (P0.z + v * dz);
+        l3norm(nx, ny, nz);
+      }
+      e0 = make_float4(px, py, pz, __uint_as_float(best_cap ? 1u : 0u));
+      e1 = make_float4(nx, ny, nz, 0.0f);
+    }
+    reinterpret_cast<float4 *>(ext + i)[0] = e0;
+    reinterpret_cast<float4 *>(ext + i)[1] = e1;
+  }
+  reinterpret_cast<float4 *>(hits)[i] = make_float4(t, u, v, __uint_as_float(best));
+}
+
 // ------------------------------------------------- sorted / persistent any-hit
 
 // Occlusion for rays taken in `order` (order[j] is the j-th ray to trace,
@@ -1568,6 +1797,17 @@ srf_status launch_sol(const Accel *a, const srf_ray *rays, uint64_t n, const flo
   return SRF_OK;
 }
 
+srf_status trace_lin(const Accel *a, const srf_ray *rays, uint64_t n, srf_hit *hits,
+                     srf_line_hit_ext *ext, hipStream_t s) {
+  if (n == 0) return SRF_OK;
+  hipLaunchKernelGGL(k_trace_lin<false>, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
+                     (const float4 *)a->nodes, (const float4 *)a->prims, a->params, rays,
+                     (uint32_t)n, hits, ext, nullptr, nullptr, max_nodes(a), (uint32_t)a->n,
+                     a->line_cap);
+  SRF_HIP(hipGetLastError());
+  return SRF_OK;
+}
+
 srf_status trace_sol(const Accel *a, const srf_ray *rays, uint64_t n, const float *scalars,
                      srf_hit *hits, srf_solid_hit_ext *ext, hipStream_t s) {
   if (n == 0) return SRF_OK;
@@ -1589,6 +1829,14 @@ srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax, uin
                      uint32_t *occluded, hipStream_t s) {
   if (n == 0) return SRF_OK;
   if (a->kind == kSol) return launch_sol<true>(a, rays, n, nullptr, nullptr, nullptr, tmax, occluded, s);
+  if (a->kind == kLin) {
+    hipLaunchKernelGGL(k_trace_lin<true>, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
+                       (const float4 *)a->nodes, (const float4 *)a->prims, a->params, rays,
+                       (uint32_t)n, nullptr, nullptr, tmax, occluded, max_nodes(a),
+                       (uint32_t)a->n, a->line_cap);
+    SRF_HIP(hipGetLastError());
+    return SRF_OK;
+  }
   const dim3 g(trace_blocks(n)), b(kTraceBlock);
   if (a->kind == kTri && small_nodes(a))
     hipLaunchKernelGGL((k_trace_tri<true, true>), g, b, 0, s, (const float4 *)a->nodes,
@@ -1615,7 +1863,7 @@ srf_status trace_any_sorted(Ctx *ctx, const Accel *a, const srf_ray *rays, const
                             uint64_t n, uint32_t key_bits, int mode, uint32_t *occluded,
                             hipStream_t s) {
   if (n == 0) return SRF_OK;
-  if (a->kind == kTet || a->kind == kSol)
+  if (a->kind == kTet || a->kind == kSol || a->kind == kLin)
     return set_error(SRF_ERR_UNSUPPORTED, "sorted any-hit takes triangles or spheres");
   if (key_bits > 30) return set_error(SRF_ERR_INVALID_ARG, "key_bits must be <= 30");
   const uint32_t nn = (uint32_t)n;

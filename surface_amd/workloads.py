@@ -36,6 +36,9 @@ TET_EXT_DTYPE = np.dtype([("d", "<f4", (4,)), ("scalar", "<f4"),
 # the varying sum, and which crossing (0 entry, 1 exit) was reported
 SOL_EXT_DTYPE = np.dtype([("d", "<f4", (8,)), ("pos", "<f4", (3,)), ("scalar", "<f4"),
                           ("nrm", "<f4", (3,)), ("side", "<u4")])
+# srf_line_hit_ext: position (org + t dir), cap flag, normal
+LIN_EXT_DTYPE = np.dtype([("pos", "<f4", (3,)), ("hit_cap", "<u4"), ("nrm", "<f4", (3,)),
+                          ("pad", "<f4")])
 NO_PRIM = np.uint32(0xFFFFFFFF)
 
 
@@ -175,6 +178,35 @@ def solid_rays(cells=32, width=256, height=256, tiled=True):
     c = float(cells)
     return pinhole_rays((1.6 * c, -0.7 * c, 1.3 * c), (0.5 * c, 0.5 * c, 0.45 * c), (0, 0, 1),
                         45.0, width, height, tiled=tiled)
+
+
+def polylines(n_curves=2000, seg_per_curve=64, seed=BASE_SEED + 11, step=0.02, radius=None,
+              extent=1.0):
+    """Random-walk polylines (streamline / hair-like) in [-extent, extent]^3:
+    positions (float32), segments (uint32, 2 vertex ids per line, joining
+    consecutive vertices of a curve) and, with radius="vary", one radius per
+    vertex (float32), else None."""
+    rng = _rng(seed)
+    start = rng.uniform(-extent, extent, (n_curves, 1, 3))
+    d = rng.normal(size=(n_curves, 3))
+    steps = np.empty((n_curves, seg_per_curve, 3))
+    for k in range(seg_per_curve):
+        d = d + rng.normal(scale=0.6, size=d.shape)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        steps[:, k] = d * step
+    P = np.concatenate([start, start + np.cumsum(steps, axis=1)], axis=1).reshape(-1, 3)
+    base = (np.arange(n_curves) * (seg_per_curve + 1))[:, None] + np.arange(seg_per_curve)[None]
+    segs = np.stack([base, base + 1], axis=-1).reshape(-1, 2)
+    R = None
+    if radius == "vary":
+        R = rng.uniform(0.002, 0.006, len(P)).astype(np.float32)
+    return (np.ascontiguousarray(P.astype(np.float32)),
+            np.ascontiguousarray(segs.astype(np.uint32)), R)
+
+
+def line_rays(width=256, height=256, tiled=True):
+    return pinhole_rays((0.3, -3.2, 0.8), (0.0, 0.0, 0.0), (0, 0, 1), 40.0, width, height,
+                        tiled=tiled)
 
 
 def plummer_spheres(n, seed=BASE_SEED + 5, a=1.0, mu=-6.0, sigma=0.5):

@@ -7,7 +7,7 @@ heightfield (522,242 triangles, 1024^2 rays) and tetra grid (1,572,864 tets,
 1024^2 rays) it stores SHA-256 digests of the tree, the permutation and the
 per-ray hits. Run in the container that has the reference checkout:
 
-    make -C oracle && python tests/golden/gen_golden.py [--full] [--only scene|sol]
+    make -C oracle && python tests/golden/gen_golden.py [--full] [--only scene|sol|lin]
 """
 from __future__ import annotations
 
@@ -135,6 +135,22 @@ def sol_case(ref):
     return out
 
 
+def lin_case(ref):
+    """LineAccel::Build + Traverse on random-walk polylines: constant radius
+    with and without end caps, and per-vertex radii (where the reference's
+    boxes can hide a closer crossing, see tests/test_oracle.py)."""
+    rays = W.line_rays(48, 48)
+    P, S, R = W.polylines(300, 30, radius="vary", seed=W.BASE_SEED + 40)
+    out = {"rays": rays, "P": P, "S": S, "R": R}
+    for name, radius, rconst, cap in (("const", None, 0.004, True), ("nocap", None, 0.004, False),
+                                      ("vary", R, 0.0, True)):
+        tree = ref.build("lin", P, S, R=radius, rconst=rconst, keep=True, cap=cap)
+        hits, ext = ref.trace(tree, rays)
+        ref.free(tree)
+        out.update({f"{name}_hits": hits, f"{name}_pos": ext["pos"], f"{name}_nrm": ext["nrm"]})
+    return out
+
+
 SCENE_ARGS = dict(count=12, lo=1100, hi=6000, seed=W.BASE_SEED + 21)
 
 
@@ -159,7 +175,7 @@ def scene_case(ref):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--full", action="store_true", help="also write full-size digests")
-    ap.add_argument("--only", choices=["scene", "sol"], help="write one fixture only")
+    ap.add_argument("--only", choices=["scene", "sol", "lin"], help="write one fixture only")
     args = ap.parse_args()
     ref = Reference("serial")
     t0 = time.time()
@@ -169,6 +185,9 @@ def main():
     if args.only == "sol":
         np.savez_compressed(os.path.join(HERE, "sol_small.npz"), **sol_case(ref))
         return
+    if args.only == "lin":
+        np.savez_compressed(os.path.join(HERE, "lin_small.npz"), **lin_case(ref))
+        return
     np.savez_compressed(os.path.join(HERE, "tri_small.npz"), **tri_case(ref, 48, 64))
     np.savez_compressed(os.path.join(HERE, "tet_small.npz"), **tet_case(ref, 6, 64))
     np.savez_compressed(os.path.join(HERE, "sph_small.npz"), **sph_case(ref, 6000, 64))
@@ -176,6 +195,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "karras.npz"), **karras_dup_case(ref))
     np.savez_compressed(os.path.join(HERE, "scene_small.npz"), **scene_case(ref))
     np.savez_compressed(os.path.join(HERE, "sol_small.npz"), **sol_case(ref))
+    np.savez_compressed(os.path.join(HERE, "lin_small.npz"), **lin_case(ref))
     print(f"small fixtures written in {time.time() - t0:.1f}s", flush=True)
     if not args.full:
         return

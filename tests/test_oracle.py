@@ -791,3 +791,112 @@ def test_solid_trace_equals_bruteforce(orc, nv):
             q = hits["prim_id"][i]
             st, ts, _, _ = orc.solid_isect(nv, o, d, V[Fr[q]].astype(np.float64))
             assert st == 1 and best in ts
+
+
+# ------------------------------------------------------------------ lines
+
+def _line_bruteforce(orc, P, S, R, rconst, o, d, cap=True):
+    """Closest CylinderIsect crossing over every line (running t as
+    TestLeafNode keeps it): (t, line) or (FLT_MAX, -1)."""
+    best, bi = np.float32(np.finfo(np.float32).max), -1
+    for li in range(len(S)):
+        a, b = S[li]
+        rr = max(R[a], R[b]) if R is not None else rconst
+        h, out, _ = orc.line_isect(P[a], P[b], rr, o, d, cap, best)
+        if h:
+            best, bi = out[0], li
+    return best, bi
+
+
+@pytest.mark.parametrize("case", ["const", "nocap", "vary"])
+def test_line_trace_matches_reference_fixture(orc, case):
+    # constant radius: hits, u, v, position and normal bit-exact with the
+    # reference's LineAccel; per-vertex radii: equal except where the
+    # reference's boxes (p0 +- r0, p1 +- r1) hide a crossing of its max(r0, r1)
+    # cylinder, and there ours is the brute-force closest
+    g = gold("lin_small.npz")
+    P, S, rays = g["P"], g["S"], g["rays"]
+    R = g["R"] if case == "vary" else None
+    rconst = 0.0 if case == "vary" else 0.004
+    t = orc.build_lin(P, S, R, rconst)
+    _tree_invariants(t["nodes"], t["indices"], len(S), 4, False)
+    hits, ext = orc.trace_lin(t, P, S, R, rconst, rays, cap=case != "nocap")
+    want = g[f"{case}_hits"]
+    same = hits.tobytes() == want.tobytes()
+    if case != "vary":
+        assert same
+        hit = want["prim_id"] != W.NO_PRIM
+        assert hit.mean() > 0.1
+        assert np.array_equal(ext["pos"][hit], g[f"{case}_pos"][hit])
+        assert np.array_equal(ext["nrm"][hit], g[f"{case}_nrm"][hit])
+        return
+    diff = np.nonzero((hits["t"] != want["t"]) | (hits["prim_id"] != want["prim_id"]))[0]
+    assert len(diff) < 0.01 * len(rays)
+    for i in diff:
+        bt, bi = _line_bruteforce(orc, P, S, R, 0.0, rays["org"][i], rays["dir"][i])
+        assert hits["t"][i] == bt and hits["prim_id"][i] == bi and bt < want["t"][i]
+
+
+@needs_ref
+def test_line_oracle_equals_live_reference(orc):
+    ref = Reference("serial")
+    P, S, _ = W.polylines(800, 48, seed=W.BASE_SEED + 41, step=0.015)
+    t = orc.build_lin(P, S, None, 0.003)
+    r = ref.build("lin", P, S, R=None, rconst=0.003, keep=True)
+    rays = W.line_rays(96, 96)
+    # plus rays from inside the polyline cloud, some axis-parallel. Only unit
+    # directions: CylinderIsect measures a cap crossing along the normalised
+    # direction but the side and the box culling along the given one, so for
+    # other lengths the reference's answer depends on its tree's visit order
+    # (and short directions make its float quadratic report crossings no
+    # cylinder has)
+    extra = rays[:2000].copy()
+    rng = np.random.default_rng(7)
+    extra["org"] = rng.uniform(-1.2, 1.2, (2000, 3)).astype(np.float32)
+    dd = rng.normal(size=(2000, 3))
+    dd[::6, 2] = 0.0
+    dd /= np.linalg.norm(dd, axis=1, keepdims=True)
+    extra["dir"] = dd.astype(np.float32)
+    rays = np.concatenate([rays, extra])
+    h1, e1 = orc.trace_lin(t, P, S, None, 0.003, rays, threads=4)
+    h2, e2 = ref.trace(r, rays, threads=4)
+    ref.free(r)
+    assert (h2["prim_id"] != W.NO_PRIM).mean() > 0.1
+    assert h1.tobytes() == h2.tobytes()
+    hit = h2["prim_id"] != W.NO_PRIM
+    assert np.array_equal(e1["pos"][hit], e2["pos"][hit])
+    assert np.array_equal(e1["nrm"][hit], e2["nrm"][hit])
+
+
+def test_line_isect_analytic(orc):
+    # unit-radius cylinder along z from 0 to 4; rays along +x at height 2
+    # meet the side at x = -1 (t = 4 from x = -5), rays along +z from below
+    # meet the lower cap at t = 1 (u = distance from the axis, v = 0)
+    p0, p1 = np.float32([0, 0, 0]), np.float32([0, 0, 4])
+    h, out, cap = orc.line_isect(p0, p1, 1.0, [-5, 0, 2], [1, 0, 0])
+    assert h and not cap and out[0] == 4.0 and out[1] == 0.0 and out[2] == 0.5
+    h, out, cap = orc.line_isect(p0, p1, 1.0, [0.5, 0, -1], [0, 0, 1])
+    assert h and cap and out[0] == 1.0 and out[1] == 0.5 and out[2] == 0.0
+    # without caps the same ray starts below the side and never meets it
+    h, out, cap = orc.line_isect(p0, p1, 1.0, [0.5, 0, -1], [0, 0, 1], cap=False)
+    assert not h
+    # from above, the upper cap (v = 1); a miss beside the cylinder; a hit
+    # beyond the running distance is rejected
+    h, out, cap = orc.line_isect(p0, p1, 1.0, [0, 0.25, 9], [0, 0, -1])
+    assert h and cap and out[0] == 5.0 and out[2] == 1.0
+    assert not orc.line_isect(p0, p1, 1.0, [-5, 2, 2], [1, 0, 0])[0]
+    assert not orc.line_isect(p0, p1, 1.0, [-5, 0, 2], [1, 0, 0], tmax=3.5)[0]
+
+
+def test_line_trace_equals_bruteforce(orc):
+    P, S, R = W.polylines(120, 20, seed=W.BASE_SEED + 42, radius="vary")
+    t = orc.build_lin(P, S, R, 0.0)
+    rays = W.line_rays(24, 24)
+    hits, _ = orc.trace_lin(t, P, S, R, 0.0, rays)
+    assert (hits["prim_id"] != W.NO_PRIM).mean() > 0.02
+    for i in range(0, len(rays), 2):
+        bt, bi = _line_bruteforce(orc, P, S, R, 0.0, rays["org"][i], rays["dir"][i])
+        assert hits["t"][i] == bt
+        if bi >= 0:
+            assert hits["prim_id"][i] == bi or _line_bruteforce(
+                orc, P, S[[hits["prim_id"][i]]], R, 0.0, rays["org"][i], rays["dir"][i])[0] == bt
