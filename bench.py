@@ -220,6 +220,51 @@ def particles_ao(torch, S, W):
     return out
 
 
+def solids_lines(torch, S, W):
+    """Hexahedra (64^3 jittered grid, 262,144 solids) and capped lines
+    (16,384 random-walk polylines x 64 = 1,048,576 segments): build and
+    1024^2 closest-hit rays each. The solid search visits every node whose
+    box the ray enters, as SolidAccel::Traverse does."""
+    out = {}
+    log("solids: 64^3 hexahedra")
+    V, F, Sc = W.solid_grid(64, 8)
+    Vd, Fd, Sd = S.device_tensor(V), S.device_tensor(F), S.device_tensor(Sc)
+    rays = S.device_tensor(W.solid_rays(64, 1024, 1024))
+    n = rays.shape[0]
+    hits = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    ext = torch.empty((n, 16), dtype=torch.float32, device="cuda")
+    a = S.SolidAccel()
+    a.Build32(Vd, Fd, 8)
+    a.TraverseSolid(rays, Sd, hits, ext)
+    tb = timed_events(torch, lambda: a.Build32(Vd, Fd, 8), 5)
+    tt = timed_events(torch, lambda: a.TraverseSolid(rays, Sd, hits, ext), 3)
+    hit = (hits[:, 3].view(torch.int32) != -1).float().mean().item()
+    out["hexahedra_262144"] = {"build_ms": round(tb, 4),
+                               "build_mprims_per_s": round(len(F) / tb / 1e3, 1),
+                               "trace_ms": round(tt, 4),
+                               "trace_mrays_per_s": round(n / tt / 1e3, 1),
+                               "hit_fraction": round(hit, 4)}
+    del a
+    log("lines: 1M capped segments")
+    P, segs, _ = W.polylines(16384, 64, seed=W.BASE_SEED + 46, step=0.01)
+    Pd, Sgd = S.device_tensor(P), S.device_tensor(segs)
+    lr = S.device_tensor(W.line_rays(1024, 1024))
+    lh = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    le = torch.empty((n, 8), dtype=torch.float32, device="cuda")
+    b = S.LineAccel()
+    b.Build(Pd, Sgd, None, 0.0015)
+    b.TraverseLines(lr, lh, le)
+    tb = timed_events(torch, lambda: b.Build(Pd, Sgd, None, 0.0015), 5)
+    tt = timed_events(torch, lambda: b.TraverseLines(lr, lh, le), 3)
+    hit = (lh[:, 3].view(torch.int32) != -1).float().mean().item()
+    out["lines_1048576"] = {"build_ms": round(tb, 4),
+                            "build_mprims_per_s": round(len(segs) / tb / 1e3, 1),
+                            "trace_ms": round(tt, 4),
+                            "trace_mrays_per_s": round(n / tt / 1e3, 1),
+                            "hit_fraction": round(hit, 4)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,6 +369,7 @@ def main():
     if not args.no_extras:
         extras["batched_scene_64_meshes_16_cameras"] = batched_scene(torch, S, W)
         extras["particles_16M_ao"] = particles_ao(torch, S, W)
+        extras["solids_and_lines"] = solids_lines(torch, S, W)
 
     cpu = None if args.no_cpu else cpu_baseline(V, F, Sc, rays_np)
 

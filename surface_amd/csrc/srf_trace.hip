@@ -1195,7 +1195,8 @@ k_trace_sol(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   const D3 ro = {s.o[0], s.o[1], s.o[2]}, rd = {s.d[0], s.d[1], s.d[2]};
   const D3 raypc = d3cross(rd, ro);
   // t comes from the crossing point along the last axis with |dir| > 0.1
-  // (:497-501); with none the reference's t is undefined and no solid is hit
+  // (render/render_accel_solid.cc:496-500); with none the reference's t is
+  // undefined and no solid is hit
   int axis = -1;
   for (int j = 0; j < 3; j++)
     if ((double)s.d[j] > 0.1 || (double)s.d[j] < -0.1) axis = j;
