@@ -743,18 +743,19 @@ __device__ __forceinline__ uint32_t level_start(const uint32_t *counts, uint32_t
   return s;
 }
 
+// One pass over level `level`'s `count` branches (stored from in_off in the
+// level-ordered list), threads start .. by stride; the next level's entries
+// are appended after them.
 template <bool M>
-__global__ void __launch_bounds__(256) k_classify_level(uint32_t n_all, int max_leaf,
-                                                       uint32_t level, const uint32_t *kchild,
-                                                       const uint32_t *kleaf,
-                                                       const uint32_t *kother,
-                                                       uint2 *levels, uint32_t *counts,
-                                                       uint32_t *cls, uint32_t *leaf_start,
-                                                       SegView v) {
+__device__ __forceinline__ void classify_pass(uint32_t n_all, int max_leaf, uint32_t level,
+                                              uint32_t count, uint32_t in_off,
+                                              const uint32_t *kchild, const uint32_t *kleaf,
+                                              const uint32_t *kother, uint2 *levels,
+                                              uint32_t *counts, uint32_t *cls,
+                                              uint32_t *leaf_start, SegView v, uint32_t start,
+                                              uint32_t stride) {
   // this level's branches and, after them, the next level's: entries are
   // (Karras index, turns | mesh << 8)
-  const uint32_t count = counts[level];
-  const uint32_t in_off = level_start(counts, level);
   const uint2 *front_in = levels + in_off;
   uint2 *front_out = levels + in_off + count;
   const uint32_t out_cap = n_all - in_off - count;
@@ -762,8 +763,7 @@ __global__ void __launch_bounds__(256) k_classify_level(uint32_t n_all, int max_
   const bool deep = level > (uint32_t)kMaxDepth32;
   // whole wavefronts step through the frontier, so the appends below can be
   // counted with one atomic per wavefront
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t j0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); j0 < count; j0 += stride) {
+  for (uint32_t j0 = start + (threadIdx.x & ~63u); j0 < count; j0 += stride) {
     const uint32_t j = j0 + lane;
     uint32_t push_l = 0, push_r = 0, cl = 0, cr = 0, turns = 0;
     BuildParams *params = nullptr;
@@ -818,6 +818,122 @@ __global__ void __launch_bounds__(256) k_classify_level(uint32_t n_all, int max_
     const uint32_t orr = base_out + (uint32_t)__popcll(ml) + (uint32_t)__popcll(mr & below);
     if (push_l && ol < out_cap) front_out[ol] = make_uint2(cl, turns | (m << 8));
     if (push_r && orr < out_cap) front_out[orr] = make_uint2(cr, (turns + 1) | (m << 8));
+  }
+}
+
+template <bool M>
+__global__ void __launch_bounds__(256) k_classify_level(uint32_t n_all, int max_leaf,
+                                                       uint32_t level, const uint32_t *kchild,
+                                                       const uint32_t *kleaf,
+                                                       const uint32_t *kother,
+                                                       uint2 *levels, uint32_t *counts,
+                                                       uint32_t *cls, uint32_t *leaf_start,
+                                                       SegView v) {
+  classify_pass<M>(n_all, max_leaf, level, counts[level], level_start(counts, level), kchild,
+                   kleaf, kother, levels, counts, cls, leaf_start, v, blockIdx.x * blockDim.x,
+                   gridDim.x * blockDim.x);
+}
+
+__device__ __forceinline__ uint32_t count_load(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Levels 0 .. top-1 in one workgroup of 1024 threads. The host picks top so
+// that none of these levels holds more than 1024 branches (level L holds at
+// most nseg * 2^L): one branch per thread, the level's list kept in LDS and
+// counted with LDS atomics, so a level costs one round of Karras-link loads
+// instead of a launch. The lists and counts still go to global memory for
+// the later levels and the refit.
+template <bool M>
+__global__ void __launch_bounds__(1024) k_classify_top(uint32_t n_all, int max_leaf, uint32_t top,
+                                                      const uint32_t *kchild,
+                                                      const uint32_t *kleaf,
+                                                      const uint32_t *kother, uint2 *levels,
+                                                      uint32_t *counts, uint32_t *cls,
+                                                      uint32_t *leaf_start, SegView v) {
+  __shared__ uint2 buf[2][1024];
+  __shared__ uint32_t cnt[2];
+  const uint32_t lane = threadIdx.x & 63u, j = threadIdx.x;
+  uint32_t count = count_load(&counts[0]);  // the roots
+  if (j < count && j < 1024u) buf[0][j] = levels[j];
+  if (j < 2) cnt[j] = 0;
+  __syncthreads();
+  uint32_t in_off = 0;
+  for (uint32_t level = 0; level < top; level++) {
+    const int cur = level & 1, nxt = cur ^ 1;
+    const bool last = level + 1 == top;
+    const bool deep = level > (uint32_t)kMaxDepth32;
+    const uint32_t out_cap = n_all - in_off - count;
+    uint32_t push_l = 0, push_r = 0, cl = 0, cr = 0, turns = 0, m = 0;
+    BuildParams *params = nullptr;
+    if (j < count) {
+      const uint2 e = buf[cur][j];
+      const uint32_t x = e.x;
+      turns = e.y & 255u;
+      m = e.y >> 8;
+      const uint32_t base = seg_off<M>(v.prim_off, m);
+      const uint32_t n = M ? v.prim_off[m + 1] - base : n_all;
+      const uint32_t i = x - base;
+      params = seg_get<M>(v, m).params;
+      if (i + 1 < n) {
+        cls[x] = 1u | (level << 1) | (turns << 8);
+        const uint32_t jo = kother[x];
+        const uint32_t l = i < jo ? i : jo, r = i < jo ? jo : i;
+        const uint32_t g = kchild[x], lb = kleaf[x];
+        const bool left_leaf = (lb & 1u) || (g - l <= (uint32_t)max_leaf) || deep;
+        const bool right_leaf = (lb & 2u) || (r - (g + 1) <= (uint32_t)max_leaf) || deep;
+        if (left_leaf)
+          leaf_start[base + l] = 1;
+        else
+          push_l = 1, cl = base + g;
+        if (right_leaf)
+          leaf_start[base + g + 1] = 1;
+        else
+          push_r = 1, cr = base + g + 1;
+      } else {
+        atomicOr(&params->error, 8u);  // a leaf handed over as a branch: never expected
+      }
+    }
+    {
+      const bool has = params != nullptr;
+      uint64_t pending = __ballot(has);
+      while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint64_t pv = (uint64_t)__shfl((long long)(uintptr_t)params, leader, 64);
+        const bool mine = has && (uint64_t)(uintptr_t)params == pv;
+        if ((int)lane == leader) atomicMax(&params->max_steps, level);
+        pending &= ~__ballot(mine);
+      }
+    }
+    const uint64_t ml = __ballot(push_l), mr = __ballot(push_r);
+    const uint32_t total = (uint32_t)(__popcll(ml) + __popcll(mr));
+    uint32_t base_out = 0;
+    if (lane == 0 && total) base_out = atomicAdd(&cnt[nxt], total);
+    base_out = __shfl(base_out, 0, 64);
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const uint32_t ol = base_out + (uint32_t)__popcll(ml & below);
+    const uint32_t orr = base_out + (uint32_t)__popcll(ml) + (uint32_t)__popcll(mr & below);
+    uint2 *front_out = levels + in_off + count;
+    if (push_l) {
+      const uint2 c = make_uint2(cl, turns | (m << 8));
+      if (ol < out_cap) front_out[ol] = c;
+      if (!last && ol < 1024u) buf[nxt][ol] = c;
+    }
+    if (push_r) {
+      const uint2 c = make_uint2(cr, (turns + 1) | (m << 8));
+      if (orr < out_cap) front_out[orr] = c;
+      if (!last && orr < 1024u) buf[nxt][orr] = c;
+    }
+    __syncthreads();
+    const uint32_t next = cnt[nxt];
+    in_off += count;
+    count = next;
+    __syncthreads();
+    if (j == 0) {
+      cnt[cur] = 0;
+      counts[level + 1] = next;
+    }
+    __syncthreads();
   }
 }
 
@@ -1345,16 +1461,13 @@ __global__ void __launch_bounds__(256) k_sub_link(float4 *nodes, uint32_t nnodes
 // their boxes (deeper levels ran in earlier launches) merges them,
 // MergeBoundingBox(left, right) (render/render_accel_triangle.cc:804-814),
 // into its parent's slot; the root's merge is BoundingBox().
+// One pass over level `level`'s `count` branches (list), threads start ..
+// by stride.
 template <bool M>
-__global__ void __launch_bounds__(256) k_refit_level(uint32_t n_all, uint32_t level,
-                                                    const uint2 *levels,
-                                                    const uint32_t *counts, NodeOut o,
-                                                    SegView v) {
-  // the branches of this level, as the classification listed them
-  const uint32_t count = counts[level];
-  const uint2 *list = levels + level_start(counts, level);
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count;
-       j += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void refit_pass(uint32_t n_all, uint32_t count, const uint2 *list,
+                                           NodeOut o, SegView v, uint32_t start,
+                                           uint32_t stride) {
+  for (uint32_t j = start + threadIdx.x; j < count; j += stride) {
     const uint2 e = list[j];
     const uint32_t x = e.x, m = e.y >> 8;
     if (x >= n_all || (M && m >= v.nseg)) continue;
@@ -1390,6 +1503,98 @@ __global__ void __launch_bounds__(256) k_refit_level(uint32_t n_all, uint32_t le
     }
     store_box_slot(reinterpret_cast<float *>(sg.nodes) + 16 * (size_t)p, (int)(link & 1u), mb);
   }
+}
+
+template <bool M>
+__global__ void __launch_bounds__(256) k_refit_level(uint32_t n_all, uint32_t level,
+                                                    const uint2 *levels,
+                                                    const uint32_t *counts, NodeOut o,
+                                                    SegView v) {
+  // the branches of this level, as the classification listed them
+  refit_pass<M>(n_all, counts[level], levels + level_start(counts, level), o, v,
+                blockIdx.x * blockDim.x, gridDim.x * blockDim.x);
+}
+
+// Levels top-1 .. 0 in one workgroup (the counterpart of k_classify_top):
+// every branch's node position and parent link are loaded into LDS first,
+// then each level costs one read of the branch's node and one store into
+// its parent; the barrier orders a level's stores before the next level's
+// reads.
+template <bool M>
+__global__ void __launch_bounds__(1024) k_refit_top(uint32_t n_all, uint32_t top,
+                                                   const uint2 *levels, const uint32_t *counts,
+                                                   NodeOut o, SegView v) {
+  __shared__ uint32_t s_pos[2048], s_link[2048], s_m[2048];
+  uint32_t total = 0;
+  for (uint32_t l = 0; l < top; l++) total += counts[l];
+  if (total > 2048u) total = 2048u;  // top is chosen so that this never clips
+  for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) {
+    const uint2 e = levels[k];
+    const uint32_t x = e.x, m = e.y >> 8;
+    uint32_t pos = 0xffffffffu, link = 0xffffffffu;
+    if (x < n_all && !(M && m >= v.nseg)) {
+      const uint32_t base = seg_off<M>(v.prim_off, m);
+      const uint32_t n = M ? v.prim_off[m + 1] - base : n_all;
+      if (x >= base && x - base + 1 < n) {
+        const Seg sg = seg_get<M>(v, m);
+        pos = o.branch_pos[x];
+        if (pos >= 2 * n - 1) {
+          atomicOr(&sg.params->error, 16u);
+          pos = 0xffffffffu;
+        } else if (pos != 0) {
+          link = o.parent_link[2 * (size_t)base + pos];
+          if ((link >> 1) >= 2 * n - 1) {
+            atomicOr(&sg.params->error, 16u);
+            pos = 0xffffffffu;
+          }
+        }
+      }
+    }
+    s_pos[k] = pos;
+    s_link[k] = link;
+    s_m[k] = m;
+  }
+  __syncthreads();
+  uint32_t end = total;
+  for (int level = (int)top - 1; level >= 0; level--) {
+    const uint32_t count = counts[level];
+    const uint32_t begin = end - count;
+    for (uint32_t k = begin + threadIdx.x; k < end; k += blockDim.x) {
+      const uint32_t pos = s_pos[k];
+      if (pos == 0xffffffffu) continue;
+      const Seg sg = seg_get<M>(v, s_m[k]);
+      const float4 *w4 = reinterpret_cast<const float4 *>(sg.nodes) + 4 * (size_t)pos;
+      const float4 a = w4[0], b = w4[1], cc = w4[2];
+      Box l, r;
+      l.mn[0] = a.x; l.mn[1] = a.y; l.mn[2] = a.z;
+      r.mn[0] = a.w; r.mn[1] = b.x; r.mn[2] = b.y;
+      l.mx[0] = b.z; l.mx[1] = b.w; l.mx[2] = cc.x;
+      r.mx[0] = cc.y; r.mx[1] = cc.z; r.mx[2] = cc.w;
+      const Box mb = merge_lr(l, r);
+      if (pos == 0) {
+        for (int q = 0; q < 3; q++) {
+          sg.params->root_bmin[q] = mb.mn[q];
+          sg.params->root_bmax[q] = mb.mx[q];
+        }
+        continue;
+      }
+      const uint32_t link = s_link[k];
+      store_box_slot(reinterpret_cast<float *>(sg.nodes) + 16 * (size_t)(link >> 1),
+                     (int)(link & 1u), mb);
+    }
+    end = begin;
+    __syncthreads();
+  }
+}
+
+// Levels 0 .. top-1 hold at most nseg * 2^level branches each (a branch has
+// two children); those of up to 1024 branches (one per thread of a
+// 1024-thread workgroup; 2047 in all) are classified and refitted by a
+// single workgroup.
+inline uint32_t top_levels(uint32_t nseg) {
+  uint32_t top = 0;
+  while (top < (uint32_t)kMaxBranchLevels && ((uint64_t)nseg << top) <= 1024u) top++;
+  return top;
 }
 
 inline uint32_t blocks_for(uint64_t n, int block, uint32_t cap = 0x7fffffffu) {
@@ -1657,8 +1862,14 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
                      0u, (size_t)N);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, b.level_counts, 0u, (size_t)64);
   SRF_DISPATCH_M(k_classify_roots, dim3(1), dim3(64), 0, stream, v, b.levels, b.level_counts);
+  // the first levels, which hold at most nseg * 2^level branches each, in
+  // one workgroup; the rest one launch per level
+  const uint32_t top = top_levels(nseg);
+  if (top > 0)
+    SRF_DISPATCH_M(k_classify_top, dim3(1), dim3(1024), 0, stream, N, max_leaf, top, b.kchild,
+                   b.kleaf, b.kother, b.levels, b.level_counts, b.cls, b.leaf_start, v);
   const uint32_t cblocks = blocks_for(N, 256, 1024);
-  for (uint32_t level = 0; level < (uint32_t)kMaxBranchLevels; level++)
+  for (uint32_t level = top; level < (uint32_t)kMaxBranchLevels; level++)
     SRF_DISPATCH_M(k_classify_level, dim3(cblocks), dim3(256), 0, stream, N, max_leaf, level,
                    b.kchild, b.kleaf, b.kother, b.levels, b.level_counts, b.cls, b.leaf_start,
                    v);
@@ -1681,11 +1892,15 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
                b.big, b.level_counts + 32, v);
   SRF_DISPATCH(k_leaves_big, dim3(blocks_for(N, 256 * (int)kBigLeaf, 1024)), dim3(256), 0, stream,
                b.ss.vals_a, b.o, b.big, b.level_counts + 32, v);
-  // bottom-up refit, one level per launch (levels are branch steps 23..0)
+  // bottom-up refit, one level per launch (levels are branch steps 23..top),
+  // then levels top-1..0 in one workgroup
   const uint32_t rblocks = blocks_for(N, 256, 2048);
-  for (int level = kMaxBranchLevels - 1; level >= 0; level--)
+  for (int level = kMaxBranchLevels - 1; level >= (int)top; level--)
     SRF_DISPATCH_M(k_refit_level, dim3(rblocks), dim3(256), 0, stream, N, (uint32_t)level,
                    b.levels, b.level_counts, b.o, v);
+  if (top > 0)
+    SRF_DISPATCH_M(k_refit_top, dim3(1), dim3(1024), 0, stream, N, top, b.levels,
+                   b.level_counts, b.o, v);
   if (big_out && !multi && kind == kSph) {
     hipLaunchKernelGGL(k_big_export, dim3(blocks_for(big_out->big_cap, 256, 1024)), dim3(256), 0,
                        stream, b.big, b.level_counts + 32, b.o, big_out->big_info,
