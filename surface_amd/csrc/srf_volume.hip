@@ -127,6 +127,30 @@ __device__ __forceinline__ float tex_eval(const float *__restrict__ vox, const G
   return lerpf(g.dz, d0, d1);
 }
 
+// The 8 voxels of the last cell a lane sampled: consecutive samples of a ray
+// (half-voxel steps) share a cell about every other step, so the adjoint's
+// replay gathers only when the cell changes. The value is tex_eval's, from
+// the same voxels in the same order.
+struct CellCache {
+  uint32_t a0;  // the cell's first corner (the others follow from it)
+  float d[8];
+};
+
+__device__ __forceinline__ float tex_eval_cached(const float *__restrict__ vox, const Geom &g,
+                                                 CellCache &c) {
+  if (g.a[0] != c.a0) {
+    for (int q = 0; q < 8; q++) c.d[q] = vox[g.a[q]];
+    c.a0 = g.a[0];
+  }
+  float d00 = lerpf(g.dx, c.d[0], c.d[1]);
+  float d10 = lerpf(g.dx, c.d[2], c.d[3]);
+  float d01 = lerpf(g.dx, c.d[4], c.d[5]);
+  float d11 = lerpf(g.dx, c.d[6], c.d[7]);
+  float d0 = lerpf(g.dy, d00, d10);
+  float d1 = lerpf(g.dy, d01, d11);
+  return lerpf(g.dz, d0, d1);
+}
+
 __device__ __forceinline__ bool vol_span(const float o[3], const float d[3], float &tn,
                                          float &tf) {
   tn = 0.0f;
@@ -457,7 +481,7 @@ __device__ __forceinline__ void cell_move(CellAcc &a, int nx0, int nx1, int ny0,
 // down to 0 in lockstep (a lane idles while k is past its own ray), so the
 // lanes of an 8x8 ray tile sample neighbouring voxels at the same time.
 template <bool kHaveCkpt, int W>
-__global__ void __launch_bounds__(kVolBwdBlock)
+__global__ void __launch_bounds__(kVolBwdBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_vol_bwd(Volume V, const float *__restrict__ tf, int N, float step,
           const srf_ray *__restrict__ rays, uint32_t n, const float *__restrict__ ckpt,
           uint32_t slots, const float *__restrict__ grad_rgba, float *__restrict__ grad_vox,
@@ -518,6 +542,8 @@ k_vol_bwd(Volume V, const float *__restrict__ tf, int N, float step,
   float Tnext = 1.0f, Tstart = 1.0f;
   TfAcc tacc;
   tacc.bin = -1;
+  CellCache vc;
+  vc.a0 = 0xffffffffu;
   CellAcc cacc;
   cacc.live = false;
   cacc.x0 = cacc.x1 = cacc.y0 = cacc.y1 = cacc.z0 = cacc.z1 = 0;
@@ -537,7 +563,7 @@ k_vol_bwd(Volume V, const float *__restrict__ tf, int N, float step,
       const float pu = o[0] + t * d[0], pv = o[1] + t * d[1], pr = o[2] + t * d[2];
       Geom g;
       tex_geom(V, pu, pv, pr, g);
-      const float s = tex_eval(V.vox, g);
+      const float s = tex_eval_cached(V.vox, g, vc);
       int i0;
       float f;
       bool inside;
