@@ -707,11 +707,22 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             srf_hit *__restrict__ hits, srf_tet_hit_ext *__restrict__ ext,
             const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded,
             uint32_t max_nodes, uint32_t nprims) {
+  // Closest hit in two passes over the same tree. Pass A visits the nearer
+  // child first by entry distance and finds the closest crossing's distance
+  // tA with little work. Pass B then runs the reference's own traversal
+  // (near child by split axis, box test at pop, TestLeafNode's acceptance)
+  // from a running distance T0 = tA (1 + 1e-5) + slack instead of FLT_MAX.
+  // The reference's sequence of accepted tetras ends below T0; up to its
+  // first acceptance below T0 it visits a superset of pass B's nodes and
+  // accepts nothing pass B needs (what it accepts lies above T0, and a node
+  // pass B culls has its entry beyond T0, far above float/double rounding of
+  // any crossing inside it); from that acceptance on both hold the same
+  // distance and make the same decisions, so hit, t, u, v, weights and tie
+  // winner are the reference's. Occlusion (kAny) keeps the single pass.
+  //
   // pending far children as (child, entry distance): the popped child is
   // checked against the current hit distance without reloading its parent
-  // (24 KB of LDS per block holds the kernel at 3 wavefronts per SIMD, which
-  // measured faster than 4 with a parent reload per pop: the traversal is
-  // bound by the vector memory path, not by latency)
+  // (24 KB of LDS per block holds the kernel at 3 wavefronts per SIMD)
   __shared__ uint2 st_node[kTraceStack * kTraceBlock];
   const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
@@ -744,6 +755,147 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
     }
     return kDone;
   };
+  // the Pluecker test of one tetra against the running distance tt; returns
+  // true when it is accepted (TestLeafNode's rule), with its outputs
+  auto test_tet = [&](uint32_t q, float tt, int &ef, double &eu, double &lu, double &et,
+                      double &lt) -> bool {
+    const float4 *Pp = prims + 4 * (size_t)q;
+    const float4 p0 = Pp[0], p1 = Pp[1], p2 = Pp[2], p3 = Pp[3];  // one round trip
+    keep_loaded(p1, p2, p3);
+    const uint32_t id = __float_as_uint(p0.w);
+    if (id == prev) return false;
+    {
+      const float bx0 = fminf(fminf(p0.x, p1.x), fminf(p2.x, p3.x)) - pad;
+      const float by0 = fminf(fminf(p0.y, p1.y), fminf(p2.y, p3.y)) - pad;
+      const float bz0 = fminf(fminf(p0.z, p1.z), fminf(p2.z, p3.z)) - pad;
+      const float bx1 = fmaxf(fmaxf(p0.x, p1.x), fmaxf(p2.x, p3.x)) + pad;
+      const float by1 = fmaxf(fmaxf(p0.y, p1.y), fmaxf(p2.y, p3.y)) + pad;
+      const float bz1 = fmaxf(fmaxf(p0.z, p1.z), fmaxf(p2.z, p3.z)) + pad;
+      float tm;
+      if (!box_span(s, bx0, by0, bz0, bx1, by1, bz1, tm)) return false;
+      if (tm > tt + 1e-5f * tt + tslack) return false;
+    }
+    const D3 vtx[4] = {{p0.x, p0.y, p0.z}, {p1.x, p1.y, p1.z}, {p2.x, p2.y, p2.z},
+                       {p3.x, p3.y, p3.z}};
+    et = 0.0;
+    lt = 0.0;
+    if (!ray_tet(ro, rd, plRay, vtx, ef, eu, lu, et, lt)) return false;
+    return (et >= 0.0) && (et < (double)tt);
+  };
+  // root box (TetraAccel::Traverse pops node 0 first)
+  float troot;
+  const bool root_hit = box_span(s, P.root_bmin[0], P.root_bmin[1], P.root_bmin[2],
+                                 P.root_bmax[0], P.root_bmax[1], P.root_bmax[2], troot);
+  if (!kAny && root_hit && troot <= t) {
+    // pass A: nearer child by entry distance, to find the closest crossing's
+    // distance tA with little work
+    float tA = FLT_MAX;
+    uint32_t cur = 0;
+    sp = 0;
+    while (cur != kDone) {
+      for (;;) {
+        if (cur >= max_nodes) {
+          cur = kDone;
+          break;
+        }
+        const float4 *nd = nodes + 4 * (size_t)cur;
+        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+        const uint4 meta = reinterpret_cast<const uint4 *>(nd)[3];
+        keep_loaded(n0, n1, n2);
+        if (meta.x != 0) {
+          for (uint32_t q = meta.w; q < min(meta.w + meta.z, nprims); q++) {
+            int ef;
+            double eu, lu, et, lt;
+            if (test_tet(q, tA, ef, eu, lu, et, lt)) tA = (float)et;
+          }
+          break;
+        }
+        float tm0, tm1;
+        const bool g0 = box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm0) && tm0 <= tA;
+        const bool g1 = box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm1) && tm1 <= tA;
+        if (g0 && g1) {
+          const bool first0 = tm0 <= tm1;
+          if (sp < kTraceStack)
+            sn[(sp++) * kTraceBlock] =
+                make_uint2(first0 ? meta.w : meta.z, __float_as_uint(first0 ? tm1 : tm0));
+          cur = first0 ? meta.z : meta.w;
+          continue;
+        }
+        if (g0 || g1) {
+          cur = g0 ? meta.z : meta.w;
+          continue;
+        }
+        break;
+      }
+      cur = kDone;
+      while (sp > 0) {
+        const uint2 e = sn[(--sp) * kTraceBlock];
+        if (__uint_as_float(e.y) <= tA) {
+          cur = e.x;
+          break;
+        }
+      }
+    }
+    // pass B: the reference's traversal and acceptance rule from a running
+    // distance just above tA instead of FLT_MAX. Should it accept nothing,
+    // the reference's answer lies beyond T0 (below it, the reference would
+    // have visited and accepted the same tetra), which only rounding far
+    // outside the margin could cause; the traversal from FLT_MAX settles it.
+    for (float t0 = tA + 1e-5f * tA + tslack; tA < FLT_MAX; t0 = FLT_MAX) {
+      t = t0;
+      sp = 0;
+      cur = troot <= t ? 0u : kDone;
+      uint4 meta;
+      while (cur != kDone) {
+        for (;;) {
+          if (cur >= max_nodes) {
+            cur = kDone;
+            break;
+          }
+          const float4 *nd = nodes + 4 * (size_t)cur;
+          const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+          meta = reinterpret_cast<const uint4 *>(nd)[3];
+          keep_loaded(n0, n1, n2);
+          if (meta.x != 0) break;
+          float tm0, tm1;
+          const bool g0 = box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm0);
+          const bool g1 = box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm1);
+          const int nearc = (s.sbits >> meta.y) & 1u;
+          const bool gn = nearc ? g1 : g0, gf = nearc ? g0 : g1;
+          const float tn = nearc ? tm1 : tm0, tf = nearc ? tm0 : tm1;
+          const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+          if (gn && tn <= t) {
+            if (gf && sp < kTraceStack) sn[(sp++) * kTraceBlock] = make_uint2(cf, __float_as_uint(tf));
+            cur = cn;
+            continue;
+          }
+          if (gf && tf <= t) {
+            cur = cf;
+            continue;
+          }
+          cur = pop_next();
+          if (cur == kDone) break;
+        }
+        if (cur == kDone) break;
+        for (uint32_t q = meta.w; q < min(meta.w + meta.z, nprims); q++) {
+          int ef;
+          double eu, lu, et, lt;
+          if (test_tet(q, t, ef, eu, lu, et, lt)) {
+            t = (float)et;
+            u = (float)eu;
+            v = (float)lu;
+            best = q;
+            subface = (uint32_t)ef;
+            tleave = lt;
+          }
+        }
+        cur = pop_next();
+      }
+      if (best != SRF_NO_PRIM || t0 == FLT_MAX) break;
+    }
+    if (best == SRF_NO_PRIM) t = FLT_MAX;
+  }
+  if (kAny) {
   uint32_t cur = kDone;
   {
     // the root's own box is tested first (TetraAccel::Traverse pops node 0)
@@ -830,6 +982,7 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
       }
     }
     cur = pop_next();
+  }
   }
   if (kAny) {
     occluded[i] = 0u;
