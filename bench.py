@@ -13,9 +13,10 @@ rays) and the differentiable volume (256^3, 1024^2 rays, fwd and fwd+bwd ms).
 
 roofline: the dominant kernel (the tetra traversal) against MI355X's 8 TB/s
 HBM peak, with algorithmic bytes per ray = 40 B (ray in, hit out) + 40 B per
-node visited + 80 B per tetra tested (64 B geometry + 16 B scalars), the node
-and tetra counts being the reference traversal's own on these rays (measured
-by the oracle and frozen below). cpu_baseline: the reference's own OpenMP code
+node loaded + 80 B per tetra tested (64 B geometry + 16 B scalars), the node
+and tetra counts being the kernel's own on these rays (both passes, counted
+once and frozen below; the reference's single traversal loads about three
+times as many, reported beside them). cpu_baseline: the reference's own OpenMP code
 (oracle/_ref/libsrfref_omp.so, built from the reference sources) on a bounded
 sample of the same rays, on this host's cores.
 """
@@ -35,16 +36,27 @@ sys.path.insert(0, ROOT)
 
 METRIC = "closest-hit Mrays/s + LBVH build Mprims/s; volume fwd+bwd ms vs plain fwd"
 HBM_PEAK_GBS = 8000.0
-# Reference traversal counts on the bench rays (oracle, stats=True):
-TET_NODES_PER_RAY = 180.0010   # tet_grid(64), tet_rays(64, 1024, 1024)
-TET_TESTS_PER_RAY = 60.3673
-TRI_NODES_PER_RAY = 64.3038    # heightfield(512), heightfield_rays(1024, 1024)
+# Work per ray on the bench rays. k_trace_tet runs two passes (nearest
+# child first to find the closest crossing, then the reference's traversal
+# from just above it); its node and tetra loads per ray, counted in an
+# instrumented build of the kernel over tet_grid(64) x tet_rays(64, 1024,
+# 1024): pass A 29.5462 nodes + 11.9270 tetras, pass B 28.5936 + 10.9334.
+TET_NODES_PER_RAY = 29.5462 + 28.5936
+TET_TESTS_PER_RAY = 11.9270 + 10.9334
+# The reference's single traversal on the same rays (oracle, stats=True)
+# loads 180.0010 nodes and 60.3673 tetras per ray.
+TET_REF_NODES_PER_RAY = 180.0010
+TET_REF_TESTS_PER_RAY = 60.3673
+# k_trace_tri follows the reference's traversal exactly (oracle counts on
+# heightfield(512), heightfield_rays(1024, 1024)).
+TRI_NODES_PER_RAY = 64.3038
 TRI_TESTS_PER_RAY = 94.4435
 TET_BYTES_PER_RAY = 40 + 40 * TET_NODES_PER_RAY + 80 * TET_TESTS_PER_RAY
+TET_REF_BYTES_PER_RAY = 40 + 40 * TET_REF_NODES_PER_RAY + 80 * TET_REF_TESTS_PER_RAY
 # HBM bytes per k_trace_tet launch on the bench rays: rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (KiB/dispatch), each scaled by the same counter on a 1 GiB copy
 # (fetch x 2.000, write x 1.000); profiles/traffic_k_trace_tet_tet64_k_trace_tri_heightfield512.md
-TET_TRACE_HBM_BYTES = (2.000 * 64221.8 + 1.000 * 49152.0) * 1024
+TET_TRACE_HBM_BYTES = (2.000 * 26327.25 + 1.000 * 49152.0) * 1024
 TRI_BYTES_PER_RAY = 40 + 64 * TRI_NODES_PER_RAY + 48 * TRI_TESTS_PER_RAY
 
 
@@ -398,7 +410,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(tet_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(tet_gbs / HBM_PEAK_GBS, 4),
                      "traffic": round(TET_TRACE_HBM_BYTES), "kernel": "k_trace_tet",
-                     "bytes_per_launch": round(TET_BYTES_PER_RAY * n_rays)},
+                     "bytes_per_launch": round(TET_BYTES_PER_RAY * n_rays),
+                     "reference_traversal_bytes_per_launch": round(TET_REF_BYTES_PER_RAY * n_rays)},
         "cpu_baseline": cpu,
         "extras": extras,
     }
