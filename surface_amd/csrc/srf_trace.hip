@@ -756,12 +756,15 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
     return kDone;
   };
   // the Pluecker test of one tetra against the running distance tt; returns
-  // true when it is accepted (TestLeafNode's rule), with its outputs
+  // true when it is accepted (TestLeafNode's rule), with its outputs. A
+  // crossing that is not accepted leaves et >= 0; et < 0 otherwise (a tetra
+  // the prefilter skips is not crossed before tt (1 + 1e-5) + tslack).
   auto test_tet = [&](uint32_t q, float tt, int &ef, double &eu, double &lu, double &et,
                       double &lt) -> bool {
     const float4 *Pp = prims + 4 * (size_t)q;
     const float4 p0 = Pp[0], p1 = Pp[1], p2 = Pp[2], p3 = Pp[3];  // one round trip
     keep_loaded(p1, p2, p3);
+    et = -1.0;
     const uint32_t id = __float_as_uint(p0.w);
     if (id == prev) return false;
     {
@@ -779,7 +782,10 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
                        {p3.x, p3.y, p3.z}};
     et = 0.0;
     lt = 0.0;
-    if (!ray_tet(ro, rd, plRay, vtx, ef, eu, lu, et, lt)) return false;
+    if (!ray_tet(ro, rd, plRay, vtx, ef, eu, lu, et, lt)) {
+      et = -1.0;
+      return false;
+    }
     return (et >= 0.0) && (et < (double)tt);
   };
   // root box (TetraAccel::Traverse pops node 0 first)
@@ -788,8 +794,11 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
                                  P.root_bmax[0], P.root_bmax[1], P.root_bmax[2], troot);
   if (!kAny && root_hit && troot <= t) {
     // pass A: nearer child by entry distance, to find the closest crossing's
-    // distance tA with little work
-    float tA = FLT_MAX;
+    // distance tA with little work. s2 is the nearest other crossing seen,
+    // ec the nearest entry of a box left out because it lies beyond the
+    // running distance.
+    float tA = FLT_MAX, s2 = FLT_MAX, ec = FLT_MAX;
+    bool dropped = false;
     uint32_t cur = 0;
     sp = 0;
     while (cur != kDone) {
@@ -806,18 +815,32 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
           for (uint32_t q = meta.w; q < min(meta.w + meta.z, nprims); q++) {
             int ef;
             double eu, lu, et, lt;
-            if (test_tet(q, tA, ef, eu, lu, et, lt)) tA = (float)et;
+            if (test_tet(q, tA, ef, eu, lu, et, lt)) {
+              s2 = fminf(s2, tA);
+              tA = (float)et;
+              u = (float)eu;
+              v = (float)lu;
+              best = q;
+              subface = (uint32_t)ef;
+              tleave = lt;
+            } else if (et >= 0.0) {
+              s2 = fminf(s2, (float)et);
+            }
           }
           break;
         }
         float tm0, tm1;
-        const bool g0 = box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm0) && tm0 <= tA;
-        const bool g1 = box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm1) && tm1 <= tA;
+        const bool b0 = box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm0);
+        const bool b1 = box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm1);
+        const bool g0 = b0 && tm0 <= tA, g1 = b1 && tm1 <= tA;
+        ec = fminf(ec, fminf(b0 && !g0 ? tm0 : FLT_MAX, b1 && !g1 ? tm1 : FLT_MAX));
         if (g0 && g1) {
           const bool first0 = tm0 <= tm1;
           if (sp < kTraceStack)
             sn[(sp++) * kTraceBlock] =
                 make_uint2(first0 ? meta.w : meta.z, __float_as_uint(first0 ? tm1 : tm0));
+          else
+            dropped = true;
           cur = first0 ? meta.z : meta.w;
           continue;
         }
@@ -834,14 +857,33 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
           cur = e.x;
           break;
         }
+        ec = fminf(ec, __uint_as_float(e.y));
       }
+    }
+    // When no other crossing and no box left out lies at or below
+    // Tc = tA (1 + 2e-6) + tslack / 10, pass A's hit is the reference's:
+    // every other tetra is crossed beyond Tc less rounding (a box entry's
+    // float slab arithmetic is within 3 float ulps of its value, the double
+    // Pluecker distance far closer), so the reference's running distance,
+    // the float of an accepted crossing, stays above the hit's distance and
+    // its boxes' entries until it reaches the hit's tetra, which it then
+    // accepts, and nothing after it is accepted. Otherwise pass B settles
+    // the result.
+    const float T0 = tA + 1e-5f * tA + tslack;
+    const float Tc = tA + 2e-6f * tA + 0.1f * tslack;
+    const bool certified = !dropped && s2 > Tc && ec > Tc;
+    if (tA < FLT_MAX && certified) t = tA;
+    if (!certified) {
+      best = SRF_NO_PRIM;
+      u = 0.0f;
+      v = 0.0f;
     }
     // pass B: the reference's traversal and acceptance rule from a running
     // distance just above tA instead of FLT_MAX. Should it accept nothing,
     // the reference's answer lies beyond T0 (below it, the reference would
     // have visited and accepted the same tetra), which only rounding far
     // outside the margin could cause; the traversal from FLT_MAX settles it.
-    for (float t0 = tA + 1e-5f * tA + tslack; tA < FLT_MAX; t0 = FLT_MAX) {
+    for (float t0 = T0; tA < FLT_MAX && !certified; t0 = FLT_MAX) {
       t = t0;
       sp = 0;
       cur = troot <= t ? 0u : kDone;
