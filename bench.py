@@ -14,9 +14,9 @@ rays) and the differentiable volume (256^3, 1024^2 rays, fwd and fwd+bwd ms).
 roofline: the dominant kernel (the tetra traversal) against MI355X's 8 TB/s
 HBM peak, with algorithmic bytes per ray = 40 B (ray in, hit out) + 40 B per
 node loaded + 80 B per tetra tested (64 B geometry + 16 B scalars), the node
-and tetra counts being the kernel's own on these rays (both passes, counted
-once and frozen below; the reference's single traversal loads about three
-times as many, reported beside them). cpu_baseline: the reference's own OpenMP code
+and tetra counts being the kernel's own on these rays (counted once and
+frozen below; the reference's traversal loads about six times as many,
+reported beside them). cpu_baseline: the reference's own OpenMP code
 (oracle/_ref/libsrfref_omp.so, built from the reference sources) on a bounded
 sample of the same rays, on this host's cores.
 """
@@ -36,13 +36,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "closest-hit Mrays/s + LBVH build Mprims/s; volume fwd+bwd ms vs plain fwd"
 HBM_PEAK_GBS = 8000.0
-# Work per ray on the bench rays. k_trace_tet runs two passes (nearest
-# child first to find the closest crossing, then the reference's traversal
-# from just above it); its node and tetra loads per ray, counted in an
-# instrumented build of the kernel over tet_grid(64) x tet_rays(64, 1024,
-# 1024): pass A 29.5462 nodes + 11.9270 tetras, pass B 28.5936 + 10.9334.
-TET_NODES_PER_RAY = 29.5462 + 28.5936
-TET_TESTS_PER_RAY = 11.9270 + 10.9334
+# Work per ray on the bench rays. k_trace_tet's pass A visits the nearer
+# child first to find the closest crossing and certifies it for almost every
+# ray; the rest rerun the reference's traversal from just above it (pass B).
+# Node and tetra loads per ray, counted in an instrumented build of the
+# kernel over tet_grid(64) x tet_rays(64, 1024, 1024): pass A 29.5462 nodes +
+# 11.9270 tetras, pass B 0.0494 + 0.0192.
+TET_NODES_PER_RAY = 29.5462 + 0.0494
+TET_TESTS_PER_RAY = 11.9270 + 0.0192
 # The reference's single traversal on the same rays (oracle, stats=True)
 # loads 180.0010 nodes and 60.3673 tetras per ray.
 TET_REF_NODES_PER_RAY = 180.0010
@@ -56,7 +57,7 @@ TET_REF_BYTES_PER_RAY = 40 + 40 * TET_REF_NODES_PER_RAY + 80 * TET_REF_TESTS_PER
 # HBM bytes per k_trace_tet launch on the bench rays: rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (KiB/dispatch), each scaled by the same counter on a 1 GiB copy
 # (fetch x 2.000, write x 1.000); profiles/traffic_k_trace_tet_tet64_k_trace_tri_heightfield512.md
-TET_TRACE_HBM_BYTES = (2.000 * 26327.25 + 1.000 * 49152.0) * 1024
+TET_TRACE_HBM_BYTES = (2.000 * 26353.69 + 1.000 * 49152.0) * 1024
 TRI_BYTES_PER_RAY = 40 + 64 * TRI_NODES_PER_RAY + 48 * TRI_TESTS_PER_RAY
 
 
