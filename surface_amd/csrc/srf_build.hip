@@ -240,6 +240,44 @@ __device__ __forceinline__ void scene_add(Box &b, const PrimSrc &s, uint32_t i) 
   }
 }
 
+// Triangle and tetra centroids, as the Morton codes use them, kept from the
+// bounds pass so that the code pass reads 12 contiguous bytes per primitive
+// instead of gathering its vertices again (scratch that the sort has not yet
+// claimed; all null for the other kinds).
+struct CenCache {
+  float *x, *y, *z;
+};
+
+// scene_add for triangles and tetras, also giving the centroid expression of
+// CalculateMortonCodes{Triangle,Tetra}Float30 (the same operations as
+// prim_code, so the codes are bit-identical).
+template <int K>
+__device__ __forceinline__ float3 scene_add_cen(Box &b, const PrimSrc &s, uint32_t i) {
+  if (K == kTri) {
+    const float one_third = 1.0f / 3.0f;
+    float3 a = load_v3(s.V, s.F[3 * (size_t)i]);
+    float3 bb = load_v3(s.V, s.F[3 * (size_t)i + 1]);
+    float3 c = load_v3(s.V, s.F[3 * (size_t)i + 2]);
+    box_add(b, a.x, a.y, a.z, FLT_EPSILON * 0.0f);
+    box_add(b, bb.x, bb.y, bb.z, FLT_EPSILON * 0.0f);
+    box_add(b, c.x, c.y, c.z, FLT_EPSILON * 0.0f);
+    return make_float3(one_third * ((a.x + bb.x) + c.x), one_third * ((a.y + bb.y) + c.y),
+                       one_third * ((a.z + bb.z) + c.z));
+  } else {
+    const float q = 1.0f / 4.0f;
+    float3 a = load_v3(s.V, s.F[4 * (size_t)i]);
+    float3 bb = load_v3(s.V, s.F[4 * (size_t)i + 1]);
+    float3 c = load_v3(s.V, s.F[4 * (size_t)i + 2]);
+    float3 d = load_v3(s.V, s.F[4 * (size_t)i + 3]);
+    box_add(b, a.x, a.y, a.z, FLT_EPSILON * 16);
+    box_add(b, bb.x, bb.y, bb.z, FLT_EPSILON * 16);
+    box_add(b, c.x, c.y, c.z, FLT_EPSILON * 16);
+    box_add(b, d.x, d.y, d.z, FLT_EPSILON * 16);
+    return make_float3(q * (((a.x + bb.x) + c.x) + d.x), q * (((a.y + bb.y) + c.y) + d.y),
+                       q * (((a.z + bb.z) + c.z) + d.z));
+  }
+}
+
 __device__ __forceinline__ void box_wave_reduce(Box &b) {
   for (int off = 32; off >= 1; off >>= 1) {
     Box o;
@@ -253,7 +291,8 @@ __device__ __forceinline__ void box_wave_reduce(Box &b) {
 
 template <int K, bool M>
 __global__ void __launch_bounds__(256) k_bounds(SegView v, Box *partials,
-                                               uint32_t *clear, uint32_t nclear) {
+                                               uint32_t *clear, uint32_t nclear,
+                                               CenCache cen) {
   // blocks [bblk_off[m], bblk_off[m+1]) reduce mesh m
   const uint32_t m = seg_find<M>(v.bblk_off, v.nseg, blockIdx.x);
   const uint32_t b0 = seg_off<M>(v.bblk_off, m);
@@ -263,9 +302,20 @@ __global__ void __launch_bounds__(256) k_bounds(SegView v, Box *partials,
   BuildParams *params = sg.params;
   Box b;
   box_init(b);
-  for (uint32_t i = (blockIdx.x - b0) * blockDim.x + threadIdx.x; i < s.n;
-       i += nblk * blockDim.x)
-    scene_add<K>(b, s, i);
+  if ((K == kTri || K == kTet) && cen.x) {
+    const uint32_t base = seg_off<M>(v.prim_off, m);
+    for (uint32_t i = (blockIdx.x - b0) * blockDim.x + threadIdx.x; i < s.n;
+         i += nblk * blockDim.x) {
+      const float3 c = scene_add_cen<K>(b, s, i);
+      cen.x[base + i] = c.x;
+      cen.y[base + i] = c.y;
+      cen.z[base + i] = c.z;
+    }
+  } else {
+    for (uint32_t i = (blockIdx.x - b0) * blockDim.x + threadIdx.x; i < s.n;
+         i += nblk * blockDim.x)
+      scene_add<K>(b, s, i);
+  }
   box_wave_reduce(b);
   __shared__ Box wbox[4];
   __shared__ bool last;
@@ -404,15 +454,18 @@ __device__ __forceinline__ void hist_add(uint32_t *h, uint32_t d, bool valid) {
 }
 
 template <int K, bool M>
-__global__ void __launch_bounds__(256) k_morton(SegView v, uint32_t *codes, uint32_t *hist) {
+__global__ void __launch_bounds__(256) k_morton(SegView v, uint32_t *codes, uint32_t *hist,
+                                               CenCache cen) {
   // blocks [mblk_off[m], mblk_off[m+1]) code mesh m; histograms per mesh
   const uint32_t m = seg_find<M>(v.mblk_off, v.nseg, blockIdx.x);
   const uint32_t b0 = seg_off<M>(v.mblk_off, m);
   const uint32_t nblk = M ? v.mblk_off[m + 1] - b0 : gridDim.x;
   const Seg sg = seg_get<M>(v, m);
   const PrimSrc &s = sg.src;
-  codes += seg_off<M>(v.prim_off, m);
+  const uint32_t pbase = seg_off<M>(v.prim_off, m);
+  codes += pbase;
   hist += 1024 * (size_t)m;
+  const bool cached = (K == kTri || K == kTet) && cen.x;
   __shared__ uint32_t h[4][256];
   for (int i = threadIdx.x; i < 4 * 256; i += blockDim.x) (&h[0][0])[i] = 0;
   __syncthreads();
@@ -424,7 +477,8 @@ __global__ void __launch_bounds__(256) k_morton(SegView v, uint32_t *codes, uint
     const bool valid = i < s.n;
     uint32_t c = 0;
     if (valid) {
-      c = prim_code<K>(s, i, p);
+      c = cached ? morton30(cen.x[pbase + i], cen.y[pbase + i], cen.z[pbase + i], p)
+                 : prim_code<K>(s, i, p);
       codes[i] = c;
     }
     hist_add(h[0], c & 255, valid);
@@ -735,6 +789,11 @@ __global__ void __launch_bounds__(64) k_classify_roots(SegView v, uint2 *front,
   if (threadIdx.x == 0) counts[0] = v.nseg;
 }
 
+// A counter another workgroup may have changed during this launch.
+__device__ __forceinline__ uint32_t count_load(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Start of level `level`'s branches in the level-ordered branch list: the
 // levels are stored one after another, level 0 (the roots) first.
 __device__ __forceinline__ uint32_t level_start(const uint32_t *counts, uint32_t level) {
@@ -744,8 +803,10 @@ __device__ __forceinline__ uint32_t level_start(const uint32_t *counts, uint32_t
 }
 
 // One pass over level `level`'s `count` branches (stored from in_off in the
-// level-ordered list), threads start .. by stride; the next level's entries
-// are appended after them.
+// level-ordered list) by workgroups of kClassifyBlock threads, the workgroup
+// at `start` stepping by `stride`; the next level's entries are appended
+// after them.
+constexpr uint32_t kClassifyBlock = 256;
 template <bool M>
 __device__ __forceinline__ void classify_pass(uint32_t n_all, int max_leaf, uint32_t level,
                                               uint32_t count, uint32_t in_off,
@@ -759,12 +820,14 @@ __device__ __forceinline__ void classify_pass(uint32_t n_all, int max_leaf, uint
   const uint2 *front_in = levels + in_off;
   uint2 *front_out = levels + in_off + count;
   const uint32_t out_cap = n_all - in_off - count;
-  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const bool deep = level > (uint32_t)kMaxDepth32;
-  // whole wavefronts step through the frontier, so the appends below can be
-  // counted with one atomic per wavefront
-  for (uint32_t j0 = start + (threadIdx.x & ~63u); j0 < count; j0 += stride) {
-    const uint32_t j = j0 + lane;
+  // the workgroup steps through the frontier as a whole, so the appends
+  // below take one atomic per workgroup and step (per-wavefront atomics on
+  // the one counter serialised the big levels)
+  __shared__ uint32_t s_tot[kClassifyBlock / 64], s_base;
+  for (uint32_t j0 = start; j0 < count; j0 += stride) {
+    const uint32_t j = j0 + threadIdx.x;
     uint32_t push_l = 0, push_r = 0, cl = 0, cr = 0, turns = 0;
     BuildParams *params = nullptr;
     uint32_t m = 0;
@@ -804,15 +867,25 @@ __device__ __forceinline__ void classify_pass(uint32_t n_all, int max_leaf, uint
         const int leader = __ffsll((long long)pending) - 1;
         const uint64_t pv = (uint64_t)__shfl((long long)(uintptr_t)params, leader, 64);
         const bool mine = has && (uint64_t)(uintptr_t)params == pv;
-        if ((int)lane == leader) atomicMax(&params->max_steps, level);
+        // skip the atomic once another wavefront has raised it this far
+        if ((int)lane == leader && count_load(&params->max_steps) < level)
+          atomicMax(&params->max_steps, level);
         pending &= ~__ballot(mine);
       }
     }
     const uint64_t ml = __ballot(push_l), mr = __ballot(push_r);
     const uint32_t total = (uint32_t)(__popcll(ml) + __popcll(mr));
-    uint32_t base_out = 0;
-    if (lane == 0 && total) base_out = atomicAdd(&counts[level + 1], total);
-    base_out = __shfl(base_out, 0, 64);
+    if (lane == 0) s_tot[wave] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t sum = 0;
+      for (uint32_t w = 0; w < kClassifyBlock / 64; w++) sum += s_tot[w];
+      s_base = sum ? atomicAdd(&counts[level + 1], sum) : 0u;
+    }
+    __syncthreads();
+    uint32_t base_out = s_base;
+    for (uint32_t w = 0; w < wave; w++) base_out += s_tot[w];
+    __syncthreads();  // s_tot and s_base are rewritten by the next step
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     const uint32_t ol = base_out + (uint32_t)__popcll(ml & below);
     const uint32_t orr = base_out + (uint32_t)__popcll(ml) + (uint32_t)__popcll(mr & below);
@@ -834,9 +907,6 @@ __global__ void __launch_bounds__(256) k_classify_level(uint32_t n_all, int max_
                    gridDim.x * blockDim.x);
 }
 
-__device__ __forceinline__ uint32_t count_load(const uint32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // Levels 0 .. top-1 in one workgroup of 1024 threads. The host picks top so
 // that none of these levels holds more than 1024 branches (level L holds at
@@ -1108,6 +1178,55 @@ __device__ __forceinline__ void leaf_box_pack(const PrimSrc &s, const uint32_t *
   // tet ComputeBoundingBox30 render/render_accel_tetra.cc:921-969
   // sph ComputeBoundingBox30 render/render_accel_particle.cc:417-462
   // sol ComputeBoundingBox30 render/render_accel_solid.cc:1120-1169
+  if (K == kTri || K == kTet) {
+    // four primitives at a time: their ids, vertex indices and vertices are
+    // loaded in three rounds instead of three dependent loads per primitive,
+    // then boxed and packed in order
+    constexpr int NVT = K == kTet ? 4 : 3;
+    for (uint32_t q0 = first; q0 < first + count; q0 += 4) {
+      const uint32_t nq = min(4u, first + count - q0);
+      uint32_t ids[4], f[4][NVT];
+      float3 vv[4][NVT];
+#pragma unroll
+      for (int j = 0; j < 4; j++) ids[j] = (uint32_t)j < nq ? sidx[q0 + j] : 0u;
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < NVT; k++)
+          f[j][k] = (uint32_t)j < nq ? s.F[(size_t)NVT * ids[j] + k] : 0u;
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < NVT; k++)
+          vv[j][k] = (uint32_t)j < nq ? load_v3(s.V, f[j][k]) : make_float3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if ((uint32_t)j >= nq) break;
+        const uint32_t q = q0 + j, id = ids[j];
+        const float3 *v = vv[j];
+        if (q == first) {
+          b.mn[0] = v[0].x - e; b.mn[1] = v[0].y - e; b.mn[2] = v[0].z - e;
+          b.mx[0] = v[0].x + e; b.mx[1] = v[0].y + e; b.mx[2] = v[0].z + e;
+        }
+        for (int k = 0; k < NVT; k++) box_add(b, v[k].x, v[k].y, v[k].z, e);
+        if (K == kTri) {
+          float4 *P = reinterpret_cast<float4 *>(prims) + 3 * (size_t)q;
+          P[0] = make_float4(v[0].x, v[0].y, v[0].z, __uint_as_float(id));
+          P[1] = make_float4(v[1].x - v[0].x, v[1].y - v[0].y, v[1].z - v[0].z, 0.0f);
+          P[2] = make_float4(v[2].x - v[0].x, v[2].y - v[0].y, v[2].z - v[0].z, 0.0f);
+        } else {
+          float4 *P = reinterpret_cast<float4 *>(prims) + 4 * (size_t)q;
+          P[0] = make_float4(v[0].x, v[0].y, v[0].z, __uint_as_float(id));
+          P[1] = make_float4(v[1].x, v[1].y, v[1].z, 0.0f);
+          P[2] = make_float4(v[2].x, v[2].y, v[2].z, 0.0f);
+          P[3] = make_float4(v[NVT - 1].x, v[NVT - 1].y, v[NVT - 1].z, 0.0f);
+          reinterpret_cast<uint4 *>(sorted_faces)[q] =
+              make_uint4(f[j][0], f[j][1], f[j][2], f[j][NVT - 1]);
+        }
+      }
+    }
+    return;
+  }
   for (uint32_t q = first; q < first + count; q++) {
     uint32_t id = sidx[q];
     if (K == kLin) {
@@ -1142,39 +1261,6 @@ __device__ __forceinline__ void leaf_box_pack(const PrimSrc &s, const uint32_t *
         box_add(b, v.x, v.y, v.z, e);
         P[k] = make_float4(v.x, v.y, v.z, k == 0 ? __uint_as_float(id) : 0.0f);
       }
-    } else if (K == kTri) {
-      uint32_t f0 = s.F[3 * (size_t)id], f1 = s.F[3 * (size_t)id + 1],
-               f2 = s.F[3 * (size_t)id + 2];
-      float3 a = load_v3(s.V, f0), bb = load_v3(s.V, f1), c = load_v3(s.V, f2);
-      if (q == first) {
-        b.mn[0] = a.x - e; b.mn[1] = a.y - e; b.mn[2] = a.z - e;
-        b.mx[0] = a.x + e; b.mx[1] = a.y + e; b.mx[2] = a.z + e;
-      }
-      box_add(b, a.x, a.y, a.z, e);
-      box_add(b, bb.x, bb.y, bb.z, e);
-      box_add(b, c.x, c.y, c.z, e);
-      float4 *P = reinterpret_cast<float4 *>(prims) + 3 * (size_t)q;
-      P[0] = make_float4(a.x, a.y, a.z, __uint_as_float(id));
-      P[1] = make_float4(bb.x - a.x, bb.y - a.y, bb.z - a.z, 0.0f);
-      P[2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
-    } else if (K == kTet) {
-      uint32_t f[4];
-      float3 v[4];
-      for (int k = 0; k < 4; k++) {
-        f[k] = s.F[4 * (size_t)id + k];
-        v[k] = load_v3(s.V, f[k]);
-      }
-      if (q == first) {
-        b.mn[0] = v[0].x - e; b.mn[1] = v[0].y - e; b.mn[2] = v[0].z - e;
-        b.mx[0] = v[0].x + e; b.mx[1] = v[0].y + e; b.mx[2] = v[0].z + e;
-      }
-      for (int k = 0; k < 4; k++) box_add(b, v[k].x, v[k].y, v[k].z, e);
-      float4 *P = reinterpret_cast<float4 *>(prims) + 4 * (size_t)q;
-      P[0] = make_float4(v[0].x, v[0].y, v[0].z, __uint_as_float(id));
-      P[1] = make_float4(v[1].x, v[1].y, v[1].z, 0.0f);
-      P[2] = make_float4(v[2].x, v[2].y, v[2].z, 0.0f);
-      P[3] = make_float4(v[3].x, v[3].y, v[3].z, 0.0f);
-      reinterpret_cast<uint4 *>(sorted_faces)[q] = make_uint4(f[0], f[1], f[2], f[3]);
     } else {
       float3 p = sph_pos(s, id);
       float r = sph_rad(s, id);
@@ -1752,7 +1838,7 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
     tile_off[m + 1] = tile_off[m] + sort_tiles(n);
     // batches: a block per 4096 primitives, so most small meshes take one
     bblk_off[m + 1] = bblk_off[m] + blocks_for(n, multi ? 256 * 16 : 256, kBoundsBlocks);
-    mblk_off[m + 1] = mblk_off[m] + blocks_for(n, 256, 4096);
+    mblk_off[m + 1] = mblk_off[m] + blocks_for(n, 256, 1024);
   }
   const uint32_t N = prim_off[nseg], tiles = tile_off[nseg];
   const uint32_t bblocks = bblk_off[nseg], mblocks = mblk_off[nseg];
@@ -1846,9 +1932,14 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
     SRF_HIP(hipGetLastError());                                                        \
   } while (0)
 
+  const CenCache cen =
+      (kind == kTri || kind == kTet)
+          ? CenCache{reinterpret_cast<float *>(b.ss.keys_b), reinterpret_cast<float *>(b.ss.vals_b),
+                     reinterpret_cast<float *>(b.kchild)}
+          : CenCache{nullptr, nullptr, nullptr};
   SRF_DISPATCH(k_bounds, dim3(bblocks), dim3(256), 0, stream, v, b.partials, clear_begin,
-               nclear);
-  SRF_DISPATCH(k_morton, dim3(mblocks), dim3(256), 0, stream, v, b.codes, b.ss.hist);
+               nclear, cen);
+  SRF_DISPATCH(k_morton, dim3(mblocks), dim3(256), 0, stream, v, b.codes, b.ss.hist, cen);
   st = multi ? run_sort_passes<true>(b.codes, N, tiles, b.ss, v, stream)
              : run_sort_passes<false>(b.codes, N, tiles, b.ss, v, stream);
   if (st != SRF_OK) return st;
