@@ -249,6 +249,80 @@ def test_triangle_edge_cases_match_oracle(S, orc, case):
     assert got.tobytes() == want.tobytes()
 
 
+def _tet_edge_rays(case, V, F, cells, rng):
+    """Rays for the tetra edge cases: exact ties on an unjittered lattice,
+    origins inside the mesh, grazing directions and self-skipping rays."""
+    rays = []
+
+    def add(org, d):
+        r = np.zeros(len(d), W.RAY_DTYPE)
+        r["org"] = org
+        r["dir"] = d
+        r["flags"] = 1
+        r["prev_prim_id"] = W.NO_PRIM
+        rays.append(r)
+
+    c = float(cells)
+    if case == "lattice":
+        # along lattice lines (every tetra around an edge is crossed at the
+        # same distance) and aimed exactly at lattice vertices
+        j, k = np.meshgrid(np.arange(cells + 1), np.arange(cells + 1), indexing="ij")
+        org = np.stack([np.full(j.size, -3.0), j.ravel(), k.ravel()], 1)
+        add(org, np.tile([1.0, 0.0, 0.0], (j.size, 1)))
+        org = np.stack([j.ravel(), np.full(j.size, c + 2.5), k.ravel() + 0.5], 1)
+        add(org, np.tile([0.0, -1.0, 0.0], (j.size, 1)))
+        eye = np.array([1.7 * c, -0.55 * c, 1.25 * c])
+        tgt = V[rng.choice(len(V), 1500, replace=False)].astype(np.float64)
+        tgt = np.concatenate([tgt, 0.5 * (tgt[:750] + tgt[750:])])  # edge and face midpoints
+        d = tgt - eye
+        add(np.tile(eye, (len(d), 1)), d / np.linalg.norm(d, axis=1, keepdims=True))
+    elif case == "inside":
+        org = rng.uniform(1.0, c - 1.0, (3000, 3))
+        d = rng.normal(size=(3000, 3))
+        add(org, d / np.linalg.norm(d, axis=1, keepdims=True))
+    elif case == "grazing":
+        # nearly parallel to the lattice planes
+        org = np.stack([np.full(2000, -2.0), rng.uniform(0, c, 2000), rng.uniform(0, c, 2000)], 1)
+        d = np.stack([np.ones(2000), rng.normal(0, 1e-4, 2000), rng.normal(0, 1e-4, 2000)], 1)
+        add(org, d / np.linalg.norm(d, axis=1, keepdims=True))
+    out = np.concatenate(rays)
+    out["dir"] = out["dir"].astype(np.float32)
+    return out
+
+
+@pytest.mark.parametrize("case", ["lattice", "inside", "grazing", "prev"])
+def test_tetra_edge_cases_match_oracle(S, orc, case):
+    """Closest hits, weights, entry faces and exit distances equal the
+    oracle's (TetraAccel::Traverse, render/render_accel_tetra.cc:2203-2289)
+    where the nearest-first pass cannot vouch for its hit: ties between
+    tetras around a lattice edge or vertex, rays starting inside the mesh,
+    grazing rays and rays that skip the tetra they start on."""
+    say(f"tetra edge case {case}")
+    cells = 12
+    rng = np.random.default_rng(23)
+    V, F, Sc = W.tet_grid(cells, jitter=0.0 if case == "lattice" else 0.2)
+    t = orc.build_tet(V, F)
+    a = _build(S, "tet", (V, F))
+    if case == "prev":
+        rays = W.tet_rays(cells, 48, 48)
+        first, _ = orc.trace_tet(t, V, F, rays, Sc)
+        rays["prev_prim_id"] = first["prim_id"]
+    else:
+        rays = _tet_edge_rays(case, V, F, cells, rng)
+    want_h, want_e = orc.trace_tet(t, V, F, rays, Sc)
+    h, e = a.TraverseTetra(dev(S, rays), dev(S, Sc))
+    hits = S.hits_to_numpy(h)
+    ext = S.hits_to_numpy(e, W.TET_EXT_DTYPE)
+    hit = want_h["prim_id"] != W.NO_PRIM
+    assert hit.mean() > 0.3
+    assert hits.tobytes() == want_h.tobytes()
+    # InterpolateTetra divides by zero where an unjittered lattice point
+    # lies on a degenerate sub-volume: those weights are NaN on both sides
+    for f in ("d", "scalar", "t_leave"):
+        assert np.array_equal(ext[f][hit], want_e[f][hit], equal_nan=True), f
+    assert np.array_equal(ext["subface"][hit], want_e["subface"][hit])
+
+
 def test_depth_capped_leaves(S, orc):
     say("depth-capped caterpillar tree")
     P, R = W.deep_chain_points()
