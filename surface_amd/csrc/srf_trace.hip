@@ -699,7 +699,9 @@ __device__ __forceinline__ bool box_span(const RaySetup &s, float bx0, float by0
   return (tmax > 0.0f) && (tmin <= tmax);
 }
 
-template <bool kAny>
+// kSmall (closest hit, node indices below 2^24): the stack in three 16-bit
+// planes, 6 bytes an entry, so that the kernel runs 4 wavefronts per SIMD.
+template <bool kAny, bool kSmall>
 __global__ void __launch_bounds__(kTraceBlock)
 k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             const uint4 *__restrict__ sorted_faces, const BuildParams *__restrict__ params,
@@ -722,8 +724,18 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   //
   // pending far children as (child, entry distance): the popped child is
   // checked against the current hit distance without reloading its parent
-  // (24 KB of LDS per block holds the kernel at 3 wavefronts per SIMD)
-  __shared__ uint2 st_node[kTraceStack * kTraceBlock];
+  // (24 KB of LDS per block holds the kernel at 3 wavefronts per SIMD).
+  // kSmall keeps 6-byte entries in 16-bit planes (18 KB): pass A stores the
+  // child and its entry distance rounded down to 24 bits (a pop may then
+  // visit a node exact culling would skip, which pass A tolerates); pass B
+  // stores the parent and the child's side and recomputes the child's exact
+  // entry distance at the pop, as its order and culling must be the
+  // reference's.
+  static_assert(!(kAny && kSmall), "occlusion keeps the 8-byte entries");
+  __shared__ uint2 st_node[kSmall ? 1 : kTraceStack * kTraceBlock];
+  __shared__ uint16_t st_p0[kSmall ? kTraceStack * kTraceBlock : 1];
+  __shared__ uint16_t st_p1[kSmall ? kTraceStack * kTraceBlock : 1];
+  __shared__ uint16_t st_p2[kSmall ? kTraceStack * kTraceBlock : 1];
   const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -745,13 +757,66 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   const float pad = 1e-5f * (2.0f * cext + omax);
   const float dfirst = s.d[0] != 0.0f ? s.d[0] : (s.d[1] != 0.0f ? s.d[1] : s.d[2]);
   const float tslack = 1e-6f * (cext + omax) / fabsf(dfirst);
-  uint2 *sn = st_node + threadIdx.x;
+  uint2 *sn = st_node + (kSmall ? 0 : threadIdx.x);
+  uint16_t *q0 = st_p0 + (kSmall ? threadIdx.x : 0), *q1 = st_p1 + (kSmall ? threadIdx.x : 0),
+           *q2 = st_p2 + (kSmall ? threadIdx.x : 0);
   int sp = 0;
+  // pass A's entries: child and entry distance (kSmall: rounded down)
+  auto push_a = [&](uint32_t child, float e) {
+    if constexpr (kSmall) {
+      const uint32_t e24 = e > 0.0f ? __float_as_uint(e) >> 8 : 0u;
+      q0[sp * kTraceBlock] = (uint16_t)child;
+      q1[sp * kTraceBlock] = (uint16_t)((child >> 16) | ((e24 & 255u) << 8));
+      q2[sp * kTraceBlock] = (uint16_t)(e24 >> 8);
+    } else {
+      sn[sp * kTraceBlock] = make_uint2(child, __float_as_uint(e));
+    }
+    sp++;
+  };
+  auto pop_a = [&](float &e) -> uint32_t {
+    --sp;
+    if constexpr (kSmall) {
+      const uint32_t a = q0[sp * kTraceBlock], b = q1[sp * kTraceBlock],
+                     c = q2[sp * kTraceBlock];
+      e = __uint_as_float(((b >> 8) | (c << 8)) << 8);
+      return a | ((b & 255u) << 16);
+    } else {
+      const uint2 x = sn[sp * kTraceBlock];
+      e = __uint_as_float(x.y);
+      return x.x;
+    }
+  };
+  // the reference-order traversals' entries: the far child with its entry
+  // distance (kSmall: its parent and side)
+  auto push_b = [&](uint32_t parent, uint32_t side, uint32_t child, float e) {
+    if constexpr (kSmall) {
+      q0[sp * kTraceBlock] = (uint16_t)parent;
+      q1[sp * kTraceBlock] = (uint16_t)((parent >> 16) | (side << 8));
+    } else {
+      sn[sp * kTraceBlock] = make_uint2(child, __float_as_uint(e));
+    }
+    sp++;
+  };
   // pops entries until one whose box the ray enters no later than t
   auto pop_next = [&]() -> uint32_t {
     while (sp > 0) {
-      const uint2 e = sn[(--sp) * kTraceBlock];
-      if (__uint_as_float(e.y) <= t) return e.x;
+      --sp;
+      if constexpr (kSmall) {
+        const uint32_t b = q1[sp * kTraceBlock];
+        const uint32_t parent = (uint32_t)q0[sp * kTraceBlock] | ((b & 255u) << 16);
+        const float4 *nd = nodes + 4 * (size_t)parent;
+        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+        const uint4 meta = reinterpret_cast<const uint4 *>(nd)[3];
+        float tm;
+        if (b >> 8)
+          box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm);
+        else
+          box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm);
+        if (tm <= t) return (b >> 8) ? meta.w : meta.z;
+      } else {
+        const uint2 e = sn[sp * kTraceBlock];
+        if (__uint_as_float(e.y) <= t) return e.x;
+      }
     }
     return kDone;
   };
@@ -837,8 +902,7 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
         if (g0 && g1) {
           const bool first0 = tm0 <= tm1;
           if (sp < kTraceStack)
-            sn[(sp++) * kTraceBlock] =
-                make_uint2(first0 ? meta.w : meta.z, __float_as_uint(first0 ? tm1 : tm0));
+            push_a(first0 ? meta.w : meta.z, first0 ? tm1 : tm0);
           else
             dropped = true;
           cur = first0 ? meta.z : meta.w;
@@ -852,12 +916,13 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
       }
       cur = kDone;
       while (sp > 0) {
-        const uint2 e = sn[(--sp) * kTraceBlock];
-        if (__uint_as_float(e.y) <= tA) {
-          cur = e.x;
+        float e;
+        const uint32_t x = pop_a(e);
+        if (e <= tA) {
+          cur = x;
           break;
         }
-        ec = fminf(ec, __uint_as_float(e.y));
+        ec = fminf(ec, e);
       }
     }
     // When no other crossing and no box left out lies at or below
@@ -907,7 +972,7 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
           const float tn = nearc ? tm1 : tm0, tf = nearc ? tm0 : tm1;
           const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
           if (gn && tn <= t) {
-            if (gf && sp < kTraceStack) sn[(sp++) * kTraceBlock] = make_uint2(cf, __float_as_uint(tf));
+            if (gf && sp < kTraceStack) push_b(cur, nearc ? 0u : 1u, cf, tf);
             cur = cn;
             continue;
           }
@@ -970,7 +1035,7 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
       const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
       // the reference pushes far then near and pops near first
       if (gn && tn <= t) {
-        if (gf && sp < kTraceStack) sn[(sp++) * kTraceBlock] = make_uint2(cf, __float_as_uint(tf));
+        if (gf && sp < kTraceStack) push_b(cur, nearc ? 0u : 1u, cf, tf);
         cur = cn;
         continue;
       }
@@ -1964,10 +2029,16 @@ srf_status trace_tri(const Accel *a, const srf_ray *rays, uint64_t n, srf_hit *h
 srf_status trace_tet(const Accel *a, const srf_ray *rays, uint64_t n, const float *scalars,
                      srf_hit *hits, srf_tet_hit_ext *ext, hipStream_t s) {
   if (n == 0) return SRF_OK;
-  hipLaunchKernelGGL(k_trace_tet<false>, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
-                     (const float4 *)a->nodes, (const float4 *)a->prims,
-                     (const uint4 *)a->sorted_faces, a->params, rays, (uint32_t)n, scalars, hits,
-                     ext, nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
+  if (max_nodes(a) < (1u << 24))
+    hipLaunchKernelGGL((k_trace_tet<false, true>), dim3(trace_blocks(n)), dim3(kTraceBlock), 0,
+                       s, (const float4 *)a->nodes, (const float4 *)a->prims,
+                       (const uint4 *)a->sorted_faces, a->params, rays, (uint32_t)n, scalars,
+                       hits, ext, nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
+  else
+    hipLaunchKernelGGL((k_trace_tet<false, false>), dim3(trace_blocks(n)), dim3(kTraceBlock), 0,
+                       s, (const float4 *)a->nodes, (const float4 *)a->prims,
+                       (const uint4 *)a->sorted_faces, a->params, rays, (uint32_t)n, scalars,
+                       hits, ext, nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
   SRF_HIP(hipGetLastError());
   return SRF_OK;
 }
@@ -2043,7 +2114,7 @@ srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax, uin
                        (const float4 *)a->prims, a->params, rays, (uint32_t)n, nullptr, tmax,
                        occluded, max_nodes(a), (uint32_t)a->n);
   else if (a->kind == kTet)
-    hipLaunchKernelGGL(k_trace_tet<true>, g, b, 0, s, (const float4 *)a->nodes,
+    hipLaunchKernelGGL((k_trace_tet<true, false>), g, b, 0, s, (const float4 *)a->nodes,
                        (const float4 *)a->prims, (const uint4 *)a->sorted_faces, a->params, rays,
                        (uint32_t)n, nullptr, nullptr, nullptr, tmax, occluded, max_nodes(a),
                        (uint32_t)a->n);
