@@ -699,8 +699,8 @@ __device__ __forceinline__ bool box_span(const RaySetup &s, float bx0, float by0
   return (tmax > 0.0f) && (tmin <= tmax);
 }
 
-// kSmall (closest hit, node indices below 2^24): the stack in three 16-bit
-// planes, 6 bytes an entry, so that the kernel runs 4 wavefronts per SIMD.
+// kSmall (node indices below 2^24): the stack in 16-bit planes, 6 bytes an
+// entry, so that the kernel runs 4 wavefronts per SIMD.
 template <bool kAny, bool kSmall>
 __global__ void __launch_bounds__(kTraceBlock)
 k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
@@ -730,8 +730,7 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   // visit a node exact culling would skip, which pass A tolerates); pass B
   // stores the parent and the child's side and recomputes the child's exact
   // entry distance at the pop, as its order and culling must be the
-  // reference's.
-  static_assert(!(kAny && kSmall), "occlusion keeps the 8-byte entries");
+  // reference's (occlusion's traversal likewise).
   __shared__ uint2 st_node[kSmall ? 1 : kTraceStack * kTraceBlock];
   __shared__ uint16_t st_p0[kSmall ? kTraceStack * kTraceBlock : 1];
   __shared__ uint16_t st_p1[kSmall ? kTraceStack * kTraceBlock : 1];
@@ -2113,12 +2112,18 @@ srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax, uin
     hipLaunchKernelGGL((k_trace_tri<true, false>), g, b, 0, s, (const float4 *)a->nodes,
                        (const float4 *)a->prims, a->params, rays, (uint32_t)n, nullptr, tmax,
                        occluded, max_nodes(a), (uint32_t)a->n);
-  else if (a->kind == kTet)
-    hipLaunchKernelGGL((k_trace_tet<true, false>), g, b, 0, s, (const float4 *)a->nodes,
-                       (const float4 *)a->prims, (const uint4 *)a->sorted_faces, a->params, rays,
-                       (uint32_t)n, nullptr, nullptr, nullptr, tmax, occluded, max_nodes(a),
-                       (uint32_t)a->n);
-  else
+  else if (a->kind == kTet) {
+    if (max_nodes(a) < (1u << 24))
+      hipLaunchKernelGGL((k_trace_tet<true, true>), g, b, 0, s, (const float4 *)a->nodes,
+                         (const float4 *)a->prims, (const uint4 *)a->sorted_faces, a->params,
+                         rays, (uint32_t)n, nullptr, nullptr, nullptr, tmax, occluded,
+                         max_nodes(a), (uint32_t)a->n);
+    else
+      hipLaunchKernelGGL((k_trace_tet<true, false>), g, b, 0, s, (const float4 *)a->nodes,
+                         (const float4 *)a->prims, (const uint4 *)a->sorted_faces, a->params,
+                         rays, (uint32_t)n, nullptr, nullptr, nullptr, tmax, occluded,
+                         max_nodes(a), (uint32_t)a->n);
+  } else
     hipLaunchKernelGGL(k_trace_sph<true>, g, b, 0, s, (const float4 *)a->nodes,
                        (const float4 *)a->prims, a->sorted_index, rays, nullptr, (uint32_t)n,
                        nullptr, tmax, occluded, max_nodes(a), (uint32_t)a->n, sph_sub(a));
