@@ -40,10 +40,10 @@ HBM_PEAK_GBS = 8000.0
 # child first to find the closest crossing and certifies it for almost every
 # ray; the rest rerun the reference's traversal from just above it (pass B).
 # Node and tetra loads per ray, counted in an instrumented build of the
-# kernel over tet_grid(64) x tet_rays(64, 1024, 1024): pass A 29.5462 nodes +
-# 11.9270 tetras, pass B 0.0494 + 0.0192.
-TET_NODES_PER_RAY = 29.5462 + 0.0494
-TET_TESTS_PER_RAY = 11.9270 + 0.0192
+# kernel over tet_grid(64) x tet_rays(64, 1024, 1024): pass A 29.5498 nodes +
+# 11.9310 tetras, pass B 0.0704 (its pops reload the parent node) + 0.0220.
+TET_NODES_PER_RAY = 29.5498 + 0.0704
+TET_TESTS_PER_RAY = 11.9310 + 0.0220
 # The reference's single traversal on the same rays (oracle, stats=True)
 # loads 180.0010 nodes and 60.3673 tetras per ray.
 TET_REF_NODES_PER_RAY = 180.0010
@@ -57,7 +57,7 @@ TET_REF_BYTES_PER_RAY = 40 + 40 * TET_REF_NODES_PER_RAY + 80 * TET_REF_TESTS_PER
 # HBM bytes per k_trace_tet launch on the bench rays: rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (KiB/dispatch), each scaled by the same counter on a 1 GiB copy
 # (fetch x 2.000, write x 1.000); profiles/traffic_k_trace_tet_tet64_k_trace_tri_heightfield512.md
-TET_TRACE_HBM_BYTES = (2.000 * 26353.69 + 1.000 * 49152.0) * 1024
+TET_TRACE_HBM_BYTES = (2.000 * 26361.25 + 1.000 * 49152.0) * 1024
 TRI_BYTES_PER_RAY = 40 + 64 * TRI_NODES_PER_RAY + 48 * TRI_TESTS_PER_RAY
 
 
