@@ -743,12 +743,19 @@ __device__ __forceinline__ void seg_local(const SegView &v, uint32_t n_all, uint
   n = M ? v.prim_off[m + 1] - base : n_all;
 }
 
+// Also clears what the classification accumulates into: leaf_start and cls
+// (one word per primitive) and the 64 level counters.
 template <bool M>
 __global__ void __launch_bounds__(256) k_karras(const uint32_t *codes, uint32_t n_all,
                                                uint32_t *kchild, uint32_t *kleaf,
-                                               uint32_t *kother, SegView v) {
+                                               uint32_t *kother, uint32_t *leaf_start,
+                                               uint32_t *cls, uint32_t *level_counts,
+                                               SegView v) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < 64) level_counts[x] = 0;
   if (x >= n_all) return;
+  leaf_start[x] = 0;
+  cls[x] = 0;
   uint32_t m, base, n;
   seg_local<M>(v, n_all, x, m, base, n);
   const uint32_t i = x - base;
@@ -1944,14 +1951,9 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
              : run_sort_passes<false>(b.codes, N, tiles, b.ss, v, stream);
   if (st != SRF_OK) return st;
   // sorted codes in `codes`, sorted local indices in ss.vals_a
-  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(N, 256, 2048)), dim3(256), 0, stream,
-                     b.leaf_start, 0u, (size_t)N);
   SRF_DISPATCH_M(k_karras, dim3(nblocks), dim3(256), 0, stream, b.codes, N, b.kchild, b.kleaf,
-                 b.kother, v);
+                 b.kother, b.leaf_start, b.cls, b.level_counts, v);
   // top-down branch classification, one launch per depth
-  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(N, 256, 2048)), dim3(256), 0, stream, b.cls,
-                     0u, (size_t)N);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, b.level_counts, 0u, (size_t)64);
   SRF_DISPATCH_M(k_classify_roots, dim3(1), dim3(64), 0, stream, v, b.levels, b.level_counts);
   // the first levels, which hold at most nseg * 2^level branches each, in
   // one workgroup; the rest one launch per level
