@@ -71,6 +71,116 @@ k_trace_tri(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   }
 }
 
+// Closest hit with one traversal per wavefront. When every ray of the
+// wavefront has the same direction signs, the reference's per-ray order
+// (near child by the split axis's sign, far child pushed when both boxes are
+// hit, no re-test at the pop) is the same static pre-order for all of them,
+// so the wavefront walks the union of their paths in that order with one
+// shared stack of (node, lane mask): a lane takes part in a node exactly when
+// its own traversal would visit it, with the distance its own traversal
+// would hold there (between two of its nodes it is masked off and changes
+// nothing), so every lane tests the same triangles in the same order as the
+// per-ray kernel and ends with the same hit. Nodes and triangles are loaded
+// once per wavefront instead of once per lane. A wavefront with mixed signs
+// runs the per-ray traversal.
+template <bool kSmall>
+__global__ void __launch_bounds__(kTraceBlock)
+k_trace_tri_wave(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
+                 const BuildParams *__restrict__ params, const srf_ray *__restrict__ rays,
+                 uint32_t n, srf_hit *__restrict__ hits, uint32_t max_nodes, uint32_t nprims) {
+  __shared__ uint32_t stack[kSmall ? 1 : kTraceStack * kTraceBlock];
+  __shared__ uint16_t stack_lo[kSmall ? kTraceStack * kTraceBlock : 1];
+  __shared__ uint8_t stack_hi[kSmall ? kTraceStack * kTraceBlock : 1];
+  __shared__ uint32_t wnode[kTraceBlock / 64][kTraceStack];
+  __shared__ uint64_t wmask[kTraceBlock / 64][kTraceStack];
+  const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t flags, prev;
+  const RaySetup s = setup_ray(rays, i, flags, prev);
+  const bool ds = (flags & SRF_RAY_DOUBLE_SIDED) != 0;
+  const float kEPS = FLT_EPSILON * (float)params->eps_scale;
+  float t = FLT_MAX, u = 0.0f, v = 0.0f;
+  uint32_t prim = SRF_NO_PRIM;
+  const uint32_t sb = __builtin_amdgcn_readfirstlane(s.sbits);
+  if (!__all(s.sbits == sb)) {
+    if (kSmall)
+      tri_traverse<false>(nodes, prims, max_nodes, nprims, s, ds, kEPS, prev, FLT_MAX,
+                          Stack24<kTraceBlock>{stack_lo + threadIdx.x, stack_hi + threadIdx.x},
+                          t, u, v, prim);
+    else
+      tri_traverse<false>(nodes, prims, max_nodes, nprims, s, ds, kEPS, prev, FLT_MAX,
+                          Stack32<kTraceBlock>{stack + threadIdx.x}, t, u, v, prim);
+  } else {
+    const bool filter = kEPS >= 1e-30f;
+    const int w = threadIdx.x >> 6;
+    const uint64_t me = 1ull << (threadIdx.x & 63u);
+    uint64_t mask = __ballot(1);
+    uint32_t cur = 0;
+    int sp = 0;
+    for (;;) {
+      // descend while some lane's traversal goes on below this node
+      uint4 meta = make_uint4(0u, 0u, 0u, 0u);
+      bool leaf = false;
+      while (cur < max_nodes) {
+        const float4 *nd = nodes + 4 * (size_t)cur;
+        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+        meta = reinterpret_cast<const uint4 *>(nd)[3];
+        keep_loaded(n0, n1, n2);
+        if (meta.x != 0) {
+          leaf = true;
+          break;
+        }
+        const bool in = (mask & me) != 0;
+        const bool h0 = in && ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
+        const bool h1 = in && ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
+        const int nearc = (sb >> meta.y) & 1u;
+        const uint64_t nm = __ballot(nearc ? h1 : h0), fm = __ballot(nearc ? h0 : h1);
+        const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+        if (nm) {
+          // the lanes hitting both boxes pushed the far child; those hitting
+          // only the far one go there next, after nothing of theirs
+          if (fm && sp < kTraceStack) {
+            wnode[w][sp] = cf;
+            wmask[w][sp] = fm;
+            sp++;
+          }
+          cur = cn;
+          mask = nm;
+        } else if (fm) {
+          cur = cf;
+          mask = fm;
+        } else {
+          break;
+        }
+      }
+      if (leaf) {
+        // TestLeafNode, render/render_accel_triangle.cc:2386-2433
+        const bool in = (mask & me) != 0;
+        const uint32_t first = meta.w, end = min(meta.w + meta.z, nprims);
+        for (uint32_t q = first; q < end; q++) {
+          const float4 *P = prims + 3 * (size_t)q;
+          const float4 A = P[0], B = P[1], C = P[2];
+          if (in && __float_as_uint(A.w) != prev) {
+            float tt, uu, vv;
+            if (tri_isect(s, A, B, C, ds, kEPS, filter, t, tt, uu, vv)) {
+              t = tt;
+              u = uu;
+              v = vv;
+              prim = __float_as_uint(A.w);
+            }
+          }
+        }
+      }
+      if (sp == 0) break;
+      --sp;
+      cur = wnode[w][sp];
+      mask = wmask[w][sp];
+    }
+    if (prim == SRF_NO_PRIM) t = FLT_MAX;
+  }
+  reinterpret_cast<float4 *>(hits)[i] = make_float4(t, u, v, __uint_as_float(prim));
+}
+
 // ------------------------------------------------------------------ spheres
 
 // SphereIsect, render/render_accel_particle.cc:2423-2512
@@ -2014,13 +2124,13 @@ srf_status trace_tri(const Accel *a, const srf_ray *rays, uint64_t n, srf_hit *h
                      hipStream_t s) {
   if (n == 0) return SRF_OK;
   if (small_nodes(a))
-    hipLaunchKernelGGL((k_trace_tri<false, true>), dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
+    hipLaunchKernelGGL((k_trace_tri_wave<true>), dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
                        (const float4 *)a->nodes, (const float4 *)a->prims, a->params, rays,
-                       (uint32_t)n, hits, nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
+                       (uint32_t)n, hits, max_nodes(a), (uint32_t)a->n);
   else
-    hipLaunchKernelGGL((k_trace_tri<false, false>), dim3(trace_blocks(n)), dim3(kTraceBlock), 0,
+    hipLaunchKernelGGL((k_trace_tri_wave<false>), dim3(trace_blocks(n)), dim3(kTraceBlock), 0,
                        s, (const float4 *)a->nodes, (const float4 *)a->prims, a->params, rays,
-                       (uint32_t)n, hits, nullptr, nullptr, max_nodes(a), (uint32_t)a->n);
+                       (uint32_t)n, hits, max_nodes(a), (uint32_t)a->n);
   SRF_HIP(hipGetLastError());
   return SRF_OK;
 }
