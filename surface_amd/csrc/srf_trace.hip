@@ -81,8 +81,9 @@ k_trace_tri(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
 // would hold there (between two of its nodes it is masked off and changes
 // nothing), so every lane tests the same triangles in the same order as the
 // per-ray kernel and ends with the same hit. Nodes and triangles are loaded
-// once per wavefront instead of once per lane. A wavefront with mixed signs
-// runs the per-ray traversal.
+// once per wavefront instead of once per lane (the node and triangle
+// indices are made wavefront-uniform, so the loads go through the scalar
+// unit). A wavefront with mixed signs runs the per-ray traversal.
 template <bool kSmall>
 __global__ void __launch_bounds__(kTraceBlock)
 k_trace_tri_wave(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
@@ -122,6 +123,7 @@ k_trace_tri_wave(const float4 *__restrict__ nodes, const float4 *__restrict__ pr
       uint4 meta = make_uint4(0u, 0u, 0u, 0u);
       bool leaf = false;
       while (cur < max_nodes) {
+        cur = __builtin_amdgcn_readfirstlane(cur);
         const float4 *nd = nodes + 4 * (size_t)cur;
         const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
         meta = reinterpret_cast<const uint4 *>(nd)[3];
@@ -156,7 +158,8 @@ k_trace_tri_wave(const float4 *__restrict__ nodes, const float4 *__restrict__ pr
       if (leaf) {
         // TestLeafNode, render/render_accel_triangle.cc:2386-2433
         const bool in = (mask & me) != 0;
-        const uint32_t first = meta.w, end = min(meta.w + meta.z, nprims);
+        const uint32_t first = __builtin_amdgcn_readfirstlane(meta.w),
+                       end = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, nprims));
         for (uint32_t q = first; q < end; q++) {
           const float4 *P = prims + 3 * (size_t)q;
           const float4 A = P[0], B = P[1], C = P[2];
@@ -173,7 +176,7 @@ k_trace_tri_wave(const float4 *__restrict__ nodes, const float4 *__restrict__ pr
       }
       if (sp == 0) break;
       --sp;
-      cur = wnode[w][sp];
+      cur = __builtin_amdgcn_readfirstlane(wnode[w][sp]);
       mask = wmask[w][sp];
     }
     if (prim == SRF_NO_PRIM) t = FLT_MAX;
