@@ -848,6 +848,11 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   __shared__ uint16_t st_p0[kSmall ? kTraceStack * kTraceBlock : 1];
   __shared__ uint16_t st_p1[kSmall ? kTraceStack * kTraceBlock : 1];
   __shared__ uint16_t st_p2[kSmall ? kTraceStack * kTraceBlock : 1];
+  // kSmall closest hit: pass A runs once per wavefront over a stack of
+  // (parent, side, lane mask) entries
+  constexpr bool kWave = kSmall && !kAny;
+  __shared__ uint32_t wv_par[kWave ? kTraceBlock / 64 : 1][kTraceStack];
+  __shared__ uint64_t wv_msk[kWave ? kTraceBlock / 64 : 1][kTraceStack];
   const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -969,14 +974,104 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   float troot;
   const bool root_hit = box_span(s, P.root_bmin[0], P.root_bmin[1], P.root_bmin[2],
                                  P.root_bmax[0], P.root_bmax[1], P.root_bmax[2], troot);
+  // pass A: the closest crossing's distance tA with little work, visiting
+  // nearer children first. s2 is the nearest other crossing seen, ec the
+  // nearest entry of a box left out because it lies beyond the running
+  // distance; any visiting order gives a valid tA, s2 and ec.
+  float tA = FLT_MAX, s2 = FLT_MAX, ec = FLT_MAX;
+  bool dropped = false;
+  if constexpr (kWave) {
+    // One traversal per wavefront: a lane takes part in a node while the
+    // node's box is entered no later than its own tA; a node's children go
+    // in the order most of its lanes would take (nearer entry first). Nodes
+    // and tetras are loaded once per wavefront. At a pop every lane of the
+    // entry recomputes the child's entry distance from the parent node.
+    const int wv = threadIdx.x >> 6;
+    const uint64_t me = 1ull << (threadIdx.x & 63u);
+    uint64_t mask = __ballot(root_hit && troot <= t);
+    uint32_t cur = mask ? 0u : kDone;
+    int wsp = 0;
+    while (cur != kDone) {
+      while (cur < max_nodes) {
+        cur = __builtin_amdgcn_readfirstlane(cur);
+        const float4 *nd = nodes + 4 * (size_t)cur;
+        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+        const uint4 meta = reinterpret_cast<const uint4 *>(nd)[3];
+        keep_loaded(n0, n1, n2);
+        const bool in = (mask & me) != 0;
+        if (meta.x != 0) {
+          const uint32_t q0 = __builtin_amdgcn_readfirstlane(meta.w);
+          const uint32_t q1 = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, nprims));
+          for (uint32_t q = q0; q < q1; q++) {
+            if (!in) continue;
+            int ef;
+            double eu, lu, et, lt;
+            if (test_tet(q, tA, ef, eu, lu, et, lt)) {
+              s2 = fminf(s2, tA);
+              tA = (float)et;
+              u = (float)eu;
+              v = (float)lu;
+              best = q;
+              subface = (uint32_t)ef;
+              tleave = lt;
+            } else if (et >= 0.0) {
+              s2 = fminf(s2, (float)et);
+            }
+          }
+          break;
+        }
+        float tm0 = 0.0f, tm1 = 0.0f;
+        const bool b0 = in && box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm0);
+        const bool b1 = in && box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm1);
+        const bool g0 = b0 && tm0 <= tA, g1 = b1 && tm1 <= tA;
+        ec = fminf(ec, fminf(b0 && !g0 ? tm0 : FLT_MAX, b1 && !g1 ? tm1 : FLT_MAX));
+        const uint64_t m0 = __ballot(g0), m1 = __ballot(g1);
+        const uint64_t pref0 = __ballot(g0 && g1 && tm0 <= tm1);
+        const int nboth = __popcll(m0 & m1);
+        const bool first0 = m1 == 0 || (m0 != 0 && 2 * __popcll(pref0) >= nboth);
+        const uint64_t mf = first0 ? m0 : m1, ms = first0 ? m1 : m0;
+        if (mf == 0) break;
+        if (ms != 0) {
+          if (wsp < kTraceStack) {
+            wv_par[wv][wsp] = cur | ((first0 ? 1u : 0u) << 31);
+            wv_msk[wv][wsp] = ms;
+            wsp++;
+          } else if (ms & me) {
+            dropped = true;
+          }
+        }
+        cur = first0 ? meta.z : meta.w;
+        mask = mf;
+      }
+      cur = kDone;
+      while (wsp > 0) {
+        --wsp;
+        const uint32_t e = __builtin_amdgcn_readfirstlane(wv_par[wv][wsp]);
+        const uint64_t em = wv_msk[wv][wsp];
+        const uint32_t parent = e & 0x7fffffffu, side = e >> 31;
+        const float4 *nd = nodes + 4 * (size_t)parent;
+        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+        const uint4 meta = reinterpret_cast<const uint4 *>(nd)[3];
+        float tm = 0.0f;
+        const bool in = (em & me) != 0;
+        if (in) {
+          if (side)
+            box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm);
+          else
+            box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm);
+        }
+        const bool keep = in && tm <= tA;
+        if (in && !keep) ec = fminf(ec, tm);
+        mask = __ballot(keep);
+        if (mask) {
+          cur = side ? meta.w : meta.z;
+          break;
+        }
+      }
+    }
+  }
   if (!kAny && root_hit && troot <= t) {
-    // pass A: nearer child by entry distance, to find the closest crossing's
-    // distance tA with little work. s2 is the nearest other crossing seen,
-    // ec the nearest entry of a box left out because it lies beyond the
-    // running distance.
-    float tA = FLT_MAX, s2 = FLT_MAX, ec = FLT_MAX;
-    bool dropped = false;
-    uint32_t cur = 0;
+    uint32_t cur = kWave ? kDone : 0u;
     sp = 0;
     while (cur != kDone) {
       for (;;) {
