@@ -13,10 +13,11 @@ rays) and the differentiable volume (256^3, 1024^2 rays, fwd and fwd+bwd ms).
 
 roofline: the dominant kernel (the tetra traversal) against MI355X's 8 TB/s
 HBM peak, with algorithmic bytes per ray = 40 B (ray in, hit out) + 40 B per
-node loaded + 80 B per tetra tested (64 B geometry + 16 B scalars), the node
-and tetra counts being the kernel's own on these rays (counted once and
-frozen below; the reference's traversal loads about six times as many,
-reported beside them). cpu_baseline: the reference's own OpenMP code
+node loaded + 80 B per tetra loaded (64 B geometry + 16 B scalars), the node
+and tetra counts being the kernel's own loads on these rays (a wavefront
+loads each node and tetra once for its 64 rays; counted once and frozen
+below; the reference's per-ray traversal would move about 100 times as many
+bytes, reported beside them). cpu_baseline: the reference's own OpenMP code
 (oracle/_ref/libsrfref_omp.so, built from the reference sources) on a bounded
 sample of the same rays, on this host's cores.
 """
@@ -36,20 +37,23 @@ sys.path.insert(0, ROOT)
 
 METRIC = "closest-hit Mrays/s + LBVH build Mprims/s; volume fwd+bwd ms vs plain fwd"
 HBM_PEAK_GBS = 8000.0
-# Work per ray on the bench rays. k_trace_tet's pass A visits the nearer
-# child first to find the closest crossing and certifies it for almost every
-# ray; the rest rerun the reference's traversal from just above it (pass B).
-# Node and tetra loads per ray, counted in an instrumented build of the
-# kernel over tet_grid(64) x tet_rays(64, 1024, 1024): pass A 29.5498 nodes +
-# 11.9310 tetras, pass B 0.0704 (its pops reload the parent node) + 0.0220.
-TET_NODES_PER_RAY = 29.5498 + 0.0704
-TET_TESTS_PER_RAY = 11.9310 + 0.0220
+# Work per ray on the bench rays. k_trace_tet's pass A runs once per
+# wavefront and loads each node and tetra once for its 64 rays; it certifies
+# almost every ray's hit, and the rest rerun the reference's traversal from
+# just above it (pass B, per ray). Loads per ray, counted in an instrumented
+# build of the kernel over tet_grid(64) x tet_rays(64, 1024, 1024): pass A
+# 60.53 nodes and 28.24 tetras per wavefront = 0.9458 + 0.4413 per ray,
+# pass B 0.0588 nodes (its pops reload the parent node) + 0.0187 tetras.
+TET_NODES_PER_RAY = 0.9458 + 0.0588
+TET_TESTS_PER_RAY = 0.4413 + 0.0187
 # The reference's single traversal on the same rays (oracle, stats=True)
 # loads 180.0010 nodes and 60.3673 tetras per ray.
 TET_REF_NODES_PER_RAY = 180.0010
 TET_REF_TESTS_PER_RAY = 60.3673
-# k_trace_tri follows the reference's traversal exactly (oracle counts on
-# heightfield(512), heightfield_rays(1024, 1024)).
+# The reference's triangle traversal (oracle counts on heightfield(512),
+# heightfield_rays(1024, 1024)); k_trace_tri_wave visits exactly these nodes
+# and triangles per ray but loads each once per wavefront, so the rate below
+# is what a per-ray traversal would have to move, not HBM traffic.
 TRI_NODES_PER_RAY = 64.3038
 TRI_TESTS_PER_RAY = 94.4435
 TET_BYTES_PER_RAY = 40 + 40 * TET_NODES_PER_RAY + 80 * TET_TESTS_PER_RAY
@@ -57,7 +61,7 @@ TET_REF_BYTES_PER_RAY = 40 + 40 * TET_REF_NODES_PER_RAY + 80 * TET_REF_TESTS_PER
 # HBM bytes per k_trace_tet launch on the bench rays: rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (KiB/dispatch), each scaled by the same counter on a 1 GiB copy
 # (fetch x 2.000, write x 1.000); profiles/traffic_k_trace_tet_tet64_k_trace_tri_heightfield512.md
-TET_TRACE_HBM_BYTES = (2.000 * 26361.25 + 1.000 * 49152.0) * 1024
+TET_TRACE_HBM_BYTES = (2.000 * 32442.81 + 1.000 * 49152.0) * 1024
 TRI_BYTES_PER_RAY = 40 + 64 * TRI_NODES_PER_RAY + 48 * TRI_TESTS_PER_RAY
 
 
@@ -353,7 +357,7 @@ def main():
         extras["heightfield_522242_tris"] = {
             "build_ms": round(tb, 4), "build_mprims_per_s": round(len(Fh) / tb / 1e3, 1),
             "trace_ms": round(tt, 4), "trace_mrays_per_s": round(n_rays / tt / 1e3, 1),
-            "trace_algorithmic_gbs": round(TRI_BYTES_PER_RAY * n_rays / (tt * 1e-3) / 1e9, 1)}
+            "per_ray_traversal_equivalent_gbs": round(TRI_BYTES_PER_RAY * n_rays / (tt * 1e-3) / 1e9, 1)}
         del tri
         log("volume 256^3")
         from surface_amd.volume import Volume
