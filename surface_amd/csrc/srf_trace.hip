@@ -434,6 +434,8 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded,
             uint32_t max_nodes, uint32_t nprims, SphSub sub) {
   __shared__ uint32_t stack[2 * kTraceStack * kTraceBlock];
+  __shared__ uint32_t wnode[kAny ? 1 : kTraceBlock / 64][kTraceStack];
+  __shared__ uint64_t wmask[kAny ? 1 : kTraceBlock / 64][kTraceStack];
   const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -447,9 +449,94 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   const float tlimit = kAny ? (tmax_in ? tmax_in[i] : FLT_MAX) : FLT_MAX;
   float t;
   uint32_t best;
-  const bool occ = sph_traverse<kAny, kTraceBlock>(nodes, prims, sorted_index, max_nodes, nprims,
-                                                   sub, s, prev, prevT, prevN, tlimit,
-                                                   stack + threadIdx.x, t, best);
+  bool occ = false;
+  const uint32_t sb = __builtin_amdgcn_readfirstlane(s.sbits);
+  if (!kAny && __all(s.sbits == sb)) {
+    // Closest hit once per wavefront, as k_trace_tri_wave: with the lanes'
+    // direction signs equal, the reference's order (near child by the split
+    // axis's sign, far child pushed, no re-test at the pop) is one static
+    // pre-order, and each lane takes part exactly in the nodes its own
+    // traversal visits, holding the distance that traversal would hold
+    // there. Small leaves are scanned once per wavefront in the reference's
+    // order; a big leaf's secondary tree, whose result does not depend on
+    // its order, runs per lane.
+    t = FLT_MAX;
+    best = SRF_NO_PRIM;
+    const int wv = threadIdx.x >> 6;
+    const uint64_t me = 1ull << (threadIdx.x & 63u);
+    uint64_t mask = __ballot(1);
+    uint32_t cur = 0;
+    int wsp = 0;
+    for (;;) {
+      uint4 meta = make_uint4(0u, 0u, 0u, 0u);
+      bool leaf = false;
+      while (cur < max_nodes) {
+        cur = __builtin_amdgcn_readfirstlane(cur);
+        const float4 *nd = nodes + 4 * (size_t)cur;
+        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+        meta = reinterpret_cast<const uint4 *>(nd)[3];
+        keep_loaded(n0, n1, n2);
+        if (meta.x != 0) {
+          leaf = true;
+          break;
+        }
+        const bool in = (mask & me) != 0;
+        const bool h0 = in && ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
+        const bool h1 = in && ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
+        const int nearc = (sb >> meta.y) & 1u;
+        const uint64_t nm = __ballot(nearc ? h1 : h0), fm = __ballot(nearc ? h0 : h1);
+        const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+        if (nm) {
+          if (fm && wsp < kTraceStack) {
+            wnode[wv][wsp] = cf;
+            wmask[wv][wsp] = fm;
+            wsp++;
+          }
+          cur = cn;
+          mask = nm;
+        } else if (fm) {
+          cur = cf;
+          mask = fm;
+        } else {
+          break;
+        }
+      }
+      if (leaf) {
+        const bool in = (mask & me) != 0;
+        if (meta.z > kBigLeaf && sub.nodes) {
+          const float4 *nd = nodes + 4 * (size_t)cur;
+          const uint32_t nbase = __float_as_uint(nd[0].w), pbase = __float_as_uint(nd[1].x);
+          if (in && meta.w < nprims && meta.z <= nprims - meta.w)
+            sph_big_leaf<false, kTraceBlock>(sub, nbase, pbase, meta.w, meta.z, sorted_index, s,
+                                             prev, prevT, prevN, FLT_MAX,
+                                             stack + threadIdx.x + kTraceStack * kTraceBlock,
+                                             t, best);
+        } else {
+          const uint32_t q0 = __builtin_amdgcn_readfirstlane(meta.w);
+          const uint32_t q1 = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, nprims));
+          for (uint32_t q = q0; q < q1; q++) {
+            const float4 c = prims[q];
+            const uint32_t sid = sorted_index[q];
+            if (!in) continue;
+            const bool self = (prev != SRF_NO_PRIM) && (sid == prev);
+            float tt;
+            if (sphere_isect(s, c, t, prevT, prevN, self, tt)) {
+              t = tt;
+              best = q;
+            }
+          }
+        }
+      }
+      if (wsp == 0) break;
+      --wsp;
+      cur = __builtin_amdgcn_readfirstlane(wnode[wv][wsp]);
+      mask = wmask[wv][wsp];
+    }
+  } else {
+    occ = sph_traverse<kAny, kTraceBlock>(nodes, prims, sorted_index, max_nodes, nprims, sub, s,
+                                          prev, prevT, prevN, tlimit, stack + threadIdx.x, t,
+                                          best);
+  }
   if (kAny) {
     occluded[i] = occ ? 1u : 0u;
     return;
