@@ -2132,6 +2132,8 @@ k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims
               const uint32_t *__restrict__ order, uint32_t n, uint32_t *__restrict__ occluded,
               uint32_t max_nodes, uint32_t nprims, uint32_t *counter, SphSub sub) {
   __shared__ uint32_t stack[(K == kSph ? 2 : 1) * kTraceStack * kTraceBlock];
+  __shared__ uint32_t wnode[K == kSph ? kTraceBlock / 64 : 1][kTraceStack];
+  __shared__ uint64_t wmask[K == kSph ? kTraceBlock / 64 : 1][kTraceStack];
   uint32_t *st = stack + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t base = (counter ? blockIdx.x : xcd_block()) * kTraceBlock + (threadIdx.x & ~63u);
@@ -2156,12 +2158,91 @@ k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims
         occ = tri_traverse<true>(nodes, prims, max_nodes, nprims, s,
                                  (flags & SRF_RAY_DOUBLE_SIDED) != 0, kEPS, prev,
                                  tlimit, Stack32<kTraceBlock>{st}, t, u, v, prim);
-      } else {
+      } else if (const uint32_t sb = __builtin_amdgcn_readfirstlane(s.sbits);
+                 !__all(s.sbits == sb)) {
+        // directions of mixed signs (rays not sorted): one traversal per lane
         const float zero[3] = {0.0f, 0.0f, 0.0f};
         float t;
         uint32_t best;
         occ = sph_traverse<true, kTraceBlock>(nodes, prims, sorted_index, max_nodes, nprims,
                                               sub, s, prev, 0.0, zero, tlimit, st, t, best);
+      } else {
+        // Once per wavefront: with the search distance fixed at tlimit, the
+        // set of nodes and spheres a lane's traversal tests does not depend
+        // on their order, and neither does whether one of them occludes, so
+        // the wavefront walks the union of its lanes' nodes in one order (the
+        // lanes' common near children) and a lane leaves once occluded.
+        const float zero[3] = {0.0f, 0.0f, 0.0f};
+        float t = tlimit;
+        uint32_t best = SRF_NO_PRIM;
+        const int wv = threadIdx.x >> 6;
+        const uint64_t me = 1ull << lane;
+        occ = false;
+        uint64_t mask = __ballot(1);
+        uint32_t cur = 0;
+        int wsp = 0;
+        for (;;) {
+          uint4 meta = make_uint4(0u, 0u, 0u, 0u);
+          bool leaf = false;
+          while (cur < max_nodes) {
+            cur = __builtin_amdgcn_readfirstlane(cur);
+            const float4 *nd = nodes + 4 * (size_t)cur;
+            const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+            meta = reinterpret_cast<const uint4 *>(nd)[3];
+            keep_loaded(n0, n1, n2);
+            if (meta.x != 0) {
+              leaf = true;
+              break;
+            }
+            const bool in = (mask & me) != 0 && !occ;
+            const bool h0 = in && ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
+            const bool h1 = in && ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
+            const int nearc = (sb >> meta.y) & 1u;
+            const uint64_t nm = __ballot(nearc ? h1 : h0), fm = __ballot(nearc ? h0 : h1);
+            const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+            if (nm) {
+              if (fm && wsp < kTraceStack) {
+                wnode[wv][wsp] = cf;
+                wmask[wv][wsp] = fm;
+                wsp++;
+              }
+              cur = cn;
+              mask = nm;
+            } else if (fm) {
+              cur = cf;
+              mask = fm;
+            } else {
+              break;
+            }
+          }
+          if (leaf) {
+            const bool in = (mask & me) != 0 && !occ;
+            if (meta.z > kBigLeaf && sub.nodes) {
+              const float4 *nd = nodes + 4 * (size_t)cur;
+              const uint32_t nbase = __float_as_uint(nd[0].w), pbase = __float_as_uint(nd[1].x);
+              if (in && meta.w < nprims && meta.z <= nprims - meta.w)
+                occ = sph_big_leaf<true, kTraceBlock>(sub, nbase, pbase, meta.w, meta.z,
+                                                      sorted_index, s, prev, 0.0, zero, tlimit,
+                                                      st + kTraceStack * kTraceBlock, t, best);
+            } else {
+              const uint32_t q0 = __builtin_amdgcn_readfirstlane(meta.w);
+              const uint32_t q1 = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, nprims));
+              for (uint32_t q = q0; q < q1; q++) {
+                const float4 c = prims[q];
+                const uint32_t sid = sorted_index[q];
+                if (!in || occ) continue;
+                const bool self = (prev != SRF_NO_PRIM) && (sid == prev);
+                float tt;
+                if (sphere_isect(s, c, t, 0.0, zero, self, tt) && tt < tlimit) occ = true;
+              }
+            }
+          }
+          if (__ballot(!occ) == 0) break;
+          if (wsp == 0) break;
+          --wsp;
+          cur = __builtin_amdgcn_readfirstlane(wnode[wv][wsp]);
+          mask = wmask[wv][wsp];
+        }
       }
       occluded[i] = occ ? 1u : 0u;
     }
