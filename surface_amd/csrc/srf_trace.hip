@@ -938,8 +938,8 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   // kSmall closest hit: pass A runs once per wavefront over a stack of
   // (parent, side, lane mask) entries
   constexpr bool kWave = kSmall && !kAny;
-  __shared__ uint32_t wv_par[kWave ? kTraceBlock / 64 : 1][kTraceStack];
-  __shared__ uint64_t wv_msk[kWave ? kTraceBlock / 64 : 1][kTraceStack];
+  __shared__ uint32_t wv_par[kSmall ? kTraceBlock / 64 : 1][kTraceStack];
+  __shared__ uint64_t wv_msk[kSmall ? kTraceBlock / 64 : 1][kTraceStack];
   const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -1295,6 +1295,82 @@ k_trace_tet(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
       if (best != SRF_NO_PRIM || t0 == FLT_MAX) break;
     }
     if (best == SRF_NO_PRIM) t = FLT_MAX;
+  }
+  if constexpr (kAny && kSmall) {
+    const uint32_t sb = __builtin_amdgcn_readfirstlane(s.sbits);
+    if (__all(s.sbits == sb)) {
+      // Occlusion once per wavefront when the lanes' direction signs agree:
+      // at the fixed search distance tlimit the nodes and tetras a lane's
+      // traversal tests, and whether one of them occludes, do not depend on
+      // the order, so the wavefront walks the union of its lanes' nodes (the
+      // lanes' common near children first) and a lane leaves once occluded.
+      const int wv = threadIdx.x >> 6;
+      const uint64_t me = 1ull << (threadIdx.x & 63u);
+      float tr;
+      const bool rin = box_span(s, P.root_bmin[0], P.root_bmin[1], P.root_bmin[2],
+                                P.root_bmax[0], P.root_bmax[1], P.root_bmax[2], tr) &&
+                       tr <= t;
+      uint64_t mask = __ballot(rin);
+      bool occ = false;
+      uint32_t cur = mask ? 0u : kDone;
+      int wsp = 0;
+      while (cur != kDone) {
+        uint4 meta = make_uint4(0u, 0u, 0u, 0u);
+        bool leaf = false;
+        while (cur < max_nodes) {
+          cur = __builtin_amdgcn_readfirstlane(cur);
+          const float4 *nd = nodes + 4 * (size_t)cur;
+          const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+          meta = reinterpret_cast<const uint4 *>(nd)[3];
+          keep_loaded(n0, n1, n2);
+          if (meta.x != 0) {
+            leaf = true;
+            break;
+          }
+          const bool in = (mask & me) != 0 && !occ;
+          float tm0 = 0.0f, tm1 = 0.0f;
+          const bool g0 = in && box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm0) && tm0 <= t;
+          const bool g1 = in && box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm1) && tm1 <= t;
+          const int nearc = (sb >> meta.y) & 1u;
+          const uint64_t nm = __ballot(nearc ? g1 : g0), fm = __ballot(nearc ? g0 : g1);
+          const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+          if (nm) {
+            if (fm && wsp < kTraceStack) {
+              wv_par[wv][wsp] = cf;
+              wv_msk[wv][wsp] = fm;
+              wsp++;
+            }
+            cur = cn;
+            mask = nm;
+          } else if (fm) {
+            cur = cf;
+            mask = fm;
+          } else {
+            break;
+          }
+        }
+        if (leaf) {
+          const bool in = (mask & me) != 0;
+          const uint32_t q0 = __builtin_amdgcn_readfirstlane(meta.w);
+          const uint32_t q1 = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, nprims));
+          for (uint32_t q = q0; q < q1; q++) {
+            if (!in || occ) continue;
+            int ef;
+            double eu, lu, et, lt;
+            if (test_tet(q, t, ef, eu, lu, et, lt) && (float)et < tlimit) occ = true;
+          }
+        }
+        if (__ballot(!occ) == 0) break;
+        cur = kDone;
+        if (wsp > 0) {
+          --wsp;
+          cur = __builtin_amdgcn_readfirstlane(wv_par[wv][wsp]);
+          mask = wv_msk[wv][wsp];
+        }
+      }
+      occluded[i] = occ ? 1u : 0u;
+      return;
+    }
   }
   if (kAny) {
   uint32_t cur = kDone;
