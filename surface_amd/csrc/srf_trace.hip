@@ -47,6 +47,8 @@ k_trace_tri(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   __shared__ uint32_t stack[kSmall ? 1 : kTraceStack * kTraceBlock];
   __shared__ uint16_t stack_lo[kSmall ? kTraceStack * kTraceBlock : 1];
   __shared__ uint8_t stack_hi[kSmall ? kTraceStack * kTraceBlock : 1];
+  __shared__ uint32_t wnode[kAny ? kTraceBlock / 64 : 1][kTraceStack];
+  __shared__ uint64_t wmask[kAny ? kTraceBlock / 64 : 1][kTraceStack];
   const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -57,7 +59,73 @@ k_trace_tri(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   float t, u, v;
   uint32_t prim;
   bool occ;
-  if (kSmall)
+  const uint32_t sb = __builtin_amdgcn_readfirstlane(s.sbits);
+  if (kAny && __all(s.sbits == sb)) {
+    // Occlusion once per wavefront when the lanes' direction signs agree: at
+    // the fixed search distance tlimit the nodes and triangles a lane's
+    // traversal tests, and whether one occludes, do not depend on the order;
+    // the wavefront walks the union of its lanes' nodes and a lane leaves
+    // once occluded.
+    const bool filter = kEPS >= 1e-30f;
+    const int wv = threadIdx.x >> 6;
+    const uint64_t me = 1ull << (threadIdx.x & 63u);
+    uint64_t mask = __ballot(1);
+    uint32_t cur = 0;
+    int wsp = 0;
+    occ = false;
+    for (;;) {
+      uint4 meta = make_uint4(0u, 0u, 0u, 0u);
+      bool leaf = false;
+      while (cur < max_nodes) {
+        cur = __builtin_amdgcn_readfirstlane(cur);
+        const float4 *nd = nodes + 4 * (size_t)cur;
+        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+        meta = reinterpret_cast<const uint4 *>(nd)[3];
+        keep_loaded(n0, n1, n2);
+        if (meta.x != 0) {
+          leaf = true;
+          break;
+        }
+        const bool in = (mask & me) != 0 && !occ;
+        const bool h0 = in && ray_aabb<true>(s, tlimit, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
+        const bool h1 = in && ray_aabb<true>(s, tlimit, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
+        const int nearc = (sb >> meta.y) & 1u;
+        const uint64_t nm = __ballot(nearc ? h1 : h0), fm = __ballot(nearc ? h0 : h1);
+        const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+        if (nm) {
+          if (fm && wsp < kTraceStack) {
+            wnode[wv][wsp] = cf;
+            wmask[wv][wsp] = fm;
+            wsp++;
+          }
+          cur = cn;
+          mask = nm;
+        } else if (fm) {
+          cur = cf;
+          mask = fm;
+        } else {
+          break;
+        }
+      }
+      if (leaf) {
+        const bool in = (mask & me) != 0;
+        const uint32_t q0 = __builtin_amdgcn_readfirstlane(meta.w);
+        const uint32_t q1 = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, nprims));
+        for (uint32_t q = q0; q < q1; q++) {
+          const float4 *P = prims + 3 * (size_t)q;
+          const float4 A = P[0], B = P[1], C = P[2];
+          if (!in || occ || __float_as_uint(A.w) == prev) continue;
+          float tt, uu, vv;
+          if (tri_isect(s, A, B, C, ds, kEPS, filter, tlimit, tt, uu, vv) && tt < tlimit)
+            occ = true;
+        }
+      }
+      if (__ballot(!occ) == 0 || wsp == 0) break;
+      --wsp;
+      cur = __builtin_amdgcn_readfirstlane(wnode[wv][wsp]);
+      mask = wmask[wv][wsp];
+    }
+  } else if (kSmall)
     occ = tri_traverse<kAny>(nodes, prims, max_nodes, nprims, s, ds, kEPS, prev, tlimit,
                              Stack24<kTraceBlock>{stack_lo + threadIdx.x, stack_hi + threadIdx.x},
                              t, u, v, prim);
