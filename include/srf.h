@@ -352,6 +352,13 @@ srf_status srf_trace_any(const srf_accel *accel, const srf_ray *rays,
                          const float *t_max, uint64_t num_rays,
                          uint32_t *occluded, void *stream);
 
+/* Testing switch, no counterpart in the reference: with on != 0 the
+ * triangle and tetra traversals of this accelerator use the kernels and
+ * stack entry layouts meant for trees of 2^24 or more nodes (32-bit node
+ * indices), so that they can be checked on trees small enough for the
+ * oracle. Results are identical either way; only speed differs. */
+srf_status srf_debug_force_wide_stack(srf_accel *accel, int on);
+
 /* ------------------------------------------------------- LBVH stages */
 /* Occlusion with ray sorting: the rays are traced in the order of a sort
  * key of key_bits bits (0: in the given order) -- the direction octant over

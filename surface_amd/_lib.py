@@ -117,6 +117,7 @@ SIGNATURES = {
     "srf_trace_lines": (_i, [_vp, _vp, _u64, _vp, _vp, _vp]),
     "srf_trace_spheres": (_i, [_vp, _vp, _vp, _u64, _vp, _vp]),
     "srf_trace_any": (_i, [_vp, _vp, _vp, _u64, _vp, _vp]),
+    "srf_debug_force_wide_stack": (_i, [_vp, _i]),
     "srf_lbvh_morton30": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
     "srf_lbvh_sort30": (_i, [_vp, _vp, _u64, _vp, _vp, _vp]),
     "srf_lbvh_radix_tree30": (_i, [_vp, _vp, _u64, _vp, _vp]),

@@ -180,6 +180,11 @@ class _Accel:
                                      _stream(stream)))
         return out
 
+    def force_wide_stack(self, on=True):
+        """Testing switch: trace with the kernels meant for trees of 2^24 or
+        more nodes (32-bit stack entries); results are identical."""
+        self._require()
+        check(self.lib.srf_debug_force_wide_stack(self.handle, 1 if on else 0))
 
     def TraverseAnySorted(self, rays, t_max=None, key_bits=0, mode="plain", out=None,
                           stream=None):
@@ -207,21 +212,31 @@ class _Accel:
 
 
 def ao_rays_spheres(primary, hits, positions, radius=None, constant_radius=0.0, per_hit=8,
-                    eps=1e-3, dist=0.05, seed=1, stream=None):
+                    eps=1e-3, dist=0.05, seed=1, stream=None, out=None):
     """Cosine-weighted ambient-occlusion rays from sphere hits
     (srf_ao_rays_spheres). Returns (rays [m, 8] float32, t_max [m],
-    pixel [m] int32) with m the number written (hits x per_hit)."""
+    pixel [m] int32) with m the number written (hits x per_hit). With
+    out = (rays, t_max, pixel, count) preallocated (n * per_hit entries and
+    one int32 counter on the device) the call only enqueues the kernel and
+    returns out; the count stays on the device."""
     lib = load()
     n = primary.shape[0]
     cap = n * per_hit
-    rays = torch.empty((cap, 8), dtype=torch.float32, device=primary.device)
-    tmax = torch.empty(cap, dtype=torch.float32, device=primary.device)
-    pixel = torch.empty(cap, dtype=torch.int32, device=primary.device)
-    count = torch.zeros(1, dtype=torch.int32, device=primary.device)
+    if out is not None:
+        rays, tmax, pixel, count = out
+        if rays.shape[0] < cap or tmax.shape[0] < cap or pixel.shape[0] < cap:
+            raise SrfError(1, "ao_rays_spheres: out buffers hold fewer than n * per_hit rays")
+    else:
+        rays = torch.empty((cap, 8), dtype=torch.float32, device=primary.device)
+        tmax = torch.empty(cap, dtype=torch.float32, device=primary.device)
+        pixel = torch.empty(cap, dtype=torch.int32, device=primary.device)
+        count = torch.zeros(1, dtype=torch.int32, device=primary.device)
     check(lib.srf_ao_rays_spheres(_ptr(primary), _ptr(hits), n, _ptr(positions), _ptr(radius),
                                   float(constant_radius), int(per_hit), float(eps), float(dist),
                                   int(seed), _ptr(rays), _ptr(tmax), _ptr(pixel), _ptr(count),
                                   _stream(stream)))
+    if out is not None:
+        return out
     m = int(count.item())
     return rays[:m], tmax[:m], pixel[:m]
 

@@ -629,6 +629,13 @@ srf_status srf_trace_any(const srf_accel *accel, const srf_ray *rays, const floa
   return trace_any(a, rays, t_max, n, occluded, reinterpret_cast<hipStream_t>(stream));
 }
 
+srf_status srf_debug_force_wide_stack(srf_accel *accel, int on) {
+  Accel *a = reinterpret_cast<Accel *>(accel);
+  if (!a) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  a->wide_stack = on ? 1u : 0u;
+  return SRF_OK;
+}
+
 srf_status srf_lbvh_morton30(srf_ctx *ctx, srf_prim_kind kind, const void *desc,
                              const float bmin[3], const float bmax[3], uint32_t *codes,
                              void *stream) {

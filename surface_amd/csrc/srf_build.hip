@@ -2130,10 +2130,17 @@ size_t sort_pairs30_bytes(uint32_t n) {
 
 srf_status sort_pairs30_at(Ctx *ctx, size_t arena_off, const uint32_t *codes_in, uint32_t n,
                            uint32_t *codes_out, uint32_t *index_out, hipStream_t stream) {
-  const uint32_t tiles = sort_tiles(n);
   srf_status st = ensure_scratch(ctx, arena_off + sort_pairs30_bytes(n), stream);
   if (st != SRF_OK) return st;
-  Arena ar{(char *)ctx->scratch, ctx->scratch_bytes, arena_off};
+  return sort_pairs30_in((char *)ctx->scratch, ctx->scratch_bytes, arena_off, codes_in, n,
+                         codes_out, index_out, stream);
+}
+
+srf_status sort_pairs30_in(char *base, size_t size, size_t arena_off, const uint32_t *codes_in,
+                           uint32_t n, uint32_t *codes_out, uint32_t *index_out,
+                           hipStream_t stream) {
+  const uint32_t tiles = sort_tiles(n);
+  Arena ar{base, size, arena_off};
   SortScratch ss;
   ss.keys_b = ar.take<uint32_t>(n);
   ss.vals_b = ar.take<uint32_t>(n);

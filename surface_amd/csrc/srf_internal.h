@@ -29,8 +29,10 @@ constexpr int kTriMaxLeaf = 16;  // render/render_accel_triangle.cc:34
 constexpr int kTetMaxLeaf = 4;   // render/render_accel_tetra.cc:34
 constexpr int kSphMaxLeaf = 16;  // render/render_accel_particle.cc:51
 constexpr int kMaxDepth32 = 22;  // render/render_accel_triangle.cc:35
-// Branch levels are bounded: root, then depths 0..22 (a child deeper than
-// 22 is a leaf), so a traversal stack never holds more than 25 entries.
+// Branch levels are bounded: the root, then depths 0..22 (a child deeper
+// than 22 is a leaf), 24 branch nodes on any root-to-leaf path. A traversal
+// pushes at most one entry per branch node on its current path (the node it
+// descends into stays in registers), so a stack of 24 entries never fills.
 constexpr int kMaxBranchLevels = 24;
 constexpr int kStackSize = 32;
 // Leaves with more primitives than this are packed by a whole block during
@@ -100,6 +102,8 @@ struct Accel {
   uint64_t nv;           // vertices (tets: for scalar lookups)
   uint32_t nvps;         // solids: vertices per solid (5, 6, 8)
   uint32_t line_cap;     // lines: LineBuildOptions::cap (end discs tested)
+  uint32_t wide_stack;   // traverse with the stack layouts of trees of 2^24 or
+                         // more nodes (srf_debug_force_wide_stack)
   uint64_t capacity;     // primitives the buffers can hold
   void *nodes;           // TriNode / TetNode / SphNode
   void *prims;           // packed primitives (sorted order)
@@ -207,5 +211,10 @@ srf_status ao_rays_spheres(const srf_ray *primary, const srf_hit *hits, uint64_t
 srf_status sort_pairs30_at(Ctx *ctx, size_t arena_off, const uint32_t *codes_in, uint32_t n,
                            uint32_t *codes_out, uint32_t *index_out, hipStream_t stream);
 size_t sort_pairs30_bytes(uint32_t n);
+// The same in a caller-provided buffer [base, base + size) from arena_off on;
+// allocates nothing.
+srf_status sort_pairs30_in(char *base, size_t size, size_t arena_off, const uint32_t *codes_in,
+                           uint32_t n, uint32_t *codes_out, uint32_t *index_out,
+                           hipStream_t stream);
 
 }  // namespace srf

@@ -2515,7 +2515,9 @@ inline uint32_t trace_blocks(uint64_t n) {
 }
 inline uint32_t max_nodes(const Accel *a) { return (uint32_t)(2 * a->n - 1); }
 // every node index fits the 3-byte stack entries
-inline bool small_nodes(const Accel *a) { return 2 * a->n - 1 <= (1u << 24); }
+inline bool small_nodes(const Accel *a) { return !a->wide_stack && 2 * a->n - 1 <= (1u << 24); }
+// every node index fits the tetra kernel's 24-bit entries
+inline bool small_tet(const Accel *a) { return !a->wide_stack && max_nodes(a) < (1u << 24); }
 inline SphSub sph_sub(const Accel *a) {
   SphSub s{};
   if (a->kind == kSph && a->nbig) {
@@ -2548,7 +2550,7 @@ srf_status trace_tri(const Accel *a, const srf_ray *rays, uint64_t n, srf_hit *h
 srf_status trace_tet(const Accel *a, const srf_ray *rays, uint64_t n, const float *scalars,
                      srf_hit *hits, srf_tet_hit_ext *ext, hipStream_t s) {
   if (n == 0) return SRF_OK;
-  if (max_nodes(a) < (1u << 24))
+  if (small_tet(a))
     hipLaunchKernelGGL((k_trace_tet<false, true>), dim3(trace_blocks(n)), dim3(kTraceBlock), 0,
                        s, (const float4 *)a->nodes, (const float4 *)a->prims,
                        (const uint4 *)a->sorted_faces, a->params, rays, (uint32_t)n, scalars,
@@ -2633,7 +2635,7 @@ srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax, uin
                        (const float4 *)a->prims, a->params, rays, (uint32_t)n, nullptr, tmax,
                        occluded, max_nodes(a), (uint32_t)a->n);
   else if (a->kind == kTet) {
-    if (max_nodes(a) < (1u << 24))
+    if (small_tet(a))
       hipLaunchKernelGGL((k_trace_tet<true, true>), g, b, 0, s, (const float4 *)a->nodes,
                          (const float4 *)a->prims, (const uint4 *)a->sorted_faces, a->params,
                          rays, (uint32_t)n, nullptr, nullptr, nullptr, tmax, occluded,
@@ -2651,22 +2653,30 @@ srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax, uin
   return SRF_OK;
 }
 
-srf_status trace_any_sorted(Ctx *ctx, const Accel *a, const srf_ray *rays, const float *tmax,
-                            uint64_t n, uint32_t key_bits, int mode, uint32_t *occluded,
-                            hipStream_t s) {
-  if (n == 0) return SRF_OK;
+namespace {
+
+// Scratch of a sorted occlusion launch: [counter][keys][sorted keys][order]
+// then the sort's own slices.
+size_t any_sorted_head(uint32_t n) { return 256 + 3 * (((size_t)4 * n + 255) & ~(size_t)255); }
+size_t any_sorted_bytes(uint32_t n) { return any_sorted_head(n) + sort_pairs30_bytes(n); }
+
+srf_status check_any_sorted(const Accel *a, uint32_t key_bits) {
   if (a->kind == kTet || a->kind == kSol || a->kind == kLin)
     return set_error(SRF_ERR_UNSUPPORTED, "sorted any-hit takes triangles or spheres");
   if (key_bits > 30) return set_error(SRF_ERR_INVALID_ARG, "key_bits must be <= 30");
-  const uint32_t nn = (uint32_t)n;
-  // scratch: [counter][keys][sorted keys][order] then the sort's own slices
-  const size_t head = 256 + 3 * (((size_t)4 * nn + 255) & ~(size_t)255);
+  return SRF_OK;
+}
+
+// The sorted occlusion launch in the buffer [base, base + size) (at least
+// any_sorted_bytes(n) when rays are sorted or the launch is persistent);
+// allocates nothing.
+srf_status trace_any_sorted_in(char *base, size_t size, const Accel *a, const srf_ray *rays,
+                               const float *tmax, uint32_t nn, uint32_t key_bits, int mode,
+                               uint32_t *occluded, hipStream_t s) {
   const uint32_t *order = nullptr;
   uint32_t *counter = nullptr;
   if (key_bits > 0 || mode == SRF_LAUNCH_PERSISTENT) {
-    srf_status st = ensure_scratch(ctx, head + sort_pairs30_bytes(nn), s);
-    if (st != SRF_OK) return st;
-    Arena ar{(char *)ctx->scratch, ctx->scratch_bytes, 0};
+    Arena ar{base, size, 0};
     counter = ar.take<uint32_t>(64);
     uint32_t *keys = ar.take<uint32_t>(nn);
     uint32_t *skeys = ar.take<uint32_t>(nn);
@@ -2675,13 +2685,13 @@ srf_status trace_any_sorted(Ctx *ctx, const Accel *a, const srf_ray *rays, const
       hipLaunchKernelGGL(k_ray_keys, dim3((nn + 255) / 256), dim3(256), 0, s, rays, nn, a->params,
                          key_bits, keys);
       SRF_HIP(hipGetLastError());
-      st = sort_pairs30_at(ctx, head, keys, nn, skeys, ord, s);
+      srf_status st = sort_pairs30_in(base, size, any_sorted_head(nn), keys, nn, skeys, ord, s);
       if (st != SRF_OK) return st;
       order = ord;
     }
     if (mode != SRF_LAUNCH_PERSISTENT) counter = nullptr;
   }
-  uint32_t blocks = trace_blocks(n);
+  uint32_t blocks = trace_blocks(nn);
   if (counter) {
     int dev = 0, cus = 0;
     SRF_HIP(hipGetDevice(&dev));
@@ -2703,6 +2713,23 @@ srf_status trace_any_sorted(Ctx *ctx, const Accel *a, const srf_ray *rays, const
   return SRF_OK;
 }
 
+}  // namespace
+
+srf_status trace_any_sorted(Ctx *ctx, const Accel *a, const srf_ray *rays, const float *tmax,
+                            uint64_t n, uint32_t key_bits, int mode, uint32_t *occluded,
+                            hipStream_t s) {
+  if (n == 0) return SRF_OK;
+  srf_status st = check_any_sorted(a, key_bits);
+  if (st != SRF_OK) return st;
+  const uint32_t nn = (uint32_t)n;
+  if (key_bits > 0 || mode == SRF_LAUNCH_PERSISTENT) {
+    st = ensure_scratch(ctx, any_sorted_bytes(nn), s);
+    if (st != SRF_OK) return st;
+  }
+  return trace_any_sorted_in((char *)ctx->scratch, ctx->scratch_bytes, a, rays, tmax, nn,
+                             key_bits, mode, occluded, s);
+}
+
 srf_status ao_rays_spheres(const srf_ray *primary, const srf_hit *hits, uint64_t n,
                            const float *positions, const float *radius, float rconst,
                            uint32_t per_hit, float eps, float dist, uint64_t seed,
@@ -2718,3 +2745,4 @@ srf_status ao_rays_spheres(const srf_ray *primary, const srf_hit *hits, uint64_t
 }
 
 }  // namespace srf
+

@@ -323,6 +323,18 @@ def test_tetra_edge_cases_match_oracle(S, orc, case):
     assert np.array_equal(ext["subface"][hit], want_e["subface"][hit])
 
 
+def _branch_levels(nodes):
+    """Branch nodes on the longest root-to-leaf path of a DFS-ordered tree."""
+    best, stack = 0, [(0, 1)]
+    while stack:
+        i, d = stack.pop()
+        if nodes[i]["flag"] == 1:
+            continue
+        best = max(best, d)
+        stack += [(int(nodes[i]["data"][0]), d + 1), (int(nodes[i]["data"][1]), d + 1)]
+    return best
+
+
 def test_depth_capped_leaves(S, orc):
     say("depth-capped caterpillar tree")
     P, R = W.deep_chain_points()
@@ -332,6 +344,10 @@ def test_depth_capped_leaves(S, orc):
     nodes, idx = a.export()
     assert canonical_sph(nodes).tobytes() == canonical_sph(t["nodes"]).tobytes()
     assert np.array_equal(idx, t["indices"])
+    # the caterpillar reaches the cap: 24 branch levels (root, depths 0..22),
+    # a traversal stack of kTraceStack = 24 entries filled to the last one
+    assert _branch_levels(t["nodes"]) == 24
+    assert a.get_info()["max_depth"] == 24
     rays = W.pinhole_rays((300.0, -900.0, 200.0), (8.0, 8.0, 8.0), (0, 0, 1), 3.0, 64, 64)
     want = orc.trace_sph(t, P, R, 0.0, rays)
     got = S.hits_to_numpy(a.Traverse(dev(S, rays)))
@@ -381,7 +397,7 @@ def test_any_hit_agrees_with_closest_hit(S, torch):
     tmax = tmax.astype(np.float32)
     occ = a.TraverseAny(dev(S, rays), torch.from_numpy(tmax).cuda()).cpu().numpy()
     want = (hits["prim_id"] != W.NO_PRIM) & (hits["t"] < tmax)
-    assert (occ.astype(bool) != want).mean() < 1e-3
+    assert np.array_equal(occ.astype(bool), want)
 
 
 # ------------------------------------------------------------------ volume
