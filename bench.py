@@ -1,32 +1,47 @@
 """Benchmark of the MI355X acceleration-structure path (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 
-Workload (a "step"): the plain tetra configuration -- 64^3 jittered hexes split
-into 1,572,864 tetrahedra with a per-vertex scalar -- built into an LBVH
-(TetraAccel::Build32 semantics) and traced with 1024x1024 pinhole primary rays
-(closest hit: entry t, InterpolateTetra weights, interpolated scalar). Inputs
-are resident in HBM before timing starts. `value` is whole-step throughput
-(rays / (build + trace time)); the JSON line also carries the trace-only and
-build-only rates, the heightfield triangle case (522,242 triangles, 1024^2
-rays) and the differentiable volume (256^3, 1024^2 rays, fwd and fwd+bwd ms).
+Headline workload (a "step"): the batched incoherent particle configuration
+-- 16,777,216 Plummer-distributed spheres with lognormal radii (mu = -6,
+sigma = 0.5) -- built into an LBVH (ParticleAccel::Build32 semantics, with
+secondary trees for the depth-capped leaves), traced with 2048x2048 pinhole
+primary rays (closest hit), 8 cosine-weighted ambient-occlusion rays
+generated per hit, and those traced for occlusion with the autotuned ray
+sort (persistent launch). Inputs are resident in HBM before timing starts.
+`value` is the step's ray throughput: (primary + AO rays) / step time.
 
-roofline: the dominant kernel (the tetra traversal) against MI355X's 8 TB/s
-HBM peak, with algorithmic bytes per ray = 40 B (ray in, hit out) + 40 B per
-node loaded + 80 B per tetra loaded (64 B geometry + 16 B scalars), the node
-and tetra counts being the kernel's own loads on these rays (a wavefront
-loads each node and tetra once for its 64 rays; counted once and frozen
-below; the reference's per-ray traversal would move about 100 times as many
-bytes, reported beside them). cpu_baseline: the reference's own OpenMP code
-(oracle/_ref/libsrfref_omp.so, built from the reference sources) on a bounded
-sample of the same rays, on this host's cores.
+Every configuration also gets its own line in `configs` with GPU times, a
+`roofline` object for its dominant kernel and a `cpu_baseline` measured on
+this host in the same run:
+  heightfield  522,242 triangles, 1024^2 closest-hit rays (k_trace_tri_wave)
+  heightfield_1m  1,048,352 triangles, 1024^2 rays (the 1 Grays/s target)
+  tetgrid      1,572,864 tetras, 1024^2 rays + InterpolateTetra (k_trace_tet)
+  scene        64 meshes (15.3 M triangles) in one segmented build, 16 x 1024^2
+               rays through the two-level trace (k_scene_tri)
+  volume       256^3 FP32 voxels, 256-entry TF, 1024^2 rays (k_vol_fwd / k_vol_bwd)
+  particles    the headline (k_trace_sph primary, k_any_ordered AO)
+
+roofline: algorithmic bytes of the dominant kernel per launch divided by its
+launch time (HIP events on the launch stream), against the 8 TB/s HBM peak.
+Trace bytes follow the hot-path formula (40 B per ray + node and primitive
+bytes) with the node and primitive loads the kernel itself issues on these
+rays, counted once in an instrumented build and frozen below; `traffic` is
+the HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by
+the same counters on a plain copy), frozen below from the committed profiles.
+cpu_baseline: the reference's own OpenMP code (oracle/_ref, built from the
+reference sources) or, for the volume march that the reference leaves to
+user GLSL, the oracle's OpenMP restatement; 3 warm-ups, median of 10 timed
+repetitions, on a bounded sample of the same workload, on this host's cores.
 """
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 import json
 import os
+import platform
+import statistics
+import subprocess
 import sys
 import time
 
@@ -37,32 +52,38 @@ sys.path.insert(0, ROOT)
 
 METRIC = "closest-hit Mrays/s + LBVH build Mprims/s; volume fwd+bwd ms vs plain fwd"
 HBM_PEAK_GBS = 8000.0
-# Work per ray on the bench rays. k_trace_tet's pass A runs once per
-# wavefront and loads each node and tetra once for its 64 rays; it certifies
-# almost every ray's hit, and the rest rerun the reference's traversal from
-# just above it (pass B, per ray). Loads per ray, counted in an instrumented
-# build of the kernel over tet_grid(64) x tet_rays(64, 1024, 1024): pass A
-# 60.53 nodes and 28.24 tetras per wavefront = 0.9458 + 0.4413 per ray,
-# pass B 0.0588 nodes (its pops reload the parent node) + 0.0187 tetras.
-TET_NODES_PER_RAY = 0.9458 + 0.0588
-TET_TESTS_PER_RAY = 0.4413 + 0.0187
-# The reference's single traversal on the same rays (oracle, stats=True)
-# loads 180.0010 nodes and 60.3673 tetras per ray.
-TET_REF_NODES_PER_RAY = 180.0010
-TET_REF_TESTS_PER_RAY = 60.3673
-# The reference's triangle traversal (oracle counts on heightfield(512),
-# heightfield_rays(1024, 1024)); k_trace_tri_wave visits exactly these nodes
-# and triangles per ray but loads each once per wavefront, so the rate below
-# is what a per-ray traversal would have to move, not HBM traffic.
-TRI_NODES_PER_RAY = 64.3038
-TRI_TESTS_PER_RAY = 94.4435
-TET_BYTES_PER_RAY = 40 + 40 * TET_NODES_PER_RAY + 80 * TET_TESTS_PER_RAY
-TET_REF_BYTES_PER_RAY = 40 + 40 * TET_REF_NODES_PER_RAY + 80 * TET_REF_TESTS_PER_RAY
-# HBM bytes per k_trace_tet launch on the bench rays: rocprofv3 FETCH_SIZE and
-# WRITE_SIZE (KiB/dispatch), each scaled by the same counter on a 1 GiB copy
-# (fetch x 2.000, write x 1.000); profiles/traffic_k_trace_tet_tet64_k_trace_tri_heightfield512.md
-TET_TRACE_HBM_BYTES = (2.000 * 32442.81 + 1.000 * 49152.0) * 1024
-TRI_BYTES_PER_RAY = 40 + 64 * TRI_NODES_PER_RAY + 48 * TRI_TESTS_PER_RAY
+
+# ---------------------------------------------------------------------------
+# Frozen per-ray load counts of each dominant kernel on its bench rays (the
+# kernel's own loads: a load made once for a wavefront of 64 rays counts
+# once), from an instrumented build of the kernels, and the reference's own
+# traversal counts (oracle stats) for comparison.
+#
+# k_trace_tet (tet_grid(64), tet_rays(64, 1024, 1024)): pass A loads 60.53
+# nodes and 28.24 tetras per wavefront = 0.9458 + 0.4413 per ray, pass B
+# 0.0588 nodes and 0.0187 tetras per ray. The reference's traversal: 180.0010
+# nodes and 60.3673 tetras per ray.
+LOADS = {
+    "k_trace_tet": {"nodes": 0.9458 + 0.0588, "prims": 0.4413 + 0.0187,
+                    "node_bytes": 40, "prim_bytes": 80, "ref_nodes": 180.0010,
+                    "ref_prims": 60.3673},
+    # heightfield(512) x heightfield_rays(1024, 1024); the reference's counts
+    # (oracle stats) are 64.3038 nodes and 94.4435 triangles per ray
+    "k_trace_tri_wave": {"nodes": None, "prims": None, "node_bytes": 64, "prim_bytes": 48,
+                         "ref_nodes": 64.3038, "ref_prims": 94.4435},
+    "k_trace_tri_wave_1m": {"nodes": None, "prims": None, "node_bytes": 64, "prim_bytes": 48,
+                            "ref_nodes": None, "ref_prims": None},
+    "k_scene_tri": {"nodes": None, "prims": None, "node_bytes": 64, "prim_bytes": 48,
+                    "ref_nodes": None, "ref_prims": None},
+    "k_trace_sph": {"nodes": None, "prims": None, "node_bytes": 64, "prim_bytes": 16,
+                    "ref_nodes": None, "ref_prims": None},
+}
+# HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, KiB per dispatch,
+# each scaled by the same counter on a 1 GiB device copy: fetch x 2.000,
+# write x 1.000); None where not yet profiled.
+TRAFFIC = {
+    "k_trace_tet": (2.000 * 32442.81 + 1.000 * 49152.0) * 1024,
+}
 
 
 def log(msg):
@@ -70,7 +91,8 @@ def log(msg):
 
 
 def timed_events(torch, fn, reps):
-    """Average ms of fn() over reps runs, by HIP events on the current stream."""
+    """Average ms of fn() over reps runs, by HIP events on the current stream
+    (the stream every call here launches on)."""
     s = torch.cuda.Event(enable_timing=True)
     e = torch.cuda.Event(enable_timing=True)
     s.record()
@@ -81,57 +103,164 @@ def timed_events(torch, fn, reps):
     return s.elapsed_time(e) / reps
 
 
-def cpu_baseline(V, F, S, rays):
-    """Reference OpenMP path on a bounded sample: full Build32 of the tet mesh
-    once, then Traverse over every 8th image row (OpenMP dynamic over rows)."""
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                return ln.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_threads():
+    """The host threads the CPU leg uses: this process's CPU share."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        return os.cpu_count() or 1
+
+
+def median_s(fn, warmup=3, reps=10):
+    """BASELINE protocol: 3 warm-ups, median of 10 timed runs (seconds)."""
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts)
+
+
+def roofline(kernel, ms, n_rays, extra_bytes_per_ray=40.0):
+    """Roofline object of a traversal kernel from its frozen load counts."""
+    c = LOADS[kernel]
+    out = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel": kernel,
+           "kernel_ms": round(ms, 4)}
+    if c["nodes"] is not None:
+        bpr = extra_bytes_per_ray + c["node_bytes"] * c["nodes"] + c["prim_bytes"] * c["prims"]
+        gbs = bpr * n_rays / (ms * 1e-3) / 1e9
+        out.update({"achieved": round(gbs, 2), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                    "bytes_per_launch": round(bpr * n_rays),
+                    "loads_per_ray": {"nodes": c["nodes"], "prims": c["prims"]}})
+    else:
+        out.update({"achieved": None, "frac": None, "bytes_per_launch": None})
+    if c["ref_nodes"] is not None:
+        rb = extra_bytes_per_ray + c["node_bytes"] * c["ref_nodes"] + c["prim_bytes"] * c["ref_prims"]
+        out["reference_traversal_bytes_per_launch"] = round(rb * n_rays)
+        out["reference_traversal_equivalent_gbs"] = round(rb * n_rays / (ms * 1e-3) / 1e9, 1)
+    tr = TRAFFIC.get(kernel)
+    out["traffic"] = round(tr) if tr is not None else None
+    if tr is not None:
+        out["traffic_gbs"] = round(tr / (ms * 1e-3) / 1e9, 2)
+        out["traffic_frac"] = round(tr / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return out
+
+
+def reference_or_none():
     try:
         from oracle.oracle import Reference, ref_available
     except ImportError:
         return None
     if not ref_available("omp"):
         return None
-    ref = Reference("omp")
-    threads = os.cpu_count() or 1
-    t0 = time.perf_counter()
-    tree = ref.build("tet", V, F, keep=True)
-    build_s = time.perf_counter() - t0
-    width = 1024
-    sample = np.ascontiguousarray(rays.reshape(-1, width)[::8].reshape(-1))
-    t0 = time.perf_counter()
-    ref.trace(tree, sample, width=width, scalars=S, threads=threads)
-    trace_s = time.perf_counter() - t0
-    ref.free(tree)
-    mrays = len(sample) / trace_s / 1e6
-    return {"value": round(mrays, 3), "unit": "Mrays/s", "cores": threads, "kind": "reference",
-            "sample": (f"TetraAccel::Traverse over {len(sample)} of the 1024^2 rays (every 8th "
-                       f"row), OpenMP dynamic over rows, {threads} threads; Build32 of all "
-                       f"{len(F)} tets took {build_s:.2f} s ({len(F) / build_s / 1e6:.2f} Mprims/s)"),
-            "build_mprims_per_s": round(len(F) / build_s / 1e6, 3)}
+    return Reference("omp")
 
 
-def traffic_probe(torch, S, W, build, trace):
-    """Kernels for the FETCH_SIZE / WRITE_SIZE counter passes: the tetra build
-    and trace (each once after a warm-up), the heightfield trace, and a plain
-    1 GiB device-to-device copy whose counters give the scale factor."""
-    build()
-    trace()
-    torch.cuda.synchronize()
-    build()
-    trace()
-    Vh, Fh = W.heightfield(512)
+def row_sample(rays, width, every):
+    """Every `every`-th chunk of `width` consecutive rays (an image row of the
+    8x8-tiled ray order holds width / 8 tiles' rows)."""
+    return np.ascontiguousarray(rays.reshape(-1, width)[::every].reshape(-1))
+
+
+# ------------------------------------------------------------------- configs
+
+def cfg_triangles(torch, S, W, ref, grid, do_cpu, kernel):
+    n_side = 1024
+    V, F = W.heightfield(grid)
+    rays_np = W.heightfield_rays(n_side, n_side)
+    n = len(rays_np)
     tri = S.TriangleAccel()
-    tri.Build32(S.device_tensor(Vh), S.device_tensor(Fh))
-    rh = S.device_tensor(W.heightfield_rays(1024, 1024))
-    hh = torch.empty((len(rh), 4), dtype=torch.float32, device="cuda")
+    Vd, Fd = S.device_tensor(V), S.device_tensor(F)
+    rh = S.device_tensor(rays_np)
+    hh = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    tri.Build32(Vd, Fd)
     tri.Traverse(rh, hh)
-    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    log(f"traffic probe done: copy of {src.numel() * 4} bytes")
+    tb = timed_events(torch, lambda: tri.Build32(Vd, Fd), 10)
+    tt = timed_events(torch, lambda: tri.Traverse(rh, hh), 10)
+    hit = float((hh[:, 3].view(torch.int32) != -1).float().mean().item())
+    out = {"workload": f"heightfield {grid}x{grid} grid = {len(F):,} FP32 triangles; LBVH build "
+                       f"+ {n_side}x{n_side} pinhole closest-hit rays (prim id, t, u, v)",
+           "prims": int(len(F)), "rays": n, "build_ms": round(tb, 4),
+           "build_mprims_per_s": round(len(F) / tb / 1e3, 1), "trace_ms": round(tt, 4),
+           "trace_mrays_per_s": round(n / tt / 1e3, 1), "hit_fraction": round(hit, 4),
+           "roofline": roofline(kernel, tt, n)}
+    del tri
+    if do_cpu and ref is not None:
+        th = cpu_threads()
+        b = median_s(lambda: ref.free(ref.build("tri", V, F, keep=True)))
+        tree = ref.build("tri", V, F, keep=True)
+        sample = row_sample(rays_np, n_side, 8)
+        t = median_s(lambda: ref.trace(tree, sample, width=n_side, threads=th))
+        ref.free(tree)
+        out["cpu_baseline"] = {
+            "value": round(len(sample) / t / 1e6, 3), "unit": "Mrays/s", "cores": th,
+            "kind": "reference", "cpu_model": cpu_model(),
+            "build_mprims_per_s": round(len(F) / b / 1e6, 3),
+            "sample": (f"TriangleAccel::Traverse over {len(sample)} of the {n} rays (every 8th "
+                       f"row), OpenMP dynamic over rows; Build32 of all {len(F)} triangles; "
+                       "3 warm-ups, median of 10")}
+    return out
 
 
-def batched_scene(torch, S, W):
+def cfg_tetgrid(torch, S, W, ref, do_cpu):
+    V, F, Sc = W.tet_grid(64)
+    rays_np = W.tet_rays(64, 1024, 1024)
+    n = len(rays_np)
+    Vd, Fd, Sd = S.device_tensor(V), S.device_tensor(F), S.device_tensor(Sc)
+    rays = S.device_tensor(rays_np)
+    hits = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    ext = torch.empty((n, 8), dtype=torch.float32, device="cuda")
+    acc = S.TetraAccel()
+    acc.Build32(Vd, Fd)
+    acc.TraverseTetra(rays, Sd, hits, ext)
+    tb = timed_events(torch, lambda: acc.Build32(Vd, Fd), 10)
+    tt = timed_events(torch, lambda: acc.TraverseTetra(rays, Sd, hits, ext), 10)
+
+    def step():
+        acc.Build32(Vd, Fd)
+        acc.TraverseTetra(rays, Sd, hits, ext)
+
+    ts = timed_events(torch, step, 10)
+    info = acc.get_info()
+    out = {"workload": "tetra grid 64^3 hexes x 6 = 1,572,864 tets; LBVH build + 1024x1024 "
+                       "pinhole closest-hit rays with InterpolateTetra scalar",
+           "prims": int(len(F)), "rays": n, "nodes": int(info["num_nodes"]),
+           "build_ms": round(tb, 4), "build_mprims_per_s": round(len(F) / tb / 1e3, 1),
+           "trace_ms": round(tt, 4), "trace_mrays_per_s": round(n / tt / 1e3, 1),
+           "step_ms": round(ts, 4), "step_mrays_per_s": round(n / ts / 1e3, 1),
+           "roofline": roofline("k_trace_tet", tt, n)}
+    del acc
+    if do_cpu and ref is not None:
+        th = cpu_threads()
+        b = median_s(lambda: ref.free(ref.build("tet", V, F, keep=True)))
+        tree = ref.build("tet", V, F, keep=True)
+        sample = row_sample(rays_np, 1024, 8)
+        t = median_s(lambda: ref.trace(tree, sample, width=1024, scalars=Sc, threads=th))
+        ref.free(tree)
+        out["cpu_baseline"] = {
+            "value": round(len(sample) / t / 1e6, 3), "unit": "Mrays/s", "cores": th,
+            "kind": "reference", "cpu_model": cpu_model(),
+            "build_mprims_per_s": round(len(F) / b / 1e6, 3),
+            "sample": (f"TetraAccel::Traverse + InterpolateTetra over {len(sample)} of the {n} "
+                       "rays (every 8th row), OpenMP dynamic over rows; Build32 of all "
+                       f"{len(F)} tets; 3 warm-ups, median of 10")}
+    return out
+
+
+def cfg_scene(torch, S, W, ref, do_cpu):
     """64 meshes (log-uniform 10k..2M triangles) built by one segmented LBVH
     build, placed by rigid transforms under a top-level BVH, traced from 16
     orbit cameras of 1024x1024 rays: plain, persistent and autotuned
@@ -161,7 +290,11 @@ def batched_scene(torch, S, W):
     allrays = torch.cat(rays)
     hits = torch.empty((nr, 4), dtype=torch.float32, device="cuda")
     ids = torch.empty(nr, dtype=torch.int32, device="cuda")
-    out = {"meshes": len(meshes), "triangles": ntris, "rays": nr,
+    out = {"workload": "64 triangle meshes, log-uniform 10k-2M triangles (perturbed "
+                       "icospheres / heightfields), one segmented LBVH build; 16 cameras x "
+                       "1024^2 rays through the two-level trace (top-level BVH + per-instance "
+                       "transforms), persistent launch, autotuned, captured in a hipGraph",
+           "meshes": len(meshes), "triangles": ntris, "rays": nr,
            "batch_build_ms": round(batch_ms, 3),
            "batch_build_mprims_per_s": round(ntris / batch_ms / 1e3, 1),
            "per_mesh_build_loop_ms": round(loop_ms, 3), "scene_create_ms": round(scene_ms, 3)}
@@ -186,54 +319,221 @@ def batched_scene(torch, S, W):
     ms = timed_events(torch, graph.replay, 3)
     out["trace_graph_16_cameras_ms"] = round(ms, 3)
     out["trace_graph_mrays_per_s"] = round(nr / ms / 1e3, 1)
+    out["roofline"] = roofline("k_scene_tri", ms, nr)
     # one camera per launch, for comparison
     ms = timed_events(torch, lambda: [scene.Trace(r, hits[:len(r)], ids[:len(r)], mode="tuned")
                                       for r in rays], 2)
     out["trace_per_camera_launches_ms"] = round(ms, 3)
     out["hit_fraction"] = round(float((ids != -1).float().mean().item()), 4)
-    del graph, scene, accs
+    del graph, scene, accs, dm, allrays, rays, hits, ids
+    torch.cuda.empty_cache()
+    if do_cpu and ref is not None:
+        th = cpu_threads()
+        trees = []
+        t0 = time.perf_counter()
+        for V, F in meshes:
+            trees.append(ref.build("tri", V, F, keep=True))
+        build_s = time.perf_counter() - t0
+        sc = ref.scene_build(trees, T)
+        sample = np.concatenate([row_sample(c, 1024, 64) for c in cams])
+        t = median_s(lambda: ref.scene_trace(sc, sample, width=1024, threads=th))
+        ref.scene_free(sc)
+        for tr in trees:
+            ref.free(tr)
+        out["cpu_baseline"] = {
+            "value": round(len(sample) / t / 1e6, 3), "unit": "Mrays/s", "cores": th,
+            "kind": "reference", "cpu_model": cpu_model(),
+            "build_mprims_per_s": round(ntris / build_s / 1e6, 3),
+            "sample": (f"RenderGraph::Trace (BVHTree + per-element TriangleAccel::Traverse) "
+                       f"over {len(sample)} of the {nr} rays (every 64th row of each camera), "
+                       "OpenMP dynamic over rows, 3 warm-ups, median of 10; the 64 Build32 "
+                       "calls timed once")}
     return out
 
 
-def particles_ao(torch, S, W):
-    """16,777,216 Plummer spheres (lognormal radii): build, 2048x2048 primary
-    closest hits, 8 cosine-weighted AO rays per hit, occlusion traced
-    unsorted (plain and persistent) and with the autotuned ray sort."""
-    log("particles: 16M spheres, 2048^2 primary + 8 AO rays per hit")
-    P, R = W.plummer_spheres(1 << 24)
-    Pd, Rd = S.device_tensor(P), S.device_tensor(R)
-    del P, R
-    acc = S.ParticleAccel()
-    acc.Build32(Pd, Rd, 0.0)
-    torch.cuda.synchronize()
-    build_ms = timed_events(torch, lambda: acc.Build32(Pd, Rd, 0.0), 3)
-    prim = S.device_tensor(W.sphere_rays(2048, 2048))
-    np_ = prim.shape[0]
-    hits = torch.empty((np_, 4), dtype=torch.float32, device="cuda")
-    acc.Traverse(prim, hits)
-    prim_ms = timed_events(torch, lambda: acc.Traverse(prim, hits), 3)
-    rays, tmax, pix = S.ao_rays_spheres(prim, hits, Pd, Rd, 0.0, 8, 1e-3, 0.05, 7)
-    gen_ms = timed_events(torch, lambda: S.ao_rays_spheres(prim, hits, Pd, Rd, 0.0, 8, 1e-3,
-                                                           0.05, 7), 3)
-    na = rays.shape[0]
-    occ = torch.empty(na, dtype=torch.int32, device="cuda")
-    out = {"spheres": 1 << 24, "primary_rays": np_, "ao_rays": na,
-           "build_ms": round(build_ms, 3), "build_mprims_per_s": round((1 << 24) / build_ms / 1e3, 1),
-           "primary_ms": round(prim_ms, 3), "primary_mrays_per_s": round(np_ / prim_ms / 1e3, 1),
-           "primary_hit_fraction": round(na / 8 / np_, 4), "ao_gen_ms": round(gen_ms, 3)}
-    for name, bits, mode in (("unsorted_plain", 0, "plain"), ("unsorted_persistent", 0, "persistent")):
-        ms = timed_events(torch, lambda: acc.TraverseAnySorted(rays, tmax, bits, mode, occ), 3)
-        out[f"ao_{name}_ms"] = round(ms, 3)
-        out[f"ao_{name}_mrays_per_s"] = round(na / ms / 1e3, 1)
-    cfg = acc.autotune_any(rays, tmax, occ)
-    out["autotuned"] = cfg
-    mode = "persistent" if cfg["persistent"] else "plain"
-    ms = timed_events(torch, lambda: acc.TraverseAnySorted(rays, tmax, cfg["key_bits"], mode, occ), 3)
-    out["ao_tuned_ms"] = round(ms, 3)
-    out["ao_tuned_mrays_per_s"] = round(na / ms / 1e3, 1)
-    out["ao_occluded_fraction"] = round(float(occ.float().mean().item()), 4)
-    del acc, rays, tmax, pix, occ, hits, prim, Pd, Rd
+def cfg_volume(torch, S, W, orc, do_cpu):
+    log("volume 256^3")
+    from surface_amd.volume import Volume
+    vox_np = W.gaussian_volume(256)
+    tf_np = W.transfer_function(256)
+    rays_np = W.volume_rays(1024, 1024)
+    n = len(rays_np)
+    vox = torch.from_numpy(vox_np).cuda()
+    tf = torch.from_numpy(tf_np).cuda()
+    vr = S.device_tensor(rays_np)
+    vol = Volume(vox, (256, 256, 256), (1, 1, 1))
+    step_len = 0.5 / 255
+    gv = torch.zeros_like(vox)
+    gt = torch.zeros_like(tf)
+    grad = torch.ones((n, 4), dtype=torch.float32, device="cuda")
+    ck = torch.empty((n, vol.checkpoint_slots(step_len)), dtype=torch.float32, device="cuda")
+    rgba = vol.march(tf, step_len, vr)
+    fwd = timed_events(torch, lambda: vol.march(tf, step_len, vr, rgba=rgba), 5)
+    vol.march(tf, step_len, vr, ck, rgba)
+    bwd = timed_events(torch, lambda: vol.march_backward(tf, step_len, vr, grad, gv, gt, ck), 3)
+
+    def fwd_bwd():
+        vol.march(tf, step_len, vr, ck, rgba)
+        vol.march_backward(tf, step_len, vr, grad, gv, gt, ck)
+
+    fwd_bwd()
+    fb = timed_events(torch, fwd_bwd, 3)
+    ns = samples_per_ray(rays_np, step_len)
+    samples = float(ns.sum())
+    fwd_bytes = 40 * n + 32 * samples
+    bwd_bytes = 56 * n + 96 * samples
+    out = {"workload": "volume 256^3 FP32 (64 anisotropic Gaussians), 256-entry RGBA TF, "
+                       "1024x1024 rays, 0.5-voxel trilinear steps, emission-absorption; "
+                       "forward and forward + adjoint (grads w.r.t. voxels and TF)",
+           "rays": n, "samples": int(samples), "fwd_ms": round(fwd, 3), "bwd_ms": round(bwd, 3),
+           "fwd_bwd_ms": round(fb, 3), "fwd_bwd_over_fwd": round(fb / fwd, 3),
+           "fwd_mrays_per_s": round(n / fwd / 1e3, 1),
+           "roofline": {"bound": "hbm", "kernel": "k_vol_fwd", "unit": "GB/s",
+                        "peak": HBM_PEAK_GBS, "kernel_ms": round(fwd, 4),
+                        "bytes_per_launch": round(fwd_bytes),
+                        "achieved": round(fwd_bytes / (fwd * 1e-3) / 1e9, 2),
+                        "frac": round(fwd_bytes / (fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "traffic": TRAFFIC.get("k_vol_fwd"),
+                        "bwd": {"kernel": "k_vol_bwd", "kernel_ms": round(bwd, 4),
+                                "bytes_per_launch": round(bwd_bytes),
+                                "achieved": round(bwd_bytes / (bwd * 1e-3) / 1e9, 2),
+                                "frac": round(bwd_bytes / (bwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                "traffic": TRAFFIC.get("k_vol_bwd")}}}
+    del vol, vox, tf, vr, gv, gt, grad, ck, rgba
     torch.cuda.empty_cache()
+    if do_cpu and orc is not None:
+        th = cpu_threads()
+        sample = row_sample(rays_np, 1024, 32)
+        dims = (256, 256, 256)
+        t = median_s(lambda: orc.volume_fwd(vox_np, dims, (1, 1, 1), tf_np, step_len, sample,
+                                            threads=th))
+        out["cpu_baseline"] = {
+            "value": round(len(sample) / t / 1e6, 4), "unit": "Mrays/s", "cores": th,
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": (f"the oracle's C restatement of the march (FilterTexture3DFloat + TF + "
+                       f"EA, OpenMP over rays) on {len(sample)} of the {n} rays (every 32nd "
+                       "row), 3 warm-ups, median of 10; the reference leaves the march to "
+                       "user GLSL and evaluates fragments serially")}
+    return out
+
+
+def samples_per_ray(rays, step):
+    """Samples of the fixed-step march per ray: t_k = t0 + (k + 1/2) step
+    below the exit of the unit cube (slab entry clamped at 0), in float32."""
+    o = rays["org"].astype(np.float32)
+    d = rays["dir"].astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = np.float32(1.0) / d
+        a = (np.float32(0.0) - o) * inv
+        b = (np.float32(1.0) - o) * inv
+    tn = np.nanmax(np.minimum(a, b), axis=1)
+    tf = np.nanmin(np.maximum(a, b), axis=1)
+    tn = np.maximum(tn, np.float32(0.0))
+    ns = np.where(tf > tn, np.ceil((tf - tn) / np.float32(step) - np.float32(0.5)), 0)
+    return np.maximum(ns, 0)
+
+
+class Particles:
+    """The headline configuration: build + primary + AO generation + AO
+    occlusion on 16,777,216 spheres and 2048^2 primary rays."""
+
+    def __init__(self, torch, S, W):
+        log("particles: 16M spheres, 2048^2 primary + 8 AO rays per hit")
+        self.torch, self.S = torch, S
+        P, R = W.plummer_spheres(1 << 24)
+        self.P_np, self.R_np = P, R
+        self.Pd, self.Rd = S.device_tensor(P), S.device_tensor(R)
+        self.acc = S.ParticleAccel()
+        self.acc.Build32(self.Pd, self.Rd, 0.0)
+        self.prim_np = W.sphere_rays(2048, 2048)
+        self.prim = S.device_tensor(self.prim_np)
+        self.np = self.prim.shape[0]
+        self.hits = torch.empty((self.np, 4), dtype=torch.float32, device="cuda")
+        self.acc.Traverse(self.prim, self.hits)
+        self.cap = 8 * self.np
+        self.ao = torch.empty((self.cap, 8), dtype=torch.float32, device="cuda")
+        self.tmax = torch.empty(self.cap, dtype=torch.float32, device="cuda")
+        self.pix = torch.empty(self.cap, dtype=torch.int32, device="cuda")
+        self.count = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self.occ = torch.empty(self.cap, dtype=torch.int32, device="cuda")
+        self.gen()
+        torch.cuda.synchronize()
+        self.na = int(self.count.item())
+        # the AO launch shape: tuned once on this frame's rays, before timing
+        self.cfg = self.acc.autotune_any(self.ao[:self.na], self.tmax[:self.na],
+                                         self.occ[:self.na])
+        self.mode = "persistent" if self.cfg["persistent"] else "plain"
+
+    def build(self):
+        self.acc.Build32(self.Pd, self.Rd, 0.0)
+
+    def primary(self):
+        self.acc.Traverse(self.prim, self.hits)
+
+    def gen(self):
+        self.S.ao_rays_spheres(self.prim, self.hits, self.Pd, self.Rd, 0.0, 8, 1e-3, 0.05, 7,
+                               out=(self.ao, self.tmax, self.pix, self.count))
+
+    def ao_trace(self):
+        # the ray count of the frame is fixed (the same hits every step)
+        self.acc.TraverseAnySorted(self.ao[:self.na], self.tmax[:self.na], self.cfg["key_bits"],
+                                   self.mode, self.occ[:self.na])
+
+    def step(self):
+        self.build()
+        self.primary()
+        self.gen()
+        self.ao_trace()
+
+
+def cfg_particles(torch, S, W, ref, pt, do_cpu):
+    out = {"workload": "16,777,216 Plummer spheres (a = 1), lognormal radii (mu = -6, "
+                       "sigma = 0.5); LBVH build, 2048x2048 pinhole primary closest hits, 8 "
+                       "cosine-weighted AO rays per hit traced for occlusion (any hit, "
+                       "autotuned ray sort, persistent launch)",
+           "spheres": 1 << 24, "primary_rays": pt.np, "ao_rays": pt.na,
+           "autotuned": pt.cfg}
+    b = timed_events(torch, pt.build, 3)
+    p = timed_events(torch, pt.primary, 3)
+    g = timed_events(torch, pt.gen, 3)
+    a = timed_events(torch, pt.ao_trace, 3)
+    out.update({"build_ms": round(b, 3), "build_mprims_per_s": round((1 << 24) / b / 1e3, 1),
+                "primary_ms": round(p, 3), "primary_mrays_per_s": round(pt.np / p / 1e3, 1),
+                "ao_gen_ms": round(g, 3), "ao_ms": round(a, 3),
+                "ao_mrays_per_s": round(pt.na / a / 1e3, 1),
+                "primary_hit_fraction": round(pt.na / 8 / pt.np, 4),
+                "ao_occluded_fraction": round(float(pt.occ[:pt.na].float().mean().item()), 4)})
+    for name, bits, mode in (("unsorted_plain", 0, "plain"),
+                             ("unsorted_persistent", 0, "persistent")):
+        ms = timed_events(torch, lambda: pt.acc.TraverseAnySorted(
+            pt.ao[:pt.na], pt.tmax[:pt.na], bits, mode, pt.occ[:pt.na]), 3)
+        out[f"ao_{name}_ms"] = round(ms, 3)
+    out["roofline"] = roofline("k_trace_sph", p, pt.np)
+    if do_cpu and ref is not None:
+        th = cpu_threads()
+        bs = median_s(lambda: ref.free(ref.build("sph", pt.P_np, R=pt.R_np, keep=True)))
+        tree = ref.build("sph", pt.P_np, R=pt.R_np, keep=True)
+        sample = row_sample(pt.prim_np, 2048, 512)
+        ps = median_s(lambda: ref.trace(tree, sample, width=2048, threads=th))
+        ao_np = pt.ao[:pt.na].cpu().numpy().view(W.RAY_DTYPE).reshape(-1)
+        ao_sample = np.ascontiguousarray(ao_np[::4096])
+        as_ = median_s(lambda: ref.trace(tree, ao_sample, width=64, threads=th))
+        ref.free(tree)
+        prate = len(sample) / ps
+        arate = len(ao_sample) / as_
+        step_s = bs + pt.np / prate + pt.na / arate
+        out["cpu_baseline"] = {
+            "value": round((pt.np + pt.na) / step_s / 1e6, 4), "unit": "Mrays/s", "cores": th,
+            "kind": "reference", "cpu_model": cpu_model(),
+            "build_mprims_per_s": round((1 << 24) / bs / 1e6, 3),
+            "primary_mrays_per_s": round(prate / 1e6, 4),
+            "ao_mrays_per_s": round(arate / 1e6, 4),
+            "sample": (f"ParticleAccel::Build32 of all 16,777,216 spheres; "
+                       f"ParticleAccel::Traverse over {len(sample)} of the {pt.np} primary rays "
+                       f"(every 512th row) and {len(ao_sample)} of the {pt.na} AO rays (every "
+                       "4096th; the reference answers shadow() with a closest-hit Traverse), "
+                       "OpenMP dynamic, 3 warm-ups, median of 10 each; value = rays of the "
+                       "step over build time + rays at the sampled rates")}
     return out
 
 
@@ -282,16 +582,83 @@ def solids_lines(torch, S, W):
     return out
 
 
+def probe(torch, S, W):
+    """Each configuration's kernels a few times and a 1 GiB device copy (the
+    scale of the HBM counters), for rocprofv3 counter passes; no JSON line."""
+    pt = Particles(torch, S, W)
+    for _ in range(2):
+        pt.step()
+    torch.cuda.synchronize()
+    del pt
+    torch.cuda.empty_cache()
+    log("probe: particles done")
+    for grid in (512, 725):
+        V, F = W.heightfield(grid)
+        tri = S.TriangleAccel()
+        Vd, Fd = S.device_tensor(V), S.device_tensor(F)
+        rh = S.device_tensor(W.heightfield_rays(1024, 1024))
+        hh = torch.empty((len(rh), 4), dtype=torch.float32, device="cuda")
+        for _ in range(2):
+            tri.Build32(Vd, Fd)
+            tri.Traverse(rh, hh)
+        torch.cuda.synchronize()
+    log("probe: heightfields done")
+    V, F, Sc = W.tet_grid(64)
+    acc = S.TetraAccel()
+    Vd, Fd, Sd = S.device_tensor(V), S.device_tensor(F), S.device_tensor(Sc)
+    rays = S.device_tensor(W.tet_rays(64, 1024, 1024))
+    hits = torch.empty((len(rays), 4), dtype=torch.float32, device="cuda")
+    ext = torch.empty((len(rays), 8), dtype=torch.float32, device="cuda")
+    for _ in range(2):
+        acc.Build32(Vd, Fd)
+        acc.TraverseTetra(rays, Sd, hits, ext)
+    torch.cuda.synchronize()
+    log("probe: tetgrid done")
+    from surface_amd.volume import Volume
+    vox = torch.from_numpy(W.gaussian_volume(256)).cuda()
+    tf = torch.from_numpy(W.transfer_function(256)).cuda()
+    vr = S.device_tensor(W.volume_rays(1024, 1024))
+    vol = Volume(vox, (256, 256, 256), (1, 1, 1))
+    step_len = 0.5 / 255
+    gv, gt = torch.zeros_like(vox), torch.zeros_like(tf)
+    grad = torch.ones((len(vr), 4), dtype=torch.float32, device="cuda")
+    ck = torch.empty((len(vr), vol.checkpoint_slots(step_len)), dtype=torch.float32,
+                     device="cuda")
+    for _ in range(2):
+        rgba = vol.march(tf, step_len, vr, ck)
+        vol.march_backward(tf, step_len, vr, grad, gv, gt, ck)
+    torch.cuda.synchronize()
+    log("probe: volume done")
+    meshes, T = W.batch_scene(64)
+    dm = [(S.device_tensor(V), S.device_tensor(F)) for V, F in meshes]
+    accs = S.build_triangles_batch(dm)
+    scene = S.Scene(accs, T)
+    allrays = torch.cat([S.device_tensor(c) for c in W.scene_cameras(T, 16, 1024, 1024)])
+    sh = torch.empty((len(allrays), 4), dtype=torch.float32, device="cuda")
+    ids = torch.empty(len(allrays), dtype=torch.int32, device="cuda")
+    for _ in range(2):
+        scene.Trace(allrays, sh, ids, mode="persistent")
+    torch.cuda.synchronize()
+    log("probe: scene done")
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"probe done: copy of {src.numel() * 4} bytes")
+    del rgba
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-extras", action="store_true", help="tetra step only")
-    ap.add_argument("--traffic-probe", action="store_true",
-                    help="one launch of each traversal kernel plus a 1 GiB device copy "
-                         "(the calibration for HBM-counter runs); prints no JSON line")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline step only (no other configurations)")
+    ap.add_argument("--probe", action="store_true",
+                    help="each configuration's kernels twice plus a 1 GiB device copy (for "
+                         "rocprofv3 counter passes); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); rays shard by "
@@ -304,93 +671,49 @@ def main():
     from surface_amd import workloads as W
 
     assert torch.cuda.is_available(), "no GPU visible"
-    log("generating the 64^3 tetra grid")
-    V, F, Sc = W.tet_grid(64)
-    rays_np = W.tet_rays(64, 1024, 1024)
-    n_rays = len(rays_np)
-    Vd, Fd, Sd = S.device_tensor(V), S.device_tensor(F), S.device_tensor(Sc)
-    rays = S.device_tensor(rays_np)
-    hits = torch.empty((n_rays, 4), dtype=torch.float32, device="cuda")
-    ext = torch.empty((n_rays, 8), dtype=torch.float32, device="cuda")
-    acc = S.TetraAccel()
-
-    def build():
-        acc.Build32(Vd, Fd)
-
-    def trace():
-        acc.TraverseTetra(rays, Sd, hits, ext)
-
-    def step():
-        build()
-        trace()
-
-    if args.traffic_probe:
-        traffic_probe(torch, S, W, build, trace)
+    if args.probe:
+        probe(torch, S, W)
         return
-
+    pt = Particles(torch, S, W)
     log(f"warmup {args.warmup}")
     for _ in range(args.warmup):
-        step()
+        pt.step()
     torch.cuda.synchronize()
     log(f"timing {args.steps} steps")
     t0 = time.perf_counter()
-    ms_step = timed_events(torch, step, args.steps)
+    ms_step = timed_events(torch, pt.step, args.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms_build = timed_events(torch, build, max(3, args.steps // 2))
-    ms_trace = timed_events(torch, trace, max(3, args.steps // 2))
-    info = acc.get_info()
-    tet_gbs = TET_BYTES_PER_RAY * n_rays / (ms_trace * 1e-3) / 1e9
+    rays_per_step = pt.np + pt.na
+    value = rays_per_step / ms_step / 1e3
 
+    do_cpu = not args.no_cpu
+    ref = reference_or_none() if do_cpu else None
+    orc = None
+    if do_cpu:
+        try:
+            from oracle.oracle import Oracle
+            orc = Oracle()
+        except (ImportError, OSError):
+            orc = None
+    log("particles: per-phase times" + (" and CPU baseline" if ref else ""))
+    parts = cfg_particles(torch, S, W, ref, pt, do_cpu)
+    del pt
+    torch.cuda.empty_cache()
+    configs = {"particles": parts}
     extras = {}
     if not args.no_extras:
-        log("heightfield triangles")
-        Vh, Fh = W.heightfield(512)
-        tri = S.TriangleAccel()
-        Vhd, Fhd = S.device_tensor(Vh), S.device_tensor(Fh)
-        rh = S.device_tensor(W.heightfield_rays(1024, 1024))
-        hh = torch.empty((n_rays, 4), dtype=torch.float32, device="cuda")
-        tri.Build32(Vhd, Fhd)
-        tri.Traverse(rh, hh)
-        tb = timed_events(torch, lambda: tri.Build32(Vhd, Fhd), 10)
-        tt = timed_events(torch, lambda: tri.Traverse(rh, hh), 10)
-        extras["heightfield_522242_tris"] = {
-            "build_ms": round(tb, 4), "build_mprims_per_s": round(len(Fh) / tb / 1e3, 1),
-            "trace_ms": round(tt, 4), "trace_mrays_per_s": round(n_rays / tt / 1e3, 1),
-            "per_ray_traversal_equivalent_gbs": round(TRI_BYTES_PER_RAY * n_rays / (tt * 1e-3) / 1e9, 1)}
-        del tri
-        log("volume 256^3")
-        from surface_amd.volume import Volume
-        vox = torch.from_numpy(W.gaussian_volume(256)).cuda()
-        tf = torch.from_numpy(W.transfer_function(256)).cuda()
-        vr = S.device_tensor(W.volume_rays(1024, 1024))
-        vol = Volume(vox, (256, 256, 256), (1, 1, 1))
-        step_len = 0.5 / 255
-        gv = torch.zeros_like(vox)
-        gt = torch.zeros_like(tf)
-        grad = torch.ones((n_rays, 4), dtype=torch.float32, device="cuda")
-        ck = torch.empty((n_rays, vol.checkpoint_slots(step_len)), dtype=torch.float32,
-                         device="cuda")
-        rgba = vol.march(tf, step_len, vr)
-        fwd = timed_events(torch, lambda: vol.march(tf, step_len, vr, rgba=rgba), 3)
-
-        def fwd_bwd():
-            vol.march(tf, step_len, vr, ck, rgba)
-            vol.march_backward(tf, step_len, vr, grad, gv, gt, ck)
-
-        fwd_bwd()
-        fb = timed_events(torch, fwd_bwd, 2)
-        extras["volume_256cubed_1M_rays"] = {"fwd_ms": round(fwd, 3), "fwd_bwd_ms": round(fb, 3),
-                                             "fwd_bwd_over_fwd": round(fb / fwd, 3)}
-
-    if not args.no_extras:
-        extras["batched_scene_64_meshes_16_cameras"] = batched_scene(torch, S, W)
-        extras["particles_16M_ao"] = particles_ao(torch, S, W)
+        log("heightfield 512")
+        configs["heightfield"] = cfg_triangles(torch, S, W, ref, 512, do_cpu, "k_trace_tri_wave")
+        log("heightfield 725 (1,048,352 triangles)")
+        configs["heightfield_1m"] = cfg_triangles(torch, S, W, ref, 725, do_cpu,
+                                                  "k_trace_tri_wave_1m")
+        log("tetra grid 64")
+        configs["tetgrid"] = cfg_tetgrid(torch, S, W, ref, do_cpu)
+        configs["volume"] = cfg_volume(torch, S, W, orc, do_cpu)
+        configs["scene"] = cfg_scene(torch, S, W, ref, do_cpu)
         extras["solids_and_lines"] = solids_lines(torch, S, W)
 
-    cpu = None if args.no_cpu else cpu_baseline(V, F, Sc, rays_np)
-
-    value = n_rays / ms_step / 1e3  # Mrays/s over build + trace
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -404,20 +727,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "tetra grid 64^3 hexes x 6 = 1,572,864 tets; LBVH build + "
-                               "1024x1024 pinhole closest-hit rays with InterpolateTetra scalar",
-                   "prims": int(len(F)), "rays": int(n_rays), "nodes": int(info["num_nodes"])},
-        "trace_mrays_per_s": round(n_rays / ms_trace / 1e3, 3),
-        "build_mprims_per_s": round(len(F) / ms_build / 1e3, 3),
-        "build_ms": round(ms_build, 4),
-        "trace_ms": round(ms_trace, 4),
+        "config": {"workload": parts["workload"], "spheres": 1 << 24,
+                   "primary_rays": parts["primary_rays"], "ao_rays": parts["ao_rays"],
+                   "rays_per_step": rays_per_step},
+        "build_ms": parts["build_ms"],
+        "primary_ms": parts["primary_ms"],
+        "ao_ms": parts["ao_ms"],
         "wall_s_timed": round(wall, 3),
-        "roofline": {"bound": "hbm", "achieved": round(tet_gbs, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(tet_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": round(TET_TRACE_HBM_BYTES), "kernel": "k_trace_tet",
-                     "bytes_per_launch": round(TET_BYTES_PER_RAY * n_rays),
-                     "reference_traversal_bytes_per_launch": round(TET_REF_BYTES_PER_RAY * n_rays)},
-        "cpu_baseline": cpu,
+        "roofline": parts["roofline"],
+        "cpu_baseline": parts.get("cpu_baseline"),
+        "configs": configs,
         "extras": extras,
     }
     print(json.dumps(line), flush=True)

@@ -65,6 +65,7 @@ typedef struct srf_ctx srf_ctx;
 typedef struct srf_accel srf_accel;
 typedef struct srf_volume srf_volume;
 typedef struct srf_scene srf_scene;
+typedef struct srf_sparse_volume srf_sparse_volume;
 
 /* Ray: the fields of render/render_ray.h:47-151 that traversal reads.
  * flags bit 0 = double_sided (Ray::double_sided; eye rays use 1,
@@ -125,6 +126,24 @@ typedef struct srf_line_hit_ext {
   float normal[3];
   float pad;
 } srf_line_hit_ext;
+
+/* The Intersection fields BuildIntersection fills for a closest hit
+ * (render/render_intersection.h:18-62): position, geometric and shading
+ * normal, tangent, binormal and the vertex ids f0..f3. Fields a kind leaves
+ * unset (triangles and tetras: tangent, binormal; triangles: f3) and every
+ * field of a miss are 0. 80 bytes. */
+typedef struct srf_isect {
+  float position[3];
+  uint32_t f0;
+  float geometric[3];
+  uint32_t f1;
+  float normal[3];
+  uint32_t f2;
+  float tangent[3];
+  uint32_t f3;
+  float binormal[3];
+  float pad;
+} srf_isect;
 
 /* Sphere self-intersection inputs, Ray::prev_hit_t / prev_hit_normal
  * (render/render_ray.h:137-139), used when prev_prim_id names a sphere. */
@@ -283,6 +302,24 @@ srf_status srf_accel_rebuild_triangles_batch(srf_ctx *ctx, uint32_t num_meshes,
                                              srf_accel *const *accels,
                                              const srf_tri_desc *descs_host, void *stream);
 
+/* Build32 with HOST arrays, as AccelBuilder hands the reference its GL client
+ * arrays (gles/gles_accel_builder.cc:545-572 -> TriangleAccel::Build32,
+ * render/render_accel_triangle.h:78; TetraAccel / ParticleAccel / SolidAccel
+ * ::Build32 and LineAccel::Build likewise). kind selects the descriptor
+ * desc points to (srf_tri_desc for SRF_PRIM_TRIANGLES, srf_tet_desc,
+ * srf_sphere_desc for SRF_PRIM_POINTS, srf_solid_desc for the three solid
+ * kinds, srf_line_desc), whose arrays are in host memory. The arrays are
+ * copied into device memory the accelerator owns (kept for
+ * BuildIntersection); the call synchronises with the build, so the host
+ * arrays may be reused once it returns. */
+srf_status srf_accel_build_host(srf_ctx *ctx, srf_prim_kind kind, const void *desc,
+                                void *stream, srf_accel **out);
+/* srf_accel_rebuild with host arrays (same kind as the accelerator);
+ * reuses the accelerator's device copies when the new arrays fit;
+ * synchronises. */
+srf_status srf_accel_rebuild_host(srf_ctx *ctx, srf_accel *accel, const void *desc,
+                                  void *stream);
+
 /* Build32 called again on an existing accelerator object (the reference
  * rebuilds in place: nodes_.clear(), render/render_accel_triangle.cc:1552;
  * AccelBuilder rebuilds dynamic draws every frame,
@@ -344,6 +381,42 @@ srf_status srf_trace_lines(const srf_accel *accel, const srf_ray *rays, uint64_t
 srf_status srf_trace_spheres(const srf_accel *accel, const srf_ray *rays,
                              const srf_sphere_prev *prev, uint64_t num_rays,
                              srf_hit *hits, void *stream);
+/* BuildIntersection for a batch of closest hits (hits from
+ * srf_trace_closest / srf_trace_tetra / srf_trace_spheres on the same rays):
+ * triangles render/render_accel_triangle.cc:2439-2507 (f0..f2, position =
+ * org + t dir, normalised geometric normal), tetras
+ * render/render_accel_tetra.cc:2124-2196 (f0..f3, position, normal of the
+ * entry face ext->subface; the weights d0..d3 are in srf_tet_hit_ext),
+ * spheres render/render_accel_particle.cc:2968-3049 (position, normal,
+ * tangent, binormal; u and v are in the hit). The reference runs it inside
+ * Traverse for every hit; here it is a separate pass that reads the caller's
+ * geometry (device pointers in the descriptor, as Traverse reads the Mesh it
+ * was built from), so rays that only need t and prim_id skip it. */
+srf_status srf_build_intersection_triangles(const srf_tri_desc *mesh, const srf_ray *rays,
+                                            const srf_hit *hits, uint64_t n, srf_isect *out,
+                                            void *stream);
+srf_status srf_build_intersection_tetras(const srf_tet_desc *tetras, const srf_ray *rays,
+                                         const srf_hit *hits, const srf_tet_hit_ext *ext,
+                                         uint64_t n, srf_isect *out, void *stream);
+srf_status srf_build_intersection_spheres(const srf_sphere_desc *particles,
+                                          const srf_ray *rays, const srf_hit *hits, uint64_t n,
+                                          srf_isect *out, void *stream);
+
+/* {Triangle,Tetra,Particle,Solid,Line}Accel::Traverse(Intersection&, Ray&)
+ * (render/render_accel_triangle.h:87) for rays in HOST memory, as
+ * RaytraceEngine calls it per pixel from its OpenMP loop
+ * (gles/gles_raytrace_engine.cc:662-880); n = 1 is the per-ray form. The
+ * rays go to the GPU in one batch and the results come back before the call
+ * returns. hits_host gets t, u, v, prim_id; ext_host (may be NULL) the
+ * kind's extras (srf_tet_hit_ext, srf_solid_hit_ext or srf_line_hit_ext,
+ * without the tetra scalar varying); isect_host (may be NULL; triangles,
+ * tetras, spheres) the BuildIntersection fields, computed from the geometry
+ * of the last build (a device build's arrays must still be alive, as the
+ * reference's Mesh must). Thread-safe: each host thread has its own stream
+ * and device staging buffer, kept until the process ends. */
+srf_status srf_trace_closest_host(const srf_accel *accel, const srf_ray *rays_host, uint64_t n,
+                                  srf_hit *hits_host, void *ext_host, srf_isect *isect_host);
+
 /* Occlusion: occluded[i] = 1 iff Traverse would report a hit with
  * t < t_max[i] (t_max NULL = FLT_MAX). The reference answers shadow() with
  * a full closest-hit Traverse (gles/gles_glsl_runtime.cc:179-217); this
@@ -475,6 +548,66 @@ srf_status srf_volume_create(srf_ctx *ctx, const float *voxels, int width,
                              int height, int depth, int wrap_s, int wrap_t,
                              int wrap_r, srf_volume **out);
 srf_status srf_volume_destroy(srf_volume *vol);
+
+/* One block of a sparse volume, VolumeBlock (render/render_prim_volume.h:
+ * 18-26): integer offset and extent in the global grid, cell count size[]
+ * (the block's voxels, x fastest, `components` interleaved per voxel), and
+ * its voxel data (DEVICE pointer, referenced). */
+typedef struct srf_volume_block {
+  int32_t offset[3];
+  int32_t extent[3];
+  int32_t size[3];
+  int32_t id;
+  const void *data;
+} srf_volume_block;
+
+/* SparseVolumeAccel::Build (render/render_accel_volume.cc:35-57): the
+ * BVHTree over the blocks' boxes (built on the host, as the reference does).
+ * global_dim = SparseVolume::globalDim (for srf_sparse_volume_fetch);
+ * data_type SRF_TEX_BYTE / _FLOAT / _DOUBLE for all blocks. */
+srf_status srf_sparse_volume_create(srf_ctx *ctx, const srf_volume_block *blocks_host,
+                                    uint32_t num_blocks, const int global_dim[3],
+                                    int components, int data_type, srf_sparse_volume **out);
+srf_status srf_sparse_volume_destroy(srf_sparse_volume *vol);
+/* SparseVolumeAccel::Sample (render/render_accel_volume.cc:59-139) at n
+ * positions (3 doubles each): values[4 i..] = the nearest voxel of the first
+ * block BVHTree::Locate finds (byte / 255, the rest 0, alpha 1 unless the
+ * volume has 4 components), or (0, 0, 0, 0) outside every block. As in the
+ * reference, the local offset is scaled by size[0] / extent[0] on all three
+ * axes. */
+srf_status srf_sparse_volume_sample(const srf_sparse_volume *vol, const double *positions,
+                                    uint64_t n, double *values, void *stream);
+/* Texture::Fetch's sparse branch (gles/gles_common.h:346-371): position =
+ * (u, v, r) * globalDim in float, Sample, RGBA as float. */
+srf_status srf_sparse_volume_fetch(const srf_sparse_volume *vol, const float *uvw, uint64_t n,
+                                   float *rgba, void *stream);
+
+/* glTexImage3D / lsglTexImage3DPointer for any dense 3D texture the
+ * reference's Texture::Fetch samples (gles/gles_common.h:344-403): data_type
+ * is LSGL_RENDER_TEXTURE3D_FORMAT_* (render/render_texture.h:16-21):
+ * SRF_TEX_BYTE (FilterTexture3DByte, voxel / 255.5f,
+ * render/render_texture.cc:832-944), SRF_TEX_FLOAT (FilterTexture3DFloat,
+ * :946-1058) or SRF_TEX_DOUBLE (FilterTexture3DDouble, :1060-1172);
+ * components = 1..4 interleaved per texel (x fastest); wrap_* as
+ * srf_volume_create. image is referenced, not copied. The object is an
+ * srf_volume: srf_volume_destroy frees it; the march and srf_volume_sample
+ * accept it when it is FP32 with one component. */
+#define SRF_TEX_BYTE 1
+#define SRF_TEX_FLOAT 8
+#define SRF_TEX_DOUBLE 16
+srf_status srf_texture3d_create(srf_ctx *ctx, const void *image, int data_type, int components,
+                                int width, int height, int depth, int wrap_s, int wrap_t,
+                                int wrap_r, srf_volume **out);
+/* The per-axis coordinate remap of Texture::Fetch (local::remap,
+ * gles/gles_common.h:263-269, applied at :377-389 before filtering): the
+ * coordinate x becomes table[clamp((int)(x * n), 0, n - 1)]. axis 0..2
+ * (s, t, r); table is a device array of n floats, referenced; NULL clears. */
+srf_status srf_texture3d_remap(srf_volume *tex, int axis, const float *table, int n);
+/* Texture::Fetch at n coordinates uvw (3 floats each): rgba[4 i..4 i + 3];
+ * a single component is splatted to all four, as the reference does. */
+srf_status srf_texture3d_fetch(const srf_volume *tex, const float *uvw, uint64_t n, float *rgba,
+                               void *stream);
+
 /* texture3D -> FilterTexture3DFloat (render/render_texture.cc:946-1058) at
  * n texture coordinates uvw (3 floats each); out[i] = sampled value. */
 srf_status srf_volume_sample(const srf_volume *vol, const float *uvw,

@@ -60,7 +60,7 @@ class Oracle:
                      "or_scene_bounds_sph", "or_transform_box", "or_scene_trace",
                      "or_scene_bounds_sol", "or_morton_sol", "or_trace_sol",
                      "or_interpolate_solid", "or_scene_bounds_lin", "or_morton_lin",
-                     "or_trace_lin"):
+                     "or_trace_lin", "or_isect_tri", "or_isect_tet", "or_isect_sph"):
             getattr(L, name).restype = None
         L.or_tlas_build.restype = _i64
         for name in ("or_build_tri", "or_build_tet", "or_build_sph", "or_build_sol",
@@ -289,6 +289,44 @@ class Oracle:
         self.lib.or_tex3d_sample(_ptr(vox), _ptr(d), _ptr(cl), _ptr(uvw), _i64(len(uvw)), _ptr(out))
         return out
 
+    def tex3d_fetch(self, image, comps, dims, clamp, uvw, remap=(None, None, None)):
+        """Texture::Fetch: image uint8 / float32 / float64 with comps
+        interleaved components; remap: per-axis float32 tables or None.
+        Returns rgba [n, 4] float32."""
+        fmt = {np.dtype(np.uint8): 1, np.dtype(np.float32): 8, np.dtype(np.float64): 16}[
+            np.asarray(image).dtype]
+        img = np.ascontiguousarray(image)
+        uvw = _c(uvw, np.float32).reshape(-1, 3)
+        out = np.zeros((len(uvw), 4), np.float32)
+        tabs = [None if t is None else _c(t, np.float32) for t in remap]
+        ptrs = (C.c_void_p * 3)(*[None if t is None else t.ctypes.data for t in tabs])
+        ns = np.array([0 if t is None else len(t) for t in tabs], np.int32)
+        self.lib.or_tex3d_fetch.restype = None
+        self.lib.or_tex3d_fetch(_ptr(img), C.c_int(fmt), C.c_int(comps), _ptr(_c(dims, np.int32)),
+                                _ptr(_c(clamp, np.int32)), ptrs, _ptr(ns), _ptr(uvw),
+                                _i64(len(uvw)), _ptr(out))
+        return out
+
+    def sparse_sample(self, blocks, data, components, pos=None, uvw=None, global_dim=(0, 0, 0)):
+        """SparseVolumeAccel::Sample over blocks (int32 [nb, 9]: offset,
+        extent, size) with voxel arrays data[b] (uint8 / float32 / float64);
+        uvw instead of pos samples Texture::Fetch's sparse branch."""
+        fmt = {np.dtype(np.uint8): 1, np.dtype(np.float32): 8, np.dtype(np.float64): 16}[
+            np.asarray(data[0]).dtype]
+        blocks = _c(blocks, np.int32).reshape(-1, 9)
+        arrs = [np.ascontiguousarray(d) for d in data]
+        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        n = len(pos) if pos is not None else len(uvw)
+        out = np.zeros((n, 4), np.float64)
+        pd = None if pos is None else _c(pos, np.float64).reshape(-1, 3)
+        ud = None if uvw is None else _c(uvw, np.float32).reshape(-1, 3)
+        self.lib.or_sparse_sample.restype = None
+        self.lib.or_sparse_sample(_ptr(blocks), ptrs, _i64(len(blocks)), C.c_int(components),
+                                  C.c_int(fmt), _ptr(_c(global_dim, np.int32)),
+                                  None if pd is None else _ptr(pd),
+                                  None if ud is None else _ptr(ud), _i64(n), _ptr(out))
+        return out
+
     @staticmethod
     def _vparams(dims, clamp, tf_size, step):
         class P(C.Structure):
@@ -325,6 +363,25 @@ class Oracle:
                                    _ptr(gv), _ptr(gt))
         return gv, gt
 
+
+    def isect(self, kind, V, F, rays, hits, ext=None):
+        """BuildIntersection fields (ISECT_DTYPE) of traced hits: kind "tri"
+        (V, F faces), "tet" (V, F, ext with the entry faces) or "sph" (V =
+        centres, F unused)."""
+        from surface_amd.workloads import ISECT_DTYPE
+        rays = np.ascontiguousarray(rays)
+        hits = np.ascontiguousarray(hits)
+        out = np.zeros(len(rays), ISECT_DTYPE)
+        Vc = _c(V, np.float32)
+        if kind == "tri":
+            self.lib.or_isect_tri(_ptr(Vc), _ptr(_c(F, np.uint32)), _ptr(rays), _ptr(hits),
+                                  _i64(len(rays)), _ptr(out))
+        elif kind == "tet":
+            self.lib.or_isect_tet(_ptr(Vc), _ptr(_c(F, np.uint32)), _ptr(rays), _ptr(hits),
+                                  _ptr(np.ascontiguousarray(ext)), _i64(len(rays)), _ptr(out))
+        else:
+            self.lib.or_isect_sph(_ptr(Vc), _ptr(rays), _ptr(hits), _i64(len(rays)), _ptr(out))
+        return out
 
     # ---- two-level scene ----
     def transform_box(self, bmin, bmax, T):
@@ -491,6 +548,16 @@ class Reference:
                            _ptr(hits), _ptr(ext), _ptr(sc), C.c_int(threads))
         return hits, ext
 
+    def trace_isect(self, tree, rays):
+        """The Intersection fields Traverse's BuildIntersection fills
+        (ISECT_DTYPE; triangles, tetras, spheres; serial)."""
+        from surface_amd.workloads import ISECT_DTYPE
+        out = np.zeros(len(rays), ISECT_DTYPE)
+        self.lib.ref_trace_isect.restype = None
+        self.lib.ref_trace_isect(_p(tree["handle"]), _ptr(np.ascontiguousarray(rays)),
+                                 _i64(len(rays)), _ptr(out))
+        return out
+
     def free(self, tree):
         if tree.get("handle"):
             self.lib.ref_free(_p(tree["handle"]))
@@ -544,6 +611,40 @@ class Reference:
         self.lib.ref_tex3d_sample(_ptr(vox), _ptr(_c(dims, np.int32)), _ptr(_c(clamp, np.int32)),
                                   _ptr(uvw), _i64(len(uvw)), _ptr(out))
         return out
+
+
+def _ref_tex3d_fetch(self, image, comps, dims, clamp, uvw):
+    """FilterTexture3D{Byte,Float,Double} of the reference (no remap)."""
+    fmt = {np.dtype(np.uint8): 1, np.dtype(np.float32): 8, np.dtype(np.float64): 16}[
+        np.asarray(image).dtype]
+    img = np.ascontiguousarray(image)
+    uvw = _c(uvw, np.float32).reshape(-1, 3)
+    out = np.zeros((len(uvw), 4), np.float32)
+    self.lib.ref_tex3d_fetch.restype = None
+    self.lib.ref_tex3d_fetch(_ptr(img), C.c_int(fmt), C.c_int(comps), _ptr(_c(dims, np.int32)),
+                             _ptr(_c(clamp, np.int32)), _ptr(uvw), _i64(len(uvw)), _ptr(out))
+    return out
+
+
+Reference.tex3d_fetch = _ref_tex3d_fetch
+
+
+def _ref_sparse_sample(self, blocks, data, components, pos):
+    """SparseVolumeAccel::Build + Sample of the reference."""
+    fmt = {np.dtype(np.uint8): 1, np.dtype(np.float32): 8, np.dtype(np.float64): 16}[
+        np.asarray(data[0]).dtype]
+    blocks = _c(blocks, np.int32).reshape(-1, 9)
+    arrs = [np.ascontiguousarray(d) for d in data]
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    pos = _c(pos, np.float64).reshape(-1, 3)
+    out = np.zeros((len(pos), 4), np.float64)
+    self.lib.ref_sparse_sample.restype = None
+    self.lib.ref_sparse_sample(_ptr(blocks), ptrs, _i64(len(blocks)), C.c_int(components),
+                               C.c_int(fmt), _ptr(pos), _i64(len(pos)), _ptr(out))
+    return out
+
+
+Reference.sparse_sample = _ref_sparse_sample
 
 
 def canonical_tri(nodes):

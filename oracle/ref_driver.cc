@@ -40,6 +40,7 @@
 #include "render_camera.h"
 #include "render_accel_tetra.h"
 #include "render_accel_triangle.h"
+#include "render_accel_volume.h"
 #include "render_prefix_tree_util.h"
 #include "render_texture.h"
 #include "render_bvh_tree.h"
@@ -395,6 +396,57 @@ void ref_trace(void *h, const void *rays_in, long long n, long long width,
   }
 }
 
+// The whole Intersection that {Triangle,Tetra,Particle}Accel::Traverse fills
+// through its BuildIntersection (render/render_accel_triangle.cc:2439-2507,
+// render/render_accel_tetra.cc:2124-2196,
+// render/render_accel_particle.cc:2968-3049): position, geometric, normal,
+// tangent, binormal, f0..f3, 80 bytes per ray, from a zeroed Intersection
+// (the fields a kind leaves unset stay 0; a miss is all 0).
+struct IsectOut {
+  float position[3];
+  unsigned int f0;
+  float geometric[3];
+  unsigned int f1;
+  float normal[3];
+  unsigned int f2;
+  float tangent[3];
+  unsigned int f3;
+  float binormal[3];
+  float pad;
+};
+
+void ref_trace_isect(void *h, const void *rays_in, long long n, void *out_v) {
+  RefAccel *a = (RefAccel *)h;
+  const RayIn *rays = (const RayIn *)rays_in;
+  IsectOut *out = (IsectOut *)out_v;
+  for (long long i = 0; i < n; i++) {
+    Ray ray = make_ray(rays[i]);
+    Intersection isect;
+    memset(&isect, 0, sizeof(isect));
+    bool hit = false;
+    if (a->kind == 0)
+      hit = a->tri_accel.Traverse(isect, ray);
+    else if (a->kind == 1)
+      hit = a->tet_accel.Traverse(isect, ray);
+    else if (a->kind == 2)
+      hit = a->sph_accel.Traverse(isect, ray);
+    IsectOut &o = out[i];
+    memset(&o, 0, sizeof(o));
+    if (!hit) continue;
+    for (int k = 0; k < 3; k++) {
+      o.position[k] = isect.position[k];
+      o.geometric[k] = isect.geometric[k];
+      o.normal[k] = isect.normal[k];
+      o.tangent[k] = isect.tangent[k];
+      o.binormal[k] = isect.binormal[k];
+    }
+    o.f0 = isect.f0;
+    o.f1 = isect.f1;
+    o.f2 = isect.f2;
+    o.f3 = isect.f3;
+  }
+}
+
 void ref_tex3d_sample(float *vox, const int dims[3], const int clamp[3],
                       const float *uvw, long long n, float *out) {
   Texture3D tex;
@@ -413,6 +465,56 @@ void ref_tex3d_sample(float *vox, const int dims[3], const int clamp[3],
   }
 }
 
+// FilterTexture3DByte / Float / Double (render/render_texture.cc:832-1172)
+// on a texture of `comps` interleaved components; data_type is the
+// reference's LSGL_RENDER_TEXTURE3D_FORMAT_* (1 byte, 8 float, 16 double).
+void ref_tex3d_fetch(void *image, int data_type, int comps, const int dims[3],
+                     const int clamp[3], const float *uvw, long long n, float *rgba) {
+  Texture3D tex;
+  memset(&tex, 0, sizeof(tex));
+  tex.image = (unsigned char *)image;
+  tex.width = dims[0];
+  tex.height = dims[1];
+  tex.depth = dims[2];
+  tex.components = comps;
+  tex.data_type = data_type;
+  for (long long i = 0; i < n; i++) {
+    float *o = rgba + 4 * i;
+    const bool cs = clamp[0] != 0, ct = clamp[1] != 0, cr = clamp[2] != 0;
+    if (data_type == LSGL_RENDER_TEXTURE3D_FORMAT_BYTE)
+      FilterTexture3DByte(o, &tex, uvw[3 * i], uvw[3 * i + 1], uvw[3 * i + 2], cs, ct, cr);
+    else if (data_type == LSGL_RENDER_TEXTURE3D_FORMAT_DOUBLE)
+      FilterTexture3DDouble(o, &tex, uvw[3 * i], uvw[3 * i + 1], uvw[3 * i + 2], cs, ct, cr);
+    else
+      FilterTexture3DFloat(o, &tex, uvw[3 * i], uvw[3 * i + 1], uvw[3 * i + 2], cs, ct, cr);
+  }
+}
+
+// SparseVolumeAccel::Build + Sample (render/render_accel_volume.cc:35-139)
+// over blocks given as 9 ints each (offset, extent, size) with their data.
+void ref_sparse_sample(const int *blocks, const void *const *data, long long nblocks,
+                       int components, int format, const double *pos, long long n,
+                       double *values) {
+  SparseVolume sv;
+  sv.components = components;
+  sv.format = format;
+  sv.maxLevel = 0;
+  for (long long b = 0; b < nblocks; b++) {
+    VolumeBlock vb;
+    memset(&vb, 0, sizeof(vb));
+    for (int k = 0; k < 3; k++) {
+      vb.offset[k] = blocks[9 * b + k];
+      vb.extent[k] = blocks[9 * b + 3 + k];
+      vb.size[k] = blocks[9 * b + 6 + k];
+    }
+    vb.id = (int)b;
+    vb.data = (const unsigned char *)data[b];
+    sv.blocks.push_back(vb);
+  }
+  SparseVolumeAccel accel;
+  accel.Build(&sv);
+  for (long long i = 0; i < n; i++) accel.Sample(values + 4 * i, pos + 3 * i);
+}
 
 // ---- two-level scenes (RenderGraph over triangle accelerators) ----
 struct RefScene {

@@ -2193,6 +2193,60 @@ void or_tex3d_sample(const float *vox, const int dims[3], const int clamp[3],
   }
 }
 
+/* Texture::Fetch (gles/gles_common.h:344-403) for a dense 3D texture: the
+ * optional per-axis remap (local::remap, :263-269: idx = (int)(x * n)
+ * clamped to [0, n-1], table[idx]), then FilterTexture3DByte (voxel / 255.5f,
+ * render/render_texture.cc:410-413,832-944), FilterTexture3DFloat
+ * (:946-1058) or FilterTexture3DDouble (double lerps from the float
+ * fractions, :1060-1172) with `comps` interleaved components; one component
+ * is splatted to RGBA, more fill rgba[0..comps) (the rest 0). format: 1 byte,
+ * 8 float, 16 double (LSGL_RENDER_TEXTURE3D_FORMAT_*). */
+static float remap_coord(float x, const float *table, int n) {
+  int idx = (int)(x * (float)n);
+  idx = idx < n - 1 ? idx : n - 1; /* std::min(n - 1, idx) */
+  idx = idx > 0 ? idx : 0;         /* std::max(., 0) */
+  return table[idx];
+}
+
+static double dlerp_(double t, double a, double b) { return (1.0 - t) * a + t * b; }
+
+void or_tex3d_fetch(const void *image, int format, int comps, const int dims[3],
+                    const int clamp[3], const float *const remap[3], const int remap_n[3],
+                    const float *uvw, int64_t n, float *rgba) {
+  for (int64_t i = 0; i < n; i++) {
+    float c3[3] = {uvw[3 * i], uvw[3 * i + 1], uvw[3 * i + 2]};
+    for (int k = 0; k < 3; k++)
+      if (remap && remap[k] && remap_n[k] > 0) c3[k] = remap_coord(c3[k], remap[k], remap_n[k]);
+    tri_geom g;
+    tex3d_geom(dims, clamp, c3[0], c3[1], c3[2], &g);
+    float vals[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c = 0; c < comps; c++) {
+      if (format == 16) {
+        const double *d = (const double *)image;
+        double D[8];
+        for (int q = 0; q < 8; q++) D[q] = d[g.a[q] * comps + c];
+        double dx = g.dx, dy = g.dy, dz = g.dz;
+        double d00 = dlerp_(dx, D[0], D[1]), d10 = dlerp_(dx, D[2], D[3]);
+        double d01 = dlerp_(dx, D[4], D[5]), d11 = dlerp_(dx, D[6], D[7]);
+        double d0 = dlerp_(dy, d00, d10), d1 = dlerp_(dy, d01, d11);
+        vals[c] = (float)dlerp_(dz, d0, d1);
+      } else {
+        float D[8];
+        for (int q = 0; q < 8; q++) {
+          const int64_t e = g.a[q] * comps + c;
+          D[q] = format == 1 ? ((const unsigned char *)image)[e] / 255.5f
+                             : ((const float *)image)[e];
+        }
+        float d00 = lerpf(g.dx, D[0], D[1]), d10 = lerpf(g.dx, D[2], D[3]);
+        float d01 = lerpf(g.dx, D[4], D[5]), d11 = lerpf(g.dx, D[6], D[7]);
+        float d0 = lerpf(g.dy, d00, d10), d1 = lerpf(g.dy, d01, d11);
+        vals[c] = lerpf(g.dz, d0, d1);
+      }
+    }
+    for (int k = 0; k < 4; k++) rgba[4 * i + k] = comps == 1 ? vals[0] : vals[k];
+  }
+}
+
 /* Volume march spec (builder-defined; the reference leaves marching to user
  * GLSL, e.g. examples/sparse_volume_render/input.frag:99-141):
  *  - the volume fills texture space [0,1]^3, rays are given in that space;
@@ -2779,4 +2833,217 @@ void or_scene_trace(const or_tlas_node *tlas, const or_tri_element *elems,
       scene_trace_one(tlas, elems, list, rays + i, hits + i, inst + i);
     free(list);
   }
+}
+
+
+/* ------------------------------------------------------ BuildIntersection */
+
+typedef struct {
+  float x, y, z;
+} v3f;
+
+static v3f v3f_at(const float *p, uint32_t i) {
+  v3f r = {p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]};
+  return r;
+}
+static v3f v3f_sub(v3f a, v3f b) {
+  v3f r = {a.x - b.x, a.y - b.y, a.z - b.z};
+  return r;
+}
+/* cross (render/render_common.h:161-166) */
+static v3f v3f_cross(v3f l, v3f r) {
+  v3f o = {l.y * r.z - l.z * r.y, l.z * r.x - l.x * r.z, l.x * r.y - l.y * r.x};
+  return o;
+}
+/* real3::normalize (render/render_common.h:69-82): float sum of squares
+ * widened to double, 1.0 / sqrt in double, each component scaled in double */
+static v3f v3f_normalize(v3f v) {
+  double length2 = (double)(v.x * v.x + v.y * v.y + v.z * v.z);
+  double length = 0.0;
+  if (fabs(length2) > 1.0e-30) length = 1.0 / sqrt(length2);
+  v3f r = {(float)(v.x * length), (float)(v.y * length), (float)(v.z * length)};
+  return r;
+}
+/* ray.origin() + isect.t * ray.direction() */
+static v3f hit_pos(const or_ray *r, float t) {
+  float tx = r->dir[0] * t, ty = r->dir[1] * t, tz = r->dir[2] * t;
+  v3f p = {r->org[0] + tx, r->org[1] + ty, r->org[2] + tz};
+  return p;
+}
+static void isect_put(or_isect *o, v3f pos, v3f geo, v3f nrm, v3f tan, v3f bin, uint32_t f0,
+                      uint32_t f1, uint32_t f2, uint32_t f3) {
+  memset(o, 0, sizeof(*o));
+  o->position[0] = pos.x, o->position[1] = pos.y, o->position[2] = pos.z;
+  o->geometric[0] = geo.x, o->geometric[1] = geo.y, o->geometric[2] = geo.z;
+  o->normal[0] = nrm.x, o->normal[1] = nrm.y, o->normal[2] = nrm.z;
+  o->tangent[0] = tan.x, o->tangent[1] = tan.y, o->tangent[2] = tan.z;
+  o->binormal[0] = bin.x, o->binormal[1] = bin.y, o->binormal[2] = bin.z;
+  o->f0 = f0, o->f1 = f1, o->f2 = f2, o->f3 = f3;
+}
+
+/* render/render_accel_triangle.cc:2439-2507 (float vertices) */
+void or_isect_tri(const float *V, const uint32_t *F, const or_ray *rays, const or_hit *hits,
+                  int64_t n, or_isect *out) {
+  const v3f z = {0, 0, 0};
+  for (int64_t i = 0; i < n; i++) {
+    if (hits[i].prim_id == 0xFFFFFFFFu) {
+      isect_put(&out[i], z, z, z, z, z, 0, 0, 0, 0);
+      continue;
+    }
+    const uint32_t *f = F + 3 * (size_t)hits[i].prim_id;
+    v3f p0 = v3f_at(V, f[0]), p1 = v3f_at(V, f[1]), p2 = v3f_at(V, f[2]);
+    v3f n3 = v3f_normalize(v3f_cross(v3f_sub(p1, p0), v3f_sub(p2, p0)));
+    isect_put(&out[i], hit_pos(&rays[i], hits[i].t), n3, n3, z, z, f[0], f[1], f[2], 0);
+  }
+}
+
+/* render/render_accel_tetra.cc:2124-2196; face_table at :2178 */
+void or_isect_tet(const float *V, const uint32_t *F, const or_ray *rays, const or_hit *hits,
+                  const or_tet_ext *ext, int64_t n, or_isect *out) {
+  static const int face_table[4][3] = {{0, 1, 3}, {1, 2, 3}, {2, 0, 3}, {0, 1, 2}};
+  const v3f z = {0, 0, 0};
+  for (int64_t i = 0; i < n; i++) {
+    if (hits[i].prim_id == 0xFFFFFFFFu) {
+      isect_put(&out[i], z, z, z, z, z, 0, 0, 0, 0);
+      continue;
+    }
+    const uint32_t *f = F + 4 * (size_t)hits[i].prim_id;
+    v3f p[4];
+    for (int k = 0; k < 4; k++) p[k] = v3f_at(V, f[k]);
+    const int *ft = face_table[ext[i].subface & 3u];
+    v3f n3 = v3f_normalize(v3f_cross(v3f_sub(p[ft[1]], p[ft[0]]), v3f_sub(p[ft[2]], p[ft[0]])));
+    isect_put(&out[i], hit_pos(&rays[i], hits[i].t), n3, n3, z, z, f[0], f[1], f[2], f[3]);
+  }
+}
+
+/* render/render_accel_particle.cc:2968-3049 with vnormalize (:2041-2056) */
+void or_isect_sph(const float *P, const or_ray *rays, const or_hit *hits, int64_t n,
+                  or_isect *out) {
+  const v3f z = {0, 0, 0};
+  for (int64_t i = 0; i < n; i++) {
+    const uint32_t id = hits[i].prim_id;
+    if (id == 0xFFFFFFFFu) {
+      isect_put(&out[i], z, z, z, z, z, 0, 0, 0, 0);
+      continue;
+    }
+    v3f pos = hit_pos(&rays[i], hits[i].t);
+    v3f cp = v3f_sub(pos, v3f_at(P, id));
+    const float kEPS = (float)(FLT_EPSILON * 16.0);
+    v3f nn = cp;
+    float len = sqrtf(cp.x * cp.x + cp.y * cp.y + cp.z * cp.z);
+    if (len > kEPS) {
+      float invlen = (float)(1.0 / len);
+      nn.x = cp.x * invlen, nn.y = cp.y * invlen, nn.z = cp.z * invlen;
+    }
+    v3f tan, bin;
+    float xz2 = nn.x * nn.x + nn.z * nn.z;
+    if (xz2 > 0.0) {
+      float xz = sqrtf(xz2);
+      float inv = (float)(1.0 / xz);
+      tan.x = (float)(-2.0 * M_PI * nn.x);
+      tan.y = (float)(2.0 * M_PI * nn.z);
+      tan.z = 0.0f;
+      bin.x = (float)(-M_PI * nn.z * inv * nn.y);
+      bin.y = (float)(-M_PI * nn.x * inv * nn.y);
+      bin.z = (float)(M_PI * xz);
+    } else {
+      tan.x = 0.0f, tan.y = 0.0f, tan.z = 1.0f;
+      bin.x = nn.y > 0.0 ? 1.0f : -1.0f, bin.y = 0.0f, bin.z = 0.0f;
+    }
+    isect_put(&out[i], pos, nn, nn, tan, bin, id, id, id, 0);
+  }
+}
+
+
+/* -------------------------------------------------- SparseVolumeAccel */
+
+/* BVHTree::Locate's first hit (render/render_bvh_tree.cc:588-654 with
+ * TestPointAABB / TestPointBVHNode, :173-209): -1 when no leaf box holds p. */
+static int sparse_locate(const or_tlas_node *nodes, int64_t nn, const double p[3]) {
+  int64_t stack[256];
+  int sp = 0;
+  int64_t node = 0;
+  if (nn <= 0) return -1;
+#define INSIDE(nd)                                                                        \
+  (p[0] >= (nd)->bmin[0] && p[0] <= (nd)->bmax[0] && p[1] >= (nd)->bmin[1] &&             \
+   p[1] <= (nd)->bmax[1] && p[2] >= (nd)->bmin[2] && p[2] <= (nd)->bmax[2])
+  for (;;) {
+    const or_tlas_node *nd = &nodes[node];
+    if (nd->is_leaf) {
+      if (INSIDE(nd)) return (int)nd->node_id;
+      if (sp < 1) return -1;
+      node = stack[--sp];
+      continue;
+    }
+    int l = nd->child[0] > 0 && INSIDE(&nodes[nd->child[0]]);
+    int r = nd->child[1] > 0 && INSIDE(&nodes[nd->child[1]]);
+    if (l && r) {
+      stack[sp++] = nd->child[1];
+      node = nd->child[0];
+    } else if (l) {
+      node = nd->child[0];
+    } else if (r) {
+      node = nd->child[1];
+    } else {
+      if (sp < 1) return -1;
+      node = stack[--sp];
+    }
+  }
+#undef INSIDE
+}
+
+/* SparseVolumeAccel::Build + Sample (render/render_accel_volume.cc:35-139):
+ * blocks (offset[3], extent[3], size[3] per block, 9 ints) with voxel data
+ * data[b]; uvw != NULL takes Texture::Fetch's position u * globalDim in
+ * float (gles/gles_common.h:353-363) instead of pos. values[4 n]. */
+void or_sparse_sample(const int32_t *blocks, const void *const *data, int64_t nblocks,
+                      int components, int format, const int global_dim[3], const double *pos,
+                      const float *uvw, int64_t n, double *values) {
+  double *boxes = (double *)malloc(sizeof(double) * 6 * (size_t)nblocks);
+  for (int64_t b = 0; b < nblocks; b++)
+    for (int k = 0; k < 3; k++) {
+      boxes[6 * b + k] = (double)blocks[9 * b + k];
+      boxes[6 * b + 3 + k] = (double)(blocks[9 * b + k] + blocks[9 * b + 3 + k]);
+    }
+  or_tlas_node *nodes = (or_tlas_node *)malloc(sizeof(or_tlas_node) * 2 * (size_t)nblocks);
+  const int64_t nn = or_tlas_build(boxes, nblocks, nodes);
+  const int comps = components >= 4 ? 4 : components;
+  for (int64_t i = 0; i < n; i++) {
+    double p[3];
+    for (int k = 0; k < 3; k++)
+      p[k] = uvw ? (double)(uvw[3 * i + k] * (float)global_dim[k]) : pos[3 * i + k];
+    double *v = values + 4 * i;
+    v[0] = v[1] = v[2] = 0.0;
+    v[3] = 1.0;
+    const int b = sparse_locate(nodes, nn, p);
+    if (b < 0) {
+      v[3] = 0.0;
+      continue;
+    }
+    const int32_t *off = blocks + 9 * b, *ext = off + 3, *size = off + 6;
+    double x = p[0] - off[0], y = p[1] - off[1], z = p[2] - off[2];
+    double scaleX = size[0] / (double)ext[0];
+    x = x * scaleX;
+    y = y * scaleX; /* the reference scales every axis by scaleX */
+    z = z * scaleX;
+    int ix = (int)x, iy = (int)y, iz = (int)z;
+    ix = ix < size[0] - 1 ? ix : size[0] - 1;
+    ix = ix > 0 ? ix : 0;
+    iy = iy < size[1] - 1 ? iy : size[1] - 1;
+    iy = iy > 0 ? iy : 0;
+    iz = iz < size[2] - 1 ? iz : size[2] - 1;
+    iz = iz > 0 ? iz : 0;
+    size_t idx = (size_t)(iz * size[0] * size[1] + iy * size[0] + ix);
+    for (int c = 0; c < comps; c++) {
+      size_t e = (size_t)comps * idx + c;
+      if (format == 1)
+        v[c] = ((const unsigned char *)data[b])[e] / 255.0f;
+      else if (format == 8)
+        v[c] = ((const float *)data[b])[e];
+      else
+        v[c] = (float)((const double *)data[b])[e];
+    }
+  }
+  free(nodes);
+  free(boxes);
 }

@@ -243,6 +243,46 @@ void or_scene_trace(const or_tlas_node *tlas, const or_tri_element *elems,
                     const or_ray *rays, int64_t nrays, or_hit *hits, uint32_t *inst,
                     int threads);
 
+/* Texture::Fetch of a dense 3D texture (byte / float / double, 1-4
+ * interleaved components, optional per-axis remap tables): rgba[4 n]. */
+void or_tex3d_fetch(const void *image, int format, int comps, const int dims[3],
+                    const int clamp[3], const float *const remap[3], const int remap_n[3],
+                    const float *uvw, int64_t n, float *rgba);
+
+/* SparseVolumeAccel::Build + Sample / Texture::Fetch's sparse branch:
+ * blocks = 9 ints per block (offset, extent, size), data[b] its voxels;
+ * uvw (may be NULL) replaces pos by u * globalDim. values[4 n]. */
+void or_sparse_sample(const int32_t *blocks, const void *const *data, int64_t nblocks,
+                      int components, int format, const int global_dim[3], const double *pos,
+                      const float *uvw, int64_t n, double *values);
+
+/* BuildIntersection fields of a closest hit (80 bytes): position, geometric
+ * and shading normal, tangent, binormal, vertex ids f0..f3; unset fields and
+ * misses are 0. */
+typedef struct {
+  float position[3];
+  uint32_t f0;
+  float geometric[3];
+  uint32_t f1;
+  float normal[3];
+  uint32_t f2;
+  float tangent[3];
+  uint32_t f3;
+  float binormal[3];
+  float pad;
+} or_isect;
+
+/* BuildIntersection for hits already traced: triangles
+ * render/render_accel_triangle.cc:2439-2507, tetras (entry face from
+ * ext->subface) render/render_accel_tetra.cc:2124-2196, spheres
+ * render/render_accel_particle.cc:2968-3049. */
+void or_isect_tri(const float *V, const uint32_t *F, const or_ray *rays, const or_hit *hits,
+                  int64_t n, or_isect *out);
+void or_isect_tet(const float *V, const uint32_t *F, const or_ray *rays, const or_hit *hits,
+                  const or_tet_ext *ext, int64_t n, or_isect *out);
+void or_isect_sph(const float *P, const or_ray *rays, const or_hit *hits, int64_t n,
+                  or_isect *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -118,6 +118,12 @@ SIGNATURES = {
     "srf_trace_spheres": (_i, [_vp, _vp, _vp, _u64, _vp, _vp]),
     "srf_trace_any": (_i, [_vp, _vp, _vp, _u64, _vp, _vp]),
     "srf_debug_force_wide_stack": (_i, [_vp, _i]),
+    "srf_build_intersection_triangles": (_i, [C.POINTER(TriDesc), _vp, _vp, _u64, _vp, _vp]),
+    "srf_build_intersection_tetras": (_i, [C.POINTER(TetDesc), _vp, _vp, _vp, _u64, _vp, _vp]),
+    "srf_build_intersection_spheres": (_i, [C.POINTER(SphereDesc), _vp, _vp, _u64, _vp, _vp]),
+    "srf_accel_build_host": (_i, [_vp, _i, _vp, _vp, C.POINTER(_vp)]),
+    "srf_accel_rebuild_host": (_i, [_vp, _vp, _vp, _vp]),
+    "srf_trace_closest_host": (_i, [_vp, _vp, _u64, _vp, _vp, _vp]),
     "srf_lbvh_morton30": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
     "srf_lbvh_sort30": (_i, [_vp, _vp, _u64, _vp, _vp, _vp]),
     "srf_lbvh_radix_tree30": (_i, [_vp, _vp, _u64, _vp, _vp]),
@@ -133,6 +139,13 @@ SIGNATURES = {
                                  C.c_float, C.c_uint64, _vp, _vp, _vp, _vp, _vp]),
     "srf_volume_create": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, C.POINTER(_vp)]),
     "srf_volume_destroy": (_i, [_vp]),
+    "srf_texture3d_create": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, C.POINTER(_vp)]),
+    "srf_texture3d_remap": (_i, [_vp, _i, _vp, _i]),
+    "srf_texture3d_fetch": (_i, [_vp, _vp, _u64, _vp, _vp]),
+    "srf_sparse_volume_create": (_i, [_vp, _vp, C.c_uint32, _vp, _i, _i, C.POINTER(_vp)]),
+    "srf_sparse_volume_destroy": (_i, [_vp]),
+    "srf_sparse_volume_sample": (_i, [_vp, _vp, _u64, _vp, _vp]),
+    "srf_sparse_volume_fetch": (_i, [_vp, _vp, _u64, _vp, _vp]),
     "srf_volume_sample": (_i, [_vp, _vp, _u64, _vp, _vp]),
     "srf_volume_march_fwd": (_i, [_vp, _vp, _i, C.c_float, _vp, _u64, _vp, _vp, _vp]),
     "srf_volume_checkpoint_slots": (_i, [_vp, C.c_float, C.POINTER(C.c_uint32)]),

@@ -180,6 +180,36 @@ class _Accel:
                                      _stream(stream)))
         return out
 
+    def _build_host(self, kind, desc, keep):
+        """srf_accel_build_host / srf_accel_rebuild_host: desc holds host
+        (numpy) arrays, kept alive in `keep` for the call; synchronises."""
+        if self.handle:
+            check(self.lib.srf_accel_rebuild_host(self.ctx.handle, self.handle, C.byref(desc),
+                                                  _stream(None)))
+        else:
+            h = C.c_void_p()
+            check(self.lib.srf_accel_build_host(self.ctx.handle, kind, C.byref(desc),
+                                                _stream(None), C.byref(h)))
+            self.handle = h
+        del keep
+        return True
+
+    def TraverseHost(self, rays, ext_dtype=None, isect=True):
+        """Closest hits of host (numpy) rays through srf_trace_closest_host:
+        returns (hits, ext or None, isect fields or None) as numpy arrays."""
+        from .workloads import ISECT_DTYPE
+        self._require()
+        rays = np.ascontiguousarray(rays)
+        n = len(rays)
+        hits = np.zeros(n, HIT_DTYPE)
+        ext = None if ext_dtype is None else np.zeros(n, ext_dtype)
+        out = np.zeros(n, ISECT_DTYPE) if isect else None
+        check(self.lib.srf_trace_closest_host(
+            self.handle, rays.ctypes.data_as(C.c_void_p), n, hits.ctypes.data_as(C.c_void_p),
+            None if ext is None else ext.ctypes.data_as(C.c_void_p),
+            None if out is None else out.ctypes.data_as(C.c_void_p)))
+        return hits, ext, out
+
     def force_wide_stack(self, on=True):
         """Testing switch: trace with the kernels meant for trees of 2^24 or
         more nodes (32-bit stack entries); results are identical."""
@@ -241,6 +271,48 @@ def ao_rays_spheres(primary, hits, positions, radius=None, constant_radius=0.0, 
     return rays[:m], tmax[:m], pixel[:m]
 
 
+def _isect_out(n, device):
+    return torch.empty((n, 20), dtype=torch.float32, device=device)
+
+
+def build_intersection_triangles(vertices, faces, rays, hits, out=None, stream=None):
+    """BuildIntersection of triangle hits (srf_build_intersection_triangles):
+    [n, 20] float32 rows in the srf_isect layout (position, f0, geometric,
+    f1, normal, f2, tangent, f3, binormal, pad)."""
+    lib = load()
+    n = rays.shape[0]
+    out = _isect_out(n, rays.device) if out is None else out
+    d = TriDesc(vertices.data_ptr(), vertices.numel() // 3, faces.data_ptr(), faces.numel() // 3)
+    check(lib.srf_build_intersection_triangles(C.byref(d), _ptr(rays), _ptr(hits), n, _ptr(out),
+                                               _stream(stream)))
+    return out
+
+
+def build_intersection_tetras(vertices, indices, rays, hits, ext, out=None, stream=None):
+    """BuildIntersection of tetra hits (entry faces from the srf_tet_hit_ext
+    rows of TraverseTetra)."""
+    lib = load()
+    n = rays.shape[0]
+    out = _isect_out(n, rays.device) if out is None else out
+    d = TetDesc(vertices.data_ptr(), vertices.numel() // 3, indices.data_ptr(),
+                indices.numel() // 4)
+    check(lib.srf_build_intersection_tetras(C.byref(d), _ptr(rays), _ptr(hits), _ptr(ext), n,
+                                            _ptr(out), _stream(stream)))
+    return out
+
+
+def build_intersection_spheres(positions, rays, hits, out=None, stream=None):
+    """BuildIntersection of sphere hits (position, normal, tangent,
+    binormal; u and v are in the hits)."""
+    lib = load()
+    n = rays.shape[0]
+    out = _isect_out(n, rays.device) if out is None else out
+    d = SphereDesc(positions.data_ptr(), None, 0.0, positions.numel() // 3)
+    check(lib.srf_build_intersection_spheres(C.byref(d), _ptr(rays), _ptr(hits), n, _ptr(out),
+                                             _stream(stream)))
+    return out
+
+
 class TriangleAccel(_Accel):
     """TriangleAccel::Build32 / Traverse / BoundingBox on the GPU."""
     kind = _lib.PRIM_TRIANGLES
@@ -250,6 +322,13 @@ class TriangleAccel(_Accel):
         d = TriDesc(vertices.data_ptr(), vertices.numel() // 3, faces.data_ptr(),
                     faces.numel() // 3)
         return self._build(self.lib.srf_accel_build_triangles, d, stream)
+
+    def Build32Host(self, vertices, faces):
+        """Build32 from host (numpy) arrays; synchronises."""
+        V = np.ascontiguousarray(vertices, np.float32)
+        F = np.ascontiguousarray(faces, np.uint32)
+        d = TriDesc(V.ctypes.data, V.size // 3, F.ctypes.data, F.size // 3)
+        return self._build_host(_lib.PRIM_TRIANGLES, d, (V, F))
 
 
 def build_triangles_batch(meshes, accels=None, ctx=None, stream=None):
@@ -287,6 +366,13 @@ class TetraAccel(_Accel):
         d = TetDesc(vertices.data_ptr(), vertices.numel() // 3, indices.data_ptr(),
                     indices.numel() // 4)
         return self._build(self.lib.srf_accel_build_tetras, d, stream)
+
+    def Build32Host(self, vertices, indices):
+        """Build32 from host (numpy) arrays; synchronises."""
+        V = np.ascontiguousarray(vertices, np.float32)
+        F = np.ascontiguousarray(indices, np.uint32)
+        d = TetDesc(V.ctypes.data, V.size // 3, F.ctypes.data, F.size // 4)
+        return self._build_host(_lib.PRIM_TETRAHEDRONS, d, (V, F))
 
     def TraverseTetra(self, rays, scalars=None, hits=None, ext=None, stream=None):
         """Closest hit + InterpolateTetra weights and the interpolated scalar;
@@ -366,6 +452,14 @@ class ParticleAccel(_Accel):
         d = SphereDesc(positions.data_ptr(), radius.data_ptr() if radius is not None else None,
                        float(constant_radius), positions.numel() // 3)
         return self._build(self.lib.srf_accel_build_spheres, d, stream)
+
+    def Build32Host(self, positions, radius=None, constant_radius=0.0):
+        """Build32 from host (numpy) arrays; synchronises."""
+        P = np.ascontiguousarray(positions, np.float32)
+        R = None if radius is None else np.ascontiguousarray(radius, np.float32)
+        d = SphereDesc(P.ctypes.data, None if R is None else R.ctypes.data,
+                       float(constant_radius), P.size // 3)
+        return self._build_host(_lib.PRIM_POINTS, d, (P, R))
 
     def TraverseSelf(self, rays, prev, hits=None, stream=None):
         """Traverse with per-ray prev_hit_t / prev_hit_normal ([n, 6] float32

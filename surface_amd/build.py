@@ -16,7 +16,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsrf.so")
-SOURCES = ["srf_api.hip", "srf_build.hip", "srf_trace.hip", "srf_scene.hip", "srf_volume.hip"]
+SOURCES = ["srf_api.hip", "srf_build.hip", "srf_trace.hip", "srf_scene.hip", "srf_volume.hip",
+           "srf_isect.hip", "srf_host.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -62,7 +63,8 @@ def build_cpp_test(verbose=False):
     root = os.path.dirname(HERE)
     oracle = os.path.join(root, "oracle")
     outs = []
-    for name, extra in (("test_accel_cpp", []), ("test_frame_cpp", ["-fopenmp", "-pthread"])):
+    for name, extra in (("test_accel_cpp", ["-pthread"]),
+                        ("test_frame_cpp", ["-fopenmp", "-pthread"])):
         src = os.path.join(root, "tests", "cpp", name + ".cpp")
         out = os.path.join(root, "tests", "cpp", name)
         cmd = [hipcc(), "--offload-arch=gfx950", "-O2", "-std=c++17", *extra, "-o", out, src,

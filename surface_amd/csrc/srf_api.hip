@@ -95,6 +95,7 @@ void free_accel(Accel *a) {
   DeviceGuard g(a->device);
   if (a->built) (void)hipEventSynchronize(a->built);
   free_buffers(a);
+  free_geometry(a);
   (void)hipFree(a->params);
   if (a->built) (void)hipEventDestroy(a->built);
   delete a;
@@ -180,6 +181,7 @@ srf_status build_common(srf_ctx *c, Kind kind, const void *desc, uint64_t n, uin
   if (st != SRF_OK) return st;
   st = build_lbvh(ctx, kind, desc, s, a);
   if (st == SRF_OK) {
+    record_geometry(a, desc);
     hipError_t e = hipEventRecord(a->built, s);
     if (e != hipSuccess) st = hip_error(e, "hipEventRecord");
   }
@@ -214,6 +216,13 @@ void scene_free(Scene *sc);
 srf_status scene_query(const Scene *sc, double bmin[3], double bmax[3], srf_tlas_node *nodes,
                        uint32_t *count);
 int scene_device(const Scene *sc);
+struct SparseVolume;
+srf_status sparse_create(Ctx *ctx, const srf_volume_block *blocks, uint32_t count,
+                         const int global_dim[3], int components, int format,
+                         SparseVolume **out);
+void sparse_free(SparseVolume *sv);
+srf_status sparse_sample(const SparseVolume *sv, const double *pos, const float *uvw, uint64_t n,
+                         double *out_d, float *out_f, hipStream_t s);
 
 }  // namespace srf
 
@@ -324,6 +333,7 @@ srf_status srf_accel_build_triangles_batch(srf_ctx *ctx, uint32_t num_meshes,
   }
   if (st == SRF_OK) st = build_lbvh_segments(c, kTri, dp.data(), accs.data(), num_meshes, s);
   for (uint32_t m = 0; m < num_meshes && st == SRF_OK; m++) {
+    record_geometry(accs[m], &descs[m]);
     hipError_t e = hipEventRecord(accs[m]->built, s);
     if (e != hipSuccess) st = hip_error(e, "hipEventRecord");
   }
@@ -373,7 +383,10 @@ srf_status srf_accel_rebuild_triangles_batch(srf_ctx *ctx, uint32_t num_meshes,
   }
   srf_status st = build_lbvh_segments(c, kTri, dp.data(), accs.data(), num_meshes, s);
   if (st != SRF_OK) return st;
-  for (uint32_t m = 0; m < num_meshes; m++) SRF_HIP(hipEventRecord(accs[m]->built, s));
+  for (uint32_t m = 0; m < num_meshes; m++) {
+    record_geometry(accs[m], &descs[m]);
+    SRF_HIP(hipEventRecord(accs[m]->built, s));
+  }
   return SRF_OK;
 }
 
@@ -426,6 +439,7 @@ srf_status srf_accel_rebuild(srf_ctx *ctx, srf_accel *accel, const void *desc, v
   a->nv = nv;
   st = build_lbvh(c, a->kind, desc, s, a);
   if (st != SRF_OK) return st;
+  record_geometry(a, desc);
   SRF_HIP(hipEventRecord(a->built, s));
   return SRF_OK;
 }
@@ -813,6 +827,47 @@ srf_status srf_ao_rays_spheres(const srf_ray *primary, const srf_hit *hits, uint
   return ao_rays_spheres(primary, hits, n, positions, radius, rconst, per_hit, eps, dist, seed,
                          out, out_tmax, out_pixel, out_count,
                          reinterpret_cast<hipStream_t>(stream));
+}
+
+srf_status srf_sparse_volume_create(srf_ctx *ctx, const srf_volume_block *blocks_host,
+                                    uint32_t num_blocks, const int global_dim[3],
+                                    int components, int data_type, srf_sparse_volume **out) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !out || (num_blocks && !blocks_host))
+    return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (num_blocks == 0) return set_error(SRF_ERR_INVALID_ARG, "no volume blocks");
+  if (components < 1) return set_error(SRF_ERR_INVALID_ARG, "components must be >= 1");
+  if (data_type != SRF_TEX_BYTE && data_type != SRF_TEX_FLOAT && data_type != SRF_TEX_DOUBLE)
+    return set_error(SRF_ERR_UNSUPPORTED, "voxel data type %d (byte 1, float 8, double 16)",
+                     data_type);
+  DeviceGuard g(c->device);
+  SparseVolume *sv = nullptr;
+  srf_status st = sparse_create(c, blocks_host, num_blocks, global_dim, components, data_type, &sv);
+  if (st != SRF_OK) return st;
+  *out = reinterpret_cast<srf_sparse_volume *>(sv);
+  return SRF_OK;
+}
+
+srf_status srf_sparse_volume_destroy(srf_sparse_volume *vol) {
+  sparse_free(reinterpret_cast<SparseVolume *>(vol));
+  return SRF_OK;
+}
+
+srf_status srf_sparse_volume_sample(const srf_sparse_volume *vol, const double *positions,
+                                    uint64_t n, double *values, void *stream) {
+  const SparseVolume *sv = reinterpret_cast<const SparseVolume *>(vol);
+  if (!sv || (n && (!positions || !values))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many samples");
+  return sparse_sample(sv, positions, nullptr, n, values, nullptr,
+                       reinterpret_cast<hipStream_t>(stream));
+}
+
+srf_status srf_sparse_volume_fetch(const srf_sparse_volume *vol, const float *uvw, uint64_t n,
+                                   float *rgba, void *stream) {
+  const SparseVolume *sv = reinterpret_cast<const SparseVolume *>(vol);
+  if (!sv || (n && (!uvw || !rgba))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many samples");
+  return sparse_sample(sv, nullptr, uvw, n, nullptr, rgba, reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -124,7 +124,22 @@ struct Accel {
   BuildParams *sub_params;
   uint64_t sub_cap_prims, sub_cap_nodes, sub_cap_trees;
   uint64_t sub_nodes_used, sub_prims_used;
+  // The geometry of the last build in device memory, which BuildIntersection
+  // reads in srf_trace_closest_host (the reference keeps the caller's Mesh
+  // pointer likewise): the caller's arrays after a device build, copies the
+  // accelerator owns (geom_own, geom_bytes) after a host build.
+  const void *geom[3];  // vertices / positions, faces / indices, radius
+  void *geom_own[3];
+  size_t geom_bytes[3];
 };
+
+// Records the device arrays of a build's descriptor in a->geom.
+void record_geometry(Accel *a, const void *desc);
+void free_geometry(Accel *a);
+// BuildIntersection launchers (srf_isect.hip) for an accelerator's recorded
+// geometry.
+srf_status launch_isect(const Accel *a, const srf_ray *rays, const srf_hit *hits,
+                        const srf_tet_hit_ext *ext, uint32_t n, srf_isect *out, hipStream_t s);
 
 // Device view of the secondary sphere trees handed to the traversal.
 struct SphSub {

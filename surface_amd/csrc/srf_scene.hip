@@ -652,3 +652,195 @@ srf_status scene_autotune(Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *h
 }
 
 }  // namespace srf
+
+// ----------------------------------------------------------- sparse volume
+// SparseVolumeAccel (render/render_accel_volume.cc:35-139): a BVHTree over
+// the blocks' integer boxes (the same top-level tree as above, built on the
+// host) and, per position, BVHTree::Locate (render/render_bvh_tree.cc:
+// 588-654: the first leaf in its visit order whose own box contains the
+// point) and the nearest voxel of that block. As in the reference, the local
+// offset is scaled by size[0] / extent[0] on every axis.
+namespace srf {
+
+struct SparseBlock {  // device copy of one srf_volume_block
+  int32_t offset[3], extent[3], size[3], id;
+  const void *data;
+};
+
+struct SparseVolume {
+  int device;
+  uint32_t count;
+  uint32_t num_nodes;
+  int components;  // min(components, 4)
+  int format;      // 1 byte, 8 float, 16 double
+  int global_dim[3];
+  TlasNode *d_nodes;
+  SparseBlock *d_blocks;
+};
+
+namespace {
+
+constexpr int kLocateStack = 64;
+
+// BVHTree::Locate's first hit (TestPointAABB / TestPointBVHNode, :173-209)
+__device__ __forceinline__ int sparse_locate(const TlasNode *__restrict__ tl, uint32_t nnodes,
+                                             const double p[3]) {
+  uint32_t stack[kLocateStack];
+  int sp = 0;
+  uint32_t node = 0;
+  auto inside = [&](const TlasNode &nd) {
+    return p[0] >= nd.bmin[0] && p[0] <= nd.bmax[0] && p[1] >= nd.bmin[1] &&
+           p[1] <= nd.bmax[1] && p[2] >= nd.bmin[2] && p[2] <= nd.bmax[2];
+  };
+  for (;;) {
+    if (node >= nnodes) return -1;
+    const TlasNode nd = load_tlas(tl, node);
+    if (nd.axis_leaf >> 8) {
+      if (inside(nd)) return (int)nd.id;
+      if (sp < 1) return -1;
+      node = stack[--sp];
+      continue;
+    }
+    const bool l = nd.child[0] > 0 && inside(load_tlas(tl, (uint32_t)nd.child[0]));
+    const bool r = nd.child[1] > 0 && inside(load_tlas(tl, (uint32_t)nd.child[1]));
+    if (l && r) {
+      if (sp < kLocateStack) stack[sp++] = (uint32_t)nd.child[1];
+      node = (uint32_t)nd.child[0];
+    } else if (l) {
+      node = (uint32_t)nd.child[0];
+    } else if (r) {
+      node = (uint32_t)nd.child[1];
+    } else {
+      if (sp < 1) return -1;
+      node = stack[--sp];
+    }
+  }
+}
+
+// SparseVolumeAccel::Sample (render/render_accel_volume.cc:59-139). With uvw
+// the position is Texture::Fetch's u * globalDim (float) per axis
+// (gles/gles_common.h:353-363) and the result goes out as float RGBA.
+__global__ void __launch_bounds__(256)
+k_sparse_sample(const TlasNode *__restrict__ tl, uint32_t nnodes,
+                const SparseBlock *__restrict__ blocks, int comps, int format,
+                const double *__restrict__ pos, const float *__restrict__ uvw, int gx, int gy,
+                int gz, uint32_t n, double4 *__restrict__ out_d, float4 *__restrict__ out_f) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double p[3];
+  if (uvw) {
+    p[0] = (double)(uvw[3 * (size_t)i] * (float)gx);
+    p[1] = (double)(uvw[3 * (size_t)i + 1] * (float)gy);
+    p[2] = (double)(uvw[3 * (size_t)i + 2] * (float)gz);
+  } else {
+    p[0] = pos[3 * (size_t)i];
+    p[1] = pos[3 * (size_t)i + 1];
+    p[2] = pos[3 * (size_t)i + 2];
+  }
+  double v[4] = {0.0, 0.0, 0.0, 1.0};
+  const int b = sparse_locate(tl, nnodes, p);
+  if (b < 0) {
+    v[3] = 0.0;
+  } else {
+    const SparseBlock &bk = blocks[b];
+    double x = p[0] - (double)bk.offset[0];
+    double y = p[1] - (double)bk.offset[1];
+    double z = p[2] - (double)bk.offset[2];
+    const double sx = (double)bk.size[0] / (double)bk.extent[0];
+    x = x * sx;
+    y = y * sx;
+    z = z * sx;
+    const int ix = max(min(bk.size[0] - 1, (int)x), 0);
+    const int iy = max(min(bk.size[1] - 1, (int)y), 0);
+    const int iz = max(min(bk.size[2] - 1, (int)z), 0);
+    const size_t idx = (size_t)(iz * bk.size[0] * bk.size[1] + iy * bk.size[0] + ix);
+    for (int c = 0; c < comps; c++) {
+      const size_t e = (size_t)comps * idx + c;
+      if (format == SRF_TEX_BYTE)
+        v[c] = (double)((float)((const unsigned char *)bk.data)[e] / 255.0f);
+      else if (format == SRF_TEX_FLOAT)
+        v[c] = (double)((const float *)bk.data)[e];
+      else
+        v[c] = (double)(float)((const double *)bk.data)[e];
+    }
+  }
+  if (out_d) out_d[i] = make_double4(v[0], v[1], v[2], v[3]);
+  if (out_f) out_f[i] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+}
+
+}  // namespace
+
+srf_status sparse_create(Ctx *ctx, const srf_volume_block *blocks, uint32_t count,
+                         const int global_dim[3], int components, int format,
+                         SparseVolume **out) {
+  std::vector<Item> items(count);
+  for (uint32_t i = 0; i < count; i++) {
+    const srf_volume_block &b = blocks[i];
+    for (int k = 0; k < 3; k++) {
+      if (b.extent[k] <= 0 || b.size[k] <= 0)
+        return set_error(SRF_ERR_INVALID_ARG, "block %u: extent and size must be > 0", i);
+      items[i].bmin[k] = (double)b.offset[k];
+      items[i].bmax[k] = (double)(b.offset[k] + b.extent[k]);
+    }
+    if (!b.data) return set_error(SRF_ERR_INVALID_ARG, "block %u: null data", i);
+    items[i].id = i;
+  }
+  std::vector<TlasNode> nodes = TlasBuilder(std::move(items)).build();
+  if (branch_depth(nodes, 0) >= (uint32_t)kLocateStack)
+    return set_error(SRF_ERR_UNSUPPORTED, "block tree deeper than %d levels", kLocateStack);
+  std::vector<SparseBlock> dev(count);
+  for (uint32_t i = 0; i < count; i++) {
+    for (int k = 0; k < 3; k++) {
+      dev[i].offset[k] = blocks[i].offset[k];
+      dev[i].extent[k] = blocks[i].extent[k];
+      dev[i].size[k] = blocks[i].size[k];
+    }
+    dev[i].id = blocks[i].id;
+    dev[i].data = blocks[i].data;
+  }
+  SparseVolume *sv = new (std::nothrow) SparseVolume();
+  if (!sv) return set_error(SRF_ERR_OUT_OF_MEMORY, "host allocation failed");
+  sv->device = ctx->device;
+  sv->count = count;
+  sv->num_nodes = (uint32_t)nodes.size();
+  sv->components = components >= 4 ? 4 : components;
+  sv->format = format;
+  for (int k = 0; k < 3; k++) sv->global_dim[k] = global_dim ? global_dim[k] : 0;
+  sv->d_nodes = nullptr;
+  sv->d_blocks = nullptr;
+  hipError_t e = hipMalloc((void **)&sv->d_nodes, sizeof(TlasNode) * nodes.size());
+  if (e == hipSuccess) e = hipMalloc((void **)&sv->d_blocks, sizeof(SparseBlock) * count);
+  if (e == hipSuccess)
+    e = hipMemcpy(sv->d_nodes, nodes.data(), sizeof(TlasNode) * nodes.size(),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(sv->d_blocks, dev.data(), sizeof(SparseBlock) * count, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(sv->d_nodes);
+    (void)hipFree(sv->d_blocks);
+    delete sv;
+    return hip_error(e, "sparse volume upload");
+  }
+  *out = sv;
+  return SRF_OK;
+}
+
+void sparse_free(SparseVolume *sv) {
+  if (!sv) return;
+  (void)hipFree(sv->d_nodes);
+  (void)hipFree(sv->d_blocks);
+  delete sv;
+}
+
+srf_status sparse_sample(const SparseVolume *sv, const double *pos, const float *uvw, uint64_t n,
+                         double *out_d, float *out_f, hipStream_t s) {
+  if (n == 0) return SRF_OK;
+  hipLaunchKernelGGL(k_sparse_sample, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s,
+                     sv->d_nodes, sv->num_nodes, sv->d_blocks, sv->components, sv->format, pos,
+                     uvw, sv->global_dim[0], sv->global_dim[1], sv->global_dim[2], (uint32_t)n,
+                     reinterpret_cast<double4 *>(out_d), reinterpret_cast<float4 *>(out_f));
+  SRF_HIP(hipGetLastError());
+  return SRF_OK;
+}
+
+}  // namespace srf

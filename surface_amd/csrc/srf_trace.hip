@@ -413,6 +413,110 @@ __device__ __forceinline__ bool sph_big_leaf(const SphSub &sub, uint32_t nbase, 
   return false;
 }
 
+// The same for all lanes of a wavefront at once (called with uniform
+// control flow; `in` says whether this lane takes part). The lanes share
+// one walk over the secondary tree -- node and sphere indices are made
+// wavefront-uniform, so their loads go through the scalar unit -- in the
+// order of their common direction signs (near child by the node's split
+// axis), with a shared stack of (node, lane mask): a lane takes part in a
+// node when its own bound test passes at its own kept distance. Whether a
+// box can be skipped depends only on the lane's kept distance, and the
+// leaf's result (the smallest (t, position) below the distance on entry;
+// kAny: any t below tlimit) does not depend on the order the spheres are
+// tested in, so every lane ends with what sph_big_leaf gives it. sb: the
+// wavefront's sign bits; wn / wm: kTraceStack stack entries of this
+// wavefront.
+template <bool kAny>
+__device__ __forceinline__ void sph_big_leaf_wave(const SphSub &sub, uint32_t nbase,
+                                                  uint32_t pbase, uint32_t first, uint32_t count,
+                                                  const uint32_t *__restrict__ sorted_index,
+                                                  const RaySetup &s, uint32_t sb, uint32_t prev,
+                                                  double prevT, const float prevN[3],
+                                                  float tlimit, bool in, uint32_t *wn,
+                                                  uint64_t *wm, float &t, uint32_t &best,
+                                                  bool &occ) {
+  const uint64_t me = 1ull << (threadIdx.x & 63u);
+  const uint64_t all = __ballot(in);
+  uint64_t mask = all;
+  bool kept = false;  // `best` is one of this leaf's spheres
+  const uint32_t nnodes = 2 * count - 1;
+  uint32_t cur = 0;
+  int wsp = 0;
+  while (mask) {
+    uint4 meta = make_uint4(0u, 0u, 0u, 0u);
+    bool leaf = false, bad = false;
+    for (;;) {
+      cur = __builtin_amdgcn_readfirstlane(cur);
+      if (cur >= nnodes || nbase + cur >= sub.nnodes) {
+        bad = true;
+        break;
+      }
+      const float4 *nd = sub.nodes + 4 * (size_t)(nbase + cur);
+      const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+      meta = reinterpret_cast<const uint4 *>(nd)[3];
+      keep_loaded(n0, n1, n2);
+      if (meta.x != 0) {
+        leaf = true;
+        break;
+      }
+      const bool act = (mask & me) != 0 && !(kAny && occ);
+      float l0 = 0.0f, l1 = 0.0f;
+      bool h0 = act && sub_box(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, l0);
+      bool h1 = act && sub_box(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, l1);
+      h0 = h0 && (kAny ? l0 < tlimit : l0 <= t);
+      h1 = h1 && (kAny ? l1 < tlimit : l1 <= t);
+      const int nearc = (sb >> meta.y) & 1u;
+      const uint64_t nm = __ballot(nearc ? h1 : h0), fm = __ballot(nearc ? h0 : h1);
+      const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+      if (nm) {
+        if (fm && wsp < kTraceStack) {
+          wn[wsp] = cf;
+          wm[wsp] = fm;
+          wsp++;
+        }
+        cur = cn;
+        mask = nm;
+      } else if (fm) {
+        cur = cf;
+        mask = fm;
+      } else {
+        break;
+      }
+    }
+    if (bad) break;
+    if (leaf) {
+      const bool act = (mask & me) != 0;
+      const uint32_t k0 = __builtin_amdgcn_readfirstlane(meta.w);
+      const uint32_t k1 = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, count));
+      for (uint32_t k = k0; k < k1; k++) {
+        const uint32_t j = pbase + k;
+        if (j >= sub.nprims) break;
+        const float4 c = sub.prims[j];
+        const uint32_t loc = sub.sidx[j];
+        if (loc >= count) continue;
+        const uint32_t q = first + loc;
+        const uint32_t sid = sorted_index[q];
+        if (!act || (kAny && occ)) continue;
+        const bool self = (prev != SRF_NO_PRIM) && (sid == prev);
+        float tt;
+        if (!sphere_isect(s, c, INFINITY, prevT, prevN, self, tt)) continue;
+        if (kAny) {
+          if (tt < tlimit) occ = true;
+        } else if (tt < t || (tt == t && kept && q < best)) {
+          t = tt;
+          best = q;
+          kept = true;
+        }
+      }
+    }
+    if (kAny && (__ballot(occ) & all) == all) break;
+    if (wsp == 0) break;
+    --wsp;
+    cur = wn[wsp];
+    mask = wm[wsp];
+  }
+}
+
 // Closest (kAny: any closer than tlimit) sphere of one ray,
 // ParticleAccel::Traverse (render/render_accel_particle.cc:3053-3218).
 // `best` receives the sorted position of the hit sphere (SRF_NO_PRIM on a
@@ -502,8 +606,8 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded,
             uint32_t max_nodes, uint32_t nprims, SphSub sub) {
   __shared__ uint32_t stack[2 * kTraceStack * kTraceBlock];
-  __shared__ uint32_t wnode[kAny ? 1 : kTraceBlock / 64][kTraceStack];
-  __shared__ uint64_t wmask[kAny ? 1 : kTraceBlock / 64][kTraceStack];
+  __shared__ uint32_t wnode[kAny ? 1 : kTraceBlock / 64][2 * kTraceStack];
+  __shared__ uint64_t wmask[kAny ? 1 : kTraceBlock / 64][2 * kTraceStack];
   const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -573,12 +677,14 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
         const bool in = (mask & me) != 0;
         if (meta.z > kBigLeaf && sub.nodes) {
           const float4 *nd = nodes + 4 * (size_t)cur;
-          const uint32_t nbase = __float_as_uint(nd[0].w), pbase = __float_as_uint(nd[1].x);
-          if (in && meta.w < nprims && meta.z <= nprims - meta.w)
-            sph_big_leaf<false, kTraceBlock>(sub, nbase, pbase, meta.w, meta.z, sorted_index, s,
-                                             prev, prevT, prevN, FLT_MAX,
-                                             stack + threadIdx.x + kTraceStack * kTraceBlock,
-                                             t, best);
+          const uint32_t nbase = __builtin_amdgcn_readfirstlane(__float_as_uint(nd[0].w));
+          const uint32_t pbase = __builtin_amdgcn_readfirstlane(__float_as_uint(nd[1].x));
+          if (meta.w < nprims && meta.z <= nprims - meta.w) {
+            bool unused = false;
+            sph_big_leaf_wave<false>(sub, nbase, pbase, meta.w, meta.z, sorted_index, s, sb, prev,
+                                     prevT, prevN, FLT_MAX, in, &wnode[wv][kTraceStack],
+                                     &wmask[wv][kTraceStack], t, best, unused);
+          }
         } else {
           const uint32_t q0 = __builtin_amdgcn_readfirstlane(meta.w);
           const uint32_t q1 = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, nprims));
@@ -2276,8 +2382,8 @@ k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims
               const uint32_t *__restrict__ order, uint32_t n, uint32_t *__restrict__ occluded,
               uint32_t max_nodes, uint32_t nprims, uint32_t *counter, SphSub sub) {
   __shared__ uint32_t stack[(K == kSph ? 2 : 1) * kTraceStack * kTraceBlock];
-  __shared__ uint32_t wnode[K == kSph ? kTraceBlock / 64 : 1][kTraceStack];
-  __shared__ uint64_t wmask[K == kSph ? kTraceBlock / 64 : 1][kTraceStack];
+  __shared__ uint32_t wnode[K == kSph ? kTraceBlock / 64 : 1][2 * kTraceStack];
+  __shared__ uint64_t wmask[K == kSph ? kTraceBlock / 64 : 1][2 * kTraceStack];
   uint32_t *st = stack + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t base = (counter ? blockIdx.x : xcd_block()) * kTraceBlock + (threadIdx.x & ~63u);
@@ -2363,11 +2469,12 @@ k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims
             const bool in = (mask & me) != 0 && !occ;
             if (meta.z > kBigLeaf && sub.nodes) {
               const float4 *nd = nodes + 4 * (size_t)cur;
-              const uint32_t nbase = __float_as_uint(nd[0].w), pbase = __float_as_uint(nd[1].x);
-              if (in && meta.w < nprims && meta.z <= nprims - meta.w)
-                occ = sph_big_leaf<true, kTraceBlock>(sub, nbase, pbase, meta.w, meta.z,
-                                                      sorted_index, s, prev, 0.0, zero, tlimit,
-                                                      st + kTraceStack * kTraceBlock, t, best);
+              const uint32_t nbase = __builtin_amdgcn_readfirstlane(__float_as_uint(nd[0].w));
+              const uint32_t pbase = __builtin_amdgcn_readfirstlane(__float_as_uint(nd[1].x));
+              if (meta.w < nprims && meta.z <= nprims - meta.w)
+                sph_big_leaf_wave<true>(sub, nbase, pbase, meta.w, meta.z, sorted_index, s, sb,
+                                        prev, 0.0, zero, tlimit, in, &wnode[wv][kTraceStack],
+                                        &wmask[wv][kTraceStack], t, best, occ);
             } else {
               const uint32_t q0 = __builtin_amdgcn_readfirstlane(meta.w);
               const uint32_t q1 = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, nprims));

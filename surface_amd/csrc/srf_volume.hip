@@ -34,9 +34,18 @@ namespace srf {
 
 struct Volume {
   int device;
-  const float *vox;
+  const float *vox;  // FP32 single-component voxels (the march); else null
   int dims[3];
   int clamp[3];
+  // Texture3D (render/render_texture.h:34-44): image, data_type (1 byte,
+  // 8 float, 16 double) and interleaved components; Texture::Fetch's
+  // per-axis remap tables (gles/gles_common.h:263-269,377-389), device
+  // arrays or null
+  const void *image;
+  int format;
+  int comps;
+  const float *remap[3];
+  int remap_n[3];
 };
 
 namespace {
@@ -190,6 +199,64 @@ __device__ __forceinline__ void load_ray(const srf_ray *rays, uint32_t i, float 
   d[0] = b.x;
   d[1] = b.y;
   d[2] = b.z;
+}
+
+// local::remap (gles/gles_common.h:263-269)
+__device__ __forceinline__ float remap_coord(float x, const float *__restrict__ table, int n) {
+  int idx = (int)(x * (float)n);
+  idx = idx < n - 1 ? idx : n - 1;
+  idx = idx > 0 ? idx : 0;
+  return table[idx];
+}
+
+__device__ __forceinline__ double dlerp(double t, double a, double b) {
+  // render/render_texture.cc:32-34
+  return (1.0 - t) * a + t * b;
+}
+
+// Texture::Fetch of a dense texture (gles/gles_common.h:344-403): remap,
+// then FilterTexture3DByte (voxel / 255.5f, render/render_texture.cc:832-944),
+// FilterTexture3DFloat (:946-1058) or FilterTexture3DDouble (:1060-1172) over
+// V.comps interleaved components; one component is splatted to RGBA.
+__global__ void __launch_bounds__(256) k_tex_fetch(Volume V, const float *__restrict__ uvw,
+                                                  uint32_t n, float4 *__restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float c[3] = {uvw[3 * (size_t)i], uvw[3 * (size_t)i + 1], uvw[3 * (size_t)i + 2]};
+  for (int k = 0; k < 3; k++)
+    if (V.remap[k]) c[k] = remap_coord(c[k], V.remap[k], V.remap_n[k]);
+  Geom g;
+  tex_geom(V, c[0], c[1], c[2], g);
+  float vals[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  const size_t cs = (size_t)V.comps;
+  for (int q = 0; q < 4; q++) {
+    if (q >= V.comps) break;
+    if (V.format == 16) {
+      const double *d = (const double *)V.image;
+      double D[8];
+      for (int k = 0; k < 8; k++) D[k] = d[(size_t)g.a[k] * cs + q];
+      const double dx = g.dx, dy = g.dy, dz = g.dz;
+      const double d00 = dlerp(dx, D[0], D[1]), d10 = dlerp(dx, D[2], D[3]);
+      const double d01 = dlerp(dx, D[4], D[5]), d11 = dlerp(dx, D[6], D[7]);
+      const double d0 = dlerp(dy, d00, d10), d1 = dlerp(dy, d01, d11);
+      vals[q] = (float)dlerp(dz, d0, d1);
+    } else {
+      float D[8];
+      if (V.format == 1) {
+        const unsigned char *b = (const unsigned char *)V.image;
+        for (int k = 0; k < 8; k++) D[k] = (float)b[(size_t)g.a[k] * cs + q] / 255.5f;
+      } else {
+        const float *f = (const float *)V.image;
+        for (int k = 0; k < 8; k++) D[k] = f[(size_t)g.a[k] * cs + q];
+      }
+      const float d00 = lerpf(g.dx, D[0], D[1]), d10 = lerpf(g.dx, D[2], D[3]);
+      const float d01 = lerpf(g.dx, D[4], D[5]), d11 = lerpf(g.dx, D[6], D[7]);
+      const float d0 = lerpf(g.dy, d00, d10), d1 = lerpf(g.dy, d01, d11);
+      vals[q] = lerpf(g.dz, d0, d1);
+    }
+  }
+  out[i] = V.comps == 1 ? make_float4(vals[0], vals[0], vals[0], vals[0])
+                        : make_float4(vals[0], vals[1], vals[2], vals[3]);
 }
 
 __global__ void __launch_bounds__(256) k_vol_sample(Volume V, const float *__restrict__ uvw,
@@ -685,6 +752,9 @@ srf_status srf_volume_create(srf_ctx *ctx, const float *voxels, int w, int h, in
   Volume *v = new Volume();
   v->device = c->device;
   v->vox = voxels;
+  v->image = voxels;
+  v->format = 8;
+  v->comps = 1;
   v->dims[0] = w;
   v->dims[1] = h;
   v->dims[2] = d;
@@ -692,6 +762,63 @@ srf_status srf_volume_create(srf_ctx *ctx, const float *voxels, int w, int h, in
   v->clamp[1] = wrap_t;
   v->clamp[2] = wrap_r;
   *out = reinterpret_cast<srf_volume *>(v);
+  return SRF_OK;
+}
+
+srf_status srf_texture3d_create(srf_ctx *ctx, const void *image, int data_type, int components,
+                                int w, int h, int d, int wrap_s, int wrap_t, int wrap_r,
+                                srf_volume **out) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !image || !out) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (data_type != SRF_TEX_BYTE && data_type != SRF_TEX_FLOAT && data_type != SRF_TEX_DOUBLE)
+    return set_error(SRF_ERR_UNSUPPORTED, "texture data type %d (byte 1, float 8, double 16)",
+                     data_type);
+  if (components < 1 || components > 4)
+    return set_error(SRF_ERR_INVALID_ARG, "components must be 1..4 (got %d)", components);
+  if (w < 1 || h < 1 || d < 1)
+    return set_error(SRF_ERR_INVALID_ARG, "texture dims must be >= 1 (got %d %d %d)", w, h, d);
+  if ((uint64_t)w * h * d >= (1ull << 32))
+    return set_error(SRF_ERR_UNSUPPORTED, "texture exceeds 2^32 texels");
+  for (int k : {wrap_s, wrap_t, wrap_r})
+    if (k != 0 && k != 1) return set_error(SRF_ERR_INVALID_ARG, "wrap must be 0 or 1");
+  Volume *v = new Volume();
+  v->device = c->device;
+  v->image = image;
+  v->format = data_type;
+  v->comps = components;
+  // the march reads FP32 single-component volumes of at least 2 voxels a side
+  v->vox = (data_type == SRF_TEX_FLOAT && components == 1 && w >= 2 && h >= 2 && d >= 2)
+               ? (const float *)image : nullptr;
+  v->dims[0] = w;
+  v->dims[1] = h;
+  v->dims[2] = d;
+  v->clamp[0] = wrap_s;
+  v->clamp[1] = wrap_t;
+  v->clamp[2] = wrap_r;
+  *out = reinterpret_cast<srf_volume *>(v);
+  return SRF_OK;
+}
+
+srf_status srf_texture3d_remap(srf_volume *tex, int axis, const float *table, int n) {
+  Volume *v = reinterpret_cast<Volume *>(tex);
+  if (!v) return set_error(SRF_ERR_INVALID_ARG, "null texture");
+  if (axis < 0 || axis > 2) return set_error(SRF_ERR_INVALID_ARG, "axis must be 0, 1 or 2");
+  if (table && n < 1) return set_error(SRF_ERR_INVALID_ARG, "remap table of %d entries", n);
+  v->remap[axis] = table;
+  v->remap_n[axis] = table ? n : 0;
+  return SRF_OK;
+}
+
+srf_status srf_texture3d_fetch(const srf_volume *tex, const float *uvw, uint64_t n, float *rgba,
+                               void *stream) {
+  const Volume *v = reinterpret_cast<const Volume *>(tex);
+  if (!v || (n && (!uvw || !rgba))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (n == 0) return SRF_OK;
+  if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many samples");
+  hipLaunchKernelGGL(k_tex_fetch, dim3(vblocks(n, 256)), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), *v, uvw, (uint32_t)n,
+                     reinterpret_cast<float4 *>(rgba));
+  SRF_HIP(hipGetLastError());
   return SRF_OK;
 }
 
@@ -704,6 +831,9 @@ srf_status srf_volume_sample(const srf_volume *vol, const float *uvw, uint64_t n
                              void *stream) {
   const Volume *v = reinterpret_cast<const Volume *>(vol);
   if (!v || (n && (!uvw || !out))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (!v->vox)
+    return set_error(SRF_ERR_UNSUPPORTED, "srf_volume_sample reads FP32 single-component "
+                                          "volumes; use srf_texture3d_fetch");
   if (n == 0) return SRF_OK;
   if (n >= (1ull << 32)) return set_error(SRF_ERR_INVALID_ARG, "too many samples");
   hipLaunchKernelGGL(k_vol_sample, dim3(vblocks(n, 256)), dim3(256), 0,
@@ -715,6 +845,9 @@ srf_status srf_volume_sample(const srf_volume *vol, const float *uvw, uint64_t n
 static srf_status check_march(const Volume *v, const float *tf, int tf_size, float step,
                               const srf_ray *rays, uint64_t n) {
   if (!v || !tf || (n && !rays)) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (!v->vox)
+    return set_error(SRF_ERR_UNSUPPORTED,
+                     "the march takes FP32 single-component volumes of >= 2 voxels a side");
   if (tf_size < 2 || tf_size > kMaxTf)
     return set_error(SRF_ERR_INVALID_ARG, "tf_size must be in [2, %d]", kMaxTf);
   if (!(step > 0.0f)) return set_error(SRF_ERR_INVALID_ARG, "step must be > 0");

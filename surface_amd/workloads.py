@@ -39,6 +39,11 @@ SOL_EXT_DTYPE = np.dtype([("d", "<f4", (8,)), ("pos", "<f4", (3,)), ("scalar", "
 # srf_line_hit_ext: position (org + t dir), cap flag, normal
 LIN_EXT_DTYPE = np.dtype([("pos", "<f4", (3,)), ("hit_cap", "<u4"), ("nrm", "<f4", (3,)),
                           ("pad", "<f4")])
+# srf_isect: the BuildIntersection fields of a closest hit (80 bytes)
+ISECT_DTYPE = np.dtype([("position", "<f4", (3,)), ("f0", "<u4"), ("geometric", "<f4", (3,)),
+                        ("f1", "<u4"), ("normal", "<f4", (3,)), ("f2", "<u4"),
+                        ("tangent", "<f4", (3,)), ("f3", "<u4"), ("binormal", "<f4", (3,)),
+                        ("pad", "<f4")])
 NO_PRIM = np.uint32(0xFFFFFFFF)
 
 
@@ -461,3 +466,64 @@ def scene_cameras(T, count=16, width=1024, height=1024, fov=40.0, tiled=True):
         out.append(pinhole_rays(tuple(eye), tuple(centre), (0, 0, 1), fov, width, height,
                                 tiled=tiled))
     return out
+
+
+def ao_rays_from_hits(rays, hits, P, R, per_hit=4, eps=1e-3, dist=0.05, seed=BASE_SEED + 12,
+                      every=1):
+    """Ambient-occlusion rays in numpy (a fixed input for tests): for every
+    `every`-th ray i with a sphere hit, `per_hit` cosine-weighted directions
+    about the outward normal at the hit point, origin p + eps r n, told to
+    skip the hit sphere (prev_prim_id). Returns (rays, t_max)."""
+    idx = np.nonzero(hits["prim_id"] != NO_PRIM)[0][::every]
+    c = hits["prim_id"][idx].astype(np.int64)
+    p = (rays["org"][idx].astype(np.float64)
+         + hits["t"][idx, None].astype(np.float64) * rays["dir"][idx].astype(np.float64))
+    n = p - P[c].astype(np.float64)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    org = p + eps * R[c, None].astype(np.float64) * n
+    rng = _rng(seed)
+    k = len(idx)
+    u1 = rng.random((k, per_hit))
+    u2 = rng.random((k, per_hit))
+    sg = np.where(n[:, 2] >= 0, 1.0, -1.0)
+    a = -1.0 / (sg + n[:, 2])
+    b = n[:, 0] * n[:, 1] * a
+    b1 = np.stack([1.0 + sg * n[:, 0] * n[:, 0] * a, sg * b, -sg * n[:, 0]], 1)
+    b2 = np.stack([b, sg + n[:, 1] * n[:, 1] * a, -n[:, 1]], 1)
+    r, phi = np.sqrt(u1), 2.0 * np.pi * u2
+    d = ((r * np.cos(phi))[..., None] * b1[:, None] + (r * np.sin(phi))[..., None] * b2[:, None]
+         + np.sqrt(np.maximum(0.0, 1.0 - u1))[..., None] * n[:, None])
+    out = np.zeros(k * per_hit, RAY_DTYPE)
+    out["org"] = np.repeat(org, per_hit, axis=0).astype(np.float32)
+    out["dir"] = d.reshape(-1, 3).astype(np.float32)
+    out["flags"] = 1
+    out["prev_prim_id"] = np.repeat(hits["prim_id"][idx], per_hit)
+    return out, np.full(k * per_hit, np.float32(dist))
+
+
+def sparse_blocks(seed=BASE_SEED + 13, grid=(4, 3, 2), block=16, components=1, dtype=np.float32,
+                  overlap=False):
+    """A sparse volume: a grid of blocks of `block`^3 cells in the global
+    grid, each with its own voxel resolution (some coarser, as LoD levels
+    are), a few blocks left out; with overlap some blocks are shifted into
+    their neighbours. Returns (blocks int32 [nb, 9]: offset, extent, size;
+    list of voxel arrays; global_dim)."""
+    rng = _rng(seed)
+    out, data = [], []
+    for z in range(grid[2]):
+        for y in range(grid[1]):
+            for x in range(grid[0]):
+                if rng.random() < 0.15:
+                    continue
+                off = np.array([x, y, z]) * block
+                if overlap and rng.random() < 0.3:
+                    off = off + rng.integers(-block // 3, block // 3, 3)
+                size = rng.choice([block, block // 2, block // 4], 3)
+                out.append(np.concatenate([off, [block] * 3, size]).astype(np.int32))
+                n = int(np.prod(size)) * components
+                if dtype == np.uint8:
+                    data.append(rng.integers(0, 256, n).astype(np.uint8))
+                else:
+                    data.append(rng.uniform(-1, 1, n).astype(dtype))
+    gdim = np.array(grid, np.int32) * block
+    return np.stack(out), data, gdim

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "../../include/srf_accel.hpp"
@@ -111,6 +112,59 @@ int main() {
     hit_count += hits[i].prim_id != SRF_NO_PRIM;
   }
   CHECK(hit_count > (int)rays.size() / 4);
+
+  // Build32 from the host arrays gives the same tree; the per-ray
+  // Traverse(Intersection&, Ray&), called from 4 host threads at once,
+  // gives the oracle's hit and BuildIntersection fields
+  // (render/render_accel_triangle.cc:2439-2507) for every ray.
+  srf::TriangleAccel host_accel(ctx);
+  CHECK(host_accel.Build32Host(V.data(), nv, F.data(), nf));
+  std::vector<or_tri_node> hnodes(info.num_nodes);
+  std::vector<uint32_t> hidx(nf);
+  CHECK(host_accel.Export(hnodes.data(), hidx.data()));
+  CHECK(std::memcmp(hnodes.data(), nodes.data(), nodes.size() * sizeof(or_tri_node)) == 0);
+  CHECK(hidx == idx);
+  std::vector<or_isect> want_is(rays.size());
+  or_isect_tri(V.data(), F.data(), reinterpret_cast<const or_ray *>(rays.data()),
+               want_hits.data(), rays.size(), want_is.data());
+  std::vector<srf::Intersection> got(rays.size());
+  std::vector<int> got_hit(rays.size(), -1);
+  std::vector<std::thread> pool;
+  for (int w = 0; w < 4; w++)
+    pool.emplace_back([&, w]() {
+      for (size_t i = w; i < rays.size(); i += 4 * 7) {
+        srf::Ray r;
+        for (int k = 0; k < 3; k++) {
+          r.org[k] = rays[i].org[k];
+          r.dir[k] = rays[i].dir[k];
+        }
+        got_hit[i] = host_accel.Traverse(got[i], r) ? 1 : 0;
+      }
+    });
+  for (auto &t : pool) t.join();
+  int checked = 0;
+  for (size_t i = 0; i < rays.size(); i++) {
+    if (got_hit[i] < 0) continue;
+    checked++;
+    const bool hit = want_hits[i].prim_id != SRF_NO_PRIM;
+    CHECK(got_hit[i] == (hit ? 1 : 0));
+    CHECK(got[i].prim_id == want_hits[i].prim_id);
+    CHECK(std::memcmp(&got[i].t, &want_hits[i].t, 4) == 0);
+    if (!hit) continue;
+    CHECK(std::memcmp(&got[i].u, &want_hits[i].u, 4) == 0);
+    CHECK(std::memcmp(&got[i].v, &want_hits[i].v, 4) == 0);
+    CHECK(std::memcmp(got[i].position, want_is[i].position, 12) == 0);
+    CHECK(std::memcmp(got[i].geometric, want_is[i].geometric, 12) == 0);
+    CHECK(std::memcmp(got[i].normal, want_is[i].normal, 12) == 0);
+    CHECK(got[i].f0 == want_is[i].f0 && got[i].f1 == want_is[i].f1 &&
+          got[i].f2 == want_is[i].f2);
+  }
+  CHECK(checked > 500);
+  // the batched host form gives the same hits
+  std::vector<srf_hit> hh(rays.size());
+  CHECK(host_accel.TraverseHost(rays.data(), rays.size(), hh.data()));
+  CHECK(std::memcmp(hh.data(), want_hits.data(), hh.size() * sizeof(srf_hit)) == 0);
+
   // a mesh below Build32's 1024-primitive floor is refused, as documented
   srf::TriangleAccel small(ctx);
   CHECK(!small.Build32(dV, nv, dF, 1000));

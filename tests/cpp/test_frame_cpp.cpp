@@ -5,7 +5,8 @@
 // builds once; every pixel's depth is the oracle's hit distance for the same
 // eye ray (oracle/liboracle.so, the restatement of TriangleAccel::Traverse),
 // written Y-down, and missed pixels stay untouched; queued closest and any-hit
-// results equal one batch trace of the same rays. Prints "PASS".
+// results equal the oracle's trace of the same shadow rays (and one batch
+// trace of them). Prints "PASS".
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -91,7 +92,8 @@ int main() {
   // shadow rays from every hit point toward a light, queued by 8 threads
   std::vector<srf_ray> sh;
   std::vector<float> tmax;
-  const float light[3] = {2.0f, -1.0f, 3.0f};
+  // a low light, so that about half of the shadow rays are occluded
+  const float light[3] = {2.5f, 0.5f, 0.2f};
   for (size_t i = 0; i < rays.size(); i++) {
     if (want[i].prim_id == SRF_NO_PRIM) continue;
     srf_ray r = rays[i];
@@ -144,13 +146,23 @@ int main() {
   CHECK(hipMemcpy(bh.data(), dH, sh.size() * sizeof(srf_hit), hipMemcpyDeviceToHost) ==
         hipSuccess);
   CHECK(hipMemcpy(bo.data(), dO, sh.size() * 4, hipMemcpyDeviceToHost) == hipSuccess);
+  // the queued results against the oracle's TriangleAccel::Traverse of the
+  // same shadow rays (shadow() = closest-hit t, gles/gles_glsl_runtime.cc:
+  // 179-217; occluded iff that t < tmax), and against the batch trace
+  std::vector<or_hit> osh(sh.size());
+  or_trace_tri(nodes.data(), idx.data(), V.data(), F.data(), eps,
+               reinterpret_cast<const or_ray *>(sh.data()), sh.size(), osh.data(), nullptr, 8);
   int occluded = 0;
   for (size_t i = 0; i < sh.size(); i++) {
-    CHECK(std::memcmp(&qc.Hit(tc[i]), &bh[i], sizeof(srf_hit)) == 0);
-    CHECK(qc.ShadowT(tc[i]) == (bh[i].prim_id == SRF_NO_PRIM ? -3.402823466e38f : bh[i].t));
-    CHECK(qa.Occluded(ta[i]) == (bo[i] != 0));
-    occluded += bo[i] != 0;
+    CHECK(std::memcmp(&qc.Hit(tc[i]), &osh[i], sizeof(srf_hit)) == 0);
+    CHECK(qc.ShadowT(tc[i]) == (osh[i].prim_id == SRF_NO_PRIM ? -3.402823466e38f : osh[i].t));
+    const bool occ = osh[i].prim_id != SRF_NO_PRIM && osh[i].t < tmax[i];
+    CHECK(qa.Occluded(ta[i]) == occ);
+    CHECK(std::memcmp(&bh[i], &osh[i], sizeof(srf_hit)) == 0);
+    CHECK((bo[i] != 0) == occ);
+    occluded += occ;
   }
+  CHECK(occluded > 0 && occluded < (int)sh.size());
   hipFree(dR);
   hipFree(dH);
   hipFree(dT);

@@ -5,9 +5,11 @@ built from the reference sources by oracle/Makefile) on small deterministic
 inputs and stores inputs + outputs as .npz fixtures; for the full-size
 heightfield (522,242 triangles, 1024^2 rays) and tetra grid (1,572,864 tets,
 1024^2 rays) it stores SHA-256 digests of the tree, the permutation and the
-per-ray hits. Run in the container that has the reference checkout:
+per-ray hits; --only full_extra adds the same for the 1M-triangle heightfield,
+the 64-mesh scene and the 16M-sphere set. Run in the container that has the
+reference checkout:
 
-    make -C oracle && python tests/golden/gen_golden.py [--full] [--only scene|sol|lin]
+    make -C oracle && python tests/golden/gen_golden.py [--full] [--only scene|sol|lin|full_extra]
 """
 from __future__ import annotations
 
@@ -172,13 +174,83 @@ def scene_case(ref):
                 meshes_sha=np.array(sha(*[a for m in meshes for a in m])))
 
 
+def row_sample(rays, width, every):
+    """Every `every`-th chunk of `width` consecutive rays."""
+    return np.ascontiguousarray(rays.reshape(-1, width)[::every].reshape(-1))
+
+
+def full_extra(ref, digests, t0):
+    """Digests at full size for the configurations whose parity was pinned
+    only at reduced size: the heightfield of 725^2 vertices (1,048,352
+    triangles, 1024^2 rays), the 64-mesh scene (every mesh's tree and
+    permutation; RenderGraph hits of every 16th ray row of the 16 cameras)
+    and the 16,777,216 Plummer spheres (tree, permutation, primary hits of
+    every 16th row of the 2048^2 rays, and occlusion of numpy AO rays from
+    every 4th of those hits, 4 per hit: closest hit t < t_max)."""
+    V, F = W.heightfield(725)
+    tree = ref.build("tri", V, F, keep=True)
+    hits, _ = ref.trace(tree, W.heightfield_rays(1024, 1024))
+    ref.free(tree)
+    digests["heightfield725"] = {
+        "num_tris": int(len(F)), "num_nodes": int(len(tree["nodes"])),
+        "nodes": sha(canonical_tri(tree["nodes"])), "indices": sha(tree["indices"]),
+        "bounds": [float(x) for x in tree["bounds"]], "hits": sha(hits),
+        "hit_fraction": float((hits["prim_id"] != W.NO_PRIM).mean())}
+    print("heightfield725 done", time.time() - t0, flush=True)
+
+    meshes, T = W.batch_scene(64)
+    trees = [ref.build("tri", Vm, Fm, keep=True) for Vm, Fm in meshes]
+    sc = ref.scene_build(trees, T)
+    cams = W.scene_cameras(T, count=16, width=1024, height=1024)
+    rays = np.concatenate([row_sample(c, 1024, 16) for c in cams])
+    hits, inst = ref.scene_trace(sc, rays, width=1024)
+    digests["scene64"] = {
+        "meshes": len(meshes), "triangles": int(sum(len(Fm) for _, Fm in meshes)),
+        "mesh_nodes": [sha(canonical_tri(t["nodes"])) for t in trees],
+        "mesh_indices": [sha(t["indices"]) for t in trees],
+        "tlas": sha(canonical_tlas(sc["tlas"])), "rays": int(len(rays)),
+        "hits": sha(hits, inst), "hit_fraction": float((inst != W.NO_PRIM).mean())}
+    ref.scene_free(sc)
+    for t in trees:
+        ref.free(t)
+    print("scene64 done", time.time() - t0, flush=True)
+
+    P, R = W.plummer_spheres(1 << 24)
+    tree = ref.build("sph", P, R=R, keep=True)
+    prim = row_sample(W.sphere_rays(2048, 2048), 2048, 16)
+    hits, _ = ref.trace(tree, prim, width=2048)
+    ao, tmax = W.ao_rays_from_hits(prim, hits, P, R, per_hit=4, every=4)
+    aoh, _ = ref.trace(tree, ao, width=1024)
+    occ = ((aoh["prim_id"] != W.NO_PRIM) & (aoh["t"] < tmax)).astype(np.int32)
+    ref.free(tree)
+    digests["plummer16M"] = {
+        "num_spheres": int(len(P)), "num_nodes": int(len(tree["nodes"])),
+        "nodes": sha(canonical_sph(tree["nodes"])), "indices": sha(tree["indices"]),
+        "bounds": [float(x) for x in tree["bounds"]], "primary_rays": int(len(prim)),
+        "primary_prim_t": sha(hits["prim_id"], hits["t"]),
+        "hit_fraction": float((hits["prim_id"] != W.NO_PRIM).mean()),
+        "ao_rays": int(len(ao)), "ao_rays_sha": sha(ao, tmax), "ao_occluded": sha(occ),
+        "ao_occluded_fraction": float(occ.mean())}
+    print("plummer16M done", time.time() - t0, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--full", action="store_true", help="also write full-size digests")
-    ap.add_argument("--only", choices=["scene", "sol", "lin"], help="write one fixture only")
+    ap.add_argument("--only", choices=["scene", "sol", "lin", "full_extra"],
+                    help="write one fixture only (full_extra: the digests of the 1M-triangle "
+                         "heightfield, the 64-mesh scene and the 16M-sphere set, merged into "
+                         "full_digests.json)")
     args = ap.parse_args()
     ref = Reference("serial")
     t0 = time.time()
+    if args.only == "full_extra":
+        path = os.path.join(HERE, "full_digests.json")
+        digests = json.load(open(path))
+        full_extra(ref, digests, t0)
+        with open(path, "w") as f:
+            json.dump(digests, f, indent=1)
+        return
     if args.only == "scene":
         np.savez_compressed(os.path.join(HERE, "scene_small.npz"), **scene_case(ref))
         return

@@ -900,3 +900,200 @@ def test_line_trace_equals_bruteforce(orc):
         if bi >= 0:
             assert hits["prim_id"][i] == bi or _line_bruteforce(
                 orc, P, S[[hits["prim_id"][i]]], R, 0.0, rays["org"][i], rays["dir"][i])[0] == bt
+
+
+# ------------------------------------------------------- BuildIntersection
+
+def _pole_spheres():
+    """Spheres on a grid with rays straight down and straight up onto their
+    poles: the normals are (0, +-1, 0), the branch of BuildIntersection where
+    n.x^2 + n.z^2 == 0."""
+    g = np.arange(40, dtype=np.float32)
+    x, z = np.meshgrid(g, g, indexing="ij")
+    P = np.stack([x.ravel(), np.zeros(x.size, np.float32), z.ravel()], 1).astype(np.float32)
+    R = np.full(len(P), 0.25, np.float32)
+    rays = np.zeros(2 * len(P), W.RAY_DTYPE)
+    rays["org"][:len(P)] = P + np.array([0, 5, 0], np.float32)
+    rays["dir"][:len(P)] = (0, -1, 0)
+    rays["org"][len(P):] = P + np.array([0, -5, 0], np.float32)
+    rays["dir"][len(P):] = (0, 1, 0)
+    rays["flags"] = 1
+    rays["prev_prim_id"] = W.NO_PRIM
+    return P, R, rays
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", ["tri", "tet", "sph", "sph_poles"])
+def test_build_intersection_oracle_equals_live_reference(orc, kind):
+    """position, geometric / shading normal, tangent, binormal and f0..f3 of
+    the oracle's BuildIntersection equal the Intersection the reference's
+    Traverse fills (render/render_accel_{triangle,tetra,particle}.cc), bit for
+    bit, including the pole branch of the sphere's tangent frame."""
+    ref = Reference("serial")
+    if kind == "tri":
+        V, F = W.heightfield(64)
+        rays = W.heightfield_rays(64, 64)
+        h = orc.trace_tri(orc.build_tri(V, F), V, F, rays)
+        got = orc.isect("tri", V, F, rays, h)
+        r = ref.build("tri", V, F, keep=True)
+    elif kind == "tet":
+        V, F, Sc = W.tet_grid(8)
+        rays = W.tet_rays(8, 64, 64)
+        h, e = orc.trace_tet(orc.build_tet(V, F), V, F, rays, Sc)
+        got = orc.isect("tet", V, F, rays, h, e)
+        r = ref.build("tet", V, F, keep=True)
+    else:
+        if kind == "sph":
+            P, R = W.plummer_spheres(20000, mu=-4.0)
+            rays = W.sphere_rays(64, 64)
+        else:
+            P, R, rays = _pole_spheres()
+        h = orc.trace_sph(orc.build_sph(P, R), P, R, 0.0, rays)
+        got = orc.isect("sph", P, None, rays, h)
+        r = ref.build("sph", P, R=R, keep=True)
+        F = None
+    want = ref.trace_isect(r, rays)
+    ref.free(r)
+    hit = h["prim_id"] != W.NO_PRIM
+    assert hit.mean() > 0.3
+    assert got.tobytes() == want.tobytes()
+    if kind == "sph_poles":
+        assert np.all(np.abs(got["normal"][hit][:, 1]) == 1.0)
+        assert np.all(np.abs(got["binormal"][hit][:, 0]) == 1.0)
+
+
+def test_build_intersection_oracle_closed_forms(orc):
+    """Without the reference: the triangle normal is the unit normal of the
+    plane (an axis-aligned quad gives exactly +-z), the position lies on the
+    ray at t, and a sphere's normal has unit length and points from the
+    centre to the hit point."""
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float32)
+    F = np.array([[0, 1, 2], [1, 3, 2]], np.uint32)
+    rays = np.zeros(2, W.RAY_DTYPE)
+    rays["org"] = [[0.25, 0.25, 2.0], [0.75, 0.75, 2.0]]
+    rays["dir"] = [[0, 0, -1], [0, 0, -1]]
+    hits = np.zeros(2, W.HIT_DTYPE)
+    hits["t"] = 2.0
+    hits["prim_id"] = [0, 1]
+    out = orc.isect("tri", V, F, rays, hits)
+    assert np.array_equal(out["normal"], np.array([[0, 0, 1], [0, 0, 1]], np.float32))
+    assert np.array_equal(out["position"][:, 2], [0, 0])
+    assert list(out["f0"]) == [0, 1] and list(out["f2"]) == [2, 2]
+    P, R = W.plummer_spheres(2000, mu=-3.0)
+    rays = W.sphere_rays(32, 32)
+    h = orc.trace_sph(orc.build_sph(P, R), P, R, 0.0, rays)
+    o = orc.isect("sph", P, None, rays, h)
+    hit = h["prim_id"] != W.NO_PRIM
+    n = o["normal"][hit].astype(np.float64)
+    assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-6)
+    c = o["position"][hit] - P[h["prim_id"][hit]]
+    assert np.allclose(n, c / np.linalg.norm(c, axis=1, keepdims=True), atol=1e-5)
+    # tangent = 2 pi (-n.x, n.z, 0), the u-derivative direction; the
+    # binormal's z component is pi * sqrt(n.x^2 + n.z^2)
+    assert np.allclose(o["tangent"][hit][:, 0], -2 * np.pi * n[:, 0], atol=1e-5)
+    assert np.allclose(o["tangent"][hit][:, 1], 2 * np.pi * n[:, 2], atol=1e-5)
+    assert np.allclose(o["binormal"][hit][:, 2], np.pi * np.hypot(n[:, 0], n[:, 2]), atol=1e-5)
+
+
+# ------------------------------------------------- byte / double textures
+
+def _tex_inputs(rng, comps, fmt):
+    dims = np.array([7, 5, 6], np.int32)
+    n = int(np.prod(dims)) * comps
+    if fmt == "byte":
+        img = rng.integers(0, 256, n).astype(np.uint8)
+    elif fmt == "float":
+        img = rng.uniform(-1, 1, n).astype(np.float32)
+    else:
+        img = rng.uniform(-1, 1, n).astype(np.float64)
+    uvw = rng.uniform(-1.5, 2.5, (3000, 3)).astype(np.float32)
+    edge = np.array([[0, 0, 0], [1, 1, 1], [0.5, 0.5, 0.5], [-0.0, 1.0, 0.25],
+                     [-0.01, 1.01, 0.999], [1e-8, -1e-8, 2.0]], np.float32)
+    return dims, img, np.concatenate([uvw, edge])
+
+
+@needs_ref
+@pytest.mark.parametrize("fmt", ["byte", "float", "double"])
+@pytest.mark.parametrize("comps", [1, 2, 3, 4])
+def test_texture_fetch_oracle_equals_live_reference(orc, fmt, comps):
+    """FilterTexture3DByte (/255.5f), FilterTexture3DFloat and
+    FilterTexture3DDouble with 1-4 interleaved components, REPEAT, CLAMP and
+    mixed wraps: the oracle's fetch equals the reference's, bit for bit."""
+    ref = Reference("serial")
+    rng = np.random.default_rng(100 + comps)
+    dims, img, uvw = _tex_inputs(rng, comps, fmt)
+    for clamp in ((0, 0, 0), (1, 1, 1), (1, 0, 1)):
+        want = ref.tex3d_fetch(img, comps, dims, clamp, uvw)
+        got = orc.tex3d_fetch(img, comps, dims, clamp, uvw)
+        assert got.tobytes() == want.tobytes(), clamp
+
+
+def test_texture_fetch_oracle_closed_forms(orc):
+    """Byte texels are value / 255.5 at the corners; a constant field is
+    returned exactly; the remap is a table lookup at (int)(x * n) clamped to
+    the table, so fetching with a remap equals fetching at the looked-up
+    coordinates (the lookup done here in numpy float32)."""
+    dims = np.array([4, 3, 5], np.int32)
+    img = np.arange(int(np.prod(dims)), dtype=np.uint8) * 3
+    corners = np.array([[0, 0, 0], [1, 1, 1], [1, 0, 0]], np.float32)
+    out = orc.tex3d_fetch(img, 1, dims, (1, 1, 1), corners)
+    for k, c in enumerate(corners):
+        idx = int(c[2] * 4) * 12 + int(c[1] * 2) * 4 + int(c[0] * 3)
+        assert out[k, 0] == np.float32(np.float32(img[idx]) / np.float32(255.5))
+    const = np.full(int(np.prod(dims)) * 2, 0.375, np.float64)
+    out = orc.tex3d_fetch(const, 2, dims, (0, 0, 0), np.random.default_rng(1).random((50, 3)))
+    assert np.all(out[:, :2] == np.float32(0.375)) and np.all(out[:, 2:] == 0)
+    rng = np.random.default_rng(3)
+    img = rng.uniform(0, 1, int(np.prod(dims)) * 4).astype(np.float32)
+    tabs = [np.sort(rng.uniform(0, 1, m)).astype(np.float32) for m in (17, 5, 64)]
+    uvw = rng.uniform(-0.2, 1.2, (2000, 3)).astype(np.float32)
+    looked = uvw.copy()
+    for k in range(3):
+        n = len(tabs[k])
+        idx = (uvw[:, k] * np.float32(n)).astype(np.float32).astype(np.int64)
+        looked[:, k] = tabs[k][np.clip(idx, 0, n - 1)]
+    a = orc.tex3d_fetch(img, 4, dims, (0, 1, 0), uvw, remap=tabs)
+    b = orc.tex3d_fetch(img, 4, dims, (0, 1, 0), looked)
+    assert a.tobytes() == b.tobytes()
+
+
+# ------------------------------------------------------ SparseVolumeAccel
+
+@needs_ref
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32, np.float64])
+@pytest.mark.parametrize("comps,overlap", [(1, False), (3, True), (4, False), (5, True)])
+def test_sparse_volume_oracle_equals_live_reference(orc, dtype, comps, overlap):
+    """SparseVolumeAccel::Build + Sample (render/render_accel_volume.cc:35-139)
+    -- the block BVHTree, Locate's first hit (overlapping blocks included),
+    the nearest voxel with the reference's scaleX on every axis, byte / 255,
+    float, double through float, alpha 1 or 0 -- bit for bit."""
+    ref = Reference("serial")
+    blocks, data, gdim = W.sparse_blocks(components=comps, dtype=dtype, overlap=overlap,
+                                         seed=W.BASE_SEED + 13 + comps)
+    rng = np.random.default_rng(comps)
+    pos = rng.uniform(-4, gdim + 4, (5000, 3))
+    # block corners and faces exactly
+    pos[:len(blocks)] = blocks[:, :3]
+    pos[len(blocks):2 * len(blocks)] = blocks[:, :3] + blocks[:, 3:6]
+    want = ref.sparse_sample(blocks, data, comps, pos)
+    got = orc.sparse_sample(blocks, data, comps, pos=pos)
+    assert got.tobytes() == want.tobytes()
+    assert 0.3 < (got[:, 3] != 0).mean() < 1.0
+
+
+def test_sparse_volume_oracle_closed_forms(orc):
+    """One block of extent 8 with 8^3 cells: the sample at a cell's corner is
+    that cell's voxel; outside every block the result is (0, 0, 0, 0); with
+    fewer cells than extent the reference scales y and z by size[0] / extent[0]
+    too (so a block of 8 x 2 x 2 cells reads y = 1.5 as cell 1, not 0)."""
+    vox = np.arange(8 * 8 * 8, dtype=np.float32)
+    blocks = np.array([[0, 0, 0, 8, 8, 8, 8, 8, 8]], np.int32)
+    pos = np.array([[3, 5, 6], [7.9, 0.1, 2.5], [-1, 0, 0], [0, 0, 8.5]], np.float64)
+    out = orc.sparse_sample(blocks, [vox], 1, pos=pos)
+    assert out[0, 0] == 6 * 64 + 5 * 8 + 3 and out[0, 3] == 1.0
+    assert out[1, 0] == 2 * 64 + 0 * 8 + 7
+    assert np.all(out[2] == 0) and np.all(out[3] == 0)
+    vox = np.arange(8 * 2 * 2, dtype=np.float32)
+    blocks = np.array([[0, 0, 0, 8, 8, 8, 8, 2, 2]], np.int32)
+    out = orc.sparse_sample(blocks, [vox], 1, pos=np.array([[0.5, 1.5, 0.5]]))
+    assert out[0, 0] == 1 * 8 + 0  # y = 1.5 * (8 / 8) -> cell 1
