@@ -308,18 +308,25 @@ __device__ __forceinline__ bool sphere_isect(const RaySetup &s, float4 c, float 
 // when no sphere inside can report a hit (for any t); else `lo` is at most
 // the distance any sphere inside can report. The box holds the spheres'
 // extents. SphereIsect's float arithmetic moves its roots away from the exact
-// ones: with u = 2^-24, its discriminant carries an error of at most about
-// 90 u |d|^2 (|o - c|^2 + r^2), so a reported root lies no nearer than the
-// entry into the ball of radius r + 1.2e-3 sqrt(|o - c|^2 + r^2) around the
-// centre (plus terms of order u |o - c|). The box is grown by 2e-3 of the
-// farthest distance from the origin to it, which bounds that for every
-// sphere inside. A ray whose origin is inside the grown box gets lo < 0.
+// ones. With u = 2^-24, s = |o - c| and A = |d|^2: oc = fl(o - c) is exact
+// for a centre c' within u s of c; against that centre the dot product is
+// within 3u |d| s, |oc|^2 and a within 3u relative, r^2 and the subtraction
+// within u, b * b within u, the double 4 a cc exact, so the computed
+// discriminant differs from the exact one of (c', r) by at most
+// 56 u A s^2 + 4 u A r^2. A root of it is the entry into the ball around c'
+// of radius sqrt(r^2 (1 + u) + 14 u s^2) <= r + 9.2e-4 s, and the remaining
+// roundings (sqrt, the root formula, the divisions) move it by O(10 u s)
+// along the ray. So a reported root lies no nearer than the entry into the
+// ball of radius r + 1.0e-3 s around c. The box is grown by 1.25e-3 of the
+// farthest distance from the origin to it (>= s + r for every sphere
+// inside), which bounds that with a margin of 25 %. A ray whose origin is
+// inside the grown box gets lo < 0.
 __device__ __forceinline__ bool sub_box(const RaySetup &s, float x0, float y0, float z0,
                                         float x1, float y1, float z1, float &lo) {
   const float fx = fmaxf(fabsf(s.o[0] - x0), fabsf(s.o[0] - x1));
   const float fy = fmaxf(fabsf(s.o[1] - y0), fabsf(s.o[1] - y1));
   const float fz = fmaxf(fabsf(s.o[2] - z0), fabsf(s.o[2] - z1));
-  const float pad = 2e-3f * sqrtf((fx * fx + fy * fy) + fz * fz);
+  const float pad = 1.25e-3f * sqrtf((fx * fx + fy * fy) + fz * fz);
   const float b0[3] = {x0 - pad, y0 - pad, z0 - pad}, b1[3] = {x1 + pad, y1 + pad, z1 + pad};
   float tn = -FLT_MAX, tf = FLT_MAX;
 #pragma unroll
@@ -598,16 +605,23 @@ __device__ __forceinline__ bool sph_traverse(const float4 *__restrict__ nodes,
   return false;
 }
 
-template <bool kAny>
+// Closest hit (and plain occlusion) of spheres. kPart selects the
+// wavefronts a launch handles: 0 all, 1 those whose lanes share direction
+// signs (the wavefront traversal; no per-lane stack, so the launch holds
+// more wavefronts per CU), 2 the others (the per-lane traversal). Part 1
+// followed by part 2 on the same rays gives every ray once.
+template <bool kAny, int kPart>
 __global__ void __launch_bounds__(kTraceBlock)
 k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             const uint32_t *__restrict__ sorted_index, const srf_ray *__restrict__ rays,
             const srf_sphere_prev *__restrict__ prevs, uint32_t n, srf_hit *__restrict__ hits,
             const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded,
             uint32_t max_nodes, uint32_t nprims, SphSub sub) {
-  __shared__ uint32_t stack[2 * kTraceStack * kTraceBlock];
-  __shared__ uint32_t wnode[kAny ? 1 : kTraceBlock / 64][2 * kTraceStack];
-  __shared__ uint64_t wmask[kAny ? 1 : kTraceBlock / 64][2 * kTraceStack];
+  constexpr bool kWavePath = !kAny && kPart != 2;
+  constexpr bool kLanePath = kPart != 1;
+  __shared__ uint32_t stack[kLanePath ? 2 * kTraceStack * kTraceBlock : 1];
+  __shared__ uint32_t wnode[kWavePath ? kTraceBlock / 64 : 1][2 * kTraceStack];
+  __shared__ uint64_t wmask[kWavePath ? kTraceBlock / 64 : 1][2 * kTraceStack];
   const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -623,7 +637,10 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   uint32_t best;
   bool occ = false;
   const uint32_t sb = __builtin_amdgcn_readfirstlane(s.sbits);
-  if (!kAny && __all(s.sbits == sb)) {
+  const bool uniform = __all(s.sbits == sb);
+  if (kPart == 1 && !uniform) return;  // left to part 2
+  if (kPart == 2 && !kAny && uniform) return;  // done by part 1
+  if (kWavePath && uniform) {
     // Closest hit once per wavefront, as k_trace_tri_wave: with the lanes'
     // direction signs equal, the reference's order (near child by the split
     // axis's sign, far child pushed, no re-test at the pop) is one static
@@ -706,7 +723,7 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
       cur = __builtin_amdgcn_readfirstlane(wnode[wv][wsp]);
       mask = wmask[wv][wsp];
     }
-  } else {
+  } else if constexpr (kLanePath) {
     occ = sph_traverse<kAny, kTraceBlock>(nodes, prims, sorted_index, max_nodes, nprims, sub, s,
                                           prev, prevT, prevN, tlimit, stack + threadIdx.x, t,
                                           best);
@@ -2374,16 +2391,21 @@ k_trace_lin(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
 // the flag goes back to its own slot). With `counter` the grid is
 // persistent: each wavefront takes 64 consecutive positions of the order at
 // a time until none is left.
-template <int K>
+// For spheres kPart selects the 64-ray groups a launch handles: 0 all, 1
+// those whose rays share direction signs (the wavefront traversal, without a
+// per-lane stack, so more wavefronts fit per CU), 2 the others, one group per
+// wavefront of a plain grid. Part 1 followed by part 2 gives every ray once.
+template <int K, int kPart>
 __global__ void __launch_bounds__(kTraceBlock)
 k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
               const uint32_t *__restrict__ sorted_index, const BuildParams *__restrict__ params,
               const srf_ray *__restrict__ rays, const float *__restrict__ tmax_in,
               const uint32_t *__restrict__ order, uint32_t n, uint32_t *__restrict__ occluded,
               uint32_t max_nodes, uint32_t nprims, uint32_t *counter, SphSub sub) {
-  __shared__ uint32_t stack[(K == kSph ? 2 : 1) * kTraceStack * kTraceBlock];
-  __shared__ uint32_t wnode[K == kSph ? kTraceBlock / 64 : 1][2 * kTraceStack];
-  __shared__ uint64_t wmask[K == kSph ? kTraceBlock / 64 : 1][2 * kTraceStack];
+  constexpr bool kLanePath = kPart != 1;
+  __shared__ uint32_t stack[kLanePath ? (K == kSph ? 2 : 1) * kTraceStack * kTraceBlock : 1];
+  __shared__ uint32_t wnode[K == kSph && kPart != 2 ? kTraceBlock / 64 : 1][2 * kTraceStack];
+  __shared__ uint64_t wmask[K == kSph && kPart != 2 ? kTraceBlock / 64 : 1][2 * kTraceStack];
   uint32_t *st = stack + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t base = (counter ? blockIdx.x : xcd_block()) * kTraceBlock + (threadIdx.x & ~63u);
@@ -2400,23 +2422,32 @@ k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims
       uint32_t flags, prev;
       const RaySetup s = setup_ray(rays, i, flags, prev);
       const float tlimit = tmax_in ? tmax_in[i] : FLT_MAX;
-      bool occ;
-      if (K == kTri) {
-        const float kEPS = FLT_EPSILON * (float)params->eps_scale;
-        float t, u, v;
-        uint32_t prim;
-        occ = tri_traverse<true>(nodes, prims, max_nodes, nprims, s,
-                                 (flags & SRF_RAY_DOUBLE_SIDED) != 0, kEPS, prev,
-                                 tlimit, Stack32<kTraceBlock>{st}, t, u, v, prim);
-      } else if (const uint32_t sb = __builtin_amdgcn_readfirstlane(s.sbits);
-                 !__all(s.sbits == sb)) {
+      bool occ = false;
+      const uint32_t sb = __builtin_amdgcn_readfirstlane(s.sbits);
+      const bool uniform = K == kSph && __all(s.sbits == sb);
+      bool skip = false;
+      if (kPart == 1 && !uniform) skip = true;  // left to part 2
+      if (kPart == 2 && uniform) skip = true;   // done by part 1
+      if (skip) {
+      } else if (K == kTri) {
+        if constexpr (kLanePath) {
+          const float kEPS = FLT_EPSILON * (float)params->eps_scale;
+          float t, u, v;
+          uint32_t prim;
+          occ = tri_traverse<true>(nodes, prims, max_nodes, nprims, s,
+                                   (flags & SRF_RAY_DOUBLE_SIDED) != 0, kEPS, prev,
+                                   tlimit, Stack32<kTraceBlock>{st}, t, u, v, prim);
+        }
+      } else if (!uniform) {
         // directions of mixed signs (rays not sorted): one traversal per lane
-        const float zero[3] = {0.0f, 0.0f, 0.0f};
-        float t;
-        uint32_t best;
-        occ = sph_traverse<true, kTraceBlock>(nodes, prims, sorted_index, max_nodes, nprims,
-                                              sub, s, prev, 0.0, zero, tlimit, st, t, best);
-      } else {
+        if constexpr (kLanePath) {
+          const float zero[3] = {0.0f, 0.0f, 0.0f};
+          float t;
+          uint32_t best;
+          occ = sph_traverse<true, kTraceBlock>(nodes, prims, sorted_index, max_nodes, nprims,
+                                                sub, s, prev, 0.0, zero, tlimit, st, t, best);
+        }
+      } else if constexpr (kPart != 2) {
         // Once per wavefront: with the search distance fixed at tlimit, the
         // set of nodes and spheres a lane's traversal tests does not depend
         // on their order, and neither does whether one of them occludes, so
@@ -2495,7 +2526,7 @@ k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims
           mask = wmask[wv][wsp];
         }
       }
-      occluded[i] = occ ? 1u : 0u;
+      if (!skip) occluded[i] = occ ? 1u : 0u;
     }
     if (!counter) break;
   }
@@ -2712,7 +2743,14 @@ srf_status trace_sol(const Accel *a, const srf_ray *rays, uint64_t n, const floa
 srf_status trace_sph(const Accel *a, const srf_ray *rays, const srf_sphere_prev *prev,
                      uint64_t n, srf_hit *hits, hipStream_t s) {
   if (n == 0) return SRF_OK;
-  hipLaunchKernelGGL(k_trace_sph<false>, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
+  // the wavefront traversal for wavefronts of equal direction signs, then
+  // the per-lane one for the rest
+  hipLaunchKernelGGL((k_trace_sph<false, 1>), dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
+                     (const float4 *)a->nodes, (const float4 *)a->prims, a->sorted_index, rays,
+                     prev, (uint32_t)n, hits, nullptr, nullptr, max_nodes(a), (uint32_t)a->n,
+                     sph_sub(a));
+  SRF_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_trace_sph<false, 2>), dim3(trace_blocks(n)), dim3(kTraceBlock), 0, s,
                      (const float4 *)a->nodes, (const float4 *)a->prims, a->sorted_index, rays,
                      prev, (uint32_t)n, hits, nullptr, nullptr, max_nodes(a), (uint32_t)a->n,
                      sph_sub(a));
@@ -2753,7 +2791,7 @@ srf_status trace_any(const Accel *a, const srf_ray *rays, const float *tmax, uin
                          rays, (uint32_t)n, nullptr, nullptr, nullptr, tmax, occluded,
                          max_nodes(a), (uint32_t)a->n);
   } else
-    hipLaunchKernelGGL(k_trace_sph<true>, g, b, 0, s, (const float4 *)a->nodes,
+    hipLaunchKernelGGL((k_trace_sph<true, 0>), g, b, 0, s, (const float4 *)a->nodes,
                        (const float4 *)a->prims, a->sorted_index, rays, nullptr, (uint32_t)n,
                        nullptr, tmax, occluded, max_nodes(a), (uint32_t)a->n, sph_sub(a));
   SRF_HIP(hipGetLastError());
@@ -2806,16 +2844,24 @@ srf_status trace_any_sorted_in(char *base, size_t size, const Accel *a, const sr
     blocks = std::min<uint32_t>(blocks, (uint32_t)cus * 16u);
     hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, s, counter);
   }
-  if (a->kind == kTri)
-    hipLaunchKernelGGL(k_any_ordered<kTri>, dim3(blocks), dim3(kTraceBlock), 0, s,
+  if (a->kind == kTri) {
+    hipLaunchKernelGGL((k_any_ordered<kTri, 0>), dim3(blocks), dim3(kTraceBlock), 0, s,
                        (const float4 *)a->nodes, (const float4 *)a->prims, a->sorted_index,
                        a->params, rays, tmax, order, nn, occluded, max_nodes(a), (uint32_t)a->n,
                        counter, SphSub{});
-  else
-    hipLaunchKernelGGL(k_any_ordered<kSph>, dim3(blocks), dim3(kTraceBlock), 0, s,
+  } else {
+    // groups of equal direction signs by the wavefront traversal, then the
+    // others per lane (one group per wavefront of a plain grid)
+    hipLaunchKernelGGL((k_any_ordered<kSph, 1>), dim3(blocks), dim3(kTraceBlock), 0, s,
                        (const float4 *)a->nodes, (const float4 *)a->prims, a->sorted_index,
                        a->params, rays, tmax, order, nn, occluded, max_nodes(a), (uint32_t)a->n,
                        counter, sph_sub(a));
+    SRF_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_any_ordered<kSph, 2>), dim3(trace_blocks(nn)), dim3(kTraceBlock), 0, s,
+                       (const float4 *)a->nodes, (const float4 *)a->prims, a->sorted_index,
+                       a->params, rays, tmax, order, nn, occluded, max_nodes(a), (uint32_t)a->n,
+                       nullptr, sph_sub(a));
+  }
   SRF_HIP(hipGetLastError());
   return SRF_OK;
 }
@@ -2852,4 +2898,3 @@ srf_status ao_rays_spheres(const srf_ray *primary, const srf_hit *hits, uint64_t
 }
 
 }  // namespace srf
-

@@ -1,10 +1,10 @@
 """The culling bound of the secondary sphere trees, checked on the CPU.
 
 A big sphere leaf is searched through a secondary tree whose boxes are grown
-by 2e-3 of the farthest origin-to-box distance (sub_box in
+by 1.25e-3 of the farthest origin-to-box distance (sub_box in
 surface_amd/csrc/srf_trace.hip). That is exact only if every t the float
 sphere test reports lies no nearer than the entry into the ball of radius
-r + 2e-3 D around the centre (D >= |o - c| + r, a lower bound of that farthest
+r + 1.25e-3 D around the centre (D >= |o - c| + r, a lower bound of that farthest
 distance). Here SphereIsect (render/render_accel_particle.cc:2423-2512) is
 restated in numpy float32 / float64 with the kernel's evaluation order, and the
 smallest padding each reported hit needs is measured over adversarial rays:
@@ -16,7 +16,10 @@ from __future__ import annotations
 import numpy as np
 
 f32, f64 = np.float32, np.float64
-PAD = 2e-3
+PAD = 1.25e-3
+# the rounding-error bound of the sphere test (sub_box's comment): a reported
+# root is never nearer than the entry into the ball of radius r + 9.2e-4 |o - c|
+BOUND = 9.2e-4
 
 
 def sphere_test(o, d, c, r):
@@ -88,7 +91,10 @@ def test_reported_hits_lie_inside_the_padded_ball():
         hits += int(keep.sum())
         worst = max(worst, float(k.max()))
     assert hits > 500_000
-    assert worst < PAD / 1.5, worst
+    # the measured worst case stays inside the analytical bound, which the
+    # kernel's padding exceeds
+    assert worst < BOUND, worst
+    assert BOUND * 1.25 < PAD * 1.01
 
 
 def test_restatement_reports_textbook_roots():
