@@ -444,6 +444,21 @@ srf_status srf_trace_any_sorted(srf_ctx *ctx, const srf_accel *accel, const srf_
                                 const float *t_max, uint64_t n, uint32_t key_bits, int mode,
                                 uint32_t *occluded, void *stream);
 
+/* srf_trace_any_sorted without allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It takes no
+ * context: the keys, order, work counter and sort slices live in a static
+ * device array of 26 MiB that each GPU the library runs on holds until the
+ * process ends, enough for n <= 1048576 rays (SRF_NOALLOC_MAX_RAYS); larger n
+ * is rejected with SRF_ERR_INVALID_ARG. Nothing is allocated or freed. Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller. The
+ * flags equal srf_trace_any_sorted's and srf_trace_any's. */
+#define SRF_NOALLOC_MAX_RAYS 1048576u
+srf_status srf_trace_any_sorted_noalloc(const srf_accel *accel, const srf_ray *rays,
+                                        const float *t_max, uint64_t n, uint32_t key_bits,
+                                        int mode, uint32_t *occluded, void *stream);
+
 typedef struct {
   uint32_t key_bits;   /* 0 = no sort */
   uint32_t persistent; /* 0 or 1 */

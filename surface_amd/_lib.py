@@ -134,6 +134,7 @@ SIGNATURES = {
     "srf_scene_trace": (_i, [_vp, _vp, _u64, _vp, _vp, _i, _vp]),
     "srf_scene_autotune": (_i, [_vp, _vp, _u64, _vp, _vp, _vp, _vp]),
     "srf_trace_any_sorted": (_i, [_vp, _vp, _vp, _vp, _u64, C.c_uint32, _i, _vp, _vp]),
+    "srf_trace_any_sorted_noalloc": (_i, [_vp, _vp, _vp, _u64, C.c_uint32, _i, _vp, _vp]),
     "srf_autotune_any": (_i, [_vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp]),
     "srf_ao_rays_spheres": (_i, [_vp, _vp, _u64, _vp, _vp, C.c_float, C.c_uint32, C.c_float,
                                  C.c_float, C.c_uint64, _vp, _vp, _vp, _vp, _vp]),

@@ -229,6 +229,21 @@ class _Accel:
                                             n, int(key_bits), m, _ptr(out), _stream(stream)))
         return out
 
+    def TraverseAnySortedNoAlloc(self, rays, t_max=None, key_bits=0, mode="plain", out=None,
+                                 stream=None):
+        """TraverseAnySorted through srf_trace_any_sorted_noalloc: the scratch
+        is the library's static per-GPU array (at most SRF_NOALLOC_MAX_RAYS
+        rays); out must be given by callers that cannot allocate."""
+        self._require()
+        n = rays.shape[0]
+        if out is None:
+            out = torch.empty(n, dtype=torch.int32, device=rays.device)
+        m = {"plain": _lib.LAUNCH_PLAIN, "persistent": _lib.LAUNCH_PERSISTENT}[mode]
+        check(self.lib.srf_trace_any_sorted_noalloc(self.handle, _ptr(rays), _ptr(t_max), n,
+                                                    int(key_bits), m, _ptr(out),
+                                                    _stream(stream)))
+        return out
+
     def autotune_any(self, rays, t_max=None, out=None, stream=None):
         """Fastest (key_bits, persistent) for TraverseAnySorted on these rays."""
         self._require()

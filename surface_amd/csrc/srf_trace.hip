@@ -31,6 +31,7 @@
 //   solids     SolidAccel::Traverse     render/render_accel_solid.cc:2345-2435
 //   lines      LineAccel::Traverse      render/render_accel_line.cc:704-774
 #include <algorithm>
+#include <mutex>
 
 #include "srf_trace_dev.h"
 
@@ -2898,3 +2899,75 @@ srf_status ao_rays_spheres(const srf_ray *primary, const srf_hit *hits, uint64_t
 }
 
 }  // namespace srf
+
+// ---------------------------------------------------------------------------
+// Sorted occlusion that allocates nothing (srf_trace_any_sorted_noalloc): the
+// keys, the order, the work counter and the sort's slices live in `scratch`,
+// a static device array each GPU holds for the life of the process. Host code
+// never passes the array's name to a kernel: it looks up the array's device
+// address once per device (hipGetSymbolAddress, with that device current)
+// and hands that pointer on. Every call on one GPU shares the array, so calls
+// that overlap on one GPU (on different streams, or from different threads)
+// race; the counter and the sort's histograms and tile flags are cleared by a
+// kernel at the start of each call.
+namespace srf {
+namespace {
+
+constexpr uint32_t kNoAllocMaxRays = 1u << 20;
+constexpr size_t kNoAllocScratchBytes = (size_t)26 << 20;  // any_sorted_bytes(2^20) + padding
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocMaxDevices];
+
+// The device address of this device's copy of `scratch` (dev is current).
+srf_status noalloc_scratch(int dev, char **out) {
+  if (dev < 0 || dev >= kNoAllocMaxDevices)
+    return set_error(SRF_ERR_UNSUPPORTED, "device %d beyond the %d the library tracks", dev,
+                     kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_error(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<char *>(g_scratch_ptr[dev]);
+  return SRF_OK;
+}
+
+}  // namespace
+}  // namespace srf
+
+extern "C" srf_status srf_trace_any_sorted_noalloc(const srf_accel *accel, const srf_ray *rays,
+                                                   const float *t_max, uint64_t n,
+                                                   uint32_t key_bits, int mode,
+                                                   uint32_t *occluded, void *stream) {
+  using namespace srf;
+  const Accel *a = reinterpret_cast<const Accel *>(accel);
+  if (!a || (n && (!rays || !occluded))) return set_error(SRF_ERR_INVALID_ARG, "null argument");
+  if (n > kNoAllocMaxRays)
+    return set_error(SRF_ERR_INVALID_ARG, "%llu rays: at most %u per call without allocation",
+                     (unsigned long long)n, kNoAllocMaxRays);
+  if (mode != SRF_LAUNCH_PLAIN && mode != SRF_LAUNCH_PERSISTENT)
+    return set_error(SRF_ERR_INVALID_ARG, "mode must be SRF_LAUNCH_PLAIN or _PERSISTENT");
+  srf_status st = check_any_sorted(a, key_bits);
+  if (st != SRF_OK) return st;
+  if (n == 0) return SRF_OK;
+  const uint32_t nn = (uint32_t)n;
+  if (any_sorted_bytes(nn) + 8 * 256 > kNoAllocScratchBytes)
+    return set_error(SRF_ERR_INTERNAL, "no-allocation scratch too small for %u rays", nn);
+  int prev = -1;
+  if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+  if (prev != a->device && hipSetDevice(a->device) != hipSuccess)
+    return set_error(SRF_ERR_HIP, "hipSetDevice(%d) failed", a->device);
+  char *base = nullptr;
+  st = noalloc_scratch(a->device, &base);
+  if (st == SRF_OK)
+    st = trace_any_sorted_in(base, kNoAllocScratchBytes, a, rays, t_max, nn, key_bits, mode,
+                             occluded, reinterpret_cast<hipStream_t>(stream));
+  if (prev >= 0 && prev != a->device) (void)hipSetDevice(prev);
+  return st;
+}
