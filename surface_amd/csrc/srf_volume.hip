@@ -16,9 +16,14 @@
 // state. The forward pass can leave each ray's transmittance at the start of
 // every 16-sample segment and after its last sample (without them the
 // adjoint first re-marches the ray once to make them). Each segment is then
-// walked from its end to its start recovering T_k = T_{k+1} / (1 - a_k); a
-// sample with 1 - a_k < 1/4 (where that division would amplify rounding)
-// gets T_k by re-marching forward from the segment's checkpoint instead.
+// walked from its end to its start recovering T_k = T_{k+1} / (1 - a_k). The
+// replay's a_k is the forward's bit for bit (same samples, same arithmetic),
+// so the forward product T_{k+1} = fl(T_k fl(1 - a_k)) is undone with one
+// rounding of the product and one of the reciprocal per sample: at most
+// about 3 ulp per sample, 45 ulp over a 16-sample segment, whatever a_k is.
+// Only an opaque sample (1 - a_k = 0) or a T_{k+1} below the normal range
+// loses T_k; such a sample gets it by re-marching forward from the segment's
+// checkpoint instead.
 // With T_k in hand the division-free recursions give the gradients:
 //   R_{k-1} = a_k c_k + (1 - a_k) R_k,  P_{k-1} = (1 - a_k) P_k,
 //   dL/dc_k = g_C T_k a_k,  dL/da_k = T_k (g_C . (c_k - R_k) + g_A P_k).
@@ -54,7 +59,8 @@ constexpr int kVolBlock = 128;
 constexpr int kVolBwdBlock = 256;
 constexpr int kMaxTf = 1024;  // entries; forward keeps the table in LDS
 constexpr int kFwdLdsEntries = 1024;  // forward LDS reserved per block (float4 entries)
-constexpr int kSegLen = 16;     // samples per checkpointed segment
+constexpr int kSegShift = 4;    // log2 of the samples per checkpointed segment
+constexpr int kSegLen = 1 << kSegShift;
 constexpr int kMaxSegNoCkpt = 8;  // LDS checkpoints per ray when recomputing
 constexpr int kMaxSamples = 65536;
 // The adjoint's TF-gradient table is kept in this many copies, lane l adding
@@ -136,25 +142,47 @@ __device__ __forceinline__ float tex_eval(const float *__restrict__ vox, const G
   return lerpf(g.dz, d0, d1);
 }
 
+// A cell's 8 voxels as four x-pairs: each pair is one 8-byte load (two
+// adjacent voxels of a row), so a sample issues four gathers instead of
+// eight. At the far x border (vx = w - 1, where the texture clamps vx1 to
+// vx) the pair starting at w - 2 is loaded and its second voxel used twice,
+// so no load leaves the row. The values equal tex_eval's loads.
+typedef float f2a4 __attribute__((ext_vector_type(2), aligned(4)));
+
+__device__ __forceinline__ void gather8(const float *__restrict__ vox, int w, const Geom &g,
+                                        float v[8]) {
+  const int bx = g.cx < w - 2 ? g.cx : w - 2;
+  const bool lo_hi = g.cx != bx;   // vx is the pair's second voxel
+  const bool hi_hi = g.cx1 != bx;  // vx1 is the pair's second voxel
+  for (int j = 0; j < 4; j++) {
+    const f2a4 p = *reinterpret_cast<const f2a4 *>(vox + (g.a[2 * j] - (uint32_t)g.cx + bx));
+    v[2 * j] = lo_hi ? p.y : p.x;
+    v[2 * j + 1] = hi_hi ? p.y : p.x;
+  }
+}
+
 // The 8 voxels of the last cell a lane sampled: consecutive samples of a ray
-// (half-voxel steps) share a cell about every other step, so the adjoint's
-// replay gathers only when the cell changes. The value is tex_eval's, from
-// the same voxels in the same order.
+// (half-voxel steps) share a cell about every other step, and only lanes
+// whose cell changed gather (the gathers' cost goes with the cache lines
+// the active lanes touch).
 struct CellCache {
   uint32_t a0;  // the cell's first corner (the others follow from it)
   float d[8];
 };
 
-__device__ __forceinline__ float tex_eval_cached(const float *__restrict__ vox, const Geom &g,
-                                                 CellCache &c) {
+__device__ __forceinline__ void cell_fetch(const float *__restrict__ vox, int w, const Geom &g,
+                                           CellCache &c) {
   if (g.a[0] != c.a0) {
-    for (int q = 0; q < 8; q++) c.d[q] = vox[g.a[q]];
+    gather8(vox, w, g, c.d);
     c.a0 = g.a[0];
   }
-  float d00 = lerpf(g.dx, c.d[0], c.d[1]);
-  float d10 = lerpf(g.dx, c.d[2], c.d[3]);
-  float d01 = lerpf(g.dx, c.d[4], c.d[5]);
-  float d11 = lerpf(g.dx, c.d[6], c.d[7]);
+}
+
+__device__ __forceinline__ float trilerp(const Geom &g, const float v[8]) {
+  float d00 = lerpf(g.dx, v[0], v[1]);
+  float d10 = lerpf(g.dx, v[2], v[3]);
+  float d01 = lerpf(g.dx, v[4], v[5]);
+  float d11 = lerpf(g.dx, v[6], v[7]);
   float d0 = lerpf(g.dy, d00, d10);
   float d1 = lerpf(g.dy, d01, d11);
   return lerpf(g.dz, d0, d1);
@@ -268,19 +296,35 @@ __global__ void __launch_bounds__(256) k_vol_sample(Volume V, const float *__res
   out[i] = tex_eval(V.vox, g);
 }
 
-// Segment length for a ray of ns samples so that its checkpoints (segment
-// starts plus the final transmittance) fit in `slots`: 16, doubled as needed
-// (only rays longer than the unit-direction bound ever need that).
-__device__ __forceinline__ int seg_len_for(int ns, uint32_t slots) {
-  int s = kSegLen;
-  while ((uint32_t)((ns + s - 1) / s + 1) > slots) s *= 2;
-  return s;
+// log2 of the segment length for a ray of ns samples so that its
+// checkpoints (segment starts plus the final transmittance) fit in `slots`:
+// 16 samples, doubled as needed (only rays longer than the unit-direction
+// bound ever need that). Segments are powers of two so that a sample's
+// segment is a shift and its place in it a mask.
+__device__ __forceinline__ int seg_shift_for(int ns, uint32_t slots) {
+  int sh = kSegShift;
+  while ((uint32_t)(((ns + (1 << sh) - 1) >> sh) + 1) > slots) sh++;
+  return sh;
 }
 
+// The number of samples k with t_k = tn + (k + 0.5) step < tfar (at most
+// kMaxSamples). t_k never decreases with k (k + 0.5 is exact, and a product
+// and a sum with a fixed operand round monotonically), so the samples are a
+// prefix 0..ns-1: start from the quotient and step to the first k whose t_k
+// fails the test, as the march itself evaluates it.
 __device__ __forceinline__ int count_samples(float tn, float tfar, float step) {
-  int ns = 0;
+  const float e = (tfar - tn) / step;
+  int ns = e > 0.0f ? (int)fminf(e, (float)kMaxSamples) : 0;
+  while (ns > 0 && !((tn + ((float)(ns - 1) + 0.5f) * step) < tfar)) ns--;
   while (ns < kMaxSamples && (tn + ((float)ns + 0.5f) * step) < tfar) ns++;
   return ns;
+}
+
+// Geometry of sample k of the ray (o, d) entering at tn.
+__device__ __forceinline__ void sample_geom(const Volume &V, const float o[3], const float d[3],
+                                            float tn, float step, int k, Geom &g) {
+  const float t = tn + ((float)k + 0.5f) * step;
+  tex_geom(V, o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2], g);
 }
 
 __global__ void __launch_bounds__(kVolBlock) k_vol_fwd(Volume V, const float *__restrict__ tf,
@@ -298,33 +342,34 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_fwd(Volume V, const float *__
   load_ray(rays, i, o, d);
   float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, T = 1.0f;
   float *ck = ckpt ? ckpt + (size_t)i * slots : nullptr;
-  float tn, tfar;
-  int k = 0, seg = kSegLen;
-  if (vol_span(o, d, tn, tfar)) {
-    if (ck) seg = seg_len_for(count_samples(tn, tfar, step), slots);
-    for (; k < kMaxSamples; k++) {
-      float t = tn + ((float)k + 0.5f) * step;
-      if (!(t < tfar)) break;
-      if (ck && (k % seg) == 0) ck[k / seg] = T;
-      Geom g;
-      tex_geom(V, o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2], g);
-      float s = tex_eval(V.vox, g);
-      int i0;
-      float f;
-      bool inside;
-      tf_index(s, N, i0, f, inside);
-      float4 a = s_tf[i0], b = s_tf[i0 + 1];
-      float cr = lerpf(f, a.x, b.x), cg = lerpf(f, a.y, b.y), cb = lerpf(f, a.z, b.z);
-      float ca = lerpf(f, a.w, b.w);
-      float w = T * ca;
-      C0 = C0 + w * cr;
-      C1 = C1 + w * cg;
-      C2 = C2 + w * cb;
-      T = T * (1.0f - ca);
-    }
+  float tn = 0.0f, tfar = 0.0f;
+  int ns = 0;
+  if (vol_span(o, d, tn, tfar)) ns = count_samples(tn, tfar, step);
+  const int sh = ck ? seg_shift_for(ns, slots) : kSegShift;
+  const int mask = (1 << sh) - 1;
+  CellCache vc;
+  vc.a0 = 0xffffffffu;
+  for (int k = 0; k < ns; k++) {
+    if (ck && (k & mask) == 0) ck[k >> sh] = T;
+    Geom g;
+    sample_geom(V, o, d, tn, step, k, g);
+    cell_fetch(V.vox, V.dims[0], g, vc);
+    const float s = trilerp(g, vc.d);
+    int i0;
+    float f;
+    bool inside;
+    tf_index(s, N, i0, f, inside);
+    float4 a = s_tf[i0], b = s_tf[i0 + 1];
+    float cr = lerpf(f, a.x, b.x), cg = lerpf(f, a.y, b.y), cb = lerpf(f, a.z, b.z);
+    float ca = lerpf(f, a.w, b.w);
+    float w = T * ca;
+    C0 = C0 + w * cr;
+    C1 = C1 + w * cg;
+    C2 = C2 + w * cb;
+    T = T * (1.0f - ca);
   }
   // transmittance after the last sample closes the last segment
-  if (ck) ck[(k + seg - 1) / seg] = T;
+  if (ck) ck[(ns + (1 << sh) - 1) >> sh] = T;
   reinterpret_cast<float4 *>(rgba)[i] = make_float4(C0, C1, C2, 1.0f - T);
 }
 
@@ -540,6 +585,135 @@ __device__ __forceinline__ void cell_move(CellAcc &a, int nx0, int nx1, int ny0,
   a.live = true;
 }
 
+// Per-ray constants of the adjoint's replay.
+struct BwdRay {
+  float o[3], d[3], tn;
+  int ns, sh, mask, ck_stride;
+  const float *ck;
+  float4 gr;
+};
+
+// Per-lane running state of the replay: the transmittance checkpoints in
+// use, the suffix sums R (colour behind the sample) and P (transmittance
+// behind it), and the TF and voxel gradient accumulators.
+struct BwdLane {
+  float Tnext, Tstart, Tpre, R0, R1, R2, P;
+  TfAcc tacc;
+  CellAcc cacc;
+};
+
+// Sample k of the adjoint's replay (a lane idles while k >= ns).
+// (d0, d1, d2) is the wavefront's direction.
+template <int W>
+__device__ __forceinline__ void bwd_sample(const Volume &V, const float4 *s_tf, int N,
+                                           float step, int k, const BwdRay &R, BwdLane &L,
+                                           CellCache &vc, float *my_gtf, VoxWindow<W> &win,
+                                           float *grad_vox, int lane, float d0, float d1,
+                                           float d2) {
+  const bool act = k < R.ns;
+  if (act && k < R.ns - 1 && (k & R.mask) == R.mask) {  // top sample of a lower segment
+    L.Tstart = L.Tpre;
+    const int sg = k >> R.sh;
+    if (sg > 0) L.Tpre = R.ck[(sg - 1) * R.ck_stride];
+  }
+  float ds = 0.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  int cx = 0, cy = 0, cz = 0, cx1 = 0, cy1 = 0, cz1 = 0;
+  if (act) {
+    Geom g;
+    sample_geom(V, R.o, R.d, R.tn, step, k, g);
+    cell_fetch(V.vox, V.dims[0], g, vc);
+    const float s = trilerp(g, vc.d);
+    int i0;
+    float f;
+    bool inside;
+    tf_index(s, N, i0, f, inside);
+    const float4 ta = s_tf[i0], tb = s_tf[i0 + 1];
+    const float c0r = lerpf(f, ta.x, tb.x), c1g = lerpf(f, ta.y, tb.y),
+                c2b = lerpf(f, ta.z, tb.z);
+    const float a = lerpf(f, ta.w, tb.w);
+    // transmittance in front of sample k
+    float Tk;
+    if ((k & R.mask) == 0) {
+      Tk = L.Tstart;
+    } else if (1.0f - a > 0.0f && L.Tnext >= FLT_MIN) {
+      Tk = L.Tnext * __builtin_amdgcn_rcpf(1.0f - a);
+    } else {
+      Tk = L.Tstart;
+      for (int j = k & ~R.mask; j < k; j++)
+        Tk = Tk * (1.0f - sample_alpha(V, s_tf, N, R.o, R.d, R.tn + ((float)j + 0.5f) * step));
+    }
+    L.Tnext = Tk;
+    const float wa = Tk * a;
+    const float g0 = R.gr.x * wa, g1 = R.gr.y * wa, g2 = R.gr.z * wa;
+    const float ga =
+        Tk * __builtin_fmaf(R.gr.x, c0r - L.R0,
+                            __builtin_fmaf(R.gr.y, c1g - L.R1,
+                                           __builtin_fmaf(R.gr.z, c2b - L.R2, R.gr.w * L.P)));
+    const float fo = 1.0f - f;
+    tf_move(L.tacc, i0, my_gtf);
+    L.tacc.g[0] = __builtin_fmaf(fo, g0, L.tacc.g[0]);
+    L.tacc.g[1] = __builtin_fmaf(fo, g1, L.tacc.g[1]);
+    L.tacc.g[2] = __builtin_fmaf(fo, g2, L.tacc.g[2]);
+    L.tacc.g[3] = __builtin_fmaf(fo, ga, L.tacc.g[3]);
+    L.tacc.g[4] = __builtin_fmaf(f, g0, L.tacc.g[4]);
+    L.tacc.g[5] = __builtin_fmaf(f, g1, L.tacc.g[5]);
+    L.tacc.g[6] = __builtin_fmaf(f, g2, L.tacc.g[6]);
+    L.tacc.g[7] = __builtin_fmaf(f, ga, L.tacc.g[7]);
+    if (inside)
+      ds = __builtin_fmaf(g0, tb.x - ta.x,
+                          __builtin_fmaf(g1, tb.y - ta.y,
+                                         __builtin_fmaf(g2, tb.z - ta.z, ga * (tb.w - ta.w)))) *
+           (float)(N - 1);
+    // the 8 corners: (cx|cx1, cy|cy1, cz|cz1), trilinear weights
+    cx = g.cx;
+    cy = g.cy;
+    cz = g.cz;
+    cx1 = g.cx1;
+    cy1 = g.cy1;
+    cz1 = g.cz1;
+    fx = g.dx;
+    fy = g.dy;
+    fz = g.dz;
+    ux = 1.0f - g.dx;
+    uy = 1.0f - g.dy;
+    uz = 1.0f - g.dz;
+    L.R0 = __builtin_fmaf(a, c0r - L.R0, L.R0);
+    L.R1 = __builtin_fmaf(a, c1g - L.R1, L.R1);
+    L.R2 = __builtin_fmaf(a, c2b - L.R2, L.R2);
+    L.P = (1.0f - a) * L.P;
+  }
+  const bool want = act && ds != 0.0f;
+  const bool inwin = want && win.set && cx >= win.bx && cy >= win.by && cz >= win.bz &&
+               cx1 < win.bx + W && cy1 < win.by + W && cz1 < win.bz + W;
+  const uint64_t out = __ballot(want && !inwin);
+  if (__popcll(out) > 8 || (!win.set && __ballot(want) != 0ull)) {
+    // re-anchor: flush, then place the window over this step's corners,
+    // leaving room in the direction the wavefront moves next
+    window_flush<W>(win, V, grad_vox, lane);
+    const int big = 1 << 30;
+    const int lox = wave_min_i(want ? cx : big), hix = wave_max_i(want ? cx1 : -big);
+    const int loy = wave_min_i(want ? cy : big), hiy = wave_max_i(want ? cy1 : -big);
+    const int loz = wave_min_i(want ? cz : big), hiz = wave_max_i(want ? cz1 : -big);
+    win.bx = (d0 > 0.0f && hix - lox < W) ? hix - (W - 1) : lox;
+    win.by = (d1 > 0.0f && hiy - loy < W) ? hiy - (W - 1) : loy;
+    win.bz = (d2 > 0.0f && hiz - loz < W) ? hiz - (W - 1) : loz;
+    win.set = true;
+  }
+  if (want) {
+    cell_move(L.cacc, cx, cx1, cy, cy1, cz, cz1, win, V, grad_vox);
+    const float wz0 = ds * uz, wz1 = ds * fz;
+    const float w00 = wz0 * uy, w10 = wz0 * fy, w01 = wz1 * uy, w11 = wz1 * fy;
+    L.cacc.g[0] = __builtin_fmaf(w00, ux, L.cacc.g[0]);
+    L.cacc.g[1] = __builtin_fmaf(w00, fx, L.cacc.g[1]);
+    L.cacc.g[2] = __builtin_fmaf(w10, ux, L.cacc.g[2]);
+    L.cacc.g[3] = __builtin_fmaf(w10, fx, L.cacc.g[3]);
+    L.cacc.g[4] = __builtin_fmaf(w01, ux, L.cacc.g[4]);
+    L.cacc.g[5] = __builtin_fmaf(w01, fx, L.cacc.g[5]);
+    L.cacc.g[6] = __builtin_fmaf(w11, ux, L.cacc.g[6]);
+    L.cacc.g[7] = __builtin_fmaf(w11, fx, L.cacc.g[7]);
+  }
+}
+
 // Dynamic LDS: tf[N] (float4), tf gradient [4N], one voxel window per
 // wavefront, and, when the forward pass left no checkpoints,
 // [kMaxSegNoCkpt + 1][block] recomputed ones.
@@ -547,8 +721,11 @@ __device__ __forceinline__ void cell_move(CellAcc &a, int nx0, int nx1, int ny0,
 // The wavefront walks sample indices k from its longest ray's last sample
 // down to 0 in lockstep (a lane idles while k is past its own ray), so the
 // lanes of an 8x8 ray tile sample neighbouring voxels at the same time.
+// The LDS of a block (about 48 KB with 12^3 windows and a 256-entry table)
+// leaves room for three blocks per CU, three waves per SIMD, so the kernel
+// may use the registers of three.
 template <bool kHaveCkpt, int W>
-__global__ void __launch_bounds__(kVolBwdBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
+__global__ void __launch_bounds__(kVolBwdBlock) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_vol_bwd(Volume V, const float *__restrict__ tf, int N, float step,
           const srf_ray *__restrict__ rays, uint32_t n, const float *__restrict__ ckpt,
           uint32_t slots, const float *__restrict__ grad_rgba, float *__restrict__ grad_vox,
@@ -578,15 +755,16 @@ k_vol_bwd(Volume V, const float *__restrict__ tf, int N, float step,
   if (live) {
     load_ray(rays, i, o, d);
     live = vol_span(o, d, tn, tfar);
+    if (!live) tn = 0.0f;
   }
-  int ns = 0, seg_len = kSegLen;
+  int ns = 0, sh = kSegShift;
   const float *ck = nullptr;
   int ck_stride = 1;
   float4 gr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (live) {
     gr = reinterpret_cast<const float4 *>(grad_rgba)[i];
-    ns = count_samples(tn, tfar, step);  // same float arithmetic as the march
-    seg_len = seg_len_for(ns, kHaveCkpt ? slots : (uint32_t)(kMaxSegNoCkpt + 1));
+    ns = count_samples(tn, tfar, step);  // the forward march's sample count
+    sh = seg_shift_for(ns, kHaveCkpt ? slots : (uint32_t)(kMaxSegNoCkpt + 1));
     if (kHaveCkpt) {
       ck = ckpt + (size_t)i * slots;
     } else {
@@ -594,130 +772,59 @@ k_vol_bwd(Volume V, const float *__restrict__ tf, int N, float step,
       float T = 1.0f;
       float *w = s_ck + threadIdx.x;
       for (int k = 0; k < ns; k++) {
-        if (k % seg_len == 0) w[(k / seg_len) * kVolBwdBlock] = T;
+        if ((k & ((1 << sh) - 1)) == 0) w[(k >> sh) * kVolBwdBlock] = T;
         T = T * (1.0f - sample_alpha(V, s_tf, N, o, d, tn + ((float)k + 0.5f) * step));
       }
-      w[((ns + seg_len - 1) / seg_len) * kVolBwdBlock] = T;
+      w[((ns + (1 << sh) - 1) >> sh) * kVolBwdBlock] = T;
       ck = w;
       ck_stride = kVolBwdBlock;
     }
   }
+  const int mask = (1 << sh) - 1;
+  // Transmittance after the top sample (Tnext) and at the start (Tstart) of
+  // the ray's last segment, and the start of the segment below (Tpre, always
+  // requested one segment ahead). Walking down, the T after a segment's top
+  // sample is the start of the segment above, which the recursion has just
+  // reached, so each segment costs one checkpoint load.
+  float Tnext = 1.0f, Tstart = 1.0f, Tpre = 1.0f;
+  if (ns > 0) {
+    const int sg = (ns - 1) >> sh;
+    Tnext = ck[(sg + 1) * ck_stride];
+    Tstart = ck[sg * ck_stride];
+    if (sg > 0) Tpre = ck[(sg - 1) * ck_stride];
+  }
   // the wavefront's travel direction when walking backwards (for anchoring)
   const float d0 = __shfl(d[0], 0, 64), d1 = __shfl(d[1], 0, 64), d2 = __shfl(d[2], 0, 64);
   const int kmax = wave_max_i(ns);
-  float R0 = 0.0f, R1 = 0.0f, R2 = 0.0f, P = 1.0f;
-  float Tnext = 1.0f, Tstart = 1.0f;
-  TfAcc tacc;
-  tacc.bin = -1;
+  BwdRay R;
+  for (int c = 0; c < 3; c++) {
+    R.o[c] = o[c];
+    R.d[c] = d[c];
+  }
+  R.tn = tn;
+  R.ns = ns;
+  R.sh = sh;
+  R.mask = mask;
+  R.ck = ck;
+  R.ck_stride = ck_stride;
+  R.gr = gr;
+  BwdLane L;
+  L.Tnext = Tnext;
+  L.Tstart = Tstart;
+  L.Tpre = Tpre;
+  L.R0 = L.R1 = L.R2 = 0.0f;
+  L.P = 1.0f;
+  L.tacc.bin = -1;
+  L.cacc.live = false;
+  L.cacc.x0 = L.cacc.x1 = L.cacc.y0 = L.cacc.y1 = L.cacc.z0 = L.cacc.z1 = 0;
+  for (int c = 0; c < 8; c++) L.cacc.g[c] = 0.0f;
   CellCache vc;
   vc.a0 = 0xffffffffu;
-  CellAcc cacc;
-  cacc.live = false;
-  cacc.x0 = cacc.x1 = cacc.y0 = cacc.y1 = cacc.z0 = cacc.z1 = 0;
-  for (int c = 0; c < 8; c++) cacc.g[c] = 0.0f;
-  for (int k = kmax - 1; k >= 0; k--) {
-    const bool act = k < ns;
-    float ds = 0.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    int cx = 0, cy = 0, cz = 0, cx1 = 0, cy1 = 0, cz1 = 0;
-    if (act) {
-      const int sg = k / seg_len;
-      const int k0 = sg * seg_len;
-      if (k == ns - 1 || (k + 1) % seg_len == 0) {  // top sample of segment sg
-        Tnext = ck[(sg + 1) * ck_stride];              // T after that sample
-        Tstart = ck[sg * ck_stride];
-      }
-      const float t = tn + ((float)k + 0.5f) * step;
-      const float pu = o[0] + t * d[0], pv = o[1] + t * d[1], pr = o[2] + t * d[2];
-      Geom g;
-      tex_geom(V, pu, pv, pr, g);
-      const float s = tex_eval_cached(V.vox, g, vc);
-      int i0;
-      float f;
-      bool inside;
-      tf_index(s, N, i0, f, inside);
-      const float4 ta = s_tf[i0], tb = s_tf[i0 + 1];
-      const float c0r = lerpf(f, ta.x, tb.x), c1g = lerpf(f, ta.y, tb.y),
-                  c2b = lerpf(f, ta.z, tb.z);
-      const float a = lerpf(f, ta.w, tb.w);
-      // transmittance in front of sample k
-      float Tk;
-      if (k == k0) {
-        Tk = Tstart;
-      } else if (1.0f - a >= 0.25f) {
-        Tk = Tnext / (1.0f - a);
-      } else {
-        Tk = Tstart;
-        for (int j = k0; j < k; j++)
-          Tk = Tk * (1.0f - sample_alpha(V, s_tf, N, o, d, tn + ((float)j + 0.5f) * step));
-      }
-      Tnext = Tk;
-      const float wa = Tk * a;
-      const float g0 = gr.x * wa, g1 = gr.y * wa, g2 = gr.z * wa;
-      const float ga =
-          Tk * ((gr.x * (c0r - R0) + gr.y * (c1g - R1) + gr.z * (c2b - R2)) + gr.w * P);
-      const float fo = 1.0f - f;
-      tf_move(tacc, i0, my_gtf);
-      tacc.g[0] += fo * g0;
-      tacc.g[1] += fo * g1;
-      tacc.g[2] += fo * g2;
-      tacc.g[3] += fo * ga;
-      tacc.g[4] += f * g0;
-      tacc.g[5] += f * g1;
-      tacc.g[6] += f * g2;
-      tacc.g[7] += f * ga;
-      if (inside)
-        ds = ((g0 * (tb.x - ta.x) + g1 * (tb.y - ta.y)) + (g2 * (tb.z - ta.z) + ga * (tb.w - ta.w))) *
-             (float)(N - 1);
-      // the 8 corners: (cx|cx1, cy|cy1, cz|cz1), trilinear weights
-      cx = g.cx;
-      cy = g.cy;
-      cz = g.cz;
-      cx1 = g.cx1;
-      cy1 = g.cy1;
-      cz1 = g.cz1;
-      fx = g.dx;
-      fy = g.dy;
-      fz = g.dz;
-      ux = 1.0f - g.dx;
-      uy = 1.0f - g.dy;
-      uz = 1.0f - g.dz;
-      R0 = a * c0r + (1.0f - a) * R0;
-      R1 = a * c1g + (1.0f - a) * R1;
-      R2 = a * c2b + (1.0f - a) * R2;
-      P = (1.0f - a) * P;
-    }
-    const bool want = act && ds != 0.0f;
-    const bool inwin = want && win.set && cx >= win.bx && cy >= win.by && cz >= win.bz &&
-                 cx1 < win.bx + W && cy1 < win.by + W && cz1 < win.bz + W;
-    const uint64_t out = __ballot(want && !inwin);
-    if (__popcll(out) > 8 || (!win.set && __ballot(want) != 0ull)) {
-      // re-anchor: flush, then place the window over this step's corners,
-      // leaving room in the direction the wavefront moves next
-      window_flush<W>(win, V, grad_vox, lane);
-      const int big = 1 << 30;
-      const int lox = wave_min_i(want ? cx : big), hix = wave_max_i(want ? cx1 : -big);
-      const int loy = wave_min_i(want ? cy : big), hiy = wave_max_i(want ? cy1 : -big);
-      const int loz = wave_min_i(want ? cz : big), hiz = wave_max_i(want ? cz1 : -big);
-      win.bx = (d0 > 0.0f && hix - lox < W) ? hix - (W - 1) : lox;
-      win.by = (d1 > 0.0f && hiy - loy < W) ? hiy - (W - 1) : loy;
-      win.bz = (d2 > 0.0f && hiz - loz < W) ? hiz - (W - 1) : loz;
-      win.set = true;
-    }
-    if (want) {
-      cell_move(cacc, cx, cx1, cy, cy1, cz, cz1, win, V, grad_vox);
-      cacc.g[0] += ds * (ux * uy * uz);
-      cacc.g[1] += ds * (fx * uy * uz);
-      cacc.g[2] += ds * (ux * fy * uz);
-      cacc.g[3] += ds * (fx * fy * uz);
-      cacc.g[4] += ds * (ux * uy * fz);
-      cacc.g[5] += ds * (fx * uy * fz);
-      cacc.g[6] += ds * (ux * fy * fz);
-      cacc.g[7] += ds * (fx * fy * fz);
-    }
-  }
-  cell_flush(cacc, win, V, grad_vox);
+  for (int k = kmax - 1; k >= 0; k--)
+    bwd_sample<W>(V, s_tf, N, step, k, R, L, vc, my_gtf, win, grad_vox, lane, d0, d1, d2);
+  cell_flush(L.cacc, win, V, grad_vox);
   window_flush<W>(win, V, grad_vox, lane);
-  flush_tf(tacc, my_gtf);
+  flush_tf(L.tacc, my_gtf);
   __syncthreads();
   for (int k = threadIdx.x; k < 4 * N; k += kVolBwdBlock) {
     float v = 0.0f;

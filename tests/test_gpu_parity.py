@@ -417,7 +417,10 @@ def test_texture_sample_bit_exact(S, torch):
 def _vol_case(dim=24):
     vox = W.gaussian_volume(dim, n_gauss=8)
     tf = W.transfer_function(32, max_alpha=0.2)
-    tf[20:24, 3] = 0.9  # a few near-opaque bins exercise the re-march fallback
+    # opaque bins (a sample there has 1 - a = 0, and T = 0 behind it) take the
+    # adjoint's re-march fallback; near-opaque ones take the division
+    tf[20:22, 3] = 1.0
+    tf[22:24, 3] = 0.9
     rays = W.volume_rays(40, 40)
     return vox, tf, rays
 
