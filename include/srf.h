@@ -230,11 +230,18 @@ typedef struct srf_node_info {
  * cleared), as RenderElement::Update computes them
  * (gles/gles_render_graph.cc:81-101). */
 typedef struct {
-  const srf_accel *accel; /* triangles */
+  const srf_accel *accel; /* any kind */
   double T[16];
   double invT[16];
   double invT33[16];
 } srf_instance;
+
+/* RenderElement::Update's matrices (gles/gles_render_graph.cc:81-101): from
+ * inst->T, invT = Matrixd::Inverse(T) and invT33 = Matrixd::Inverse(T with
+ * row 3 cleared), by the same Cramer's-rule inverse
+ * (render/render_matrix.h:40-134), so a scene gets the reference's
+ * matrices bit for bit. Host only. */
+srf_status srf_instance_update(srf_instance *inst);
 
 /* A top-level node as BVHTree keeps it (BVHNode, render/render_bvh_tree.h):
  * box, children (0 = none), split axis, leaf flag, element index. 72 B. */
@@ -510,9 +517,11 @@ srf_status srf_lbvh_radix_tree30(srf_ctx *ctx, const uint32_t *sorted_codes,
 /* RenderGraph::Build (gles/gles_render_graph.cc:103-143): world boxes of the
  * elements (TransformBoundingBox of each accelerator's BoundingBox(), :19-79)
  * and the top-level BVHTree over them (render/render_bvh_tree.cc:319-502),
- * built on the host; waits for the accelerators' builds. The scene keeps the
- * accelerators' device arrays: rebuilding one of them requires a new scene,
- * as the reference rebuilds its graph. */
+ * built on the host; waits for the accelerators' builds. Elements may be
+ * accelerators of any kind (triangles, tetrahedra, spheres, solids, lines),
+ * as RenderGraph::Trace switches on the element's primitive type
+ * (:224-264). The scene keeps the accelerators' device arrays: rebuilding
+ * one of them requires a new scene, as the reference rebuilds its graph. */
 srf_status srf_scene_create(srf_ctx *ctx, uint32_t num_instances,
                             const srf_instance *instances_host, srf_scene **out);
 srf_status srf_scene_destroy(srf_scene *scene);
@@ -544,7 +553,15 @@ typedef struct {
  * rays: hits[i] = {world distance, u, v, prim_id} of the nearest element
  * hit, instance_ids[i] its element (~0 on a miss; may be NULL). The ray's
  * prev_prim_id is not used (no element is the ray's previous one).
- * mode: srf_launch_mode. */
+ * mode: srf_launch_mode. A scene of triangle accelerators only is one
+ * kernel (capturable in a hipGraph). A scene mixing kinds is traced in
+ * element passes: each element's rays go to that accelerator's own
+ * closest-hit launch (srf_trace_closest), then one kernel walks the top
+ * level per ray over those hits; it allocates scratch (about 4 bytes per
+ * element and ray plus 48 per ray entering an element's box), synchronises
+ * the stream once per chunk of 2^26 / elements rays, cannot be captured in
+ * a graph, and serialises calls on the same scene. A solid element's u and
+ * v are 0 (the reference never writes them). */
 srf_status srf_scene_trace(const srf_scene *scene, const srf_ray *rays, uint64_t n,
                            srf_hit *hits, uint32_t *instance_ids, int mode, void *stream);
 /* Autotuner: traces the given rays with every launch configuration (block

@@ -32,6 +32,18 @@ class TriElement(C.Structure):
                 ("F", C.c_void_p), ("eps_scale", C.c_double), ("T", C.c_double * 16),
                 ("invT", C.c_double * 16), ("invT33", C.c_double * 16)]
 
+class Element(C.Structure):
+    """or_element: one element of a scene mixing primitive kinds."""
+    _fields_ = [("kind", C.c_int32), ("nv", C.c_int32), ("cap", C.c_int32),
+                ("rconst", C.c_float), ("nodes", C.c_void_p), ("indices", C.c_void_p),
+                ("V", C.c_void_p), ("F", C.c_void_p), ("R", C.c_void_p),
+                ("eps_scale", C.c_double), ("T", C.c_double * 16), ("invT", C.c_double * 16),
+                ("invT33", C.c_double * 16)]
+
+
+ELEMENT_KINDS = {"tri": (0, TRI_NODE), "tet": (1, TET_NODE), "sph": (2, SPH_NODE),
+                 "sol": (3, TET_NODE), "lin": (4, TET_NODE)}
+
 _p = C.c_void_p
 _i64 = C.c_int64
 
@@ -60,7 +72,8 @@ class Oracle:
                      "or_scene_bounds_sph", "or_transform_box", "or_scene_trace",
                      "or_scene_bounds_sol", "or_morton_sol", "or_trace_sol",
                      "or_interpolate_solid", "or_scene_bounds_lin", "or_morton_lin",
-                     "or_trace_lin", "or_isect_tri", "or_isect_tet", "or_isect_sph"):
+                     "or_trace_lin", "or_isect_tri", "or_isect_tet", "or_isect_sph",
+                     "or_scene_trace_mixed"):
             getattr(L, name).restype = None
         L.or_tlas_build.restype = _i64
         for name in ("or_build_tri", "or_build_tet", "or_build_sph", "or_build_sol",
@@ -421,6 +434,54 @@ class Oracle:
         return hits, inst
 
 
+    def build_element(self, e):
+        """The tree of one element of a mixed scene (a dict as
+        surface_amd.workloads.mixed_scene makes them)."""
+        k = e["kind"]
+        if k == "tri":
+            return self.build_tri(e["V"], e["F"])
+        if k == "tet":
+            return self.build_tet(e["V"], e["F"])
+        if k == "sph":
+            return self.build_sph(e["V"], e.get("R"), e.get("rconst", 0.0))
+        if k == "sol":
+            return self.build_sol(e["V"], e["F"], e["nv"])
+        return self.build_lin(e["V"], e["F"], e.get("R"), e.get("rconst", 0.0))
+
+    def scene_trace_mixed(self, tlas, elements, rays, threads=1):
+        """elements: list of dicts with kind ("tri", "tet", "sph", "sol" or
+        "lin"), the element's tree (from the matching build_*), its arrays
+        (V, F; spheres: V = centres, R or rconst; solids: nv; lines: R or
+        rconst, cap) and T, invT, invT33 (4x4 float64)."""
+        from surface_amd.workloads import HIT_DTYPE
+        keep = []
+        arr = (Element * len(elements))()
+        for k, e in enumerate(elements):
+            code, ndt = ELEMENT_KINDS[e["kind"]]
+            nodes = _c(e["tree"]["nodes"], ndt)
+            idx = _c(e["tree"]["indices"], np.uint32)
+            V = _c(e["V"], np.float32)
+            F = None if e.get("F") is None else _c(e["F"], np.uint32)
+            R = None if e.get("R") is None else _c(e["R"], np.float32)
+            keep += [nodes, idx, V, F, R]
+            arr[k].kind, arr[k].nv, arr[k].cap = code, int(e.get("nv", 0)), int(e.get("cap", 1))
+            arr[k].rconst = float(e.get("rconst", 0.0))
+            arr[k].nodes, arr[k].indices = _ptr(nodes).value, _ptr(idx).value
+            arr[k].V = _ptr(V).value
+            arr[k].F = None if F is None else _ptr(F).value
+            arr[k].R = None if R is None else _ptr(R).value
+            arr[k].eps_scale = e["tree"].get("eps_scale", 0.0)
+            for name in ("T", "invT", "invT33"):
+                getattr(arr[k], name)[:] = list(np.asarray(e[name], np.float64).reshape(16))
+        tl = _c(tlas, TLAS_NODE)
+        rays = np.ascontiguousarray(rays)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        inst = np.zeros(len(rays), np.uint32)
+        self.lib.or_scene_trace_mixed(_ptr(tl), arr, _ptr(rays), _i64(len(rays)), _ptr(hits),
+                                      _ptr(inst), C.c_int(threads))
+        return hits, inst
+
+
 def ref_available(variant="serial"):
     return os.path.exists(os.path.join(HERE, "_ref", f"libsrfref_{variant}.so"))
 
@@ -564,7 +625,8 @@ class Reference:
             tree["handle"] = None
 
     def scene_build(self, trees, T):
-        """RenderGraph over triangle accelerators (trees from build(..., keep=True))
+        """RenderGraph over accelerators of any kind (trees from build(...,
+        keep=True); each becomes the RenderElement glDrawElements records)
         with object-to-world matrices T [n, 4, 4]; returns the reference's
         invT, invT33, world boxes, top-level BVH nodes and the graph handle."""
         n = len(trees)

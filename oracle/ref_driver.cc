@@ -516,12 +516,39 @@ void ref_sparse_sample(const int *blocks, const void *const *data, long long nbl
   for (long long i = 0; i < n; i++) accel.Sample(values + 4 * i, pos + 3 * i);
 }
 
-// ---- two-level scenes (RenderGraph over triangle accelerators) ----
+// ---- two-level scenes (RenderGraph over accelerators of any kind) ----
 struct RefScene {
   lsgl::RenderGraph graph;
 };
 
-// Builds a RenderGraph of n triangle elements (accels from ref_build_tri,
+// The RenderElement primitive type and accelerator of a built RefAccel, as
+// glDrawElements records them (gles/gles_primitive.cc:137-293).
+static void element_of(RefAccel *a, lsgl::AccelBuilder::PrimitiveType *ty, GLenum *mode,
+                       unsigned char **accel, double bmin[3], double bmax[3]) {
+  typedef lsgl::AccelBuilder AB;
+  if (a->kind == 1) {
+    *ty = AB::PRIMITIVE_TETRAHEDRONS, *mode = GL_TETRAHEDRONS_EXT;
+    *accel = reinterpret_cast<unsigned char *>(&a->tet_accel);
+  } else if (a->kind == 2) {
+    *ty = AB::PRIMITIVE_POINTS, *mode = GL_POINTS;
+    *accel = reinterpret_cast<unsigned char *>(&a->sph_accel);
+  } else if (a->kind == 3) {
+    const int nv = a->sol.numVertsPerSolid;
+    *ty = nv == 5 ? AB::PRIMITIVE_PYRAMIDS : (nv == 6 ? AB::PRIMITIVE_PRISMS
+                                                       : AB::PRIMITIVE_HEXAHEDRONS);
+    *mode = nv == 5 ? GL_PYRAMIDS_EXT : (nv == 6 ? GL_PRISMS_EXT : GL_HEXAHEDRONS_EXT);
+    *accel = reinterpret_cast<unsigned char *>(&a->sol_accel);
+  } else if (a->kind == 4) {
+    *ty = AB::PRIMITIVE_LINES, *mode = GL_LINES;
+    *accel = reinterpret_cast<unsigned char *>(&a->lin_accel);
+  } else {
+    *ty = AB::PRIMITIVE_TRIANGLES, *mode = GL_TRIANGLES;
+    *accel = reinterpret_cast<unsigned char *>(&a->tri_accel);
+  }
+  ref_bounds(a, bmin, bmax);
+}
+
+// Builds a RenderGraph of n elements (accels from ref_build_*, any kind;
 // T[i] = 16 doubles, row-major T_[4][4]); writes each element's inverse
 // matrices and world box as RenderElement::Update computes them, and the
 // top-level BVHTree over those boxes (nodes: 6 doubles box + child[2] +
@@ -534,11 +561,13 @@ void *ref_scene_build(int n, void **accels, const double *T, double *invT_out,
   for (int i = 0; i < n; i++) {
     RefAccel *a = (RefAccel *)accels[i];
     double bmin[3], bmax[3], M[4][4];
-    a->tri_accel.BoundingBox(bmin, bmax);
+    lsgl::AccelBuilder::PrimitiveType ty;
+    GLenum mode;
+    unsigned char *acc;
+    element_of(a, &ty, &mode, &acc, bmin, bmax);
     memcpy(M, T + 16 * i, sizeof(M));
-    lsgl::RenderElement e(lsgl::AccelBuilder::PRIMITIVE_TRIANGLES, GL_TRIANGLES,
-                          reinterpret_cast<unsigned char *>(&a->tri_accel), bmin, bmax, M, 0,
-                          NULL, FragmentState(), lsgl::ShadingState());
+    lsgl::RenderElement e(ty, mode, acc, bmin, bmax, M, 0, NULL, FragmentState(),
+                          lsgl::ShadingState());
     s->graph.AddRenderElement(e);
     memcpy(invT_out + 16 * i, e.invT_, sizeof(M));
     memcpy(invT33_out + 16 * i, e.invT33_, sizeof(M));

@@ -2836,6 +2836,97 @@ void or_scene_trace(const or_tlas_node *tlas, const or_tri_element *elems,
 }
 
 
+/* RenderGraph::Trace with the primitive-kind switch of
+ * gles/gles_render_graph.cc:224-264; the loop is scene_trace_one's. */
+static void element_trace_one(const or_element *e, const or_ray *lr, or_hit *bh) {
+  switch (e->kind) {
+    case 0:
+      trace_tri_one((const or_tri_node *)e->nodes, e->indices, e->V, e->F, (float)e->eps_scale,
+                    lr, bh, NULL);
+      break;
+    case 1:
+      trace_tet_one((const or_tet_node *)e->nodes, e->indices, e->V, e->F, NULL, lr, bh, NULL,
+                    NULL);
+      break;
+    case 2:
+      trace_sph_one((const or_sph_node *)e->nodes, e->indices, e->V, e->R, e->rconst, lr, 0.0,
+                    NULL, bh, NULL);
+      break;
+    case 3:
+      trace_sol_one((const or_tet_node *)e->nodes, e->indices, e->V, e->F, e->nv, NULL, lr, bh,
+                    NULL, NULL);
+      break;
+    default:
+      trace_lin_one((const or_tet_node *)e->nodes, e->indices, e->V, e->F, e->R, e->rconst,
+                    e->cap, lr, bh, NULL, NULL);
+      break;
+  }
+}
+
+static void scene_trace_one_mixed(const or_tlas_node *tl, const or_element *el,
+                                  tlas_isect *list, const or_ray *r, or_hit *h,
+                                  uint32_t *inst) {
+  double ro[3] = {r->org[0], r->org[1], r->org[2]};
+  double rd[3] = {r->dir[0], r->dir[1], r->dir[2]};
+  tlas_ray tr;
+  tlas_setup(ro, rd, &tr);
+  h->t = FLT_MAX;
+  h->u = h->v = 0.0f;
+  h->prim_id = 0xffffffffu;
+  *inst = 0xffffffffu;
+  int64_t cnt = tlas_trace(tl, &tr, list);
+  double nearest = DBL_MAX;
+  for (int64_t i = 0; i < cnt; i++) {
+    if (nearest < list[i].tmin) continue;
+    const or_element *e = el + list[i].id;
+    double lo[3], ld[3];
+    mult_v(e->invT, ro, lo);
+    mult_v(e->invT33, rd, ld);
+    or_ray lr;
+    for (int k = 0; k < 3; k++) {
+      lr.org[k] = (float)lo[k];
+      lr.dir[k] = (float)ld[k];
+    }
+    lr.flags = r->flags & 1u;
+    lr.prev_prim_id = 0xffffffffu;
+    or_hit bh;
+    element_trace_one(e, &lr, &bh);
+    if (bh.prim_id == 0xffffffffu) continue;
+    float lp[3], wp[3];
+    for (int k = 0; k < 3; k++) lp[k] = (float)(lo[k] + (double)bh.t * ld[k]);
+    double lpd[3] = {lp[0], lp[1], lp[2]}, wpd[3];
+    mult_v(e->T, lpd, wpd);
+    for (int k = 0; k < 3; k++) wp[k] = (float)wpd[k];
+    float px = wp[0] - r->org[0], py = wp[1] - r->org[1], pz = wp[2] - r->org[2];
+    float tw = sqrtf((px * px + py * py) + pz * pz);
+    if (tw < nearest) {
+      nearest = tw;
+      *h = bh;
+      h->t = tw;
+      *inst = list[i].id;
+    }
+  }
+}
+
+void or_scene_trace_mixed(const or_tlas_node *tlas, const or_element *elems,
+                          const or_ray *rays, int64_t nrays, or_hit *hits, uint32_t *inst,
+                          int threads) {
+  const int64_t nleaves = tlas_leaves(tlas);
+  (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+#endif
+  {
+    tlas_isect *list = (tlas_isect *)malloc(sizeof(tlas_isect) * (size_t)(nleaves + 1));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+    for (int64_t i = 0; i < nrays; i++)
+      scene_trace_one_mixed(tlas, elems, list, rays + i, hits + i, inst + i);
+    free(list);
+  }
+}
+
 /* ------------------------------------------------------ BuildIntersection */
 
 typedef struct {

@@ -243,6 +243,35 @@ void or_scene_trace(const or_tlas_node *tlas, const or_tri_element *elems,
                     const or_ray *rays, int64_t nrays, or_hit *hits, uint32_t *inst,
                     int threads);
 
+/* One element of a scene that mixes primitive kinds (RenderGraph::Trace's
+ * switch, gles/gles_render_graph.cc:224-264): kind 0 triangles (nodes are
+ * or_tri_node, V/F, eps_scale), 1 tetrahedra (or_tet_node, V/F), 2 spheres
+ * (or_sph_node, positions V, radii R or rconst), 3 solids (or_tet_node, V/F,
+ * nv vertices each), 4 lines (or_tet_node, positions V, segments F, radii R
+ * or rconst, cap); transforms as in or_tri_element. */
+typedef struct {
+  int32_t kind;
+  int32_t nv;
+  int32_t cap;
+  float rconst;
+  const void *nodes;
+  const uint32_t *indices;
+  const float *V;
+  const uint32_t *F;
+  const float *R;
+  double eps_scale;
+  double T[16];
+  double invT[16];
+  double invT33[16];
+} or_element;
+
+/* RenderGraph::Trace per ray over elements of any kind: each element's hit
+ * is its accelerator's own closest hit (its Traverse) of the ray moved into
+ * its space; t = world distance, u/v/prim_id the element's. */
+void or_scene_trace_mixed(const or_tlas_node *tlas, const or_element *elems,
+                          const or_ray *rays, int64_t nrays, or_hit *hits, uint32_t *inst,
+                          int threads);
+
 /* Texture::Fetch of a dense 3D texture (byte / float / double, 1-4
  * interleaved components, optional per-axis remap tables): rgba[4 n]. */
 void or_tex3d_fetch(const void *image, int format, int comps, const int dims[3],

@@ -132,6 +132,7 @@ SIGNATURES = {
     "srf_scene_bounds": (_i, [_vp, _vp, _vp]),
     "srf_scene_export": (_i, [_vp, _vp, C.POINTER(C.c_uint32)]),
     "srf_scene_trace": (_i, [_vp, _vp, _u64, _vp, _vp, _i, _vp]),
+    "srf_instance_update": (_i, [_vp]),
     "srf_scene_autotune": (_i, [_vp, _vp, _u64, _vp, _vp, _vp, _vp]),
     "srf_trace_any_sorted": (_i, [_vp, _vp, _vp, _vp, _u64, C.c_uint32, _i, _vp, _vp]),
     "srf_trace_any_sorted_noalloc": (_i, [_vp, _vp, _vp, _u64, C.c_uint32, _i, _vp, _vp]),

@@ -542,9 +542,8 @@ __all__ = ["Context", "TriangleAccel", "TetraAccel", "ParticleAccel", "SrfError"
 
 def rigid_inverses(T):
     """invT and invT33 of object-to-world matrices T [n, 4, 4] (row-vector
-    convention) computed with numpy. The reference's RenderElement::Update
-    computes them with its own Cramer-rule inverse; pass those matrices to
-    Scene to reproduce its results bit for bit."""
+    convention) computed with numpy; element_inverses gives the reference's
+    own (RenderElement::Update's Cramer-rule inverse), bit for bit."""
     T = np.asarray(T, np.float64)
     invT = np.linalg.inv(T)
     T33 = T.copy()
@@ -552,9 +551,26 @@ def rigid_inverses(T):
     return invT, np.linalg.inv(T33)
 
 
+def element_inverses(T):
+    """invT and invT33 of object-to-world matrices T [n, 4, 4] as
+    RenderElement::Update computes them (srf_instance_update; host only)."""
+    T = np.asarray(T, np.float64).reshape(-1, 4, 4)
+    lib = _lib.load()
+    invT = np.zeros_like(T)
+    invT33 = np.zeros_like(T)
+    for i in range(len(T)):
+        inst = _lib.Instance()
+        inst.T[:] = list(T[i].reshape(16))
+        check(lib.srf_instance_update(C.byref(inst)))
+        invT[i] = np.array(inst.invT[:]).reshape(4, 4)
+        invT33[i] = np.array(inst.invT33[:]).reshape(4, 4)
+    return invT, invT33
+
+
 class Scene:
-    """RenderGraph over triangle accelerators: Build (top-level BVH over the
-    elements' world boxes) at construction, Trace for batches of rays."""
+    """RenderGraph over accelerators of any kind (triangles, tetrahedra,
+    spheres, solids, lines): Build (top-level BVH over the elements' world
+    boxes) at construction, Trace for batches of rays."""
 
     def __init__(self, accels, T, invT=None, invT33=None, ctx=None):
         self.ctx = ctx or default_context()
@@ -562,7 +578,7 @@ class Scene:
         self.accels = list(accels)  # keeps the elements alive
         T = np.asarray(T, np.float64).reshape(-1, 4, 4)
         if invT is None or invT33 is None:
-            invT, invT33 = rigid_inverses(T)
+            invT, invT33 = element_inverses(T)
         n = len(self.accels)
         arr = (_lib.Instance * n)()
         for i, a in enumerate(self.accels):

@@ -208,6 +208,7 @@ srf_status fetch_params_for(const Accel *a, BuildParams *p) { return fetch_param
 
 struct Scene;
 srf_status scene_create(Ctx *ctx, uint32_t n, const srf_instance *inst, Scene **out);
+void cramer_inverse(const double in[16], double m[16]);
 srf_status scene_trace(const Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
                        uint32_t *inst_out, int mode, hipStream_t stream);
 srf_status scene_autotune(Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
@@ -698,6 +699,16 @@ srf_status srf_lbvh_radix_tree30(srf_ctx *ctx, const uint32_t *sorted_codes, uin
   if (n >= (1ull << 31)) return set_error(SRF_ERR_INVALID_ARG, "too many keys");
   DeviceGuard g(c->device);
   return launch_radix_tree(sorted_codes, (uint32_t)n, out, reinterpret_cast<hipStream_t>(stream));
+}
+
+srf_status srf_instance_update(srf_instance *inst) {
+  if (!inst) return set_error(SRF_ERR_INVALID_ARG, "null instance");
+  double t33[16];
+  memcpy(t33, inst->T, sizeof(t33));
+  t33[12] = t33[13] = t33[14] = 0.0;  // row 3: the translation
+  cramer_inverse(inst->T, inst->invT);
+  cramer_inverse(t33, inst->invT33);
+  return SRF_OK;
 }
 
 srf_status srf_scene_create(srf_ctx *ctx, uint32_t n, const srf_instance *inst,

@@ -1,6 +1,7 @@
-// srf_scene.hip -- two-level scenes: many triangle accelerators placed by
-// transforms, a top-level BVH over their world boxes, and one kernel that
-// traces rays through both levels.
+// srf_scene.hip -- two-level scenes: many accelerators placed by transforms,
+// a top-level BVH over their world boxes, and the trace through both levels
+// (one kernel when every element is a triangle accelerator; element passes
+// when the scene mixes primitive kinds, see the note above scene_trace_mixed).
 //
 // The top level follows the reference's RenderGraph exactly:
 //   world box    TransformBoundingBox          gles/gles_render_graph.cc:19-79
@@ -25,6 +26,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -57,6 +60,15 @@ struct alignas(16) SceneInst {
   double invT33[12];
 };
 
+// Scratch of the element passes of a mixed scene's trace (grown on demand).
+struct MixedScratch {
+  std::mutex mu;                 // one mixed trace per scene at a time
+  void *buf = nullptr;
+  size_t bytes = 0;
+  std::vector<uint32_t> counts;  // rays per element (host copy)
+  std::vector<uint32_t> off;     // element list offsets
+};
+
 struct Scene {
   int device;
   uint32_t count;       // elements
@@ -66,9 +78,13 @@ struct Scene {
   TlasNode *d_nodes;
   SceneInst *d_inst;
   uint32_t *d_counter;  // work counter of the persistent launch
+  uint32_t *d_leaf;     // element -> its top-level leaf node
   std::vector<TlasNode> h_nodes;
   int cus;              // compute units of the device
   srf_launch_config tuned;
+  bool mixed;           // some element is not a triangle accelerator
+  std::vector<const srf_accel *> accels;
+  MixedScratch *mx;
 };
 
 namespace {
@@ -437,7 +453,192 @@ k_scene_tri(const TlasNode *__restrict__ tl, uint32_t nnodes, const SceneInst *_
 
 __global__ void k_zero_counter(uint32_t *c) { *c = 0u; }
 
+__global__ void __launch_bounds__(256) k_zero_words(uint32_t *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// The element passes of a mixed scene. For every ray and element, the test
+// the element walk applies to the element's top-level leaf (the ray enters
+// the element's world box). The counting pass adds the entering rays per
+// element; the filling pass (kFill) gives each such pair slot j of the
+// element's list (cursor[e]), writes the ray moved into the element's space
+// there as the element walk forms it (invT / invT33 in double, then float),
+// and records inv[e * n + i] = j (~0 for the other pairs).
+template <bool kFill>
+__global__ void __launch_bounds__(256)
+k_elem_select(const TlasNode *__restrict__ tl, const uint32_t *__restrict__ leaf,
+              const SceneInst *__restrict__ inst, uint32_t ninst,
+              const srf_ray *__restrict__ rays, uint32_t n, uint32_t *counts,
+              const uint32_t *__restrict__ off, srf_ray *__restrict__ lrays,
+              uint32_t *__restrict__ inv) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool live = i < n;
+  double ro[3] = {0.0, 0.0, 0.0}, rd[3] = {0.0, 0.0, 1.0};
+  uint32_t flags = 0;
+  if (live) {
+    const float4 *r4 = reinterpret_cast<const float4 *>(rays + i);
+    const float4 ra = r4[0], rb = r4[1];
+    ro[0] = ra.x, ro[1] = ra.y, ro[2] = ra.z;
+    rd[0] = rb.x, rd[1] = rb.y, rd[2] = rb.z;
+    flags = __float_as_uint(ra.w) & SRF_RAY_DOUBLE_SIDED;
+  }
+  TlasRay tr;
+  tlas_setup(ro, rd, tr);
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (uint32_t e = 0; e < ninst; e++) {
+    bool sel = false;
+    if (live) {
+      double t0;
+      sel = tlas_box(tr, load_tlas(tl, leaf[e]), t0);
+    }
+    const uint64_t m = __ballot(sel);
+    uint32_t base = 0;
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      if ((int)lane == leader) base = atomicAdd(&counts[e], (uint32_t)__popcll(m));
+      base = (uint32_t)__shfl((int)base, leader, 64);
+    }
+    if (!kFill) continue;
+    const uint32_t j = base + (uint32_t)__popcll(m & below);
+    if (live) inv[(size_t)e * n + i] = sel ? j : 0xffffffffu;
+    if (sel) {
+      const SceneInst *el = inst + e;
+      double lo[3], ld[3];
+      mult_rows(el->invT, ro, lo);
+      mult_rows(el->invT33, rd, ld);
+      float4 *w = reinterpret_cast<float4 *>(lrays + off[e] + j);
+      w[0] = make_float4((float)lo[0], (float)lo[1], (float)lo[2], __uint_as_float(flags));
+      w[1] = make_float4((float)ld[0], (float)ld[1], (float)ld[2], __uint_as_float(SRF_NO_PRIM));
+    }
+  }
+}
+
+// RenderGraph::Trace from the elements' own traces: the element walk of
+// scene_ray, with each element's bottom-level hit read from its list
+// (lhits[off[e] + inv[e * n + i]]) instead of traced here. The bottom-level
+// trace of an element does not depend on the hits before it (it starts from
+// t_max = FLT_MAX), so the walk's result is the same.
+template <int kBlock>
+__global__ void __launch_bounds__(kBlock)
+k_scene_resolve(const TlasNode *__restrict__ tl, uint32_t nnodes,
+                const SceneInst *__restrict__ inst, uint32_t ninst,
+                const srf_ray *__restrict__ rays, uint32_t n, const uint32_t *__restrict__ inv,
+                const uint32_t *__restrict__ off, const srf_hit *__restrict__ lhits,
+                srf_hit *__restrict__ hits, uint32_t *__restrict__ inst_out, uint32_t tdepth) {
+  extern __shared__ uint32_t smem[];
+  uint32_t *tst = smem + threadIdx.x;
+  const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 *r4 = reinterpret_cast<const float4 *>(rays + i);
+  const float4 ra = r4[0], rb = r4[1];
+  const float wo[3] = {ra.x, ra.y, ra.z};
+  const double ro[3] = {ra.x, ra.y, ra.z}, rd[3] = {rb.x, rb.y, rb.z};
+  TlasRay tr;
+  tlas_setup(ro, rd, tr);
+  double nearest = DBL_MAX;
+  float bt = FLT_MAX, bu = 0.0f, bv = 0.0f;
+  uint32_t bprim = SRF_NO_PRIM, binst = SRF_NO_PRIM;
+  bool have_prev = false;
+  double prev_t = 0.0;
+  uint32_t prev_rank = 0;
+  for (uint32_t pass = 0; pass < ninst; pass++) {
+    double nt = 0.0;
+    uint32_t nr = 0, nid = 0;
+    if (!next_element<kBlock>(tl, nnodes, tr, tst, tdepth, have_prev, prev_t, prev_rank, nt, nr,
+                              nid))
+      break;
+    if (nearest < nt) break;  // every later element is skipped as well
+    have_prev = true;
+    prev_t = nt;
+    prev_rank = nr;
+    if (nid >= ninst) continue;
+    const uint32_t j = inv[(size_t)nid * n + i];
+    if (j == 0xffffffffu) continue;
+    const float4 h = reinterpret_cast<const float4 *>(lhits)[off[nid] + j];
+    const uint32_t prim = __float_as_uint(h.w);
+    if (prim == SRF_NO_PRIM) continue;
+    const SceneInst *e = inst + nid;
+    double lo[3], ld[3];
+    mult_rows(e->invT, ro, lo);
+    mult_rows(e->invT33, rd, ld);
+    // world distance of the local hit point (real3 arithmetic in float)
+    double lp[3], wp[3];
+    for (int k = 0; k < 3; k++) lp[k] = (double)(float)(lo[k] + (double)h.x * ld[k]);
+    mult_rows(e->T, lp, wp);
+    const float px = (float)wp[0] - wo[0], py = (float)wp[1] - wo[1], pz = (float)wp[2] - wo[2];
+    const float tw = sqrtf((px * px + py * py) + pz * pz);
+    if ((double)tw < nearest) {
+      nearest = tw;
+      bt = tw;
+      bu = h.y;
+      bv = h.z;
+      bprim = prim;
+      binst = nid;
+    }
+  }
+  reinterpret_cast<float4 *>(hits)[i] = make_float4(bt, bu, bv, __uint_as_float(bprim));
+  if (inst_out) inst_out[i] = binst;
+}
+
 }  // namespace
+
+// Matrixd::Inverse (render/render_matrix.h:40-134): the cofactor inverse by
+// Cramer's rule of Intel's published SSE note, in the reference's operation
+// order (products of transposed-source pairs, each cofactor a sum of three
+// products minus a sum of three, the determinant from row 0, then every
+// entry times 1 / det), so the inverses are the reference's bit for bit.
+// Its last cofactor repeats the source's slip (element 0 where the note
+// reads element 10); MultV never reads that entry.
+void cramer_inverse(const double in[16], double m[16]) {
+  double s[16];  // transposed source
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) s[i + 4 * j] = in[4 * i + j];
+  auto pairs = [&](const int a[12][2], double p[12]) {
+    for (int k = 0; k < 12; k++) p[k] = s[a[k][0]] * s[a[k][1]];
+  };
+  static const int lo[12][2] = {{10, 15}, {11, 14}, {9, 15}, {11, 13}, {9, 14}, {10, 13},
+                                {8, 15},  {11, 12}, {8, 14}, {10, 12}, {8, 13}, {9, 12}};
+  static const int hi[12][2] = {{2, 7}, {3, 6}, {1, 7}, {3, 5}, {1, 6}, {2, 5},
+                                {0, 7}, {3, 4}, {0, 6}, {2, 4}, {0, 5}, {1, 4}};
+  // each cofactor: (p[a0] s[b0] + p[a1] s[b1] + p[a2] s[b2])
+  //              - (p[c0] s[d0] + p[c1] s[d1] + p[c2] s[d2])
+  struct Cof {
+    int8_t a[3], b[3], c[3], d[3];
+  };
+  static const Cof first[8] = {
+      {{0, 3, 4}, {5, 6, 7}, {1, 2, 5}, {5, 6, 7}},
+      {{1, 6, 9}, {4, 6, 7}, {0, 7, 8}, {4, 6, 7}},
+      {{2, 7, 10}, {4, 5, 7}, {3, 6, 11}, {4, 5, 7}},
+      {{5, 8, 11}, {4, 5, 6}, {4, 9, 10}, {4, 5, 6}},
+      {{1, 2, 5}, {1, 2, 3}, {0, 3, 4}, {1, 2, 3}},
+      {{0, 7, 8}, {0, 2, 3}, {1, 6, 9}, {0, 2, 3}},
+      {{3, 6, 11}, {0, 1, 3}, {2, 7, 10}, {0, 1, 3}},
+      {{4, 9, 10}, {0, 1, 2}, {5, 8, 11}, {0, 1, 2}}};
+  static const Cof second[8] = {
+      {{0, 3, 4}, {13, 14, 15}, {1, 2, 5}, {13, 14, 15}},
+      {{1, 6, 9}, {12, 14, 15}, {0, 7, 8}, {12, 14, 15}},
+      {{2, 7, 10}, {12, 13, 15}, {3, 6, 11}, {12, 13, 15}},
+      {{5, 8, 11}, {12, 13, 14}, {4, 9, 10}, {12, 13, 14}},
+      {{2, 5, 1}, {10, 11, 9}, {4, 0, 3}, {11, 9, 10}},
+      {{8, 0, 7}, {11, 8, 10}, {6, 9, 1}, {10, 11, 8}},
+      {{6, 11, 3}, {9, 11, 8}, {10, 2, 7}, {11, 8, 9}},
+      {{10, 4, 9}, {10, 8, 9}, {8, 11, 5}, {9, 0, 8}}};
+  double p[12];
+  auto cof = [&](const Cof &c) {
+    double v = p[c.a[0]] * s[c.b[0]] + p[c.a[1]] * s[c.b[1]] + p[c.a[2]] * s[c.b[2]];
+    v -= p[c.c[0]] * s[c.d[0]] + p[c.c[1]] * s[c.d[1]] + p[c.c[2]] * s[c.d[2]];
+    return v;
+  };
+  pairs(lo, p);
+  for (int k = 0; k < 8; k++) m[k] = cof(first[k]);
+  pairs(hi, p);
+  for (int k = 0; k < 8; k++) m[8 + k] = cof(second[k]);
+  double det = s[0] * m[0] + s[1] * m[1] + s[2] * m[2] + s[3] * m[3];
+  det = 1.0 / det;
+  for (int k = 0; k < 16; k++) m[k] *= det;
+}
 
 // ------------------------------------------------------------------ API
 
@@ -456,8 +657,6 @@ srf_status scene_create(Ctx *ctx, uint32_t n, const srf_instance *inst, Scene **
   for (uint32_t i = 0; i < n; i++) {
     const Accel *a = reinterpret_cast<const Accel *>(inst[i].accel);
     if (!a) return set_error(SRF_ERR_INVALID_ARG, "instance %u: null accelerator", i);
-    if (a->kind != kTri)
-      return set_error(SRF_ERR_UNSUPPORTED, "instance %u: scenes hold triangle accelerators", i);
     if (a->device != ctx->device)
       return set_error(SRF_ERR_INVALID_ARG, "instance %u: accelerator on another device", i);
     BuildParams p;
@@ -486,6 +685,13 @@ srf_status scene_create(Ctx *ctx, uint32_t n, const srf_instance *inst, Scene **
   if (!sc) return set_error(SRF_ERR_OUT_OF_MEMORY, "host allocation failed");
   sc->device = ctx->device;
   sc->count = n;
+  sc->mixed = false;
+  sc->mx = nullptr;
+  sc->d_leaf = nullptr;
+  for (uint32_t i = 0; i < n; i++) {
+    sc->accels.push_back(inst[i].accel);
+    if (reinterpret_cast<const Accel *>(inst[i].accel)->kind != kTri) sc->mixed = true;
+  }
   sc->h_nodes = TlasBuilder(items).build();
   sc->num_nodes = (uint32_t)sc->h_nodes.size();
   sc->depth = branch_depth(sc->h_nodes, 0);
@@ -503,6 +709,17 @@ srf_status scene_create(Ctx *ctx, uint32_t n, const srf_instance *inst, Scene **
   if (e == hipSuccess) e = hipMalloc(&sc->d_nodes, sizeof(TlasNode) * sc->num_nodes);
   if (e == hipSuccess) e = hipMalloc(&sc->d_inst, sizeof(SceneInst) * n);
   if (e == hipSuccess) e = hipMalloc(&sc->d_counter, 256);
+  // each element's top-level leaf (the element passes test it directly)
+  std::vector<uint32_t> leaf(n, 0u);
+  for (uint32_t k = 0; k < sc->num_nodes; k++)
+    if ((sc->h_nodes[k].axis_leaf >> 8) && sc->h_nodes[k].id < n) leaf[sc->h_nodes[k].id] = k;
+  if (e == hipSuccess) e = hipMalloc(&sc->d_leaf, sizeof(uint32_t) * n);
+  if (e == hipSuccess)
+    e = hipMemcpy(sc->d_leaf, leaf.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess && sc->mixed) {
+    sc->mx = new (std::nothrow) MixedScratch();
+    if (!sc->mx) e = hipErrorOutOfMemory;
+  }
   if (e == hipSuccess)
     e = hipMemcpy(sc->d_nodes, sc->h_nodes.data(), sizeof(TlasNode) * sc->num_nodes,
                   hipMemcpyHostToDevice);
@@ -549,6 +766,11 @@ void scene_free(Scene *sc) {
   if (sc->d_nodes) (void)hipFree(sc->d_nodes);
   if (sc->d_inst) (void)hipFree(sc->d_inst);
   if (sc->d_counter) (void)hipFree(sc->d_counter);
+  if (sc->d_leaf) (void)hipFree(sc->d_leaf);
+  if (sc->mx) {
+    if (sc->mx->buf) (void)hipFree(sc->mx->buf);
+    delete sc->mx;
+  }
   delete sc;
 }
 
@@ -589,8 +811,108 @@ srf_status scene_launch(const Scene *sc, const srf_launch_config &cfg, const srf
   return SRF_OK;
 }
 
+// A scene that mixes primitive kinds is traced in element passes (the one
+// kernel above inlines the triangle traversal only): the rays entering each
+// element's world box are counted (one readback, so this path synchronises
+// the stream once per chunk of rays), listed per element in the element's
+// space, traced by that element's own closest-hit launch (srf_trace_closest:
+// triangles, tetrahedra, spheres, lines or solids, exactly as a single-level
+// trace of that accelerator), and k_scene_resolve walks the top level per
+// ray in RenderGraph::Trace's order over those results. Rays go in chunks of
+// at most 2^26 / elements so the (element, ray) slot map stays within
+// 256 MB. The scratch is the scene's own; traces of one mixed scene are
+// serialised.
+srf_status scene_trace_mixed(const Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
+                             uint32_t *inst_out, hipStream_t stream) {
+  MixedScratch &mx = *sc->mx;
+  std::lock_guard<std::mutex> lk(mx.mu);
+  const uint32_t ne = sc->count;
+  const uint32_t tdepth = std::max<uint32_t>(sc->depth, 1u);
+  constexpr int kResolveBlock = 128;
+  const size_t lds = sizeof(uint32_t) * (size_t)tdepth * kResolveBlock;
+  if (lds > 64 * 1024)
+    return set_error(SRF_ERR_UNSUPPORTED, "top-level BVH %u levels deep exceeds the LDS stack",
+                     sc->depth);
+  const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(n, (1ull << 26) / ne));
+  mx.counts.assign(ne, 0u);
+  mx.off.assign(ne + 1, 0u);
+  for (uint64_t c0 = 0; c0 < n; c0 += chunk) {
+    const uint32_t cn = (uint32_t)std::min<uint64_t>(chunk, n - c0);
+    const srf_ray *cr = rays + c0;
+    const uint32_t sblocks = (cn + 255) / 256;
+    // counting pass: the cursors live in the first words of the scratch
+    size_t need = 256 + 4 * (size_t)ne * 2;
+    if (need > mx.bytes) {
+      if (mx.buf) (void)hipFree(mx.buf);
+      mx.buf = nullptr;
+      mx.bytes = 0;
+      SRF_HIP(hipMalloc(&mx.buf, need));
+      mx.bytes = need;
+    }
+    uint32_t *d_counts = reinterpret_cast<uint32_t *>(mx.buf);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, stream, d_counts, ne);
+    hipLaunchKernelGGL((k_elem_select<false>), dim3(sblocks), dim3(256), 0, stream, sc->d_nodes,
+                       sc->d_leaf, sc->d_inst, ne, cr, cn, d_counts, nullptr, nullptr, nullptr);
+    SRF_HIP(hipGetLastError());
+    SRF_HIP(hipMemcpyAsync(mx.counts.data(), d_counts, 4 * (size_t)ne, hipMemcpyDeviceToHost,
+                           stream));
+    SRF_HIP(hipStreamSynchronize(stream));
+    uint64_t total = 0;
+    for (uint32_t e = 0; e < ne; e++) {
+      mx.off[e] = (uint32_t)total;
+      total += mx.counts[e];
+    }
+    mx.off[ne] = (uint32_t)total;
+    // layout: cursors, offsets, the (element, ray) slot map, local rays, hits
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_off = up(4 * (size_t)ne), o_inv = o_off + up(4 * ((size_t)ne + 1));
+    const size_t o_ray = o_inv + up(4 * (size_t)ne * cn), o_hit = o_ray + up(32 * total);
+    need = o_hit + up(16 * total) + 256;
+    if (need > mx.bytes) {
+      (void)hipFree(mx.buf);
+      mx.buf = nullptr;
+      mx.bytes = 0;
+      SRF_HIP(hipMalloc(&mx.buf, need));
+      mx.bytes = need;
+    }
+    char *b = reinterpret_cast<char *>(mx.buf);
+    uint32_t *cursor = reinterpret_cast<uint32_t *>(b);
+    uint32_t *d_off = reinterpret_cast<uint32_t *>(b + o_off);
+    uint32_t *inv = reinterpret_cast<uint32_t *>(b + o_inv);
+    srf_ray *lrays = reinterpret_cast<srf_ray *>(b + o_ray);
+    srf_hit *lhits = reinterpret_cast<srf_hit *>(b + o_hit);
+    SRF_HIP(hipMemcpyAsync(d_off, mx.off.data(), 4 * ((size_t)ne + 1), hipMemcpyHostToDevice,
+                           stream));
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, stream, cursor, ne);
+    hipLaunchKernelGGL((k_elem_select<true>), dim3(sblocks), dim3(256), 0, stream, sc->d_nodes,
+                       sc->d_leaf, sc->d_inst, ne, cr, cn, cursor, d_off, lrays, inv);
+    SRF_HIP(hipGetLastError());
+    for (uint32_t e = 0; e < ne; e++) {
+      if (!mx.counts[e]) continue;
+      srf_status st = srf_trace_closest(sc->accels[e], lrays + mx.off[e], mx.counts[e],
+                                        lhits + mx.off[e], stream);
+      if (st != SRF_OK) return st;
+    }
+    uint32_t rblocks = (cn + kResolveBlock - 1) / kResolveBlock;
+    rblocks = (rblocks + kXcdPad - 1) / kXcdPad * kXcdPad;  // see xcd_block
+    hipLaunchKernelGGL((k_scene_resolve<kResolveBlock>), dim3(rblocks), dim3(kResolveBlock), lds,
+                       stream, sc->d_nodes, sc->num_nodes, sc->d_inst, ne, cr, cn, inv, d_off,
+                       lhits, hits + c0, inst_out ? inst_out + c0 : nullptr, tdepth);
+    SRF_HIP(hipGetLastError());
+    // the next chunk reuses the scratch
+    if (c0 + chunk < n) SRF_HIP(hipStreamSynchronize(stream));
+  }
+  return SRF_OK;
+}
+
 srf_status scene_trace(const Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
                        uint32_t *inst_out, int mode, hipStream_t stream) {
+  if (sc->mixed) {
+    if (mode != SRF_LAUNCH_TUNED && mode != SRF_LAUNCH_PLAIN && mode != SRF_LAUNCH_PERSISTENT)
+      return set_error(SRF_ERR_INVALID_ARG, "launch mode %d", mode);
+    if (n == 0) return SRF_OK;
+    return scene_trace_mixed(sc, rays, n, hits, inst_out, stream);
+  }
   srf_launch_config cfg = sc->tuned;
   if (mode == SRF_LAUNCH_PLAIN) {
     cfg.persistent = 0;
@@ -608,6 +930,19 @@ srf_status scene_trace(const Scene *sc, const srf_ray *rays, uint64_t n, srf_hit
 // SRF_LAUNCH_TUNED.
 srf_status scene_autotune(Scene *sc, const srf_ray *rays, uint64_t n, srf_hit *hits,
                           uint32_t *inst_out, hipStream_t stream, srf_launch_config *best) {
+  if (sc->mixed) {
+    // the element passes have no launch shape to choose: one timed trace
+    const auto t0 = std::chrono::steady_clock::now();
+    srf_status st = scene_trace_mixed(sc, rays, n, hits, inst_out, stream);
+    if (st != SRF_OK) return st;
+    SRF_HIP(hipStreamSynchronize(stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    srf_launch_config win{128u, 0u, 0u,
+                          (float)std::chrono::duration<double, std::milli>(t1 - t0).count()};
+    sc->tuned = win;
+    if (best) *best = win;
+    return SRF_OK;
+  }
   hipEvent_t e0, e1;
   SRF_HIP(hipEventCreate(&e0));
   hipError_t he = hipEventCreate(&e1);

@@ -454,6 +454,52 @@ def batch_scene(count=64, lo=10_000, hi=2_000_000, seed=BASE_SEED + 3, spacing=1
     return meshes, np.array(T)
 
 
+def mixed_scene(kinds=("tri", "tet", "sph", "sol", "lin"), copies=2, seed=BASE_SEED + 14,
+                spacing=1.5):
+    """A two-level scene mixing primitive kinds, `copies` elements of each:
+    "tri" a perturbed geodesic sphere (2,880 triangles, radius 0.5), "tet"
+    an 8^3-cell tetrahedral grid (3,072 tets) and "sol" an 11^3 hexahedral
+    grid (1,331 hexahedra), both scaled into the unit cube about the origin,
+    "sph" 4,000 Plummer spheres (a = 0.25, lognormal radii mu = -5.5), "lin"
+    100 random-walk polylines of 32 segments (radius 0.01). Each element
+    gets a random rotation and a place on a jittered lattice as in
+    batch_scene. Returns (elements, T): elements are dicts with "kind" and
+    the arrays the matching build takes (V, F, R, rconst, nv), T [n, 4, 4]
+    in the row-vector convention."""
+    rng = _rng(seed)
+    elems = []
+    for c in range(copies):
+        for kind in kinds:
+            s = seed + 100 * c + len(elems)
+            if kind == "tri":
+                V, F = geodesic_sphere(12, seed=s)
+                elems.append(dict(kind="tri", V=V, F=F))
+            elif kind == "tet":
+                V, F, _ = tet_grid(8, seed=s)
+                V = (V / np.float32(8.0) - np.float32(0.5)).astype(np.float32)
+                elems.append(dict(kind="tet", V=np.ascontiguousarray(V), F=F))
+            elif kind == "sol":
+                V, F, _ = solid_grid(11, nv=8, seed=s)
+                V = (V / np.float32(11.0) - np.float32(0.5)).astype(np.float32)
+                elems.append(dict(kind="sol", V=np.ascontiguousarray(V), F=F, nv=8))
+            elif kind == "sph":
+                P, R = plummer_spheres(4000, seed=s, a=0.25, mu=-5.5)
+                elems.append(dict(kind="sph", V=P, R=R, rconst=0.0))
+            else:
+                P, S, _ = polylines(100, 32, seed=s, extent=0.4)
+                elems.append(dict(kind="lin", V=P, F=S, R=None, rconst=0.01, cap=1))
+    n = len(elems)
+    side = int(np.ceil(n ** (1.0 / 3.0)))
+    T = []
+    for i in range(n):
+        cell = np.array([i % side, (i // side) % side, i // (side * side)], np.float64)
+        M = np.eye(4)
+        M[:3, :3] = _rotation(rng).T
+        M[3, :3] = spacing * (cell + rng.uniform(-0.15, 0.15, 3))
+        T.append(M)
+    return elems, np.array(T)
+
+
 def scene_cameras(T, count=16, width=1024, height=1024, fov=40.0, tiled=True):
     """`count` pinhole cameras on a horizontal orbit around the scene's
     centre, all looking at it; returns a list of ray arrays."""

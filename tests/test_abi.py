@@ -66,3 +66,47 @@ def test_cpp_host_classes_compile_against_the_header(tmp_path):
     r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", inc,
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_instance_update_equals_reference_element_matrices():
+    """srf_instance_update (host code of the library, no device call) gives
+    RenderElement::Update's invT and invT33 bit for bit: against the matrices
+    the reference's own RenderGraph computed for rigid, scaled and sheared
+    transforms, and against the closed-form inverse of a translation."""
+    import numpy as np
+
+    from oracle.oracle import Reference, ref_available
+    from surface_amd import _lib
+    from surface_amd import workloads as W
+    lib = _lib.load()
+
+    def update(T):
+        inst = _lib.Instance()
+        inst.T[:] = list(np.asarray(T, np.float64).reshape(16))
+        assert lib.srf_instance_update(C.byref(inst)) == 0
+        return np.array(inst.invT[:]).reshape(4, 4), np.array(inst.invT33[:]).reshape(4, 4)
+
+    M = np.eye(4)
+    M[3, :3] = [2.0, -3.0, 0.5]
+    invT, invT33 = update(M)
+    want = np.eye(4)
+    want[3, :3] = [-2.0, 3.0, -0.5]
+    assert np.array_equal(invT[:, :3], want[:, :3]) and np.array_equal(invT33, np.eye(4))
+    if not ref_available():
+        pytest.skip("reference build not present")
+    ref = Reference("serial")
+    meshes, T = W.batch_scene(count=8, lo=1100, hi=1500, seed=W.BASE_SEED + 40)
+    rng = np.random.default_rng(3)
+    for i in range(0, 8, 2):  # scale and shear every other element
+        S = np.eye(4)
+        S[:3, :3] = np.diag(rng.uniform(0.3, 3.0, 3)) + np.triu(rng.uniform(-0.4, 0.4, (3, 3)), 1)
+        T[i] = S @ T[i]
+    trees = [ref.build("tri", V, F, keep=True) for V, F in meshes]
+    sc = ref.scene_build(trees, T)
+    for i in range(len(T)):
+        invT, invT33 = update(T[i])
+        assert invT.tobytes() == sc["invT"][i].tobytes(), i
+        assert invT33.tobytes() == sc["invT33"][i].tobytes(), i
+    ref.scene_free(sc)
+    for t in trees:
+        ref.free(t)

@@ -1,7 +1,8 @@
 """Two-level scenes on the GPU (srf_scene_*): the top-level BVH and every
 ray's nearest element hit bit-exact against the reference's RenderGraph
 (fixture) and against the oracle on larger scenes, for the plain and the
-persistent (work-counter) launch."""
+persistent (work-counter) launch, and for scenes that mix triangle, tetra,
+particle, solid and line elements."""
 from __future__ import annotations
 
 import hashlib
@@ -97,13 +98,55 @@ def test_scene_matches_oracle(S, orc, count):
         assert (want_i != W.NO_PRIM).mean() > 0.05
 
 
-def test_scene_rejects_non_triangle_elements(S):
-    from surface_amd._lib import SrfError
-    V, F, _ = W.tet_grid(6)  # 1,296 tets
-    tet = S.TetraAccel()
-    tet.Build32(S.device_tensor(V), S.device_tensor(F))
-    with pytest.raises(SrfError):
-        S.Scene([tet], np.eye(4)[None])
+def _gpu_element(S, e):
+    k = e["kind"]
+    if k == "tri":
+        a = S.TriangleAccel()
+        a.Build32(S.device_tensor(e["V"]), S.device_tensor(e["F"]))
+    elif k == "tet":
+        a = S.TetraAccel()
+        a.Build32(S.device_tensor(e["V"]), S.device_tensor(e["F"]))
+    elif k == "sph":
+        a = S.ParticleAccel()
+        a.Build32(S.device_tensor(e["V"]), S.device_tensor(e["R"]), e.get("rconst", 0.0))
+    elif k == "sol":
+        a = S.SolidAccel()
+        a.Build32(S.device_tensor(e["V"]), S.device_tensor(e["F"]), e["nv"])
+    else:
+        a = S.LineAccel()
+        a.Build(S.device_tensor(e["V"]), S.device_tensor(e["F"]), None, e["rconst"],
+                cap=bool(e.get("cap", 1)))
+    return a
+
+
+@pytest.mark.parametrize("copies", [1, 2])
+def test_mixed_scene_matches_oracle(S, orc, copies):
+    """RenderGraph::Trace over triangle, tetra, particle, hexahedron and line
+    elements (the element passes): every ray's world hit and element equal
+    the oracle's mixed walk, whose walk equals the reference's RenderGraph
+    (tests/test_oracle.py); the launch modes agree."""
+    elems, T = W.mixed_scene(copies=copies)
+    say(f"mixed scene of {len(elems)} elements vs oracle")
+    invT, invT33 = S.rigid_inverses(T)
+    accs = [_gpu_element(S, e) for e in elems]
+    sc = S.Scene(accs, T, invT, invT33)
+    trees = [orc.build_element(e) for e in elems]
+    xb = np.array([orc.transform_box(t["bounds"][:3], t["bounds"][3:], T[i])
+                   for i, t in enumerate(trees)])
+    tl = orc.tlas_build(xb)
+    assert canonical_tlas(sc.export_tlas()).tobytes() == canonical_tlas(tl).tobytes()
+    els = [dict(e, tree=trees[i], T=T[i], invT=invT[i], invT33=invT33[i])
+           for i, e in enumerate(elems)]
+    kinds = set()
+    for k, cam in enumerate(W.scene_cameras(T, count=2, width=128, height=128)):
+        want_h, want_i = orc.scene_trace_mixed(tl, els, cam, threads=8)
+        rays = S.device_tensor(cam)
+        for mode in ("plain", "persistent", "tuned"):
+            got_h, got_i = _hits(S, *sc.Trace(rays, mode=mode))
+            assert got_h.tobytes() == want_h.tobytes(), mode
+            assert np.array_equal(got_i, want_i), mode
+        kinds |= {elems[j]["kind"] for j in np.unique(want_i[want_i != W.NO_PRIM])}
+    assert kinds == {"tri", "tet", "sph", "sol", "lin"}
 
 
 def test_scene_trace_in_hip_graph_replays_match_eager(S):

@@ -658,6 +658,105 @@ def test_scene_oracle_equals_live_reference(orc):
         ref.free(t)
 
 
+def _mixed_trees(orc, elems):
+    return [orc.build_element(e) for e in elems]
+
+
+def _mixed_elements(elems, trees, T, invT, invT33):
+    return [dict(e, tree=trees[i], T=T[i], invT=invT[i], invT33=invT33[i])
+            for i, e in enumerate(elems)]
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference build not present")
+def test_mixed_scene_oracle_equals_live_reference(orc):
+    """RenderGraph::Trace over triangle, tetra, particle and hexahedron
+    elements (the kinds whose trees the oracle reproduces; the reference's
+    line tree is its own spatial-median one): the oracle's mixed walk equals
+    the reference's, hit for hit."""
+    from oracle.oracle import Reference, canonical_tlas
+    ref = Reference("serial")
+    elems, T = W.mixed_scene(kinds=("tri", "tet", "sph", "sol"), copies=2)
+    rtrees = []
+    for e in elems:
+        rtrees.append(ref.build(e["kind"], e["V"], e.get("F"), R=e.get("R"),
+                                rconst=e.get("rconst", 0.0), keep=True, nv=e.get("nv", 0)))
+    sc = ref.scene_build(rtrees, T)
+    trees = _mixed_trees(orc, elems)
+    for t, r in zip(trees, rtrees):
+        assert np.array_equal(t["bounds"], r["bounds"])
+    els = _mixed_elements(elems, trees, T, sc["invT"], sc["invT33"])
+    xb = np.array([orc.transform_box(t["bounds"][:3], t["bounds"][3:], T[i])
+                   for i, t in enumerate(trees)])
+    assert np.array_equal(xb, sc["xbox"])
+    tl = orc.tlas_build(xb)
+    assert canonical_tlas(tl).tobytes() == canonical_tlas(sc["tlas"]).tobytes()
+    hit_kinds = set()
+    is_sol = np.array([e["kind"] == "sol" for e in elems] + [False])
+    for r in W.scene_cameras(T, count=3, width=96, height=96):
+        h1, i1 = ref.scene_trace(sc, r)
+        h2, i2 = orc.scene_trace_mixed(tl, els, r, threads=4)
+        assert np.array_equal(i1, i2)
+        assert h1["t"].tobytes() == h2["t"].tobytes()
+        assert np.array_equal(h1["prim_id"], h2["prim_id"])
+        # SolidAccel::Traverse never writes u and v (the reference returns
+        # whatever its Intersection held); every other kind's are exact
+        sol = is_sol[np.minimum(i2, len(elems))]
+        assert h1[~sol].tobytes() == h2[~sol].tobytes()
+        hit_kinds |= {elems[k]["kind"] for k in np.unique(i2[i2 != W.NO_PRIM])}
+    assert hit_kinds == {"tri", "tet", "sph", "sol"}
+    ref.scene_free(sc)
+    for t in rtrees:
+        ref.free(t)
+
+
+def test_mixed_scene_trace_equals_bruteforce_over_elements(orc):
+    """Every kind, lines included: the ordered, culled walk gives the
+    nearest world hit over all elements, each traced on its own."""
+    elems, T = W.mixed_scene()
+    invT = np.linalg.inv(T)
+    invT33 = np.array([np.linalg.inv(np.block([[M[:3, :3], np.zeros((3, 1))],
+                                               [np.zeros((1, 3)), np.ones((1, 1))]]))
+                       for M in T])
+    trees = _mixed_trees(orc, elems)
+    els = _mixed_elements(elems, trees, T, invT, invT33)
+    xb = np.array([orc.transform_box(t["bounds"][:3], t["bounds"][3:], T[i])
+                   for i, t in enumerate(trees)])
+    tl = orc.tlas_build(xb)
+    rays = W.scene_cameras(T, count=1, width=64, height=64)[0]
+    hits, inst = orc.scene_trace_mixed(tl, els, rays)
+    best = np.full(len(rays), np.inf)
+    o = rays["org"].astype(np.float64)
+    d = rays["dir"].astype(np.float64)
+    for e in els:
+        lr = rays.copy()
+        lo = o @ e["invT"][:3, :3] + e["invT"][3, :3]
+        ld = d @ e["invT33"][:3, :3] + e["invT33"][3, :3]
+        lr["org"], lr["dir"] = lo.astype(np.float32), ld.astype(np.float32)
+        lr["prev_prim_id"] = W.NO_PRIM
+        k = e["kind"]
+        if k == "tri":
+            h = orc.trace_tri(e["tree"], e["V"], e["F"], lr)
+        elif k == "tet":
+            h = orc.trace_tet(e["tree"], e["V"], e["F"], lr)[0]
+        elif k == "sph":
+            h = orc.trace_sph(e["tree"], e["V"], e["R"], e["rconst"], lr)
+        elif k == "sol":
+            h = orc.trace_sol(e["tree"], e["V"], e["F"], lr)[0]
+        else:
+            h = orc.trace_lin(e["tree"], e["V"], e["F"], e.get("R"), e["rconst"], lr)[0]
+        ok = h["prim_id"] != W.NO_PRIM
+        lp = (lo + h["t"].astype(np.float64)[:, None] * ld).astype(np.float32).astype(np.float64)
+        wp = (lp @ e["T"][:3, :3] + e["T"][3, :3]).astype(np.float32)
+        with np.errstate(over="ignore", invalid="ignore"):
+            tw = np.sqrt(np.sum((wp - rays["org"]) ** 2, axis=1, dtype=np.float32))
+        best = np.where(ok & (tw < best), tw, best)
+    hit = inst != W.NO_PRIM
+    assert hit.mean() > 0.1
+    assert {elems[k]["kind"] for k in np.unique(inst[hit])} == {"tri", "tet", "sph", "sol", "lin"}
+    assert np.array_equal(np.isfinite(best), hit)
+    assert np.allclose(hits["t"][hit], best[hit], rtol=1e-6, atol=0)
+
+
 # ------------------------------------------------------------------ solids
 
 SOL_NV = (8, 6, 5)
