@@ -177,6 +177,53 @@ class FrameTracer {
   // do; pixels whose ray misses (or is discarded) are left untouched.
   bool Render(const srf_accel *accel, const Camera &cam, int width, int height,
               const Shader &shade, float *rgba, float *depth, void *stream = nullptr) {
+    return render(
+        cam, width, height,
+        [&](const srf_ray *r, uint64_t n, srf_hit *h) {
+          return srf_trace_closest(accel, r, n, h, stream) == SRF_OK;
+        },
+        [&](int x, int y, size_t i, float c[4]) { return shade(x, y, rays_[i], hits_[i], c); },
+        rgba, depth, stream);
+  }
+
+  // The same frame over a two-level scene (RenderGraph::Trace per eye ray,
+  // as OnData does for a frame of several draws): hits are world-space and
+  // the shader also receives the element hit.
+  using SceneShader = std::function<bool(int x, int y, const srf_ray &ray, const srf_hit &hit,
+                                         uint32_t element, float rgba[4])>;
+  bool RenderScene(const srf_scene *scene, const Camera &cam, int width, int height,
+                   const SceneShader &shade, float *rgba, float *depth,
+                   void *stream = nullptr) {
+    const size_t n = (size_t)width * height;
+    if (!delem_.ensure(n * sizeof(uint32_t))) return false;
+    elems_.resize(n);
+    return render(
+        cam, width, height,
+        [&](const srf_ray *r, uint64_t cnt, srf_hit *h) {
+          hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+          return srf_scene_trace(scene, r, cnt, h, (uint32_t *)delem_.p, SRF_LAUNCH_TUNED,
+                                 stream) == SRF_OK &&
+                 hipMemcpyAsync(elems_.data(), delem_.p, cnt * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, s) == hipSuccess;
+        },
+        [&](int x, int y, size_t i, float c[4]) {
+          return shade(x, y, rays_[i], hits_[i], elems_[i], c);
+        },
+        rgba, depth, stream);
+  }
+
+  double RenderMs() const { return ms_; }
+  // million eye rays per second of the last frame, the figure
+  // RaytraceEngine::OnData prints
+  double Mraycasts() const { return ms_ > 0.0 ? (double)rays_traced_ / (ms_ * 1e3) : 0.0; }
+  const std::vector<srf_hit> &Hits() const { return hits_; }
+
+ private:
+  // Generate, trace (trace(rays, n, hits) on device arrays, enqueued on
+  // stream), read back, shade (shade_at(x, y, pixel, rgba)).
+  template <class TraceFn, class ShadeFn>
+  bool render(const Camera &cam, int width, int height, const TraceFn &trace,
+              const ShadeFn &shade_at, float *rgba, float *depth, void *stream) {
     const auto t0 = std::chrono::steady_clock::now();
     const size_t n = (size_t)width * height;
     rays_.resize(n);
@@ -188,9 +235,7 @@ class FrameTracer {
     if (hipMemcpyAsync(dray_.p, rays_.data(), n * sizeof(srf_ray), hipMemcpyHostToDevice, s) !=
         hipSuccess)
       return false;
-    if (srf_trace_closest(accel, (const srf_ray *)dray_.p, n, (srf_hit *)dhit_.p, stream) !=
-        SRF_OK)
-      return false;
+    if (!trace((const srf_ray *)dray_.p, n, (srf_hit *)dhit_.p)) return false;
     if (hipMemcpyAsync(hits_.data(), dhit_.p, n * sizeof(srf_hit), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
@@ -201,7 +246,7 @@ class FrameTracer {
         const size_t i = (size_t)y * width + x;
         if (!valid_[i] || hits_[i].prim_id == SRF_NO_PRIM) continue;
         float col[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (!shade(x, y, rays_[i], hits_[i], col)) continue;
+        if (!shade_at(x, y, i, col)) continue;
         const size_t o = (size_t)(height - y - 1) * width + x;
         for (int k = 0; k < 4; k++) rgba[4 * o + k] = col[k];
         if (depth) depth[o] = hits_[i].t;
@@ -213,17 +258,11 @@ class FrameTracer {
     return true;
   }
 
-  double RenderMs() const { return ms_; }
-  // million eye rays per second of the last frame, the figure
-  // RaytraceEngine::OnData prints
-  double Mraycasts() const { return ms_ > 0.0 ? (double)rays_traced_ / (ms_ * 1e3) : 0.0; }
-  const std::vector<srf_hit> &Hits() const { return hits_; }
-
- private:
   std::vector<srf_ray> rays_;
   std::vector<uint8_t> valid_;
   std::vector<srf_hit> hits_;
-  frame_detail::DevBuf dray_, dhit_;
+  std::vector<uint32_t> elems_;
+  frame_detail::DevBuf dray_, dhit_, delem_;
   double ms_ = 0.0;
   size_t rays_traced_ = 0;
 };

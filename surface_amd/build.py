@@ -64,7 +64,8 @@ def build_cpp_test(verbose=False):
     oracle = os.path.join(root, "oracle")
     outs = []
     for name, extra in (("test_accel_cpp", ["-pthread"]),
-                        ("test_frame_cpp", ["-fopenmp", "-pthread"])):
+                        ("test_frame_cpp", ["-fopenmp", "-pthread"]),
+                        ("test_gl_cpp", ["-fopenmp", "-pthread"])):
         src = os.path.join(root, "tests", "cpp", name + ".cpp")
         out = os.path.join(root, "tests", "cpp", name)
         cmd = [hipcc(), "--offload-arch=gfx950", "-O2", "-std=c++17", *extra, "-o", out, src,

@@ -27,3 +27,14 @@ def test_cpp_frame_loop_and_ray_queues():
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stderr
     assert "PASS" in r.stdout
+
+
+GL = os.path.join(os.path.dirname(__file__), "cpp", "test_gl_cpp")
+
+
+def test_cpp_gl_draw_path_over_mixed_elements():
+    assert os.path.exists(GL), "tests/cpp/test_gl_cpp not built (run __graft_entry__.build())"
+    r = subprocess.run([GL], capture_output=True, text=True, timeout=180)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    assert "PASS" in r.stdout
