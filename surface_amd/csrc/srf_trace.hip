@@ -38,6 +38,22 @@
 namespace srf {
 namespace {
 
+// A node read by a whole wavefront (uniform index: scalar loads).
+struct WaveNode {
+  float4 n0, n1, n2;
+  uint4 meta;
+};
+__device__ __forceinline__ WaveNode load_wave_node(const float4 *__restrict__ nodes,
+                                                   uint32_t idx) {
+  const float4 *p = nodes + 4 * (size_t)idx;
+  WaveNode r;
+  r.n0 = p[0];
+  r.n1 = p[1];
+  r.n2 = p[2];
+  r.meta = reinterpret_cast<const uint4 *>(p)[3];
+  return r;
+}
+
 // kSmall: node indices below 2^24, the 3-byte stack
 template <bool kAny, bool kSmall>
 __global__ void __launch_bounds__(kTraceBlock)
@@ -448,10 +464,12 @@ __device__ __forceinline__ void sph_big_leaf_wave(const SphSub &sub, uint32_t nb
   uint64_t mask = all;
   bool kept = false;  // `best` is one of this leaf's spheres
   const uint32_t nnodes = 2 * count - 1;
+  // a sphere's own id matters only to a lane with a previous hit
+  const bool any_prev = __ballot(in && prev != SRF_NO_PRIM) != 0;
   uint32_t cur = 0;
   int wsp = 0;
+  WaveNode nd;
   while (mask) {
-    uint4 meta = make_uint4(0u, 0u, 0u, 0u);
     bool leaf = false, bad = false;
     for (;;) {
       cur = __builtin_amdgcn_readfirstlane(cur);
@@ -459,23 +477,21 @@ __device__ __forceinline__ void sph_big_leaf_wave(const SphSub &sub, uint32_t nb
         bad = true;
         break;
       }
-      const float4 *nd = sub.nodes + 4 * (size_t)(nbase + cur);
-      const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
-      meta = reinterpret_cast<const uint4 *>(nd)[3];
-      keep_loaded(n0, n1, n2);
-      if (meta.x != 0) {
+      nd = load_wave_node(sub.nodes, nbase + cur);
+      if (nd.meta.x != 0) {
         leaf = true;
         break;
       }
+      const uint32_t c0 = nd.meta.z, c1 = nd.meta.w;
       const bool act = (mask & me) != 0 && !(kAny && occ);
       float l0 = 0.0f, l1 = 0.0f;
-      bool h0 = act && sub_box(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, l0);
-      bool h1 = act && sub_box(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, l1);
+      bool h0 = act && sub_box(s, nd.n0.x, nd.n0.y, nd.n0.z, nd.n1.z, nd.n1.w, nd.n2.x, l0);
+      bool h1 = act && sub_box(s, nd.n0.w, nd.n1.x, nd.n1.y, nd.n2.y, nd.n2.z, nd.n2.w, l1);
       h0 = h0 && (kAny ? l0 < tlimit : l0 <= t);
       h1 = h1 && (kAny ? l1 < tlimit : l1 <= t);
-      const int nearc = (sb >> meta.y) & 1u;
+      const int nearc = (sb >> nd.meta.y) & 1u;
       const uint64_t nm = __ballot(nearc ? h1 : h0), fm = __ballot(nearc ? h0 : h1);
-      const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+      const uint32_t cn = nearc ? c1 : c0, cf = nearc ? c0 : c1;
       if (nm) {
         if (fm && wsp < kTraceStack) {
           wn[wsp] = cf;
@@ -494,26 +510,39 @@ __device__ __forceinline__ void sph_big_leaf_wave(const SphSub &sub, uint32_t nb
     if (bad) break;
     if (leaf) {
       const bool act = (mask & me) != 0;
-      const uint32_t k0 = __builtin_amdgcn_readfirstlane(meta.w);
-      const uint32_t k1 = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, count));
-      for (uint32_t k = k0; k < k1; k++) {
-        const uint32_t j = pbase + k;
-        if (j >= sub.nprims) break;
-        const float4 c = sub.prims[j];
-        const uint32_t loc = sub.sidx[j];
-        if (loc >= count) continue;
-        const uint32_t q = first + loc;
-        const uint32_t sid = sorted_index[q];
-        if (!act || (kAny && occ)) continue;
-        const bool self = (prev != SRF_NO_PRIM) && (sid == prev);
-        float tt;
-        if (!sphere_isect(s, c, INFINITY, prevT, prevN, self, tt)) continue;
-        if (kAny) {
-          if (tt < tlimit) occ = true;
-        } else if (tt < t || (tt == t && kept && q < best)) {
-          t = tt;
-          best = q;
-          kept = true;
+      const uint32_t k0 = __builtin_amdgcn_readfirstlane(nd.meta.w);
+      const uint32_t k1 = __builtin_amdgcn_readfirstlane(min(nd.meta.w + nd.meta.z, count));
+      // the leaf's spheres two at a time (order and rules unchanged)
+      for (uint32_t k = k0; k < k1; k += 2) {
+        float4 c[2];
+        uint32_t loc[2];
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) {
+          // past the leaf's end (or the array's) the last entry is loaded again, unused
+          uint32_t j = pbase + min(k + (uint32_t)jj, k1 - 1);
+          j = j < sub.nprims ? j : sub.nprims - 1;
+          c[jj] = sub.prims[j];
+          loc[jj] = sub.sidx[j];
+        }
+        keep_loaded_s2(c[0], c[1]);
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) {
+          const uint32_t kk = k + (uint32_t)jj;
+          if (kk >= k1 || pbase + kk >= sub.nprims) break;
+          if (loc[jj] >= count) continue;
+          const uint32_t q = first + loc[jj];
+          const uint32_t sid = any_prev ? sorted_index[q] : SRF_NO_PRIM;
+          if (!act || (kAny && occ)) continue;
+          const bool self = (prev != SRF_NO_PRIM) && (sid == prev);
+          float tt;
+          if (!sphere_isect(s, c[jj], INFINITY, prevT, prevN, self, tt)) continue;
+          if (kAny) {
+            if (tt < tlimit) occ = true;
+          } else if (tt < t || (tt == t && kept && q < best)) {
+            t = tt;
+            best = q;
+            kept = true;
+          }
         }
       }
     }
@@ -634,8 +663,8 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
     for (int k = 0; k < 3; k++) prevN[k] = prevs[i].prev_hit_normal[k];
   }
   const float tlimit = kAny ? (tmax_in ? tmax_in[i] : FLT_MAX) : FLT_MAX;
-  float t;
-  uint32_t best;
+  float t = FLT_MAX;
+  uint32_t best = SRF_NO_PRIM;
   bool occ = false;
   const uint32_t sb = __builtin_amdgcn_readfirstlane(s.sbits);
   const bool uniform = __all(s.sbits == sb);
@@ -657,25 +686,32 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
     uint64_t mask = __ballot(1);
     uint32_t cur = 0;
     int wsp = 0;
+    // the descent's node in scalar registers, both children requested before
+    // its boxes are tested, so the next node of the descent is in registers
+    // by the time the tests have picked it
+    WaveNode nd;
+    bool have = false;
     for (;;) {
-      uint4 meta = make_uint4(0u, 0u, 0u, 0u);
       bool leaf = false;
       while (cur < max_nodes) {
         cur = __builtin_amdgcn_readfirstlane(cur);
-        const float4 *nd = nodes + 4 * (size_t)cur;
-        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
-        meta = reinterpret_cast<const uint4 *>(nd)[3];
-        keep_loaded(n0, n1, n2);
-        if (meta.x != 0) {
+        if (!have) nd = load_wave_node(nodes, cur);
+        have = false;
+        if (nd.meta.x != 0) {
           leaf = true;
           break;
         }
+        const uint32_t c0 = nd.meta.z, c1 = nd.meta.w;
+        const WaveNode p0 = load_wave_node(nodes, c0 < max_nodes ? c0 : 0u);
+        const WaveNode p1 = load_wave_node(nodes, c1 < max_nodes ? c1 : 0u);
         const bool in = (mask & me) != 0;
-        const bool h0 = in && ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
-        const bool h1 = in && ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
-        const int nearc = (sb >> meta.y) & 1u;
+        const bool h0 = in && ray_aabb<true>(s, t, nd.n0.x, nd.n0.y, nd.n0.z, nd.n1.z, nd.n1.w,
+                                             nd.n2.x);
+        const bool h1 = in && ray_aabb<true>(s, t, nd.n0.w, nd.n1.x, nd.n1.y, nd.n2.y, nd.n2.z,
+                                             nd.n2.w);
+        const int nearc = (sb >> nd.meta.y) & 1u;
         const uint64_t nm = __ballot(nearc ? h1 : h0), fm = __ballot(nearc ? h0 : h1);
-        const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+        const uint32_t cn = nearc ? c1 : c0, cf = nearc ? c0 : c1;
         if (nm) {
           if (fm && wsp < kTraceStack) {
             wnode[wv][wsp] = cf;
@@ -690,13 +726,15 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
         } else {
           break;
         }
+        nd = cur == c0 ? p0 : p1;
+        have = cur < max_nodes;
       }
+      const uint4 meta = nd.meta;
       if (leaf) {
         const bool in = (mask & me) != 0;
         if (meta.z > kBigLeaf && sub.nodes) {
-          const float4 *nd = nodes + 4 * (size_t)cur;
-          const uint32_t nbase = __builtin_amdgcn_readfirstlane(__float_as_uint(nd[0].w));
-          const uint32_t pbase = __builtin_amdgcn_readfirstlane(__float_as_uint(nd[1].x));
+          const uint32_t nbase = __builtin_amdgcn_readfirstlane(__float_as_uint(nd.n0.w));
+          const uint32_t pbase = __builtin_amdgcn_readfirstlane(__float_as_uint(nd.n1.x));
           if (meta.w < nprims && meta.z <= nprims - meta.w) {
             bool unused = false;
             sph_big_leaf_wave<false>(sub, nbase, pbase, meta.w, meta.z, sorted_index, s, sb, prev,
@@ -704,17 +742,29 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
                                      &wmask[wv][kTraceStack], t, best, unused);
           }
         } else {
+          // the leaf's spheres in order, loaded two at a time
           const uint32_t q0 = __builtin_amdgcn_readfirstlane(meta.w);
           const uint32_t q1 = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, nprims));
-          for (uint32_t q = q0; q < q1; q++) {
-            const float4 c = prims[q];
-            const uint32_t sid = sorted_index[q];
-            if (!in) continue;
-            const bool self = (prev != SRF_NO_PRIM) && (sid == prev);
-            float tt;
-            if (sphere_isect(s, c, t, prevT, prevN, self, tt)) {
-              t = tt;
-              best = q;
+          for (uint32_t q = q0; q < q1; q += 2) {
+            float4 c[2];
+            uint32_t sid[2];
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+              const uint32_t qq = min(q + (uint32_t)j, q1 - 1);  // past the end: unused
+              c[j] = prims[qq];
+              sid[j] = sorted_index[qq];
+            }
+            keep_loaded_s2(c[0], c[1]);
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+              if (q + (uint32_t)j >= q1) break;
+              if (!in) continue;
+              const bool self = (prev != SRF_NO_PRIM) && (sid[j] == prev);
+              float tt;
+              if (sphere_isect(s, c[j], t, prevT, prevN, self, tt)) {
+                t = tt;
+                best = q + (uint32_t)j;
+              }
             }
           }
         }
@@ -723,6 +773,7 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
       --wsp;
       cur = __builtin_amdgcn_readfirstlane(wnode[wv][wsp]);
       mask = wmask[wv][wsp];
+      have = false;
     }
   } else if constexpr (kLanePath) {
     occ = sph_traverse<kAny, kTraceBlock>(nodes, prims, sorted_index, max_nodes, nprims, sub, s,

@@ -125,6 +125,13 @@ __device__ __forceinline__ void keep_loaded(const float4 &a, const float4 &b, co
                "v"(b.w), "v"(c.x), "v"(c.y), "v"(c.z), "v"(c.w));
 }
 
+// The same for two values loaded through the scalar unit (wavefront-uniform
+// addresses): the loads are issued together and waited for once.
+__device__ __forceinline__ void keep_loaded_s2(const float4 &a, const float4 &b) {
+  asm volatile("" ::"s"(a.x), "s"(a.y), "s"(a.z), "s"(a.w), "s"(b.x), "s"(b.y), "s"(b.z),
+               "s"(b.w));
+}
+
 // A lane's pending far children in LDS, kStride entries apart (one column
 // per lane, so a wavefront's accesses never share a bank row twice).
 template <int kStride>
