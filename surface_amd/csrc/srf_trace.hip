@@ -98,14 +98,16 @@ k_trace_tri(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
         const float4 *nd = nodes + 4 * (size_t)cur;
         const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
         meta = reinterpret_cast<const uint4 *>(nd)[3];
-        keep_loaded(n0, n1, n2);
+        keep_loaded_s(n0, n1, n2);
         if (meta.x != 0) {
           leaf = true;
           break;
         }
         const bool in = (mask & me) != 0 && !occ;
-        const bool h0 = in && ray_aabb<true>(s, tlimit, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
-        const bool h1 = in && ray_aabb<true>(s, tlimit, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
+        const bool h0 =
+            in && ray_aabb_wave<true>(s, sb, tlimit, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
+        const bool h1 =
+            in && ray_aabb_wave<true>(s, sb, tlimit, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
         const int nearc = (sb >> meta.y) & 1u;
         const uint64_t nm = __ballot(nearc ? h1 : h0), fm = __ballot(nearc ? h0 : h1);
         const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
@@ -156,6 +158,85 @@ k_trace_tri(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   }
 }
 
+// The wavefront traversal of k_trace_tri_wave for rays whose direction signs
+// are all SB (sign[k] << k): with the signs fixed at compile time the box
+// tests choose their near and far planes without per-lane selects.
+template <uint32_t SB>
+__device__ __forceinline__ void tri_wave_closest(const float4 *__restrict__ nodes,
+                                              const float4 *__restrict__ prims,
+                                              uint32_t max_nodes, uint32_t nprims,
+                                              const RaySetup &s, bool ds, float kEPS,
+                                              uint32_t prev, uint32_t *wnode, uint64_t *wmask,
+                                              float &t, float &u, float &v, uint32_t &prim) {
+  constexpr uint32_t sb = SB;
+  const bool filter = kEPS >= 1e-30f;
+  const uint64_t me = 1ull << (threadIdx.x & 63u);
+  uint64_t mask = __ballot(1);
+  uint32_t cur = 0;
+  int sp = 0;
+  for (;;) {
+    // descend while some lane's traversal goes on below this node
+    uint4 meta = make_uint4(0u, 0u, 0u, 0u);
+    bool leaf = false;
+    while (cur < max_nodes) {
+      cur = __builtin_amdgcn_readfirstlane(cur);
+      const float4 *nd = nodes + 4 * (size_t)cur;
+      const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+      meta = reinterpret_cast<const uint4 *>(nd)[3];
+      keep_loaded_s(n0, n1, n2);
+      if (meta.x != 0) {
+        leaf = true;
+        break;
+      }
+      const bool in = (mask & me) != 0;
+      const bool h0 = in && ray_aabb_wave<true>(s, SB, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
+      const bool h1 = in && ray_aabb_wave<true>(s, SB, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
+      const int nearc = (sb >> meta.y) & 1u;
+      const uint64_t nm = __ballot(nearc ? h1 : h0), fm = __ballot(nearc ? h0 : h1);
+      const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
+      if (nm) {
+        // the lanes hitting both boxes pushed the far child; those hitting
+        // only the far one go there next, after nothing of theirs
+        if (fm && sp < kTraceStack) {
+          wnode[sp] = cf;
+          wmask[sp] = fm;
+          sp++;
+        }
+        cur = cn;
+        mask = nm;
+      } else if (fm) {
+        cur = cf;
+        mask = fm;
+      } else {
+        break;
+      }
+    }
+    if (leaf) {
+      // TestLeafNode, render/render_accel_triangle.cc:2386-2433
+      const bool in = (mask & me) != 0;
+      const uint32_t first = __builtin_amdgcn_readfirstlane(meta.w),
+                     end = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, nprims));
+      for (uint32_t q = first; q < end; q++) {
+        const float4 *P = prims + 3 * (size_t)q;
+        const float4 A = P[0], B = P[1], C = P[2];
+        if (in && __float_as_uint(A.w) != prev) {
+          float tt, uu, vv;
+          if (tri_isect(s, A, B, C, ds, kEPS, filter, t, tt, uu, vv)) {
+            t = tt;
+            u = uu;
+            v = vv;
+            prim = __float_as_uint(A.w);
+          }
+        }
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    cur = __builtin_amdgcn_readfirstlane(wnode[sp]);
+    mask = wmask[sp];
+  }
+}
+
 // Closest hit with one traversal per wavefront. When every ray of the
 // wavefront has the same direction signs, the reference's per-ray order
 // (near child by the split axis's sign, far child pushed when both boxes are
@@ -197,73 +278,26 @@ k_trace_tri_wave(const float4 *__restrict__ nodes, const float4 *__restrict__ pr
       tri_traverse<false>(nodes, prims, max_nodes, nprims, s, ds, kEPS, prev, FLT_MAX,
                           Stack32<kTraceBlock>{stack + threadIdx.x}, t, u, v, prim);
   } else {
-    const bool filter = kEPS >= 1e-30f;
-    const int w = threadIdx.x >> 6;
-    const uint64_t me = 1ull << (threadIdx.x & 63u);
-    uint64_t mask = __ballot(1);
-    uint32_t cur = 0;
-    int sp = 0;
-    for (;;) {
-      // descend while some lane's traversal goes on below this node
-      uint4 meta = make_uint4(0u, 0u, 0u, 0u);
-      bool leaf = false;
-      while (cur < max_nodes) {
-        cur = __builtin_amdgcn_readfirstlane(cur);
-        const float4 *nd = nodes + 4 * (size_t)cur;
-        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
-        meta = reinterpret_cast<const uint4 *>(nd)[3];
-        keep_loaded(n0, n1, n2);
-        if (meta.x != 0) {
-          leaf = true;
-          break;
-        }
-        const bool in = (mask & me) != 0;
-        const bool h0 = in && ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
-        const bool h1 = in && ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
-        const int nearc = (sb >> meta.y) & 1u;
-        const uint64_t nm = __ballot(nearc ? h1 : h0), fm = __ballot(nearc ? h0 : h1);
-        const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
-        if (nm) {
-          // the lanes hitting both boxes pushed the far child; those hitting
-          // only the far one go there next, after nothing of theirs
-          if (fm && sp < kTraceStack) {
-            wnode[w][sp] = cf;
-            wmask[w][sp] = fm;
-            sp++;
-          }
-          cur = cn;
-          mask = nm;
-        } else if (fm) {
-          cur = cf;
-          mask = fm;
-        } else {
-          break;
-        }
-      }
-      if (leaf) {
-        // TestLeafNode, render/render_accel_triangle.cc:2386-2433
-        const bool in = (mask & me) != 0;
-        const uint32_t first = __builtin_amdgcn_readfirstlane(meta.w),
-                       end = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, nprims));
-        for (uint32_t q = first; q < end; q++) {
-          const float4 *P = prims + 3 * (size_t)q;
-          const float4 A = P[0], B = P[1], C = P[2];
-          if (in && __float_as_uint(A.w) != prev) {
-            float tt, uu, vv;
-            if (tri_isect(s, A, B, C, ds, kEPS, filter, t, tt, uu, vv)) {
-              t = tt;
-              u = uu;
-              v = vv;
-              prim = __float_as_uint(A.w);
-            }
-          }
-        }
-      }
-      if (sp == 0) break;
-      --sp;
-      cur = __builtin_amdgcn_readfirstlane(wnode[w][sp]);
-      mask = wmask[w][sp];
+    uint32_t *wn = wnode[threadIdx.x >> 6];
+    uint64_t *wm = wmask[threadIdx.x >> 6];
+#define SRF_TRI_WAVE(B)                                                                    \
+  case B:                                                                                  \
+    tri_wave_closest<B>(nodes, prims, max_nodes, nprims, s, ds, kEPS, prev, wn, wm, t, u, v, \
+                        prim);                                                             \
+    break;
+    switch (sb) {
+      SRF_TRI_WAVE(0)
+      SRF_TRI_WAVE(1)
+      SRF_TRI_WAVE(2)
+      SRF_TRI_WAVE(3)
+      SRF_TRI_WAVE(4)
+      SRF_TRI_WAVE(5)
+      SRF_TRI_WAVE(6)
+      SRF_TRI_WAVE(7)
+      default:
+        break;
     }
+#undef SRF_TRI_WAVE
     if (prim == SRF_NO_PRIM) t = FLT_MAX;
   }
   reinterpret_cast<float4 *>(hits)[i] = make_float4(t, u, v, __uint_as_float(prim));
@@ -272,9 +306,16 @@ k_trace_tri_wave(const float4 *__restrict__ nodes, const float4 *__restrict__ pr
 // ------------------------------------------------------------------ spheres
 
 // SphereIsect, render/render_accel_particle.cc:2423-2512
+// `tcut`: the caller keeps a reported t only if it is at most tcut, so a
+// sphere whose two roots both certainly lie beyond it is rejected before the
+// exact roots are formed: q is computed exactly as below, and q / a and
+// cc / q are estimated with hardware reciprocals (within 3 ulp of the
+// correctly rounded quotients; the margin is 2e-5 relative). q itself may
+// carry cancellation error (its formula is the reference's), which the
+// estimates share with the exact roots, so it costs no accuracy here.
 __device__ __forceinline__ bool sphere_isect(const RaySetup &s, float4 c, float tin,
                                              double prevT, const float *prevN, bool self,
-                                             float &tout) {
+                                             float tcut, float &tout) {
   const float ocx = s.o[0] - c.x, ocy = s.o[1] - c.y, ocz = s.o[2] - c.z;
   const float dx = s.d[0], dy = s.d[1], dz = s.d[2];
   const float radius = c.w;
@@ -287,7 +328,16 @@ __device__ __forceinline__ bool sphere_isect(const RaySetup &s, float4 c, float 
   if (disc == 0.0f) {
     t0 = t1 = (float)(-0.5 * (double)(b / a));
   } else {
-    float dsq = sqrtf(disc);
+    const float dsq = sqrtf(disc);
+    {
+      // q below, exactly (halving is exact above FLT_MIN); only the two
+      // divisions are estimated
+      const float qe = (b < 0.0f) ? (-b - dsq) * 0.5f : (-b + dsq) * 0.5f;
+      const float lim = tcut * 1.00002f;
+      if (fabsf(qe) >= FLT_MIN && a >= FLT_MIN && qe * __builtin_amdgcn_rcpf(a) > lim &&
+          cc * __builtin_amdgcn_rcpf(qe) > lim)
+        return false;
+    }
     float q;
     if (b < 0.0f)
       q = (float)((double)(-b - dsq) / 2.0);
@@ -338,12 +388,24 @@ __device__ __forceinline__ bool sphere_isect(const RaySetup &s, float4 c, float 
 // farthest distance from the origin to it (>= s + r for every sphere
 // inside), which bounds that with a margin of 25 %. A ray whose origin is
 // inside the grown box gets lo < 0.
-__device__ __forceinline__ bool sub_box(const RaySetup &s, float x0, float y0, float z0,
-                                        float x1, float y1, float z1, float &lo) {
+//
+// The padding is computed once per big leaf (sub_pad) from the root's two
+// boxes, which contain every box below them: the farthest distance from the
+// origin to them is at least the farthest distance to any node's box, so the
+// padding is at least the one the bound asks for at every node.
+__device__ __forceinline__ float sub_pad(const RaySetup &s, float4 n0, float4 n1, float4 n2) {
+  const float x0 = fminf(n0.x, n0.w), y0 = fminf(n0.y, n1.x), z0 = fminf(n0.z, n1.y);
+  const float x1 = fmaxf(n1.z, n2.y), y1 = fmaxf(n1.w, n2.z), z1 = fmaxf(n2.x, n2.w);
   const float fx = fmaxf(fabsf(s.o[0] - x0), fabsf(s.o[0] - x1));
   const float fy = fmaxf(fabsf(s.o[1] - y0), fabsf(s.o[1] - y1));
   const float fz = fmaxf(fabsf(s.o[2] - z0), fabsf(s.o[2] - z1));
-  const float pad = 1.25e-3f * sqrtf((fx * fx + fy * fy) + fz * fz);
+  // (the hardware square root is within 1 ulp; the bound has a 25 % margin)
+  return 1.25e-3f * __builtin_amdgcn_sqrtf((fx * fx + fy * fy) + fz * fz);
+}
+
+// The slab test of a box grown by `pad` on every side (sub_pad's value).
+__device__ __forceinline__ bool sub_box(const RaySetup &s, float pad, float x0, float y0,
+                                        float z0, float x1, float y1, float z1, float &lo) {
   const float b0[3] = {x0 - pad, y0 - pad, z0 - pad}, b1[3] = {x1 + pad, y1 + pad, z1 + pad};
   float tn = -FLT_MAX, tf = FLT_MAX;
 #pragma unroll
@@ -379,6 +441,7 @@ __device__ __forceinline__ bool sph_big_leaf(const SphSub &sub, uint32_t nbase, 
   bool kept = false;  // `best` is one of this leaf's spheres
   int sp = 0;
   uint32_t cur = 0;
+  float pad = 0.0f;
   const uint32_t nnodes = 2 * count - 1;
   for (;;) {
     uint4 meta;
@@ -392,9 +455,10 @@ __device__ __forceinline__ bool sph_big_leaf(const SphSub &sub, uint32_t nbase, 
       meta = reinterpret_cast<const uint4 *>(nd)[3];
       keep_loaded(n0, n1, n2);
       if (meta.x != 0) break;
+      if (cur == 0) pad = sub_pad(s, n0, n1, n2);  // the root, visited first
       float l0, l1;
-      bool h0 = sub_box(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, l0);
-      bool h1 = sub_box(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, l1);
+      bool h0 = sub_box(s, pad, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, l0);
+      bool h1 = sub_box(s, pad, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, l1);
       h0 = h0 && (kAny ? l0 < tlimit : l0 <= t);
       h1 = h1 && (kAny ? l1 < tlimit : l1 <= t);
       const bool one_first = h1 && (!h0 || l1 < l0);
@@ -422,7 +486,7 @@ __device__ __forceinline__ bool sph_big_leaf(const SphSub &sub, uint32_t nbase, 
       const uint32_t q = first + loc;
       const bool self = (prev != SRF_NO_PRIM) && (sorted_index[q] == prev);
       float tt;
-      if (!sphere_isect(s, c, INFINITY, prevT, prevN, self, tt)) continue;
+      if (!sphere_isect(s, c, INFINITY, prevT, prevN, self, kAny ? tlimit : t, tt)) continue;
       if (kAny) {
         if (tt < tlimit) return true;
       } else if (tt < t || (tt == t && kept && q < best)) {
@@ -468,6 +532,7 @@ __device__ __forceinline__ void sph_big_leaf_wave(const SphSub &sub, uint32_t nb
   const bool any_prev = __ballot(in && prev != SRF_NO_PRIM) != 0;
   uint32_t cur = 0;
   int wsp = 0;
+  float pad = 0.0f;
   WaveNode nd;
   while (mask) {
     bool leaf = false, bad = false;
@@ -485,8 +550,9 @@ __device__ __forceinline__ void sph_big_leaf_wave(const SphSub &sub, uint32_t nb
       const uint32_t c0 = nd.meta.z, c1 = nd.meta.w;
       const bool act = (mask & me) != 0 && !(kAny && occ);
       float l0 = 0.0f, l1 = 0.0f;
-      bool h0 = act && sub_box(s, nd.n0.x, nd.n0.y, nd.n0.z, nd.n1.z, nd.n1.w, nd.n2.x, l0);
-      bool h1 = act && sub_box(s, nd.n0.w, nd.n1.x, nd.n1.y, nd.n2.y, nd.n2.z, nd.n2.w, l1);
+      if (cur == 0) pad = sub_pad(s, nd.n0, nd.n1, nd.n2);  // the root, visited first
+      bool h0 = act && sub_box(s, pad, nd.n0.x, nd.n0.y, nd.n0.z, nd.n1.z, nd.n1.w, nd.n2.x, l0);
+      bool h1 = act && sub_box(s, pad, nd.n0.w, nd.n1.x, nd.n1.y, nd.n2.y, nd.n2.z, nd.n2.w, l1);
       h0 = h0 && (kAny ? l0 < tlimit : l0 <= t);
       h1 = h1 && (kAny ? l1 < tlimit : l1 <= t);
       const int nearc = (sb >> nd.meta.y) & 1u;
@@ -535,7 +601,8 @@ __device__ __forceinline__ void sph_big_leaf_wave(const SphSub &sub, uint32_t nb
           if (!act || (kAny && occ)) continue;
           const bool self = (prev != SRF_NO_PRIM) && (sid == prev);
           float tt;
-          if (!sphere_isect(s, c[jj], INFINITY, prevT, prevN, self, tt)) continue;
+          if (!sphere_isect(s, c[jj], INFINITY, prevT, prevN, self, kAny ? tlimit : t, tt))
+            continue;
           if (kAny) {
             if (tt < tlimit) occ = true;
           } else if (tt < t || (tt == t && kept && q < best)) {
@@ -620,7 +687,7 @@ __device__ __forceinline__ bool sph_traverse(const float4 *__restrict__ nodes,
       const float4 c = prims[q];
       const bool self = (prev != SRF_NO_PRIM) && (sorted_index[q] == prev);
       float tt;
-      if (sphere_isect(s, c, t, prevT, prevN, self, tt)) {
+      if (sphere_isect(s, c, t, prevT, prevN, self, kAny ? tlimit : t, tt)) {
         if (kAny) {
           if (tt < tlimit) return true;
         } else {
@@ -705,10 +772,10 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
         const WaveNode p0 = load_wave_node(nodes, c0 < max_nodes ? c0 : 0u);
         const WaveNode p1 = load_wave_node(nodes, c1 < max_nodes ? c1 : 0u);
         const bool in = (mask & me) != 0;
-        const bool h0 = in && ray_aabb<true>(s, t, nd.n0.x, nd.n0.y, nd.n0.z, nd.n1.z, nd.n1.w,
-                                             nd.n2.x);
-        const bool h1 = in && ray_aabb<true>(s, t, nd.n0.w, nd.n1.x, nd.n1.y, nd.n2.y, nd.n2.z,
-                                             nd.n2.w);
+        const bool h0 = in && ray_aabb_wave<true>(s, sb, t, nd.n0.x, nd.n0.y, nd.n0.z, nd.n1.z,
+                                                  nd.n1.w, nd.n2.x);
+        const bool h1 = in && ray_aabb_wave<true>(s, sb, t, nd.n0.w, nd.n1.x, nd.n1.y, nd.n2.y,
+                                                  nd.n2.z, nd.n2.w);
         const int nearc = (sb >> nd.meta.y) & 1u;
         const uint64_t nm = __ballot(nearc ? h1 : h0), fm = __ballot(nearc ? h0 : h1);
         const uint32_t cn = nearc ? c1 : c0, cf = nearc ? c0 : c1;
@@ -761,7 +828,7 @@ k_trace_sph(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
               if (!in) continue;
               const bool self = (prev != SRF_NO_PRIM) && (sid[j] == prev);
               float tt;
-              if (sphere_isect(s, c[j], t, prevT, prevN, self, tt)) {
+              if (sphere_isect(s, c[j], t, prevT, prevN, self, t, tt)) {
                 t = tt;
                 best = q + (uint32_t)j;
               }
@@ -2522,14 +2589,16 @@ k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims
             const float4 *nd = nodes + 4 * (size_t)cur;
             const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
             meta = reinterpret_cast<const uint4 *>(nd)[3];
-            keep_loaded(n0, n1, n2);
+            keep_loaded_s(n0, n1, n2);
             if (meta.x != 0) {
               leaf = true;
               break;
             }
             const bool in = (mask & me) != 0 && !occ;
-            const bool h0 = in && ray_aabb<true>(s, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
-            const bool h1 = in && ray_aabb<true>(s, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
+            const bool h0 =
+                in && ray_aabb_wave<true>(s, sb, t, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x);
+            const bool h1 =
+                in && ray_aabb_wave<true>(s, sb, t, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w);
             const int nearc = (sb >> meta.y) & 1u;
             const uint64_t nm = __ballot(nearc ? h1 : h0), fm = __ballot(nearc ? h0 : h1);
             const uint32_t cn = nearc ? meta.w : meta.z, cf = nearc ? meta.z : meta.w;
@@ -2567,7 +2636,7 @@ k_any_ordered(const float4 *__restrict__ nodes, const float4 *__restrict__ prims
                 if (!in || occ) continue;
                 const bool self = (prev != SRF_NO_PRIM) && (sid == prev);
                 float tt;
-                if (sphere_isect(s, c, t, 0.0, zero, self, tt) && tt < tlimit) occ = true;
+                if (sphere_isect(s, c, t, 0.0, zero, self, tlimit, tt) && tt < tlimit) occ = true;
               }
             }
           }

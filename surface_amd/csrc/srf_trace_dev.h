@@ -68,6 +68,34 @@ __device__ __forceinline__ bool ray_aabb(const RaySetup &s, float maxT, float bx
   return (tmax > 0.0f) && (tmin <= tmax) && (tmin <= maxT);
 }
 
+// The same test for a wavefront whose rays all have the direction signs
+// `sb` (sign[k] << k, wavefront-uniform) and a node loaded through the
+// scalar unit: the near and far planes are chosen once for the wavefront
+// (scalar selects) instead of per lane; the arithmetic is ray_aabb's.
+template <bool kClampTmax>
+__device__ __forceinline__ bool ray_aabb_wave(const RaySetup &s, uint32_t sb, float maxT,
+                                              float bx0, float by0, float bz0, float bx1,
+                                              float by1, float bz1) {
+  const float mnx = (sb & 1u) ? bx1 : bx0, mxx = (sb & 1u) ? bx0 : bx1;
+  const float mny = (sb & 2u) ? by1 : by0, mxy = (sb & 2u) ? by0 : by1;
+  const float mnz = (sb & 4u) ? bz1 : bz0, mxz = (sb & 4u) ? bz0 : bz1;
+  float tminx = (mnx - s.o[0]) * s.inv[0];
+  float tmaxx = (mxx - s.o[0]) * s.inv[0];
+  float tminy = (mny - s.o[1]) * s.inv[1];
+  float tmaxy = (mxy - s.o[1]) * s.inv[1];
+  float tmin = (tminx > tminy) ? tminx : tminy;
+  float tmax = (tmaxx < tmaxy) ? tmaxx : tmaxy;
+  float tminz = (mnz - s.o[2]) * s.inv[2];
+  float tmaxz = (mxz - s.o[2]) * s.inv[2];
+  tmin = (tmin > tminz) ? tmin : tminz;
+  tmax = (tmax < tmaxz) ? tmax : tmaxz;
+  if (kClampTmax) {
+    tmax = (maxT < tmax) ? maxT : tmax;
+    return (tmax > 0.0f) && (tmin <= tmax);
+  }
+  return (tmax > 0.0f) && (tmin <= tmax) && (tmin <= maxT);
+}
+
 // ---------------------------------------------------------------- triangles
 
 // TriangleIsect (Moller-Trumbore), render/render_accel_triangle.cc:1887-1938,
@@ -125,8 +153,14 @@ __device__ __forceinline__ void keep_loaded(const float4 &a, const float4 &b, co
                "v"(b.w), "v"(c.x), "v"(c.y), "v"(c.z), "v"(c.w));
 }
 
-// The same for two values loaded through the scalar unit (wavefront-uniform
-// addresses): the loads are issued together and waited for once.
+// The same for values loaded through the scalar unit (wavefront-uniform
+// addresses): the loads are issued together and waited for once, and the
+// values stay in scalar registers.
+__device__ __forceinline__ void keep_loaded_s(const float4 &a, const float4 &b, const float4 &c) {
+  asm volatile("" ::"s"(a.x), "s"(a.y), "s"(a.z), "s"(a.w), "s"(b.x), "s"(b.y), "s"(b.z),
+               "s"(b.w), "s"(c.x), "s"(c.y), "s"(c.z), "s"(c.w));
+}
+
 __device__ __forceinline__ void keep_loaded_s2(const float4 &a, const float4 &b) {
   asm volatile("" ::"s"(a.x), "s"(a.y), "s"(a.z), "s"(a.w), "s"(b.x), "s"(b.y), "s"(b.z),
                "s"(b.w));

@@ -1,8 +1,9 @@
 """The culling bound of the secondary sphere trees, checked on the CPU.
 
 A big sphere leaf is searched through a secondary tree whose boxes are grown
-by 1.25e-3 of the farthest origin-to-box distance (sub_box in
-surface_amd/csrc/srf_trace.hip). That is exact only if every t the float
+by 1.25e-3 of the farthest distance from the origin to the tree's root boxes
+(sub_pad in surface_amd/csrc/srf_trace.hip), which is at least the farthest
+distance to any box below them. That is exact only if every t the float
 sphere test reports lies no nearer than the entry into the ball of radius
 r + 1.25e-3 D around the centre (D >= |o - c| + r, a lower bound of that farthest
 distance). Here SphereIsect (render/render_accel_particle.cc:2423-2512) is
@@ -17,7 +18,7 @@ import numpy as np
 
 f32, f64 = np.float32, np.float64
 PAD = 1.25e-3
-# the rounding-error bound of the sphere test (sub_box's comment): a reported
+# the rounding-error bound of the sphere test (sub_pad's comment): a reported
 # root is never nearer than the entry into the ball of radius r + 9.2e-4 |o - c|
 BOUND = 9.2e-4
 
