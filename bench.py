@@ -54,35 +54,47 @@ METRIC = "closest-hit Mrays/s + LBVH build Mprims/s; volume fwd+bwd ms vs plain 
 HBM_PEAK_GBS = 8000.0
 
 # ---------------------------------------------------------------------------
-# Frozen per-ray load counts of each dominant kernel on its bench rays (the
-# kernel's own loads: a load made once for a wavefront of 64 rays counts
-# once), from an instrumented build of the kernels, and the reference's own
-# traversal counts (oracle stats) for comparison.
-#
-# k_trace_tet (tet_grid(64), tet_rays(64, 1024, 1024)): pass A loads 60.53
-# nodes and 28.24 tetras per wavefront = 0.9458 + 0.4413 per ray, pass B
-# 0.0588 nodes and 0.0187 tetras per ray. The reference's traversal: 180.0010
-# nodes and 60.3673 tetras per ray.
-LOADS = {
-    "k_trace_tet": {"nodes": 0.9458 + 0.0588, "prims": 0.4413 + 0.0187,
-                    "node_bytes": 40, "prim_bytes": 80, "ref_nodes": 180.0010,
-                    "ref_prims": 60.3673},
-    # heightfield(512) x heightfield_rays(1024, 1024); the reference's counts
-    # (oracle stats) are 64.3038 nodes and 94.4435 triangles per ray
-    "k_trace_tri_wave": {"nodes": None, "prims": None, "node_bytes": 64, "prim_bytes": 48,
-                         "ref_nodes": 64.3038, "ref_prims": 94.4435},
-    "k_trace_tri_wave_1m": {"nodes": None, "prims": None, "node_bytes": 64, "prim_bytes": 48,
-                            "ref_nodes": None, "ref_prims": None},
-    "k_scene_tri": {"nodes": None, "prims": None, "node_bytes": 64, "prim_bytes": 48,
-                    "ref_nodes": None, "ref_prims": None},
-    "k_trace_sph": {"nodes": None, "prims": None, "node_bytes": 64, "prim_bytes": 16,
-                    "ref_nodes": None, "ref_prims": None},
+# roofline.achieved follows the hot-path table's algorithmic bytes: per ray
+# 40 B (ray in, hit out) + node bytes x nodes + primitive bytes x primitives,
+# with the REFERENCE traversal's per-ray node and primitive counts on the
+# configuration's rays (the oracle's stats, which count what
+# {Triangle,Tetra,Particle}Accel::Traverse visits), frozen here; build
+# bytes per primitive likewise. A wavefront traversal loads each node and
+# primitive once for up to 64 rays, so these rates can exceed the HBM peak:
+# `traffic` (rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, each scaled by
+# what the counter reads for a plain 1 GiB device copy: fetch x 2.000, write
+# x 1.000) is the HBM traffic the kernel really causes, frozen from the
+# committed profiles, and `own_bytes` (where counted) the kernel's own loads.
+NODE_PRIM_BYTES = {"tri": (64, 48), "tet": (40, 80), "sph": (56, 16)}
+BUILD_BYTES = {"tri": 288, "tet": 336, "sph": 192}
+REF_COUNTS = {
+    # oracle stats over all 1024^2 rays
+    "heightfield512": (64.3038, 94.4435),
+    "heightfield725": (79.4602, 128.2012),
+    "tetgrid64": (180.0010, 60.3673),
+    # 65,536 of the 2048^2 primary rays (every 64th row); the reference scans
+    # a depth-capped leaf's spheres one by one (13,788 tests per ray)
+    "plummer16M_primary": (123.38, 13787.78),
+    # every element whose world box the ray enters traced to its end: an
+    # upper bound of RenderGraph::Trace, which skips elements beyond its hit
+    # (every 64th row of the 16 cameras)
+    "scene64_upper": (103.0319, 47.7509),
 }
-# HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, KiB per dispatch,
-# each scaled by the same counter on a 1 GiB device copy: fetch x 2.000,
-# write x 1.000); None where not yet profiled.
+# the kernel's own per-ray node and primitive loads (an instrumented build):
+# k_trace_tet pass A 60.53 nodes and 28.24 tetras per wavefront of 64 rays,
+# pass B 0.0588 nodes and 0.0187 tetras per ray
+OWN_LOADS = {"k_trace_tet": (0.9458 + 0.0588, 0.4413 + 0.0187)}
+# HBM bytes per launch (FETCH_SIZE x 2 + WRITE_SIZE, KiB -> bytes) from the
+# counter passes over `bench.py --probe` summarised in
+# profiles/counters_bench_probe.md; a configuration missing here reports null
 TRAFFIC = {
-    "k_trace_tet": (2.000 * 32442.81 + 1.000 * 49152.0) * 1024,
+    "heightfield512": 218_016_005,
+    "heightfield725": 464_982_015,
+    "tetgrid64": 116_767_268,
+    "plummer16M_primary": 1_425_681_257,  # both launches
+    "scene64_upper": 23_582_978_364,      # persistent launch, 128-lane blocks
+    "k_vol_fwd": 276_618_486,             # the march without checkpoints
+    "k_vol_bwd": 1_374_864_457,
 }
 
 
@@ -134,29 +146,38 @@ def median_s(fn, warmup=3, reps=10):
     return statistics.median(ts)
 
 
-def roofline(kernel, ms, n_rays, extra_bytes_per_ray=40.0):
-    """Roofline object of a traversal kernel from its frozen load counts."""
-    c = LOADS[kernel]
+def roofline(kernel, ms, n_rays, kind, counts, launches=1):
+    """Roofline object of a traversal kernel: `counts` names the frozen
+    reference traversal counts, `kind` the node/primitive sizes."""
+    nb, pb = NODE_PRIM_BYTES[kind]
+    nodes, prims = REF_COUNTS[counts]
+    bpr = 40.0 + nb * nodes + pb * prims
+    gbs = bpr * n_rays / (ms * 1e-3) / 1e9
     out = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel": kernel,
-           "kernel_ms": round(ms, 4)}
-    if c["nodes"] is not None:
-        bpr = extra_bytes_per_ray + c["node_bytes"] * c["nodes"] + c["prim_bytes"] * c["prims"]
-        gbs = bpr * n_rays / (ms * 1e-3) / 1e9
-        out.update({"achieved": round(gbs, 2), "frac": round(gbs / HBM_PEAK_GBS, 4),
-                    "bytes_per_launch": round(bpr * n_rays),
-                    "loads_per_ray": {"nodes": c["nodes"], "prims": c["prims"]}})
-    else:
-        out.update({"achieved": None, "frac": None, "bytes_per_launch": None})
-    if c["ref_nodes"] is not None:
-        rb = extra_bytes_per_ray + c["node_bytes"] * c["ref_nodes"] + c["prim_bytes"] * c["ref_prims"]
-        out["reference_traversal_bytes_per_launch"] = round(rb * n_rays)
-        out["reference_traversal_equivalent_gbs"] = round(rb * n_rays / (ms * 1e-3) / 1e9, 1)
-    tr = TRAFFIC.get(kernel)
+           "launches": launches, "kernel_ms": round(ms, 4),
+           "achieved": round(gbs, 2), "frac": round(gbs / HBM_PEAK_GBS, 4),
+           "bytes_per_launch": round(bpr * n_rays),
+           "reference_loads_per_ray": {"nodes": nodes, "prims": prims}}
+    own = OWN_LOADS.get(kernel)
+    if own is not None:
+        ob = 40.0 + nb * own[0] + pb * own[1]
+        out["own_bytes"] = round(ob * n_rays)
+        out["own_frac"] = round(ob * n_rays / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    tr = TRAFFIC.get(counts)
     out["traffic"] = round(tr) if tr is not None else None
     if tr is not None:
-        out["traffic_gbs"] = round(tr / (ms * 1e-3) / 1e9, 2)
         out["traffic_frac"] = round(tr / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     return out
+
+
+def build_roofline(kind, prims, ms):
+    """The LBVH build against HBM: algorithmic bytes per primitive (Morton,
+    sort, Karras, leaves and refit) x primitives / build time."""
+    bpp = BUILD_BYTES[kind]
+    gbs = bpp * prims / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_prim": bpp,
+            "bytes_per_build": bpp * prims, "achieved": round(gbs, 2),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def reference_or_none():
@@ -196,7 +217,8 @@ def cfg_triangles(torch, S, W, ref, grid, do_cpu, kernel):
            "prims": int(len(F)), "rays": n, "build_ms": round(tb, 4),
            "build_mprims_per_s": round(len(F) / tb / 1e3, 1), "trace_ms": round(tt, 4),
            "trace_mrays_per_s": round(n / tt / 1e3, 1), "hit_fraction": round(hit, 4),
-           "roofline": roofline(kernel, tt, n)}
+           "roofline": roofline(kernel, tt, n, "tri", f"heightfield{grid}"),
+           "build_roofline": build_roofline("tri", len(F), tb)}
     del tri
     if do_cpu and ref is not None:
         th = cpu_threads()
@@ -241,7 +263,8 @@ def cfg_tetgrid(torch, S, W, ref, do_cpu):
            "build_ms": round(tb, 4), "build_mprims_per_s": round(len(F) / tb / 1e3, 1),
            "trace_ms": round(tt, 4), "trace_mrays_per_s": round(n / tt / 1e3, 1),
            "step_ms": round(ts, 4), "step_mrays_per_s": round(n / ts / 1e3, 1),
-           "roofline": roofline("k_trace_tet", tt, n)}
+           "roofline": roofline("k_trace_tet", tt, n, "tet", "tetgrid64"),
+           "build_roofline": build_roofline("tet", len(F), tb)}
     del acc
     if do_cpu and ref is not None:
         th = cpu_threads()
@@ -319,7 +342,8 @@ def cfg_scene(torch, S, W, ref, do_cpu):
     ms = timed_events(torch, graph.replay, 3)
     out["trace_graph_16_cameras_ms"] = round(ms, 3)
     out["trace_graph_mrays_per_s"] = round(nr / ms / 1e3, 1)
-    out["roofline"] = roofline("k_scene_tri", ms, nr)
+    out["roofline"] = roofline("k_scene_tri", ms, nr, "tri", "scene64_upper")
+    out["build_roofline"] = build_roofline("tri", ntris, batch_ms)
     # one camera per launch, for comparison
     ms = timed_events(torch, lambda: [scene.Trace(r, hits[:len(r)], ids[:len(r)], mode="tuned")
                                       for r in rays], 2)
@@ -394,11 +418,17 @@ def cfg_volume(torch, S, W, orc, do_cpu):
                         "achieved": round(fwd_bytes / (fwd * 1e-3) / 1e9, 2),
                         "frac": round(fwd_bytes / (fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                         "traffic": TRAFFIC.get("k_vol_fwd"),
+                        "traffic_frac": (round(TRAFFIC["k_vol_fwd"] / (fwd * 1e-3) / 1e9 /
+                                               HBM_PEAK_GBS, 4)
+                                         if TRAFFIC.get("k_vol_fwd") else None),
                         "bwd": {"kernel": "k_vol_bwd", "kernel_ms": round(bwd, 4),
                                 "bytes_per_launch": round(bwd_bytes),
                                 "achieved": round(bwd_bytes / (bwd * 1e-3) / 1e9, 2),
                                 "frac": round(bwd_bytes / (bwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                                "traffic": TRAFFIC.get("k_vol_bwd")}}}
+                                "traffic": TRAFFIC.get("k_vol_bwd"),
+                                "traffic_frac": (round(TRAFFIC["k_vol_bwd"] / (bwd * 1e-3) /
+                                                       1e9 / HBM_PEAK_GBS, 4)
+                                                 if TRAFFIC.get("k_vol_bwd") else None)}}}
     del vol, vox, tf, vr, gv, gt, grad, ck, rgba
     torch.cuda.empty_cache()
     if do_cpu and orc is not None:
@@ -508,7 +538,8 @@ def cfg_particles(torch, S, W, ref, pt, do_cpu):
         ms = timed_events(torch, lambda: pt.acc.TraverseAnySorted(
             pt.ao[:pt.na], pt.tmax[:pt.na], bits, mode, pt.occ[:pt.na]), 3)
         out[f"ao_{name}_ms"] = round(ms, 3)
-    out["roofline"] = roofline("k_trace_sph", p, pt.np)
+    out["roofline"] = roofline("k_trace_sph", p, pt.np, "sph", "plummer16M_primary", launches=2)
+    out["build_roofline"] = build_roofline("sph", 1 << 24, b)
     if do_cpu and ref is not None:
         th = cpu_threads()
         bs = median_s(lambda: ref.free(ref.build("sph", pt.P_np, R=pt.R_np, keep=True)))
@@ -624,8 +655,11 @@ def probe(torch, S, W):
     grad = torch.ones((len(vr), 4), dtype=torch.float32, device="cuda")
     ck = torch.empty((len(vr), vol.checkpoint_slots(step_len)), dtype=torch.float32,
                      device="cuda")
+    rgba = vol.march(tf, step_len, vr)
+    for _ in range(2):  # the march as the bench times it, without checkpoints
+        vol.march(tf, step_len, vr, rgba=rgba)
     for _ in range(2):
-        rgba = vol.march(tf, step_len, vr, ck)
+        vol.march(tf, step_len, vr, ck, rgba)
         vol.march_backward(tf, step_len, vr, grad, gv, gt, ck)
     torch.cuda.synchronize()
     log("probe: volume done")
@@ -707,7 +741,7 @@ def main():
         configs["heightfield"] = cfg_triangles(torch, S, W, ref, 512, do_cpu, "k_trace_tri_wave")
         log("heightfield 725 (1,048,352 triangles)")
         configs["heightfield_1m"] = cfg_triangles(torch, S, W, ref, 725, do_cpu,
-                                                  "k_trace_tri_wave_1m")
+                                                  "k_trace_tri_wave")
         log("tetra grid 64")
         configs["tetgrid"] = cfg_tetgrid(torch, S, W, ref, do_cpu)
         configs["volume"] = cfg_volume(torch, S, W, orc, do_cpu)
