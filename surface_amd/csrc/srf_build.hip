@@ -809,111 +809,142 @@ __device__ __forceinline__ uint32_t level_start(const uint32_t *counts, uint32_t
   return s;
 }
 
-// One pass over level `level`'s `count` branches (stored from in_off in the
-// level-ordered list) by workgroups of kClassifyBlock threads, the workgroup
-// at `start` stepping by `stride`; the next level's entries are appended
-// after them.
 constexpr uint32_t kClassifyBlock = 256;
+
+// Levels `level` and level + 1 in one launch: each thread classifies one
+// branch of level `level` and then its branch children, and only the
+// branches of level + 2 are appended (the next launch's heads); level + 1
+// needs no list of its own, because its branches are reached from their
+// parents here and in the paired refit below. Each node's decision is the
+// collapse rule (a child is a leaf iff it is a Karras leaf, holds at most
+// max_leaf + 1 primitives, or lies below the depth cap), so the tree does
+// not depend on how the levels are grouped into launches. The workgroup
+// steps through the heads as a whole, so the appends take one atomic per
+// workgroup and step (per-wavefront atomics on the one counter serialised
+// the big levels).
 template <bool M>
-__device__ __forceinline__ void classify_pass(uint32_t n_all, int max_leaf, uint32_t level,
-                                              uint32_t count, uint32_t in_off,
+__device__ __forceinline__ void classify_node(uint32_t x, uint32_t base, uint32_t n,
+                                              uint32_t level, uint32_t turns, int max_leaf,
                                               const uint32_t *kchild, const uint32_t *kleaf,
-                                              const uint32_t *kother, uint2 *levels,
-                                              uint32_t *counts, uint32_t *cls,
-                                              uint32_t *leaf_start, SegView v, uint32_t start,
-                                              uint32_t stride) {
-  // this level's branches and, after them, the next level's: entries are
-  // (Karras index, turns | mesh << 8)
+                                              const uint32_t *kother, uint32_t *cls,
+                                              uint32_t *leaf_start, BuildParams *params,
+                                              uint32_t &cl, uint32_t &cr, bool &bl, bool &br) {
+  bl = br = false;
+  cl = cr = 0;
+  const uint32_t i = x - base;
+  if (i + 1 >= n) {
+    atomicOr(&params->error, 8u);  // a leaf handed over as a branch: never expected
+    return;
+  }
+  cls[x] = 1u | (level << 1) | (turns << 8);
+  const bool deep = level > (uint32_t)kMaxDepth32;
+  const uint32_t jo = kother[x];
+  const uint32_t l = i < jo ? i : jo, r = i < jo ? jo : i;
+  const uint32_t g = kchild[x], lb = kleaf[x];
+  const bool left_leaf = (lb & 1u) || (g - l <= (uint32_t)max_leaf) || deep;
+  const bool right_leaf = (lb & 2u) || (r - (g + 1) <= (uint32_t)max_leaf) || deep;
+  if (left_leaf)
+    leaf_start[base + l] = 1;
+  else
+    bl = true, cl = base + g;
+  if (right_leaf)
+    leaf_start[base + g + 1] = 1;
+  else
+    br = true, cr = base + g + 1;
+}
+
+// the deepest branch level per mesh: one atomic per wavefront and mesh
+__device__ __forceinline__ void note_depth(BuildParams *params, uint32_t level) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool has = params != nullptr;
+  uint64_t pending = __ballot(has);
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint64_t pv = (uint64_t)__shfl((long long)(uintptr_t)params, leader, 64);
+    const bool mine = has && (uint64_t)(uintptr_t)params == pv;
+    // the largest level among this mesh's lanes
+    uint32_t top = mine ? level : 0u;
+    for (int off = 32; off >= 1; off >>= 1) top = max(top, (uint32_t)__shfl_xor((int)top, off, 64));
+    if ((int)lane == leader && count_load(&params->max_steps) < top)
+      atomicMax(&params->max_steps, top);
+    pending &= ~__ballot(mine);
+  }
+}
+
+template <bool M>
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_pair(
+    uint32_t n_all, int max_leaf, uint32_t level, const uint32_t *kchild, const uint32_t *kleaf,
+    const uint32_t *kother, uint2 *levels, uint32_t *counts, uint32_t *cls,
+    uint32_t *leaf_start, SegView v) {
+  const uint32_t count = counts[level];
+  const uint32_t in_off = level_start(counts, level);
   const uint2 *front_in = levels + in_off;
+  // level + 1 keeps an empty list, so level + 2's list follows this one
   uint2 *front_out = levels + in_off + count;
   const uint32_t out_cap = n_all - in_off - count;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const bool deep = level > (uint32_t)kMaxDepth32;
-  // the workgroup steps through the frontier as a whole, so the appends
-  // below take one atomic per workgroup and step (per-wavefront atomics on
-  // the one counter serialised the big levels)
   __shared__ uint32_t s_tot[kClassifyBlock / 64], s_base;
-  for (uint32_t j0 = start; j0 < count; j0 += stride) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < count; j0 += stride) {
     const uint32_t j = j0 + threadIdx.x;
-    uint32_t push_l = 0, push_r = 0, cl = 0, cr = 0, turns = 0;
+    // up to four branches of level + 2: the children's children
+    uint32_t gc[4] = {0, 0, 0, 0}, gt[4] = {0, 0, 0, 0};
+    bool gp[4] = {false, false, false, false};
     BuildParams *params = nullptr;
-    uint32_t m = 0;
+    uint32_t m = 0, deepest = level;
     if (j < count) {
       const uint2 e = front_in[j];
-      const uint32_t x = e.x;
-      turns = e.y & 255u;
+      const uint32_t x = e.x, turns = e.y & 255u;
       m = e.y >> 8;
       const uint32_t base = seg_off<M>(v.prim_off, m);
       const uint32_t n = M ? v.prim_off[m + 1] - base : n_all;
-      const uint32_t i = x - base;
       params = seg_get<M>(v, m).params;
-      if (i + 1 < n) {
-        cls[x] = 1u | (level << 1) | (turns << 8);
-        const uint32_t jo = kother[x];
-        const uint32_t l = i < jo ? i : jo, r = i < jo ? jo : i;
-        const uint32_t g = kchild[x], lb = kleaf[x];
-        const bool left_leaf = (lb & 1u) || (g - l <= (uint32_t)max_leaf) || deep;
-        const bool right_leaf = (lb & 2u) || (r - (g + 1) <= (uint32_t)max_leaf) || deep;
-        if (left_leaf)
-          leaf_start[base + l] = 1;
-        else
-          push_l = 1, cl = base + g;
-        if (right_leaf)
-          leaf_start[base + g + 1] = 1;
-        else
-          push_r = 1, cr = base + g + 1;
-      } else {
-        atomicOr(&params->error, 8u);  // a leaf handed over as a branch: never expected
+      uint32_t c[2];
+      bool b[2];
+      classify_node<M>(x, base, n, level, turns, max_leaf, kchild, kleaf, kother, cls,
+                       leaf_start, params, c[0], c[1], b[0], b[1]);
+      if (level + 1 < (uint32_t)kMaxBranchLevels) {
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+          if (!b[side]) continue;
+          deepest = level + 1;
+          const uint32_t ct = turns + (uint32_t)side;
+          classify_node<M>(c[side], base, n, level + 1, ct, max_leaf, kchild, kleaf, kother,
+                           cls, leaf_start, params, gc[2 * side], gc[2 * side + 1],
+                           gp[2 * side], gp[2 * side + 1]);
+          gt[2 * side] = ct;
+          gt[2 * side + 1] = ct + 1;
+        }
       }
     }
-    // deepest branch per mesh: one atomic per wavefront and mesh present
-    {
-      const bool has = params != nullptr;
-      uint64_t pending = __ballot(has);
-      while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const uint64_t pv = (uint64_t)__shfl((long long)(uintptr_t)params, leader, 64);
-        const bool mine = has && (uint64_t)(uintptr_t)params == pv;
-        // skip the atomic once another wavefront has raised it this far
-        if ((int)lane == leader && count_load(&params->max_steps) < level)
-          atomicMax(&params->max_steps, level);
-        pending &= ~__ballot(mine);
-      }
+    note_depth(params, deepest);
+    uint64_t mk[4];
+    uint32_t total = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      mk[q] = __ballot(gp[q]);
+      total += (uint32_t)__popcll(mk[q]);
     }
-    const uint64_t ml = __ballot(push_l), mr = __ballot(push_r);
-    const uint32_t total = (uint32_t)(__popcll(ml) + __popcll(mr));
     if (lane == 0) s_tot[wave] = total;
     __syncthreads();
     if (threadIdx.x == 0) {
       uint32_t sum = 0;
       for (uint32_t w = 0; w < kClassifyBlock / 64; w++) sum += s_tot[w];
-      s_base = sum ? atomicAdd(&counts[level + 1], sum) : 0u;
+      s_base = sum ? atomicAdd(&counts[level + 2], sum) : 0u;
     }
     __syncthreads();
-    uint32_t base_out = s_base;
-    for (uint32_t w = 0; w < wave; w++) base_out += s_tot[w];
+    uint32_t at = s_base;
+    for (uint32_t w = 0; w < wave; w++) at += s_tot[w];
     __syncthreads();  // s_tot and s_base are rewritten by the next step
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    const uint32_t ol = base_out + (uint32_t)__popcll(ml & below);
-    const uint32_t orr = base_out + (uint32_t)__popcll(ml) + (uint32_t)__popcll(mr & below);
-    if (push_l && ol < out_cap) front_out[ol] = make_uint2(cl, turns | (m << 8));
-    if (push_r && orr < out_cap) front_out[orr] = make_uint2(cr, (turns + 1) | (m << 8));
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t o = at + (uint32_t)__popcll(mk[q] & below);
+      if (gp[q] && o < out_cap) front_out[o] = make_uint2(gc[q], gt[q] | (m << 8));
+      at += (uint32_t)__popcll(mk[q]);
+    }
   }
 }
-
-template <bool M>
-__global__ void __launch_bounds__(256) k_classify_level(uint32_t n_all, int max_leaf,
-                                                       uint32_t level, const uint32_t *kchild,
-                                                       const uint32_t *kleaf,
-                                                       const uint32_t *kother,
-                                                       uint2 *levels, uint32_t *counts,
-                                                       uint32_t *cls, uint32_t *leaf_start,
-                                                       SegView v) {
-  classify_pass<M>(n_all, max_leaf, level, counts[level], level_start(counts, level), kchild,
-                   kleaf, kother, levels, counts, cls, leaf_start, v, blockIdx.x * blockDim.x,
-                   gridDim.x * blockDim.x);
-}
-
 
 // Levels 0 .. top-1 in one workgroup of 1024 threads. The host picks top so
 // that none of these levels holds more than 1024 branches (level L holds at
@@ -1550,17 +1581,25 @@ __global__ void __launch_bounds__(256) k_sub_link(float4 *nodes, uint32_t nnodes
   }
 }
 
-// One level of the bottom-up refit: every branch whose children already hold
-// their boxes (deeper levels ran in earlier launches) merges them,
+// The bottom-up refit: each branch merges its two slots,
 // MergeBoundingBox(left, right) (render/render_accel_triangle.cc:804-814),
-// into its parent's slot; the root's merge is BoundingBox().
-// One pass over level `level`'s `count` branches (list), threads start ..
-// by stride.
+// into its parent's slot; the root's merge is BoundingBox(). Levels
+// `level` + 1 and `level` in one launch (the counterpart of
+// k_classify_pair): each thread takes one branch of level `level`, first
+// merges each branch child's two slots into the branch's slot for that
+// child (the child's own slots are complete: its children are leaves or
+// branches of level + 2, refit by the previous launch), then merges the
+// branch's two slots into its parent's slot, or the root box. The merges and
+// their order (left, right) are the same whichever launch does them, so the
+// boxes do not depend on the grouping either.
 template <bool M>
-__device__ __forceinline__ void refit_pass(uint32_t n_all, uint32_t count, const uint2 *list,
-                                           NodeOut o, SegView v, uint32_t start,
-                                           uint32_t stride) {
-  for (uint32_t j = start + threadIdx.x; j < count; j += stride) {
+__global__ void __launch_bounds__(256) k_refit_pair(uint32_t n_all, uint32_t level,
+                                                   const uint2 *levels, const uint32_t *counts,
+                                                   NodeOut o, SegView v) {
+  const uint32_t count = counts[level];
+  const uint2 *list = levels + level_start(counts, level);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count;
+       j += gridDim.x * blockDim.x) {
     const uint2 e = list[j];
     const uint32_t x = e.x, m = e.y >> 8;
     if (x >= n_all || (M && m >= v.nseg)) continue;
@@ -1569,18 +1608,44 @@ __device__ __forceinline__ void refit_pass(uint32_t n_all, uint32_t count, const
     if (x < base || x - base + 1 >= n) continue;
     const Seg sg = seg_get<M>(v, m);
     const uint32_t pos = o.branch_pos[x];
-    if (pos >= 2 * n - 1) {
+    const uint32_t max_nodes = 2 * n - 1;
+    if (pos >= max_nodes) {
       atomicOr(&sg.params->error, 16u);
       continue;
     }
-    const float4 *w4 = reinterpret_cast<const float4 *>(sg.nodes) + 4 * (size_t)pos;
+    float *words = reinterpret_cast<float *>(sg.nodes) + 16 * (size_t)pos;
+    const float4 *w4 = reinterpret_cast<const float4 *>(words);
     const float4 a = w4[0], b = w4[1], cc = w4[2];
-    Box l, r;
-    l.mn[0] = a.x; l.mn[1] = a.y; l.mn[2] = a.z;
-    r.mn[0] = a.w; r.mn[1] = b.x; r.mn[2] = b.y;
-    l.mx[0] = b.z; l.mx[1] = b.w; l.mx[2] = cc.x;
-    r.mx[0] = cc.y; r.mx[1] = cc.z; r.mx[2] = cc.w;
-    const Box mb = merge_lr(l, r);
+    const uint4 meta = reinterpret_cast<const uint4 *>(w4)[3];
+    Box lr[2];
+    lr[0].mn[0] = a.x; lr[0].mn[1] = a.y; lr[0].mn[2] = a.z;
+    lr[1].mn[0] = a.w; lr[1].mn[1] = b.x; lr[1].mn[2] = b.y;
+    lr[0].mx[0] = b.z; lr[0].mx[1] = b.w; lr[0].mx[2] = cc.x;
+    lr[1].mx[0] = cc.y; lr[1].mx[1] = cc.z; lr[1].mx[2] = cc.w;
+    bool bad = false;
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      const uint32_t cp = side ? meta.w : meta.z;
+      if (cp >= max_nodes) {
+        bad = true;
+        continue;
+      }
+      const float4 *c4 = reinterpret_cast<const float4 *>(sg.nodes) + 4 * (size_t)cp;
+      if (reinterpret_cast<const uint4 *>(c4)[3].x != 0) continue;  // a leaf: already in place
+      const float4 ca = c4[0], cb = c4[1], ccc = c4[2];
+      Box l, r;
+      l.mn[0] = ca.x; l.mn[1] = ca.y; l.mn[2] = ca.z;
+      r.mn[0] = ca.w; r.mn[1] = cb.x; r.mn[2] = cb.y;
+      l.mx[0] = cb.z; l.mx[1] = cb.w; l.mx[2] = ccc.x;
+      r.mx[0] = ccc.y; r.mx[1] = ccc.z; r.mx[2] = ccc.w;
+      lr[side] = merge_lr(l, r);
+      store_box_slot(words, side, lr[side]);
+    }
+    if (bad) {
+      atomicOr(&sg.params->error, 16u);
+      continue;
+    }
+    const Box mb = merge_lr(lr[0], lr[1]);
     if (pos == 0) {
       for (int k = 0; k < 3; k++) {
         sg.params->root_bmin[k] = mb.mn[k];
@@ -1590,22 +1655,12 @@ __device__ __forceinline__ void refit_pass(uint32_t n_all, uint32_t count, const
     }
     const uint32_t link = o.parent_link[2 * (size_t)base + pos];
     const uint32_t p = link >> 1;
-    if (p >= 2 * n - 1) {
+    if (p >= max_nodes) {
       atomicOr(&sg.params->error, 16u);
       continue;
     }
     store_box_slot(reinterpret_cast<float *>(sg.nodes) + 16 * (size_t)p, (int)(link & 1u), mb);
   }
-}
-
-template <bool M>
-__global__ void __launch_bounds__(256) k_refit_level(uint32_t n_all, uint32_t level,
-                                                    const uint2 *levels,
-                                                    const uint32_t *counts, NodeOut o,
-                                                    SegView v) {
-  // the branches of this level, as the classification listed them
-  refit_pass<M>(n_all, counts[level], levels + level_start(counts, level), o, v,
-                blockIdx.x * blockDim.x, gridDim.x * blockDim.x);
 }
 
 // Levels top-1 .. 0 in one workgroup (the counterpart of k_classify_top):
@@ -1961,9 +2016,11 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
   if (top > 0)
     SRF_DISPATCH_M(k_classify_top, dim3(1), dim3(1024), 0, stream, N, max_leaf, top, b.kchild,
                    b.kleaf, b.kother, b.levels, b.level_counts, b.cls, b.leaf_start, v);
+  // two levels per launch from `top` on: heads top, top + 2, ... (the lists
+  // of the levels between stay empty)
   const uint32_t cblocks = blocks_for(N, 256, 1024);
-  for (uint32_t level = top; level < (uint32_t)kMaxBranchLevels; level++)
-    SRF_DISPATCH_M(k_classify_level, dim3(cblocks), dim3(256), 0, stream, N, max_leaf, level,
+  for (uint32_t level = top; level < (uint32_t)kMaxBranchLevels; level += 2)
+    SRF_DISPATCH_M(k_classify_pair, dim3(cblocks), dim3(256), 0, stream, N, max_leaf, level,
                    b.kchild, b.kleaf, b.kother, b.levels, b.level_counts, b.cls, b.leaf_start,
                    v);
   const uint32_t scan_blocks = blocks_for(N, kScanTile);
@@ -1988,8 +2045,10 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
   // bottom-up refit, one level per launch (levels are branch steps 23..top),
   // then levels top-1..0 in one workgroup
   const uint32_t rblocks = blocks_for(N, 256, 2048);
-  for (int level = kMaxBranchLevels - 1; level >= (int)top; level--)
-    SRF_DISPATCH_M(k_refit_level, dim3(rblocks), dim3(256), 0, stream, N, (uint32_t)level,
+  int last_head = (int)top;
+  while (last_head + 2 < kMaxBranchLevels) last_head += 2;
+  for (int level = last_head; level >= (int)top; level -= 2)
+    SRF_DISPATCH_M(k_refit_pair, dim3(rblocks), dim3(256), 0, stream, N, (uint32_t)level,
                    b.levels, b.level_counts, b.o, v);
   if (top > 0)
     SRF_DISPATCH_M(k_refit_top, dim3(1), dim3(1024), 0, stream, N, top, b.levels,
