@@ -810,18 +810,20 @@ __device__ __forceinline__ uint32_t level_start(const uint32_t *counts, uint32_t
 }
 
 constexpr uint32_t kClassifyBlock = 256;
+// branch levels per classification and refit launch below the top levels
+constexpr int kLevelSpan = 3;
 
-// Levels `level` and level + 1 in one launch: each thread classifies one
-// branch of level `level` and then its branch children, and only the
-// branches of level + 2 are appended (the next launch's heads); level + 1
-// needs no list of its own, because its branches are reached from their
-// parents here and in the paired refit below. Each node's decision is the
-// collapse rule (a child is a leaf iff it is a Karras leaf, holds at most
-// max_leaf + 1 primitives, or lies below the depth cap), so the tree does
-// not depend on how the levels are grouped into launches. The workgroup
-// steps through the heads as a whole, so the appends take one atomic per
-// workgroup and step (per-wavefront atomics on the one counter serialised
-// the big levels).
+// Levels `level` .. `level` + S - 1 in one launch: each thread classifies
+// one branch of level `level` and then its branch descendants down to level
+// + S - 1, and only the branches of level + S are appended (the next
+// launch's heads); the levels in between need no lists of their own,
+// because their branches are reached from their heads here and in the
+// refit below. Each node's decision is the collapse rule (a child is a leaf
+// iff it is a Karras leaf, holds at most max_leaf + 1 primitives, or lies
+// below the depth cap), so the tree does not depend on how the levels are
+// grouped into launches. The workgroup steps through the heads as a whole,
+// so the appends take one atomic per workgroup and step (per-wavefront
+// atomics on the one counter serialised the big levels).
 template <bool M>
 __device__ __forceinline__ void classify_node(uint32_t x, uint32_t base, uint32_t n,
                                               uint32_t level, uint32_t turns, int max_leaf,
@@ -871,15 +873,16 @@ __device__ __forceinline__ void note_depth(BuildParams *params, uint32_t level) 
   }
 }
 
-template <bool M>
-__global__ void __launch_bounds__(kClassifyBlock) k_classify_pair(
+template <bool M, int S>
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_span(
     uint32_t n_all, int max_leaf, uint32_t level, const uint32_t *kchild, const uint32_t *kleaf,
     const uint32_t *kother, uint2 *levels, uint32_t *counts, uint32_t *cls,
     uint32_t *leaf_start, SegView v) {
+  constexpr int kOut = 1 << S;  // branches of level + S one head can reach
   const uint32_t count = counts[level];
   const uint32_t in_off = level_start(counts, level);
   const uint2 *front_in = levels + in_off;
-  // level + 1 keeps an empty list, so level + 2's list follows this one
+  // the levels in between keep empty lists, so level + S's list follows
   uint2 *front_out = levels + in_off + count;
   const uint32_t out_cap = n_all - in_off - count;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -887,42 +890,51 @@ __global__ void __launch_bounds__(kClassifyBlock) k_classify_pair(
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < count; j0 += stride) {
     const uint32_t j = j0 + threadIdx.x;
-    // up to four branches of level + 2: the children's children
-    uint32_t gc[4] = {0, 0, 0, 0}, gt[4] = {0, 0, 0, 0};
-    bool gp[4] = {false, false, false, false};
+    // the current layer's branches (Karras index, turns), widest 2^S
+    uint32_t lx[kOut], lt[kOut];
+    bool lp[kOut];
+#pragma unroll
+    for (int q = 0; q < kOut; q++) lx[q] = lt[q] = 0, lp[q] = false;
     BuildParams *params = nullptr;
     uint32_t m = 0, deepest = level;
     if (j < count) {
       const uint2 e = front_in[j];
-      const uint32_t x = e.x, turns = e.y & 255u;
+      lx[0] = e.x;
+      lt[0] = e.y & 255u;
+      lp[0] = true;
       m = e.y >> 8;
       const uint32_t base = seg_off<M>(v.prim_off, m);
       const uint32_t n = M ? v.prim_off[m + 1] - base : n_all;
       params = seg_get<M>(v, m).params;
-      uint32_t c[2];
-      bool b[2];
-      classify_node<M>(x, base, n, level, turns, max_leaf, kchild, kleaf, kother, cls,
-                       leaf_start, params, c[0], c[1], b[0], b[1]);
-      if (level + 1 < (uint32_t)kMaxBranchLevels) {
 #pragma unroll
-        for (int side = 0; side < 2; side++) {
-          if (!b[side]) continue;
-          deepest = level + 1;
-          const uint32_t ct = turns + (uint32_t)side;
-          classify_node<M>(c[side], base, n, level + 1, ct, max_leaf, kchild, kleaf, kother,
-                           cls, leaf_start, params, gc[2 * side], gc[2 * side + 1],
-                           gp[2 * side], gp[2 * side + 1]);
-          gt[2 * side] = ct;
-          gt[2 * side + 1] = ct + 1;
+      for (int d = 0; d < S; d++) {
+        // layer d (level + d, at most 2^d branches) -> layer d + 1
+        uint32_t nx[kOut], nt[kOut];
+        bool np[kOut];
+#pragma unroll
+        for (int q = 0; q < kOut; q++) nx[q] = nt[q] = 0, np[q] = false;
+        if (level + (uint32_t)d < (uint32_t)kMaxBranchLevels) {
+#pragma unroll
+          for (int q = 0; q < (1 << d); q++) {
+            if (!lp[q]) continue;
+            deepest = level + (uint32_t)d;
+            classify_node<M>(lx[q], base, n, level + (uint32_t)d, lt[q], max_leaf, kchild,
+                             kleaf, kother, cls, leaf_start, params, nx[2 * q], nx[2 * q + 1],
+                             np[2 * q], np[2 * q + 1]);
+            nt[2 * q] = lt[q];
+            nt[2 * q + 1] = lt[q] + 1;
+          }
         }
+#pragma unroll
+        for (int q = 0; q < kOut; q++) lx[q] = nx[q], lt[q] = nt[q], lp[q] = np[q];
       }
     }
     note_depth(params, deepest);
-    uint64_t mk[4];
+    uint64_t mk[kOut];
     uint32_t total = 0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      mk[q] = __ballot(gp[q]);
+    for (int q = 0; q < kOut; q++) {
+      mk[q] = __ballot(lp[q]);
       total += (uint32_t)__popcll(mk[q]);
     }
     if (lane == 0) s_tot[wave] = total;
@@ -930,7 +942,7 @@ __global__ void __launch_bounds__(kClassifyBlock) k_classify_pair(
     if (threadIdx.x == 0) {
       uint32_t sum = 0;
       for (uint32_t w = 0; w < kClassifyBlock / 64; w++) sum += s_tot[w];
-      s_base = sum ? atomicAdd(&counts[level + 2], sum) : 0u;
+      s_base = sum ? atomicAdd(&counts[level + S], sum) : 0u;
     }
     __syncthreads();
     uint32_t at = s_base;
@@ -938,9 +950,9 @@ __global__ void __launch_bounds__(kClassifyBlock) k_classify_pair(
     __syncthreads();  // s_tot and s_base are rewritten by the next step
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < kOut; q++) {
       const uint32_t o = at + (uint32_t)__popcll(mk[q] & below);
-      if (gp[q] && o < out_cap) front_out[o] = make_uint2(gc[q], gt[q] | (m << 8));
+      if (lp[q] && o < out_cap) front_out[o] = make_uint2(lx[q], lt[q] | (m << 8));
       at += (uint32_t)__popcll(mk[q]);
     }
   }
@@ -1584,18 +1596,28 @@ __global__ void __launch_bounds__(256) k_sub_link(float4 *nodes, uint32_t nnodes
 // The bottom-up refit: each branch merges its two slots,
 // MergeBoundingBox(left, right) (render/render_accel_triangle.cc:804-814),
 // into its parent's slot; the root's merge is BoundingBox(). Levels
-// `level` + 1 and `level` in one launch (the counterpart of
-// k_classify_pair): each thread takes one branch of level `level`, first
-// merges each branch child's two slots into the branch's slot for that
-// child (the child's own slots are complete: its children are leaves or
-// branches of level + 2, refit by the previous launch), then merges the
-// branch's two slots into its parent's slot, or the root box. The merges and
-// their order (left, right) are the same whichever launch does them, so the
-// boxes do not depend on the grouping either.
-template <bool M>
-__global__ void __launch_bounds__(256) k_refit_pair(uint32_t n_all, uint32_t level,
+// `level` + S - 1 .. `level` in one launch (the counterpart of
+// k_classify_span): each thread takes one branch of level `level`, loads
+// its branch descendants down to level + S - 1 (their own slots below that
+// are complete: leaves, or branches of level + S refit by the previous
+// launch), merges them bottom-up into their parents' slots, in registers
+// and in memory, and finally merges the head into its parent's slot, or the
+// root box. The merges and their order (left, right) are the same whichever
+// launch does them, so the boxes do not depend on the grouping.
+__device__ __forceinline__ void load_slots(const float4 *w4, Box &l, Box &r, uint4 &meta) {
+  const float4 a = w4[0], b = w4[1], c = w4[2];
+  meta = reinterpret_cast<const uint4 *>(w4)[3];
+  l.mn[0] = a.x; l.mn[1] = a.y; l.mn[2] = a.z;
+  r.mn[0] = a.w; r.mn[1] = b.x; r.mn[2] = b.y;
+  l.mx[0] = b.z; l.mx[1] = b.w; l.mx[2] = c.x;
+  r.mx[0] = c.y; r.mx[1] = c.z; r.mx[2] = c.w;
+}
+
+template <bool M, int S>
+__global__ void __launch_bounds__(256) k_refit_span(uint32_t n_all, uint32_t level,
                                                    const uint2 *levels, const uint32_t *counts,
                                                    NodeOut o, SegView v) {
+  constexpr int kNodes = (1 << S) - 1;  // the head's subtree down to level + S - 1
   const uint32_t count = counts[level];
   const uint2 *list = levels + level_start(counts, level);
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count;
@@ -1613,39 +1635,49 @@ __global__ void __launch_bounds__(256) k_refit_pair(uint32_t n_all, uint32_t lev
       atomicOr(&sg.params->error, 16u);
       continue;
     }
-    float *words = reinterpret_cast<float *>(sg.nodes) + 16 * (size_t)pos;
-    const float4 *w4 = reinterpret_cast<const float4 *>(words);
-    const float4 a = w4[0], b = w4[1], cc = w4[2];
-    const uint4 meta = reinterpret_cast<const uint4 *>(w4)[3];
-    Box lr[2];
-    lr[0].mn[0] = a.x; lr[0].mn[1] = a.y; lr[0].mn[2] = a.z;
-    lr[1].mn[0] = a.w; lr[1].mn[1] = b.x; lr[1].mn[2] = b.y;
-    lr[0].mx[0] = b.z; lr[0].mx[1] = b.w; lr[0].mx[2] = cc.x;
-    lr[1].mx[0] = cc.y; lr[1].mx[1] = cc.z; lr[1].mx[2] = cc.w;
+    // the subtree in heap order (node k's children 2k + 1, 2k + 2): node
+    // positions, whether each is a branch, and its two slots
+    uint32_t np[kNodes];
+    bool br[kNodes];
+    Box sl[kNodes][2];
+    np[0] = pos;
+    br[0] = true;
     bool bad = false;
 #pragma unroll
-    for (int side = 0; side < 2; side++) {
-      const uint32_t cp = side ? meta.w : meta.z;
-      if (cp >= max_nodes) {
-        bad = true;
+    for (int k = 0; k < kNodes; k++) {
+      if (k > 0) br[k] = br[(k - 1) >> 1];  // a child of a branch: load it
+      if (!br[k]) continue;
+      uint4 meta;
+      load_slots(reinterpret_cast<const float4 *>(sg.nodes) + 4 * (size_t)np[k], sl[k][0],
+                 sl[k][1], meta);
+      if (k > 0 && meta.x != 0) {  // a leaf: its box is already in its parent's slot
+        br[k] = false;
         continue;
       }
-      const float4 *c4 = reinterpret_cast<const float4 *>(sg.nodes) + 4 * (size_t)cp;
-      if (reinterpret_cast<const uint4 *>(c4)[3].x != 0) continue;  // a leaf: already in place
-      const float4 ca = c4[0], cb = c4[1], ccc = c4[2];
-      Box l, r;
-      l.mn[0] = ca.x; l.mn[1] = ca.y; l.mn[2] = ca.z;
-      r.mn[0] = ca.w; r.mn[1] = cb.x; r.mn[2] = cb.y;
-      l.mx[0] = cb.z; l.mx[1] = cb.w; l.mx[2] = ccc.x;
-      r.mx[0] = ccc.y; r.mx[1] = ccc.z; r.mx[2] = ccc.w;
-      lr[side] = merge_lr(l, r);
-      store_box_slot(words, side, lr[side]);
+      if (2 * k + 2 < kNodes) {
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+          const uint32_t cp = side ? meta.w : meta.z;
+          if (cp >= max_nodes) bad = true;
+          np[2 * k + 1 + side] = cp < max_nodes ? cp : 0u;
+        }
+      }
     }
     if (bad) {
       atomicOr(&sg.params->error, 16u);
       continue;
     }
-    const Box mb = merge_lr(lr[0], lr[1]);
+    // bottom-up: each branch below the head merges its slots into its
+    // parent's slot for it (in registers and in memory)
+#pragma unroll
+    for (int k = kNodes - 1; k > 0; k--) {
+      if (!br[k]) continue;
+      const int par = (k - 1) >> 1, side = (k - 1) & 1;
+      sl[par][side] = merge_lr(sl[k][0], sl[k][1]);
+      store_box_slot(reinterpret_cast<float *>(sg.nodes) + 16 * (size_t)np[par], side,
+                     sl[par][side]);
+    }
+    const Box mb = merge_lr(sl[0][0], sl[0][1]);
     if (pos == 0) {
       for (int k = 0; k < 3; k++) {
         sg.params->root_bmin[k] = mb.mn[k];
@@ -2016,13 +2048,20 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
   if (top > 0)
     SRF_DISPATCH_M(k_classify_top, dim3(1), dim3(1024), 0, stream, N, max_leaf, top, b.kchild,
                    b.kleaf, b.kother, b.levels, b.level_counts, b.cls, b.leaf_start, v);
-  // two levels per launch from `top` on: heads top, top + 2, ... (the lists
-  // of the levels between stay empty)
+  // kLevelSpan levels per launch from `top` on: heads top, top + span, ...
+  // (the lists of the levels between stay empty)
   const uint32_t cblocks = blocks_for(N, 256, 1024);
-  for (uint32_t level = top; level < (uint32_t)kMaxBranchLevels; level += 2)
-    SRF_DISPATCH_M(k_classify_pair, dim3(cblocks), dim3(256), 0, stream, N, max_leaf, level,
-                   b.kchild, b.kleaf, b.kother, b.levels, b.level_counts, b.cls, b.leaf_start,
-                   v);
+  for (uint32_t level = top; level < (uint32_t)kMaxBranchLevels; level += kLevelSpan) {
+    if (multi)
+      hipLaunchKernelGGL((k_classify_span<true, kLevelSpan>), dim3(cblocks), dim3(256), 0,
+                         stream, N, max_leaf, level, b.kchild, b.kleaf, b.kother, b.levels,
+                         b.level_counts, b.cls, b.leaf_start, v);
+    else
+      hipLaunchKernelGGL((k_classify_span<false, kLevelSpan>), dim3(cblocks), dim3(256), 0,
+                         stream, N, max_leaf, level, b.kchild, b.kleaf, b.kother, b.levels,
+                         b.level_counts, b.cls, b.leaf_start, v);
+    SRF_HIP(hipGetLastError());
+  }
   const uint32_t scan_blocks = blocks_for(N, kScanTile);
   hipLaunchKernelGGL(k_scan_reduce, dim3(scan_blocks), dim3(kScanBlock), 0, stream,
                      b.leaf_start, N, b.partial);
@@ -2046,10 +2085,16 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
   // then levels top-1..0 in one workgroup
   const uint32_t rblocks = blocks_for(N, 256, 2048);
   int last_head = (int)top;
-  while (last_head + 2 < kMaxBranchLevels) last_head += 2;
-  for (int level = last_head; level >= (int)top; level -= 2)
-    SRF_DISPATCH_M(k_refit_pair, dim3(rblocks), dim3(256), 0, stream, N, (uint32_t)level,
-                   b.levels, b.level_counts, b.o, v);
+  while (last_head + kLevelSpan < kMaxBranchLevels) last_head += kLevelSpan;
+  for (int level = last_head; level >= (int)top; level -= kLevelSpan) {
+    if (multi)
+      hipLaunchKernelGGL((k_refit_span<true, kLevelSpan>), dim3(rblocks), dim3(256), 0, stream,
+                         N, (uint32_t)level, b.levels, b.level_counts, b.o, v);
+    else
+      hipLaunchKernelGGL((k_refit_span<false, kLevelSpan>), dim3(rblocks), dim3(256), 0, stream,
+                         N, (uint32_t)level, b.levels, b.level_counts, b.o, v);
+    SRF_HIP(hipGetLastError());
+  }
   if (top > 0)
     SRF_DISPATCH_M(k_refit_top, dim3(1), dim3(1024), 0, stream, N, top, b.levels,
                    b.level_counts, b.o, v);
