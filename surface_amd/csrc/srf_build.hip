@@ -1,10 +1,10 @@
 // srf_build.hip -- LBVH construction on gfx950, stream-ordered, no host sync.
 //
 // Pipeline (one launch each unless noted), for n primitives:
-//   bounds   fused min/max over every primitive's vertices, one partial per
-//            block, the last block to finish (atomic ticket) reduces the
-//            partials and derives inv/kEPS/epsScale; it also clears the
-//            Morton histogram and the sort's tile counters.
+//   bounds   fused min/max over every primitive's vertices, one box per
+//            block; a second launch merges each mesh's block boxes and
+//            derives inv/kEPS/epsScale (no device-scope fences); it also
+//            clears the Morton histogram and the sort's tile counters.
 //   morton   centroid -> 30-bit code + identity index, and all four 8-bit
 //            digit histograms of the codes (LDS, then one global atomic per
 //            bin per block).
@@ -13,18 +13,21 @@
 //            through LDS so each digit run leaves as one coalesced burst.
 //   karras   one thread per internal node (Karras 2012): split, leaf flags,
 //            range end, parent pointers.
-//   classify one thread per internal node: branch or collapsed (range <= leaf
-//            limit or depth > 22), by a bounded walk of the parent chain
-//            that also yields the preorder terms; marks leaf starts.
+//   classify top-down from the roots: branch or collapsed (range <= leaf
+//            limit or depth > 22), the first levels in one workgroup, then
+//            three levels per launch; marks leaf starts and lists each
+//            launch's heads by level.
 //   scan     exclusive scan of the leaf starts (3 launches).
 //   emit     one thread per branch: writes the node at its depth-first
 //            preorder position pos = ancestors + 2*leaves_left - right_turns,
 //            which is where the reference's serial builder puts it, and the
 //            leaf records of its leaf children.
 //   leaves   one thread per leaf: leaf box + packed primitives, written to
-//            the leaf's own slot and its parent's slot for it.
-//   refit    one launch per branch level (23..0): each branch merges its two
-//            child boxes into its parent's slot.
+//            the leaf's own slot and its parent's slot for it; leaves of more
+//            than kBigLeaf primitives by whole blocks.
+//   refit    bottom-up, three branch levels per launch, then the first
+//            levels in one workgroup: each branch merges its two child boxes
+//            into its parent's slot.
 // A batch of meshes runs the same launches over all meshes at once (see the
 // segment notes below).
 #include <float.h>
@@ -149,6 +152,11 @@ struct SegView {
   const uint32_t *bblk_off;
   const uint32_t *mblk_off;
   const uint32_t *leaf_base;    // written by k_seg_leaves
+  // M = true: the mesh of every primitive-space index (written by k_morton)
+  // and of every global leaf (written by k_emit), so the per-node and
+  // per-leaf kernels read one word instead of searching the prefix arrays
+  uint32_t *seg_of;
+  uint32_t *leaf_seg;
   uint32_t nseg;
 };
 
@@ -289,6 +297,48 @@ __device__ __forceinline__ void box_wave_reduce(Box &b) {
   }
 }
 
+// The scene box's parameters: Morton scale, epsScale and kEPS; the
+// counters and flags the later stages accumulate into start at zero.
+template <int K>
+__device__ __forceinline__ void write_params(BuildParams *params, const Box &r) {
+  BuildParams p;
+  for (int k = 0; k < 3; k++) {
+    p.bmin[k] = r.mn[k];
+    p.bmax[k] = r.mx[k];
+    // kDIV / (bmax - bmin), render/render_prefix_tree_util.cc:242-245
+    p.inv[k] = (float)1024 / (r.mx[k] - r.mn[k]);
+  }
+  // epsScale = max extent, render/render_accel_triangle.cc:1451-1463
+  double bs0 = (double)(r.mx[0] - r.mn[0]);
+  double bs1 = (double)(r.mx[1] - r.mn[1]);
+  double bs2 = (double)(r.mx[2] - r.mn[2]);
+  double es = bs0;
+  if (es < bs1) es = bs1;
+  if (es < bs2) es = bs2;
+  p.eps_scale = es;
+  if (K == kTri)
+    p.kEPS = FLT_EPSILON * (float)es;  // ComputeBoundingBox30's real kEPS
+  else if (K == kTet || K == kSol || K == kLin)
+    p.kEPS = FLT_EPSILON * 16;
+  else
+    p.kEPS = (float)(FLT_EPSILON * 16.0);
+  for (int k = 0; k < 3; k++) {
+    p.root_bmin[k] = FLT_MAX;
+    p.root_bmax[k] = -FLT_MAX;
+  }
+  p.num_leaves = 0;
+  p.num_nodes = 0;
+  p.max_steps = 0;
+  p.error = 0;
+  p.ticket = 0;
+  p.pad[0] = p.pad[1] = p.pad[2] = 0;
+  *params = p;
+}
+
+// The block's box of its mesh's primitives. A mesh reduced by one block
+// writes its parameters here; a mesh of several blocks leaves per-block
+// boxes for k_bounds_final (no device-scope fence or ticket: on MI355X each
+// such fence writes back the L2 of its XCD).
 template <int K, bool M>
 __global__ void __launch_bounds__(256) k_bounds(SegView v, Box *partials,
                                                uint32_t *clear, uint32_t nclear,
@@ -299,7 +349,6 @@ __global__ void __launch_bounds__(256) k_bounds(SegView v, Box *partials,
   const uint32_t nblk = M ? v.bblk_off[m + 1] - b0 : gridDim.x;
   const Seg sg = seg_get<M>(v, m);
   const PrimSrc &s = sg.src;
-  BuildParams *params = sg.params;
   Box b;
   box_init(b);
   if ((K == kTri || K == kTet) && cen.x) {
@@ -318,79 +367,45 @@ __global__ void __launch_bounds__(256) k_bounds(SegView v, Box *partials,
   }
   box_wave_reduce(b);
   __shared__ Box wbox[4];
-  __shared__ bool last;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) wbox[wave] = b;
-  __syncthreads();
-  // a mesh with one block needs no partials, fence or ticket (many small
-  // meshes: each device-scope fence writes back the L2)
-  if (threadIdx.x == 0 && nblk > 1) {
-    Box r = wbox[0];
-    for (int w = 1; w < 4; w++) box_merge(r, wbox[w]);
-    partials[blockIdx.x] = r;
-    __threadfence();
-    uint32_t t = atomicAdd(&params->ticket, 1u);
-    last = (t == nblk - 1);
-  }
   // clear the histogram / tile counters the next stages accumulate into
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nclear;
        i += gridDim.x * blockDim.x)
     clear[i] = 0;
   __syncthreads();
-  if (nblk > 1) {
-    if (!last) return;
-    __threadfence();
-    box_init(b);
-    for (uint32_t i = b0 + threadIdx.x; i < b0 + nblk; i += blockDim.x) {
-      Box o;
-      for (int k = 0; k < 3; k++) {
-        o.mn[k] =
-            __hip_atomic_load(&partials[i].mn[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        o.mx[k] =
-            __hip_atomic_load(&partials[i].mx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      box_merge(b, o);
-    }
-    box_wave_reduce(b);
-    __syncthreads();
-    if (lane == 0) wbox[wave] = b;
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
     Box r = wbox[0];
     for (int w = 1; w < 4; w++) box_merge(r, wbox[w]);
-    BuildParams p;
-    for (int k = 0; k < 3; k++) {
-      p.bmin[k] = r.mn[k];
-      p.bmax[k] = r.mx[k];
-      // kDIV / (bmax - bmin), render/render_prefix_tree_util.cc:242-245
-      p.inv[k] = (float)1024 / (r.mx[k] - r.mn[k]);
-    }
-    // epsScale = max extent, render/render_accel_triangle.cc:1451-1463
-    double bs0 = (double)(r.mx[0] - r.mn[0]);
-    double bs1 = (double)(r.mx[1] - r.mn[1]);
-    double bs2 = (double)(r.mx[2] - r.mn[2]);
-    double es = bs0;
-    if (es < bs1) es = bs1;
-    if (es < bs2) es = bs2;
-    p.eps_scale = es;
-    if (K == kTri)
-      p.kEPS = FLT_EPSILON * (float)es;  // ComputeBoundingBox30's real kEPS
-    else if (K == kTet || K == kSol || K == kLin)
-      p.kEPS = FLT_EPSILON * 16;
+    if (nblk > 1)
+      partials[blockIdx.x] = r;
     else
-      p.kEPS = (float)(FLT_EPSILON * 16.0);
-    for (int k = 0; k < 3; k++) {
-      p.root_bmin[k] = FLT_MAX;
-      p.root_bmax[k] = -FLT_MAX;
-    }
-    p.num_leaves = 0;
-    p.num_nodes = 0;
-    p.max_steps = 0;
-    p.error = 0;
-    p.ticket = 0;
-    p.pad[0] = p.pad[1] = p.pad[2] = 0;
-    *params = p;
+      write_params<K>(sg.params, r);
+  }
+}
+
+// One block per mesh of several bounds blocks: their boxes merged, then the
+// mesh's parameters.
+template <int K, bool M>
+__global__ void __launch_bounds__(256) k_bounds_final(SegView v, const Box *partials,
+                                                     uint32_t nblk_one) {
+  const uint32_t m = blockIdx.x;
+  if (M && m >= v.nseg) return;
+  const uint32_t b0 = seg_off<M>(v.bblk_off, m);
+  const uint32_t nblk = M ? v.bblk_off[m + 1] - b0 : nblk_one;
+  if (nblk <= 1) return;
+  Box b;
+  box_init(b);
+  for (uint32_t i = b0 + threadIdx.x; i < b0 + nblk; i += blockDim.x) box_merge(b, partials[i]);
+  box_wave_reduce(b);
+  __shared__ Box wbox[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wbox[wave] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Box r = wbox[0];
+    for (int w = 1; w < 4; w++) box_merge(r, wbox[w]);
+    write_params<K>(seg_get<M>(v, m).params, r);
   }
 }
 
@@ -480,6 +495,7 @@ __global__ void __launch_bounds__(256) k_morton(SegView v, uint32_t *codes, uint
       c = cached ? morton30(cen.x[pbase + i], cen.y[pbase + i], cen.z[pbase + i], p)
                  : prim_code<K>(s, i, p);
       codes[i] = c;
+      if (M) v.seg_of[pbase + i] = m;
     }
     hist_add(h[0], c & 255, valid);
     hist_add(h[1], (c >> 8) & 255, valid);
@@ -738,7 +754,7 @@ __device__ __forceinline__ void radix_node30(const uint32_t *c, int i, int n, ui
 template <bool M>
 __device__ __forceinline__ void seg_local(const SegView &v, uint32_t n_all, uint32_t x,
                                           uint32_t &m, uint32_t &base, uint32_t &n) {
-  m = seg_find<M>(v.prim_off, v.nseg, x);
+  m = M ? min(v.seg_of[x], v.nseg - 1) : 0u;
   base = seg_off<M>(v.prim_off, m);
   n = M ? v.prim_off[m + 1] - base : n_all;
 }
@@ -1145,11 +1161,6 @@ __global__ void __launch_bounds__(256) k_seg_leaves(SegView v, const uint32_t *l
   }
 }
 
-// Clears the bounds tickets of every mesh (fresh parameter blocks).
-__global__ void __launch_bounds__(64) k_seg_reset(SegView v) {
-  for (uint32_t m = threadIdx.x; m < v.nseg; m += blockDim.x) v.seg[m].params->ticket = 0;
-}
-
 // ---- node emission ----
 struct NodeOut {
   uint32_t *parent_link;  // per node: parent pos << 1 | side
@@ -1206,12 +1217,14 @@ __global__ void __launch_bounds__(256) k_emit(uint32_t n_all, int max_leaf,
   parent_link[posR] = (pos << 1) | 1u;
   bool deep = steps > (uint32_t)kMaxDepth32;
   if ((lb & 1u) || (g - l <= (uint32_t)max_leaf) || deep) {
+    if (M) v.leaf_seg[yl] = m;
     o.leaf_pos[yl] = posL;
     o.leaf_first[yl] = l;
     o.leaf_count[yl] = g + 1 - l;
     write_meta(nodes, posL, true, 0, g + 1 - l, l);
   }
   if ((lb & 2u) || (r - (g + 1) <= (uint32_t)max_leaf) || deep) {
+    if (M) v.leaf_seg[yr] = m;
     o.leaf_pos[yr] = posR;
     o.leaf_first[yr] = g + 1;
     o.leaf_count[yr] = r - g;
@@ -1357,7 +1370,7 @@ __global__ void __launch_bounds__(256) k_leaves(uint32_t n_all, const uint32_t *
                                                SegView v) {
   const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
   if (y >= *total_leaves || y >= n_all) return;
-  const uint32_t m = seg_find<M>(v.leaf_base, v.nseg, y);
+  const uint32_t m = M ? min(v.leaf_seg[y], v.nseg - 1) : 0u;
   const Seg sg = seg_get<M>(v, m);
   const PrimSrc &s = sg.src;
   const uint32_t base = seg_off<M>(v.prim_off, m);
@@ -1498,7 +1511,7 @@ __global__ void __launch_bounds__(256) k_leaves_big(const uint32_t *sidx_all, No
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (uint32_t j = blockIdx.x; j < nbig; j += gridDim.x) {
     const uint32_t y = big[j];
-    const uint32_t m = seg_find<M>(v.leaf_base, v.nseg, y);
+    const uint32_t m = M ? min(v.leaf_seg[y], v.nseg - 1) : 0u;
     const Seg sg = seg_get<M>(v, m);
     const PrimSrc &s = sg.src;
     const uint32_t base = seg_off<M>(v.prim_off, m);
@@ -1554,6 +1567,144 @@ __global__ void __launch_bounds__(256) k_leaves_big(const uint32_t *sidx_all, No
         store_box_slot(reinterpret_cast<float *>(sg.nodes) + 16 * (size_t)p, (int)(link & 1u), b);
     }
     __syncthreads();
+  }
+}
+
+// The big leaves of a one-mesh sphere build in chunks of kBigChunk spheres:
+// the leaves of depth-capped clusters hold up to 10^5 spheres, and one block
+// per leaf left the whole launch waiting on the largest. k_big_chunks
+// numbers the (leaf, chunk) units (one workgroup scans the chunk counts),
+// k_leaves_big_part packs each unit's spheres and reduces its candidates to
+// (value, index) extremes, and k_leaves_big_combine reduces each leaf's
+// units. Ext's order is total on (value, index), so the extremes, and with
+// them the box bits, are the single block's whatever the grouping.
+constexpr uint32_t kBigChunk = 2048;
+
+__global__ void __launch_bounds__(1024) k_big_chunks(const uint32_t *big,
+                                                     const uint32_t *big_count, NodeOut o,
+                                                     uint32_t *unit_off) {
+  __shared__ uint32_t s[1024];
+  __shared__ uint32_t carry;
+  const uint32_t nbig = *big_count;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < nbig; b0 += 1024) {
+    const uint32_t j = b0 + threadIdx.x;
+    const uint32_t c = j < nbig ? (o.leaf_count[big[j]] + kBigChunk - 1) / kBigChunk : 0u;
+    s[threadIdx.x] = c;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {  // inclusive scan in LDS
+      const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (j < nbig) unit_off[j] = carry + s[threadIdx.x] - c;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += s[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) unit_off[nbig] = carry;
+}
+
+template <int K>
+__global__ void __launch_bounds__(256) k_leaves_big_part(const uint32_t *big,
+                                                        const uint32_t *big_count, NodeOut o,
+                                                        const uint32_t *unit_off, Ext *part,
+                                                        SegView v) {
+  __shared__ Ext red[6][4];
+  const uint32_t nbig = *big_count, units = unit_off[nbig];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const Seg sg = seg_get<false>(v, 0);
+  const PrimSrc &s = sg.src;
+  const float e = sg.params->kEPS;
+  for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+    // the leaf whose units hold u: the last j with unit_off[j] <= u
+    uint32_t lo = 0, hi = nbig;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (unit_off[mid] <= u) lo = mid; else hi = mid;
+    }
+    const uint32_t y = big[lo];
+    const uint32_t first = o.leaf_first[y], count = o.leaf_count[y];
+    const uint32_t c0 = (u - unit_off[lo]) * kBigChunk;
+    const uint32_t c1 = min(c0 + kBigChunk, count);
+    Ext ex[6];
+    for (int c = 0; c < 3; c++) {
+      ex[c] = Ext{FLT_MAX, 0xffffffffu};
+      ex[3 + c] = Ext{-FLT_MAX, 0xffffffffu};
+    }
+    for (uint32_t q = first + c0 + threadIdx.x; q < first + c1; q += blockDim.x) {
+      float lo3[8][3], hi3[8][3];
+      const int nc = pack_prim<K>(s, sg.sorted_index, q, e, sg.prims, sg.sorted_faces, lo3, hi3);
+      for (int k = 0; k < nc; k++) {
+        const uint32_t at = (q - first) * 8u + (uint32_t)k;
+        for (int c = 0; c < 3; c++) {
+          ex[c] = ext_min(ex[c], Ext{lo3[k][c], at});
+          ex[3 + c] = ext_max(ex[3 + c], Ext{hi3[k][c], at});
+        }
+      }
+    }
+    for (int c = 0; c < 6; c++) {
+      for (int off = 32; off >= 1; off >>= 1) {
+        Ext o2{__shfl_xor(ex[c].v, off, 64), (uint32_t)__shfl_xor((int)ex[c].at, off, 64)};
+        ex[c] = c < 3 ? ext_min(ex[c], o2) : ext_max(ex[c], o2);
+      }
+      if (lane == 0) red[c][wave] = ex[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      const int c = threadIdx.x;
+      Ext r = red[c][0];
+      for (int w = 1; w < (int)(blockDim.x / 64); w++)
+        r = c < 3 ? ext_min(r, red[c][w]) : ext_max(r, red[c][w]);
+      part[6 * (size_t)u + c] = r;
+    }
+    __syncthreads();
+  }
+}
+
+// One thread per big leaf: its units' extremes, then the leaf's own slot and
+// its parent's slot (as k_leaves_big writes them).
+__global__ void __launch_bounds__(256) k_leaves_big_combine(const uint32_t *big,
+                                                           const uint32_t *big_count,
+                                                           NodeOut o, const uint32_t *unit_off,
+                                                           const Ext *part, SegView v) {
+  const uint32_t nbig = *big_count;
+  const Seg sg = seg_get<false>(v, 0);
+  const PrimSrc &s = sg.src;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nbig;
+       j += gridDim.x * blockDim.x) {
+    const uint32_t y = big[j];
+    Ext ex[6];
+    for (int c = 0; c < 3; c++) {
+      ex[c] = Ext{FLT_MAX, 0xffffffffu};
+      ex[3 + c] = Ext{-FLT_MAX, 0xffffffffu};
+    }
+    for (uint32_t u = unit_off[j]; u < unit_off[j + 1]; u++)
+      for (int c = 0; c < 6; c++)
+        ex[c] = c < 3 ? ext_min(ex[c], part[6 * (size_t)u + c])
+                      : ext_max(ex[c], part[6 * (size_t)u + c]);
+    Box b;
+    for (int c = 0; c < 3; c++) {
+      b.mn[c] = ex[c].v;
+      b.mx[c] = ex[3 + c].v;
+    }
+    const uint32_t node = o.leaf_pos[y];
+    if (node == 0 || node >= 2 * s.n - 1) {
+      atomicOr(&sg.params->error, 4u);
+      continue;
+    }
+    float4 *w4 = reinterpret_cast<float4 *>(sg.nodes) + 4 * (size_t)node;
+    w4[0] = make_float4(b.mn[0], b.mn[1], b.mn[2], 0.0f);
+    w4[1] = make_float4(0.0f, 0.0f, b.mx[0], b.mx[1]);
+    w4[2] = make_float4(b.mx[2], 0.0f, 0.0f, 0.0f);
+    const uint32_t link = o.parent_link[node];
+    const uint32_t p = link >> 1;
+    if (p >= 2 * s.n - 1)
+      atomicOr(&sg.params->error, 4u);
+    else
+      store_box_slot(reinterpret_cast<float *>(sg.nodes) + 16 * (size_t)p, (int)(link & 1u), b);
   }
 }
 
@@ -1859,11 +2010,15 @@ struct BuildScratch {
   uint32_t *total;
   NodeOut o;
   Box *partials;
+  uint32_t *big_units;  // big leaves' unit offsets (one-mesh sphere builds)
+  void *big_part;       // per unit: 6 (value, index) extremes
   // many-mesh only
   Seg *seg;
   uint32_t *offs;       // prim_off, tile_off, bblk_off, mblk_off (nseg + 1 each)
   uint32_t *leaf_base;  // nseg + 1
   uint32_t *sidx;       // sorted local indices before they move to each mesh
+  uint32_t *seg_of;     // n: mesh of each primitive-space index
+  uint32_t *leaf_seg;   // n: mesh of each global leaf
 };
 
 // Takes every slice from the arena; returns the bytes used (the arena may be
@@ -1894,13 +2049,19 @@ size_t layout_build(Arena &ar, uint32_t n, uint32_t tiles, uint32_t bblocks, uin
   b.o.leaf_first = ar.take<uint32_t>(n);
   b.o.leaf_count = ar.take<uint32_t>(n);
   b.partials = ar.take<Box>(bblocks);
+  // a big leaf holds more than kBigLeaf primitives, a unit at most kBigChunk
+  const size_t max_big = (size_t)n / (kBigLeaf + 1) + 2;
+  b.big_units = ar.take<uint32_t>(max_big + 1);
+  b.big_part = ar.take<uint64_t>(6 * (max_big + (size_t)n / kBigChunk + 2));
   b.seg = nullptr;
-  b.offs = b.leaf_base = b.sidx = nullptr;
+  b.offs = b.leaf_base = b.sidx = b.seg_of = b.leaf_seg = nullptr;
   if (multi) {
     b.seg = ar.take<Seg>(nseg);
     b.offs = ar.take<uint32_t>(4 * ((size_t)nseg + 1));
     b.leaf_base = ar.take<uint32_t>((size_t)nseg + 1);
     b.sidx = ar.take<uint32_t>(n);
+    b.seg_of = ar.take<uint32_t>(n);
+    b.leaf_seg = ar.take<uint32_t>(n);
   }
   return ar.off;
 }
@@ -1965,6 +2126,8 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
     v.bblk_off = b.offs + 2 * (nseg + 1);
     v.mblk_off = b.offs + 3 * (nseg + 1);
     v.leaf_base = b.leaf_base;
+    v.seg_of = b.seg_of;
+    v.leaf_seg = b.leaf_seg;
     b.ss.vals_a = b.sidx;
   } else {
     b.ss.vals_a = segs[0].sorted_index;  // the sorted index lands in the accel
@@ -1976,15 +2139,6 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
                               : ((kind == kTet || kind == kSol || kind == kLin) ? kTetMaxLeaf
                                                                                 : kSphMaxLeaf);
 
-  // zero the bounds tickets (the last bounds block rewrites each params block)
-  if (multi) {
-    hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(64), 0, stream, v);
-  } else {
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream,
-                       reinterpret_cast<uint32_t *>(segs[0].params), 0u,
-                       sizeof(BuildParams) / 4);
-  }
-  SRF_HIP(hipGetLastError());
   // the region [status, hist] is contiguous: clear it inside k_bounds
   uint32_t *clear_begin = b.ss.status;
   uint32_t nclear = (uint32_t)((b.ss.hist + 1024 * (size_t)nseg) - clear_begin);
@@ -2033,6 +2187,8 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
           : CenCache{nullptr, nullptr, nullptr};
   SRF_DISPATCH(k_bounds, dim3(bblocks), dim3(256), 0, stream, v, b.partials, clear_begin,
                nclear, cen);
+  SRF_DISPATCH(k_bounds_final, dim3(multi ? nseg : 1u), dim3(256), 0, stream, v, b.partials,
+               bblocks);
   SRF_DISPATCH(k_morton, dim3(mblocks), dim3(256), 0, stream, v, b.codes, b.ss.hist, cen);
   st = multi ? run_sort_passes<true>(b.codes, N, tiles, b.ss, v, stream)
              : run_sort_passes<false>(b.codes, N, tiles, b.ss, v, stream);
@@ -2079,8 +2235,22 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
                b.kleaf, b.kother, b.cls, b.leaf_rank, b.o, v);
   SRF_DISPATCH(k_leaves, dim3(nblocks), dim3(256), 0, stream, N, b.ss.vals_a, b.o, b.total,
                b.big, b.level_counts + 32, v);
-  SRF_DISPATCH(k_leaves_big, dim3(blocks_for(N, 256 * (int)kBigLeaf, 1024)), dim3(256), 0, stream,
-               b.ss.vals_a, b.o, b.big, b.level_counts + 32, v);
+  if (kind == kSph && !multi) {
+    // big leaves in chunks (depth-capped clusters hold up to 10^5 spheres)
+    Ext *part = reinterpret_cast<Ext *>(b.big_part);
+    hipLaunchKernelGGL(k_big_chunks, dim3(1), dim3(1024), 0, stream, b.big,
+                       b.level_counts + 32, b.o, b.big_units);
+    hipLaunchKernelGGL(k_leaves_big_part<kSph>, dim3(blocks_for(N, 256 * (int)kBigLeaf, 2048)),
+                       dim3(256), 0, stream, b.big, b.level_counts + 32, b.o, b.big_units, part,
+                       v);
+    hipLaunchKernelGGL(k_leaves_big_combine, dim3(blocks_for(N / kBigLeaf + 1, 256, 256)),
+                       dim3(256), 0, stream, b.big, b.level_counts + 32, b.o, b.big_units, part,
+                       v);
+    SRF_HIP(hipGetLastError());
+  } else {
+    SRF_DISPATCH(k_leaves_big, dim3(blocks_for(N, 256 * (int)kBigLeaf, 1024)), dim3(256), 0,
+                 stream, b.ss.vals_a, b.o, b.big, b.level_counts + 32, v);
+  }
   // bottom-up refit, one level per launch (levels are branch steps 23..top),
   // then levels top-1..0 in one workgroup
   const uint32_t rblocks = blocks_for(N, 256, 2048);

@@ -91,7 +91,7 @@ struct alignas(16) BuildParams {
   uint32_t num_nodes;
   uint32_t max_steps;  // deepest branch (root = 0 steps)
   uint32_t error;      // nonzero: a bounded wait gave up (never expected)
-  uint32_t ticket;     // last-block counter of the bounds reduction
+  uint32_t ticket;     // unused (kept zero; the block keeps its 128-byte layout)
   uint32_t pad[3];
 };
 
