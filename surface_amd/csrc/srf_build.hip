@@ -990,8 +990,15 @@ __global__ void __launch_bounds__(1024) k_classify_top(uint32_t n_all, int max_l
   __shared__ uint2 buf[2][1024];
   __shared__ uint32_t cnt[2];
   const uint32_t lane = threadIdx.x & 63u, j = threadIdx.x;
-  uint32_t count = count_load(&counts[0]);  // the roots
-  if (j < count && j < 1024u) buf[0][j] = levels[j];
+  // the roots (one per mesh; the host runs this kernel only for at most 1024
+  // meshes), written here rather than by k_classify_roots
+  uint32_t count = v.nseg;
+  if (j < count) {
+    const uint2 e = make_uint2(seg_off<M>(v.prim_off, j), j << 8);
+    buf[0][j] = e;
+    levels[j] = e;
+  }
+  if (j == 0) counts[0] = count;
   if (j < 2) cnt[j] = 0;
   __syncthreads();
   uint32_t in_off = 0;
@@ -1132,6 +1139,41 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_downsweep(const uint32_t *i
   __shared__ uint32_t tmp[8];
   uint32_t total;
   uint32_t pre = block_excl_scan256(sum, tmp, &total) + partial[blockIdx.x];
+  for (int k = 0; k < kScanItems; k++) {
+    uint32_t idx = base + k;
+    if (idx < n) out[idx] = pre;
+    pre += v[k];
+    if (idx == n - 1) {
+      *total_out = pre;
+      if (params) {
+        params->num_leaves = pre;
+        params->num_nodes = 2 * pre - 1;
+      }
+    }
+  }
+}
+
+// k_scan_downsweep for at most kScanFuse tiles: each block adds up the
+// partial sums of the tiles before it itself (at most kScanFuse / 256 loads
+// per thread), so k_scan_partials is not launched.
+constexpr uint32_t kScanFuse = 1024;
+__global__ void __launch_bounds__(kScanBlock) k_scan_downsweep_fused(
+    const uint32_t *in, uint32_t n, const uint32_t *partial, uint32_t *out,
+    uint32_t *total_out, BuildParams *params) {
+  __shared__ uint32_t tmp[8];
+  uint32_t mine = 0, before = 0;
+  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kScanBlock) mine += partial[b];
+  (void)block_excl_scan256(mine, tmp, &before);  // the block-wide total
+  uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+  for (int k = 0; k < kScanItems; k++) {
+    uint32_t idx = base + k;
+    v[k] = idx < n ? in[idx] : 0;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t pre = block_excl_scan256(sum, tmp, &total) + before;
   for (int k = 0; k < kScanItems; k++) {
     uint32_t idx = base + k;
     if (idx < n) out[idx] = pre;
@@ -2197,13 +2239,15 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
   SRF_DISPATCH_M(k_karras, dim3(nblocks), dim3(256), 0, stream, b.codes, N, b.kchild, b.kleaf,
                  b.kother, b.leaf_start, b.cls, b.level_counts, v);
   // top-down branch classification, one launch per depth
-  SRF_DISPATCH_M(k_classify_roots, dim3(1), dim3(64), 0, stream, v, b.levels, b.level_counts);
   // the first levels, which hold at most nseg * 2^level branches each, in
-  // one workgroup; the rest one launch per level
+  // one workgroup (which also lists the roots); the rest in spans below
   const uint32_t top = top_levels(nseg);
   if (top > 0)
     SRF_DISPATCH_M(k_classify_top, dim3(1), dim3(1024), 0, stream, N, max_leaf, top, b.kchild,
                    b.kleaf, b.kother, b.levels, b.level_counts, b.cls, b.leaf_start, v);
+  else
+    SRF_DISPATCH_M(k_classify_roots, dim3(1), dim3(64), 0, stream, v, b.levels,
+                   b.level_counts);
   // kLevelSpan levels per launch from `top` on: heads top, top + span, ...
   // (the lists of the levels between stay empty)
   const uint32_t cblocks = blocks_for(N, 256, 1024);
@@ -2221,10 +2265,16 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
   const uint32_t scan_blocks = blocks_for(N, kScanTile);
   hipLaunchKernelGGL(k_scan_reduce, dim3(scan_blocks), dim3(kScanBlock), 0, stream,
                      b.leaf_start, N, b.partial);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, stream, b.partial, scan_blocks);
-  hipLaunchKernelGGL(k_scan_downsweep, dim3(scan_blocks), dim3(kScanBlock), 0, stream,
-                     b.leaf_start, N, b.partial, b.leaf_rank, b.total,
-                     multi ? nullptr : segs[0].params);
+  if (scan_blocks <= kScanFuse) {
+    hipLaunchKernelGGL(k_scan_downsweep_fused, dim3(scan_blocks), dim3(kScanBlock), 0, stream,
+                       b.leaf_start, N, b.partial, b.leaf_rank, b.total,
+                       multi ? nullptr : segs[0].params);
+  } else {
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, stream, b.partial, scan_blocks);
+    hipLaunchKernelGGL(k_scan_downsweep, dim3(scan_blocks), dim3(kScanBlock), 0, stream,
+                       b.leaf_start, N, b.partial, b.leaf_rank, b.total,
+                       multi ? nullptr : segs[0].params);
+  }
   SRF_HIP(hipGetLastError());
   if (multi) {
     hipLaunchKernelGGL(k_seg_leaves, dim3(1), dim3(256), 0, stream, v, b.leaf_rank, b.total,
