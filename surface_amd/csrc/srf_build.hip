@@ -1332,6 +1332,41 @@ __device__ __forceinline__ void leaf_box_pack(const PrimSrc &s, const uint32_t *
     }
     return;
   }
+  if (K == kSph) {
+    // eight spheres at a time: ids, then centres and radii, in two rounds
+    for (uint32_t q0 = first; q0 < first + count; q0 += 8) {
+      const uint32_t nq = min(8u, first + count - q0);
+      uint32_t ids[8];
+      float3 pp[8];
+      float rr[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) ids[j] = (uint32_t)j < nq ? sidx[q0 + j] : 0u;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        pp[j] = (uint32_t)j < nq ? sph_pos(s, ids[j]) : make_float3(0.0f, 0.0f, 0.0f);
+        rr[j] = (uint32_t)j < nq ? sph_rad(s, ids[j]) : 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        if ((uint32_t)j >= nq) break;
+        const uint32_t q = q0 + j;
+        const float3 p = pp[j];
+        const float r = rr[j];
+        const float lo0 = (p.x - r) - e, lo1 = (p.y - r) - e, lo2 = (p.z - r) - e;
+        const float hi0 = (p.x + r) + e, hi1 = (p.y + r) + e, hi2 = (p.z + r) + e;
+        if (q == first) {
+          b.mn[0] = lo0; b.mn[1] = lo1; b.mn[2] = lo2;
+          b.mx[0] = hi0; b.mx[1] = hi1; b.mx[2] = hi2;
+        }
+        b.mn[0] = fmin_ref(b.mn[0], lo0); b.mn[1] = fmin_ref(b.mn[1], lo1);
+        b.mn[2] = fmin_ref(b.mn[2], lo2);
+        b.mx[0] = fmax_ref(b.mx[0], hi0); b.mx[1] = fmax_ref(b.mx[1], hi1);
+        b.mx[2] = fmax_ref(b.mx[2], hi2);
+        reinterpret_cast<float4 *>(prims)[q] = make_float4(p.x, p.y, p.z, r);
+      }
+    }
+    return;
+  }
   for (uint32_t q = first; q < first + count; q++) {
     uint32_t id = sidx[q];
     if (K == kLin) {
