@@ -705,6 +705,97 @@ k_onesweep(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ va
   }
 }
 
+// A batched build's meshes of one sort tile (the secondary sphere trees'
+// big leaves hold about a thousand spheres each) are sorted by one workgroup
+// each, all four digit passes in LDS, and own no onesweep tiles: the
+// same stable LSD ranking as k_onesweep's (wave, item, lane order), so the
+// permutation is identical, without four passes over mostly empty tiles.
+// Keys are read from and the sorted pairs written to where the four passes
+// leave them.
+__global__ void __launch_bounds__(kSortBlock) k_sort_local(uint32_t *keys, uint32_t *vals,
+                                                          SegView v) {
+  __shared__ uint32_t s_keys[kSortTile];
+  __shared__ uint32_t s_vals[kSortTile];
+  __shared__ uint32_t s_whist[4][256];
+  __shared__ uint32_t s_texcl[256];
+  __shared__ uint32_t s_tmp[8];
+  const uint32_t m = blockIdx.x;
+  if (m >= v.nseg) return;
+  const uint32_t base = v.prim_off[m], n = v.prim_off[m + 1] - base;
+  if (n > (uint32_t)kSortTile) return;  // the onesweep passes sort this mesh
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t seg = wave * (kSortItems * 64);
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+#pragma unroll
+  for (int k = 0; k < kSortItems; k++) {
+    const uint32_t idx = seg + k * 64 + lane;
+    const bool ok = idx < n;
+    key[k] = ok ? keys[base + idx] : 0xffffffffu;
+    val[k] = ok ? idx : 0u;
+  }
+  for (int pass = 0; pass < 4; pass++) {
+    const int shift = 8 * pass;
+    for (int i = threadIdx.x; i < 4 * 256; i += kSortBlock) (&s_whist[0][0])[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortItems; k++) {
+      const uint32_t idx = seg + k * 64 + lane;
+      const bool ok = idx < n;
+      const uint32_t d = (key[k] >> shift) & 255u;
+      uint64_t peers = __ballot(ok);
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const uint64_t bm = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? bm : ~bm;
+      }
+      const uint32_t before = __popcll(peers & lt_mask), cnt = __popcll(peers);
+      const uint32_t prev = ok ? s_whist[wave][d] : 0u;
+      rank[k] = prev + before;
+      if (ok && before == cnt - 1) s_whist[wave][d] = prev + cnt;
+    }
+    __syncthreads();
+    // per-digit wave offsets and the digit's start in the tile (thread = digit)
+    const uint32_t d = threadIdx.x;
+    uint32_t run = 0;
+    for (int w = 0; w < 4; w++) {
+      const uint32_t c = s_whist[w][d];
+      s_whist[w][d] = run;
+      run += c;
+    }
+    s_texcl[d] = block_excl_scan256(run, s_tmp, nullptr);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortItems; k++) {
+      const uint32_t idx = seg + k * 64 + lane;
+      if (idx < n) {
+        const uint32_t dd = (key[k] >> shift) & 255u;
+        const uint32_t pos = s_texcl[dd] + s_whist[wave][dd] + rank[k];
+        s_keys[pos] = key[k];
+        s_vals[pos] = val[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortItems; k++) {
+      const uint32_t idx = seg + k * 64 + lane;
+      if (idx < n) {
+        key[k] = s_keys[idx];
+        val[k] = s_vals[idx];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < kSortItems; k++) {
+    const uint32_t idx = seg + k * 64 + lane;
+    if (idx < n) {
+      keys[base + idx] = key[k];
+      vals[base + idx] = val[k];
+    }
+  }
+}
+
 // ---- Karras radix tree ----
 // Delta30, render/render_prefix_tree_util.cc:501-526
 __device__ __forceinline__ int delta30(const uint32_t *c, int i, int j, int n) {
@@ -2060,6 +2151,7 @@ struct SortScratch {
 template <bool M>
 srf_status run_sort_passes(uint32_t *keys_a, uint32_t n, uint32_t tiles, const SortScratch &ss,
                            const SegView &v, hipStream_t stream) {
+  if (tiles == 0) return SRF_OK;  // a batch of single-tile meshes (k_sort_local)
   uint32_t *src_k = keys_a, *src_v = nullptr, *dst_k = ss.keys_b, *dst_v = ss.vals_b;
   for (int pass = 0; pass < 4; pass++) {
     hipLaunchKernelGGL(k_onesweep<M>, dim3(tiles), dim3(kSortBlock), 0, stream, src_k, src_v,
@@ -2167,7 +2259,9 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
       return set_error(SRF_ERR_UNSUPPORTED, "batch of %llu primitives exceeds 2^30",
                        (unsigned long long)total);
     prim_off[m + 1] = prim_off[m] + n;
-    tile_off[m + 1] = tile_off[m] + sort_tiles(n);
+    // a batch's meshes of one tile are sorted by k_sort_local and own no
+    // onesweep tiles
+    tile_off[m + 1] = tile_off[m] + ((multi && n <= (uint32_t)kSortTile) ? 0u : sort_tiles(n));
     // batches: a block per 4096 primitives, so most small meshes take one
     bblk_off[m + 1] = bblk_off[m] + blocks_for(n, multi ? 256 * 16 : 256, kBoundsBlocks);
     mblk_off[m + 1] = mblk_off[m] + blocks_for(n, 256, 1024);
@@ -2267,6 +2361,11 @@ srf_status build_segs(Ctx *ctx, Kind kind, std::vector<Seg> &segs, hipStream_t s
   SRF_DISPATCH(k_bounds_final, dim3(multi ? nseg : 1u), dim3(256), 0, stream, v, b.partials,
                bblocks);
   SRF_DISPATCH(k_morton, dim3(mblocks), dim3(256), 0, stream, v, b.codes, b.ss.hist, cen);
+  if (multi) {
+    hipLaunchKernelGGL(k_sort_local, dim3(nseg), dim3(kSortBlock), 0, stream, b.codes,
+                       b.ss.vals_a, v);
+    SRF_HIP(hipGetLastError());
+  }
   st = multi ? run_sort_passes<true>(b.codes, N, tiles, b.ss, v, stream)
              : run_sort_passes<false>(b.codes, N, tiles, b.ss, v, stream);
   if (st != SRF_OK) return st;

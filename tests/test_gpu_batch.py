@@ -81,6 +81,24 @@ def test_batch_rebuild_with_other_meshes(S, orc):
         _check(S, orc, acc, V, F)
 
 
+@pytest.mark.parametrize("mix", ["single_tile_only", "mixed"])
+def test_batch_of_single_tile_meshes(S, orc, mix):
+    """Meshes of at most one sort tile (3,840 primitives) are sorted by one
+    workgroup each (all four digit passes in LDS) and own no onesweep tiles;
+    alone or next to larger meshes their trees must stay Build32's."""
+    hi = 3500 if mix == "single_tile_only" else 30000
+    meshes, _ = W.batch_scene(9, 1100, hi, seed=W.BASE_SEED + 14)
+    say(f"batch ({mix}): " + ", ".join(str(len(F)) for _, F in meshes))
+    if mix == "single_tile_only":
+        assert max(len(F) for _, F in meshes) <= 3840
+    else:
+        assert min(len(F) for _, F in meshes) <= 3840 < max(len(F) for _, F in meshes)
+    dm = [(S.device_tensor(V), S.device_tensor(F)) for V, F in meshes]
+    accs = S.build_triangles_batch(dm)
+    for acc, (V, F) in zip(accs, meshes):
+        _check(S, orc, acc, V, F)
+
+
 def test_batch_rejects_small_mesh(S):
     from surface_amd._lib import SrfError
     good = W.heightfield(40)
