@@ -216,16 +216,30 @@ __device__ __forceinline__ void tri_wave_closest(const float4 *__restrict__ node
       const bool in = (mask & me) != 0;
       const uint32_t first = __builtin_amdgcn_readfirstlane(meta.w),
                      end = __builtin_amdgcn_readfirstlane(min(meta.w + meta.z, nprims));
-      for (uint32_t q = first; q < end; q++) {
-        const float4 *P = prims + 3 * (size_t)q;
-        const float4 A = P[0], B = P[1], C = P[2];
-        if (in && __float_as_uint(A.w) != prev) {
-          float tt, uu, vv;
-          if (tri_isect(s, A, B, C, ds, kEPS, filter, t, tt, uu, vv)) {
-            t = tt;
-            u = uu;
-            v = vv;
-            prim = __float_as_uint(A.w);
+      for (uint32_t q = first; q < end; q += 4) {
+        // four triangles' records in one round of scalar loads (past the
+        // leaf's end the last one again, unused), tested in order
+        float4 A[4], B[4], C[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const float4 *P = prims + 3 * (size_t)min(q + (uint32_t)j, end - 1);
+          A[j] = P[0];
+          B[j] = P[1];
+          C[j] = P[2];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) keep_loaded_s(A[j], B[j], C[j]);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (q + (uint32_t)j >= end) break;
+          if (in && __float_as_uint(A[j].w) != prev) {
+            float tt, uu, vv;
+            if (tri_isect(s, A[j], B[j], C[j], ds, kEPS, filter, t, tt, uu, vv)) {
+              t = tt;
+              u = uu;
+              v = vv;
+              prim = __float_as_uint(A[j].w);
+            }
           }
         }
       }
