@@ -88,13 +88,13 @@ OWN_LOADS = {"k_trace_tet": (0.9458 + 0.0588, 0.4413 + 0.0187)}
 # counter passes over `bench.py --probe` summarised in
 # profiles/counters_bench_probe.md; a configuration missing here reports null
 TRAFFIC = {
-    "heightfield512": 218_016_005,
-    "heightfield725": 464_982_015,
-    "tetgrid64": 116_767_268,
-    "plummer16M_primary": 1_425_681_257,  # both launches
-    "scene64_upper": 23_582_978_364,      # persistent launch, 128-lane blocks
-    "k_vol_fwd": 276_618_486,             # the march without checkpoints
-    "k_vol_bwd": 1_374_864_457,
+    "heightfield512": 213_381_047,
+    "heightfield725": 451_907_850,
+    "tetgrid64": 116_764_639,
+    "plummer16M_primary": 1_426_698_490,  # both launches
+    "scene64_upper": 23_626_790_032,      # persistent launch, 128-lane blocks
+    "k_vol_fwd": 276_263_308,             # the march without checkpoints
+    "k_vol_bwd": 1_375_299_227,
 }
 
 
@@ -219,6 +219,19 @@ def cfg_triangles(torch, S, W, ref, grid, do_cpu, kernel):
            "trace_mrays_per_s": round(n / tt / 1e3, 1), "hit_fraction": round(hit, 4),
            "roofline": roofline(kernel, tt, n, "tri", f"heightfield{grid}"),
            "build_roofline": build_roofline("tri", len(F), tb)}
+    if grid == 725:
+        # Four cameras' frames in one launch: a single frame's launch ends
+        # with its slowest wavefronts (grazing rays near the horizon) running
+        # nearly alone; several frames fill that tail with other work.
+        rc = S.device_tensor(W.heightfield_cameras(4, n_side, n_side))
+        hc = torch.empty((len(rc), 4), dtype=torch.float32, device="cuda")
+        tri.Traverse(rc, hc)
+        tc = timed_events(torch, lambda: tri.Traverse(rc, hc), 10)
+        out["four_cameras"] = {"rays": int(len(rc)), "trace_ms": round(tc, 4),
+                               "trace_mrays_per_s": round(len(rc) / tc / 1e3, 1),
+                               "workload": "4 cameras turned 90 degrees apart about the grid "
+                                           "centre, 4 x 1024x1024 rays in one Traverse"}
+        del rc, hc
     del tri
     if do_cpu and ref is not None:
         th = cpu_threads()

@@ -15,6 +15,7 @@ Workloads (sizes as named):
   gaussian_volume(256)      256^3 float32, sum of 64 anisotropic Gaussians in [0,1]
   transfer_function(256)    256-entry RGBA table, smooth ramps, alpha <= 0.05
   pinhole_rays(...)         one primary ray per pixel centre, 8x8-tile order
+  heightfield_cameras(4)    4 frames of the heightfield camera, 90 degrees apart
   batch_scene(64)           64 meshes, log-uniform 10k..2M triangles, alternating
                             perturbed geodesic spheres and heightfields, each
                             with a random rigid transform; 16 orbit cameras
@@ -303,6 +304,22 @@ def heightfield_rays(width=1024, height=1024, tiled=True):
     """Camera framing the [0,1]^2 heightfield from above at an angle."""
     return pinhole_rays((0.5, -0.1, 1.0), (0.5, 0.45, 0.0), (0, 0, 1), 48.0,
                         width, height, tiled)
+
+
+def heightfield_cameras(count=4, width=1024, height=1024, tiled=True):
+    """`count` frames of the heightfield camera turned about the grid's
+    vertical centre line by 360/count degrees each (the first is
+    heightfield_rays), concatenated: rays of several cameras for one launch."""
+    frames = [heightfield_rays(width, height, tiled)]
+    for k in range(1, count):
+        a = 2.0 * np.pi * k / count
+        c, s = np.cos(a), np.sin(a)
+        ex, ey = 0.0, -0.6  # eye (0.5, -0.1) relative to the centre (0.5, 0.5)
+        tx, ty = 0.0, -0.05  # target (0.5, 0.45)
+        eye = (0.5 + c * ex - s * ey, 0.5 + s * ex + c * ey, 1.0)
+        target = (0.5 + c * tx - s * ty, 0.5 + s * tx + c * ty, 0.0)
+        frames.append(pinhole_rays(eye, target, (0, 0, 1), 48.0, width, height, tiled))
+    return np.concatenate(frames)
 
 
 def tet_rays(cells=64, width=1024, height=1024, tiled=True):
