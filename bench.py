@@ -88,13 +88,13 @@ OWN_LOADS = {"k_trace_tet": (0.9458 + 0.0588, 0.4413 + 0.0187)}
 # counter passes over `bench.py --probe` summarised in
 # profiles/counters_bench_probe.md; a configuration missing here reports null
 TRAFFIC = {
-    "heightfield512": 213_381_047,
-    "heightfield725": 451_907_850,
-    "tetgrid64": 116_764_639,
-    "plummer16M_primary": 1_426_698_490,  # both launches
-    "scene64_upper": 23_626_790_032,      # persistent launch, 128-lane blocks
-    "k_vol_fwd": 276_263_308,             # the march without checkpoints
-    "k_vol_bwd": 1_375_299_227,
+    "heightfield512": 212_830_143,
+    "heightfield725": 452_779_556,
+    "tetgrid64": 131_817_300,
+    "plummer16M_primary": 1_425_300_605,  # both launches
+    "scene64_upper": 23_603_015_759,      # persistent launch, 128-lane blocks
+    "k_vol_fwd": 275_953_791,             # the march without checkpoints
+    "k_vol_bwd": 1_376_455_261,
 }
 
 
