@@ -2131,8 +2131,13 @@ __device__ void sol_interpolate(float d_out[8], const float p[3], const float4 *
   for (int i = NV; i < 8; i++) d_out[i] = 0.0f;
 }
 
+// Pyramids (142 VGPRs uncapped) run at 4 wavefronts per SIMD with 14 VGPRs
+// spilled (1024^2 rays on 32^3 cells of six pyramids: 8.55 -> 8.0-8.3 ms);
+// prisms and hexahedra (162 and 206 VGPRs) spill heavily under any cap and
+// are slower with one, so they are left uncapped.
 template <int NV, bool kAny>
 __global__ void __launch_bounds__(kTraceBlock)
+    __attribute__((amdgpu_waves_per_eu(NV == 5 ? 4 : 1)))
 k_trace_sol(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             const BuildParams *__restrict__ params, const srf_ray *__restrict__ rays, uint32_t n,
             const float *__restrict__ scalars, srf_hit *__restrict__ hits,
