@@ -2419,6 +2419,8 @@ k_trace_lin(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             const float *__restrict__ tmax_in, uint32_t *__restrict__ occluded, uint32_t max_nodes,
             uint32_t nprims, uint32_t cap) {
   __shared__ uint32_t st_node[kTraceStack * kTraceBlock];
+  __shared__ uint32_t wnode[kAny ? 1 : kTraceBlock / 64][kTraceStack];
+  __shared__ uint64_t wmask[kAny ? 1 : kTraceBlock / 64][kTraceStack];
   const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -2439,6 +2441,82 @@ k_trace_lin(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
       cur = 0;
   }
   uint4 meta;
+  const uint32_t sb = __builtin_amdgcn_readfirstlane(s.sbits);
+  if (!kAny && __all(s.sbits == sb)) {
+    // Closest hit once per wavefront when the lanes' direction signs agree.
+    // A node's boxes are culled by entry alone (no distance), so the nodes a
+    // ray visits do not depend on its running hit, and with shared signs
+    // every lane's pre-order (near child by the split axis' sign, far child
+    // pushed when both boxes are entered) is one static order: the
+    // wavefront walks the union of the lanes' visits in that order with one
+    // stack of (node, lane mask), and each lane tests its own leaves' lines
+    // in its own order, as in the per-ray loop below.
+    const int wv = threadIdx.x >> 6;
+    const uint64_t me = 1ull << (threadIdx.x & 63u);
+    uint64_t mask = __ballot(cur == 0);
+    uint32_t c = 0;
+    int wsp = 0;
+    while (mask) {
+      uint4 m = make_uint4(0u, 0u, 0u, 0u);
+      bool leaf = false;
+      while (c < max_nodes) {
+        c = __builtin_amdgcn_readfirstlane(c);
+        const float4 *nd = nodes + 4 * (size_t)c;
+        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+        m = reinterpret_cast<const uint4 *>(nd)[3];
+        keep_loaded_s(n0, n1, n2);
+        if (m.x != 0) {
+          leaf = true;
+          break;
+        }
+        const bool in = (mask & me) != 0;
+        float tm0, tm1;
+        const bool g0 =
+            in && box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm0) && tm0 <= FLT_MAX;
+        const bool g1 =
+            in && box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm1) && tm1 <= FLT_MAX;
+        const uint32_t nearc = (sb >> m.y) & 1u;
+        const uint64_t nm = __ballot(nearc ? g1 : g0), fm = __ballot(nearc ? g0 : g1);
+        const uint32_t cn = nearc ? m.w : m.z, cf = nearc ? m.z : m.w;
+        if (nm) {
+          if (fm && wsp < kTraceStack) {
+            wnode[wv][wsp] = cf;
+            wmask[wv][wsp] = fm;
+            wsp++;
+          }
+          c = cn;
+          mask = nm;
+        } else if (fm) {
+          c = cf;
+          mask = fm;
+        } else {
+          break;
+        }
+      }
+      if (leaf) {
+        // TestLeafNode, render/render_accel_line.cc:579-633
+        const bool in = (mask & me) != 0;
+        const uint32_t q0 = __builtin_amdgcn_readfirstlane(m.w);
+        const uint32_t q1 = __builtin_amdgcn_readfirstlane(min(m.w + m.z, nprims));
+        for (uint32_t q = q0; q < q1; q++) {
+          const float4 P0 = prims[2 * (size_t)q], P1 = prims[2 * (size_t)q + 1];
+          keep_loaded_s2(P0, P1);
+          if (!in) continue;
+          bool hc;
+          if (line_isect(t, u, v, hc, P0, P1, s, cap != 0)) {
+            best = __float_as_uint(P0.w);
+            best_q = q;
+            best_cap = hc;
+          }
+        }
+      }
+      if (wsp == 0) break;
+      --wsp;
+      c = __builtin_amdgcn_readfirstlane(wnode[wv][wsp]);
+      mask = wmask[wv][wsp];
+    }
+    cur = kDone;
+  }
   while (cur != kDone) {
     for (;;) {
       if (cur >= max_nodes) {
