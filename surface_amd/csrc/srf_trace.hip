@@ -2145,6 +2145,8 @@ k_trace_sol(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
             uint32_t *__restrict__ occluded, uint32_t max_nodes, uint32_t nprims) {
   constexpr int NE = SolTopo<NV>::kEdges;
   __shared__ uint32_t st_node[kTraceStack * kTraceBlock];
+  __shared__ uint32_t wnode[kAny ? 1 : kTraceBlock / 64][kTraceStack];
+  __shared__ uint64_t wmask[kAny ? 1 : kTraceBlock / 64][kTraceStack];
   const uint32_t i = xcd_block() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t flags, prev;
@@ -2176,6 +2178,94 @@ k_trace_sol(const float4 *__restrict__ nodes, const float4 *__restrict__ prims,
   const double oj = axis == 0 ? ro.x : (axis == 1 ? ro.y : ro.z);
   const double dj = axis == 0 ? rd.x : (axis == 1 ? rd.y : rd.z);
   uint4 meta;
+  const uint32_t sb = __builtin_amdgcn_readfirstlane(s.sbits);
+  if (!kAny && __all(s.sbits == sb)) {
+    // Closest hit once per wavefront when the lanes' direction signs agree:
+    // boxes are culled by entry alone, so a ray's visits do not depend on
+    // its running hit and every lane's pre-order is one static order (the
+    // line kernel's shared-stack walk); each lane tests its own leaves'
+    // solids in its own order, the solids loaded through the scalar unit.
+    const int wv = threadIdx.x >> 6;
+    const uint64_t me = 1ull << (threadIdx.x & 63u);
+    uint64_t mask = __ballot(cur == 0);
+    uint32_t c = 0;
+    int wsp = 0;
+    while (mask) {
+      uint4 m = make_uint4(0u, 0u, 0u, 0u);
+      bool leaf = false;
+      while (c < max_nodes) {
+        c = __builtin_amdgcn_readfirstlane(c);
+        const float4 *nd = nodes + 4 * (size_t)c;
+        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2];
+        m = reinterpret_cast<const uint4 *>(nd)[3];
+        keep_loaded_s(n0, n1, n2);
+        if (m.x != 0) {
+          leaf = true;
+          break;
+        }
+        const bool in = (mask & me) != 0;
+        float tm0, tm1;
+        const bool g0 =
+            in && box_span(s, n0.x, n0.y, n0.z, n1.z, n1.w, n2.x, tm0) && tm0 <= FLT_MAX;
+        const bool g1 =
+            in && box_span(s, n0.w, n1.x, n1.y, n2.y, n2.z, n2.w, tm1) && tm1 <= FLT_MAX;
+        const uint32_t nearc = (sb >> m.y) & 1u;
+        const uint64_t nm = __ballot(nearc ? g1 : g0), fm = __ballot(nearc ? g0 : g1);
+        const uint32_t cn = nearc ? m.w : m.z, cf = nearc ? m.z : m.w;
+        if (nm) {
+          if (fm && wsp < kTraceStack) {
+            wnode[wv][wsp] = cf;
+            wmask[wv][wsp] = fm;
+            wsp++;
+          }
+          c = cn;
+          mask = nm;
+        } else if (fm) {
+          c = cf;
+          mask = fm;
+        } else {
+          break;
+        }
+      }
+      if (leaf && (mask & me) != 0) {
+        // TestLeafNode, render/render_accel_solid.cc:2206-2333
+        const uint32_t q0 = __builtin_amdgcn_readfirstlane(m.w);
+        const uint32_t q1 = __builtin_amdgcn_readfirstlane(min(m.w + m.z, nprims));
+        for (uint32_t q = q0; q < q1; q++) {
+          float4 V[NV];
+          const float4 *Pq = prims + (size_t)NV * q;
+#pragma unroll
+          for (int k = 0; k < NV; k++) V[k] = Pq[k];
+          const uint32_t id = __float_as_uint(V[0].w);
+          if (id == prev) continue;
+          double ws[NE];
+          uint32_t m0, m1;
+          sol_sides<NV>(V, rd, raypc, ws, m0, m1);
+          const int f0 = sol_face<NV>(m0, m1);
+          if (f0 < 0) continue;
+          const int f1 = sol_face<NV>(m1, m0);
+          if (f1 < 0) continue;
+          float c0[3], c1[3];
+          sol_point<NV>(V, ws, rd, f0, 1 << axis, c0);
+          sol_point<NV>(V, ws, rd, f1, 1 << axis, c1);
+          const float tc[2] = {(float)(((double)c0[axis] - oj) / dj),
+                               (float)(((double)c1[axis] - oj) / dj)};
+          for (int k = 0; k < 2; k++) {
+            if ((tc[k] >= 0.0f) && (tc[k] < t)) {
+              t = tc[k];
+              best = q;
+              side = (uint32_t)k;
+            }
+          }
+        }
+      }
+      if (wsp == 0) break;
+      --wsp;
+      c = __builtin_amdgcn_readfirstlane(wnode[wv][wsp]);
+      mask = wmask[wv][wsp];
+    }
+    cur = kDone;
+  }
   while (cur != kDone) {
     for (;;) {
       if (cur >= max_nodes) {
